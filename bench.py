@@ -1,0 +1,146 @@
+"""Benchmark of the GA generation step on one MI355X.
+
+A step = one full generation (ne_advance_generation): evaluate every genome's games, plan parents,
+assemble children (crossover + mutation), swap.  Workload: 64x64 toroidal grid (4,096 genomes),
+4,739 synthetic actions, 2,048 secrets in 64 block shards of 32 -> 393,216 games per generation.
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_MFMA_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+WORKLOAD = "c2_64x64_a4739"
+
+
+def algorithmic_eval_flops(decisions, encoded_turns, A):
+    # per decision: 2*(321*256 + 256*128 + 128*64) + 128*A ; per encoded turn (145->64): 2*145*64
+    return decisions * (246272 + 128 * A) + encoded_turns * 18560
+
+
+def cpu_baseline(pkg, rt, wl, A, seconds=12.0):
+    """The oracle (fp32, scalar C++, all host threads) on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(threads, 64))
+    slots, _, _, _ = rt.generation(0)
+    offsets, ids = wl["offsets"], wl["ids"]
+    # probe with a few genomes to size the sample to about `seconds` of CPU work
+    probe = max(threads, 8)
+    genomes = [rt.download_slot(s) for s in slots[:probe]]
+    t0 = time.perf_counter()
+    r = orc.evaluate(genomes, 0, A, A, wl["catalog"], offsets[:probe + 1], ids, mode=0, threads=threads)
+    dt = time.perf_counter() - t0
+    games = 3 * int(offsets[probe])
+    rate = games / dt
+    n = int(min(len(slots), max(probe, rate * seconds / (games / probe))))
+    n = max(probe, (n // threads) * threads)
+    if n > probe:
+        genomes = [rt.download_slot(s) for s in slots[:n]]
+        t0 = time.perf_counter()
+        r = orc.evaluate(genomes, 0, A, A, wl["catalog"], offsets[:n + 1], ids, mode=0, threads=threads)
+        dt = time.perf_counter() - t0
+        games = 3 * int(offsets[n])
+        rate = games / dt
+    assert r["status"] == 0
+    return {"value": rate, "unit": "game-evals/s", "cores": threads, "kind": "port",
+            "sample": f"fitness evaluation of the first {n} genomes of generation 0 ({games} games, "
+                      f"{r['decisions']} decisions) at A={A}, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__ as ge
+    pkg = ge.import_package()
+    rtm = pkg.runtime
+    wl = pkg.synth.workload(WORKLOAD)
+    A, side = wl["actions"], wl["side"]
+    n = side * side
+    stride = rtm.slot_stride_bytes(A)
+    rt = pkg.SlabRuntime(slab_bytes=2 * n * stride + stride, generation_bytes=n * stride, action_count=A,
+                         grid_columns=side)
+    rt.upload_catalog(wl["catalog"])
+    rt.upload_secrets_csr(wl["offsets"], wl["ids"])
+    rt.bootstrap(n, seed=7, variance_scale=1.0, tail_std=0.1)
+    counts = rtm.word_counts(A, A)
+    cfg = rtm.default_assembly_config(mutation_probability=0.01, mutation_sigma=0.02)
+    games_per_gen = 3 * int(wl["offsets"][-1])
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(pkg, rt, wl, A)
+        print(f"[bench] cpu baseline {cpu['value']:.1f} game-evals/s on {cpu['cores']} threads", file=sys.stderr,
+              flush=True)
+
+    gen = 0
+    for _ in range(args.warmup):
+        rt.advance(7 + 2 + gen, counts, cfg)
+        gen += 1
+    eval_ms, flops, decisions = [], [], 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rt.advance(7 + 2 + gen, counts, cfg)
+        gen += 1
+        st = rt.stage_times()
+        eval_ms.append(st.evaluate_ms)
+        flops.append(algorithmic_eval_flops(st.decisions, st.encoded_turns, A))
+        decisions += st.decisions
+    elapsed = time.perf_counter() - t0
+    st = rt.stage_times()
+    value = games_per_gen * args.steps / elapsed
+    eval_avg_ms = float(np.mean(eval_ms))
+    achieved = float(np.mean(flops)) / (eval_avg_ms * 1e-3) / 1e12
+    summary = rt.summary()
+    rt.close()
+    out = {
+        "metric": "game-evals/sec (plain FP32 generation step)",
+        "value": value,
+        "unit": "game-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "64x64 toroidal grid (4096 genomes), 4739 Zipf(1.1) synthetic actions, 2048 "
+                               "secrets in 64 block shards of 32, 3 episodes per secret",
+                   "genomes": n, "actions": A, "games_per_generation": games_per_gen,
+                   "mutation_probability": 0.01, "mutation_sigma": 0.02},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_MFMA_TFLOPS, "traffic": None,
+                     "kernel": "EvaluateKernel", "kernel_ms": eval_avg_ms},
+        "stages_ms": {"evaluate": st.evaluate_ms, "finalize": st.finalize_ms, "plan": st.plan_ms,
+                      "assemble": st.assemble_ms, "bookkeeping": st.bookkeeping_ms},
+        "generations_per_s": args.steps / elapsed,
+        "decisions_per_generation": decisions / args.steps,
+        "best_fitness": summary.best_fitness,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

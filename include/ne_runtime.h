@@ -1,0 +1,256 @@
+/* ne_runtime.h -- C ABI of the MI355X GA generation-step runtime.
+ *
+ * Each entry point replaces one call of the reference's device runtime
+ * (src/genetic_algorithm/device/slab_runtime.hpp and
+ * src/genetic_algorithm/genotype_slab/device_runtime.hpp); the replaced call is cited beside it.
+ * Plain pointers and sizes only.  Every call returns an ne_status (0 = ok) and has finished all of
+ * its device work when it returns.  A handle is not thread-safe; different handles are independent.
+ * All host buffers are owned by the caller.
+ */
+#ifndef NE_RUNTIME_H
+#define NE_RUNTIME_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  0..16 are the reference's DeviceSlabGARuntimeStatusCode values
+ * (slab_runtime.hpp:21-39); codes above 16 are new. */
+typedef enum ne_status {
+    NE_OK = 0,
+    NE_DEVICE_FAILURE = 1,            /* kCudaFailure: a HIP call failed */
+    NE_INVALID_RUNTIME_CONFIG = 2,
+    NE_INVALID_SLAB = 3,
+    NE_INVALID_GENERATION = 4,
+    NE_INVALID_TRAINING_SHARD = 5,
+    NE_GUESS_APPEND_FAILED = 6,
+    NE_POLICY_FORWARD_FAILED = 7,
+    NE_ACTION_SELECTION_FAILED = 8,
+    NE_POPULATION_NOT_EVALUATED = 9,
+    NE_INVALID_ASSEMBLY_CONFIG = 10,
+    NE_PARENT_SELECTION_FAILED = 11,
+    NE_INVALID_ASSEMBLY_PLAN = 12,
+    NE_INVALID_PARENT_INDEX = 13,
+    NE_SLAB_FULL = 14,
+    NE_OUTPUT_EMBEDDING_INJECTION_FAILED = 15,
+    NE_SLAB_REPACK_FAILED = 16,
+    NE_INVALID_ARGUMENT = 17,         /* null pointer, short buffer, size out of range */
+    NE_UNSUPPORTED = 18,              /* a configuration this build does not implement */
+    NE_CALLBACK_FAILED = 19           /* a caller callback returned non-zero */
+} ne_status;
+
+typedef struct ne_runtime ne_runtime;
+
+/* DeviceSlabGARuntimeConfig (slab_runtime.hpp:41-48) plus layout parameters. */
+typedef struct ne_runtime_config {
+    uint64_t genotype_slab_byte_budget;   /* slab bytes; slot count = budget / slot stride */
+    uint64_t generation_byte_budget;      /* bytes one generation may occupy */
+    uint64_t host_spillover_byte_budget;  /* must be 0: host spill is not built */
+    uint64_t action_count;                /* A: output-embedding rows per genome */
+    uint64_t population_size_ceiling;     /* 0 = none; else a multiple of grid_column_count */
+    uint64_t grid_column_count;           /* columns of the toroidal grid */
+    uint32_t encoder_hidden;              /* 0: 145->64 turn encoder (only value built) */
+    uint32_t reserved0;
+    void *stream;                         /* hipStream_t to run on, or NULL for a private stream */
+} ne_runtime_config;
+
+/* WordCountSchedule (src/training_folder/training_data.hpp:24-28). */
+typedef struct ne_word_count_schedule {
+    uint64_t initial_word_count;
+    uint64_t word_count_step;
+    uint64_t word_count_step_period_generations;
+} ne_word_count_schedule;
+
+/* RuntimeWordCounts (src/genetic_algorithm/device/runtime_common.hpp:10-16). */
+typedef struct ne_word_counts {
+    uint64_t training_word_count;
+    uint64_t action_space_word_count;
+    ne_word_count_schedule training_word_schedule;
+    uint64_t shard_initial_radius;
+    uint64_t shard_radius_growth_period_generations;
+} ne_word_counts;
+
+/* TrainingDataShardReleaseHistory (training_data.hpp:47-51): release generation per shard. */
+typedef struct ne_shard_release_history {
+    const uint64_t *release_generations;
+    uint64_t release_count;
+} ne_shard_release_history;
+
+/* ParentSelectionConfig / BreedingConfig / MutationConfig
+ * (selection.hpp:14-17, breeding.hpp:10-14, mutation.hpp:11-15). */
+typedef struct ne_assembly_config {
+    uint64_t cellular_breeding_radius;    /* reference default 2 */
+    float rank_exponent;                  /* reference default 0.5 */
+    float crossover_temperature_level1;   /* 0.02 */
+    float crossover_temperature_level2;   /* 0.01 */
+    float crossover_temperature_level3;   /* 0.005 */
+    float mutation_probability;           /* 0.0001 */
+    float mutation_sigma;                 /* 0.02 */
+    float output_tail_row_scale_mutation_probability; /* 0 */
+    float reserved0;
+} ne_assembly_config;
+
+/* PendingOutputEmbeddingInjection (genotype_slab/device_runtime.hpp:352-356). */
+typedef struct ne_pending_injection {
+    uint32_t enabled;
+    uint32_t reserved0;
+    uint64_t first_catalog_word_index;    /* must equal the current action count */
+    uint64_t injection_count;
+} ne_pending_injection;
+
+/* Generation-0 initialisation: weights ~ N(0, s_l^2) with s_l = gain*sqrt(scale/fan_in),
+ * biases 0, tails ~ N(0, tail_stddev^2).  The reference uses scale 2, gain 1, tail 0.05
+ * (genome_ops.cuh:58-133). */
+typedef struct ne_init_config {
+    float weight_variance_scale;
+    float weight_gain;
+    float tail_stddev;
+    float reserved0;
+} ne_init_config;
+
+/* PopulationFitnessSummary (runtime_common.hpp:18-26). */
+typedef struct ne_fitness_summary {
+    float best_fitness;
+    float average_fitness;
+    uint64_t best_index;
+    uint32_t best_slot_index;
+    uint32_t reserved0;
+    uint64_t generation_index;
+    uint64_t action_count;
+    uint64_t population_size;
+} ne_fitness_summary;
+
+/* GenotypeSlabLayout (genotype_slab/layout.hpp:20-25) and generation shape. */
+typedef struct ne_slab_info {
+    uint64_t action_count;
+    uint64_t slot_stride_bytes;
+    uint64_t slot_count;
+    uint64_t slab_bytes;
+    uint64_t policy_parameter_count;      /* fp32 values before the tail rows */
+    uint64_t max_generation_size;
+    uint64_t current_generation_index;
+    uint64_t current_generation_size;
+    uint64_t next_generation_size;
+    uint64_t free_slot_count;
+} ne_slab_info;
+
+/* Device timings of the last call (milliseconds, HIP events) and on-device work counters. */
+typedef struct ne_stage_times {
+    float evaluate_ms;                    /* game-play kernel */
+    float finalize_ms;                    /* fitness normalisation + summary */
+    float plan_ms;                        /* parent selection */
+    float assemble_ms;                    /* crossover + mutation (all batches) */
+    float repack_ms;                      /* widening repack */
+    float inject_ms;                      /* output-embedding injection */
+    float bookkeeping_ms;                 /* refcount, GC, releases */
+    float reserved0;
+    uint64_t decisions;                   /* policy decisions taken by the last evaluation */
+    uint64_t encoded_turns;               /* turns run through the encoder */
+    uint64_t games;                       /* episodes played */
+    uint64_t children;                    /* children assembled by the last advance */
+} ne_stage_times;
+
+/* Callbacks of TryAdvanceGenerationOnDevice (slab_runtime.hpp:81-84).  Non-zero return = failure. */
+typedef struct ne_callbacks {
+    int (*post_fitness_evaluation)(ne_runtime *rt, const ne_word_counts *counts, void *user);
+    int (*training_shard_release)(ne_runtime *rt, const ne_word_counts *counts,
+                                  ne_pending_injection *inout_injection, void *user);
+    void *user;
+} ne_callbacks;
+
+/* Trace kinds for ne_download_trace (parity mode, see ne_set_trace). */
+enum {
+    NE_TRACE_GAMES = 1,        /* per game: ne_game_trace */
+    NE_TRACE_POLICY = 2,       /* per game and decision: 64 fp32 policy values */
+    NE_TRACE_XOVER = 4         /* per child: 16 uint32 crossover decision words */
+};
+
+typedef struct ne_game_trace {
+    uint32_t guesses[6];       /* action index per decision, 0xFFFFFFFF when unused */
+    float best[6];             /* fp32 score of the chosen action */
+    float second[6];           /* best fp32 score among the other unmasked actions */
+    uint32_t decisions;
+    uint32_t turn_count;
+    float score;
+    uint32_t won;
+} ne_game_trace;
+
+/* ---- lifecycle ---------------------------------------------------------------------------- */
+/* TryCreateDeviceSlabGARuntimeBuffers (slab_runtime.hpp:119). */
+int ne_create(const ne_runtime_config *config, ne_runtime **out_runtime);
+/* DestroyDeviceSlabGARuntimeBuffers (slab_runtime.hpp:121). */
+void ne_destroy(ne_runtime *rt);
+
+/* UploadTrainingWordCatalogToDeviceConstantMemory (training_data.hpp:375-381): letters[5*n], each
+ * 0..25.  The catalog belongs to the handle, not to the process. */
+int ne_upload_catalog(ne_runtime *rt, const uint8_t *letters, uint64_t word_count);
+
+/* Per-cell training secrets as CSR: cell c plays secret_ids[cell_offsets[c] .. cell_offsets[c+1]).
+ * Replaces the shard runtime set upload (slab_runtime.cu:207-236) when set; ne_clear_secrets
+ * returns to the reference shard schedule built from ne_word_counts each evaluation. */
+int ne_upload_secrets_csr(ne_runtime *rt, const uint32_t *cell_offsets, uint64_t cell_count,
+                          const uint32_t *secret_ids, uint64_t secret_count);
+int ne_clear_secrets(ne_runtime *rt);
+
+/* TryBootstrapRandomCurrentGenerationOnDevice (slab_runtime.hpp:123-124).  init may be NULL. */
+int ne_bootstrap(ne_runtime *rt, uint64_t generation_size, uint32_t generation_seed,
+                 uint64_t generation_index, const ne_init_config *init);
+
+/* TryEvaluateCurrentGenerationFitnessOnDevice (slab_runtime.hpp:137-139). */
+int ne_evaluate(ne_runtime *rt, const ne_word_counts *counts, const ne_shard_release_history *history);
+
+/* TryReadPopulationFitnessSummaryFromDevice (slab_runtime.hpp:141-142). */
+int ne_read_summary(ne_runtime *rt, ne_fitness_summary *out);
+
+/* TryAdvanceGenerationOnDevice (slab_runtime.hpp:144-150): evaluate, plan, widen+inject when
+ * injection->enabled, assemble, swap.  config/injection/callbacks/history may be NULL. */
+int ne_advance_generation(ne_runtime *rt, uint32_t generation_seed, const ne_word_counts *counts,
+                          const ne_assembly_config *config, const ne_pending_injection *injection,
+                          uint32_t verbose, const ne_callbacks *callbacks,
+                          const ne_shard_release_history *history);
+
+/* ---- transfers ---------------------------------------------------------------------------- */
+int ne_get_slab_info(ne_runtime *rt, ne_slab_info *out);
+/* TryDownloadSlabSlotBytesFromDevice (slab_runtime.hpp:128-129). */
+int ne_download_slot(ne_runtime *rt, uint32_t slot_index, void *dst, uint64_t capacity, uint64_t *written);
+/* TryUploadSlabToDevice for one slot (device_runtime.hpp:415); the slot must be occupied. */
+int ne_upload_slot(ne_runtime *rt, uint32_t slot_index, const void *src, uint64_t byte_count);
+/* TryDownloadCurrent/NextGenerationFromDevice (slab_runtime.hpp:131-135); which: 0 current,
+ * 1 next.  Any output pointer may be NULL.  Arrays hold capacity entries. */
+int ne_download_generation(ne_runtime *rt, int which, uint32_t *slot_indices, float *fitness,
+                           uint32_t *evaluation_counts, uint8_t *has_fitness, uint64_t capacity,
+                           uint64_t *count);
+/* Parent pairs of the last plan, 2 uint32 per child. */
+int ne_download_plan(ne_runtime *rt, uint32_t *pairs, uint64_t capacity_pairs, uint64_t *count);
+/* Per-cell local training-word counts of the last evaluation. */
+int ne_download_local_word_counts(ne_runtime *rt, uint32_t *dst, uint64_t capacity, uint64_t *count);
+
+/* ---- status, timing, parity ----------------------------------------------------------------- */
+/* Host only, no device work: the reference shard schedule as per-cell CSR -- a global foundation
+ * shard plus golden-ratio-centred shards whose radius grows every growth period
+ * (TryBuildTrainingDataShardRuntimeSet, DoesTrainingDataShardCoverCell and the local word order of
+ * TryResolveLocalTrainingWord: training_data.hpp:150-316, evaluation_ops.cuh:122-169).
+ * offsets holds rows*cols+1 entries; *id_count receives the number of ids (ids may be NULL to size). */
+int ne_build_shard_csr(const ne_word_counts *counts, const ne_shard_release_history *history,
+                       uint64_t generation_index, uint64_t rows, uint64_t cols, uint64_t epicenter_rows,
+                       uint32_t *offsets, uint32_t *ids, uint64_t ids_capacity, uint64_t *id_count);
+
+/* TryReadDeviceSlabGARuntimeStatus (slab_runtime.hpp:183-184): last sticky status. */
+int ne_read_status(ne_runtime *rt);
+/* DeviceSlabGARuntimeStatusCodeString (slab_runtime.hpp:186). */
+const char *ne_status_string(int status);
+int ne_get_stage_times(ne_runtime *rt, ne_stage_times *out);
+/* Parity mode: flags = OR of NE_TRACE_*; the next evaluate/advance records traces. */
+int ne_set_trace(ne_runtime *rt, uint32_t flags);
+/* Copies a recorded trace; game traces are ordered by individual, then game slot (W fresh games by
+ * entry, then 2W prefilled games: slot W+q is entry q/2, variant 1+q%2). */
+int ne_download_trace(ne_runtime *rt, uint32_t kind, void *dst, uint64_t capacity_bytes, uint64_t *written);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NE_RUNTIME_H */

@@ -1,0 +1,938 @@
+// ne_kernels.hip -- HIP kernels of the GA generation step for gfx950 (CDNA4, wave64).
+//
+// Evaluation: one workgroup per genome (blockIdx.y splits large game lists), one wave per tile of
+// 16 games.  All dense contractions run on v_mfma_f32_16x16x4_f32 in the transposed form
+// (weights = A operand, games = B operand), so each layer's accumulator layout is directly the
+// next layer's B operand and activations never leave registers.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ne_device.hpp"
+#include "ne_kernels.hpp"
+
+namespace ne {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ inline void SetStatus(int* status, int code) { atomicCAS(status, 0, code); }
+
+__device__ inline v4f Mfma(float a, float b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+__device__ inline v4f Load4(const float* p) { return *reinterpret_cast<const v4f*>(p); }
+
+// Raw buffer loads: one 128-bit descriptor (SGPRs) per genome, a 32-bit per-lane byte offset and a
+// uniform byte offset, so unrolled loops do not each hold a 64-bit per-lane address.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr int kRsrcFlags = 0x00020000;
+
+__device__ inline Rsrc MakeRsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, int(bytes), kRsrcFlags);
+}
+__device__ inline float BufF(Rsrc r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ inline v4f Buf4(Rsrc r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+
+// Register-only Wordle feedback (no per-letter count array): a non-green position p of letter L is
+// yellow iff the number of earlier non-green positions of letter L is below the number of
+// non-green solution positions holding L -- the same result as the reference's two passes
+// (src/wordle/feedback.hpp:170-203).
+__device__ inline uint32_t FeedbackRegs(uint32_t guess, uint32_t sol) {
+    uint32_t g[5], s[5];
+    bool green[5];
+#pragma unroll
+    for (int p = 0; p < 5; ++p) {
+        g[p] = WordLetter(guess, p);
+        s[p] = WordLetter(sol, p);
+        green[p] = g[p] == s[p];
+    }
+    uint32_t fb = 0;
+#pragma unroll
+    for (int p = 0; p < 5; ++p) {
+        if (green[p]) continue;
+        int avail = 0, before = 0;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            avail += (!green[q] && s[q] == g[p]) ? 1 : 0;
+            before += (q < p && !green[q] && g[q] == g[p]) ? 1 : 0;
+        }
+        fb |= (before < avail ? 1u : 2u) << (2 * p);
+    }
+    return fb;
+}
+
+// ================================================================================================
+// Evaluation kernel.
+template <bool kTrace>
+__global__ __launch_bounds__(512) void EvaluateKernel(EvalArgs p) {
+    __shared__ float s_enc[kTurnFeatures * kEncOut];  // encoder weights, transposed [feature][out]
+    __shared__ float s_encb[kEncOut];
+
+    const uint32_t n = blockIdx.x;
+    const uint32_t slot = p.slot_indices[n];
+    if (slot >= p.slot_count) {
+        if (threadIdx.x == 0) SetStatus(p.status, 4);  // kInvalidGeneration
+        return;
+    }
+    const float* genome = reinterpret_cast<const float*>(p.slab + uint64_t(slot) * p.stride);
+
+    for (uint32_t idx = threadIdx.x; idx < uint32_t(kEncOut * kTurnFeatures); idx += blockDim.x) {
+        const uint32_t o = idx / kTurnFeatures, k = idx - o * kTurnFeatures;
+        s_enc[k * kEncOut + o] = genome[kOffEncW + idx];
+    }
+    if (threadIdx.x < kEncOut) s_encb[threadIdx.x] = genome[kOffEncB + threadIdx.x];
+    __syncthreads();
+
+    const uint32_t W = p.cell_offsets[n + 1] - p.cell_offsets[n];
+    if (W == 0) {
+        if (threadIdx.x == 0) SetStatus(p.status, 5);  // kInvalidTrainingShard
+        return;
+    }
+    if (threadIdx.x == 0 && blockIdx.y == 0) p.local_w[n] = W;
+    const uint32_t* local = p.secret_ids + p.cell_offsets[n];
+    const uint32_t tiles_fresh = (W + 15) / 16;
+    const uint32_t tiles = tiles_fresh + (2 * W + 15) / 16;
+
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t j = lane & 15u;   // A-operand row / B-operand column within a 16-tile
+    const uint32_t gq = lane >> 4;   // k group (0..3)
+    const uint32_t waves = blockDim.x >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+
+    const Rsrc gr = MakeRsrc(genome, uint32_t(kPolicyParams * 4));
+    // per-lane byte offsets of each operand stream
+    const uint32_t off_b0 = uint32_t(kOffB0 + 4 * gq) * 4;
+    const uint32_t off_virgin = uint32_t(kOffW0 + 4 * gq * kModelIn) * 4;
+    const uint32_t off_fold = uint32_t(kOffW0 + j * kModelIn + 1 + 16 * gq) * 4;
+    const uint32_t off_w1 = uint32_t(kOffW1 + j * kH0 + 4 * gq) * 4;
+    const uint32_t off_b1 = uint32_t(kOffB1 + 4 * gq) * 4;
+    const uint32_t off_w2 = uint32_t(kOffW2 + (16 * (j >> 2) + (j & 3)) * kH1 + 4 * gq) * 4;
+    const uint32_t off_b2 = uint32_t(kOffB2 + 16 * gq) * 4;
+    const float* tails = genome + kPolicyParams;
+    const uint32_t A = p.action_count;
+    const uint32_t a_tiles = (A + 15) / 16;
+
+    for (uint32_t tile = blockIdx.y * waves + wave; tile < tiles; tile += gridDim.y * waves) {
+        const bool fresh = tile < tiles_fresh;
+        const uint32_t game = fresh ? tile * 16 + j : (tile - tiles_fresh) * 16 + j;
+        const bool valid = fresh ? game < W : game < 2 * W;
+        const uint32_t game_slot = fresh ? game : W + game;
+
+        // ---- episode initialisation (evaluation_ops.cuh:171-237)
+        // history: hist[0..tc) are the guessed words, most recent first; (last, prev) are the two
+        // most recent turns with their feedback, which is all the turn-slot fold ever needs
+        uint32_t hist[5] = {0, 0, 0, 0, 0};
+        uint32_t last_code = 0, last_fb = 0, prev_code = 0, prev_fb = 0;
+        GameTrace* tr = nullptr;
+        uint32_t sol = 0;
+        int tc = 0;
+        bool won = false;
+        bool active = false;
+        uint32_t decisions = 0;
+        if (valid) {
+            const uint32_t entry = fresh ? game : (game >> 1);
+            const uint32_t variant = fresh ? 0u : 1u + (game & 1u);
+            sol = p.codes[local[entry]];
+            bool ok = true;
+            if (variant != 0) {
+                const uint32_t first = variant == 1 ? entry + 1 : entry + 3;
+                const uint32_t c1 = p.codes[local[first % W]];
+                const uint32_t c2 = p.codes[local[(first + 1) % W]];
+                tc = 1;
+                won = c1 == sol;
+                if (won) {
+                    ok = false;  // the second prefill would be appended to a finished grid
+                } else {
+                    prev_code = c1;
+                    prev_fb = FeedbackRegs(c1, sol);
+                    last_code = c2;
+                    last_fb = FeedbackRegs(c2, sol);
+                    hist[0] = c2;
+                    hist[1] = c1;
+                    tc = 2;
+                    won = c2 == sol;
+                }
+            }
+            if (!ok) {
+                if (gq == 0) SetStatus(p.status, 6);  // kGuessAppendFailed
+            } else {
+                active = !won && tc < kMaxTurns;
+            }
+            if (kTrace && p.game_trace != nullptr && gq == 0) {
+                tr = p.game_trace + p.game_offsets[n] + game_slot;
+                for (int d = 0; d < 6; ++d) {
+                    tr->guesses[d] = kInvalidSlot;
+                    tr->best[d] = 0.0f;
+                    tr->second[d] = 0.0f;
+                }
+            }
+        }
+
+        const int base_tc = fresh ? 0 : 2;
+        v4f acc[16];  // layer-0 pre-activations, neuron tile t rows 4*gq+r, game column j
+        int folded = 0;
+        for (int step = 0; step < kMaxTurns; ++step) {
+            if (!__any(active)) break;
+            const int ttc = base_tc + step;  // turn count of every active game in this tile
+            if (step == 0 || (fresh && step == 1)) {
+#pragma unroll
+                for (int t = 0; t < 16; ++t) acc[t] = Buf4(gr, off_b0, 64 * t);
+                if (fresh && step == 0) {  // virgin flag: model input x[0] = 1 on an empty grid
+#pragma unroll
+                    for (int t = 0; t < 16; ++t)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) acc[t][r] += BufF(gr, off_virgin, uint32_t(16 * t + r) * kModelIn * 4);
+                }
+                folded = 0;
+            }
+            // ---- fold newly occupied turn slots: acc += W0[:, 1+64*slot ...] . enc(turn slot)
+            for (int ts = folded; ts < ttc; ++ts) {
+                const bool is_last = ts == ttc - 1;
+                const uint32_t code = is_last ? last_code : prev_code;
+                const uint32_t fb = is_last ? last_fb : prev_fb;
+                float e[16];
+#pragma unroll
+                for (int s = 0; s < 16; ++s) e[s] = s_encb[16 * gq + s];
+                // features in increasing index order: 5 letters then 5 tiles (turn_features.hpp:60-81)
+#pragma unroll
+                for (int f = 0; f < 10; ++f) {
+                    const uint32_t k = f < 5 ? uint32_t(f) * 26u + WordLetter(code, f)
+                                             : 130u + uint32_t(f - 5) * 3u + ((fb >> (2 * (f - 5))) & 3u);
+                    const float* col = s_enc + k * kEncOut + 16 * gq;
+#pragma unroll
+                    for (int s = 0; s < 16; s += 4) {
+                        const v4f c = *reinterpret_cast<const v4f*>(col + s);
+                        e[s] += c[0];
+                        e[s + 1] += c[1];
+                        e[s + 2] += c[2];
+                        e[s + 3] += c[3];
+                    }
+                }
+                const uint32_t soff_slot = uint32_t(64 * ts) * 4;
+#pragma unroll
+                for (int t = 0; t < 16; ++t) {
+                    float a[16];
+#pragma unroll
+                    for (int s = 0; s < 16; ++s) a[s] = BufF(gr, off_fold, soff_slot + uint32_t(16 * t * kModelIn + s) * 4);
+#pragma unroll
+                    for (int s = 0; s < 16; ++s) acc[t] = Mfma(a[s], e[s], acc[t]);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            folded = ttc;
+
+            // ---- layer 1: h1 = W1 . relu(h0) + b1
+            v4f h1[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) h1[u] = Buf4(gr, off_b1, 64 * u);
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                float hv[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) hv[r] = fmaxf(acc[t][r], 0.0f);
+                v4f w[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) w[u] = Buf4(gr, off_w1, uint32_t(16 * u * kH0 + 16 * t) * 4);
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) h1[u] = Mfma(w[u][r], hv[r], h1[u]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            // ---- layer 2 with output rows permuted so lane group gq holds policy dims 16*gq..16*gq+15
+            v4f pv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pv[u] = Buf4(gr, off_b2, 16 * u);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                float hv[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) hv[r] = fmaxf(h1[t][r], 0.0f);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const v4f w = Buf4(gr, off_w2, uint32_t(4 * u * kH1 + 16 * t) * 4);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) pv[u] = Mfma(w[r], hv[r], pv[u]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            float pol[16];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) pol[4 * u + r] = pv[u][r];
+
+            // ---- scoring: S = E . P with E rows = [26 fixed | 38 tail] per action; masked argmax
+            float best = -__builtin_inff();
+            float second = -__builtin_inff();
+            uint32_t besta = kInvalidSlot;
+            for (uint32_t at = 0; at < a_tiles; ++at) {
+                const uint32_t arow = at * 16 + j;
+                const uint32_t ac = arow < A ? arow : A - 1;
+                float ev[16];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const uint32_t kk = 16 * gq + 2 * q;
+                    const float* src = kk < uint32_t(kFixed) ? p.fixed + uint64_t(ac) * 32 + kk
+                                                             : tails + uint64_t(ac) * kTail + (kk - kFixed);
+                    const float2 v = *reinterpret_cast<const float2*>(src);
+                    ev[2 * q] = v.x;
+                    ev[2 * q + 1] = v.y;
+                }
+                v4f sc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int s = 0; s < 16; ++s) sc = Mfma(ev[s], pol[s], sc);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const uint32_t a = at * 16 + 4 * gq + r;
+                    if (a >= A) continue;
+                    const float v = sc[r];
+                    const bool better = v > best || (v == best && a < besta);
+                    if (better || (kTrace && v > second)) {
+                        const uint32_t ca = p.codes[a];
+                        bool masked = false;
+#pragma unroll
+                        for (int t = 0; t < 5; ++t) masked |= (t < tc) && hist[t] == ca;
+                        if (!masked) {
+                            if (better) {
+                                second = fmaxf(second, best);
+                                best = v;
+                                besta = a;
+                            } else {
+                                second = v;
+                            }
+                        }
+                    }
+                }
+            }
+            // reduce over the four lane groups that share game column j
+#pragma unroll
+            for (int off = 16; off <= 32; off <<= 1) {
+                const float ob = __shfl_xor(best, off);
+                const uint32_t oa = __shfl_xor(besta, off);
+                const float os = __shfl_xor(second, off);
+                if (ob > best || (ob == best && oa < besta)) {
+                    second = fmaxf(fmaxf(second, best), os);
+                    best = ob;
+                    besta = oa;
+                } else {
+                    second = fmaxf(fmaxf(second, ob), os);
+                }
+            }
+
+            if (kTrace && active && p.policy_trace != nullptr) {
+                float* dst = p.policy_trace +
+                             ((p.game_offsets[n] + game_slot) * uint64_t(kMaxTurns) + decisions) * kPol + 16 * gq;
+#pragma unroll
+                for (int s = 0; s < 16; ++s) dst[s] = pol[s];
+            }
+            // ---- apply the guess (wordle_grid.hpp:308-316)
+            if (active) {
+                if (besta == kInvalidSlot) {
+                    if (gq == 0) SetStatus(p.status, 8);  // kActionSelectionFailed
+                    active = false;
+                } else {
+                    const uint32_t code = p.codes[besta];
+#pragma unroll
+                    for (int t = 4; t > 0; --t) hist[t] = hist[t - 1];
+                    hist[0] = code;
+                    prev_code = last_code;
+                    prev_fb = last_fb;
+                    last_code = code;
+                    last_fb = FeedbackRegs(code, sol);
+                    if (kTrace && tr != nullptr) {  // every active game has taken `step` decisions
+                        tr->guesses[step] = besta;
+                        tr->best[step] = best;
+                        tr->second[step] = second;
+                    }
+                    ++tc;
+                    ++decisions;
+                    won = code == sol;
+                    active = !won && tc < kMaxTurns;
+                }
+            }
+        }
+
+        // ---- score the finished episodes (evaluation_ops.cuh:239-242) and sum over the tile
+        float score = (valid && won) ? 10.0f + float(kMaxTurns - tc) : 0.0f;
+        uint32_t dec = (valid && gq == 0) ? decisions : 0u;
+        // turns that went through the encoder: every turn present at the game's last decision
+        uint32_t enc = (valid && gq == 0 && decisions > 0) ? uint32_t(base_tc) + decisions - 1u : 0u;
+        if (gq != 0) score = 0.0f;
+        if (kTrace && tr != nullptr) {
+            tr->decisions = decisions;
+            tr->turn_count = uint32_t(tc);
+            tr->score = score;
+            tr->won = won ? 1u : 0u;
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            score += __shfl_xor(score, off);
+            dec += __shfl_xor(dec, off);
+            enc += __shfl_xor(enc, off);
+        }
+        if (lane == 0) {
+            atomicAdd(p.raw_fitness + n, score);
+            atomicAdd(p.counters + 0, (unsigned long long)dec);
+            atomicAdd(p.counters + 1, (unsigned long long)enc);
+        }
+    }
+}
+
+// Fitness normalisation (FinalizeSlabGenerationFitnessKernel, slab_runtime.cu:551-586 and
+// NormalizeFitnessForSelection, evaluation_ops.cuh:97-111).
+__global__ void FinalizeKernel(const float* raw, const uint32_t* local_w, uint32_t n_genomes, float* fitness,
+                               uint32_t* eval_counts, uint8_t* has_fitness, int* status) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_genomes) return;
+    const uint32_t W = local_w[i];
+    if (W == 0) {
+        SetStatus(status, 5);
+        return;
+    }
+    const float maximum = 3.0f * 15.0f * float(W);
+    const float f = raw[i] / maximum;
+    fitness[i] = f >= 1.0f ? 1.0f : (f > kFitnessFloor ? f : kFitnessFloor);
+    eval_counts[i] += 1u;
+    has_fitness[i] = 1;
+}
+
+// Population summary (SummarizeSlabGenerationKernel, slab_runtime.cu:588-631): sequential fp32 sum.
+__global__ void SummaryKernel(const float* fitness, const uint8_t* has_fitness, const uint32_t* slots,
+                              uint32_t n_genomes, SummaryRecord* out, int* status) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float best = 0.0f, sum = 0.0f;
+    uint32_t bi = 0;
+    for (uint32_t i = 0; i < n_genomes; ++i) {
+        if (has_fitness[i] == 0) {
+            SetStatus(status, 9);
+            return;
+        }
+        const float f = fitness[i];
+        sum += f;
+        if (i == 0 || f > best) {
+            best = f;
+            bi = i;
+        }
+    }
+    out->best_fitness = best;
+    out->average_fitness = sum / float(n_genomes);
+    out->best_index = bi;
+    out->best_slot = slots[bi];
+}
+
+// ================================================================================================
+// Parent selection (BuildSlabAssemblyPlanKernel, slab_runtime.cu:633-675;
+// TrySelectCellularParentPairDevice and helpers, selection_ops.cuh:26-263).
+__device__ inline uint64_t SplitMix64(uint64_t v) {
+    v += 0x9E3779B97F4A7C15ull;
+    v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
+    v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
+    return v ^ (v >> 31);
+}
+
+__device__ inline bool RanksBefore(float lf, uint32_t li, float rf, uint32_t ri, uint64_t seed) {
+    if (lf < rf) return true;
+    if (lf > rf) return false;
+    const uint64_t lk = SplitMix64(seed ^ uint64_t(li)), rk = SplitMix64(seed ^ uint64_t(ri));
+    return lk < rk || (lk == rk && li < ri);
+}
+
+__device__ inline uint32_t AxisDist(uint32_t a, uint32_t b, uint32_t side) {
+    const uint32_t f = a >= b ? a - b : b - a;
+    const uint32_t w = side - f;
+    return f < w ? f : w;
+}
+
+__device__ inline uint32_t ChebyshevDist(uint32_t rows, uint32_t cols, uint32_t a, uint32_t b) {
+    const uint32_t dr = AxisDist(a / cols, b / cols, rows);
+    const uint32_t dc = AxisDist(a % cols, b % cols, cols);
+    return dr > dc ? dr : dc;
+}
+
+constexpr int kPlanBlock = 64;
+constexpr int kMaxMoore = 25;
+
+__global__ __launch_bounds__(kPlanBlock) void PlanKernel(PlanArgs p) {
+    __shared__ uint32_t s_cand[kPlanBlock][kMaxMoore];
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= p.child_count) return;
+    const uint32_t cols = p.cols, rows = p.cur_size / cols;
+    const uint32_t focal = (c / cols) * cols + (c % cols);
+    const bool listed = p.radius <= 2;
+    uint32_t* cand = s_cand[threadIdx.x];
+    uint32_t ncand = 0;
+    if (listed) {
+        const int r = int(p.radius);
+        const int fr = int(focal / cols), fc = int(focal % cols);
+        for (int dr = -r; dr <= r; ++dr)
+            for (int dc = -r; dc <= r; ++dc) {
+                int rr = (fr + dr) % int(rows);
+                if (rr < 0) rr += int(rows);
+                int cc = (fc + dc) % int(cols);
+                if (cc < 0) cc += int(cols);
+                const uint32_t idx = uint32_t(rr) * cols + uint32_t(cc);
+                bool dup = false;
+                for (uint32_t i = 0; i < ncand; ++i) dup |= cand[i] == idx;
+                if (!dup) cand[ncand++] = idx;
+            }
+    }
+    const uint32_t total = listed ? ncand : p.cur_size;
+    auto cand_at = [&](uint32_t i) -> uint32_t { return listed ? cand[i] : i; };
+    auto eligible = [&](uint32_t i, uint32_t excluded) -> bool {
+        if (i == excluded || p.has_fitness[i] == 0) return false;
+        return listed || ChebyshevDist(rows, cols, focal, i) <= p.radius;
+    };
+    const uint32_t seed = p.generation_seed ^ kPlanSeedXor;
+    uint32_t chosen[2] = {0, 0};
+    for (int k = 0; k < 2; ++k) {
+        const U4 w = Philox4x32(c, p.gen_index, uint32_t(k), 0, seed, kTagPlan);
+        const uint64_t tie = (uint64_t(w.x) << 32) ^ uint64_t(w.y);
+        const uint32_t excluded = k == 0 ? kInvalidSlot : chosen[0];
+        auto rank_of = [&](uint32_t i) {
+            uint32_t rank = 1;
+            for (uint32_t q = 0; q < total; ++q) {
+                const uint32_t jdx = cand_at(q);
+                if (jdx == i || !eligible(jdx, excluded)) continue;
+                if (RanksBefore(p.fitness[jdx], jdx, p.fitness[i], i, tie)) ++rank;
+            }
+            return rank;
+        };
+        float sum = 0.0f;
+        bool found = false;
+        uint32_t fallback = 0;
+        for (uint32_t q = 0; q < total; ++q) {
+            const uint32_t i = cand_at(q);
+            if (!eligible(i, excluded)) continue;
+            sum += p.rank_weights[rank_of(i)];
+            fallback = i;
+            found = true;
+        }
+        if (!found || sum <= 0.0f) {
+            SetStatus(p.status, 11);  // kParentSelectionFailed
+            return;
+        }
+        const float threshold = UniformOpenClosed(w.z) * sum;
+        float cum = 0.0f;
+        uint32_t pick = fallback;
+        for (uint32_t q = 0; q < total; ++q) {
+            const uint32_t i = cand_at(q);
+            if (!eligible(i, excluded)) continue;
+            cum += p.rank_weights[rank_of(i)];
+            if (threshold <= cum) {
+                pick = i;
+                break;
+            }
+        }
+        chosen[k] = pick;
+    }
+    p.pairs[2 * c] = chosen[0];
+    p.pairs[2 * c + 1] = chosen[1];
+}
+
+// ================================================================================================
+// Crossover decisions (genome_ops.cuh:210-221,264-299,344-376), one thread per child.
+__global__ void XoverDecisionKernel(XoverArgs p) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= p.child_count) return;
+    XoverRecord r;
+    const U4 s0 = Philox4x32(c, p.gen_index, 0, 0, p.seed, kTagXovr);
+    const bool enc_first = !BernoulliDraw(p.t1, s0.x);
+    const bool trunk_first = !BernoulliDraw(p.t1, s0.y);
+    const bool tail_first = !BernoulliDraw(p.t1, s0.z);
+    for (int l = 0; l < 6; ++l) {
+        r.layer_first[l] = 0;
+        r.neuron[l] = kInvalidSlot;
+    }
+    for (int l = 0; l < kDenseLayers; ++l) {
+        const U4 d = Philox4x32(c, p.gen_index, 1 + uint32_t(l), 0, p.seed, kTagXovr);
+        const bool base = l < kEncoderLayers ? enc_first : trunk_first;
+        const bool lf = BernoulliDraw(p.t2, d.x) ? !base : base;
+        r.layer_first[l] = lf ? 1u : 0u;
+        if (BernoulliDraw(p.t3, d.y)) r.neuron[l] = d.z % DenseLayerAt(l).out;
+    }
+    r.tail_first = tail_first ? 1u : 0u;
+    r.row2 = kInvalidSlot;
+    r.splice_row = kInvalidSlot;
+    r.splice_point = 0;
+    const U4 d8 = Philox4x32(c, p.gen_index, 8, 0, p.seed, kTagXovr);
+    const U4 d9 = Philox4x32(c, p.gen_index, 9, 0, p.seed, kTagXovr);
+    if (p.parent_action_count > 0 && BernoulliDraw(p.t2, d8.x)) r.row2 = d8.y % p.parent_action_count;
+    if (p.parent_action_count > 0 && BernoulliDraw(p.t3, d8.z)) {
+        r.splice_row = d8.w % p.parent_action_count;
+        r.splice_point = 1u + d9.x % uint32_t(kTail - 1);
+    }
+    p.records[c] = r;
+}
+
+__device__ inline bool ElementDonorFirst(const XoverRecord& r, uint32_t e) {
+    if (e < uint32_t(kPolicyParams)) {
+#pragma unroll
+        for (int l = 0; l < kDenseLayers; ++l) {
+            const DenseLayerDesc d = DenseLayerAt(l);
+            const bool in_w = e >= d.w && e < d.w + d.in * d.out;
+            const bool in_b = e >= d.b && e < d.b + d.out;
+            if (!in_w && !in_b) continue;
+            const bool lf = r.layer_first[l] != 0;
+            if (r.neuron[l] == kInvalidSlot) return lf;
+            const uint32_t nb = r.neuron[l];
+            const bool hit = in_w ? (e >= d.w + nb * d.in && e < d.w + (nb + 1) * d.in) : (e == d.b + nb);
+            return hit ? !lf : lf;
+        }
+        return true;
+    }
+    const bool base = r.tail_first != 0;
+    const uint32_t t = e - uint32_t(kPolicyParams);
+    bool src = base;
+    const bool in_row2 = r.row2 != kInvalidSlot && t >= r.row2 * kTail && t < (r.row2 + 1) * kTail;
+    if (in_row2) src = !base;
+    if (r.splice_row != kInvalidSlot && t >= r.splice_row * kTail && t < (r.splice_row + 1) * kTail) {
+        const bool row_src = in_row2 ? !base : base;
+        src = (t - r.splice_row * kTail) < r.splice_point ? row_src : !row_src;
+    }
+    return src;
+}
+
+// Crossover + mutation, element-parallel: each thread builds 4 consecutive values of one child
+// (BreedAndMutateGenome, genome_ops.cuh:386-407, on counter-based draws).  Writes the full slot.
+__global__ __launch_bounds__(256) void BreedKernel(BreedArgs p) {
+    const uint32_t bc = blockIdx.y;  // child within the batch
+    const uint32_t child = p.child_offset + bc;
+    __shared__ XoverRecord s_rec;
+    __shared__ const float* s_p1;
+    __shared__ const float* s_p2;
+    __shared__ float* s_child;
+    __shared__ int s_bad;
+    if (threadIdx.x == 0) {
+        s_rec = p.records[child];
+        const uint32_t pa = p.pairs[2 * child], pb = p.pairs[2 * child + 1];
+        const uint32_t sa = p.parent_slots[pa], sb = p.parent_slots[pb];
+        const uint32_t cs = p.free_stack[p.free_count_before - 1 - bc];
+        s_bad = (sa >= p.slot_count || sb >= p.slot_count || cs >= p.slot_count) ? 1 : 0;
+        if (s_bad) SetStatus(p.status, 13);  // kInvalidParentIndex
+        s_p1 = reinterpret_cast<const float*>(p.slab + uint64_t(sa) * p.parent_stride);
+        s_p2 = reinterpret_cast<const float*>(p.slab + uint64_t(sb) * p.parent_stride);
+        s_child = reinterpret_cast<float*>(p.slab + uint64_t(cs) * p.child_stride);
+        if (blockIdx.x == 0 && !s_bad) {
+            p.child_slots[child] = cs;
+            p.slot_state[cs] = 1u;
+        }
+    }
+    __syncthreads();
+    if (s_bad) return;
+    const XoverRecord r = s_rec;
+    const float* p1 = s_p1;
+    const float* p2 = s_p2;
+    float* out = s_child;
+    const uint32_t n_parent = uint32_t(kPolicyParams) + p.parent_action_count * kTail;
+    const uint32_t n_slot = uint32_t(p.child_stride / 4);
+    const bool mutate = p.mutation_p > 0.0f && p.mutation_sigma > 0.0f;
+    const uint32_t quads = n_slot / 4;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += gridDim.x * blockDim.x) {
+        const uint32_t e0 = 4 * q;
+        v4f v = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (e0 < n_parent) {
+            const bool d0 = ElementDonorFirst(r, e0), d1 = ElementDonorFirst(r, e0 + 1);
+            const bool d2 = ElementDonorFirst(r, e0 + 2), d3 = ElementDonorFirst(r, e0 + 3);
+            if (d0 == d1 && d1 == d2 && d2 == d3 && e0 + 4 <= n_parent) {
+                v = Load4((d0 ? p1 : p2) + e0);
+            } else {
+                const bool dd[4] = {d0, d1, d2, d3};
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (e0 + i < n_parent) v[i] = (dd[i] ? p1 : p2)[e0 + i];
+            }
+            if (mutate) {
+                const U4 w = Philox4x32(q, child, p.gen_index, 0, p.seed, kTagMuta);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if (e0 + i < n_parent && BernoulliDraw(p.mutation_p, Lane(w, i))) {
+                        const U4 z = Philox4x32(e0 + i, child, p.gen_index, 0, p.seed, kTagNorm);
+                        v[i] = __builtin_fmaf(p.mutation_sigma, NormalFromWords(z.x, z.y), v[i]);
+                    }
+                }
+            }
+            if (p.row_scale_p > 0.0f) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint32_t e = e0 + i;
+                    if (e < uint32_t(kPolicyParams) || e >= n_parent) continue;
+                    const uint32_t row = (e - uint32_t(kPolicyParams)) / kTail;
+                    const U4 w = Philox4x32(row, child, p.gen_index, 0, p.seed, kTagRscl);
+                    if (BernoulliDraw(p.row_scale_p, w.x))
+                        v[i] = v[i] * (BernoulliDraw(0.5f, w.y) ? 1.02f : 0.98f);
+                }
+            }
+        }
+        *reinterpret_cast<v4f*>(out + e0) = v;
+    }
+}
+
+// ================================================================================================
+// Generation 0 (InitializeRandomGenome, genome_ops.cuh:58-143; BootstrapRandomGenerationKernel,
+// device_runtime.cu:258-314): weights std_l * z, biases 0, tails tail_std * z.
+__global__ __launch_bounds__(256) void BootstrapKernel(BootArgs p) {
+    const uint32_t o = blockIdx.y;
+    __shared__ uint32_t s_slot;
+    if (threadIdx.x == 0) {
+        s_slot = p.free_stack[p.free_count_before - 1 - o];
+        if (blockIdx.x == 0) {
+            p.slots[o] = s_slot;
+            p.slot_state[s_slot] = 1u;
+        }
+    }
+    __syncthreads();
+    float* g = reinterpret_cast<float*>(p.slab + uint64_t(s_slot) * p.stride);
+    const uint32_t n_values = uint32_t(kPolicyParams) + p.action_count * kTail;
+    const uint32_t n_slot = uint32_t(p.stride / 4);
+    for (uint32_t e2 = (blockIdx.x * blockDim.x + threadIdx.x) * 2; e2 < n_slot; e2 += gridDim.x * blockDim.x * 2) {
+        const U4 w = Philox4x32(e2 >> 1, o, p.gen_index, 0, p.seed, kTagBoot);
+        float v[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const uint32_t e = e2 + i;
+            if (e >= n_values) {
+                v[i] = 0.0f;
+                continue;
+            }
+            float sd = p.tail_std;
+            bool bias = false;
+            if (e < uint32_t(kPolicyParams)) {
+#pragma unroll
+                for (int l = 0; l < kDenseLayers; ++l) {
+                    const DenseLayerDesc d = DenseLayerAt(l);
+                    if (e >= d.w && e < d.w + d.in * d.out) sd = p.layer_std[l];
+                    if (e >= d.b && e < d.b + d.out) bias = true;
+                }
+            }
+            const float z = i == 0 ? NormalFromWords(w.x, w.y) : NormalFromWords(w.z, w.w);
+            v[i] = bias ? 0.0f : sd * z;
+        }
+        *reinterpret_cast<float2*>(g + e2) = make_float2(v[0], v[1]);
+    }
+}
+
+// ================================================================================================
+// Slab bookkeeping (reference_counter.hpp:135-220, device_runtime.cu:372-446,619-663).  Slot
+// pushes and pops happen in index order on one thread, so slot assignment is deterministic.
+__global__ void RefcountKernel(const uint32_t* pairs, uint32_t child_count, uint32_t cur_size, uint32_t* refcounts,
+                               int* status) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= child_count) return;
+    const uint32_t a = pairs[2 * c], b = pairs[2 * c + 1];
+    if (a >= cur_size || b >= cur_size) {
+        SetStatus(status, 13);  // kInvalidParentIndex
+        return;
+    }
+    atomicAdd(refcounts + a, 1u);
+    atomicAdd(refcounts + b, 1u);
+}
+
+// Frees every parent whose reference count is zero and whose slot is still held.
+__global__ void CollectReleasedParentsKernel(uint32_t* cur_slots, uint32_t cur_size, const uint32_t* refcounts,
+                                             uint32_t* slot_state, uint32_t* free_stack, uint32_t* free_count,
+                                             uint32_t slot_count, int* status) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t fc = *free_count;
+    for (uint32_t i = 0; i < cur_size; ++i) {
+        if (refcounts[i] != 0) continue;
+        const uint32_t s = cur_slots[i];
+        if (s == kInvalidSlot) continue;
+        if (s >= slot_count || slot_state[s] == 0 || fc >= slot_count) {
+            SetStatus(status, 3);  // kInvalidSlab
+            return;
+        }
+        slot_state[s] = 0;
+        free_stack[fc++] = s;
+        cur_slots[i] = kInvalidSlot;
+    }
+    *free_count = fc;
+}
+
+__global__ void ReleaseBatchKernel(const uint32_t* pairs, uint32_t child_offset, uint32_t batch, uint32_t* refcounts,
+                                   int* status) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= batch) return;
+    const uint32_t i = child_offset + c;
+    const uint32_t a = pairs[2 * i], b = pairs[2 * i + 1];
+    if (atomicSub(refcounts + a, 1u) == 0u || atomicSub(refcounts + b, 1u) == 0u) SetStatus(status, 3);
+}
+
+__global__ void PopSlotsKernel(uint32_t* free_count, uint32_t count) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *free_count -= count;
+}
+
+__global__ void InitSlabMetadataKernel(uint32_t* slot_state, uint32_t* free_stack, uint32_t* free_count,
+                                       uint32_t slot_count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < slot_count) {
+        slot_state[i] = 0;
+        free_stack[i] = slot_count - 1 - i;  // device_runtime.cu:230-256: stack [S-1 ... 0]
+    }
+    if (i == 0) *free_count = slot_count;
+}
+
+// ================================================================================================
+// Widening repack (repacking.hpp:104-287; PrepareSlabForExpandedActionCountKernel,
+// device_runtime.cu:978-1033).  The plan is computed on one thread; the copies run across the chip
+// through a staging buffer so sources and destinations never overlap.
+__global__ void RepackPlanKernel(RepackArgs p) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t survivors = 0;
+    for (uint32_t i = 0; i < p.cur_size; ++i) {
+        if (p.refcounts[i] == 0) continue;
+        const uint32_t s = p.cur_slots[i];
+        if (s == kInvalidSlot || s >= p.old_slot_count || p.slot_state[s] == 0 || p.owner[s] != kInvalidSlot) {
+            SetStatus(p.status, 16);
+            return;
+        }
+        p.owner[s] = i;
+        ++survivors;
+    }
+    if (survivors == 0 || survivors > p.new_slot_count) {
+        SetStatus(p.status, 16);
+        return;
+    }
+    const uint32_t base = p.new_slot_count - survivors;
+    uint32_t rank = 0;
+    for (uint32_t s = 0; s < p.old_slot_count; ++s) {
+        const uint32_t i = p.owner[s];
+        if (i == kInvalidSlot) continue;
+        p.move_src[rank] = s;
+        p.cur_slots[i] = base + rank;
+        ++rank;
+    }
+    for (uint32_t i = 0; i < p.cur_size; ++i)
+        if (p.refcounts[i] == 0) p.cur_slots[i] = kInvalidSlot;
+    for (uint32_t s = 0; s < p.new_slot_count; ++s) p.slot_state[s] = s >= base ? 1u : 0u;
+    for (uint32_t f = 0; f < base; ++f) p.free_stack[f] = base - 1 - f;
+    *p.free_count = base;
+    *p.survivors = survivors;
+}
+
+__global__ void RepackGatherKernel(const uint8_t* slab, uint64_t old_stride, const uint32_t* move_src,
+                                   uint32_t survivors, uint8_t* staging) {
+    const uint32_t r = blockIdx.y;
+    if (r >= survivors) return;
+    const v4f* src = reinterpret_cast<const v4f*>(slab + uint64_t(move_src[r]) * old_stride);
+    v4f* dst = reinterpret_cast<v4f*>(staging + uint64_t(r) * old_stride);
+    const uint64_t n = old_stride / 16;
+    for (uint64_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void RepackScatterKernel(uint8_t* slab, uint64_t new_stride, uint64_t old_stride, uint32_t base,
+                                    uint32_t survivors, const uint8_t* staging) {
+    const uint32_t r = blockIdx.y;
+    if (r >= survivors) return;
+    const v4f* src = reinterpret_cast<const v4f*>(staging + uint64_t(r) * old_stride);
+    v4f* dst = reinterpret_cast<v4f*>(slab + uint64_t(base + r) * new_stride);
+    const uint64_t n_old = old_stride / 16, n_new = new_stride / 16;
+    const v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint64_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_new; i += uint64_t(gridDim.x) * blockDim.x)
+        dst[i] = i < n_old ? src[i] : zero;
+}
+
+__global__ void FillU32Kernel(uint32_t* dst, uint32_t value, uint64_t count) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < count; i += uint64_t(gridDim.x) * blockDim.x)
+        dst[i] = value;
+}
+
+// ================================================================================================
+// Launch wrappers (host side).
+void LaunchEvaluate(const EvalArgs& a, uint32_t n_genomes, uint32_t waves, uint32_t chunks, bool trace,
+                    hipStream_t s) {
+    const dim3 grid(n_genomes, chunks);
+    const dim3 block(64 * waves);
+    if (trace)
+        hipLaunchKernelGGL(EvaluateKernel<true>, grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL(EvaluateKernel<false>, grid, block, 0, s, a);
+}
+
+void LaunchFinalize(const float* raw, const uint32_t* local_w, uint32_t n, float* fitness, uint32_t* evals,
+                    uint8_t* has, int* status, hipStream_t s) {
+    hipLaunchKernelGGL(FinalizeKernel, dim3((n + 255) / 256), dim3(256), 0, s, raw, local_w, n, fitness, evals, has,
+                       status);
+}
+
+void LaunchSummary(const float* fitness, const uint8_t* has, const uint32_t* slots, uint32_t n, SummaryRecord* out,
+                   int* status, hipStream_t s) {
+    hipLaunchKernelGGL(SummaryKernel, dim3(1), dim3(1), 0, s, fitness, has, slots, n, out, status);
+}
+
+void LaunchPlan(const PlanArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(PlanKernel, dim3((a.child_count + kPlanBlock - 1) / kPlanBlock), dim3(kPlanBlock), 0, s, a);
+}
+
+void LaunchXover(const XoverArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(XoverDecisionKernel, dim3((a.child_count + 127) / 128), dim3(128), 0, s, a);
+}
+
+void LaunchBreed(const BreedArgs& a, uint32_t batch, hipStream_t s) {
+    const uint32_t quads = uint32_t(a.child_stride / 16);
+    uint32_t blocks_x = (quads + 255) / 256;
+    if (blocks_x > 64) blocks_x = 64;
+    hipLaunchKernelGGL(BreedKernel, dim3(blocks_x, batch), dim3(256), 0, s, a);
+}
+
+void LaunchBootstrap(const BootArgs& a, uint32_t organisms, hipStream_t s) {
+    const uint32_t pairs = uint32_t(a.stride / 8);
+    uint32_t blocks_x = (pairs + 255) / 256;
+    if (blocks_x > 64) blocks_x = 64;
+    hipLaunchKernelGGL(BootstrapKernel, dim3(blocks_x, organisms), dim3(256), 0, s, a);
+}
+
+void LaunchRefcount(const uint32_t* pairs, uint32_t child_count, uint32_t cur_size, uint32_t* refcounts, int* status,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(RefcountKernel, dim3((child_count + 255) / 256), dim3(256), 0, s, pairs, child_count, cur_size,
+                       refcounts, status);
+}
+
+void LaunchCollectReleased(uint32_t* cur_slots, uint32_t cur_size, const uint32_t* refcounts, uint32_t* slot_state,
+                           uint32_t* free_stack, uint32_t* free_count, uint32_t slot_count, int* status,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(CollectReleasedParentsKernel, dim3(1), dim3(1), 0, s, cur_slots, cur_size, refcounts,
+                       slot_state, free_stack, free_count, slot_count, status);
+}
+
+void LaunchReleaseBatch(const uint32_t* pairs, uint32_t child_offset, uint32_t batch, uint32_t* refcounts,
+                        int* status, hipStream_t s) {
+    hipLaunchKernelGGL(ReleaseBatchKernel, dim3((batch + 255) / 256), dim3(256), 0, s, pairs, child_offset, batch,
+                       refcounts, status);
+}
+
+void LaunchPopSlots(uint32_t* free_count, uint32_t count, hipStream_t s) {
+    hipLaunchKernelGGL(PopSlotsKernel, dim3(1), dim3(1), 0, s, free_count, count);
+}
+
+void LaunchInitSlabMetadata(uint32_t* slot_state, uint32_t* free_stack, uint32_t* free_count, uint32_t slot_count,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(InitSlabMetadataKernel, dim3((slot_count + 255) / 256), dim3(256), 0, s, slot_state, free_stack,
+                       free_count, slot_count);
+}
+
+void LaunchRepackPlan(const RepackArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(RepackPlanKernel, dim3(1), dim3(1), 0, s, a);
+}
+
+void LaunchRepackCopies(uint8_t* slab, uint64_t old_stride, uint64_t new_stride, const uint32_t* move_src,
+                        uint32_t survivors, uint32_t base, uint8_t* staging, hipStream_t s) {
+    const uint32_t bx = 16;
+    hipLaunchKernelGGL(RepackGatherKernel, dim3(bx, survivors), dim3(256), 0, s, slab, old_stride, move_src, survivors,
+                       staging);
+    hipLaunchKernelGGL(RepackScatterKernel, dim3(bx, survivors), dim3(256), 0, s, slab, new_stride, old_stride, base,
+                       survivors, staging);
+}
+
+void LaunchFillU32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t s) {
+    if (count == 0) return;
+    uint64_t blocks = (count + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(FillU32Kernel, dim3(uint32_t(blocks)), dim3(256), 0, s, dst, value, count);
+}
+
+}  // namespace ne

@@ -1,0 +1,137 @@
+// ne_kernels.hpp -- argument blocks and launch wrappers of the GA kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ne_device.hpp"
+
+namespace ne {
+
+struct SummaryRecord {
+    float best_fitness;
+    float average_fitness;
+    uint32_t best_index;
+    uint32_t best_slot;
+};
+
+struct EvalArgs {
+    const uint8_t* slab;
+    uint64_t stride;
+    uint32_t slot_count;
+    uint32_t action_count;         // selectable actions: min(slab A, runtime A)
+    const uint32_t* slot_indices;  // current generation
+    const uint32_t* cell_offsets;  // CSR of local secrets per cell (= individual)
+    const uint32_t* secret_ids;
+    const uint32_t* codes;         // packed catalog words
+    const float* fixed;            // [catalog][32] fixed word features (26 used)
+    float* raw_fitness;            // [N] summed episode scores
+    uint32_t* local_w;             // [N]
+    unsigned long long* counters;  // [0] decisions, [1] encoded turns, [2] games
+    int* status;
+    GameTrace* game_trace;         // optional (trace mode)
+    float* policy_trace;           // optional: [games][6][64]
+    const uint64_t* game_offsets;  // [N] first game index of each individual (trace mode)
+};
+
+struct PlanArgs {
+    const float* fitness;
+    const uint8_t* has_fitness;
+    uint32_t cur_size;
+    uint32_t child_count;
+    uint32_t cols;
+    uint32_t gen_index;
+    uint32_t radius;
+    uint32_t generation_seed;
+    const float* rank_weights;     // [rank] -> weight, rank 1..cur_size
+    uint32_t* pairs;
+    int* status;
+};
+
+struct XoverArgs {
+    uint32_t child_count;
+    uint32_t gen_index;
+    uint32_t seed;
+    uint32_t parent_action_count;
+    float t1, t2, t3;
+    XoverRecord* records;
+};
+
+struct BreedArgs {
+    uint8_t* slab;
+    uint64_t parent_stride;
+    uint64_t child_stride;
+    const uint32_t* pairs;
+    const uint32_t* parent_slots;
+    const XoverRecord* records;
+    const uint32_t* free_stack;
+    uint32_t free_count_before;
+    uint32_t child_offset;
+    uint32_t* child_slots;
+    uint32_t* slot_state;
+    uint32_t parent_action_count;
+    uint32_t gen_index;
+    uint32_t seed;
+    float mutation_p;
+    float mutation_sigma;
+    float row_scale_p;
+    uint32_t slot_count;
+    int* status;
+};
+
+struct BootArgs {
+    uint8_t* slab;
+    uint64_t stride;
+    uint32_t action_count;
+    const uint32_t* free_stack;
+    uint32_t free_count_before;
+    uint32_t* slots;
+    uint32_t* slot_state;
+    uint32_t gen_index;
+    uint32_t seed;
+    float layer_std[kDenseLayers];
+    float tail_std;
+};
+
+struct RepackArgs {
+    uint32_t* cur_slots;
+    uint32_t cur_size;
+    const uint32_t* refcounts;
+    uint32_t* slot_state;
+    uint32_t* free_stack;
+    uint32_t* free_count;
+    uint32_t* owner;       // [old_slot_count], filled with kInvalidSlot by the caller
+    uint32_t* move_src;    // [cur_size]
+    uint32_t* survivors;   // out
+    uint32_t old_slot_count;
+    uint32_t new_slot_count;
+    int* status;
+};
+
+void LaunchEvaluate(const EvalArgs& a, uint32_t n_genomes, uint32_t waves, uint32_t chunks, bool trace,
+                    hipStream_t s);
+void LaunchFinalize(const float* raw, const uint32_t* local_w, uint32_t n, float* fitness, uint32_t* evals,
+                    uint8_t* has, int* status, hipStream_t s);
+void LaunchSummary(const float* fitness, const uint8_t* has, const uint32_t* slots, uint32_t n, SummaryRecord* out,
+                   int* status, hipStream_t s);
+void LaunchPlan(const PlanArgs& a, hipStream_t s);
+void LaunchXover(const XoverArgs& a, hipStream_t s);
+void LaunchBreed(const BreedArgs& a, uint32_t batch, hipStream_t s);
+void LaunchBootstrap(const BootArgs& a, uint32_t organisms, hipStream_t s);
+void LaunchRefcount(const uint32_t* pairs, uint32_t child_count, uint32_t cur_size, uint32_t* refcounts, int* status,
+                    hipStream_t s);
+void LaunchCollectReleased(uint32_t* cur_slots, uint32_t cur_size, const uint32_t* refcounts, uint32_t* slot_state,
+                           uint32_t* free_stack, uint32_t* free_count, uint32_t slot_count, int* status,
+                           hipStream_t s);
+void LaunchReleaseBatch(const uint32_t* pairs, uint32_t child_offset, uint32_t batch, uint32_t* refcounts,
+                        int* status, hipStream_t s);
+void LaunchPopSlots(uint32_t* free_count, uint32_t count, hipStream_t s);
+void LaunchInitSlabMetadata(uint32_t* slot_state, uint32_t* free_stack, uint32_t* free_count, uint32_t slot_count,
+                            hipStream_t s);
+void LaunchRepackPlan(const RepackArgs& a, hipStream_t s);
+void LaunchRepackCopies(uint8_t* slab, uint64_t old_stride, uint64_t new_stride, const uint32_t* move_src,
+                        uint32_t survivors, uint32_t base, uint8_t* staging, hipStream_t s);
+void LaunchFillU32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t s);
+
+}  // namespace ne

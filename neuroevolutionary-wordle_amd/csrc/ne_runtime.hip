@@ -1,0 +1,1005 @@
+// ne_runtime.hip -- host orchestration of the GA generation step behind the C ABI in
+// include/ne_runtime.h.  One handle owns its slab, generation buffers, catalog and stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/ne_runtime.h"
+#include "ne_device.hpp"
+#include "ne_kernels.hpp"
+
+using namespace ne;
+
+namespace {
+
+constexpr uint64_t kCatalogCapacity = 1u << 20;
+
+struct StageEvents {
+    hipEvent_t begin = nullptr, end = nullptr;
+};
+
+}  // namespace
+
+struct ne_runtime {
+    ne_runtime_config config{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+
+    // slab
+    uint8_t* slab = nullptr;
+    uint64_t slab_bytes = 0;
+    uint64_t action_count = 0;
+    uint64_t stride = 0;
+    uint64_t slot_count = 0;
+    uint32_t* slot_state = nullptr;
+    uint32_t* free_stack = nullptr;
+    uint32_t* free_count = nullptr;  // device scalar
+
+    // generations
+    uint64_t max_gen = 0;
+    uint64_t generation_byte_budget = 0;
+    uint64_t cur_index = 0, cur_size = 0, next_index = 0, next_size = 0, planned_children = 0;
+    uint32_t* cur_slots = nullptr;
+    float* cur_fit = nullptr;
+    uint32_t* cur_evals = nullptr;
+    uint8_t* cur_has = nullptr;
+    uint32_t* nxt_slots = nullptr;
+    float* nxt_fit = nullptr;
+    uint32_t* nxt_evals = nullptr;
+    uint8_t* nxt_has = nullptr;
+
+    // plan and assembly
+    uint32_t* pairs = nullptr;
+    uint32_t* refcounts = nullptr;
+    XoverRecord* xover = nullptr;
+    float* rank_weights = nullptr;
+    uint64_t rank_weights_len = 0;
+    float rank_weights_exponent = -1.0f;
+
+    // evaluation
+    float* raw = nullptr;
+    uint32_t* local_w = nullptr;
+    unsigned long long* counters = nullptr;
+    int* status = nullptr;
+    SummaryRecord* summary = nullptr;
+    uint32_t* d_offsets = nullptr;
+    uint32_t* d_secrets = nullptr;
+    uint64_t d_secrets_cap = 0;
+    std::vector<uint32_t> h_offsets, h_secrets;
+    bool csr_from_caller = false;
+
+    // catalog
+    uint32_t* codes = nullptr;
+    float* fixed = nullptr;
+    uint64_t catalog_count = 0;
+    std::vector<uint8_t> h_catalog;
+
+    // repack
+    uint32_t* owner = nullptr;
+    uint64_t owner_cap = 0;
+    uint32_t* move_src = nullptr;
+    uint32_t* d_survivors = nullptr;
+    uint8_t* staging = nullptr;
+    uint64_t staging_cap = 0;
+
+    // trace
+    uint32_t trace_flags = 0;
+    GameTrace* d_trace = nullptr;
+    float* d_policy_trace = nullptr;
+    uint64_t* d_game_offsets = nullptr;
+    uint64_t trace_games = 0;
+    uint64_t trace_capacity = 0;
+
+    // timing
+    StageEvents ev[8];
+    ne_stage_times times{};
+    ne_fitness_summary last_summary{};
+    bool have_summary = false;
+    int last_status = NE_OK;
+};
+
+namespace {
+
+bool Ok(hipError_t e) { return e == hipSuccess; }
+
+template <typename T>
+bool Alloc(T** p, uint64_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return Ok(hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T)));
+}
+
+template <typename T>
+void Free(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+int Fail(ne_runtime* rt, int code) {
+    if (rt) rt->last_status = code;
+    return code;
+}
+
+int Sync(ne_runtime* rt) {
+    if (!Ok(hipStreamSynchronize(rt->stream))) return Fail(rt, NE_DEVICE_FAILURE);
+    return NE_OK;
+}
+
+// Synchronise, then read and clear the sticky device status (first error wins).
+int Finish(ne_runtime* rt) {
+    if (Sync(rt) != NE_OK) return NE_DEVICE_FAILURE;
+    int st = 0;
+    if (!Ok(hipMemcpy(&st, rt->status, sizeof(int), hipMemcpyDeviceToHost))) return Fail(rt, NE_DEVICE_FAILURE);
+    if (st != 0) {
+        (void)hipMemset(rt->status, 0, sizeof(int));
+        return Fail(rt, st);
+    }
+    return NE_OK;
+}
+
+uint32_t HostFreeCount(ne_runtime* rt) {
+    uint32_t v = 0;
+    (void)hipMemcpy(&v, rt->free_count, sizeof(uint32_t), hipMemcpyDeviceToHost);
+    return v;
+}
+
+uint64_t SlotCountForBudget(uint64_t budget, uint64_t action_count, uint64_t ceiling) {
+    // PopulationSizeForGenotypeBudgetBytes (genome/dynamic_layout.hpp:191-205) on this build's stride.
+    const uint64_t stride = SlotStrideBytes(action_count);
+    if (stride == 0 || budget < stride) return 0;
+    const uint64_t n = budget / stride;
+    return (ceiling == 0 || n < ceiling) ? n : ceiling;
+}
+
+void EventBegin(ne_runtime* rt, int i) { (void)hipEventRecord(rt->ev[i].begin, rt->stream); }
+void EventEnd(ne_runtime* rt, int i) { (void)hipEventRecord(rt->ev[i].end, rt->stream); }
+float EventMs(ne_runtime* rt, int i) {
+    float ms = 0.0f;
+    if (!Ok(hipEventElapsedTime(&ms, rt->ev[i].begin, rt->ev[i].end))) return 0.0f;
+    return ms;
+}
+
+// ---- reference shard schedule -> per-cell CSR -------------------------------------------------
+// TryBuildTrainingDataShardRuntimeSet and DoesTrainingDataShardCoverCell
+// (src/training_folder/training_data.hpp:150-316): a global foundation shard of the first
+// initial_word_count catalog words, then shards of word_count_step words centred on golden-ratio
+// cells whose radius grows every growth period; a cell's local list is the covering shards' words
+// in shard order.
+struct Shard {
+    uint64_t first, count, row, col, radius;
+    bool global;
+};
+
+uint64_t RadiusAt(uint64_t intro, uint64_t gen, uint64_t initial, uint64_t period) {
+    if (period == 0 || gen < intro) return 0;
+    const uint64_t steps = (gen - intro) / period;
+    if (steps > (~uint64_t(0)) - initial) return ~uint64_t(0);
+    return initial + steps;
+}
+
+uint64_t AxisDistance(uint64_t a, uint64_t b, uint64_t side) {
+    const uint64_t f = a >= b ? a - b : b - a;
+    const uint64_t w = side - f;
+    return f < w ? f : w;
+}
+
+int BuildShardCsrCore(const ne_word_counts* wc, const ne_shard_release_history* hist, uint64_t generation,
+                      uint64_t rows, uint64_t cols, uint64_t epi_rows, uint64_t catalog_count,
+                      std::vector<uint32_t>& offsets, std::vector<uint32_t>& secrets) {
+    const ne_word_count_schedule& sc = wc->training_word_schedule;
+    const uint64_t introduced = wc->training_word_count;
+    if (sc.initial_word_count == 0 || sc.word_count_step_period_generations == 0 || introduced == 0 ||
+        introduced > catalog_count || wc->shard_radius_growth_period_generations == 0 || rows == 0 || cols == 0 ||
+        epi_rows == 0)
+        return NE_INVALID_TRAINING_SHARD;
+    const uint64_t epi_cells = epi_rows * cols;
+    std::vector<Shard> shards;
+    const uint64_t foundation = std::min(sc.initial_word_count, introduced);
+    shards.push_back({0, foundation, 0, 0, 0, true});
+    if (sc.word_count_step != 0 && foundation != introduced) {
+        uint64_t next = foundation, local = 0;
+        while (next < introduced) {
+            const uint64_t cnt = std::min(introduced - next, sc.word_count_step);
+            const uint64_t hist_index = shards.size();
+            uint64_t intro = (local + 1) * sc.word_count_step_period_generations;
+            if (hist) {
+                if (hist_index >= hist->release_count) return NE_INVALID_TRAINING_SHARD;
+                intro = hist->release_generations[hist_index];
+            }
+            // DeterministicTrainingShardCenterCoordinate (training_data.hpp:164-178): wrapping
+            // 64-bit product with the golden-ratio stride, then modulo the epicenter cell count
+            const uint64_t cell = ((local + 1) * 11400714819323198485ull) % epi_cells;
+            uint64_t crow = cell / cols;
+            if (crow >= rows) crow = rows - 1;
+            shards.push_back({next, cnt, crow, cell % cols,
+                              RadiusAt(intro, generation, wc->shard_initial_radius,
+                                       wc->shard_radius_growth_period_generations),
+                              false});
+            next += cnt;
+            ++local;
+        }
+    }
+    const uint64_t cells = rows * cols;
+    offsets.assign(cells + 1, 0);
+    secrets.clear();
+    for (uint64_t c = 0; c < cells; ++c) {
+        offsets[c] = uint32_t(secrets.size());
+        for (const Shard& s : shards) {
+            bool covers = s.global;
+            if (!covers) {
+                const uint64_t d = std::max(AxisDistance(s.row, c / cols, rows), AxisDistance(s.col, c % cols, cols));
+                covers = d <= s.radius;
+            }
+            if (!covers) continue;
+            for (uint64_t k = 0; k < s.count; ++k) secrets.push_back(uint32_t(s.first + k));
+        }
+    }
+    offsets[cells] = uint32_t(secrets.size());
+    return NE_OK;
+}
+
+int BuildShardCsr(ne_runtime* rt, const ne_word_counts* wc, const ne_shard_release_history* hist, uint64_t rows,
+                  uint64_t cols, std::vector<uint32_t>& offsets, std::vector<uint32_t>& secrets) {
+    return BuildShardCsrCore(wc, hist, rt->cur_index, rows, cols, rt->max_gen / cols, rt->catalog_count, offsets,
+                             secrets);
+}
+
+int EnsureRankWeights(ne_runtime* rt, float exponent, uint64_t n) {
+    // rank^exponent (selection_ops.cuh:59-61) tabulated once on the host for ranks 1..n.
+    if (rt->rank_weights && rt->rank_weights_len >= n + 1 && rt->rank_weights_exponent == exponent) return NE_OK;
+    Free(rt->rank_weights);
+    std::vector<float> w(n + 1, 0.0f);
+    for (uint64_t r = 1; r <= n; ++r) w[r] = float(std::pow(double(r), double(exponent)));
+    if (!Alloc(&rt->rank_weights, n + 1) ||
+        !Ok(hipMemcpy(rt->rank_weights, w.data(), (n + 1) * sizeof(float), hipMemcpyHostToDevice)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    rt->rank_weights_len = n + 1;
+    rt->rank_weights_exponent = exponent;
+    return NE_OK;
+}
+
+ne_assembly_config DefaultAssembly() {
+    ne_assembly_config c{};
+    c.cellular_breeding_radius = 2;
+    c.rank_exponent = 0.5f;
+    c.crossover_temperature_level1 = 0.02f;
+    c.crossover_temperature_level2 = 0.01f;
+    c.crossover_temperature_level3 = 0.005f;
+    c.mutation_probability = 0.0001f;
+    c.mutation_sigma = 0.02f;
+    c.output_tail_row_scale_mutation_probability = 0.0f;
+    return c;
+}
+
+bool ValidAssembly(const ne_assembly_config& c) {
+    // IsValidGenerationAssemblyConfig (generation_assembly.hpp:15-19).
+    auto unit = [](float v) { return v >= 0.0f && v <= 1.0f; };
+    return c.cellular_breeding_radius > 0 && c.rank_exponent >= 0.0f && c.rank_exponent <= 4.0f &&
+           unit(c.crossover_temperature_level1) && unit(c.crossover_temperature_level2) &&
+           unit(c.crossover_temperature_level3) && unit(c.mutation_probability) && c.mutation_sigma >= 0.0f &&
+           unit(c.output_tail_row_scale_mutation_probability);
+}
+
+int ClearGenerationBuffers(ne_runtime* rt, bool next) {
+    uint32_t* slots = next ? rt->nxt_slots : rt->cur_slots;
+    LaunchFillU32(slots, kInvalidSlot, rt->max_gen, rt->stream);
+    bool ok = Ok(hipMemsetAsync(next ? rt->nxt_fit : rt->cur_fit, 0, rt->max_gen * sizeof(float), rt->stream));
+    ok &= Ok(hipMemsetAsync(next ? rt->nxt_evals : rt->cur_evals, 0, rt->max_gen * sizeof(uint32_t), rt->stream));
+    ok &= Ok(hipMemsetAsync(next ? rt->nxt_has : rt->cur_has, 0, rt->max_gen, rt->stream));
+    return ok ? NE_OK : Fail(rt, NE_DEVICE_FAILURE);
+}
+
+int EnsureTrace(ne_runtime* rt, uint64_t games) {
+    if (rt->trace_capacity >= games && rt->d_trace) return NE_OK;
+    Free(rt->d_trace);
+    Free(rt->d_policy_trace);
+    if (!Alloc(&rt->d_trace, games)) return Fail(rt, NE_DEVICE_FAILURE);
+    if ((rt->trace_flags & NE_TRACE_POLICY) && !Alloc(&rt->d_policy_trace, games * uint64_t(kMaxTurns) * kPol))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    rt->trace_capacity = games;
+    return NE_OK;
+}
+
+}  // namespace
+
+// TryEvaluateCurrentGenerationFitnessOnDevice (slab_runtime.cu:1086-1175): per-cell secrets, one
+// game-play kernel, then fitness normalisation and the population summary.
+static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard_release_history* hist) {
+    const uint64_t cols = rt->config.grid_column_count;
+    if (!wc) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (rt->cur_size == 0 || cols == 0 || rt->cur_size % cols != 0) return Fail(rt, NE_INVALID_GENERATION);
+    if (rt->catalog_count == 0) return Fail(rt, NE_INVALID_TRAINING_SHARD);
+    if (wc->training_word_count > rt->catalog_count || wc->action_space_word_count > rt->catalog_count ||
+        wc->training_word_count > wc->action_space_word_count)
+        return Fail(rt, NE_INVALID_TRAINING_SHARD);
+    const uint64_t rows = rt->cur_size / cols;
+    std::vector<uint32_t> built_offsets, built_secrets;
+    const std::vector<uint32_t>* offs = &rt->h_offsets;
+    const std::vector<uint32_t>* secs = &rt->h_secrets;
+    if (!rt->csr_from_caller) {
+        const int st = BuildShardCsr(rt, wc, hist, rows, cols, built_offsets, built_secrets);
+        if (st != NE_OK) return Fail(rt, st);
+        offs = &built_offsets;
+        secs = &built_secrets;
+    } else if (rt->h_offsets.size() < rt->cur_size + 1) {
+        return Fail(rt, NE_INVALID_TRAINING_SHARD);
+    }
+    for (uint64_t i = 0; i < (*offs)[rt->cur_size]; ++i)
+        if ((*secs)[i] >= rt->catalog_count) return Fail(rt, NE_INVALID_TRAINING_SHARD);
+    const uint64_t n_secrets = (*offs)[rt->cur_size];
+    if (n_secrets > rt->d_secrets_cap) {
+        Free(rt->d_secrets);
+        if (!Alloc(&rt->d_secrets, n_secrets)) return Fail(rt, NE_DEVICE_FAILURE);
+        rt->d_secrets_cap = n_secrets;
+    }
+    bool ok = Ok(hipMemcpyAsync(rt->d_offsets, offs->data(), (rt->cur_size + 1) * sizeof(uint32_t),
+                                hipMemcpyHostToDevice, rt->stream));
+    if (n_secrets)
+        ok &= Ok(hipMemcpyAsync(rt->d_secrets, secs->data(), n_secrets * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                rt->stream));
+    uint64_t max_tiles = 0, total_games = 0;
+    std::vector<uint64_t> game_offsets(rt->cur_size, 0);
+    for (uint64_t c = 0; c < rt->cur_size; ++c) {
+        const uint64_t W = (*offs)[c + 1] - (*offs)[c];
+        if (W == 0) return Fail(rt, NE_INVALID_TRAINING_SHARD);
+        max_tiles = std::max(max_tiles, (W + 15) / 16 + (2 * W + 15) / 16);
+        game_offsets[c] = total_games;
+        total_games += 3 * W;
+    }
+    const bool trace = (rt->trace_flags & (NE_TRACE_GAMES | NE_TRACE_POLICY)) != 0;
+    if (trace) {
+        if (EnsureTrace(rt, total_games) != NE_OK) return rt->last_status;
+        Free(rt->d_game_offsets);
+        if (!Alloc(&rt->d_game_offsets, rt->cur_size)) return Fail(rt, NE_DEVICE_FAILURE);
+        ok &= Ok(hipMemcpyAsync(rt->d_game_offsets, game_offsets.data(), rt->cur_size * sizeof(uint64_t),
+                                hipMemcpyHostToDevice, rt->stream));
+        ok &= Ok(hipMemsetAsync(rt->d_trace, 0xFF, total_games * sizeof(GameTrace), rt->stream));
+        rt->trace_games = total_games;
+    }
+    ok &= Ok(hipMemsetAsync(rt->cur_fit, 0, rt->max_gen * sizeof(float), rt->stream));
+    ok &= Ok(hipMemsetAsync(rt->cur_evals, 0, rt->max_gen * sizeof(uint32_t), rt->stream));
+    ok &= Ok(hipMemsetAsync(rt->cur_has, 0, rt->max_gen, rt->stream));
+    ok &= Ok(hipMemsetAsync(rt->raw, 0, rt->max_gen * sizeof(float), rt->stream));
+    ok &= Ok(hipMemsetAsync(rt->local_w, 0, rt->max_gen * sizeof(uint32_t), rt->stream));
+    ok &= Ok(hipMemsetAsync(rt->counters, 0, 4 * sizeof(unsigned long long), rt->stream));
+    ok &= Ok(hipMemsetAsync(rt->status, 0, sizeof(int), rt->stream));
+    if (!ok) return Fail(rt, NE_DEVICE_FAILURE);
+
+    EvalArgs a{};
+    a.slab = rt->slab;
+    a.stride = rt->stride;
+    a.slot_count = uint32_t(rt->slot_count);
+    a.action_count = uint32_t(std::min(rt->action_count, wc->action_space_word_count));
+    if (a.action_count == 0) return Fail(rt, NE_INVALID_TRAINING_SHARD);
+    a.slot_indices = rt->cur_slots;
+    a.cell_offsets = rt->d_offsets;
+    a.secret_ids = rt->d_secrets;
+    a.codes = rt->codes;
+    a.fixed = rt->fixed;
+    a.raw_fitness = rt->raw;
+    a.local_w = rt->local_w;
+    a.counters = rt->counters;
+    a.status = rt->status;
+    a.game_trace = trace ? rt->d_trace : nullptr;
+    a.policy_trace = (rt->trace_flags & NE_TRACE_POLICY) ? rt->d_policy_trace : nullptr;
+    a.game_offsets = trace ? rt->d_game_offsets : nullptr;
+    const uint32_t waves = uint32_t(std::min<uint64_t>(max_tiles, 8));
+    const uint32_t chunks = uint32_t((max_tiles + waves - 1) / waves);
+    EventBegin(rt, 0);
+    LaunchEvaluate(a, uint32_t(rt->cur_size), waves, chunks, trace, rt->stream);
+    EventEnd(rt, 0);
+    if (!Ok(hipGetLastError())) return Fail(rt, NE_DEVICE_FAILURE);
+    int st = Finish(rt);
+    if (st != NE_OK) return st;
+    EventBegin(rt, 1);
+    LaunchFinalize(rt->raw, rt->local_w, uint32_t(rt->cur_size), rt->cur_fit, rt->cur_evals, rt->cur_has, rt->status,
+                   rt->stream);
+    LaunchSummary(rt->cur_fit, rt->cur_has, rt->cur_slots, uint32_t(rt->cur_size), rt->summary, rt->status,
+                  rt->stream);
+    EventEnd(rt, 1);
+    st = Finish(rt);
+    if (st != NE_OK) return st;
+    unsigned long long counters[4] = {0, 0, 0, 0};
+    SummaryRecord sr{};
+    if (!Ok(hipMemcpy(counters, rt->counters, sizeof(counters), hipMemcpyDeviceToHost)) ||
+        !Ok(hipMemcpy(&sr, rt->summary, sizeof(sr), hipMemcpyDeviceToHost)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    rt->times.evaluate_ms = EventMs(rt, 0);
+    rt->times.finalize_ms = EventMs(rt, 1);
+    rt->times.decisions = counters[0];
+    rt->times.games = total_games;
+    // encoded turns: a game with d >= 1 decisions encodes every turn present at its last decision
+    rt->times.encoded_turns = counters[1];
+    rt->last_summary.best_fitness = sr.best_fitness;
+    rt->last_summary.average_fitness = sr.average_fitness;
+    rt->last_summary.best_index = sr.best_index;
+    rt->last_summary.best_slot_index = sr.best_slot;
+    rt->last_summary.generation_index = rt->cur_index;
+    rt->last_summary.action_count = rt->action_count;
+    rt->last_summary.population_size = rt->cur_size;
+    rt->have_summary = true;
+    rt->last_status = NE_OK;
+    return NE_OK;
+}
+
+// TryPlanNextGenerationShape (slab_runtime.cu:919-939).
+static bool PlanShape(ne_runtime* rt, const ne_pending_injection& inj, uint64_t* next_a, uint64_t* next_n) {
+    *next_a = rt->action_count;
+    if (inj.enabled && (inj.injection_count == 0 || inj.first_catalog_word_index != rt->action_count)) return false;
+    if (inj.enabled) *next_a += inj.injection_count;
+    uint64_t n = SlotCountForBudget(rt->generation_byte_budget, *next_a, rt->max_gen);
+    n = (n / rt->config.grid_column_count) * rt->config.grid_column_count;
+    if (n > rt->cur_size) n = rt->cur_size;
+    *next_n = n;
+    return n > 0;
+}
+
+static int EnsureRepackBuffers(ne_runtime* rt, uint64_t staging_bytes) {
+    if (rt->owner_cap < rt->slot_count) {
+        Free(rt->owner);
+        if (!Alloc(&rt->owner, rt->slot_count)) return Fail(rt, NE_DEVICE_FAILURE);
+        rt->owner_cap = rt->slot_count;
+    }
+    if (!rt->move_src && !Alloc(&rt->move_src, rt->max_gen)) return Fail(rt, NE_DEVICE_FAILURE);
+    if (!rt->d_survivors && !Alloc(&rt->d_survivors, 1)) return Fail(rt, NE_DEVICE_FAILURE);
+    if (rt->staging_cap < staging_bytes) {
+        Free(rt->staging);
+        if (!Alloc(&rt->staging, staging_bytes)) return Fail(rt, NE_DEVICE_FAILURE);
+        rt->staging_cap = staging_bytes;
+    }
+    return NE_OK;
+}
+
+// TryPrepareSlabForExpandedActionCountOnDevice (genotype_slab/device_runtime.cu:1338-1432).
+static int Widen(ne_runtime* rt, uint64_t next_a) {
+    const uint64_t new_stride = SlotStrideBytes(next_a);
+    const uint64_t new_slots = rt->slab_bytes / new_stride;
+    if (next_a <= rt->action_count || new_slots == 0 || rt->planned_children == 0)
+        return Fail(rt, NE_SLAB_REPACK_FAILED);
+    if (EnsureRepackBuffers(rt, rt->cur_size * rt->stride) != NE_OK) return rt->last_status;
+    bool ok = Ok(hipMemsetAsync(rt->refcounts, 0, rt->max_gen * sizeof(uint32_t), rt->stream));
+    if (!ok) return Fail(rt, NE_DEVICE_FAILURE);
+    EventBegin(rt, 4);
+    LaunchRefcount(rt->pairs, uint32_t(rt->planned_children), uint32_t(rt->cur_size), rt->refcounts, rt->status,
+                   rt->stream);
+    LaunchCollectReleased(rt->cur_slots, uint32_t(rt->cur_size), rt->refcounts, rt->slot_state, rt->free_stack,
+                          rt->free_count, uint32_t(rt->slot_count), rt->status, rt->stream);
+    LaunchFillU32(rt->owner, kInvalidSlot, rt->slot_count, rt->stream);
+    RepackArgs ra{};
+    ra.cur_slots = rt->cur_slots;
+    ra.cur_size = uint32_t(rt->cur_size);
+    ra.refcounts = rt->refcounts;
+    ra.slot_state = rt->slot_state;
+    ra.free_stack = rt->free_stack;
+    ra.free_count = rt->free_count;
+    ra.owner = rt->owner;
+    ra.move_src = rt->move_src;
+    ra.survivors = rt->d_survivors;
+    ra.old_slot_count = uint32_t(rt->slot_count);
+    ra.new_slot_count = uint32_t(new_slots);
+    ra.status = rt->status;
+    // slot_state/free_stack must hold new_slots entries: they were sized for the largest slot count
+    if (new_slots > rt->slot_count) return Fail(rt, NE_SLAB_REPACK_FAILED);
+    LaunchRepackPlan(ra, rt->stream);
+    int st = Finish(rt);
+    if (st != NE_OK) return Fail(rt, st == NE_INVALID_SLAB ? NE_INVALID_SLAB : NE_SLAB_REPACK_FAILED);
+    uint32_t survivors = 0;
+    if (!Ok(hipMemcpy(&survivors, rt->d_survivors, sizeof(uint32_t), hipMemcpyDeviceToHost)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    LaunchRepackCopies(rt->slab, rt->stride, new_stride, rt->move_src, survivors, uint32_t(new_slots - survivors),
+                       rt->staging, rt->stream);
+    EventEnd(rt, 4);
+    st = Finish(rt);
+    if (st != NE_OK) return st;
+    rt->times.repack_ms = EventMs(rt, 4);
+    rt->action_count = next_a;
+    rt->stride = new_stride;
+    rt->slot_count = new_slots;
+    return NE_OK;
+}
+
+// Assembly: validate, reference counts, GC of unreferenced parents, batches of children bounded by
+// free slots, parent release after each batch (device_runtime.cu:1434-1557).
+static int Assemble(ne_runtime* rt, uint32_t seed, const ne_assembly_config& cfg, uint64_t parent_a) {
+    rt->next_index = rt->cur_index + 1;
+    rt->next_size = rt->planned_children;
+    int st = ClearGenerationBuffers(rt, true);
+    if (st != NE_OK) return st;
+    if (!Ok(hipMemsetAsync(rt->refcounts, 0, rt->max_gen * sizeof(uint32_t), rt->stream))) return Fail(rt, NE_DEVICE_FAILURE);
+    EventBegin(rt, 6);
+    LaunchRefcount(rt->pairs, uint32_t(rt->next_size), uint32_t(rt->cur_size), rt->refcounts, rt->status, rt->stream);
+    LaunchCollectReleased(rt->cur_slots, uint32_t(rt->cur_size), rt->refcounts, rt->slot_state, rt->free_stack,
+                          rt->free_count, uint32_t(rt->slot_count), rt->status, rt->stream);
+    XoverArgs xa{};
+    xa.child_count = uint32_t(rt->next_size);
+    xa.gen_index = uint32_t(rt->cur_index);
+    xa.seed = seed;
+    xa.parent_action_count = uint32_t(parent_a);
+    xa.t1 = cfg.crossover_temperature_level1;
+    xa.t2 = cfg.crossover_temperature_level2;
+    xa.t3 = cfg.crossover_temperature_level3;
+    xa.records = rt->xover;
+    LaunchXover(xa, rt->stream);
+    EventEnd(rt, 6);
+    st = Finish(rt);
+    if (st != NE_OK) return st;
+    float book_ms = EventMs(rt, 6);
+    float breed_ms = 0.0f;
+    uint64_t offset = 0;
+    while (offset < rt->next_size) {
+        const uint32_t fc = HostFreeCount(rt);
+        if (fc == 0) return Fail(rt, NE_SLAB_FULL);
+        const uint64_t batch = std::min<uint64_t>(rt->next_size - offset, fc);
+        BreedArgs b{};
+        b.slab = rt->slab;
+        b.parent_stride = rt->stride;
+        b.child_stride = rt->stride;
+        b.pairs = rt->pairs;
+        b.parent_slots = rt->cur_slots;
+        b.records = rt->xover;
+        b.free_stack = rt->free_stack;
+        b.free_count_before = fc;
+        b.child_offset = uint32_t(offset);
+        b.child_slots = rt->nxt_slots;
+        b.slot_state = rt->slot_state;
+        b.parent_action_count = uint32_t(parent_a);
+        b.gen_index = uint32_t(rt->cur_index);
+        b.seed = seed;
+        b.mutation_p = cfg.mutation_probability;
+        b.mutation_sigma = cfg.mutation_sigma;
+        b.row_scale_p = cfg.output_tail_row_scale_mutation_probability;
+        b.slot_count = uint32_t(rt->slot_count);
+        b.status = rt->status;
+        EventBegin(rt, 3);
+        LaunchBreed(b, uint32_t(batch), rt->stream);
+        EventEnd(rt, 3);
+        EventBegin(rt, 7);
+        LaunchPopSlots(rt->free_count, uint32_t(batch), rt->stream);
+        LaunchReleaseBatch(rt->pairs, uint32_t(offset), uint32_t(batch), rt->refcounts, rt->status, rt->stream);
+        LaunchCollectReleased(rt->cur_slots, uint32_t(rt->cur_size), rt->refcounts, rt->slot_state, rt->free_stack,
+                              rt->free_count, uint32_t(rt->slot_count), rt->status, rt->stream);
+        EventEnd(rt, 7);
+        st = Finish(rt);
+        if (st != NE_OK) return st;
+        breed_ms += EventMs(rt, 3);
+        book_ms += EventMs(rt, 7);
+        offset += batch;
+    }
+    rt->times.assemble_ms = breed_ms;
+    rt->times.bookkeeping_ms = book_ms;
+    rt->times.children = rt->next_size;
+    return NE_OK;
+}
+
+// SwapDeviceSlabGenerationBuffers (slab_runtime.cu:2128-2135).
+static void SwapGenerations(ne_runtime* rt) {
+    std::swap(rt->cur_slots, rt->nxt_slots);
+    std::swap(rt->cur_fit, rt->nxt_fit);
+    std::swap(rt->cur_evals, rt->nxt_evals);
+    std::swap(rt->cur_has, rt->nxt_has);
+    rt->cur_index = rt->next_index;
+    rt->cur_size = rt->next_size;
+    rt->next_index = 0;
+    rt->next_size = 0;
+    rt->planned_children = 0;
+}
+
+// ================================================================================================
+extern "C" {
+
+int ne_create(const ne_runtime_config* config, ne_runtime** out_runtime) {
+    if (!config || !out_runtime) return NE_INVALID_ARGUMENT;
+    *out_runtime = nullptr;
+    const ne_runtime_config& c = *config;
+    if (c.encoder_hidden != 0) return NE_UNSUPPORTED;
+    if (c.host_spillover_byte_budget != 0) return NE_UNSUPPORTED;
+    // IsValidDeviceSlabGARuntimeConfig (slab_runtime.hpp:50-60) on this build's stride.
+    if (c.genotype_slab_byte_budget < c.generation_byte_budget || c.action_count == 0 || c.grid_column_count == 0 ||
+        (c.population_size_ceiling != 0 && c.population_size_ceiling % c.grid_column_count != 0))
+        return NE_INVALID_RUNTIME_CONFIG;
+    const uint64_t slots = SlotCountForBudget(c.genotype_slab_byte_budget, c.action_count, 0);
+    uint64_t max_gen = SlotCountForBudget(c.generation_byte_budget, c.action_count, c.population_size_ceiling);
+    max_gen = (max_gen / c.grid_column_count) * c.grid_column_count;
+    if (slots == 0 || max_gen == 0 || slots < max_gen || slots >= 0xFFFFFFFFull) return NE_INVALID_RUNTIME_CONFIG;
+    ne_runtime* rt = new (std::nothrow) ne_runtime();
+    if (!rt) return NE_DEVICE_FAILURE;
+    rt->config = c;
+    (void)hipGetDevice(&rt->device);
+    if (c.stream) {
+        rt->stream = reinterpret_cast<hipStream_t>(c.stream);
+    } else {
+        if (!Ok(hipStreamCreateWithFlags(&rt->stream, hipStreamNonBlocking))) {
+            delete rt;
+            return NE_DEVICE_FAILURE;
+        }
+        rt->own_stream = true;
+    }
+    rt->action_count = c.action_count;
+    rt->stride = SlotStrideBytes(c.action_count);
+    rt->slot_count = slots;
+    rt->slab_bytes = c.genotype_slab_byte_budget;
+    rt->max_gen = max_gen;
+    rt->generation_byte_budget = max_gen * rt->stride;
+    bool ok = Alloc(&rt->slab, slots * rt->stride);
+    ok &= Alloc(&rt->slot_state, slots) && Alloc(&rt->free_stack, slots) && Alloc(&rt->free_count, 1);
+    ok &= Alloc(&rt->cur_slots, max_gen) && Alloc(&rt->cur_fit, max_gen) && Alloc(&rt->cur_evals, max_gen) &&
+          Alloc(&rt->cur_has, max_gen);
+    ok &= Alloc(&rt->nxt_slots, max_gen) && Alloc(&rt->nxt_fit, max_gen) && Alloc(&rt->nxt_evals, max_gen) &&
+          Alloc(&rt->nxt_has, max_gen);
+    ok &= Alloc(&rt->pairs, 2 * max_gen) && Alloc(&rt->refcounts, max_gen) && Alloc(&rt->xover, max_gen);
+    ok &= Alloc(&rt->raw, max_gen) && Alloc(&rt->local_w, max_gen) && Alloc(&rt->counters, 4) &&
+          Alloc(&rt->status, 1) && Alloc(&rt->summary, 1) && Alloc(&rt->d_offsets, max_gen + 1);
+    for (auto& e : rt->ev) ok &= Ok(hipEventCreate(&e.begin)) && Ok(hipEventCreate(&e.end));
+    if (ok) {
+        ok &= Ok(hipMemsetAsync(rt->slab, 0, slots * rt->stride, rt->stream));
+        ok &= Ok(hipMemsetAsync(rt->status, 0, sizeof(int), rt->stream));
+        ok &= Ok(hipMemsetAsync(rt->pairs, 0, 2 * max_gen * sizeof(uint32_t), rt->stream));
+        LaunchInitSlabMetadata(rt->slot_state, rt->free_stack, rt->free_count, uint32_t(slots), rt->stream);
+        rt->times = ne_stage_times{};
+    }
+    if (ok) ok = ClearGenerationBuffers(rt, false) == NE_OK && ClearGenerationBuffers(rt, true) == NE_OK;
+    if (ok) ok = Sync(rt) == NE_OK;
+    if (!ok) {
+        ne_destroy(rt);
+        return NE_DEVICE_FAILURE;
+    }
+    *out_runtime = rt;
+    return NE_OK;
+}
+
+void ne_destroy(ne_runtime* rt) {
+    if (!rt) return;
+    if (rt->stream) (void)hipStreamSynchronize(rt->stream);
+    Free(rt->slab);
+    Free(rt->slot_state);
+    Free(rt->free_stack);
+    Free(rt->free_count);
+    Free(rt->cur_slots);
+    Free(rt->cur_fit);
+    Free(rt->cur_evals);
+    Free(rt->cur_has);
+    Free(rt->nxt_slots);
+    Free(rt->nxt_fit);
+    Free(rt->nxt_evals);
+    Free(rt->nxt_has);
+    Free(rt->pairs);
+    Free(rt->refcounts);
+    Free(rt->xover);
+    Free(rt->rank_weights);
+    Free(rt->raw);
+    Free(rt->local_w);
+    Free(rt->counters);
+    Free(rt->status);
+    Free(rt->summary);
+    Free(rt->d_offsets);
+    Free(rt->d_secrets);
+    Free(rt->codes);
+    Free(rt->fixed);
+    Free(rt->owner);
+    Free(rt->move_src);
+    Free(rt->d_survivors);
+    Free(rt->staging);
+    Free(rt->d_trace);
+    Free(rt->d_policy_trace);
+    Free(rt->d_game_offsets);
+    for (auto& e : rt->ev) {
+        if (e.begin) (void)hipEventDestroy(e.begin);
+        if (e.end) (void)hipEventDestroy(e.end);
+    }
+    if (rt->own_stream && rt->stream) (void)hipStreamDestroy(rt->stream);
+    delete rt;
+}
+
+int ne_upload_catalog(ne_runtime* rt, const uint8_t* letters, uint64_t word_count) {
+    if (!rt || !letters || word_count == 0 || word_count > kCatalogCapacity) return Fail(rt, NE_INVALID_ARGUMENT);
+    std::vector<uint32_t> codes(word_count);
+    std::vector<float> fixed(word_count * 32, 0.0f);
+    for (uint64_t a = 0; a < word_count; ++a) {
+        uint32_t code = 0;
+        int counts[kAlpha] = {0};
+        for (int p = 0; p < 5; ++p) {
+            const uint8_t l = letters[5 * a + p];
+            if (l >= kAlpha) return Fail(rt, NE_INVALID_ARGUMENT);
+            code |= uint32_t(l) << (5 * p);
+            ++counts[l];
+        }
+        codes[a] = code;
+        // MaterializeFixedWordFeatures (output_embedding.hpp:179-205): count, or -1 when absent
+        for (int l = 0; l < kAlpha; ++l) fixed[a * 32 + l] = counts[l] == 0 ? -1.0f : float(counts[l]);
+    }
+    Free(rt->codes);
+    Free(rt->fixed);
+    if (!Alloc(&rt->codes, word_count) || !Alloc(&rt->fixed, word_count * 32) ||
+        !Ok(hipMemcpy(rt->codes, codes.data(), word_count * 4, hipMemcpyHostToDevice)) ||
+        !Ok(hipMemcpy(rt->fixed, fixed.data(), word_count * 32 * 4, hipMemcpyHostToDevice)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    rt->catalog_count = word_count;
+    rt->h_catalog.assign(letters, letters + 5 * word_count);
+    return NE_OK;
+}
+
+int ne_upload_secrets_csr(ne_runtime* rt, const uint32_t* cell_offsets, uint64_t cell_count, const uint32_t* secret_ids,
+                          uint64_t secret_count) {
+    if (!rt || !cell_offsets || cell_count == 0 || cell_count > rt->max_gen) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (cell_offsets[0] != 0 || cell_offsets[cell_count] != secret_count || (secret_count && !secret_ids))
+        return Fail(rt, NE_INVALID_ARGUMENT);
+    for (uint64_t c = 0; c < cell_count; ++c)
+        if (cell_offsets[c + 1] < cell_offsets[c]) return Fail(rt, NE_INVALID_ARGUMENT);
+    rt->h_offsets.assign(cell_offsets, cell_offsets + cell_count + 1);
+    rt->h_secrets.assign(secret_ids, secret_ids + secret_count);
+    rt->csr_from_caller = true;
+    return NE_OK;
+}
+
+int ne_clear_secrets(ne_runtime* rt) {
+    if (!rt) return NE_INVALID_ARGUMENT;
+    rt->h_offsets.clear();
+    rt->h_secrets.clear();
+    rt->csr_from_caller = false;
+    return NE_OK;
+}
+
+int ne_bootstrap(ne_runtime* rt, uint64_t generation_size, uint32_t generation_seed, uint64_t generation_index,
+                 const ne_init_config* init) {
+    if (!rt) return NE_INVALID_ARGUMENT;
+    if (generation_size == 0 || generation_size > rt->max_gen || generation_size % rt->config.grid_column_count != 0)
+        return Fail(rt, NE_INVALID_GENERATION);
+    ne_init_config ic{2.0f, 1.0f, 0.05f, 0.0f};
+    if (init) ic = *init;
+    bool ok = ClearGenerationBuffers(rt, false) == NE_OK && ClearGenerationBuffers(rt, true) == NE_OK;
+    ok &= Ok(hipMemsetAsync(rt->pairs, 0, 2 * rt->max_gen * sizeof(uint32_t), rt->stream));
+    ok &= Ok(hipMemsetAsync(rt->refcounts, 0, rt->max_gen * sizeof(uint32_t), rt->stream));
+    ok &= Ok(hipMemsetAsync(rt->status, 0, sizeof(int), rt->stream));
+    if (!ok) return Fail(rt, NE_DEVICE_FAILURE);
+    LaunchInitSlabMetadata(rt->slot_state, rt->free_stack, rt->free_count, uint32_t(rt->slot_count), rt->stream);
+    BootArgs b{};
+    b.slab = rt->slab;
+    b.stride = rt->stride;
+    b.action_count = uint32_t(rt->action_count);
+    b.free_stack = rt->free_stack;
+    b.free_count_before = uint32_t(rt->slot_count);
+    b.slots = rt->cur_slots;
+    b.slot_state = rt->slot_state;
+    b.gen_index = uint32_t(generation_index);
+    b.seed = generation_seed;
+    for (int l = 0; l < kDenseLayers; ++l) {
+        // HeNormalStddev (genome_ops.cuh:61-63): gain * sqrt(scale / fan_in)
+        b.layer_std[l] = ic.weight_gain * std::sqrt(ic.weight_variance_scale / float(DenseLayerAt(l).in));
+    }
+    b.tail_std = ic.tail_stddev;
+    LaunchBootstrap(b, uint32_t(generation_size), rt->stream);
+    LaunchPopSlots(rt->free_count, uint32_t(generation_size), rt->stream);
+    const int st = Finish(rt);
+    if (st != NE_OK) return st;
+    rt->cur_index = generation_index;
+    rt->cur_size = generation_size;
+    rt->next_index = rt->next_size = rt->planned_children = 0;
+    rt->have_summary = false;
+    return NE_OK;
+}
+
+int ne_evaluate(ne_runtime* rt, const ne_word_counts* counts, const ne_shard_release_history* history) {
+    if (!rt) return NE_INVALID_ARGUMENT;
+    return EvaluateFull(rt, counts, history);
+}
+
+int ne_read_summary(ne_runtime* rt, ne_fitness_summary* out) {
+    if (!rt || !out) return NE_INVALID_ARGUMENT;
+    if (!rt->have_summary) return Fail(rt, NE_POPULATION_NOT_EVALUATED);
+    *out = rt->last_summary;
+    return NE_OK;
+}
+
+int ne_advance_generation(ne_runtime* rt, uint32_t generation_seed, const ne_word_counts* counts,
+                          const ne_assembly_config* config, const ne_pending_injection* injection, uint32_t verbose,
+                          const ne_callbacks* callbacks, const ne_shard_release_history* history) {
+    (void)verbose;
+    if (!rt || !counts) return Fail(rt, NE_INVALID_ARGUMENT);
+    const ne_assembly_config cfg = config ? *config : DefaultAssembly();
+    if (!ValidAssembly(cfg)) return Fail(rt, NE_INVALID_ASSEMBLY_CONFIG);
+    ne_pending_injection inj{};
+    if (injection) inj = *injection;
+    rt->times = ne_stage_times{};
+    int st = EvaluateFull(rt, counts, history);
+    if (st != NE_OK) return st;
+    if (callbacks && callbacks->training_shard_release &&
+        callbacks->training_shard_release(rt, counts, &inj, callbacks->user) != 0)
+        return Fail(rt, NE_CALLBACK_FAILED);
+    uint64_t next_a = 0, next_n = 0;
+    if (!PlanShape(rt, inj, &next_a, &next_n)) return Fail(rt, NE_INVALID_ASSEMBLY_CONFIG);
+    if (inj.enabled) return Fail(rt, NE_UNSUPPORTED);  // output-embedding injection is not built yet
+    const uint64_t radius = cfg.cellular_breeding_radius;
+    st = EnsureRankWeights(rt, cfg.rank_exponent, std::max<uint64_t>(rt->cur_size, 25));
+    if (st != NE_OK) return st;
+    PlanArgs pa{};
+    pa.fitness = rt->cur_fit;
+    pa.has_fitness = rt->cur_has;
+    pa.cur_size = uint32_t(rt->cur_size);
+    pa.child_count = uint32_t(next_n);
+    pa.cols = uint32_t(rt->config.grid_column_count);
+    pa.gen_index = uint32_t(rt->cur_index);
+    pa.radius = uint32_t(std::min<uint64_t>(radius, 0xFFFFFFFFu));
+    pa.generation_seed = generation_seed;
+    pa.rank_weights = rt->rank_weights;
+    pa.pairs = rt->pairs;
+    pa.status = rt->status;
+    EventBegin(rt, 2);
+    LaunchPlan(pa, rt->stream);
+    EventEnd(rt, 2);
+    st = Finish(rt);
+    if (st != NE_OK) return st;
+    rt->times.plan_ms = EventMs(rt, 2);
+    rt->planned_children = next_n;
+    if (callbacks && callbacks->post_fitness_evaluation &&
+        callbacks->post_fitness_evaluation(rt, counts, callbacks->user) != 0)
+        return Fail(rt, NE_CALLBACK_FAILED);
+    const uint64_t parent_a = rt->action_count;
+    if (inj.enabled) {
+        st = Widen(rt, next_a);
+        if (st != NE_OK) return st;
+    }
+    st = Assemble(rt, generation_seed, cfg, parent_a);
+    if (st != NE_OK) return st;
+    SwapGenerations(rt);
+    rt->last_status = NE_OK;
+    return NE_OK;
+}
+
+int ne_get_slab_info(ne_runtime* rt, ne_slab_info* out) {
+    if (!rt || !out) return NE_INVALID_ARGUMENT;
+    out->action_count = rt->action_count;
+    out->slot_stride_bytes = rt->stride;
+    out->slot_count = rt->slot_count;
+    out->slab_bytes = rt->slab_bytes;
+    out->policy_parameter_count = kPolicyParams;
+    out->max_generation_size = rt->max_gen;
+    out->current_generation_index = rt->cur_index;
+    out->current_generation_size = rt->cur_size;
+    out->next_generation_size = rt->next_size;
+    out->free_slot_count = HostFreeCount(rt);
+    return NE_OK;
+}
+
+int ne_download_slot(ne_runtime* rt, uint32_t slot_index, void* dst, uint64_t capacity, uint64_t* written) {
+    if (!rt || !dst || slot_index >= rt->slot_count || capacity < rt->stride) return Fail(rt, NE_INVALID_ARGUMENT);
+    uint32_t state = 0;
+    if (!Ok(hipMemcpy(&state, rt->slot_state + slot_index, 4, hipMemcpyDeviceToHost))) return Fail(rt, NE_DEVICE_FAILURE);
+    if (state == 0) return Fail(rt, NE_INVALID_SLAB);
+    if (!Ok(hipMemcpy(dst, rt->slab + uint64_t(slot_index) * rt->stride, rt->stride, hipMemcpyDeviceToHost)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    if (written) *written = rt->stride;
+    return NE_OK;
+}
+
+int ne_upload_slot(ne_runtime* rt, uint32_t slot_index, const void* src, uint64_t byte_count) {
+    if (!rt || !src || slot_index >= rt->slot_count || byte_count > rt->stride) return Fail(rt, NE_INVALID_ARGUMENT);
+    uint32_t state = 0;
+    if (!Ok(hipMemcpy(&state, rt->slot_state + slot_index, 4, hipMemcpyDeviceToHost))) return Fail(rt, NE_DEVICE_FAILURE);
+    if (state == 0) return Fail(rt, NE_INVALID_SLAB);
+    if (!Ok(hipMemcpy(rt->slab + uint64_t(slot_index) * rt->stride, src, byte_count, hipMemcpyHostToDevice)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    return NE_OK;
+}
+
+int ne_download_generation(ne_runtime* rt, int which, uint32_t* slot_indices, float* fitness,
+                           uint32_t* evaluation_counts, uint8_t* has_fitness, uint64_t capacity, uint64_t* count) {
+    if (!rt || (which != 0 && which != 1)) return Fail(rt, NE_INVALID_ARGUMENT);
+    const uint64_t n = which == 0 ? rt->cur_size : rt->next_size;
+    if (count) *count = n;
+    if (n == 0) return Fail(rt, NE_INVALID_GENERATION);
+    if (capacity < n) return Fail(rt, NE_INVALID_ARGUMENT);
+    bool ok = true;
+    if (slot_indices)
+        ok &= Ok(hipMemcpy(slot_indices, which == 0 ? rt->cur_slots : rt->nxt_slots, n * 4, hipMemcpyDeviceToHost));
+    if (fitness) ok &= Ok(hipMemcpy(fitness, which == 0 ? rt->cur_fit : rt->nxt_fit, n * 4, hipMemcpyDeviceToHost));
+    if (evaluation_counts)
+        ok &= Ok(hipMemcpy(evaluation_counts, which == 0 ? rt->cur_evals : rt->nxt_evals, n * 4, hipMemcpyDeviceToHost));
+    if (has_fitness) ok &= Ok(hipMemcpy(has_fitness, which == 0 ? rt->cur_has : rt->nxt_has, n, hipMemcpyDeviceToHost));
+    return ok ? NE_OK : Fail(rt, NE_DEVICE_FAILURE);
+}
+
+int ne_download_plan(ne_runtime* rt, uint32_t* pairs, uint64_t capacity_pairs, uint64_t* count) {
+    if (!rt || !pairs) return Fail(rt, NE_INVALID_ARGUMENT);
+    const uint64_t n = rt->planned_children ? rt->planned_children : rt->cur_size;
+    if (count) *count = n;
+    if (capacity_pairs < n) return Fail(rt, NE_INVALID_ARGUMENT);
+    return Ok(hipMemcpy(pairs, rt->pairs, n * 8, hipMemcpyDeviceToHost)) ? NE_OK : Fail(rt, NE_DEVICE_FAILURE);
+}
+
+int ne_download_local_word_counts(ne_runtime* rt, uint32_t* dst, uint64_t capacity, uint64_t* count) {
+    if (!rt || !dst || capacity < rt->cur_size) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (count) *count = rt->cur_size;
+    return Ok(hipMemcpy(dst, rt->local_w, rt->cur_size * 4, hipMemcpyDeviceToHost)) ? NE_OK
+                                                                                    : Fail(rt, NE_DEVICE_FAILURE);
+}
+
+int ne_build_shard_csr(const ne_word_counts* counts, const ne_shard_release_history* history,
+                       uint64_t generation_index, uint64_t rows, uint64_t cols, uint64_t epicenter_rows,
+                       uint32_t* offsets, uint32_t* ids, uint64_t ids_capacity, uint64_t* id_count) {
+    if (!counts || !offsets || !id_count) return NE_INVALID_ARGUMENT;
+    std::vector<uint32_t> offs, secs;
+    const int st = BuildShardCsrCore(counts, history, generation_index, rows, cols, epicenter_rows, 4739, offs, secs);
+    if (st != NE_OK) return st;
+    *id_count = secs.size();
+    std::memcpy(offsets, offs.data(), offs.size() * sizeof(uint32_t));
+    if (secs.size() > ids_capacity || (!ids && !secs.empty())) return NE_INVALID_ARGUMENT;
+    if (!secs.empty()) std::memcpy(ids, secs.data(), secs.size() * sizeof(uint32_t));
+    return NE_OK;
+}
+
+int ne_read_status(ne_runtime* rt) { return rt ? rt->last_status : NE_INVALID_ARGUMENT; }
+
+const char* ne_status_string(int status) {
+    switch (status) {
+    case NE_OK: return "ok";
+    case NE_DEVICE_FAILURE: return "device failure";
+    case NE_INVALID_RUNTIME_CONFIG: return "invalid runtime config";
+    case NE_INVALID_SLAB: return "invalid slab";
+    case NE_INVALID_GENERATION: return "invalid generation";
+    case NE_INVALID_TRAINING_SHARD: return "invalid training shard";
+    case NE_GUESS_APPEND_FAILED: return "guess append failed";
+    case NE_POLICY_FORWARD_FAILED: return "policy forward failed";
+    case NE_ACTION_SELECTION_FAILED: return "action selection failed";
+    case NE_POPULATION_NOT_EVALUATED: return "population not evaluated";
+    case NE_INVALID_ASSEMBLY_CONFIG: return "invalid assembly config";
+    case NE_PARENT_SELECTION_FAILED: return "parent selection failed";
+    case NE_INVALID_ASSEMBLY_PLAN: return "invalid assembly plan";
+    case NE_INVALID_PARENT_INDEX: return "invalid parent index";
+    case NE_SLAB_FULL: return "slab full";
+    case NE_OUTPUT_EMBEDDING_INJECTION_FAILED: return "output embedding injection failed";
+    case NE_SLAB_REPACK_FAILED: return "slab repack failed";
+    case NE_INVALID_ARGUMENT: return "invalid argument";
+    case NE_UNSUPPORTED: return "unsupported configuration";
+    case NE_CALLBACK_FAILED: return "callback failed";
+    default: return "unknown status";
+    }
+}
+
+int ne_get_stage_times(ne_runtime* rt, ne_stage_times* out) {
+    if (!rt || !out) return NE_INVALID_ARGUMENT;
+    *out = rt->times;
+    return NE_OK;
+}
+
+int ne_set_trace(ne_runtime* rt, uint32_t flags) {
+    if (!rt) return NE_INVALID_ARGUMENT;
+    if ((flags & NE_TRACE_POLICY) && !(rt->trace_flags & NE_TRACE_POLICY)) {
+        Free(rt->d_trace);
+        Free(rt->d_policy_trace);
+        rt->trace_capacity = 0;
+    }
+    rt->trace_flags = flags;
+    return NE_OK;
+}
+
+int ne_download_trace(ne_runtime* rt, uint32_t kind, void* dst, uint64_t capacity_bytes, uint64_t* written) {
+    if (!rt || !dst) return Fail(rt, NE_INVALID_ARGUMENT);
+    const void* src = nullptr;
+    uint64_t bytes = 0;
+    if (kind == NE_TRACE_GAMES) {
+        src = rt->d_trace;
+        bytes = rt->trace_games * sizeof(GameTrace);
+    } else if (kind == NE_TRACE_POLICY) {
+        src = rt->d_policy_trace;
+        bytes = rt->trace_games * uint64_t(kMaxTurns) * kPol * sizeof(float);
+    } else if (kind == NE_TRACE_XOVER) {
+        src = rt->xover;
+        bytes = (rt->cur_size) * sizeof(XoverRecord);
+    } else {
+        return Fail(rt, NE_INVALID_ARGUMENT);
+    }
+    if (!src || bytes == 0) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (capacity_bytes < bytes) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (!Ok(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost))) return Fail(rt, NE_DEVICE_FAILURE);
+    if (written) *written = bytes;
+    return NE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,356 @@
+"""ctypes mirror of include/ne_runtime.h and a thin Python wrapper named after the reference's
+slab_device runtime calls (src/genetic_algorithm/device/slab_runtime.hpp:119-186).
+
+The wrapper only marshals arguments; every call goes through the HIP library libne_wordle.so and
+raises RuntimeError with the runtime's status when a call fails.  Loading fails loudly when the
+library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libne_wordle.so")
+
+STATUS_NAMES = {
+    0: "ok", 1: "device failure", 2: "invalid runtime config", 3: "invalid slab", 4: "invalid generation",
+    5: "invalid training shard", 6: "guess append failed", 7: "policy forward failed",
+    8: "action selection failed", 9: "population not evaluated", 10: "invalid assembly config",
+    11: "parent selection failed", 12: "invalid assembly plan", 13: "invalid parent index", 14: "slab full",
+    15: "output embedding injection failed", 16: "slab repack failed", 17: "invalid argument",
+    18: "unsupported configuration", 19: "callback failed",
+}
+
+TRACE_GAMES = 1
+TRACE_POLICY = 2
+TRACE_XOVER = 4
+
+POLICY_PARAMS = 132928
+TAIL = 38
+
+
+class RuntimeConfig(C.Structure):
+    _fields_ = [("genotype_slab_byte_budget", C.c_uint64), ("generation_byte_budget", C.c_uint64),
+                ("host_spillover_byte_budget", C.c_uint64), ("action_count", C.c_uint64),
+                ("population_size_ceiling", C.c_uint64), ("grid_column_count", C.c_uint64),
+                ("encoder_hidden", C.c_uint32), ("reserved0", C.c_uint32), ("stream", C.c_void_p)]
+
+
+class WordCountSchedule(C.Structure):
+    _fields_ = [("initial_word_count", C.c_uint64), ("word_count_step", C.c_uint64),
+                ("word_count_step_period_generations", C.c_uint64)]
+
+
+class WordCounts(C.Structure):
+    _fields_ = [("training_word_count", C.c_uint64), ("action_space_word_count", C.c_uint64),
+                ("training_word_schedule", WordCountSchedule), ("shard_initial_radius", C.c_uint64),
+                ("shard_radius_growth_period_generations", C.c_uint64)]
+
+
+class ShardReleaseHistory(C.Structure):
+    _fields_ = [("release_generations", C.POINTER(C.c_uint64)), ("release_count", C.c_uint64)]
+
+
+class AssemblyConfig(C.Structure):
+    _fields_ = [("cellular_breeding_radius", C.c_uint64), ("rank_exponent", C.c_float),
+                ("crossover_temperature_level1", C.c_float), ("crossover_temperature_level2", C.c_float),
+                ("crossover_temperature_level3", C.c_float), ("mutation_probability", C.c_float),
+                ("mutation_sigma", C.c_float), ("output_tail_row_scale_mutation_probability", C.c_float),
+                ("reserved0", C.c_float)]
+
+
+class PendingInjection(C.Structure):
+    _fields_ = [("enabled", C.c_uint32), ("reserved0", C.c_uint32), ("first_catalog_word_index", C.c_uint64),
+                ("injection_count", C.c_uint64)]
+
+
+class InitConfig(C.Structure):
+    _fields_ = [("weight_variance_scale", C.c_float), ("weight_gain", C.c_float), ("tail_stddev", C.c_float),
+                ("reserved0", C.c_float)]
+
+
+class FitnessSummary(C.Structure):
+    _fields_ = [("best_fitness", C.c_float), ("average_fitness", C.c_float), ("best_index", C.c_uint64),
+                ("best_slot_index", C.c_uint32), ("reserved0", C.c_uint32), ("generation_index", C.c_uint64),
+                ("action_count", C.c_uint64), ("population_size", C.c_uint64)]
+
+
+class SlabInfo(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("action_count", "slot_stride_bytes", "slot_count", "slab_bytes",
+                                          "policy_parameter_count", "max_generation_size",
+                                          "current_generation_index", "current_generation_size",
+                                          "next_generation_size", "free_slot_count")]
+
+
+class StageTimes(C.Structure):
+    _fields_ = [("evaluate_ms", C.c_float), ("finalize_ms", C.c_float), ("plan_ms", C.c_float),
+                ("assemble_ms", C.c_float), ("repack_ms", C.c_float), ("inject_ms", C.c_float),
+                ("bookkeeping_ms", C.c_float), ("reserved0", C.c_float), ("decisions", C.c_uint64),
+                ("encoded_turns", C.c_uint64), ("games", C.c_uint64), ("children", C.c_uint64)]
+
+
+GAME_TRACE_DTYPE = np.dtype([("guesses", "<u4", 6), ("best", "<f4", 6), ("second", "<f4", 6),
+                             ("decisions", "<u4"), ("turn_count", "<u4"), ("score", "<f4"), ("won", "<u4")])
+
+POST_FITNESS_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(WordCounts), C.c_void_p)
+SHARD_RELEASE_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(WordCounts), C.POINTER(PendingInjection), C.c_void_p)
+
+
+class Callbacks(C.Structure):
+    _fields_ = [("post_fitness_evaluation", POST_FITNESS_CB), ("training_shard_release", SHARD_RELEASE_CB),
+                ("user", C.c_void_p)]
+
+
+EXPORTED_SYMBOLS = [
+    "ne_create", "ne_destroy", "ne_upload_catalog", "ne_upload_secrets_csr", "ne_clear_secrets", "ne_bootstrap",
+    "ne_evaluate", "ne_read_summary", "ne_advance_generation", "ne_get_slab_info", "ne_download_slot",
+    "ne_upload_slot", "ne_download_generation", "ne_download_plan", "ne_download_local_word_counts",
+    "ne_read_status", "ne_status_string", "ne_get_stage_times", "ne_set_trace", "ne_download_trace",
+    "ne_build_shard_csr",
+]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads libne_wordle.so (raises if it was not built: there is no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP runtime library not built: {path} (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    p, u64, u32, i32 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int
+    sig = {
+        "ne_create": (i32, [C.POINTER(RuntimeConfig), C.POINTER(p)]),
+        "ne_destroy": (None, [p]),
+        "ne_upload_catalog": (i32, [p, p, u64]),
+        "ne_upload_secrets_csr": (i32, [p, p, u64, p, u64]),
+        "ne_clear_secrets": (i32, [p]),
+        "ne_bootstrap": (i32, [p, u64, u32, u64, C.POINTER(InitConfig)]),
+        "ne_evaluate": (i32, [p, C.POINTER(WordCounts), C.POINTER(ShardReleaseHistory)]),
+        "ne_read_summary": (i32, [p, C.POINTER(FitnessSummary)]),
+        "ne_advance_generation": (i32, [p, u32, C.POINTER(WordCounts), C.POINTER(AssemblyConfig),
+                                        C.POINTER(PendingInjection), u32, C.POINTER(Callbacks),
+                                        C.POINTER(ShardReleaseHistory)]),
+        "ne_get_slab_info": (i32, [p, C.POINTER(SlabInfo)]),
+        "ne_download_slot": (i32, [p, u32, p, u64, C.POINTER(u64)]),
+        "ne_upload_slot": (i32, [p, u32, p, u64]),
+        "ne_download_generation": (i32, [p, i32, p, p, p, p, u64, C.POINTER(u64)]),
+        "ne_download_plan": (i32, [p, p, u64, C.POINTER(u64)]),
+        "ne_download_local_word_counts": (i32, [p, p, u64, C.POINTER(u64)]),
+        "ne_read_status": (i32, [p]),
+        "ne_status_string": (C.c_char_p, [i32]),
+        "ne_get_stage_times": (i32, [p, C.POINTER(StageTimes)]),
+        "ne_set_trace": (i32, [p, u32]),
+        "ne_download_trace": (i32, [p, u32, p, u64, C.POINTER(u64)]),
+        "ne_build_shard_csr": (i32, [C.POINTER(WordCounts), C.POINTER(ShardReleaseHistory), u64, u64, u64, u64,
+                                     p, p, u64, C.POINTER(u64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class RuntimeError_(RuntimeError):
+    def __init__(self, call, status):
+        super().__init__(f"{call} failed: status {status} ({STATUS_NAMES.get(status, 'unknown')})")
+        self.status = status
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def default_assembly_config(**kw):
+    c = AssemblyConfig(cellular_breeding_radius=2, rank_exponent=0.5, crossover_temperature_level1=0.02,
+                       crossover_temperature_level2=0.01, crossover_temperature_level3=0.005,
+                       mutation_probability=0.0001, mutation_sigma=0.02,
+                       output_tail_row_scale_mutation_probability=0.0)
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def word_counts(training, actions, initial=None, step=0, period=1, radius0=0, growth=2):
+    return WordCounts(training_word_count=training, action_space_word_count=actions,
+                      training_word_schedule=WordCountSchedule(initial if initial is not None else training, step,
+                                                               period),
+                      shard_initial_radius=radius0, shard_radius_growth_period_generations=growth)
+
+
+def slot_stride_bytes(action_count):
+    return ((POLICY_PARAMS + TAIL * action_count) * 4 + 255) // 256 * 256
+
+
+class SlabRuntime:
+    """One runtime handle: TryCreateDeviceSlabGARuntimeBuffers ... DestroyDeviceSlabGARuntimeBuffers."""
+
+    def __init__(self, slab_bytes, generation_bytes, action_count, grid_columns, population_ceiling=0):
+        self.lib = load_library()
+        cfg = RuntimeConfig(genotype_slab_byte_budget=slab_bytes, generation_byte_budget=generation_bytes,
+                            host_spillover_byte_budget=0, action_count=action_count,
+                            population_size_ceiling=population_ceiling, grid_column_count=grid_columns,
+                            encoder_hidden=0, reserved0=0, stream=None)
+        h = C.c_void_p()
+        st = self.lib.ne_create(C.byref(cfg), C.byref(h))
+        if st != 0:
+            raise RuntimeError_("ne_create", st)
+        self.h = h
+        self.grid_columns = grid_columns
+
+    def close(self):
+        if self.h:
+            self.lib.ne_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, call, st):
+        if st != 0:
+            raise RuntimeError_(call, st)
+
+    # --- inputs
+    def upload_catalog(self, letters):
+        letters = np.ascontiguousarray(letters, dtype=np.uint8)
+        self._check("ne_upload_catalog", self.lib.ne_upload_catalog(self.h, _ptr(letters), letters.shape[0]))
+
+    def upload_secrets_csr(self, offsets, secrets):
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint32)
+        secrets = np.ascontiguousarray(secrets, dtype=np.uint32)
+        self._check("ne_upload_secrets_csr", self.lib.ne_upload_secrets_csr(
+            self.h, _ptr(offsets), offsets.size - 1, _ptr(secrets), secrets.size))
+
+    def clear_secrets(self):
+        self._check("ne_clear_secrets", self.lib.ne_clear_secrets(self.h))
+
+    # --- generation step
+    def bootstrap(self, generation_size, seed, generation_index=0, variance_scale=2.0, gain=1.0, tail_std=0.05):
+        ic = InitConfig(variance_scale, gain, tail_std, 0.0)
+        self._check("ne_bootstrap", self.lib.ne_bootstrap(self.h, generation_size, seed, generation_index,
+                                                          C.byref(ic)))
+
+    def evaluate(self, counts, history=None):
+        hist = None
+        if history is not None:
+            arr = np.ascontiguousarray(history, dtype=np.uint64)
+            self._hist_keep = arr
+            hist = C.byref(ShardReleaseHistory(arr.ctypes.data_as(C.POINTER(C.c_uint64)), arr.size))
+        self._check("ne_evaluate", self.lib.ne_evaluate(self.h, C.byref(counts), hist))
+
+    def advance(self, seed, counts, config=None, injection=None, callbacks=None, history=None):
+        cfg = C.byref(config) if config is not None else None
+        inj = C.byref(injection) if injection is not None else None
+        cbs = C.byref(callbacks) if callbacks is not None else None
+        hist = None
+        if history is not None:
+            arr = np.ascontiguousarray(history, dtype=np.uint64)
+            self._hist_keep = arr
+            hist = C.byref(ShardReleaseHistory(arr.ctypes.data_as(C.POINTER(C.c_uint64)), arr.size))
+        self._check("ne_advance_generation",
+                    self.lib.ne_advance_generation(self.h, seed, C.byref(counts), cfg, inj, 0, cbs, hist))
+
+    def summary(self):
+        s = FitnessSummary()
+        self._check("ne_read_summary", self.lib.ne_read_summary(self.h, C.byref(s)))
+        return s
+
+    # --- transfers
+    def info(self):
+        s = SlabInfo()
+        self._check("ne_get_slab_info", self.lib.ne_get_slab_info(self.h, C.byref(s)))
+        return s
+
+    def generation(self, which=0):
+        info = self.info()
+        cap = info.max_generation_size
+        slots = np.zeros(cap, np.uint32)
+        fit = np.zeros(cap, np.float32)
+        evals = np.zeros(cap, np.uint32)
+        has = np.zeros(cap, np.uint8)
+        n = C.c_uint64()
+        self._check("ne_download_generation", self.lib.ne_download_generation(
+            self.h, which, _ptr(slots), _ptr(fit), _ptr(evals), _ptr(has), cap, C.byref(n)))
+        k = n.value
+        return slots[:k].copy(), fit[:k].copy(), evals[:k].copy(), has[:k].copy()
+
+    def download_slot(self, slot):
+        info = self.info()
+        buf = np.zeros(info.slot_stride_bytes // 4, np.float32)
+        w = C.c_uint64()
+        self._check("ne_download_slot", self.lib.ne_download_slot(self.h, int(slot), _ptr(buf), buf.nbytes,
+                                                                  C.byref(w)))
+        return buf
+
+    def upload_slot(self, slot, values):
+        values = np.ascontiguousarray(values, dtype=np.float32)
+        self._check("ne_upload_slot", self.lib.ne_upload_slot(self.h, int(slot), _ptr(values), values.nbytes))
+
+    def plan(self):
+        info = self.info()
+        cap = info.max_generation_size
+        pairs = np.zeros((cap, 2), np.uint32)
+        n = C.c_uint64()
+        self._check("ne_download_plan", self.lib.ne_download_plan(self.h, _ptr(pairs), cap, C.byref(n)))
+        return pairs[:n.value].copy()
+
+    def local_word_counts(self):
+        info = self.info()
+        out = np.zeros(info.current_generation_size, np.uint32)
+        n = C.c_uint64()
+        self._check("ne_download_local_word_counts",
+                    self.lib.ne_download_local_word_counts(self.h, _ptr(out), out.size, C.byref(n)))
+        return out
+
+    def stage_times(self):
+        s = StageTimes()
+        self._check("ne_get_stage_times", self.lib.ne_get_stage_times(self.h, C.byref(s)))
+        return s
+
+    def set_trace(self, flags):
+        self._check("ne_set_trace", self.lib.ne_set_trace(self.h, flags))
+
+    def download_trace(self, kind, count):
+        if kind == TRACE_GAMES:
+            out = np.zeros(count, GAME_TRACE_DTYPE)
+        elif kind == TRACE_POLICY:
+            out = np.zeros((count, 6, 64), np.float32)
+        else:
+            out = np.zeros((count, 16), np.uint32)
+        w = C.c_uint64()
+        self._check("ne_download_trace", self.lib.ne_download_trace(self.h, kind, _ptr(out), out.nbytes,
+                                                                    C.byref(w)))
+        return out
+
+    def status(self):
+        return self.lib.ne_read_status(self.h)
+
+
+def build_shard_csr(counts, generation_index, rows, cols, epicenter_rows=None, history=None):
+    """Host-only shard schedule -> (offsets, ids) through ne_build_shard_csr."""
+    lib = load_library()
+    epi = rows if epicenter_rows is None else epicenter_rows
+    offsets = np.zeros(rows * cols + 1, np.uint32)
+    n = C.c_uint64()
+    hist = None
+    if history is not None:
+        arr = np.ascontiguousarray(history, dtype=np.uint64)
+        hist = C.byref(ShardReleaseHistory(arr.ctypes.data_as(C.POINTER(C.c_uint64)), arr.size))
+    st = lib.ne_build_shard_csr(C.byref(counts), hist, generation_index, rows, cols, epi, _ptr(offsets), None, 0,
+                                C.byref(n))
+    if st not in (0, 17):
+        raise RuntimeError_("ne_build_shard_csr", st)
+    ids = np.zeros(max(n.value, 1), np.uint32)
+    st = lib.ne_build_shard_csr(C.byref(counts), hist, generation_index, rows, cols, epi, _ptr(offsets), _ptr(ids),
+                                ids.size, C.byref(n))
+    if st != 0:
+        raise RuntimeError_("ne_build_shard_csr", st)
+    return offsets, ids[:n.value]

@@ -1,0 +1,242 @@
+"""ctypes binding of the CPU oracle (liboracle.so) and of the reference-header probe (_ref/).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+leg, never by the product.  See ne_oracle.cpp for what each function restates.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libneref.so")
+
+GAME_TRACE_DTYPE = np.dtype([("guesses", "<u4", 6), ("best", "<f4", 6), ("second", "<f4", 6),
+                             ("decisions", "<u4"), ("turn_count", "<u4"), ("score", "<f4"), ("won", "<u4")])
+
+P = C.c_void_p
+U64, U32, I32, F32 = C.c_uint64, C.c_uint32, C.c_int, C.c_float
+
+_SIGS = {
+    "oro_philox": (None, [P, P, P]),
+    "oro_u01": (F32, [U32]),
+    "oro_normal_from_words": (F32, [U32, U32]),
+    "oro_det_log": (C.c_double, [C.c_double]),
+    "oro_det_sqrt": (C.c_double, [C.c_double]),
+    "oro_policy_param_count": (U64, [I32]),
+    "oro_feedback": (None, [P, P, P]),
+    "oro_policy_forward": (I32, [P, I32, P, P, I32, P, P]),
+    "oro_fixed_features": (None, [P, P]),
+    "oro_encode_turn": (None, [P, I32, P, P, P, P]),
+    "oro_score_action": (F32, [P, P, P]),
+    "oro_evaluate": (I32, [P, U64, I32, U64, U64, P, U64, P, P, I32, I32, P, P, P, P, P, P]),
+    "oro_replay_game": (I32, [P, I32, U64, P, P, U64, U64, P, U32, P, P, P, P, P]),
+    "oro_summary": (None, [P, U64, P, P, P]),
+    "oro_normalize_fitness": (F32, [F32, U64]),
+    "oro_rank_weights": (None, [F32, U64, P]),
+    "oro_moore": (I32, [U64, U64, U64, U64, P]),
+    "oro_chebyshev": (U64, [U64, U64, U64, U64]),
+    "oro_plan": (I32, [P, P, U64, U64, U64, U64, U64, P, U32, P]),
+    "oro_xover_decisions": (None, [I32, U64, U32, U32, U32, F32, F32, F32, P]),
+    "oro_xover_record_words": (U64, []),
+    "oro_breed_child": (None, [I32, P, P, P, U64, U64, U32, U32, U32, F32, F32, F32, F32, F32, F32, P]),
+    "oro_bootstrap_genome": (None, [I32, P, U64, U32, U32, U32, P, F32]),
+}
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_LIB):
+            raise RuntimeError("oracle not built: run oracle/build_oracle.py")
+        L = C.CDLL(ORACLE_LIB)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ref():
+    """The reference-header probe, or None when it was not built (reference sources absent)."""
+    global _ref
+    if _ref is None and os.path.exists(REF_LIB):
+        R = C.CDLL(REF_LIB)
+        R.ref_feedback.argtypes = [P, P, P]
+        R.ref_feedback.restype = I32
+        R.ref_hint_group.argtypes = [P, P, P, P]
+        R.ref_hint_group.restype = I32
+        R.ref_moore.argtypes = [U64, U64, U64, U64, P]
+        R.ref_moore.restype = I32
+        R.ref_chebyshev.argtypes = [U64, U64, U64, U64]
+        R.ref_chebyshev.restype = U64
+        R.ref_shard_csr.argtypes = [U64] * 10 + [P, U64, P, P, U64]
+        R.ref_shard_csr.restype = I32
+        _ref = R
+    return _ref
+
+
+def ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().oro_philox(ptr(c), ptr(k), ptr(out))
+    return out
+
+
+def feedback(guess, sol):
+    g = np.asarray(guess, np.uint8)
+    s = np.asarray(sol, np.uint8)
+    out = np.zeros(5, np.uint8)
+    lib().oro_feedback(ptr(g), ptr(s), ptr(out))
+    return out
+
+
+def word(text):
+    return np.frombuffer(text.encode(), np.uint8) - ord("A")
+
+
+def policy_param_count(enc_hidden=0):
+    return int(lib().oro_policy_param_count(enc_hidden))
+
+
+def policy_forward(genome, turns, enc_hidden=0):
+    """turns: list of (word letters[5], tiles[5]).  Returns (fp32 policy, fp64 policy)."""
+    g = np.ascontiguousarray(genome, np.float32)
+    tc = len(turns)
+    letters = np.zeros((max(tc, 1), 5), np.uint8)
+    tiles = np.zeros((max(tc, 1), 5), np.uint8)
+    for i, (w, t) in enumerate(turns):
+        letters[i] = w
+        tiles[i] = t
+    o32 = np.zeros(64, np.float32)
+    o64 = np.zeros(64, np.float64)
+    st = lib().oro_policy_forward(ptr(g), enc_hidden, ptr(letters), ptr(tiles), tc, ptr(o32), ptr(o64))
+    if st != 0:
+        raise RuntimeError(f"oro_policy_forward status {st}")
+    return o32, o64
+
+
+def evaluate(genomes, enc_hidden, A_genome, A_runtime, catalog, offsets, ids, mode=0, threads=1, trace=False):
+    """genomes: list/array of fp32 genome vectors.  Returns dict with fitness, raw, local_w, decisions,
+    traces (per game, ordered like the runtime's trace)."""
+    N = len(genomes)
+    keep = [np.ascontiguousarray(g, np.float32) for g in genomes]
+    ptrs = (C.c_void_p * N)(*[g.ctypes.data for g in keep])
+    cat = np.ascontiguousarray(catalog, np.uint8)
+    offsets = np.ascontiguousarray(offsets, np.uint32)
+    ids = np.ascontiguousarray(ids, np.uint32)
+    fit = np.zeros(N, np.float32)
+    raw = np.zeros(N, np.float32)
+    lw = np.zeros(N, np.uint32)
+    dec = C.c_uint64()
+    game_offsets = (3 * np.concatenate([[0], np.cumsum(np.diff(offsets.astype(np.uint64)))[:-1]])).astype(np.uint64)
+    traces = np.zeros(int(3 * offsets[-1]), GAME_TRACE_DTYPE) if trace else None
+    st = lib().oro_evaluate(C.cast(ptrs, C.c_void_p), N, enc_hidden, A_genome, A_runtime, ptr(cat), cat.shape[0],
+                            ptr(offsets), ptr(ids), mode, threads, ptr(fit), ptr(raw), ptr(lw), C.byref(dec),
+                            ptr(game_offsets), ptr(traces))
+    return dict(status=st, fitness=fit, raw=raw, local_w=lw, decisions=dec.value, traces=traces,
+                game_offsets=game_offsets)
+
+
+def replay_game(genome, A, catalog, local, slot, guesses, enc_hidden=0):
+    g = np.ascontiguousarray(genome, np.float32)
+    cat = np.ascontiguousarray(catalog, np.uint8)
+    loc = np.ascontiguousarray(local, np.uint32)
+    gs = np.ascontiguousarray(guesses, np.uint32)
+    n = len(gs)
+    pol = np.zeros((max(n, 1), 64), np.float64)
+    best = np.zeros(max(n, 1), np.float64)
+    second = np.zeros(max(n, 1), np.float64)
+    arg = np.zeros(max(n, 1), np.uint32)
+    chosen = np.zeros(max(n, 1), np.float64)
+    st = lib().oro_replay_game(ptr(g), enc_hidden, A, ptr(cat), ptr(loc), loc.size, slot, ptr(gs), n, ptr(pol),
+                               ptr(best), ptr(second), ptr(arg), ptr(chosen))
+    return dict(status=st, policy=pol[:n], best=best[:n], second=second[:n], argmax=arg[:n], chosen=chosen[:n])
+
+
+def summary(fitness):
+    f = np.ascontiguousarray(fitness, np.float32)
+    b = C.c_float()
+    m = C.c_float()
+    i = C.c_uint64()
+    lib().oro_summary(ptr(f), f.size, C.byref(b), C.byref(m), C.byref(i))
+    return b.value, m.value, i.value
+
+
+def rank_weights(exponent, n):
+    w = np.zeros(n + 1, np.float32)
+    lib().oro_rank_weights(exponent, n, ptr(w))
+    return w
+
+
+def plan(fitness, has, cur_size, child_count, cols, gen_index, radius, wtable, generation_seed):
+    f = np.ascontiguousarray(fitness, np.float32)
+    h = np.ascontiguousarray(has, np.uint8)
+    w = np.ascontiguousarray(wtable, np.float32)
+    pairs = np.zeros((child_count, 2), np.uint32)
+    st = lib().oro_plan(ptr(f), ptr(h), cur_size, child_count, cols, gen_index, radius, ptr(w), generation_seed,
+                        ptr(pairs))
+    return st, pairs
+
+
+def xover_decisions(A_parent, seed, gen, child, t1, t2, t3, enc_hidden=0):
+    rec = np.zeros(int(lib().oro_xover_record_words()), np.uint32)
+    lib().oro_xover_decisions(enc_hidden, A_parent, seed, gen, child, t1, t2, t3, ptr(rec))
+    return rec
+
+
+def breed_child(p1, p2, A_parent, A_child, seed, gen, child, t1, t2, t3, p, sigma, p_scale, enc_hidden=0):
+    n_child = policy_param_count(enc_hidden) + 38 * A_child
+    a = np.ascontiguousarray(p1, np.float32)
+    b = np.ascontiguousarray(p2, np.float32)
+    out = np.zeros(n_child, np.float32)
+    rec = np.zeros(int(lib().oro_xover_record_words()), np.uint32)
+    lib().oro_breed_child(enc_hidden, ptr(a), ptr(b), ptr(out), A_parent, A_child, seed, gen, child, t1, t2, t3, p,
+                          sigma, p_scale, ptr(rec))
+    return out, rec
+
+
+def bootstrap_genome(A, seed, gen, organism, layer_std, tail_std, enc_hidden=0):
+    n = policy_param_count(enc_hidden) + 38 * A
+    out = np.zeros(n, np.float32)
+    ls = np.ascontiguousarray(layer_std, np.float32)
+    lib().oro_bootstrap_genome(enc_hidden, ptr(out), A, seed, gen, organism, ptr(ls), tail_std)
+    return out
+
+
+def layer_stds(variance_scale=2.0, gain=1.0, fan_ins=(145, 321, 256, 128)):
+    return np.array([np.float32(gain) * np.sqrt(np.float32(variance_scale) / np.float32(f)) for f in fan_ins],
+                    np.float32)
+
+
+def encode_turn(genome, letters, tiles, enc_hidden=0):
+    g = np.ascontiguousarray(genome, np.float32)
+    l = np.ascontiguousarray(letters, np.uint8)
+    t = np.ascontiguousarray(tiles, np.uint8)
+    o32 = np.zeros(64, np.float32)
+    o64 = np.zeros(64, np.float64)
+    lib().oro_encode_turn(ptr(g), enc_hidden, ptr(l), ptr(t), ptr(o32), ptr(o64))
+    return o32, o64
+
+
+def fixed_features(word_letters):
+    w = np.ascontiguousarray(word_letters, np.uint8)
+    out = np.zeros(26, np.float32)
+    lib().oro_fixed_features(ptr(w), ptr(out))
+    return out
+
+
+def score_action(policy, fixed, tail):
+    return lib().oro_score_action(ptr(np.ascontiguousarray(policy, np.float32)),
+                                  ptr(np.ascontiguousarray(fixed, np.float32)),
+                                  ptr(np.ascontiguousarray(tail, np.float32)))

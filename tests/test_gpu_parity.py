@@ -1,0 +1,234 @@
+"""Parity of the HIP path against the CPU oracle on the same inputs and the same counter-based draws.
+
+Bar: bit-exact for every integer stage (feedback, episode bookkeeping, selection, crossover decisions,
+slot bookkeeping) and for the draw-driven fp32 stages (bootstrap, mutation); fp32 forward within 1e-5
+(normwise, against the host fp64 forward on the states the GPU visited) with argmax agreement and equal
+fitness wherever the top-2 score margin exceeds the tolerance.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FWD_TOL = 1e-5        # normwise relative error of the fp32 policy/scores vs the host fp64 forward
+MARGIN_TOL = 1e-4     # decisions whose fp64 top-2 margin is below this are near-ties
+
+
+def _runtime(pkg, side, A, catalog, offsets, ids, seed=9, var=1.0, tail=0.1, slack=3):
+    rtm = pkg.runtime
+    stride = rtm.slot_stride_bytes(A)
+    n = side * side
+    rt = pkg.SlabRuntime(slab_bytes=slack * n * stride, generation_bytes=n * stride, action_count=A,
+                         grid_columns=side)
+    rt.upload_catalog(catalog)
+    rt.upload_secrets_csr(offsets, ids)
+    rt.bootstrap(n, seed=seed, variance_scale=var, tail_std=tail)
+    return rt
+
+
+@pytest.fixture(scope="module")
+def c1(pkg):
+    """The 16x16 / A=512 workload (256 genomes, 16 secrets per cell)."""
+    wl = pkg.synth.workload("c1_16x16_a512")
+    rt = _runtime(pkg, 16, 512, wl["catalog"], wl["offsets"], wl["ids"])
+    yield wl, rt
+    rt.close()
+
+
+def test_bootstrap_bit_exact(pkg, orc, c1):
+    wl, rt = c1
+    slots, _, _, _ = rt.generation(0)
+    assert list(slots) == list(range(256)), "generation 0 takes slots 0..N-1 in order"
+    stds = orc.layer_stds(1.0)
+    n_vals = orc.policy_param_count() + 38 * 512
+    for organism in (0, 1, 77, 255):
+        gpu = rt.download_slot(slots[organism])
+        ref = orc.bootstrap_genome(512, 9, 0, organism, stds, 0.1)
+        assert np.array_equal(gpu[:n_vals].view(np.uint32), ref.view(np.uint32)), organism
+        assert not gpu[n_vals:].any()
+
+
+def test_evaluation_parity_c1(pkg, orc, c1):
+    wl, rt = c1
+    rt.set_trace(pkg.runtime.TRACE_GAMES | pkg.runtime.TRACE_POLICY)
+    counts = pkg.runtime.word_counts(512, 512)
+    rt.evaluate(counts)
+    slots, fit, evals, has = rt.generation(0)
+    assert has.all() and (evals == 1).all()
+    games = 3 * int(wl["offsets"][-1])
+    gtr = rt.download_trace(pkg.runtime.TRACE_GAMES, games)
+    ptr_ = rt.download_trace(pkg.runtime.TRACE_POLICY, games)
+    genomes = [rt.download_slot(s) for s in slots]
+    ref = orc.evaluate(genomes, 0, 512, 512, wl["catalog"], wl["offsets"], wl["ids"], mode=0, threads=8,
+                       trace=True)
+    assert ref["status"] == 0
+    rtr = ref["traces"]
+    assert (gtr["decisions"] > 0).all()
+    assert int(rt.stage_times().decisions) == int(gtr["decisions"].sum())
+
+    offsets = wl["offsets"]
+    max_err = 0.0
+    checked = agree = near = 0
+    genome_clean = np.ones(256, bool)
+    for n in range(256):
+        W = int(offsets[n + 1] - offsets[n])
+        local = wl["ids"][offsets[n]:offsets[n + 1]]
+        base = 3 * int(offsets[n])
+        for slot in range(3 * W):
+            g = gtr[base + slot]
+            d = int(g["decisions"])
+            rep = orc.replay_game(genomes[n], 512, wl["catalog"], local, slot, g["guesses"][:d])
+            assert rep["status"] == 0
+            for k in range(d):
+                p64 = rep["policy"][k]
+                p32 = ptr_[base + slot, k].astype(np.float64)
+                err = np.max(np.abs(p32 - p64)) / max(np.max(np.abs(p64)), 1e-30)
+                max_err = max(max_err, err)
+                margin = rep["best"][k] - rep["second"][k]
+                checked += 1
+                if margin > MARGIN_TOL * max(1.0, abs(rep["best"][k])):
+                    agree += int(rep["argmax"][k] == g["guesses"][k])
+                    assert rep["argmax"][k] == g["guesses"][k], (n, slot, k, margin)
+                else:
+                    near += 1
+                    genome_clean[n] = False
+                # the chosen action's fp32 score against fp64
+                assert abs(g["best"][k] - rep["chosen"][k]) <= FWD_TOL * max(1.0, np.max(np.abs(p64))) * 64
+    assert max_err <= FWD_TOL, max_err
+    # fitness identical wherever no decision of the genome was a near-tie
+    assert np.array_equal(fit[genome_clean], ref["fitness"][genome_clean])
+    # and the GPU's games equal the oracle's fp32 games wherever the oracle's own margins are clear
+    same = np.mean(gtr["guesses"] == rtr["guesses"])
+    assert same >= 0.99, same
+    print(f"decisions {checked}, near-ties {near}, max normwise fwd err {max_err:.2e}, guess agreement {same:.5f}")
+
+
+def test_summary_matches_oracle(pkg, orc, c1):
+    wl, rt = c1
+    rt.evaluate(pkg.runtime.word_counts(512, 512))
+    _, fit, _, _ = rt.generation(0)
+    s = rt.summary()
+    b, m, i = orc.summary(fit)
+    assert s.best_fitness == b and s.average_fitness == m and s.best_index == i
+
+
+def test_plan_bit_exact(pkg, orc, c1):
+    wl, rt = c1
+    counts = pkg.runtime.word_counts(512, 512)
+    rt.evaluate(counts)
+    _, fit, _, has = rt.generation(0)
+    # plant a varied fitness field through the trace-free path: advance plans from the GPU fitness
+    cfg = pkg.runtime.default_assembly_config(mutation_probability=0.0)
+    seed = 7 + 2
+    pre_slots, _, _, _ = rt.generation(0)
+    rt.advance(seed, counts, cfg)
+    gpu_pairs = rt.plan()
+    w = orc.rank_weights(0.5, 256)
+    st, ref_pairs = orc.plan(fit, has, 256, 256, 16, 0, 2, w, seed)
+    assert st == 0
+    assert np.array_equal(gpu_pairs, ref_pairs)
+    assert (gpu_pairs[:, 0] != gpu_pairs[:, 1]).all()
+
+
+def test_plan_varied_fitness_and_radii(pkg, orc):
+    """Selection on a planted fitness field with ties, radii 1, 2 and 3 (scan path)."""
+    cat = pkg.synth.make_catalog(64)
+    sec = pkg.synth.sample_secrets(64, 32)
+    off, ids = pkg.synth.block_csr(8, 8, 4, sec)
+    for radius in (1, 2, 3):
+        rt = _runtime(pkg, 8, 64, cat, off, ids, seed=31, var=2.0, tail=0.5)
+        counts = pkg.runtime.word_counts(64, 64)
+        cfg = pkg.runtime.default_assembly_config(cellular_breeding_radius=radius, rank_exponent=0.75)
+        rt.advance(1234 + radius, counts, cfg)
+        gpu_pairs = rt.plan()
+        rt.close()
+        rt2 = _runtime(pkg, 8, 64, cat, off, ids, seed=31, var=2.0, tail=0.5)
+        rt2.evaluate(counts)
+        _, fit, _, has = rt2.generation(0)
+        rt2.close()
+        w = orc.rank_weights(0.75, 64)
+        st, ref_pairs = orc.plan(fit, has, 64, 64, 8, 0, radius, w, 1234 + radius)
+        assert st == 0 and np.array_equal(gpu_pairs, ref_pairs), radius
+
+
+def test_assembly_bit_exact(pkg, orc):
+    cat = pkg.synth.make_catalog(96)
+    sec = pkg.synth.sample_secrets(96, 32)
+    off, ids = pkg.synth.block_csr(8, 8, 4, sec)
+    A = 96
+    rt = _runtime(pkg, 8, A, cat, off, ids, seed=5, var=1.0, tail=0.1)
+    counts = pkg.runtime.word_counts(96, A)
+    slots, _, _, _ = rt.generation(0)
+    parents = [rt.download_slot(s) for s in slots]
+    temps = (0.3, 0.2, 0.4)
+    cfg = pkg.runtime.default_assembly_config(crossover_temperature_level1=temps[0],
+                                              crossover_temperature_level2=temps[1],
+                                              crossover_temperature_level3=temps[2], mutation_probability=0.05,
+                                              mutation_sigma=0.02, output_tail_row_scale_mutation_probability=0.1)
+    seed = 4242
+    rt.advance(seed, counts, cfg)
+    pairs = rt.plan()
+    xrec = rt.download_trace(pkg.runtime.TRACE_XOVER, 64)
+    child_slots, _, _, _ = rt.generation(0)
+    assert rt.info().current_generation_index == 1
+    n_vals = orc.policy_param_count() + 38 * A
+    for c in range(64):
+        a, b = pairs[c]
+        ref, rec = orc.breed_child(parents[a][:n_vals], parents[b][:n_vals], A, A, seed, 0, c, *temps, 0.05, 0.02, 0.1)
+        got = rt.download_slot(child_slots[c])
+        assert np.array_equal(got[:n_vals].view(np.uint32), ref.view(np.uint32)), c
+        assert not got[n_vals:].any()
+        assert np.array_equal(xrec[c], rec), c
+    rt.close()
+
+
+def test_copy_first_parent_when_temperatures_and_mutation_are_zero(pkg, orc):
+    """Reference KAT (slab_runtime_test.cu:414-481): temperatures 0 and mutation 0 -> child == parent 1."""
+    cat = pkg.synth.make_catalog(48)
+    sec = pkg.synth.sample_secrets(48, 16)
+    off, ids = pkg.synth.block_csr(4, 4, 2, sec)
+    rt = _runtime(pkg, 4, 48, cat, off, ids)
+    slots, _, _, _ = rt.generation(0)
+    parents = [rt.download_slot(s) for s in slots]
+    cfg = pkg.runtime.default_assembly_config(crossover_temperature_level1=0.0, crossover_temperature_level2=0.0,
+                                              crossover_temperature_level3=0.0, mutation_probability=0.0)
+    rt.advance(77, pkg.runtime.word_counts(48, 48), cfg)
+    pairs = rt.plan()
+    children, _, _, _ = rt.generation(0)
+    for c in range(16):
+        assert np.array_equal(rt.download_slot(children[c]), parents[pairs[c, 0]])
+    rt.close()
+
+
+def test_masking_end_to_end(pkg):
+    """Reference KAT (single_model_device_runtime_test.cu:133-170): a zero genome scores every action 0,
+    so it guesses the lowest unmasked index: CRANE, then SLATE (CRANE masked), then TRACE and wins."""
+    words = np.stack([np.frombuffer(w.encode(), np.uint8) - 65 for w in ("CRANE", "SLATE", "TRACE")])
+    rtm = pkg.runtime
+    stride = rtm.slot_stride_bytes(3)
+    rt = pkg.SlabRuntime(slab_bytes=4 * stride, generation_bytes=stride, action_count=3, grid_columns=1)
+    rt.upload_catalog(words)
+    rt.upload_secrets_csr(np.array([0, 1], np.uint32), np.array([2], np.uint32))
+    rt.bootstrap(1, seed=1)
+    slots, _, _, _ = rt.generation(0)
+    rt.upload_slot(slots[0], np.zeros(stride // 4, np.float32))
+    rt.set_trace(rtm.TRACE_GAMES)
+    # W = 1: the variant-1 prefill equals the solution, so the reference fails with kGuessAppendFailed
+    with pytest.raises(RuntimeError) as err:
+        rt.evaluate(rtm.word_counts(3, 3))
+    assert err.value.status == 6
+    rt.close()
+    rt = pkg.SlabRuntime(slab_bytes=4 * stride, generation_bytes=stride, action_count=3, grid_columns=1)
+    rt.upload_catalog(words)
+    rt.upload_secrets_csr(np.array([0, 2], np.uint32), np.array([2, 0], np.uint32))  # W = 2
+    rt.bootstrap(1, seed=1)
+    slots, _, _, _ = rt.generation(0)
+    rt.upload_slot(slots[0], np.zeros(stride // 4, np.float32))
+    rt.set_trace(rtm.TRACE_GAMES)
+    rt.evaluate(rtm.word_counts(3, 3))
+    tr = rt.download_trace(rtm.TRACE_GAMES, 6)
+    # fresh game for secret TRACE: CRANE, SLATE, TRACE -> won at turn 3, score 13
+    assert list(tr[0]["guesses"][:3]) == [0, 1, 2] and tr[0]["score"] == 13.0
+    # W = 2, variant 1 of entry 0 prefills local[1], local[0] = CRANE, TRACE: won by the prefill, score 14
+    assert tr[2]["decisions"] == 0 and tr[2]["score"] == 14.0
+    rt.close()
