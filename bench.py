@@ -32,7 +32,7 @@ def cpu_baseline(pkg, rt, wl, A, seconds=12.0):
     import oracle as orc
 
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(threads, 64))
+    threads = max(1, min(threads, 16))  # the box's CPU share for one GPU
     slots, _, _, _ = rt.generation(0)
     offsets, ids = wl["offsets"], wl["ids"]
     # probe with a few genomes to size the sample to about `seconds` of CPU work

@@ -24,6 +24,9 @@ HIPCC_FLAGS = [
 SOURCES = ["ne_kernels.hip", "ne_runtime.hip"]
 
 
+DRIVER = os.path.join(HERE, "run_ga")
+
+
 def build(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = LIB + ".tmp"
@@ -35,7 +38,22 @@ def build(verbose=False):
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libne_wordle.so")
     os.replace(tmp, LIB)
+    build_driver(verbose)
     return LIB
+
+
+def build_driver(verbose=False):
+    """The C++ GA driver over the host API (host/slab_runtime.hpp), linked to libne_wordle.so."""
+    cmd = ["g++", "-O2", "-std=c++17", os.path.join(HERE, "host", "run_ga.cpp"), "-o", DRIVER + ".tmp",
+           "-L" + HERE, "-l:libne_wordle.so", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed building run_ga")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

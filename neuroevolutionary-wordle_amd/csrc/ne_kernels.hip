@@ -64,6 +64,132 @@ __device__ inline uint32_t FeedbackRegs(uint32_t guess, uint32_t sol) {
     return fb;
 }
 
+// ---- trunk pieces shared by game evaluation and output-embedding injection -----------------
+// Lane (j, gq) of a wave: j = lane%16 is the A-operand row / B-operand column, gq = lane/16 the
+// k group.  Layer-0 accumulators acc[t] hold neuron rows 16t+4gq..16t+4gq+3 of game column j.
+struct TrunkLane {
+    Rsrc gr;
+    uint32_t off_b0, off_virgin, off_fold, off_w1, off_b1, off_w2, off_b2;
+};
+
+__device__ __forceinline__ TrunkLane MakeTrunkLane(const float* genome, uint32_t j, uint32_t gq) {
+    TrunkLane t;
+    t.gr = MakeRsrc(genome, uint32_t(kPolicyParams * 4));
+    t.off_b0 = uint32_t(kOffB0 + 4 * gq) * 4;
+    t.off_virgin = uint32_t(kOffW0 + 4 * gq * kModelIn) * 4;
+    t.off_fold = uint32_t(kOffW0 + j * kModelIn + 1 + 16 * gq) * 4;
+    t.off_w1 = uint32_t(kOffW1 + j * kH0 + 4 * gq) * 4;
+    t.off_b1 = uint32_t(kOffB1 + 4 * gq) * 4;
+    t.off_w2 = uint32_t(kOffW2 + (16 * (j >> 2) + (j & 3)) * kH1 + 4 * gq) * 4;
+    t.off_b2 = uint32_t(kOffB2 + 16 * gq) * 4;
+    return t;
+}
+
+__device__ __forceinline__ void InitLayer0(v4f (&acc)[16], const TrunkLane& tl) {
+#pragma unroll
+    for (int t = 0; t < 16; ++t) acc[t] = Buf4(tl.gr, tl.off_b0, 64 * t);
+}
+
+// Virgin flag (model input x[0] = 1 on an empty grid): adds column 0 of W0 where `lane_virgin`.
+__device__ __forceinline__ void AddVirgin(v4f (&acc)[16], const TrunkLane& tl, bool lane_virgin) {
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float w = BufF(tl.gr, tl.off_virgin, uint32_t(16 * t + r) * kModelIn * 4);
+            if (lane_virgin) acc[t][r] += w;
+        }
+}
+
+// Encoder 145->64 on one turn from the LDS copy (transposed [feature][out]): dims 16gq..16gq+15.
+// Features in increasing index order, 5 letters then 5 tiles (turn_features.hpp:60-81), added to
+// the bias one by one -- the reference's sequential sum over a one-hot input.
+__device__ __forceinline__ void EncodeTurnLds(const float* s_enc, const float* s_encb, uint32_t code, uint32_t fb,
+                                              uint32_t gq, float (&e)[16]) {
+#pragma unroll
+    for (int s = 0; s < 16; ++s) e[s] = s_encb[16 * gq + s];
+#pragma unroll
+    for (int f = 0; f < 10; ++f) {
+        const uint32_t k = f < 5 ? uint32_t(f) * 26u + WordLetter(code, f)
+                                 : 130u + uint32_t(f - 5) * 3u + ((fb >> (2 * (f - 5))) & 3u);
+        const float* col = s_enc + k * kEncOut + 16 * gq;
+#pragma unroll
+        for (int s = 0; s < 16; s += 4) {
+            const v4f c = *reinterpret_cast<const v4f*>(col + s);
+            e[s] += c[0];
+            e[s + 1] += c[1];
+            e[s + 2] += c[2];
+            e[s + 3] += c[3];
+        }
+    }
+}
+
+// acc += W0[:, 1+64*slot .. 1+64*slot+63] . e
+__device__ __forceinline__ void FoldSlot(v4f (&acc)[16], const TrunkLane& tl, int slot, const float (&e)[16]) {
+    const uint32_t soff_slot = uint32_t(64 * slot) * 4;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        float a[16];
+#pragma unroll
+        for (int s = 0; s < 16; ++s) a[s] = BufF(tl.gr, tl.off_fold, soff_slot + uint32_t(16 * t * kModelIn + s) * 4);
+#pragma unroll
+        for (int s = 0; s < 16; ++s) acc[t] = Mfma(a[s], e[s], acc[t]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// Layers 1 and 2: pol[s] = policy dim 16gq+s of game column j.  Layer 2's output rows are permuted
+// so each lane group holds 16 consecutive policy dims.
+__device__ __forceinline__ void TrunkToPolicy(const v4f (&acc)[16], const TrunkLane& tl, float (&pol)[16]) {
+    v4f h1[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) h1[u] = Buf4(tl.gr, tl.off_b1, 64 * u);
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        float hv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) hv[r] = fmaxf(acc[t][r], 0.0f);
+        v4f w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) w[u] = Buf4(tl.gr, tl.off_w1, uint32_t(16 * u * kH0 + 16 * t) * 4);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) h1[u] = Mfma(w[u][r], hv[r], h1[u]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    v4f pv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pv[u] = Buf4(tl.gr, tl.off_b2, 16 * u);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        float hv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) hv[r] = fmaxf(h1[t][r], 0.0f);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const v4f w = Buf4(tl.gr, tl.off_w2, uint32_t(4 * u * kH1 + 16 * t) * 4);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) pv[u] = Mfma(w[r], hv[r], pv[u]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) pol[4 * u + r] = pv[u][r];
+}
+
+// Stages a genome's 145->64 encoder into LDS, transposed to [feature][out].
+__device__ __forceinline__ void StageEncoder(const float* genome, float* s_enc, float* s_encb) {
+    for (uint32_t idx = threadIdx.x; idx < uint32_t(kEncOut * kTurnFeatures); idx += blockDim.x) {
+        const uint32_t o = idx / kTurnFeatures, k = idx - o * kTurnFeatures;
+        s_enc[k * kEncOut + o] = genome[kOffEncW + idx];
+    }
+    if (threadIdx.x < kEncOut) s_encb[threadIdx.x] = genome[kOffEncB + threadIdx.x];
+    __syncthreads();
+}
+
 // ================================================================================================
 // Evaluation kernel.
 template <bool kTrace>
@@ -79,12 +205,7 @@ __global__ __launch_bounds__(512) void EvaluateKernel(EvalArgs p) {
     }
     const float* genome = reinterpret_cast<const float*>(p.slab + uint64_t(slot) * p.stride);
 
-    for (uint32_t idx = threadIdx.x; idx < uint32_t(kEncOut * kTurnFeatures); idx += blockDim.x) {
-        const uint32_t o = idx / kTurnFeatures, k = idx - o * kTurnFeatures;
-        s_enc[k * kEncOut + o] = genome[kOffEncW + idx];
-    }
-    if (threadIdx.x < kEncOut) s_encb[threadIdx.x] = genome[kOffEncB + threadIdx.x];
-    __syncthreads();
+    StageEncoder(genome, s_enc, s_encb);
 
     const uint32_t W = p.cell_offsets[n + 1] - p.cell_offsets[n];
     if (W == 0) {
@@ -102,15 +223,7 @@ __global__ __launch_bounds__(512) void EvaluateKernel(EvalArgs p) {
     const uint32_t waves = blockDim.x >> 6;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
-    const Rsrc gr = MakeRsrc(genome, uint32_t(kPolicyParams * 4));
-    // per-lane byte offsets of each operand stream
-    const uint32_t off_b0 = uint32_t(kOffB0 + 4 * gq) * 4;
-    const uint32_t off_virgin = uint32_t(kOffW0 + 4 * gq * kModelIn) * 4;
-    const uint32_t off_fold = uint32_t(kOffW0 + j * kModelIn + 1 + 16 * gq) * 4;
-    const uint32_t off_w1 = uint32_t(kOffW1 + j * kH0 + 4 * gq) * 4;
-    const uint32_t off_b1 = uint32_t(kOffB1 + 4 * gq) * 4;
-    const uint32_t off_w2 = uint32_t(kOffW2 + (16 * (j >> 2) + (j & 3)) * kH1 + 4 * gq) * 4;
-    const uint32_t off_b2 = uint32_t(kOffB2 + 16 * gq) * 4;
+    const TrunkLane tl = MakeTrunkLane(genome, j, gq);
     const float* tails = genome + kPolicyParams;
     const uint32_t A = p.action_count;
     const uint32_t a_tiles = (A + 15) / 16;
@@ -178,92 +291,20 @@ __global__ __launch_bounds__(512) void EvaluateKernel(EvalArgs p) {
             if (!__any(active)) break;
             const int ttc = base_tc + step;  // turn count of every active game in this tile
             if (step == 0 || (fresh && step == 1)) {
-#pragma unroll
-                for (int t = 0; t < 16; ++t) acc[t] = Buf4(gr, off_b0, 64 * t);
-                if (fresh && step == 0) {  // virgin flag: model input x[0] = 1 on an empty grid
-#pragma unroll
-                    for (int t = 0; t < 16; ++t)
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) acc[t][r] += BufF(gr, off_virgin, uint32_t(16 * t + r) * kModelIn * 4);
-                }
+                InitLayer0(acc, tl);
+                if (fresh && step == 0) AddVirgin(acc, tl, true);
                 folded = 0;
             }
             // ---- fold newly occupied turn slots: acc += W0[:, 1+64*slot ...] . enc(turn slot)
             for (int ts = folded; ts < ttc; ++ts) {
                 const bool is_last = ts == ttc - 1;
-                const uint32_t code = is_last ? last_code : prev_code;
-                const uint32_t fb = is_last ? last_fb : prev_fb;
                 float e[16];
-#pragma unroll
-                for (int s = 0; s < 16; ++s) e[s] = s_encb[16 * gq + s];
-                // features in increasing index order: 5 letters then 5 tiles (turn_features.hpp:60-81)
-#pragma unroll
-                for (int f = 0; f < 10; ++f) {
-                    const uint32_t k = f < 5 ? uint32_t(f) * 26u + WordLetter(code, f)
-                                             : 130u + uint32_t(f - 5) * 3u + ((fb >> (2 * (f - 5))) & 3u);
-                    const float* col = s_enc + k * kEncOut + 16 * gq;
-#pragma unroll
-                    for (int s = 0; s < 16; s += 4) {
-                        const v4f c = *reinterpret_cast<const v4f*>(col + s);
-                        e[s] += c[0];
-                        e[s + 1] += c[1];
-                        e[s + 2] += c[2];
-                        e[s + 3] += c[3];
-                    }
-                }
-                const uint32_t soff_slot = uint32_t(64 * ts) * 4;
-#pragma unroll
-                for (int t = 0; t < 16; ++t) {
-                    float a[16];
-#pragma unroll
-                    for (int s = 0; s < 16; ++s) a[s] = BufF(gr, off_fold, soff_slot + uint32_t(16 * t * kModelIn + s) * 4);
-#pragma unroll
-                    for (int s = 0; s < 16; ++s) acc[t] = Mfma(a[s], e[s], acc[t]);
-                    __builtin_amdgcn_sched_barrier(0);
-                }
+                EncodeTurnLds(s_enc, s_encb, is_last ? last_code : prev_code, is_last ? last_fb : prev_fb, gq, e);
+                FoldSlot(acc, tl, ts, e);
             }
             folded = ttc;
-
-            // ---- layer 1: h1 = W1 . relu(h0) + b1
-            v4f h1[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) h1[u] = Buf4(gr, off_b1, 64 * u);
-#pragma unroll
-            for (int t = 0; t < 16; ++t) {
-                float hv[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) hv[r] = fmaxf(acc[t][r], 0.0f);
-                v4f w[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) w[u] = Buf4(gr, off_w1, uint32_t(16 * u * kH0 + 16 * t) * 4);
-#pragma unroll
-                for (int u = 0; u < 8; ++u)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) h1[u] = Mfma(w[u][r], hv[r], h1[u]);
-                __builtin_amdgcn_sched_barrier(0);
-            }
-            // ---- layer 2 with output rows permuted so lane group gq holds policy dims 16*gq..16*gq+15
-            v4f pv[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) pv[u] = Buf4(gr, off_b2, 16 * u);
-#pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                float hv[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) hv[r] = fmaxf(h1[t][r], 0.0f);
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const v4f w = Buf4(gr, off_w2, uint32_t(4 * u * kH1 + 16 * t) * 4);
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) pv[u] = Mfma(w[r], hv[r], pv[u]);
-                }
-                __builtin_amdgcn_sched_barrier(0);
-            }
             float pol[16];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) pol[4 * u + r] = pv[u][r];
+            TrunkToPolicy(acc, tl, pol);
 
             // ---- scoring: S = E . P with E rows = [26 fixed | 38 tail] per action; masked argmax
             float best = -__builtin_inff();
@@ -842,6 +883,119 @@ __global__ void FillU32Kernel(uint32_t* dst, uint32_t value, uint64_t count) {
 }
 
 // ================================================================================================
+// Output-embedding injection on growth generations (TryInjectExpandedOutputEmbeddingTailsConcurrently,
+// injection_ops.cuh:21-64; output_embedding_injection.hpp:25-205).
+
+// TrainableTailNorm: sequential sum of squares over the 38 values, then sqrt.
+__device__ inline float TailNorm(const float* row) {
+    float s = 0.0f;
+    for (int f = 0; f < kTail; ++f) s += row[f] * row[f];
+    return sqrtf(s);
+}
+
+// Median of a child's existing tail-row norms (TryComputeMedianTailNorm): bitonic sort in LDS.
+__global__ __launch_bounds__(256) void MedianNormKernel(InjectArgs p) {
+    __shared__ float s[kMaxSortedNorms];
+    const uint32_t c = blockIdx.x;
+    const uint32_t slot = p.child_slots[c];
+    const float* tails = reinterpret_cast<const float*>(p.slab + uint64_t(slot) * p.stride) + kPolicyParams;
+    const uint32_t A = p.parent_action_count;
+    uint32_t n2 = 1;
+    while (n2 < A) n2 <<= 1;
+    for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) s[i] = i < A ? TailNorm(tails + uint64_t(i) * kTail) : __builtin_inff();
+    __syncthreads();
+    for (uint32_t k = 2; k <= n2; k <<= 1) {
+        for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) {
+                const uint32_t ixj = i ^ jj;
+                if (ixj > i) {
+                    const bool ascending = (i & k) == 0;
+                    const float a = s[i], b = s[ixj];
+                    if ((a > b) == ascending) {
+                        s[i] = b;
+                        s[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t up = A / 2;
+        p.medians[c] = (A % 2) ? s[up] : 0.5f * (s[up - 1] + s[up]);
+    }
+}
+
+// Seeds the new tail rows of one child from its policy on the 3 hint grids of each new word, then
+// rescales each row to the child's median tail norm.  One workgroup per child; each wave runs tiles
+// of 5 words x 3 grids through the trunk (lane column j = 3*word + grid, column 15 idle).
+__global__ __launch_bounds__(512) void InjectKernel(InjectArgs p) {
+    __shared__ float s_enc[kTurnFeatures * kEncOut];
+    __shared__ float s_encb[kEncOut];
+    const uint32_t c = blockIdx.x;
+    const uint32_t slot = p.child_slots[c];
+    float* genome = reinterpret_cast<float*>(p.slab + uint64_t(slot) * p.stride);
+    StageEncoder(genome, s_enc, s_encb);
+    const uint32_t lane = threadIdx.x & 63u, j = lane & 15u, gq = lane >> 4;
+    const uint32_t waves = blockDim.x >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const TrunkLane tl = MakeTrunkLane(genome, j, gq);
+    const uint32_t tiles = (p.count + 4) / 5;
+    for (uint32_t tile = wave; tile < tiles; tile += waves) {
+        const uint32_t word = tile * 5 + j / 3, grid = j % 3;
+        const bool valid = j < 15 && word < p.count;
+        HintState st = {0, {0, 0, 0, 0}, {0, 0, 0, 0}};
+        if (valid) st = p.states[word * 3 + grid];
+        const int tc = valid ? int(st.tc) : 0;
+        int max_tc = tc;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) max_tc = max(max_tc, __shfl_xor(max_tc, off));
+        max_tc = __builtin_amdgcn_readfirstlane(max_tc);
+        v4f acc[16];
+        InitLayer0(acc, tl);
+        AddVirgin(acc, tl, valid && tc == 0);
+        for (int ts = 0; ts < max_tc; ++ts) {
+            const uint32_t code = ts == 0 ? st.code[0] : (ts == 1 ? st.code[1] : (ts == 2 ? st.code[2] : st.code[3]));
+            const uint32_t fb = ts == 0 ? st.fb[0] : (ts == 1 ? st.fb[1] : (ts == 2 ? st.fb[2] : st.fb[3]));
+            float e[16];
+            EncodeTurnLds(s_enc, s_encb, code, fb, gq, e);
+            if (ts >= tc) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) e[q] = 0.0f;
+            }
+            FoldSlot(acc, tl, ts, e);
+        }
+        float pol[16];
+        TrunkToPolicy(acc, tl, pol);
+        // mean over the word's 3 grids in grid order: ((0 + p0) + p1) + p2, times 1/3
+        float* row = genome + kPolicyParams + uint64_t(p.parent_action_count + word) * kTail;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const float p1 = __shfl_down(pol[q], 1);
+            const float p2 = __shfl_down(pol[q], 2);
+            const float mean = (((0.0f + pol[q]) + p1) + p2) * (1.0f / 3.0f);
+            const int dim = int(16 * gq) + q;
+            if (valid && grid == 0 && dim >= kFixed) row[dim - kFixed] = mean;
+        }
+    }
+    __syncthreads();
+    const float target = p.medians[c];
+    for (uint32_t k = threadIdx.x; k < p.count; k += blockDim.x) {
+        float* row = genome + kPolicyParams + uint64_t(p.parent_action_count + k) * kTail;
+        float scale = 0.0f;
+        if (target > 1.0e-6f) {
+            const float cur = TailNorm(row);
+            if (!(cur > 1.0e-6f)) {
+                SetStatus(p.status, 15);  // kOutputEmbeddingInjectionFailed
+                continue;
+            }
+            scale = target / cur;
+        }
+        for (int f = 0; f < kTail; ++f) row[f] = row[f] * scale;
+    }
+}
+
+// ================================================================================================
 // Launch wrappers (host side).
 void LaunchEvaluate(const EvalArgs& a, uint32_t n_genomes, uint32_t waves, uint32_t chunks, bool trace,
                     hipStream_t s) {
@@ -935,4 +1089,13 @@ void LaunchFillU32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t s)
     hipLaunchKernelGGL(FillU32Kernel, dim3(uint32_t(blocks)), dim3(256), 0, s, dst, value, count);
 }
 
+}  // namespace ne
+
+namespace ne {
+void LaunchInject(const InjectArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(MedianNormKernel, dim3(a.children), dim3(256), 0, s, a);
+    const uint32_t tiles = (a.count + 4) / 5;
+    const uint32_t waves = tiles < 8 ? (tiles == 0 ? 1 : tiles) : 8;
+    hipLaunchKernelGGL(InjectKernel, dim3(a.children), dim3(64 * waves), 0, s, a);
+}
 }  // namespace ne

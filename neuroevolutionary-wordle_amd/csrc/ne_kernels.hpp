@@ -109,6 +109,28 @@ struct RepackArgs {
     int* status;
 };
 
+constexpr uint32_t kMaxSortedNorms = 8192;  // largest parent action count the median sort handles
+
+// One hint-grid state: turn count and, per turn, the packed guess and packed feedback.
+struct HintState {
+    uint32_t tc;
+    uint32_t code[4];
+    uint32_t fb[4];
+};
+
+struct InjectArgs {
+    uint8_t* slab;
+    uint64_t stride;
+    const uint32_t* child_slots;
+    uint32_t children;
+    uint32_t parent_action_count;
+    uint32_t count;              // new tail rows per child
+    const HintState* states;     // [count][3]
+    float* medians;              // [children]
+    int* status;
+};
+
+void LaunchInject(const InjectArgs& a, hipStream_t s);
 void LaunchEvaluate(const EvalArgs& a, uint32_t n_genomes, uint32_t waves, uint32_t chunks, bool trace,
                     hipStream_t s);
 void LaunchFinalize(const float* raw, const uint32_t* local_w, uint32_t n, float* fitness, uint32_t* evals,

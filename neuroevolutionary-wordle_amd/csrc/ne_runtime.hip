@@ -578,6 +578,95 @@ static int Assemble(ne_runtime* rt, uint32_t seed, const ne_assembly_config& cfg
     return NE_OK;
 }
 
+// TryBuildHintGridGroup (src/wordle/hint_grid.hpp:76-164) on packed words: grid 0 holds every
+// distinct left rotation of the word whose feedback is all yellow; grids 1 and 2 each hold one swap
+// of two differing positions whose feedback is 3 green + 2 yellow, the first two distinct ones in
+// (i < j) order.  Fails when fewer than two such swaps exist.
+static bool BuildHintStates(const uint8_t* w, HintState out[3]) {
+    auto pack = [](const uint8_t* l) {
+        uint32_t c = 0;
+        for (int q = 0; q < 5; ++q) c |= uint32_t(l[q]) << (5 * q);
+        return c;
+    };
+    const uint32_t sol = pack(w);
+    for (int g = 0; g < 3; ++g) out[g] = HintState{0, {0, 0, 0, 0}, {0, 0, 0, 0}};
+    constexpr uint32_t kAllYellow = 0x155;  // tile 1 in each 2-bit field
+    for (int rot = 1; rot < 5; ++rot) {
+        uint8_t g[5];
+        for (int q = 0; q < 5; ++q) g[q] = w[(q + rot) % 5];
+        const uint32_t code = pack(g);
+        if (code == sol) continue;
+        bool dup = false;
+        for (uint32_t k = 0; k < out[0].tc; ++k) dup |= out[0].code[k] == code;
+        if (dup) continue;
+        const uint32_t fb = FeedbackPacked(code, sol);
+        if (fb != kAllYellow) continue;
+        out[0].code[out[0].tc] = code;
+        out[0].fb[out[0].tc] = fb;
+        ++out[0].tc;
+    }
+    int chosen = 0;
+    for (int a = 0; a < 5; ++a)
+        for (int b = a + 1; b < 5; ++b) {
+            if (w[a] == w[b]) continue;
+            uint8_t g[5];
+            for (int q = 0; q < 5; ++q) g[q] = w[q];
+            std::swap(g[a], g[b]);
+            const uint32_t code = pack(g);
+            const uint32_t fb = FeedbackPacked(code, sol);
+            int greens = 0, yellows = 0;
+            for (int q = 0; q < 5; ++q) {
+                const uint32_t t = (fb >> (2 * q)) & 3u;
+                greens += t == 0;
+                yellows += t == 1;
+            }
+            if (greens != 3 || yellows != 2) continue;
+            bool dup = false;
+            for (int k = 0; k < chosen; ++k) dup |= out[1 + k].code[0] == code;
+            if (dup) continue;
+            out[1 + chosen] = HintState{1, {code, 0, 0, 0}, {fb, 0, 0, 0}};
+            if (++chosen == 2) return true;
+        }
+    return false;
+}
+
+// Injection of the new output-embedding rows into every assembled child
+// (AssembleInjectedChildBatchKernel's injection half, genotype_slab/device_runtime.cu:518-617).
+static int Inject(ne_runtime* rt, uint64_t parent_a, const ne_pending_injection& inj) {
+    if (parent_a == 0 || parent_a > kMaxSortedNorms) return Fail(rt, NE_UNSUPPORTED);
+    std::vector<HintState> states(inj.injection_count * 3);
+    for (uint64_t k = 0; k < inj.injection_count; ++k)
+        if (!BuildHintStates(rt->h_catalog.data() + 5 * (inj.first_catalog_word_index + k), &states[3 * k]))
+            return Fail(rt, NE_OUTPUT_EMBEDDING_INJECTION_FAILED);
+    HintState* d_states = nullptr;
+    float* d_medians = nullptr;
+    if (!Alloc(&d_states, states.size()) || !Alloc(&d_medians, rt->next_size) ||
+        !Ok(hipMemcpy(d_states, states.data(), states.size() * sizeof(HintState), hipMemcpyHostToDevice))) {
+        Free(d_states);
+        Free(d_medians);
+        return Fail(rt, NE_DEVICE_FAILURE);
+    }
+    InjectArgs a{};
+    a.slab = rt->slab;
+    a.stride = rt->stride;
+    a.child_slots = rt->nxt_slots;
+    a.children = uint32_t(rt->next_size);
+    a.parent_action_count = uint32_t(parent_a);
+    a.count = uint32_t(inj.injection_count);
+    a.states = d_states;
+    a.medians = d_medians;
+    a.status = rt->status;
+    EventBegin(rt, 5);
+    LaunchInject(a, rt->stream);
+    EventEnd(rt, 5);
+    const int st = Finish(rt);
+    Free(d_states);
+    Free(d_medians);
+    if (st != NE_OK) return st;
+    rt->times.inject_ms = EventMs(rt, 5);
+    return NE_OK;
+}
+
 // SwapDeviceSlabGenerationBuffers (slab_runtime.cu:2128-2135).
 static void SwapGenerations(ne_runtime* rt) {
     std::swap(rt->cur_slots, rt->nxt_slots);
@@ -814,7 +903,8 @@ int ne_advance_generation(ne_runtime* rt, uint32_t generation_seed, const ne_wor
         return Fail(rt, NE_CALLBACK_FAILED);
     uint64_t next_a = 0, next_n = 0;
     if (!PlanShape(rt, inj, &next_a, &next_n)) return Fail(rt, NE_INVALID_ASSEMBLY_CONFIG);
-    if (inj.enabled) return Fail(rt, NE_UNSUPPORTED);  // output-embedding injection is not built yet
+    if (inj.enabled && inj.first_catalog_word_index + inj.injection_count > rt->catalog_count)
+        return Fail(rt, NE_OUTPUT_EMBEDDING_INJECTION_FAILED);
     const uint64_t radius = cfg.cellular_breeding_radius;
     st = EnsureRankWeights(rt, cfg.rank_exponent, std::max<uint64_t>(rt->cur_size, 25));
     if (st != NE_OK) return st;
@@ -847,6 +937,10 @@ int ne_advance_generation(ne_runtime* rt, uint32_t generation_seed, const ne_wor
     }
     st = Assemble(rt, generation_seed, cfg, parent_a);
     if (st != NE_OK) return st;
+    if (inj.enabled) {
+        st = Inject(rt, parent_a, inj);
+        if (st != NE_OK) return st;
+    }
     SwapGenerations(rt);
     rt->last_status = NE_OK;
     return NE_OK;

@@ -1,0 +1,105 @@
+// run_ga.cpp -- thin C++ driver of the GA generation step through the host API of slab_runtime.hpp:
+// bootstrap, then `generations` calls of TryAdvanceGenerationOnDevice with the reference CLI's
+// generation-seed convention (seed + 2 + g, src/cli/run_genetic_algorithm.cu:1883-1884), then a
+// final evaluation, printing one summary line per generation.
+//
+// usage: run_ga <workload_dir> <grid_columns> <generations> <seed> [mutation_probability]
+// workload_dir holds catalog.u8 (5 letter indices per word), offsets.u32 (cells+1) and secrets.u32.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "slab_runtime.hpp"
+
+using namespace ne_amd::slab_device;
+
+template <typename T>
+static bool ReadAll(const std::string& path, std::vector<T>& out) {
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
+    if (!f) return false;
+    const std::streamsize n = f.tellg();
+    f.seekg(0);
+    out.resize(static_cast<std::size_t>(n) / sizeof(T));
+    return static_cast<bool>(f.read(reinterpret_cast<char*>(out.data()), n));
+}
+
+static int Fail(const DeviceSlabGARuntimeBuffers& b, const char* what) {
+    DeviceSlabGARuntimeStatusCode st = DeviceSlabGARuntimeStatusCode::kOk;
+    (void)TryReadDeviceSlabGARuntimeStatus(b, st);
+    std::fprintf(stderr, "%s failed: %s\n", what, DeviceSlabGARuntimeStatusCodeString(st));
+    return 1;
+}
+
+int main(int argc, char** argv) {
+    if (argc < 5) {
+        std::fprintf(stderr, "usage: %s <workload_dir> <grid_columns> <generations> <seed> [mutation_p]\n", argv[0]);
+        return 2;
+    }
+    const std::string dir = argv[1];
+    const std::size_t cols = std::strtoull(argv[2], nullptr, 10);
+    const std::size_t generations = std::strtoull(argv[3], nullptr, 10);
+    const std::uint32_t seed = static_cast<std::uint32_t>(std::strtoul(argv[4], nullptr, 10));
+    const float mutation_p = argc > 5 ? std::strtof(argv[5], nullptr) : 0.01f;
+    std::vector<std::uint8_t> catalog;
+    std::vector<std::uint32_t> offsets, secrets;
+    if (!ReadAll(dir + "/catalog.u8", catalog) || !ReadAll(dir + "/offsets.u32", offsets) ||
+        !ReadAll(dir + "/secrets.u32", secrets) || offsets.size() < 2) {
+        std::fprintf(stderr, "cannot read the workload in %s\n", dir.c_str());
+        return 2;
+    }
+    const std::size_t actions = catalog.size() / 5;
+    const std::size_t cells = offsets.size() - 1;
+    const std::size_t stride = ((132928 + 38 * actions) * 4 + 255) / 256 * 256;
+
+    DeviceSlabGARuntimeConfig config{};
+    config.genotype_slab_byte_budget_bytes = 2 * cells * stride + stride;
+    config.generation_byte_budget_bytes = cells * stride;
+    config.action_count = actions;
+    config.grid_column_count = cols;
+    DeviceSlabGARuntimeBuffers buffers{};
+    if (!TryCreateDeviceSlabGARuntimeBuffers(buffers, config)) {
+        std::fprintf(stderr, "runtime creation failed\n");
+        return 1;
+    }
+    if (!TryUploadTrainingWordCatalog(buffers, catalog) || !TryUploadTrainingSecrets(buffers, offsets, secrets))
+        return Fail(buffers, "input upload");
+    const ne_init_config init{1.0f, 1.0f, 0.1f, 0.0f};
+    if (!TryBootstrapRandomCurrentGenerationOnDevice(buffers, cells, seed, 0, &init)) return Fail(buffers, "bootstrap");
+
+    RuntimeWordCounts counts{};
+    counts.training_word_count = actions;
+    counts.action_space_word_count = actions;
+    counts.training_word_schedule.initial_word_count = actions;
+    GenerationAssemblyConfig assembly{};
+    assembly.mutation.mutation_probability = mutation_p;
+    std::size_t evaluated = 0;
+    const PostFitnessEvaluationCallback count_evaluations = [&](const DeviceSlabGARuntimeBuffers&,
+                                                                const RuntimeWordCounts&) {
+        ++evaluated;
+        return true;
+    };
+    const auto start = std::chrono::steady_clock::now();
+    for (std::size_t g = 0; g < generations; ++g) {
+        const std::uint32_t generation_seed = seed + 2u + static_cast<std::uint32_t>(g);
+        if (!TryAdvanceGenerationOnDevice(buffers, generation_seed, counts, assembly, {}, false, count_evaluations))
+            return Fail(buffers, "advance");
+        PopulationFitnessSummary s{};
+        if (!TryReadPopulationFitnessSummaryFromDevice(buffers, s)) return Fail(buffers, "summary");
+        std::printf("generation %zu: best %.6f mean %.6f best_index %zu population %zu\n", s.generation_index,
+                    s.best_fitness, s.average_fitness, s.best_index, s.population_size);
+    }
+    if (!TryEvaluateCurrentGenerationFitnessOnDevice(buffers, counts)) return Fail(buffers, "final evaluation");
+    PopulationFitnessSummary s{};
+    (void)TryReadPopulationFitnessSummaryFromDevice(buffers, s);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    std::printf("final generation %zu: best %.6f mean %.6f (%zu post-fitness callbacks, %.3f s)\n",
+                s.generation_index, s.best_fitness, s.average_fitness, evaluated, secs);
+    SlabGeneration current{};
+    if (!TryDownloadCurrentGenerationFromDevice(buffers, current) || current.slot_indices.size() != cells)
+        return Fail(buffers, "download");
+    DestroyDeviceSlabGARuntimeBuffers(buffers);
+    return evaluated == generations ? 0 : 1;
+}

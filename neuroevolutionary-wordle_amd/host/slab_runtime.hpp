@@ -1,0 +1,323 @@
+// slab_runtime.hpp -- C++ host API with the reference's names and argument meaning
+// (src/genetic_algorithm/device/slab_runtime.hpp:119-186), implemented over the C ABI of
+// include/ne_runtime.h.  Header-only; link against libne_wordle.so.
+//
+// Differences from the reference, all at the boundary: the catalog and the per-cell secrets belong
+// to the runtime handle (not to process-global constant memory), downloads fill std::vector, and
+// every call returns false with the runtime status readable through TryReadDeviceSlabGARuntimeStatus.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <functional>
+#include <vector>
+
+#include "../../include/ne_runtime.h"
+
+namespace ne_amd::slab_device {
+
+enum class DeviceSlabGARuntimeStatusCode : int {
+    kOk = 0,
+    kCudaFailure = 1,
+    kInvalidRuntimeConfig = 2,
+    kInvalidSlab = 3,
+    kInvalidGeneration = 4,
+    kInvalidTrainingShard = 5,
+    kGuessAppendFailed = 6,
+    kPolicyForwardFailed = 7,
+    kActionSelectionFailed = 8,
+    kPopulationNotEvaluated = 9,
+    kInvalidAssemblyConfig = 10,
+    kParentSelectionFailed = 11,
+    kInvalidAssemblyPlan = 12,
+    kInvalidParentIndex = 13,
+    kSlabFull = 14,
+    kOutputEmbeddingInjectionFailed = 15,
+    kSlabRepackFailed = 16,
+    kInvalidArgument = 17,
+    kUnsupported = 18,
+    kCallbackFailed = 19,
+};
+
+struct DeviceSlabGARuntimeConfig {
+    std::size_t genotype_slab_byte_budget_bytes = 0;
+    std::size_t generation_byte_budget_bytes = 0;
+    std::size_t host_spillover_byte_budget_bytes = 0;
+    std::size_t action_count = 0;
+    std::size_t population_size_ceiling = 0;
+    std::size_t grid_column_count = 0;
+};
+
+struct WordCountSchedule {
+    std::size_t initial_word_count = 20;
+    std::size_t word_count_step = 0;
+    std::size_t word_count_step_period_generations = 1;
+};
+
+struct RuntimeWordCounts {
+    std::size_t training_word_count = 20;
+    std::size_t action_space_word_count = 20;
+    WordCountSchedule training_word_schedule{};
+    std::size_t shard_initial_radius = 0;
+    std::size_t shard_radius_growth_period_generations = 2;
+};
+
+struct ParentSelectionConfig {
+    std::size_t cellular_breeding_radius = 2;
+    float rank_exponent = 0.5f;
+};
+struct BreedingConfig {
+    float crossover_temperature_level1 = 0.02f;
+    float crossover_temperature_level2 = 0.01f;
+    float crossover_temperature_level3 = 0.005f;
+};
+struct MutationConfig {
+    float mutation_probability = 0.0001f;
+    float mutation_sigma = 0.02f;
+    float output_tail_row_scale_mutation_probability = 0.0f;
+};
+struct GenerationAssemblyConfig {
+    ParentSelectionConfig parent_selection{};
+    BreedingConfig breeding{};
+    MutationConfig mutation{};
+};
+
+struct PendingOutputEmbeddingInjection {
+    bool enabled = false;
+    std::size_t first_catalog_word_index = 0;
+    std::size_t injection_count = 0;
+};
+
+struct PopulationFitnessSummary {
+    float best_fitness = 0.0f;
+    float average_fitness = 0.0f;
+    std::size_t best_index = 0;
+    std::uint32_t best_slot_index = 0xFFFFFFFFu;
+    std::size_t generation_index = 0;
+    std::size_t action_count = 0;
+    std::size_t population_size = 0;
+};
+
+struct SlabGeneration {
+    std::size_t generation_index = 0;
+    std::vector<std::uint32_t> slot_indices;
+    std::vector<float> fitness;
+    std::vector<std::uint32_t> evaluation_counts;
+    std::vector<std::uint8_t> has_fitness;
+};
+
+struct TrainingDataShardReleaseHistory {
+    std::vector<std::uint64_t> release_generations;
+};
+
+struct DeviceSlabGARuntimeBuffers {
+    ne_runtime* handle = nullptr;
+    std::size_t grid_column_count = 0;
+};
+
+using PostFitnessEvaluationCallback = std::function<bool(const DeviceSlabGARuntimeBuffers&, const RuntimeWordCounts&)>;
+using TrainingShardReleaseCallback = std::function<bool(const DeviceSlabGARuntimeBuffers&, const RuntimeWordCounts&,
+                                                        PendingOutputEmbeddingInjection&)>;
+
+namespace detail {
+
+inline ne_word_counts ToC(const RuntimeWordCounts& w) {
+    ne_word_counts c{};
+    c.training_word_count = w.training_word_count;
+    c.action_space_word_count = w.action_space_word_count;
+    c.training_word_schedule.initial_word_count = w.training_word_schedule.initial_word_count;
+    c.training_word_schedule.word_count_step = w.training_word_schedule.word_count_step;
+    c.training_word_schedule.word_count_step_period_generations =
+        w.training_word_schedule.word_count_step_period_generations;
+    c.shard_initial_radius = w.shard_initial_radius;
+    c.shard_radius_growth_period_generations = w.shard_radius_growth_period_generations;
+    return c;
+}
+
+inline RuntimeWordCounts FromC(const ne_word_counts& c) {
+    RuntimeWordCounts w{};
+    w.training_word_count = c.training_word_count;
+    w.action_space_word_count = c.action_space_word_count;
+    w.training_word_schedule = {c.training_word_schedule.initial_word_count, c.training_word_schedule.word_count_step,
+                                c.training_word_schedule.word_count_step_period_generations};
+    w.shard_initial_radius = c.shard_initial_radius;
+    w.shard_radius_growth_period_generations = c.shard_radius_growth_period_generations;
+    return w;
+}
+
+struct CallbackContext {
+    const DeviceSlabGARuntimeBuffers* buffers;
+    const PostFitnessEvaluationCallback* post;
+    const TrainingShardReleaseCallback* release;
+};
+
+inline int PostTrampoline(ne_runtime*, const ne_word_counts* counts, void* user) {
+    auto* ctx = static_cast<CallbackContext*>(user);
+    return (*ctx->post)(*ctx->buffers, FromC(*counts)) ? 0 : 1;
+}
+
+inline int ReleaseTrampoline(ne_runtime*, const ne_word_counts* counts, ne_pending_injection* inj, void* user) {
+    auto* ctx = static_cast<CallbackContext*>(user);
+    PendingOutputEmbeddingInjection p{inj->enabled != 0, inj->first_catalog_word_index, inj->injection_count};
+    if (!(*ctx->release)(*ctx->buffers, FromC(*counts), p)) return 1;
+    inj->enabled = p.enabled ? 1u : 0u;
+    inj->first_catalog_word_index = p.first_catalog_word_index;
+    inj->injection_count = p.injection_count;
+    return 0;
+}
+
+}  // namespace detail
+
+// TryCreateDeviceSlabGARuntimeBuffers (slab_runtime.hpp:119)
+inline bool TryCreateDeviceSlabGARuntimeBuffers(DeviceSlabGARuntimeBuffers& buffers,
+                                                const DeviceSlabGARuntimeConfig& config) {
+    ne_runtime_config c{};
+    c.genotype_slab_byte_budget = config.genotype_slab_byte_budget_bytes;
+    c.generation_byte_budget = config.generation_byte_budget_bytes;
+    c.host_spillover_byte_budget = config.host_spillover_byte_budget_bytes;
+    c.action_count = config.action_count;
+    c.population_size_ceiling = config.population_size_ceiling;
+    c.grid_column_count = config.grid_column_count;
+    buffers = {};
+    if (ne_create(&c, &buffers.handle) != NE_OK) return false;
+    buffers.grid_column_count = config.grid_column_count;
+    return true;
+}
+
+// DestroyDeviceSlabGARuntimeBuffers (slab_runtime.hpp:121)
+inline void DestroyDeviceSlabGARuntimeBuffers(DeviceSlabGARuntimeBuffers& buffers) noexcept {
+    ne_destroy(buffers.handle);
+    buffers = {};
+}
+
+// UploadTrainingWordCatalogToDeviceConstantMemory (training_data.hpp:375-381), per handle.
+inline bool TryUploadTrainingWordCatalog(DeviceSlabGARuntimeBuffers& buffers, const std::vector<std::uint8_t>& letters) {
+    return ne_upload_catalog(buffers.handle, letters.data(), letters.size() / 5) == NE_OK;
+}
+
+// Per-cell secrets as CSR; replaces the shard schedule when set.
+inline bool TryUploadTrainingSecrets(DeviceSlabGARuntimeBuffers& buffers, const std::vector<std::uint32_t>& offsets,
+                                     const std::vector<std::uint32_t>& secret_ids) {
+    return ne_upload_secrets_csr(buffers.handle, offsets.data(), offsets.size() - 1, secret_ids.data(),
+                                 secret_ids.size()) == NE_OK;
+}
+
+// TryBootstrapRandomCurrentGenerationOnDevice (slab_runtime.hpp:123-124)
+inline bool TryBootstrapRandomCurrentGenerationOnDevice(DeviceSlabGARuntimeBuffers& buffers,
+                                                        std::size_t generation_size, std::uint32_t generation_seed,
+                                                        std::size_t generation_index = 0,
+                                                        const ne_init_config* init = nullptr) {
+    return ne_bootstrap(buffers.handle, generation_size, generation_seed, generation_index, init) == NE_OK;
+}
+
+// TryEvaluateCurrentGenerationFitnessOnDevice (slab_runtime.hpp:137-139)
+inline bool TryEvaluateCurrentGenerationFitnessOnDevice(DeviceSlabGARuntimeBuffers& buffers,
+                                                        const RuntimeWordCounts& counts,
+                                                        const TrainingDataShardReleaseHistory* history = nullptr) {
+    const ne_word_counts c = detail::ToC(counts);
+    ne_shard_release_history h{};
+    if (history) {
+        h.release_generations = history->release_generations.data();
+        h.release_count = history->release_generations.size();
+    }
+    return ne_evaluate(buffers.handle, &c, history ? &h : nullptr) == NE_OK;
+}
+
+// TryReadPopulationFitnessSummaryFromDevice (slab_runtime.hpp:141-142)
+inline bool TryReadPopulationFitnessSummaryFromDevice(const DeviceSlabGARuntimeBuffers& buffers,
+                                                      PopulationFitnessSummary& summary) {
+    ne_fitness_summary s{};
+    if (ne_read_summary(buffers.handle, &s) != NE_OK) return false;
+    summary = {s.best_fitness, s.average_fitness, s.best_index, s.best_slot_index, s.generation_index,
+               s.action_count, s.population_size};
+    return true;
+}
+
+// TryAdvanceGenerationOnDevice (slab_runtime.hpp:144-150)
+inline bool TryAdvanceGenerationOnDevice(DeviceSlabGARuntimeBuffers& buffers, std::uint32_t generation_seed,
+                                         const RuntimeWordCounts& counts, const GenerationAssemblyConfig& config = {},
+                                         const PendingOutputEmbeddingInjection& injection = {}, bool verbose = false,
+                                         const PostFitnessEvaluationCallback& post_fitness_evaluation_callback = {},
+                                         const TrainingShardReleaseCallback& training_shard_release_callback = {},
+                                         const TrainingDataShardReleaseHistory* history = nullptr) {
+    const ne_word_counts c = detail::ToC(counts);
+    ne_assembly_config a{};
+    a.cellular_breeding_radius = config.parent_selection.cellular_breeding_radius;
+    a.rank_exponent = config.parent_selection.rank_exponent;
+    a.crossover_temperature_level1 = config.breeding.crossover_temperature_level1;
+    a.crossover_temperature_level2 = config.breeding.crossover_temperature_level2;
+    a.crossover_temperature_level3 = config.breeding.crossover_temperature_level3;
+    a.mutation_probability = config.mutation.mutation_probability;
+    a.mutation_sigma = config.mutation.mutation_sigma;
+    a.output_tail_row_scale_mutation_probability = config.mutation.output_tail_row_scale_mutation_probability;
+    ne_pending_injection inj{};
+    inj.enabled = injection.enabled ? 1u : 0u;
+    inj.first_catalog_word_index = injection.first_catalog_word_index;
+    inj.injection_count = injection.injection_count;
+    detail::CallbackContext ctx{&buffers, &post_fitness_evaluation_callback, &training_shard_release_callback};
+    ne_callbacks cbs{};
+    cbs.post_fitness_evaluation = post_fitness_evaluation_callback ? detail::PostTrampoline : nullptr;
+    cbs.training_shard_release = training_shard_release_callback ? detail::ReleaseTrampoline : nullptr;
+    cbs.user = &ctx;
+    ne_shard_release_history h{};
+    if (history) {
+        h.release_generations = history->release_generations.data();
+        h.release_count = history->release_generations.size();
+    }
+    return ne_advance_generation(buffers.handle, generation_seed, &c, &a, &inj, verbose ? 1u : 0u, &cbs,
+                                 history ? &h : nullptr) == NE_OK;
+}
+
+inline bool TryDownloadGenerationFromDevice(const DeviceSlabGARuntimeBuffers& buffers, int which,
+                                            SlabGeneration& generation) {
+    ne_slab_info info{};
+    if (ne_get_slab_info(buffers.handle, &info) != NE_OK) return false;
+    const std::size_t cap = info.max_generation_size;
+    generation.slot_indices.assign(cap, 0);
+    generation.fitness.assign(cap, 0.0f);
+    generation.evaluation_counts.assign(cap, 0);
+    generation.has_fitness.assign(cap, 0);
+    uint64_t n = 0;
+    if (ne_download_generation(buffers.handle, which, generation.slot_indices.data(), generation.fitness.data(),
+                               generation.evaluation_counts.data(), generation.has_fitness.data(), cap, &n) != NE_OK)
+        return false;
+    generation.slot_indices.resize(n);
+    generation.fitness.resize(n);
+    generation.evaluation_counts.resize(n);
+    generation.has_fitness.resize(n);
+    generation.generation_index = which == 0 ? info.current_generation_index : info.current_generation_index + 1;
+    return true;
+}
+
+// TryDownloadCurrentGenerationFromDevice / TryDownloadNextGenerationFromDevice (slab_runtime.hpp:131-135)
+inline bool TryDownloadCurrentGenerationFromDevice(const DeviceSlabGARuntimeBuffers& b, SlabGeneration& g) {
+    return TryDownloadGenerationFromDevice(b, 0, g);
+}
+inline bool TryDownloadNextGenerationFromDevice(const DeviceSlabGARuntimeBuffers& b, SlabGeneration& g) {
+    return TryDownloadGenerationFromDevice(b, 1, g);
+}
+
+// TryDownloadSlabSlotBytesFromDevice (slab_runtime.hpp:128-129)
+inline bool TryDownloadSlabSlotBytesFromDevice(const DeviceSlabGARuntimeBuffers& buffers, std::uint32_t slot_index,
+                                               std::vector<std::uint8_t>& slot_bytes) {
+    ne_slab_info info{};
+    if (ne_get_slab_info(buffers.handle, &info) != NE_OK) return false;
+    slot_bytes.assign(info.slot_stride_bytes, 0);
+    uint64_t written = 0;
+    return ne_download_slot(buffers.handle, slot_index, slot_bytes.data(), slot_bytes.size(), &written) == NE_OK;
+}
+
+// TryReadDeviceSlabGARuntimeStatus (slab_runtime.hpp:183-184)
+inline bool TryReadDeviceSlabGARuntimeStatus(const DeviceSlabGARuntimeBuffers& buffers,
+                                             DeviceSlabGARuntimeStatusCode& status_code) {
+    if (!buffers.handle) return false;
+    status_code = static_cast<DeviceSlabGARuntimeStatusCode>(ne_read_status(buffers.handle));
+    return true;
+}
+
+// DeviceSlabGARuntimeStatusCodeString (slab_runtime.hpp:186)
+inline const char* DeviceSlabGARuntimeStatusCodeString(DeviceSlabGARuntimeStatusCode status_code) noexcept {
+    return ne_status_string(static_cast<int>(status_code));
+}
+
+}  // namespace ne_amd::slab_device

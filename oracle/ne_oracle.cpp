@@ -582,6 +582,73 @@ void BreedChild(const Layout& L, const float* p1, const float* p2, float* child,
     for (uint64_t e = n; e < L.P + A_child * kTail; ++e) child[e] = 0.0f;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Hint grids (src/wordle/hint_grid.hpp:19-164): grid 0 appends every distinct left rotation of the
+// word whose feedback is all yellow; grids 1 and 2 each hold one swap of two differing positions
+// whose feedback is 3 green + 2 yellow, the first two distinct such swaps in (i<j) order.
+bool BuildHintGroup(const uint8_t* w, Grid grids[3]) {
+    for (int i = 0; i < 3; ++i) std::memset(&grids[i], 0, sizeof(Grid));
+    std::memcpy(grids[0].sol, w, 5);
+    uint8_t seen[4][5];
+    int nseen = 0;
+    for (int rot = 1; rot < 5; ++rot) {
+        uint8_t g[5];
+        for (int p = 0; p < 5; ++p) g[p] = w[(p + rot) % 5];
+        if (SameWord(g, w)) continue;
+        bool dup = false;
+        for (int k = 0; k < nseen; ++k) dup |= SameWord(seen[k], g);
+        if (dup) continue;
+        uint8_t fb[5];
+        Feedback(g, w, fb);
+        bool all_yellow = true;
+        for (int p = 0; p < 5; ++p) all_yellow &= fb[p] == 1;
+        if (!all_yellow) continue;
+        if (!Append(grids[0], g)) return false;
+        std::memcpy(seen[nseen++], g, 5);
+    }
+    uint8_t chosen[2][5];
+    int nchosen = 0;
+    for (int a = 0; a < 5; ++a)
+        for (int b = a + 1; b < 5; ++b) {
+            if (w[a] == w[b]) continue;
+            uint8_t g[5];
+            std::memcpy(g, w, 5);
+            std::swap(g[a], g[b]);
+            uint8_t fb[5];
+            Feedback(g, w, fb);
+            int greens = 0, yellows = 0;
+            for (int p = 0; p < 5; ++p) { greens += fb[p] == 0; yellows += fb[p] == 1; }
+            if (greens != 3 || yellows != 2) continue;
+            bool dup = false;
+            for (int k = 0; k < nchosen; ++k) dup |= SameWord(chosen[k], g);
+            if (dup) continue;
+            Grid& grid = grids[1 + nchosen];
+            std::memcpy(grid.sol, w, 5);
+            if (!Append(grid, g)) return false;
+            std::memcpy(chosen[nchosen++], g, 5);
+            if (nchosen == 2) return true;
+        }
+    return false;
+}
+
+// TrainableTailNorm (output_embedding_injection.hpp:70-79): sequential sum of squares, sqrt.
+float TailNorm(const float* t) {
+    float s = 0.0f;
+    for (int j = 0; j < kTail; ++j) s += t[j] * t[j];
+    return std::sqrt(s);
+}
+
+// TryComputeMedianTailNorm (output_embedding_injection.hpp:80-104): the upper-middle order
+// statistic, averaged with the lower-middle one for an even count.
+float MedianTailNorm(const float* tails, uint64_t count) {
+    std::vector<float> n(count);
+    for (uint64_t a = 0; a < count; ++a) n[a] = TailNorm(tails + a * kTail);
+    std::sort(n.begin(), n.end());
+    const uint64_t up = count / 2;
+    if (count % 2) return n[up];
+    return 0.5f * (n[up - 1] + n[up]);
+}
+
 }  // namespace
 
 // ================================================================================================
@@ -807,6 +874,90 @@ int oro_plan(const float* fit, const uint8_t* has, uint64_t cur_size, uint64_t c
     return kOk;
 }
 
+// Slot bookkeeping of one assembly (device_runtime.cu:1434-1557 with TryAllocateSlabSlot /
+// TryReleaseSlabSlot / TryReleaseParentReference, slab_allocator.hpp:271-354,
+// reference_counter.hpp:135-220) in the build's deterministic order: parents with no planned child
+// are freed in index order; batch b gives child i the slot free_stack[free_count-1-i]; after the
+// batch every parent whose count reached zero is freed in index order.  Returns a status and the
+// number of batches.
+int oro_assemble_slots(uint32_t slot_count, uint32_t* free_stack, uint32_t* free_count, uint32_t* slot_state,
+                       uint32_t* cur_slots, uint32_t cur_size, const uint32_t* pairs, uint32_t child_count,
+                       uint32_t* child_slots, uint32_t* batches_out) {
+    std::vector<uint32_t> ref(cur_size, 0);
+    for (uint32_t c = 0; c < child_count; ++c) {
+        if (pairs[2 * c] >= cur_size || pairs[2 * c + 1] >= cur_size) return kInvalidParentIndex;
+        ++ref[pairs[2 * c]];
+        ++ref[pairs[2 * c + 1]];
+    }
+    auto collect = [&]() -> int {
+        for (uint32_t i = 0; i < cur_size; ++i) {
+            if (ref[i] != 0 || cur_slots[i] == 0xFFFFFFFFu) continue;
+            const uint32_t sl = cur_slots[i];
+            if (sl >= slot_count || slot_state[sl] == 0 || *free_count >= slot_count) return kInvalidSlab;
+            slot_state[sl] = 0;
+            free_stack[(*free_count)++] = sl;
+            cur_slots[i] = 0xFFFFFFFFu;
+        }
+        return kOk;
+    };
+    int st = collect();
+    if (st != kOk) return st;
+    uint32_t offset = 0, batches = 0;
+    while (offset < child_count) {
+        const uint32_t fc = *free_count;
+        if (fc == 0) return kSlabFull;
+        const uint32_t batch = std::min(child_count - offset, fc);
+        for (uint32_t i = 0; i < batch; ++i) {
+            const uint32_t sl = free_stack[fc - 1 - i];
+            child_slots[offset + i] = sl;
+            slot_state[sl] = 1;
+        }
+        *free_count = fc - batch;
+        for (uint32_t i = 0; i < batch; ++i) {
+            --ref[pairs[2 * (offset + i)]];
+            --ref[pairs[2 * (offset + i) + 1]];
+        }
+        st = collect();
+        if (st != kOk) return st;
+        offset += batch;
+        ++batches;
+    }
+    if (batches_out) *batches_out = batches;
+    return kOk;
+}
+
+// Widening repack of the slot bookkeeping (repacking.hpp:104-287; device_runtime.cu:978-1033):
+// parents without a planned child are freed, the referenced ones keep their slot order and move to
+// slots [S'-k, S') of the wider layout, the free stack becomes [S'-k-1 ... 0].  moves[r] receives the
+// old slot of survivor r.  Returns a status.
+int oro_repack_slots(uint32_t new_slot_count, uint32_t* cur_slots, uint32_t cur_size, const uint32_t* pairs,
+                     uint32_t child_count, uint32_t* slot_state, uint32_t* free_stack, uint32_t* free_count,
+                     uint32_t* moves, uint32_t* survivors_out) {
+    std::vector<uint32_t> ref(cur_size, 0);
+    for (uint32_t c = 0; c < child_count; ++c) {
+        ++ref[pairs[2 * c]];
+        ++ref[pairs[2 * c + 1]];
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> surv;  // (old slot, parent)
+    for (uint32_t i = 0; i < cur_size; ++i) {
+        if (ref[i] == 0) { cur_slots[i] = 0xFFFFFFFFu; continue; }
+        surv.emplace_back(cur_slots[i], i);
+    }
+    std::sort(surv.begin(), surv.end());
+    const uint32_t k = uint32_t(surv.size());
+    if (k == 0 || k > new_slot_count) return kSlabRepackFailed;
+    const uint32_t base = new_slot_count - k;
+    for (uint32_t r = 0; r < k; ++r) {
+        moves[r] = surv[r].first;
+        cur_slots[surv[r].second] = base + r;
+    }
+    for (uint32_t sl = 0; sl < new_slot_count; ++sl) slot_state[sl] = sl >= base ? 1u : 0u;
+    for (uint32_t f = 0; f < base; ++f) free_stack[f] = base - 1 - f;
+    *free_count = base;
+    *survivors_out = k;
+    return kOk;
+}
+
 void oro_xover_decisions(int enc_hidden, uint64_t A_parent, uint32_t seed, uint32_t gen, uint32_t child, float t1,
                          float t2, float t3, uint32_t* rec_words) {
     XoverRecord r{};
@@ -823,6 +974,62 @@ void oro_breed_child(int enc_hidden, const float* p1, const float* p2, float* ch
     BreedChild(MakeLayout(enc_hidden), p1, p2, child, A_parent, A_child, seed, gen, ci, t1, t2, t3,
                MutationCfg{p, sigma, p_scale}, &r);
     if (rec_words) std::memcpy(rec_words, &r, sizeof(r));
+}
+
+int oro_hint_group(const uint8_t* word, uint32_t* turn_counts, uint8_t* letters, uint8_t* tiles) {
+    Grid g[3];
+    if (!BuildHintGroup(word, g)) return 0;
+    for (int i = 0; i < 3; ++i) {
+        turn_counts[i] = uint32_t(g[i].tc);
+        for (int t = 0; t < g[i].tc && t < 5; ++t)
+            for (int p = 0; p < 5; ++p) {
+                letters[(i * 5 + t) * 5 + p] = g[i].guess[t][p];
+                tiles[(i * 5 + t) * 5 + p] = g[i].tile[t][p];
+            }
+    }
+    return 1;
+}
+
+float oro_median_tail_norm(const float* tails, uint64_t count) { return MedianTailNorm(tails, count); }
+
+// Output-embedding injection into one child (TryInjectExpandedOutputEmbeddingTailsConcurrently,
+// injection_ops.cuh:21-64; TrySeedOutputEmbeddingTailFromHintGridsConcurrently and
+// TryScaleTrainableTailToNormConcurrently, output_embedding_injection.hpp:107-205): for each new word,
+// the mean over its 3 hint grids of policy dims 26..63, rescaled to the median norm of the child's
+// existing rows.  mode 0 = fp32 sequential forward, 1 = fp64 forward.  Returns a status.
+int oro_inject(const float* genome_in, float* genome_out, int enc_hidden, uint64_t parent_a,
+               const uint8_t* catalog, uint64_t first_word, uint64_t count, int mode) {
+    const Layout L = MakeLayout(enc_hidden);
+    if (parent_a == 0 || count == 0 || first_word != parent_a) return kOutputEmbeddingInjectionFailed;
+    const float target = MedianTailNorm(genome_in + L.P, parent_a);
+    for (uint64_t k = 0; k < count; ++k) {
+        Grid grids[3];
+        if (!BuildHintGroup(catalog + 5 * (first_word + k), grids)) return kOutputEmbeddingInjectionFailed;
+        float sums[kTail];
+        for (int f = 0; f < kTail; ++f) sums[f] = 0.0f;
+        for (int gi = 0; gi < 3; ++gi) {
+            float pol[kPol];
+            if (mode == 0) {
+                if (!PolicyForward<float>(genome_in, L, grids[gi], pol)) return kOutputEmbeddingInjectionFailed;
+            } else {
+                double p64[kPol];
+                if (!PolicyForward<double>(genome_in, L, grids[gi], p64)) return kOutputEmbeddingInjectionFailed;
+                for (int d = 0; d < kPol; ++d) pol[d] = float(p64[d]);
+            }
+            for (int f = 0; f < kTail; ++f) sums[f] += pol[kFixed + f];
+        }
+        float* row = genome_out + L.P + (parent_a + k) * kTail;
+        const float third = 1.0f / 3.0f;
+        for (int f = 0; f < kTail; ++f) row[f] = sums[f] * third;
+        float scale = 0.0f;
+        if (target > 1.0e-6f) {
+            const float cur = TailNorm(row);
+            if (!(cur > 1.0e-6f)) return kOutputEmbeddingInjectionFailed;
+            scale = target / cur;
+        }
+        for (int f = 0; f < kTail; ++f) row[f] = row[f] * scale;
+    }
+    return kOk;
 }
 
 // Generation-0 genome (InitializeRandomGenome, genome_ops.cuh:58-143) on the counter-based draws:

@@ -40,6 +40,11 @@ _SIGS = {
     "oro_plan": (I32, [P, P, U64, U64, U64, U64, U64, P, U32, P]),
     "oro_xover_decisions": (None, [I32, U64, U32, U32, U32, F32, F32, F32, P]),
     "oro_xover_record_words": (U64, []),
+    "oro_assemble_slots": (I32, [U32, P, P, P, P, U32, P, U32, P, P]),
+    "oro_hint_group": (I32, [P, P, P, P]),
+    "oro_repack_slots": (I32, [U32, P, U32, P, U32, P, P, P, P, P]),
+    "oro_median_tail_norm": (F32, [P, U64]),
+    "oro_inject": (I32, [P, P, I32, U64, P, U64, U64, I32]),
     "oro_breed_child": (None, [I32, P, P, P, U64, U64, U32, U32, U32, F32, F32, F32, F32, F32, F32, P]),
     "oro_bootstrap_genome": (None, [I32, P, U64, U32, U32, U32, P, F32]),
 }
@@ -240,3 +245,62 @@ def score_action(policy, fixed, tail):
     return lib().oro_score_action(ptr(np.ascontiguousarray(policy, np.float32)),
                                   ptr(np.ascontiguousarray(fixed, np.float32)),
                                   ptr(np.ascontiguousarray(tail, np.float32)))
+
+
+def slab_after_bootstrap(slot_count, n):
+    """Free stack / count / slot states after creation and a bootstrap of n organisms."""
+    stack = np.array([slot_count - 1 - i for i in range(slot_count)], np.uint32)
+    state = np.zeros(slot_count, np.uint32)
+    state[:n] = 1
+    return stack, np.uint32(slot_count - n), state, np.arange(n, dtype=np.uint32)
+
+
+def assemble_slots(slot_count, stack, free_count, state, cur_slots, pairs):
+    stack = stack.copy()
+    state = state.copy()
+    cur = np.ascontiguousarray(cur_slots, np.uint32).copy()
+    pr = np.ascontiguousarray(pairs, np.uint32)
+    fc = np.array([free_count], np.uint32)
+    child = np.zeros(len(pr), np.uint32)
+    nb = np.zeros(1, np.uint32)
+    st = lib().oro_assemble_slots(slot_count, ptr(stack), ptr(fc), ptr(state), ptr(cur), cur.size, ptr(pr), len(pr),
+                                  ptr(child), ptr(nb))
+    return dict(status=st, stack=stack, free_count=int(fc[0]), state=state, parent_slots=cur, child_slots=child,
+                batches=int(nb[0]))
+
+
+def hint_group(word_letters):
+    w = np.ascontiguousarray(word_letters, np.uint8)
+    tc = np.zeros(3, np.uint32)
+    letters = np.zeros((3, 5, 5), np.uint8)
+    tiles_ = np.zeros((3, 5, 5), np.uint8)
+    ok = lib().oro_hint_group(ptr(w), ptr(tc), ptr(letters), ptr(tiles_))
+    return ok, tc, letters, tiles_
+
+
+def median_tail_norm(tails):
+    t = np.ascontiguousarray(tails, np.float32)
+    return lib().oro_median_tail_norm(ptr(t), t.size // 38)
+
+
+def inject(genome, parent_a, catalog, first_word, count, mode=0, enc_hidden=0):
+    """Returns (status, genome with rows [parent_a, parent_a+count) injected)."""
+    g = np.ascontiguousarray(genome, np.float32)
+    out = g.copy()
+    cat = np.ascontiguousarray(catalog, np.uint8)
+    st = lib().oro_inject(ptr(g), ptr(out), enc_hidden, parent_a, ptr(cat), first_word, count, mode)
+    return st, out
+
+
+def repack_slots(new_slot_count, cur_slots, pairs):
+    cur = np.ascontiguousarray(cur_slots, np.uint32).copy()
+    pr = np.ascontiguousarray(pairs, np.uint32)
+    state = np.zeros(new_slot_count, np.uint32)
+    stack = np.zeros(new_slot_count, np.uint32)
+    fc = np.zeros(1, np.uint32)
+    moves = np.zeros(len(cur), np.uint32)
+    k = np.zeros(1, np.uint32)
+    st = lib().oro_repack_slots(new_slot_count, ptr(cur), cur.size, ptr(pr), len(pr), ptr(state), ptr(stack), ptr(fc),
+                                ptr(moves), ptr(k))
+    return dict(status=st, parent_slots=cur, state=state, stack=stack, free_count=int(fc[0]),
+                moves=moves[:int(k[0])])

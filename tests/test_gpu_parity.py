@@ -232,3 +232,145 @@ def test_masking_end_to_end(pkg):
     # W = 2, variant 1 of entry 0 prefills local[1], local[0] = CRANE, TRACE: won by the prefill, score 14
     assert tr[2]["decisions"] == 0 and tr[2]["score"] == 14.0
     rt.close()
+
+
+def test_slot_bookkeeping_tight_slab(pkg, orc):
+    """A slab with room for only 20 extra genomes: children are assembled in several batches as parents
+    are released; slot assignment equals the oracle's bookkeeping on the same plan."""
+    cat = pkg.synth.make_catalog(48)
+    sec = pkg.synth.sample_secrets(48, 16)
+    off, ids = pkg.synth.block_csr(8, 8, 4, sec)
+    rtm = pkg.runtime
+    stride = rtm.slot_stride_bytes(48)
+    rt = pkg.SlabRuntime(slab_bytes=(64 + 20) * stride, generation_bytes=64 * stride, action_count=48,
+                         grid_columns=8)
+    rt.upload_catalog(cat)
+    rt.upload_secrets_csr(off, ids)
+    rt.bootstrap(64, seed=3, variance_scale=2.0, tail_std=0.5)
+    rt.advance(99, rtm.word_counts(48, 48), rtm.default_assembly_config(mutation_probability=0.0))
+    pairs = rt.plan()
+    children, _, _, _ = rt.generation(0)
+    info = rt.info()
+    stack, fc, state, cur = orc.slab_after_bootstrap(84, 64)
+    r = orc.assemble_slots(84, stack, fc, state, cur, pairs)
+    assert r["status"] == 0
+    assert children.tolist() == r["child_slots"].tolist()
+    assert info.free_slot_count == r["free_count"]
+    rt.close()
+
+
+def test_genuinely_full_slab_reports_slab_full(pkg):
+    cat = pkg.synth.make_catalog(8)
+    rtm = pkg.runtime
+    stride = rtm.slot_stride_bytes(8)
+    rt = pkg.SlabRuntime(slab_bytes=2 * stride, generation_bytes=2 * stride, action_count=8, grid_columns=1)
+    rt.upload_catalog(cat)
+    rt.upload_secrets_csr(np.array([0, 4, 8], np.uint32), np.arange(8, dtype=np.uint32))  # W = 4
+    rt.bootstrap(2, seed=3)
+    with pytest.raises(RuntimeError) as err:
+        rt.advance(5, rtm.word_counts(8, 8))
+    assert err.value.status == 14
+    rt.close()
+
+
+def test_cpp_driver_runs_generations(pkg, tmp_path):
+    """The C++ host API (host/slab_runtime.hpp) end to end: bootstrap, two TryAdvanceGenerationOnDevice
+    calls with a post-fitness callback, a final evaluation (cf. the reference's CLI smoke test,
+    CMakeLists.txt:708-714)."""
+    import os
+    import subprocess
+    cat = pkg.synth.make_catalog(64)
+    sec = pkg.synth.sample_secrets(64, 32)
+    off, ids = pkg.synth.block_csr(8, 8, 4, sec)
+    cat.astype(np.uint8).tofile(tmp_path / "catalog.u8")
+    off.astype(np.uint32).tofile(tmp_path / "offsets.u32")
+    ids.astype(np.uint32).tofile(tmp_path / "secrets.u32")
+    driver = os.path.join(os.path.dirname(pkg.runtime.LIB_PATH), "run_ga")
+    res = subprocess.run([driver, str(tmp_path), "8", "2", "7"], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    lines = res.stdout.strip().splitlines()
+    # each advance reports the evaluation of the generation it bred from
+    assert lines[0].startswith("generation 0:") and lines[1].startswith("generation 1:")
+    assert lines[-1].startswith("final generation 2:") and "2 post-fitness callbacks" in lines[-1]
+
+
+def _catalog_with_injectable_tail(pkg, orc, n, first):
+    """A synthetic catalog whose words from `first` on all have a hint-grid group."""
+    cat = pkg.synth.make_catalog(4 * n, seed=0x1234)
+    ok = np.array([orc.hint_group(w)[0] == 1 for w in cat])
+    head = cat[:first]
+    rest = [w for w, good in zip(cat[first:], ok[first:]) if good]
+    return np.concatenate([head, np.stack(rest[:n - first])]).astype(np.uint8)
+
+
+def test_widening_repack_and_injection(pkg, orc):
+    """A growth generation: repack to the wider stride, breed with the old action count, inject the new
+    rows from hint grids (rows 32..47).  Old part bit-exact, new rows within fp32 tolerance, slots as
+    the oracle's repack + assembly bookkeeping."""
+    rtm = pkg.runtime
+    A0, inj, side = 32, 16, 8
+    n = side * side
+    cat = _catalog_with_injectable_tail(pkg, orc, 64, A0)
+    sec = pkg.synth.sample_secrets(A0, 32, seed=77)
+    off, ids = pkg.synth.block_csr(side, side, 4, sec)
+    s0, s1 = rtm.slot_stride_bytes(A0), rtm.slot_stride_bytes(A0 + inj)
+    slab_bytes = 2 * n * s1 + 3 * s1
+    rt = pkg.SlabRuntime(slab_bytes=slab_bytes, generation_bytes=n * s1, action_count=A0, grid_columns=side)
+    rt.upload_catalog(cat)
+    rt.upload_secrets_csr(off, ids)
+    rt.bootstrap(n, seed=21, variance_scale=1.0, tail_std=0.1)
+    slots0, _, _, _ = rt.generation(0)
+    n0 = orc.policy_param_count() + 38 * A0
+    parents = [rt.download_slot(s)[:n0] for s in slots0]
+    temps = (0.1, 0.1, 0.2)
+    cfg = rtm.default_assembly_config(crossover_temperature_level1=temps[0], crossover_temperature_level2=temps[1],
+                                      crossover_temperature_level3=temps[2], mutation_probability=0.02)
+    injection = rtm.PendingInjection(enabled=1, reserved0=0, first_catalog_word_index=A0, injection_count=inj)
+    rt.advance(606, rtm.word_counts(A0, A0), cfg, injection=injection)
+    info = rt.info()
+    assert info.action_count == A0 + inj and info.slot_stride_bytes == s1
+    pairs = rt.plan()
+    children, _, _, _ = rt.generation(0)
+    # slot bookkeeping: repack then assembly
+    rep = orc.repack_slots(slab_bytes // s1, slots0, pairs)
+    assert rep["status"] == 0
+    asm = orc.assemble_slots(slab_bytes // s1, rep["stack"], rep["free_count"], rep["state"], rep["parent_slots"],
+                             pairs)
+    assert asm["status"] == 0 and children.tolist() == asm["child_slots"].tolist()
+    n1 = orc.policy_param_count() + 38 * (A0 + inj)
+    worst = 0.0
+    for c in range(n):
+        a, b = pairs[c]
+        ref, _ = orc.breed_child(parents[a], parents[b], A0, A0 + inj, 606, 0, c, *temps, 0.02, 0.02, 0.0)
+        st, ref_inj = orc.inject(ref, A0, cat, A0, inj, mode=0)
+        assert st == 0
+        got = rt.download_slot(children[c])
+        assert np.array_equal(got[:n0].view(np.uint32), ref[:n0].view(np.uint32)), c
+        assert not got[n1:].any()
+        target = orc.median_tail_norm(ref[orc.policy_param_count():n0])
+        new_got = got[n0:n1].reshape(inj, 38).astype(np.float64)
+        new_ref = ref_inj[n0:n1].reshape(inj, 38).astype(np.float64)
+        err = np.abs(new_got - new_ref).max() / target
+        worst = max(worst, err)
+        norms = np.sqrt((new_got ** 2).sum(axis=1))
+        assert np.all(np.abs(norms - target) <= 1e-5 * target), c
+    assert worst <= 1e-4, worst
+    rt.close()
+
+
+def test_injection_of_a_word_without_hint_grids_fails(pkg, orc):
+    rtm = pkg.runtime
+    cat = pkg.synth.make_catalog(16, seed=5)
+    cat[8] = 0  # AAAAA: no two differing positions to swap
+    sec = pkg.synth.sample_secrets(8, 8, seed=3)
+    off, ids = pkg.synth.block_csr(4, 4, 4, sec)
+    s1 = rtm.slot_stride_bytes(16)
+    rt = pkg.SlabRuntime(slab_bytes=3 * 16 * s1, generation_bytes=16 * s1, action_count=8, grid_columns=4)
+    rt.upload_catalog(cat)
+    rt.upload_secrets_csr(off, ids)
+    rt.bootstrap(16, seed=2)
+    inj = rtm.PendingInjection(enabled=1, reserved0=0, first_catalog_word_index=8, injection_count=8)
+    with pytest.raises(RuntimeError) as err:
+        rt.advance(3, rtm.word_counts(8, 8), injection=inj)
+    assert err.value.status == 15
+    rt.close()

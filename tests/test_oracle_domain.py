@@ -268,3 +268,59 @@ def test_fitness_normalisation_closed_forms(orc):
     assert L.oro_normalize_fitness(45.0 * 16, 16) == 1.0
     assert L.oro_normalize_fitness(14.0, 2) == np.float32(14.0) / np.float32(90.0)
     assert L.oro_normalize_fitness(1e9, 3) == 1.0
+
+
+# ---- hint grids (reference tests/wordle/hint_grid_test.cpp:138-229) ---------------------------
+def _grid_words(letters, tc):
+    return ["".join(chr(65 + int(c)) for c in letters[t]) for t in range(tc)]
+
+
+def test_hint_group_known_answers(orc):
+    ok, tc, letters, tl = orc.hint_group(orc.word("SPARE"))
+    assert ok == 1
+    assert tc[0] == 4 and all((tl[0, t] == 1).all() for t in range(4))  # 4 all-yellow rotations
+    assert _grid_words(letters[1], 1) == ["PSARE"] and _grid_words(letters[2], 1) == ["APSRE"]
+    ok, tc, letters, tl = orc.hint_group(orc.word("AABCD"))
+    assert ok == 1 and _grid_words(letters[0], tc[0]) == ["BCDAA", "CDAAB"]  # rotations 2 and 3 only
+    ok, tc, letters, tl = orc.hint_group(orc.word("LEVEL"))
+    assert ok == 1 and _grid_words(letters[1], 1) == ["ELVEL"] and tl[1, 0].tolist() == [1, 1, 0, 0, 0]
+    assert orc.hint_group(orc.word("AAAAA"))[0] == 0
+
+
+def test_hint_groups_against_reference_vectors(orc, golden):
+    for i, w in enumerate(golden["words"]):
+        ok, tc, letters, tl = orc.hint_group(w)
+        assert ok == golden["hint_ok"][i], i
+        if not ok:
+            continue
+        assert tc.tolist() == golden["hint_tc"][i].tolist()
+        for g in range(3):
+            for t in range(tc[g]):
+                assert letters[g, t].tolist() == golden["hint_letters"][i, g, t].tolist()
+                assert tl[g, t].tolist() == golden["hint_tiles"][i, g, t].tolist()
+
+
+def test_median_tail_norm_closed_forms(orc):
+    rows = np.zeros((5, 38), np.float32)
+    for a, v in enumerate([3.0, 1.0, 4.0, 1.5, 9.0]):
+        rows[a, 0] = v
+    assert orc.median_tail_norm(rows) == 3.0           # odd count: the middle value
+    assert orc.median_tail_norm(rows[:4]) == np.float32(0.5) * (np.float32(1.5) + np.float32(3.0))
+
+
+def test_injected_rows_have_the_median_norm(orc):
+    """slab_runtime_test.cu:577-686: every injected tail has the median norm of the existing rows."""
+    rng = np.random.default_rng(4)
+    P = orc.policy_param_count(0)
+    cat = np.stack([orc.word(w) for w in ("CRANE", "SLATE", "TRACE", "SPARE", "LEVEL", "MONEY")])
+    g = (rng.standard_normal(P + 38 * 6) * 0.1).astype(np.float32)
+    g[P + 38 * 4:] = 0
+    st, out = orc.inject(g, 4, cat, 4, 2)
+    assert st == 0
+    target = orc.median_tail_norm(g[P:P + 38 * 4])
+    for k in (4, 5):
+        row = out[P + 38 * k:P + 38 * (k + 1)].astype(np.float64)
+        assert abs(np.sqrt((row ** 2).sum()) - target) <= 1e-2 * target
+    assert np.array_equal(out[:P + 38 * 4], g[:P + 38 * 4])
+    st, _ = orc.inject(g, 4, cat, 3, 2)
+    assert st == 15  # the first injected word must follow the existing rows

@@ -1,0 +1,35 @@
+"""Pins the oracle's slot bookkeeping against the reference's known answers
+(tests/genetic_algorithm/device/genotype_slab_device_runtime_test.cu)."""
+import numpy as np
+
+
+def test_slot_reuse_order_known_answer(orc):
+    """genotype_slab_device_runtime_test.cu:447-538: 3 slots, parents in slots 0,1,2, plan
+    (0,1),(0,0): child 0 reuses the collected zero-reference parent's slot 2, child 1 the slot 1 freed
+    after parent 1's last reference."""
+    stack, fc, state, cur = orc.slab_after_bootstrap(3, 3)
+    r = orc.assemble_slots(3, stack, fc, state, cur, np.array([[0, 1], [0, 0]], np.uint32))
+    assert r["status"] == 0
+    assert r["child_slots"].tolist() == [2, 1]
+    assert r["batches"] == 2
+    assert r["free_count"] == 1 and r["stack"][0] == 0  # parent 0 freed after its last child
+
+
+def test_genuinely_full_slab(orc):
+    """genotype_slab_device_runtime_test.cu:1084-1146: 2 slots, 2 parents both referenced -> slab full."""
+    stack, fc, state, cur = orc.slab_after_bootstrap(2, 2)
+    r = orc.assemble_slots(2, stack, fc, state, cur, np.array([[0, 1], [1, 0]], np.uint32))
+    assert r["status"] == 14
+
+
+def test_every_parent_freed_and_children_distinct(orc):
+    rng = np.random.default_rng(5)
+    n, slots = 64, 80
+    stack, fc, state, cur = orc.slab_after_bootstrap(slots, n)
+    pairs = rng.integers(0, n, size=(n, 2)).astype(np.uint32)
+    r = orc.assemble_slots(slots, stack, fc, state, cur, pairs)
+    assert r["status"] == 0
+    assert len(set(r["child_slots"].tolist())) == n
+    assert (r["parent_slots"] == 0xFFFFFFFF).all()
+    assert r["free_count"] == slots - n
+    assert r["state"].sum() == n
