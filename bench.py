@@ -64,11 +64,25 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="first run one plain device-to-device copy of BYTES (a reference kernel of known "
+                         "traffic for hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        import torch
+        x = torch.empty(args.calibrate_copy // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        x.fill_(1.0)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy of {args.calibrate_copy} bytes done", file=sys.stderr, flush=True)
+        del x, y
 
     import __graft_entry__ as ge
     pkg = ge.import_package()

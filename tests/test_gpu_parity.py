@@ -337,9 +337,13 @@ def test_widening_repack_and_injection(pkg, orc):
     asm = orc.assemble_slots(slab_bytes // s1, rep["stack"], rep["free_count"], rep["state"], rep["parent_slots"],
                              pairs)
     assert asm["status"] == 0 and children.tolist() == asm["child_slots"].tolist()
+    # the generation byte budget was fixed at the old stride, so the population shrinks to whole
+    # rows that fit at the new stride (TryPlanNextGenerationShape, slab_runtime.cu:919-939)
+    expect_children = ((n * s1 // s0) // side * side * s0 // s1) // side * side
+    assert len(children) == len(pairs) == min(n, expect_children)
     n1 = orc.policy_param_count() + 38 * (A0 + inj)
     worst = 0.0
-    for c in range(n):
+    for c in range(len(children)):
         a, b = pairs[c]
         ref, _ = orc.breed_child(parents[a], parents[b], A0, A0 + inj, 606, 0, c, *temps, 0.02, 0.02, 0.0)
         st, ref_inj = orc.inject(ref, A0, cat, A0, inj, mode=0)
