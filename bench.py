@@ -18,7 +18,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
-WORKLOAD = "c2_64x64_a4739"
+WORKLOAD = "grid64_a4739"
 
 
 def algorithmic_eval_flops(decisions, encoded_turns, A):

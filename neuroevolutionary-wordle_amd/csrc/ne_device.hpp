@@ -49,6 +49,29 @@ __host__ __device__ constexpr uint64_t SlotStrideBytes(uint64_t action_count) {
     return action_count == 0 ? 0 : ((GenomeValueCount(action_count) * 4 + 255) / 256) * 256;
 }
 
+// ---- scoring layout ----------------------------------------------------------------------------
+// An action's score is the 64-term dot product of its embedding row [26 fixed | 38 tail] with the
+// policy vector (output_embedding.hpp:207-225).  On v_mfma_f32_16x16x4_f32 it takes 17 k-steps per
+// tile of 16 actions: 7 over the fixed features and 10 over the tail, so that every k-step reads a
+// single source.  Lane group g supplies, at step s, policy dim ScoreDim(s, g) (-1: a zero pad).
+constexpr int kFixedSteps = 7;
+constexpr int kTailSteps = 10;
+constexpr int kScoreSteps = kFixedSteps + kTailSteps;
+__host__ __device__ inline int ScoreDim(int s, int g) {
+    if (s < kFixedSteps) {
+        const int d = kFixedSteps * g + s;
+        return d < kFixed ? d : -1;
+    }
+    const int t = kTailSteps * g + (s - kFixedSteps);
+    return t < kTail ? kFixed + t : -1;
+}
+// Per tile of 16 actions, the catalog-wide image read by the scoring loop (identical in HBM and
+// LDS): two planes of [64 lanes][4] fixed-feature values (lane i + 16g holds steps 4h..4h+3 of
+// action row i for lane group g; step 7 and pads are 0), then the 16 actions' canonical indices
+// (the lowest catalog index holding the same word; a guessed word masks every copy of itself).
+constexpr int kImgTileWords = 2 * 64 * 4 + 16;  // 528
+constexpr int kImgTileBytes = 4 * kImgTileWords;
+
 // Dense layers in crossover / mutation order (reference genome_ops.cuh:264-309).
 struct DenseLayerDesc {
     uint32_t w, b, in, out;

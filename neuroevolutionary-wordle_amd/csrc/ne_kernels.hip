@@ -69,19 +69,20 @@ __device__ inline uint32_t FeedbackRegs(uint32_t guess, uint32_t sol) {
 // k group.  Layer-0 accumulators acc[t] hold neuron rows 16t+4gq..16t+4gq+3 of game column j.
 struct TrunkLane {
     Rsrc gr;
-    uint32_t off_b0, off_virgin, off_fold, off_w1, off_b1, off_w2, off_b2;
+    uint32_t j, gq;
+    uint32_t off_b0, off_virgin, off_fold, off_w1, off_b1;
 };
 
 __device__ __forceinline__ TrunkLane MakeTrunkLane(const float* genome, uint32_t j, uint32_t gq) {
     TrunkLane t;
     t.gr = MakeRsrc(genome, uint32_t(kPolicyParams * 4));
+    t.j = j;
+    t.gq = gq;
     t.off_b0 = uint32_t(kOffB0 + 4 * gq) * 4;
     t.off_virgin = uint32_t(kOffW0 + 4 * gq * kModelIn) * 4;
     t.off_fold = uint32_t(kOffW0 + j * kModelIn + 1 + 16 * gq) * 4;
     t.off_w1 = uint32_t(kOffW1 + j * kH0 + 4 * gq) * 4;
     t.off_b1 = uint32_t(kOffB1 + 4 * gq) * 4;
-    t.off_w2 = uint32_t(kOffW2 + (16 * (j >> 2) + (j & 3)) * kH1 + 4 * gq) * 4;
-    t.off_b2 = uint32_t(kOffB2 + 16 * gq) * 4;
     return t;
 }
 
@@ -138,9 +139,10 @@ __device__ __forceinline__ void FoldSlot(v4f (&acc)[16], const TrunkLane& tl, in
     }
 }
 
-// Layers 1 and 2: pol[s] = policy dim 16gq+s of game column j.  Layer 2's output rows are permuted
-// so each lane group holds 16 consecutive policy dims.
-__device__ __forceinline__ void TrunkToPolicy(const v4f (&acc)[16], const TrunkLane& tl, float (&pol)[16]) {
+// Layers 1 and 2: pol[s] = policy dim ScoreDim(s, gq) of game column j (0 where that is a pad).
+// Layer 2 runs 5 row tiles; row i of tile u computes policy dim ScoreDim(4u + i%4, i/4), so the
+// accumulator register r of tile u in lane group gq is exactly scoring step 4u + r.
+__device__ __forceinline__ void TrunkToPolicy(const v4f (&acc)[16], const TrunkLane& tl, float (&pol)[kScoreSteps]) {
     v4f h1[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) h1[u] = Buf4(tl.gr, tl.off_b1, 64 * u);
@@ -149,35 +151,54 @@ __device__ __forceinline__ void TrunkToPolicy(const v4f (&acc)[16], const TrunkL
         float hv[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) hv[r] = fmaxf(acc[t][r], 0.0f);
-        v4f w[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) w[u] = Buf4(tl.gr, tl.off_w1, uint32_t(16 * u * kH0 + 16 * t) * 4);
+        for (int half = 0; half < 2; ++half) {
+            v4f w[4];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+            for (int u = 0; u < 4; ++u)
+                w[u] = Buf4(tl.gr, tl.off_w1, uint32_t(16 * (4 * half + u) * kH0 + 16 * t) * 4);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) h1[u] = Mfma(w[u][r], hv[r], h1[u]);
-        __builtin_amdgcn_sched_barrier(0);
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) h1[4 * half + u] = Mfma(w[u][r], hv[r], h1[4 * half + u]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
     }
-    v4f pv[4];
+    // Policy dim of step s in lane group g: 7g + s for s < 7, else 19 + 10g + s (pads read any
+    // in-bounds value and are zeroed below).
+    constexpr int kRowTiles = (kScoreSteps + 3) / 4;
+    const uint32_t jr = tl.j & 3u, jg = tl.j >> 2;
+    const uint32_t row_f = 7u * jg + jr, row_t = 19u + 10u * jg + jr;
+    uint32_t roff[kRowTiles];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) pv[u] = Buf4(tl.gr, tl.off_b2, 16 * u);
+    for (int u = 0; u < kRowTiles; ++u) {
+        const uint32_t d = min(4u * u + jr < uint32_t(kFixedSteps) ? row_f + 4u * u : row_t + 4u * u, 63u);
+        roff[u] = uint32_t(kOffW2 + d * kH1 + 4 * tl.gq) * 4;
+    }
+    const uint32_t bias_f = uint32_t(kOffB2 + 7 * tl.gq) * 4, bias_t = uint32_t(kOffB2 + 19 + 10 * tl.gq) * 4;
+    v4f pv[kRowTiles];
+#pragma unroll
+    for (int u = 0; u < kRowTiles; ++u)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int st = 4 * u + r < kScoreSteps ? 4 * u + r : 0;
+            pv[u][r] = BufF(tl.gr, st < kFixedSteps ? bias_f : bias_t, uint32_t(4 * st));
+        }
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
         float hv[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) hv[r] = fmaxf(h1[t][r], 0.0f);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const v4f w = Buf4(tl.gr, tl.off_w2, uint32_t(4 * u * kH1 + 16 * t) * 4);
+        for (int u = 0; u < kRowTiles; ++u) {
+            const v4f w = Buf4(tl.gr, roff[u], uint32_t(16 * t) * 4);
 #pragma unroll
             for (int r = 0; r < 4; ++r) pv[u] = Mfma(w[r], hv[r], pv[u]);
         }
         __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) pol[4 * u + r] = pv[u][r];
+    for (int s = 0; s < kScoreSteps; ++s) pol[s] = ScoreDim(s, int(tl.gq)) < 0 ? 0.0f : pv[s / 4][s % 4];
 }
 
 // Stages a genome's 145->64 encoder into LDS, transposed to [feature][out].
@@ -192,10 +213,125 @@ __device__ __forceinline__ void StageEncoder(const float* genome, float* s_enc, 
 
 // ================================================================================================
 // Evaluation kernel.
+//
+// One workgroup per genome (blockIdx.y splits long game lists), one wave per tile of 16 games; the
+// waves of a workgroup take their decisions in lockstep.  Per decision each wave runs its own trunk,
+// then all waves score their games against every action together: the output embedding streams
+// through LDS in chunks of kChunkTiles action tiles (double-buffered, loaded by the whole workgroup
+// one chunk ahead), so a genome's embedding is read from HBM/L2 once per workgroup and decision.
+constexpr int kChunkTiles = 2;
+constexpr int kLdsRow = 42;  // floats per tail row in LDS: 38 values, 2 zero pads, 2 unused
+constexpr int kChunkRows = 16 * kChunkTiles;
+constexpr uint32_t kImgChunkWords = uint32_t(kChunkTiles * kImgTileWords);
+constexpr uint32_t kImgChunkUnits = kImgChunkWords / 2;          // 8-byte load units
+constexpr uint32_t kTailChunkUnits = uint32_t(kChunkRows * kTail / 2);
+constexpr uint32_t kTailChunkBytes = uint32_t(kChunkRows * kTail * 4);
+constexpr int kStageUnits = 2;  // register-staged units per thread and region
+static_assert((kTail % 2) == 0 && kImgTileWords % 2 == 0, "8-byte load units");
+
+typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+
+__device__ inline v2u Buf2(Rsrc r, uint32_t voff) {
+    return __builtin_bit_cast(v2u, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
+}
+
+struct ChunkStage {
+    v2u img[kStageUnits];
+    v2u tail[kStageUnits];
+};
+
+// Issues the loads of chunk c (out-of-range bytes read as 0: the descriptors' sizes bound them).
+__device__ __forceinline__ void LoadChunk(ChunkStage& st, Rsrc img_r, Rsrc tail_r, uint32_t c, uint32_t tid,
+                                          uint32_t bd) {
+#pragma unroll
+    for (int m = 0; m < kStageUnits; ++m) {
+        const uint32_t u = tid + uint32_t(m) * bd;
+        if (u < kImgChunkUnits) st.img[m] = Buf2(img_r, c * (4 * kImgChunkWords) + 8 * u);
+        if (u < kTailChunkUnits) st.tail[m] = Buf2(tail_r, c * kTailChunkBytes + 8 * u);
+    }
+}
+
+__device__ __forceinline__ void StoreTailUnit(float* tail_lds, uint32_t u, v2u v) {
+    const uint32_t row = u / uint32_t(kTail / 2), q = u - row * uint32_t(kTail / 2);
+    *reinterpret_cast<v2u*>(tail_lds + row * kLdsRow + 2 * q) = v;
+}
+
+// Writes chunk c into one LDS buffer; units beyond the staged ones (workgroups of fewer than
+// kImgChunkUnits / kStageUnits threads) are loaded and written here directly.
+__device__ __forceinline__ void WriteChunk(const ChunkStage& st, uint32_t* img_lds, float* tail_lds, Rsrc img_r,
+                                           Rsrc tail_r, uint32_t c, uint32_t tid, uint32_t bd) {
+#pragma unroll
+    for (int m = 0; m < kStageUnits; ++m) {
+        const uint32_t u = tid + uint32_t(m) * bd;
+        if (u < kImgChunkUnits) *reinterpret_cast<v2u*>(img_lds + 2 * u) = st.img[m];
+        if (u < kTailChunkUnits) StoreTailUnit(tail_lds, u, st.tail[m]);
+    }
+    for (uint32_t u = tid + kStageUnits * bd; u < kImgChunkUnits; u += bd)
+        *reinterpret_cast<v2u*>(img_lds + 2 * u) = Buf2(img_r, c * (4 * kImgChunkWords) + 8 * u);
+    for (uint32_t u = tid + kStageUnits * bd; u < kTailChunkUnits; u += bd)
+        StoreTailUnit(tail_lds, u, Buf2(tail_r, c * kTailChunkBytes + 8 * u));
+}
+
+// Masked argmax state of one lane: candidates in increasing action order, the first unmasked one
+// taken whatever its score, later ones only when strictly greater (evaluation_ops.cuh:429-445).
+struct ArgmaxLane {
+    float best, second;
+    uint32_t besta;
+};
+
+// Scores one action tile (16 actions) against the lane group's policy slots and folds the 4 rows
+// this lane owns (actions abase..abase+3) into its argmax.
 template <bool kTrace>
-__global__ __launch_bounds__(512) void EvaluateKernel(EvalArgs p) {
+__device__ __forceinline__ void ScoreTile(const uint32_t* img_tile, const float* tail_tile, uint32_t lane, uint32_t j,
+                                          uint32_t gq, const float (&pol)[kScoreSteps], uint32_t abase, uint32_t A,
+                                          const uint32_t (&hist)[5], ArgmaxLane& am) {
+    const float* fx = reinterpret_cast<const float*>(img_tile);
+    const v4f f0 = *reinterpret_cast<const v4f*>(fx + 4 * lane);
+    const v4f f1 = *reinterpret_cast<const v4f*>(fx + 256 + 4 * lane);
+    const v4u cn = *reinterpret_cast<const v4u*>(img_tile + 512 + 4 * gq);
+    const float* tr = tail_tile + j * kLdsRow + kTailSteps * gq;
+    float ev[kScoreSteps];
+    ev[0] = f0[0];
+    ev[1] = f0[1];
+    ev[2] = f0[2];
+    ev[3] = f0[3];
+    ev[4] = f1[0];
+    ev[5] = f1[1];
+    ev[6] = f1[2];
+#pragma unroll
+    for (int q = 0; q < kTailSteps / 2; ++q) {
+        const float2 v = *reinterpret_cast<const float2*>(tr + 2 * q);
+        ev[kFixedSteps + 2 * q] = v.x;
+        ev[kFixedSteps + 2 * q + 1] = v.y;
+    }
+    v4f sc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < kScoreSteps; ++s) sc = Mfma(ev[s], pol[s], sc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t a = abase + uint32_t(r);
+        const float v = sc[r];
+        const uint32_t c = cn[r];
+        const bool masked = (c == hist[0]) | (c == hist[1]) | (c == hist[2]) | (c == hist[3]) | (c == hist[4]);
+        const bool ok = (a < A) & !masked;
+        const bool first = am.besta == kInvalidSlot;
+        const bool better = ok & (first | (v > am.best));
+        if (kTrace) {
+            const float demoted = first ? am.second : fmaxf(am.second, am.best);
+            am.second = better ? demoted : ((ok & (v > am.second)) ? v : am.second);
+        }
+        am.best = better ? v : am.best;
+        am.besta = better ? a : am.besta;
+    }
+}
+
+template <bool kTrace>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3, 3))) void EvaluateKernel(EvalArgs p) {
     __shared__ float s_enc[kTurnFeatures * kEncOut];  // encoder weights, transposed [feature][out]
     __shared__ float s_encb[kEncOut];
+    __shared__ __align__(16) uint32_t s_img[2][kImgChunkWords];
+    __shared__ __align__(16) float s_tail[2][kChunkRows * kLdsRow];
 
     const uint32_t n = blockIdx.x;
     const uint32_t slot = p.slot_indices[n];
@@ -204,40 +340,54 @@ __global__ __launch_bounds__(512) void EvaluateKernel(EvalArgs p) {
         return;
     }
     const float* genome = reinterpret_cast<const float*>(p.slab + uint64_t(slot) * p.stride);
+    const uint32_t tid = threadIdx.x, bd = blockDim.x;
 
+    // the two pad floats of every LDS tail row stay 0 (the chunk writes never touch them)
+    for (uint32_t idx = tid; idx < uint32_t(2 * kChunkRows); idx += bd) {
+        float* row = &s_tail[idx / kChunkRows][(idx % kChunkRows) * kLdsRow];
+        row[kTail] = 0.0f;
+        row[kTail + 1] = 0.0f;
+    }
     StageEncoder(genome, s_enc, s_encb);
 
     const uint32_t W = p.cell_offsets[n + 1] - p.cell_offsets[n];
     if (W == 0) {
-        if (threadIdx.x == 0) SetStatus(p.status, 5);  // kInvalidTrainingShard
+        if (tid == 0) SetStatus(p.status, 5);  // kInvalidTrainingShard
         return;
     }
-    if (threadIdx.x == 0 && blockIdx.y == 0) p.local_w[n] = W;
+    if (tid == 0 && blockIdx.y == 0) p.local_w[n] = W;
     const uint32_t* local = p.secret_ids + p.cell_offsets[n];
     const uint32_t tiles_fresh = (W + 15) / 16;
     const uint32_t tiles = tiles_fresh + (2 * W + 15) / 16;
 
-    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t lane = tid & 63u;
     const uint32_t j = lane & 15u;   // A-operand row / B-operand column within a 16-tile
     const uint32_t gq = lane >> 4;   // k group (0..3)
-    const uint32_t waves = blockDim.x >> 6;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t waves = bd >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
 
     const TrunkLane tl = MakeTrunkLane(genome, j, gq);
-    const float* tails = genome + kPolicyParams;
     const uint32_t A = p.action_count;
     const uint32_t a_tiles = (A + 15) / 16;
+    const uint32_t chunks = (a_tiles + kChunkTiles - 1) / kChunkTiles;
+    const Rsrc img_r = MakeRsrc(p.img, p.img_bytes);
+    const Rsrc tail_r = MakeRsrc(genome + kPolicyParams, uint32_t(p.stride - kPolicyParams * 4));
 
-    for (uint32_t tile = blockIdx.y * waves + wave; tile < tiles; tile += gridDim.y * waves) {
+    const uint32_t per_round = gridDim.y * waves;
+    const uint32_t first_tile = blockIdx.y * waves;
+    const uint32_t rounds = tiles > first_tile ? (tiles - first_tile + per_round - 1) / per_round : 0u;
+    for (uint32_t round = 0; round < rounds; ++round) {
+        const uint32_t tile = first_tile + round * per_round + wave;
         const bool fresh = tile < tiles_fresh;
         const uint32_t game = fresh ? tile * 16 + j : (tile - tiles_fresh) * 16 + j;
-        const bool valid = fresh ? game < W : game < 2 * W;
+        const bool valid = tile < tiles && (fresh ? game < W : game < 2 * W);
         const uint32_t game_slot = fresh ? game : W + game;
 
         // ---- episode initialisation (evaluation_ops.cuh:171-237)
-        // history: hist[0..tc) are the guessed words, most recent first; (last, prev) are the two
-        // most recent turns with their feedback, which is all the turn-slot fold ever needs
-        uint32_t hist[5] = {0, 0, 0, 0, 0};
+        // hist[0..tc) are the canonical indices of the guessed words, most recent first (unused
+        // entries never match); (last, prev) are the two most recent turns with their feedback,
+        // which is all the turn-slot fold ever needs
+        uint32_t hist[5] = {kInvalidSlot, kInvalidSlot, kInvalidSlot, kInvalidSlot, kInvalidSlot};
         uint32_t last_code = 0, last_fb = 0, prev_code = 0, prev_fb = 0;
         GameTrace* tr = nullptr;
         uint32_t sol = 0;
@@ -252,8 +402,9 @@ __global__ __launch_bounds__(512) void EvaluateKernel(EvalArgs p) {
             bool ok = true;
             if (variant != 0) {
                 const uint32_t first = variant == 1 ? entry + 1 : entry + 3;
-                const uint32_t c1 = p.codes[local[first % W]];
-                const uint32_t c2 = p.codes[local[(first + 1) % W]];
+                const uint32_t w1 = local[first % W], w2 = local[(first + 1) % W];
+                const uint32_t c1 = p.codes[w1];
+                const uint32_t c2 = p.codes[w2];
                 tc = 1;
                 won = c1 == sol;
                 if (won) {
@@ -263,8 +414,8 @@ __global__ __launch_bounds__(512) void EvaluateKernel(EvalArgs p) {
                     prev_fb = FeedbackRegs(c1, sol);
                     last_code = c2;
                     last_fb = FeedbackRegs(c2, sol);
-                    hist[0] = c2;
-                    hist[1] = c1;
+                    hist[0] = p.canon[w2];
+                    hist[1] = p.canon[w1];
                     tc = 2;
                     won = c2 == sol;
                 }
@@ -288,87 +439,76 @@ __global__ __launch_bounds__(512) void EvaluateKernel(EvalArgs p) {
         v4f acc[16];  // layer-0 pre-activations, neuron tile t rows 4*gq+r, game column j
         int folded = 0;
         for (int step = 0; step < kMaxTurns; ++step) {
-            if (!__any(active)) break;
-            const int ttc = base_tc + step;  // turn count of every active game in this tile
-            if (step == 0 || (fresh && step == 1)) {
-                InitLayer0(acc, tl);
-                if (fresh && step == 0) AddVirgin(acc, tl, true);
-                folded = 0;
+            if (!__syncthreads_or(active ? 1 : 0)) break;  // the whole workgroup leaves together
+            const bool wave_active = __any(active ? 1 : 0) != 0;
+            float pol[kScoreSteps];
+            if (wave_active) {
+                const int ttc = base_tc + step;  // turn count of every active game in this tile
+                if (step == 0 || (fresh && step == 1)) {
+                    InitLayer0(acc, tl);
+                    if (fresh && step == 0) AddVirgin(acc, tl, true);
+                    folded = 0;
+                }
+                // ---- fold newly occupied turn slots: acc += W0[:, 1+64*slot ...] . enc(turn slot)
+                for (int ts = folded; ts < ttc; ++ts) {
+                    const bool is_last = ts == ttc - 1;
+                    float e[16];
+                    EncodeTurnLds(s_enc, s_encb, is_last ? last_code : prev_code, is_last ? last_fb : prev_fb, gq,
+                                  e);
+                    FoldSlot(acc, tl, ts, e);
+                }
+                folded = ttc;
+                TrunkToPolicy(acc, tl, pol);
+            } else {
+#pragma unroll
+                for (int s = 0; s < kScoreSteps; ++s) pol[s] = 0.0f;
             }
-            // ---- fold newly occupied turn slots: acc += W0[:, 1+64*slot ...] . enc(turn slot)
-            for (int ts = folded; ts < ttc; ++ts) {
-                const bool is_last = ts == ttc - 1;
-                float e[16];
-                EncodeTurnLds(s_enc, s_encb, is_last ? last_code : prev_code, is_last ? last_fb : prev_fb, gq, e);
-                FoldSlot(acc, tl, ts, e);
-            }
-            folded = ttc;
-            float pol[16];
-            TrunkToPolicy(acc, tl, pol);
+            ChunkStage st;
+            LoadChunk(st, img_r, tail_r, 0, tid, bd);
+            WriteChunk(st, s_img[0], s_tail[0], img_r, tail_r, 0, tid, bd);
+            __syncthreads();
 
-            // ---- scoring: S = E . P with E rows = [26 fixed | 38 tail] per action; masked argmax
-            float best = -__builtin_inff();
-            float second = -__builtin_inff();
-            uint32_t besta = kInvalidSlot;
-            for (uint32_t at = 0; at < a_tiles; ++at) {
-                const uint32_t arow = at * 16 + j;
-                const uint32_t ac = arow < A ? arow : A - 1;
-                float ev[16];
+            // ---- scoring: S = E . P over all actions; masked argmax
+            ArgmaxLane am = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot};
+            for (uint32_t c = 0; c < chunks; ++c) {
+                const bool more = c + 1 < chunks;
+                if (more) LoadChunk(st, img_r, tail_r, c + 1, tid, bd);
+                if (wave_active) {
+                    const uint32_t* img = s_img[c & 1];
+                    const float* tails = s_tail[c & 1];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const uint32_t kk = 16 * gq + 2 * q;
-                    const float* src = kk < uint32_t(kFixed) ? p.fixed + uint64_t(ac) * 32 + kk
-                                                             : tails + uint64_t(ac) * kTail + (kk - kFixed);
-                    const float2 v = *reinterpret_cast<const float2*>(src);
-                    ev[2 * q] = v.x;
-                    ev[2 * q + 1] = v.y;
+                    for (int k = 0; k < kChunkTiles; ++k)
+                        ScoreTile<kTrace>(img + k * kImgTileWords, tails + k * 16 * kLdsRow, lane, j, gq, pol,
+                                          (c * kChunkTiles + k) * 16 + 4 * gq, A, hist, am);
                 }
-                v4f sc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-                for (int s = 0; s < 16; ++s) sc = Mfma(ev[s], pol[s], sc);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const uint32_t a = at * 16 + 4 * gq + r;
-                    if (a >= A) continue;
-                    const float v = sc[r];
-                    const bool better = v > best || (v == best && a < besta);
-                    if (better || (kTrace && v > second)) {
-                        const uint32_t ca = p.codes[a];
-                        bool masked = false;
-#pragma unroll
-                        for (int t = 0; t < 5; ++t) masked |= (t < tc) && hist[t] == ca;
-                        if (!masked) {
-                            if (better) {
-                                second = fmaxf(second, best);
-                                best = v;
-                                besta = a;
-                            } else {
-                                second = v;
-                            }
-                        }
-                    }
-                }
+                if (more) WriteChunk(st, s_img[(c + 1) & 1], s_tail[(c + 1) & 1], img_r, tail_r, c + 1, tid, bd);
+                __syncthreads();
             }
-            // reduce over the four lane groups that share game column j
+            float best = am.best, second = am.second;
+            uint32_t besta = am.besta;
+            // reduce over the four lane groups that share game column j (lower index wins ties)
 #pragma unroll
             for (int off = 16; off <= 32; off <<= 1) {
                 const float ob = __shfl_xor(best, off);
                 const uint32_t oa = __shfl_xor(besta, off);
                 const float os = __shfl_xor(second, off);
-                if (ob > best || (ob == best && oa < besta)) {
-                    second = fmaxf(fmaxf(second, best), os);
+                const bool take = oa != kInvalidSlot && (besta == kInvalidSlot || ob > best || (ob == best && oa < besta));
+                if (take) {
+                    second = besta == kInvalidSlot ? fmaxf(second, os) : fmaxf(fmaxf(second, best), os);
                     best = ob;
                     besta = oa;
                 } else {
-                    second = fmaxf(fmaxf(second, ob), os);
+                    second = oa == kInvalidSlot ? fmaxf(second, os) : fmaxf(fmaxf(second, ob), os);
                 }
             }
 
             if (kTrace && active && p.policy_trace != nullptr) {
-                float* dst = p.policy_trace +
-                             ((p.game_offsets[n] + game_slot) * uint64_t(kMaxTurns) + decisions) * kPol + 16 * gq;
+                float* dst = p.policy_trace + ((p.game_offsets[n] + game_slot) * uint64_t(kMaxTurns) + decisions) * kPol;
 #pragma unroll
-                for (int s = 0; s < 16; ++s) dst[s] = pol[s];
+                for (int s = 0; s < kScoreSteps; ++s) {
+                    const int d = ScoreDim(s, int(gq));
+                    if (d >= 0) dst[d] = pol[s];
+                }
             }
             // ---- apply the guess (wordle_grid.hpp:308-316)
             if (active) {
@@ -379,7 +519,7 @@ __global__ __launch_bounds__(512) void EvaluateKernel(EvalArgs p) {
                     const uint32_t code = p.codes[besta];
 #pragma unroll
                     for (int t = 4; t > 0; --t) hist[t] = hist[t - 1];
-                    hist[0] = code;
+                    hist[0] = p.canon[besta];
                     prev_code = last_code;
                     prev_fb = last_fb;
                     last_code = code;
@@ -415,7 +555,7 @@ __global__ __launch_bounds__(512) void EvaluateKernel(EvalArgs p) {
             dec += __shfl_xor(dec, off);
             enc += __shfl_xor(enc, off);
         }
-        if (lane == 0) {
+        if (lane == 0 && tile < tiles) {
             atomicAdd(p.raw_fitness + n, score);
             atomicAdd(p.counters + 0, (unsigned long long)dec);
             atomicAdd(p.counters + 1, (unsigned long long)enc);
@@ -965,16 +1105,16 @@ __global__ __launch_bounds__(512) void InjectKernel(InjectArgs p) {
             }
             FoldSlot(acc, tl, ts, e);
         }
-        float pol[16];
+        float pol[kScoreSteps];
         TrunkToPolicy(acc, tl, pol);
         // mean over the word's 3 grids in grid order: ((0 + p0) + p1) + p2, times 1/3
         float* row = genome + kPolicyParams + uint64_t(p.parent_action_count + word) * kTail;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
+        for (int q = kFixedSteps; q < kScoreSteps; ++q) {
             const float p1 = __shfl_down(pol[q], 1);
             const float p2 = __shfl_down(pol[q], 2);
             const float mean = (((0.0f + pol[q]) + p1) + p2) * (1.0f / 3.0f);
-            const int dim = int(16 * gq) + q;
+            const int dim = ScoreDim(q, int(gq));
             if (valid && grid == 0 && dim >= kFixed) row[dim - kFixed] = mean;
         }
     }

@@ -25,7 +25,9 @@ struct EvalArgs {
     const uint32_t* cell_offsets;  // CSR of local secrets per cell (= individual)
     const uint32_t* secret_ids;
     const uint32_t* codes;         // packed catalog words
-    const float* fixed;            // [catalog][32] fixed word features (26 used)
+    const uint32_t* canon;         // [catalog] lowest index holding the same word
+    const uint32_t* img;           // [catalog tiles][kImgTileWords] scoring image
+    uint32_t img_bytes;
     float* raw_fitness;            // [N] summed episode scores
     uint32_t* local_w;             // [N]
     unsigned long long* counters;  // [0] decisions, [1] encoded turns, [2] games

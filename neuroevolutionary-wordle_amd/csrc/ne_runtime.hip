@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/ne_runtime.h"
@@ -77,7 +78,9 @@ struct ne_runtime {
 
     // catalog
     uint32_t* codes = nullptr;
-    float* fixed = nullptr;
+    uint32_t* canon = nullptr;  // lowest catalog index holding the same word
+    uint32_t* img = nullptr;    // scoring image, kImgTileWords per tile of 16 words
+    uint64_t img_bytes = 0;
     uint64_t catalog_count = 0;
     std::vector<uint8_t> h_catalog;
 
@@ -382,7 +385,9 @@ static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard
     a.cell_offsets = rt->d_offsets;
     a.secret_ids = rt->d_secrets;
     a.codes = rt->codes;
-    a.fixed = rt->fixed;
+    a.canon = rt->canon;
+    a.img = rt->img;
+    a.img_bytes = uint32_t(rt->img_bytes);
     a.raw_fitness = rt->raw;
     a.local_w = rt->local_w;
     a.counters = rt->counters;
@@ -770,7 +775,8 @@ void ne_destroy(ne_runtime* rt) {
     Free(rt->d_offsets);
     Free(rt->d_secrets);
     Free(rt->codes);
-    Free(rt->fixed);
+    Free(rt->canon);
+    Free(rt->img);
     Free(rt->owner);
     Free(rt->move_src);
     Free(rt->d_survivors);
@@ -788,8 +794,9 @@ void ne_destroy(ne_runtime* rt) {
 
 int ne_upload_catalog(ne_runtime* rt, const uint8_t* letters, uint64_t word_count) {
     if (!rt || !letters || word_count == 0 || word_count > kCatalogCapacity) return Fail(rt, NE_INVALID_ARGUMENT);
-    std::vector<uint32_t> codes(word_count);
-    std::vector<float> fixed(word_count * 32, 0.0f);
+    std::vector<uint32_t> codes(word_count), canon(word_count);
+    std::vector<float> fixed(word_count * kFixed, 0.0f);
+    std::unordered_map<uint32_t, uint32_t> first_index;
     for (uint64_t a = 0; a < word_count; ++a) {
         uint32_t code = 0;
         int counts[kAlpha] = {0};
@@ -800,14 +807,34 @@ int ne_upload_catalog(ne_runtime* rt, const uint8_t* letters, uint64_t word_coun
             ++counts[l];
         }
         codes[a] = code;
+        canon[a] = first_index.emplace(code, uint32_t(a)).first->second;
         // MaterializeFixedWordFeatures (output_embedding.hpp:179-205): count, or -1 when absent
-        for (int l = 0; l < kAlpha; ++l) fixed[a * 32 + l] = counts[l] == 0 ? -1.0f : float(counts[l]);
+        for (int l = 0; l < kAlpha; ++l) fixed[a * kFixed + l] = counts[l] == 0 ? -1.0f : float(counts[l]);
+    }
+    // scoring image: lane i + 16g of a tile holds steps 0..6 of action row i for lane group g
+    const uint64_t tiles = (word_count + 15) / 16;
+    std::vector<uint32_t> img(tiles * kImgTileWords, 0u);
+    for (uint64_t t = 0; t < tiles; ++t) {
+        uint32_t* tile = img.data() + t * kImgTileWords;
+        for (int lane = 0; lane < 64; ++lane) {
+            const uint64_t a = t * 16 + (lane & 15);
+            if (a >= word_count) continue;
+            for (int s = 0; s < kFixedSteps; ++s) {
+                const int d = ScoreDim(s, lane >> 4);
+                const float v = d < 0 ? 0.0f : fixed[a * kFixed + d];
+                std::memcpy(&tile[(s / 4) * 256 + 4 * lane + (s % 4)], &v, 4);
+            }
+        }
+        for (int i = 0; i < 16; ++i) tile[512 + i] = t * 16 + i < word_count ? canon[t * 16 + i] : kInvalidSlot;
     }
     Free(rt->codes);
-    Free(rt->fixed);
-    if (!Alloc(&rt->codes, word_count) || !Alloc(&rt->fixed, word_count * 32) ||
+    Free(rt->canon);
+    Free(rt->img);
+    rt->img_bytes = tiles * kImgTileBytes;
+    if (!Alloc(&rt->codes, word_count) || !Alloc(&rt->canon, word_count) || !Alloc(&rt->img, img.size()) ||
         !Ok(hipMemcpy(rt->codes, codes.data(), word_count * 4, hipMemcpyHostToDevice)) ||
-        !Ok(hipMemcpy(rt->fixed, fixed.data(), word_count * 32 * 4, hipMemcpyHostToDevice)))
+        !Ok(hipMemcpy(rt->canon, canon.data(), word_count * 4, hipMemcpyHostToDevice)) ||
+        !Ok(hipMemcpy(rt->img, img.data(), img.size() * 4, hipMemcpyHostToDevice)))
         return Fail(rt, NE_DEVICE_FAILURE);
     rt->catalog_count = word_count;
     rt->h_catalog.assign(letters, letters + 5 * word_count);

@@ -71,9 +71,9 @@ def ragged_csr(cells, secret_pool, low=32, high=1024, seed=RAGGED_SEED):
 
 WORKLOADS = {
     # name: grid side, actions, secrets, block side
-    "c1_16x16_a512": dict(side=16, actions=512, secrets=256, block=4),
-    "c2_64x64_a4739": dict(side=64, actions=4739, secrets=2048, block=8),
-    "c5_128x128_a4739": dict(side=128, actions=4739, secrets=8192, block=8),
+    "grid16_a512": dict(side=16, actions=512, secrets=256, block=4),
+    "grid64_a4739": dict(side=64, actions=4739, secrets=2048, block=8),
+    "grid128_a4739": dict(side=128, actions=4739, secrets=8192, block=8),
 }
 
 

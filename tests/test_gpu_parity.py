@@ -27,16 +27,16 @@ def _runtime(pkg, side, A, catalog, offsets, ids, seed=9, var=1.0, tail=0.1, sla
 
 
 @pytest.fixture(scope="module")
-def c1(pkg):
+def grid16(pkg):
     """The 16x16 / A=512 workload (256 genomes, 16 secrets per cell)."""
-    wl = pkg.synth.workload("c1_16x16_a512")
+    wl = pkg.synth.workload("grid16_a512")
     rt = _runtime(pkg, 16, 512, wl["catalog"], wl["offsets"], wl["ids"])
     yield wl, rt
     rt.close()
 
 
-def test_bootstrap_bit_exact(pkg, orc, c1):
-    wl, rt = c1
+def test_bootstrap_bit_exact(pkg, orc, grid16):
+    wl, rt = grid16
     slots, _, _, _ = rt.generation(0)
     assert list(slots) == list(range(256)), "generation 0 takes slots 0..N-1 in order"
     stds = orc.layer_stds(1.0)
@@ -48,36 +48,40 @@ def test_bootstrap_bit_exact(pkg, orc, c1):
         assert not gpu[n_vals:].any()
 
 
-def test_evaluation_parity_c1(pkg, orc, c1):
-    wl, rt = c1
+def _check_evaluation(pkg, orc, rt, catalog, offsets, ids, A, n_genomes, min_same=0.99):
+    """Evaluates generation 0 with traces and checks it against the oracle (see the module bar)."""
     rt.set_trace(pkg.runtime.TRACE_GAMES | pkg.runtime.TRACE_POLICY)
-    counts = pkg.runtime.word_counts(512, 512)
+    counts = pkg.runtime.word_counts(A, A)
     rt.evaluate(counts)
     slots, fit, evals, has = rt.generation(0)
     assert has.all() and (evals == 1).all()
-    games = 3 * int(wl["offsets"][-1])
+    games = 3 * int(offsets[-1])
     gtr = rt.download_trace(pkg.runtime.TRACE_GAMES, games)
     ptr_ = rt.download_trace(pkg.runtime.TRACE_POLICY, games)
     genomes = [rt.download_slot(s) for s in slots]
-    ref = orc.evaluate(genomes, 0, 512, 512, wl["catalog"], wl["offsets"], wl["ids"], mode=0, threads=8,
-                       trace=True)
+    ref = orc.evaluate(genomes, 0, A, A, catalog, offsets, ids, mode=0, threads=8, trace=True)
     assert ref["status"] == 0
     rtr = ref["traces"]
-    assert (gtr["decisions"] > 0).all()
     assert int(rt.stage_times().decisions) == int(gtr["decisions"].sum())
-
-    offsets = wl["offsets"]
-    max_err = 0.0
-    checked = agree = near = 0
-    genome_clean = np.ones(256, bool)
-    for n in range(256):
+    for n in range(n_genomes):  # fresh games always decide; prefilled ones may win on the prefill (W = 2, 4)
         W = int(offsets[n + 1] - offsets[n])
-        local = wl["ids"][offsets[n]:offsets[n + 1]]
+        assert (gtr["decisions"][3 * int(offsets[n]):3 * int(offsets[n]) + W] > 0).all()
+    words = np.asarray(catalog).reshape(-1, 5)
+
+    max_err = 0.0
+    checked = near = 0
+    genome_clean = np.ones(n_genomes, bool)
+    for n in range(n_genomes):
+        W = int(offsets[n + 1] - offsets[n])
+        local = ids[offsets[n]:offsets[n + 1]]
         base = 3 * int(offsets[n])
         for slot in range(3 * W):
             g = gtr[base + slot]
             d = int(g["decisions"])
-            rep = orc.replay_game(genomes[n], 512, wl["catalog"], local, slot, g["guesses"][:d])
+            # a guessed word (any copy of it) is never guessed again
+            played = [bytes(words[a]) for a in g["guesses"][:d]]
+            assert len(set(played)) == d, (n, slot, g["guesses"][:d])
+            rep = orc.replay_game(genomes[n], A, catalog, local, slot, g["guesses"][:d])
             assert rep["status"] == 0
             for k in range(d):
                 p64 = rep["policy"][k]
@@ -87,7 +91,6 @@ def test_evaluation_parity_c1(pkg, orc, c1):
                 margin = rep["best"][k] - rep["second"][k]
                 checked += 1
                 if margin > MARGIN_TOL * max(1.0, abs(rep["best"][k])):
-                    agree += int(rep["argmax"][k] == g["guesses"][k])
                     assert rep["argmax"][k] == g["guesses"][k], (n, slot, k, margin)
                 else:
                     near += 1
@@ -99,12 +102,38 @@ def test_evaluation_parity_c1(pkg, orc, c1):
     assert np.array_equal(fit[genome_clean], ref["fitness"][genome_clean])
     # and the GPU's games equal the oracle's fp32 games wherever the oracle's own margins are clear
     same = np.mean(gtr["guesses"] == rtr["guesses"])
-    assert same >= 0.99, same
+    assert same >= min_same, same
     print(f"decisions {checked}, near-ties {near}, max normwise fwd err {max_err:.2e}, guess agreement {same:.5f}")
 
 
-def test_summary_matches_oracle(pkg, orc, c1):
-    wl, rt = c1
+def test_evaluation_parity_grid16(pkg, orc, grid16):
+    wl, rt = grid16
+    _check_evaluation(pkg, orc, rt, wl["catalog"], wl["offsets"], wl["ids"], 512, 256)
+
+
+def test_evaluation_parity_duplicate_words_ragged_shards(pkg, orc):
+    """Duplicate catalog words (a guess masks every copy), ragged cells of 1..70 secrets (workgroups of
+    2 waves up to split workgroups of 8 waves over 14 tiles) and 23 action tiles (a half-used last chunk)."""
+    A, side = 365, 4
+    rng = np.random.default_rng(0xD0B1E)
+    catalog = pkg.synth.make_catalog(A, seed=0xD0B1E).reshape(-1, 5).copy()
+    for k in range(40, A, 9):  # every 9th word from 40 on repeats an earlier word
+        catalog[k] = catalog[int(rng.integers(0, k))]
+    # cells of 1 or 3 secrets make a prefilled episode repeat its solution (GuessAppendFailed)
+    widths = np.array([2, 4, 5, 16, 17, 33, 70, 6, 9, 48, 2, 64, 7, 31, 8, 10])
+    offsets = np.concatenate([[0], np.cumsum(widths)]).astype(np.uint32)
+    # secrets: distinct words within a cell, so drawn from first occurrences only
+    firsts = np.array(sorted({bytes(w): i for i, w in reversed(list(enumerate(catalog)))}.values()))
+    ids = np.concatenate([rng.choice(firsts, size=w, replace=False) for w in widths]).astype(np.uint32)
+    rt = _runtime(pkg, side, A, catalog, offsets, ids, seed=21)
+    try:
+        _check_evaluation(pkg, orc, rt, catalog, offsets, ids, A, side * side, min_same=0.98)
+    finally:
+        rt.close()
+
+
+def test_summary_matches_oracle(pkg, orc, grid16):
+    wl, rt = grid16
     rt.evaluate(pkg.runtime.word_counts(512, 512))
     _, fit, _, _ = rt.generation(0)
     s = rt.summary()
@@ -112,8 +141,8 @@ def test_summary_matches_oracle(pkg, orc, c1):
     assert s.best_fitness == b and s.average_fitness == m and s.best_index == i
 
 
-def test_plan_bit_exact(pkg, orc, c1):
-    wl, rt = c1
+def test_plan_bit_exact(pkg, orc, grid16):
+    wl, rt = grid16
     counts = pkg.runtime.word_counts(512, 512)
     rt.evaluate(counts)
     _, fit, _, has = rt.generation(0)
