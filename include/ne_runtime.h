@@ -146,7 +146,7 @@ typedef struct ne_stage_times {
     float repack_ms;                      /* widening repack */
     float inject_ms;                      /* output-embedding injection */
     float bookkeeping_ms;                 /* refcount, GC, releases */
-    float reserved0;
+    float prepare_ms;                     /* per-evaluation encoder transpose */
     uint64_t decisions;                   /* policy decisions taken by the last evaluation */
     uint64_t encoded_turns;               /* turns run through the encoder */
     uint64_t games;                       /* episodes played */

@@ -125,6 +125,38 @@ __device__ __forceinline__ void EncodeTurnLds(const float* s_enc, const float* s
     }
 }
 
+// The same encoder from the transposed HBM copy of the genome's encoder ([feature][out], written
+// by TransposeEncoderKernel once per evaluation): the bias from the genome, then the 10 active
+// columns in increasing feature order, 128-bit loads through L2.
+__device__ __forceinline__ void EncodeTurnGlobal(Rsrc enc_t, const TrunkLane& tl, uint32_t code, uint32_t fb,
+                                                 float (&e)[16]) {
+    uint32_t koff[10];
+#pragma unroll
+    for (int f = 0; f < 10; ++f) {
+        const uint32_t k = f < 5 ? uint32_t(f) * 26u + WordLetter(code, f)
+                                 : 130u + uint32_t(f - 5) * 3u + ((fb >> (2 * (f - 5))) & 3u);
+        koff[f] = (k * kEncOut + 16 * tl.gq) * 4;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        v4f acc = Buf4(tl.gr, uint32_t(kOffEncB + 16 * tl.gq) * 4, uint32_t(16 * q));
+        v4f col[10];
+#pragma unroll
+        for (int f = 0; f < 10; ++f) col[f] = Buf4(enc_t, koff[f], uint32_t(16 * q));
+#pragma unroll
+        for (int f = 0; f < 10; ++f) {
+            acc[0] += col[f][0];
+            acc[1] += col[f][1];
+            acc[2] += col[f][2];
+            acc[3] += col[f][3];
+        }
+        e[4 * q] = acc[0];
+        e[4 * q + 1] = acc[1];
+        e[4 * q + 2] = acc[2];
+        e[4 * q + 3] = acc[3];
+    }
+}
+
 // acc += W0[:, 1+64*slot .. 1+64*slot+63] . e
 __device__ __forceinline__ void FoldSlot(v4f (&acc)[16], const TrunkLane& tl, int slot, const float (&e)[16]) {
     const uint32_t soff_slot = uint32_t(64 * slot) * 4;
@@ -211,6 +243,19 @@ __device__ __forceinline__ void StageEncoder(const float* genome, float* s_enc, 
     __syncthreads();
 }
 
+// Writes each genome's 145->64 encoder transposed to [feature][out] (one workgroup per genome).
+__global__ void TransposeEncoderKernel(const uint8_t* slab, uint64_t stride, const uint32_t* slot_indices,
+                                       uint32_t slot_count, float* enc_t) {
+    const uint32_t slot = slot_indices[blockIdx.x];
+    if (slot >= slot_count) return;  // reported by the evaluation kernel
+    const float* genome = reinterpret_cast<const float*>(slab + uint64_t(slot) * stride);
+    float* dst = enc_t + uint64_t(blockIdx.x) * (kTurnFeatures * kEncOut);
+    for (uint32_t idx = threadIdx.x; idx < uint32_t(kEncOut * kTurnFeatures); idx += blockDim.x) {
+        const uint32_t k = idx / kEncOut, o = idx - k * kEncOut;
+        dst[idx] = genome[kOffEncW + o * kTurnFeatures + k];
+    }
+}
+
 // ================================================================================================
 // Evaluation kernel.
 //
@@ -219,14 +264,23 @@ __device__ __forceinline__ void StageEncoder(const float* genome, float* s_enc, 
 // then all waves score their games against every action together: the output embedding streams
 // through LDS in chunks of kChunkTiles action tiles (double-buffered, loaded by the whole workgroup
 // one chunk ahead), so a genome's embedding is read from HBM/L2 once per workgroup and decision.
-constexpr int kChunkTiles = 2;
+#ifndef NE_CHUNK_TILES
+#define NE_CHUNK_TILES 1  // action tiles per LDS chunk (build-time tunable)
+#endif
+#ifndef NE_EVAL_WAVES_PER_EU
+#define NE_EVAL_WAVES_PER_EU 3  // register budget: 3 waves/SIMD = four 3-wave workgroups per CU
+#endif
+constexpr int kChunkTiles = NE_CHUNK_TILES;
 constexpr int kLdsRow = 42;  // floats per tail row in LDS: 38 values, 2 zero pads, 2 unused
 constexpr int kChunkRows = 16 * kChunkTiles;
 constexpr uint32_t kImgChunkWords = uint32_t(kChunkTiles * kImgTileWords);
 constexpr uint32_t kImgChunkUnits = kImgChunkWords / 2;          // 8-byte load units
 constexpr uint32_t kTailChunkUnits = uint32_t(kChunkRows * kTail / 2);
 constexpr uint32_t kTailChunkBytes = uint32_t(kChunkRows * kTail * 4);
-constexpr int kStageUnits = 2;  // register-staged units per thread and region
+// register-staged load units per thread, sized for 3-wave workgroups (smaller ones load the rest
+// synchronously)
+constexpr int kStageImg = int((kImgChunkUnits + 191) / 192);
+constexpr int kStageTail = int((kTailChunkUnits + 191) / 192);
 static_assert((kTail % 2) == 0 && kImgTileWords % 2 == 0, "8-byte load units");
 
 typedef uint32_t v2u __attribute__((ext_vector_type(2)));
@@ -237,17 +291,21 @@ __device__ inline v2u Buf2(Rsrc r, uint32_t voff) {
 }
 
 struct ChunkStage {
-    v2u img[kStageUnits];
-    v2u tail[kStageUnits];
+    v2u img[kStageImg];
+    v2u tail[kStageTail];
 };
 
 // Issues the loads of chunk c (out-of-range bytes read as 0: the descriptors' sizes bound them).
 __device__ __forceinline__ void LoadChunk(ChunkStage& st, Rsrc img_r, Rsrc tail_r, uint32_t c, uint32_t tid,
                                           uint32_t bd) {
 #pragma unroll
-    for (int m = 0; m < kStageUnits; ++m) {
+    for (int m = 0; m < kStageImg; ++m) {
         const uint32_t u = tid + uint32_t(m) * bd;
         if (u < kImgChunkUnits) st.img[m] = Buf2(img_r, c * (4 * kImgChunkWords) + 8 * u);
+    }
+#pragma unroll
+    for (int m = 0; m < kStageTail; ++m) {
+        const uint32_t u = tid + uint32_t(m) * bd;
         if (u < kTailChunkUnits) st.tail[m] = Buf2(tail_r, c * kTailChunkBytes + 8 * u);
     }
 }
@@ -258,18 +316,22 @@ __device__ __forceinline__ void StoreTailUnit(float* tail_lds, uint32_t u, v2u v
 }
 
 // Writes chunk c into one LDS buffer; units beyond the staged ones (workgroups of fewer than
-// kImgChunkUnits / kStageUnits threads) are loaded and written here directly.
+// kImgChunkUnits / kStageImg threads) are loaded and written here directly.
 __device__ __forceinline__ void WriteChunk(const ChunkStage& st, uint32_t* img_lds, float* tail_lds, Rsrc img_r,
                                            Rsrc tail_r, uint32_t c, uint32_t tid, uint32_t bd) {
 #pragma unroll
-    for (int m = 0; m < kStageUnits; ++m) {
+    for (int m = 0; m < kStageImg; ++m) {
         const uint32_t u = tid + uint32_t(m) * bd;
         if (u < kImgChunkUnits) *reinterpret_cast<v2u*>(img_lds + 2 * u) = st.img[m];
+    }
+#pragma unroll
+    for (int m = 0; m < kStageTail; ++m) {
+        const uint32_t u = tid + uint32_t(m) * bd;
         if (u < kTailChunkUnits) StoreTailUnit(tail_lds, u, st.tail[m]);
     }
-    for (uint32_t u = tid + kStageUnits * bd; u < kImgChunkUnits; u += bd)
+    for (uint32_t u = tid + kStageImg * bd; u < kImgChunkUnits; u += bd)
         *reinterpret_cast<v2u*>(img_lds + 2 * u) = Buf2(img_r, c * (4 * kImgChunkWords) + 8 * u);
-    for (uint32_t u = tid + kStageUnits * bd; u < kTailChunkUnits; u += bd)
+    for (uint32_t u = tid + kStageTail * bd; u < kTailChunkUnits; u += bd)
         StoreTailUnit(tail_lds, u, Buf2(tail_r, c * kTailChunkBytes + 8 * u));
 }
 
@@ -327,9 +389,7 @@ __device__ __forceinline__ void ScoreTile(const uint32_t* img_tile, const float*
 }
 
 template <bool kTrace>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3, 3))) void EvaluateKernel(EvalArgs p) {
-    __shared__ float s_enc[kTurnFeatures * kEncOut];  // encoder weights, transposed [feature][out]
-    __shared__ float s_encb[kEncOut];
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void EvaluateKernel(EvalArgs p) {
     __shared__ __align__(16) uint32_t s_img[2][kImgChunkWords];
     __shared__ __align__(16) float s_tail[2][kChunkRows * kLdsRow];
 
@@ -348,7 +408,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         row[kTail] = 0.0f;
         row[kTail + 1] = 0.0f;
     }
-    StageEncoder(genome, s_enc, s_encb);
+    __syncthreads();
+    const Rsrc enc_r = MakeRsrc(p.enc_t + uint64_t(n) * (kTurnFeatures * kEncOut), uint32_t(kTurnFeatures * kEncOut * 4));
 
     const uint32_t W = p.cell_offsets[n + 1] - p.cell_offsets[n];
     if (W == 0) {
@@ -453,8 +514,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 for (int ts = folded; ts < ttc; ++ts) {
                     const bool is_last = ts == ttc - 1;
                     float e[16];
-                    EncodeTurnLds(s_enc, s_encb, is_last ? last_code : prev_code, is_last ? last_fb : prev_fb, gq,
-                                  e);
+                    EncodeTurnGlobal(enc_r, tl, is_last ? last_code : prev_code, is_last ? last_fb : prev_fb, e);
                     FoldSlot(acc, tl, ts, e);
                 }
                 folded = ttc;
@@ -1137,6 +1197,11 @@ __global__ __launch_bounds__(512) void InjectKernel(InjectArgs p) {
 
 // ================================================================================================
 // Launch wrappers (host side).
+void LaunchTransposeEncoder(const uint8_t* slab, uint64_t stride, const uint32_t* slots, uint32_t n,
+                            uint32_t slot_count, float* enc_t, hipStream_t s) {
+    hipLaunchKernelGGL(TransposeEncoderKernel, dim3(n), dim3(256), 0, s, slab, stride, slots, slot_count, enc_t);
+}
+
 void LaunchEvaluate(const EvalArgs& a, uint32_t n_genomes, uint32_t waves, uint32_t chunks, bool trace,
                     hipStream_t s) {
     const dim3 grid(n_genomes, chunks);

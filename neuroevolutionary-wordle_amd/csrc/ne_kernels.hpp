@@ -28,6 +28,7 @@ struct EvalArgs {
     const uint32_t* canon;         // [catalog] lowest index holding the same word
     const uint32_t* img;           // [catalog tiles][kImgTileWords] scoring image
     uint32_t img_bytes;
+    const float* enc_t;            // [N][145][64] transposed encoders of the current generation
     float* raw_fitness;            // [N] summed episode scores
     uint32_t* local_w;             // [N]
     unsigned long long* counters;  // [0] decisions, [1] encoded turns, [2] games
@@ -133,6 +134,8 @@ struct InjectArgs {
 };
 
 void LaunchInject(const InjectArgs& a, hipStream_t s);
+void LaunchTransposeEncoder(const uint8_t* slab, uint64_t stride, const uint32_t* slots, uint32_t n,
+                            uint32_t slot_count, float* enc_t, hipStream_t s);
 void LaunchEvaluate(const EvalArgs& a, uint32_t n_genomes, uint32_t waves, uint32_t chunks, bool trace,
                     hipStream_t s);
 void LaunchFinalize(const float* raw, const uint32_t* local_w, uint32_t n, float* fitness, uint32_t* evals,

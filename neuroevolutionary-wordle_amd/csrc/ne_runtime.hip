@@ -65,6 +65,7 @@ struct ne_runtime {
     float rank_weights_exponent = -1.0f;
 
     // evaluation
+    float* enc_t = nullptr;  // [max_gen][145][64] transposed encoders of the generation being evaluated
     float* raw = nullptr;
     uint32_t* local_w = nullptr;
     unsigned long long* counters = nullptr;
@@ -101,7 +102,7 @@ struct ne_runtime {
     uint64_t trace_capacity = 0;
 
     // timing
-    StageEvents ev[8];
+    StageEvents ev[9];
     ne_stage_times times{};
     ne_fitness_summary last_summary{};
     bool have_summary = false;
@@ -395,8 +396,31 @@ static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard
     a.game_trace = trace ? rt->d_trace : nullptr;
     a.policy_trace = (rt->trace_flags & NE_TRACE_POLICY) ? rt->d_policy_trace : nullptr;
     a.game_offsets = trace ? rt->d_game_offsets : nullptr;
-    const uint32_t waves = uint32_t(std::min<uint64_t>(max_tiles, 8));
+    // Waves per workgroup: 6-wave workgroups of this kernel's register footprint fit once per CU, 2-4
+    // wave ones fill a CU (12 waves); pick the size that leaves the fewest idle wave slots over all
+    // genomes (ties: the larger).  A genome's tiles then spread over ceil(max_tiles / waves) workgroups.
+    uint32_t waves = 1;
+    uint64_t best_cost = ~0ull;
+    for (uint32_t w = 4; w >= 1; --w) {
+        uint64_t cost = 0;
+        for (uint64_t c = 0; c < rt->cur_size; ++c) {
+            const uint64_t W = (*offs)[c + 1] - (*offs)[c];
+            const uint64_t t = (W + 15) / 16 + (2 * W + 15) / 16;
+            cost += (t + w - 1) / w * w;
+        }
+        if (cost < best_cost) {
+            best_cost = cost;
+            waves = w;
+        }
+    }
     const uint32_t chunks = uint32_t((max_tiles + waves - 1) / waves);
+    if (rt->enc_t == nullptr && !Alloc(&rt->enc_t, rt->max_gen * uint64_t(kTurnFeatures * kEncOut)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    a.enc_t = rt->enc_t;
+    EventBegin(rt, 8);
+    LaunchTransposeEncoder(rt->slab, rt->stride, rt->cur_slots, uint32_t(rt->cur_size), uint32_t(rt->slot_count),
+                           rt->enc_t, rt->stream);
+    EventEnd(rt, 8);
     EventBegin(rt, 0);
     LaunchEvaluate(a, uint32_t(rt->cur_size), waves, chunks, trace, rt->stream);
     EventEnd(rt, 0);
@@ -417,6 +441,7 @@ static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard
         !Ok(hipMemcpy(&sr, rt->summary, sizeof(sr), hipMemcpyDeviceToHost)))
         return Fail(rt, NE_DEVICE_FAILURE);
     rt->times.evaluate_ms = EventMs(rt, 0);
+    rt->times.prepare_ms = EventMs(rt, 8);
     rt->times.finalize_ms = EventMs(rt, 1);
     rt->times.decisions = counters[0];
     rt->times.games = total_games;
@@ -767,6 +792,7 @@ void ne_destroy(ne_runtime* rt) {
     Free(rt->refcounts);
     Free(rt->xover);
     Free(rt->rank_weights);
+    Free(rt->enc_t);
     Free(rt->raw);
     Free(rt->local_w);
     Free(rt->counters);

@@ -86,7 +86,7 @@ class SlabInfo(C.Structure):
 class StageTimes(C.Structure):
     _fields_ = [("evaluate_ms", C.c_float), ("finalize_ms", C.c_float), ("plan_ms", C.c_float),
                 ("assemble_ms", C.c_float), ("repack_ms", C.c_float), ("inject_ms", C.c_float),
-                ("bookkeeping_ms", C.c_float), ("reserved0", C.c_float), ("decisions", C.c_uint64),
+                ("bookkeeping_ms", C.c_float), ("prepare_ms", C.c_float), ("decisions", C.c_uint64),
                 ("encoded_turns", C.c_uint64), ("games", C.c_uint64), ("children", C.c_uint64)]
 
 
