@@ -248,11 +248,15 @@ __global__ void TransposeEncoderKernel(const uint8_t* slab, uint64_t stride, con
                                        uint32_t slot_count, float* enc_t) {
     const uint32_t slot = slot_indices[blockIdx.x];
     if (slot >= slot_count) return;  // reported by the evaluation kernel
+    __shared__ float s_w[kEncOut * kTurnFeatures];  // [out][feature], as stored in the genome
     const float* genome = reinterpret_cast<const float*>(slab + uint64_t(slot) * stride);
     float* dst = enc_t + uint64_t(blockIdx.x) * (kTurnFeatures * kEncOut);
+    for (uint32_t idx = threadIdx.x; idx < uint32_t(kEncOut * kTurnFeatures); idx += blockDim.x)
+        s_w[idx] = genome[kOffEncW + idx];
+    __syncthreads();
     for (uint32_t idx = threadIdx.x; idx < uint32_t(kEncOut * kTurnFeatures); idx += blockDim.x) {
         const uint32_t k = idx / kEncOut, o = idx - k * kEncOut;
-        dst[idx] = genome[kOffEncW + o * kTurnFeatures + k];
+        dst[idx] = s_w[o * kTurnFeatures + k];  // 145 = 17 (mod 64): consecutive o hit distinct banks
     }
 }
 
@@ -641,23 +645,64 @@ __global__ void FinalizeKernel(const float* raw, const uint32_t* local_w, uint32
     has_fitness[i] = 1;
 }
 
-// Population summary (SummarizeSlabGenerationKernel, slab_runtime.cu:588-631): sequential fp32 sum.
-__global__ void SummaryKernel(const float* fitness, const uint8_t* has_fitness, const uint32_t* slots,
-                              uint32_t n_genomes, SummaryRecord* out, int* status) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    float best = 0.0f, sum = 0.0f;
-    uint32_t bi = 0;
-    for (uint32_t i = 0; i < n_genomes; ++i) {
-        if (has_fitness[i] == 0) {
-            SetStatus(status, 9);
-            return;
+// Population summary (SummarizeSlabGenerationKernel, slab_runtime.cu:588-631): best = max with the
+// lowest index on ties (a parallel reduction, exact), mean = the sequential fp32 sum in index order
+// (thread 0 over an LDS copy, the reference's summation order).
+constexpr uint32_t kSummaryChunk = 8192;
+__global__ __launch_bounds__(1024) void SummaryKernel(const float* fitness, const uint8_t* has_fitness,
+                                                      const uint32_t* slots, uint32_t n_genomes, SummaryRecord* out,
+                                                      int* status) {
+    __shared__ float s_fit[kSummaryChunk];
+    __shared__ float s_best[16];
+    __shared__ uint32_t s_bi[16];
+    __shared__ int s_missing;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    if (tid == 0) s_missing = 0;
+    float best = -__builtin_inff();
+    uint32_t bi = kInvalidSlot;
+    float sum = 0.0f;
+    for (uint32_t base = 0; base < n_genomes; base += kSummaryChunk) {
+        const uint32_t len = min(kSummaryChunk, n_genomes - base);
+        __syncthreads();
+        for (uint32_t k = tid; k < len; k += blockDim.x) {
+            const float f = fitness[base + k];
+            if (has_fitness[base + k] == 0) s_missing = 1;
+            s_fit[k] = f;
+            if (bi == kInvalidSlot || f > best) {  // k increases per thread: ties keep the lower index
+                best = f;
+                bi = base + k;
+            }
         }
-        const float f = fitness[i];
-        sum += f;
-        if (i == 0 || f > best) {
-            best = f;
-            bi = i;
+        __syncthreads();
+        if (tid == 0)
+            for (uint32_t k = 0; k < len; ++k) sum += s_fit[k];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const float ob = __shfl_xor(best, off);
+        const uint32_t oi = __shfl_xor(bi, off);
+        if (oi != kInvalidSlot && (bi == kInvalidSlot || ob > best || (ob == best && oi < bi))) {
+            best = ob;
+            bi = oi;
         }
+    }
+    if (lane == 0) {
+        s_best[wave] = best;
+        s_bi[wave] = bi;
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    if (s_missing) {
+        SetStatus(status, 9);
+        return;
+    }
+    for (uint32_t w = 1; w < (blockDim.x >> 6); ++w)
+        if (s_bi[w] != kInvalidSlot && (bi == kInvalidSlot || s_best[w] > best || (s_best[w] == best && s_bi[w] < bi))) {
+            best = s_best[w];
+            bi = s_bi[w];
+        }
+    if (bi == kInvalidSlot) {  // empty generation
+        best = 0.0f;
+        bi = 0;
     }
     out->best_fitness = best;
     out->average_fitness = sum / float(n_genomes);
@@ -774,6 +819,113 @@ __global__ __launch_bounds__(kPlanBlock) void PlanKernel(PlanArgs p) {
     p.pairs[2 * c + 1] = chosen[1];
 }
 
+// Radius <= 2 (the Moore list of <= 25 distinct cells): one wave per child.  Lane q owns candidate q:
+// it builds the (dr, dc)-ordered, de-duplicated list, hashes its tie key once and counts the eligible
+// candidates ranked before it; lane 0 then sums the rank weights and scans them in candidate order,
+// exactly the sequential fp32 arithmetic of the one-thread form above.
+constexpr int kPlanWaves = 4;
+__global__ __launch_bounds__(64 * kPlanWaves) void PlanListedKernel(PlanArgs p) {
+    __shared__ uint32_t s_raw[kPlanWaves][32];
+    __shared__ uint32_t s_idx[kPlanWaves][32];
+    __shared__ float s_fit[kPlanWaves][32];
+    __shared__ uint64_t s_key[kPlanWaves][32];
+    __shared__ uint32_t s_elig[kPlanWaves][32];
+    __shared__ float s_wt[kPlanWaves][32];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t c = blockIdx.x * kPlanWaves + wave;
+    if (c >= p.child_count) return;  // whole wave
+    const uint32_t cols = p.cols, rows = p.cur_size / cols;
+    const int r = int(p.radius), side = 2 * r + 1, cells = side * side;
+    const int fr = int(c / cols), fc = int(c % cols);
+    const bool in_list = int(lane) < cells;
+    uint32_t idx = 0;
+    if (in_list) {
+        int rr = (fr + int(lane) / side - r) % int(rows);
+        if (rr < 0) rr += int(rows);
+        int cc = (fc + int(lane) % side - r) % int(cols);
+        if (cc < 0) cc += int(cols);
+        idx = uint32_t(rr) * cols + uint32_t(cc);
+        s_raw[wave][lane] = idx;
+    }
+    __builtin_amdgcn_wave_barrier();
+    bool keep = in_list;
+    for (uint32_t q = 0; q < lane && keep; ++q) keep = s_raw[wave][q] != idx;
+    const uint64_t keep_mask = __ballot(keep ? 1 : 0);
+    const uint32_t ncand = uint32_t(__popcll(keep_mask));
+    const uint32_t pos = __builtin_amdgcn_mbcnt_hi(uint32_t(keep_mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(keep_mask), 0u));
+    if (keep) s_idx[wave][pos] = idx;
+    __builtin_amdgcn_wave_barrier();
+    const bool mine = lane < ncand;
+    const uint32_t my = mine ? s_idx[wave][lane] : 0u;
+    const float my_f = mine ? p.fitness[my] : 0.0f;
+    const bool my_has = mine && p.has_fitness[my] != 0;
+    const uint32_t seed = p.generation_seed ^ kPlanSeedXor;
+    uint32_t chosen[2] = {0, 0};
+    for (int k = 0; k < 2; ++k) {
+        const U4 w = Philox4x32(c, p.gen_index, uint32_t(k), 0, seed, kTagPlan);
+        const uint64_t tie = (uint64_t(w.x) << 32) ^ uint64_t(w.y);
+        const uint32_t excluded = k == 0 ? kInvalidSlot : chosen[0];
+        const bool elig = my_has && my != excluded;
+        const uint64_t key = SplitMix64(tie ^ uint64_t(my));
+        if (mine) {
+            s_fit[wave][lane] = my_f;
+            s_key[wave][lane] = key;
+            s_elig[wave][lane] = elig ? 1u : 0u;
+        }
+        __builtin_amdgcn_wave_barrier();
+        uint32_t rank = 1;
+        if (elig)
+            for (uint32_t q = 0; q < ncand; ++q) {
+                if (q == lane || s_elig[wave][q] == 0) continue;
+                const float f = s_fit[wave][q];
+                const uint64_t kq = s_key[wave][q];
+                const uint32_t iq = s_idx[wave][q];
+                if (f < my_f || (f == my_f && (kq < key || (kq == key && iq < my)))) ++rank;
+            }
+        if (mine) s_wt[wave][lane] = elig ? p.rank_weights[rank] : 0.0f;
+        __builtin_amdgcn_wave_barrier();
+        uint32_t pick = 0;
+        int fail = 0;
+        if (lane == 0) {
+            float sum = 0.0f;
+            bool found = false;
+            uint32_t fallback = 0;
+            for (uint32_t q = 0; q < ncand; ++q) {
+                if (s_elig[wave][q] == 0) continue;
+                sum += s_wt[wave][q];
+                fallback = s_idx[wave][q];
+                found = true;
+            }
+            if (!found || sum <= 0.0f) {
+                fail = 1;
+            } else {
+                const float threshold = UniformOpenClosed(w.z) * sum;
+                float cum = 0.0f;
+                pick = fallback;
+                for (uint32_t q = 0; q < ncand; ++q) {
+                    if (s_elig[wave][q] == 0) continue;
+                    cum += s_wt[wave][q];
+                    if (threshold <= cum) {
+                        pick = s_idx[wave][q];
+                        break;
+                    }
+                }
+            }
+        }
+        fail = __builtin_amdgcn_readfirstlane(fail);
+        if (fail) {
+            if (lane == 0) SetStatus(p.status, 11);  // kParentSelectionFailed
+            return;
+        }
+        chosen[k] = __builtin_amdgcn_readfirstlane(pick);
+        __builtin_amdgcn_wave_barrier();  // the LDS rows are rewritten for the second parent
+    }
+    if (lane == 0) {
+        p.pairs[2 * c] = chosen[0];
+        p.pairs[2 * c + 1] = chosen[1];
+    }
+}
+
 // ================================================================================================
 // Crossover decisions (genome_ops.cuh:210-221,264-299,344-376), one thread per child.
 __global__ void XoverDecisionKernel(XoverArgs p) {
@@ -847,6 +999,11 @@ __global__ __launch_bounds__(256) void BreedKernel(BreedArgs p) {
     __shared__ const float* s_p2;
     __shared__ float* s_child;
     __shared__ int s_bad;
+    // Positions where the donor can change (layer, neuron-row, bias, tail-row and splice bounds):
+    // a wave whose 256-value window holds none of them takes one donor for the whole window.
+    constexpr int kMaxBreaks = 48;
+    __shared__ uint32_t s_breaks[kMaxBreaks];
+    __shared__ uint32_t s_nbreaks;
     if (threadIdx.x == 0) {
         s_rec = p.records[child];
         const uint32_t pa = p.pairs[2 * child], pb = p.pairs[2 * child + 1];
@@ -862,29 +1019,71 @@ __global__ __launch_bounds__(256) void BreedKernel(BreedArgs p) {
             p.slot_state[cs] = 1u;
         }
     }
+    const uint32_t n_parent = uint32_t(kPolicyParams) + p.parent_action_count * kTail;
+    if (threadIdx.x == 0) {
+        const XoverRecord& r = s_rec;
+        uint32_t nb = 0;
+        auto add = [&](uint32_t b) { s_breaks[nb++] = b; };
+        for (int l = 0; l < kDenseLayers; ++l) {
+            const DenseLayerDesc d = DenseLayerAt(l);
+            add(d.w);
+            add(d.b);
+            add(d.b + d.out);
+            if (r.neuron[l] != kInvalidSlot) {
+                add(d.w + r.neuron[l] * d.in);
+                add(d.w + (r.neuron[l] + 1) * d.in);
+                add(d.b + r.neuron[l]);
+                add(d.b + r.neuron[l] + 1);
+            }
+        }
+        const uint32_t P = uint32_t(kPolicyParams);
+        if (r.row2 != kInvalidSlot) {
+            add(P + r.row2 * kTail);
+            add(P + (r.row2 + 1) * kTail);
+        }
+        if (r.splice_row != kInvalidSlot) {
+            add(P + r.splice_row * kTail);
+            add(P + r.splice_row * kTail + r.splice_point);
+            add(P + (r.splice_row + 1) * kTail);
+        }
+        add(n_parent);
+        s_nbreaks = nb;
+    }
     __syncthreads();
     if (s_bad) return;
     const XoverRecord r = s_rec;
     const float* p1 = s_p1;
     const float* p2 = s_p2;
     float* out = s_child;
-    const uint32_t n_parent = uint32_t(kPolicyParams) + p.parent_action_count * kTail;
+    const uint32_t nbreaks = s_nbreaks;
     const uint32_t n_slot = uint32_t(p.child_stride / 4);
     const bool mutate = p.mutation_p > 0.0f && p.mutation_sigma > 0.0f;
     const uint32_t quads = n_slot / 4;
+    const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += gridDim.x * blockDim.x) {
         const uint32_t e0 = 4 * q;
+        // the wave's window [w0, w0 + 256): uniform donor unless a break lies strictly inside
+        const uint32_t w0 = __builtin_amdgcn_readfirstlane(e0 - 4 * lane);
+        bool uniform = w0 + 256 <= n_parent;
+        for (uint32_t k = 0; k < nbreaks && uniform; ++k) {
+            const uint32_t b = s_breaks[k];
+            uniform = !(b > w0 && b < w0 + 256);
+        }
         v4f v = {0.0f, 0.0f, 0.0f, 0.0f};
         if (e0 < n_parent) {
-            const bool d0 = ElementDonorFirst(r, e0), d1 = ElementDonorFirst(r, e0 + 1);
-            const bool d2 = ElementDonorFirst(r, e0 + 2), d3 = ElementDonorFirst(r, e0 + 3);
-            if (d0 == d1 && d1 == d2 && d2 == d3 && e0 + 4 <= n_parent) {
-                v = Load4((d0 ? p1 : p2) + e0);
+            if (uniform) {
+                v = Load4((ElementDonorFirst(r, w0) ? p1 : p2) + e0);
             } else {
-                const bool dd[4] = {d0, d1, d2, d3};
+                const bool d0 = ElementDonorFirst(r, e0), d1 = ElementDonorFirst(r, e0 + 1);
+                const bool d2 = ElementDonorFirst(r, e0 + 2), d3 = ElementDonorFirst(r, e0 + 3);
+                if (d0 == d1 && d1 == d2 && d2 == d3 && e0 + 4 <= n_parent) {
+                    v = Load4((d0 ? p1 : p2) + e0);
+                } else {
+                    const bool dd[4] = {d0, d1, d2, d3};
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    if (e0 + i < n_parent) v[i] = (dd[i] ? p1 : p2)[e0 + i];
+                    for (int i = 0; i < 4; ++i)
+                        if (e0 + i < n_parent) v[i] = (dd[i] ? p1 : p2)[e0 + i];
+                }
             }
             if (mutate) {
                 const U4 w = Philox4x32(q, child, p.gen_index, 0, p.seed, kTagMuta);
@@ -972,25 +1171,55 @@ __global__ void RefcountKernel(const uint32_t* pairs, uint32_t child_count, uint
     atomicAdd(refcounts + b, 1u);
 }
 
-// Frees every parent whose reference count is zero and whose slot is still held.
-__global__ void CollectReleasedParentsKernel(uint32_t* cur_slots, uint32_t cur_size, const uint32_t* refcounts,
-                                             uint32_t* slot_state, uint32_t* free_stack, uint32_t* free_count,
-                                             uint32_t slot_count, int* status) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Frees every parent whose reference count is zero and whose slot is still held, pushing the slots in
+// generation-index order (TryCollectZeroReferenceParents, reference_counter.hpp:135-220).  One
+// workgroup of 1024 threads: a ballot prefix sum gives each released parent its stack position; the
+// slot-state exchange catches a slot released twice.
+__global__ __launch_bounds__(1024) void CollectReleasedParentsKernel(uint32_t* cur_slots, uint32_t cur_size,
+                                                                     const uint32_t* refcounts, uint32_t* slot_state,
+                                                                     uint32_t* free_stack, uint32_t* free_count,
+                                                                     uint32_t slot_count, int* status) {
+    __shared__ uint32_t s_wave_total[16];
+    __shared__ int s_bad;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    if (tid == 0) s_bad = 0;
     uint32_t fc = *free_count;
-    for (uint32_t i = 0; i < cur_size; ++i) {
-        if (refcounts[i] != 0) continue;
-        const uint32_t s = cur_slots[i];
-        if (s == kInvalidSlot) continue;
-        if (s >= slot_count || slot_state[s] == 0 || fc >= slot_count) {
-            SetStatus(status, 3);  // kInvalidSlab
-            return;
+    __syncthreads();
+    for (uint32_t base = 0; base < cur_size; base += blockDim.x) {
+        const uint32_t i = base + tid;
+        uint32_t s = kInvalidSlot;
+        bool release = false;
+        if (i < cur_size && refcounts[i] == 0) {
+            s = cur_slots[i];
+            release = s != kInvalidSlot;
         }
-        slot_state[s] = 0;
-        free_stack[fc++] = s;
-        cur_slots[i] = kInvalidSlot;
+        const uint64_t mask = __ballot(release ? 1 : 0);
+        const uint32_t rank_in_wave = __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+        if (lane == 0) s_wave_total[wave] = uint32_t(__popcll(mask));
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) {
+            before += w < wave ? s_wave_total[w] : 0u;
+            total += s_wave_total[w];
+        }
+        if (release) {
+            const uint32_t pos = fc + before + rank_in_wave;
+            if (s >= slot_count || pos >= slot_count || atomicExch(slot_state + s, 0u) == 0u) {
+                atomicExch(&s_bad, 1);
+            } else {
+                free_stack[pos] = s;
+                cur_slots[i] = kInvalidSlot;
+            }
+        }
+        fc += total;
+        __syncthreads();  // s_wave_total is rewritten next round
     }
-    *free_count = fc;
+    if (tid == 0) {
+        if (s_bad)
+            SetStatus(status, 3);  // kInvalidSlab
+        else
+            *free_count = fc;
+    }
 }
 
 __global__ void ReleaseBatchKernel(const uint32_t* pairs, uint32_t child_offset, uint32_t batch, uint32_t* refcounts,
@@ -1220,11 +1449,15 @@ void LaunchFinalize(const float* raw, const uint32_t* local_w, uint32_t n, float
 
 void LaunchSummary(const float* fitness, const uint8_t* has, const uint32_t* slots, uint32_t n, SummaryRecord* out,
                    int* status, hipStream_t s) {
-    hipLaunchKernelGGL(SummaryKernel, dim3(1), dim3(1), 0, s, fitness, has, slots, n, out, status);
+    hipLaunchKernelGGL(SummaryKernel, dim3(1), dim3(1024), 0, s, fitness, has, slots, n, out, status);
 }
 
 void LaunchPlan(const PlanArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(PlanKernel, dim3((a.child_count + kPlanBlock - 1) / kPlanBlock), dim3(kPlanBlock), 0, s, a);
+    if (a.radius <= 2)
+        hipLaunchKernelGGL(PlanListedKernel, dim3((a.child_count + kPlanWaves - 1) / kPlanWaves), dim3(64 * kPlanWaves),
+                           0, s, a);
+    else
+        hipLaunchKernelGGL(PlanKernel, dim3((a.child_count + kPlanBlock - 1) / kPlanBlock), dim3(kPlanBlock), 0, s, a);
 }
 
 void LaunchXover(const XoverArgs& a, hipStream_t s) {
@@ -1254,7 +1487,7 @@ void LaunchRefcount(const uint32_t* pairs, uint32_t child_count, uint32_t cur_si
 void LaunchCollectReleased(uint32_t* cur_slots, uint32_t cur_size, const uint32_t* refcounts, uint32_t* slot_state,
                            uint32_t* free_stack, uint32_t* free_count, uint32_t slot_count, int* status,
                            hipStream_t s) {
-    hipLaunchKernelGGL(CollectReleasedParentsKernel, dim3(1), dim3(1), 0, s, cur_slots, cur_size, refcounts,
+    hipLaunchKernelGGL(CollectReleasedParentsKernel, dim3(1), dim3(1024), 0, s, cur_slots, cur_size, refcounts,
                        slot_state, free_stack, free_count, slot_count, status);
 }
 
