@@ -269,75 +269,44 @@ __global__ void TransposeEncoderKernel(const uint8_t* slab, uint64_t stride, con
 // through LDS in chunks of kChunkTiles action tiles (double-buffered, loaded by the whole workgroup
 // one chunk ahead), so a genome's embedding is read from HBM/L2 once per workgroup and decision.
 #ifndef NE_CHUNK_TILES
-#define NE_CHUNK_TILES 1  // action tiles per LDS chunk (build-time tunable)
+#define NE_CHUNK_TILES 4  // action tiles per LDS chunk (build-time tunable)
 #endif
 #ifndef NE_EVAL_WAVES_PER_EU
 #define NE_EVAL_WAVES_PER_EU 3  // register budget: 3 waves/SIMD = four 3-wave workgroups per CU
 #endif
 constexpr int kChunkTiles = NE_CHUNK_TILES;
-constexpr int kLdsRow = 42;  // floats per tail row in LDS: 38 values, 2 zero pads, 2 unused
-constexpr int kChunkRows = 16 * kChunkTiles;
-constexpr uint32_t kImgChunkWords = uint32_t(kChunkTiles * kImgTileWords);
-constexpr uint32_t kImgChunkUnits = kImgChunkWords / 2;          // 8-byte load units
-constexpr uint32_t kTailChunkUnits = uint32_t(kChunkRows * kTail / 2);
-constexpr uint32_t kTailChunkBytes = uint32_t(kChunkRows * kTail * 4);
-// register-staged load units per thread, sized for 3-wave workgroups (smaller ones load the rest
-// synchronously)
-constexpr int kStageImg = int((kImgChunkUnits + 191) / 192);
-constexpr int kStageTail = int((kTailChunkUnits + 191) / 192);
-static_assert((kTail % 2) == 0 && kImgTileWords % 2 == 0, "8-byte load units");
+constexpr uint32_t kImgChunkBytes = uint32_t(kChunkTiles * kImgTileBytes);
+constexpr uint32_t kTailChunkBytes = uint32_t(kChunkTiles * 16 * kTail * 4);
+// Chunks move by LDS DMA (buffer_load_dwordx4 ... lds): one instruction moves 64 lanes x 16 bytes to
+// consecutive LDS, so each region is a whole number of 1 KB transfers.
+constexpr uint32_t kDmaBytes = 1024;
+constexpr uint32_t kImgDma = (kImgChunkBytes + kDmaBytes - 1) / kDmaBytes;
+constexpr uint32_t kTailDma = (kTailChunkBytes + kDmaBytes - 1) / kDmaBytes;
+constexpr uint32_t kChunkBufBytes = (kImgDma + kTailDma) * kDmaBytes;
+constexpr uint32_t kTailRegion = kImgDma * kDmaBytes;  // byte offset of the tail rows in a buffer
+static_assert(kImgTileBytes % 16 == 0 && (16 * kTail * 4) % 16 == 0, "16-byte transfer units");
 
-typedef uint32_t v2u __attribute__((ext_vector_type(2)));
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) void* LdsPtr;
 
-__device__ inline v2u Buf2(Rsrc r, uint32_t voff) {
-    return __builtin_bit_cast(v2u, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
-}
-
-struct ChunkStage {
-    v2u img[kStageImg];
-    v2u tail[kStageTail];
-};
-
-// Issues the loads of chunk c (out-of-range bytes read as 0: the descriptors' sizes bound them).
-__device__ __forceinline__ void LoadChunk(ChunkStage& st, Rsrc img_r, Rsrc tail_r, uint32_t c, uint32_t tid,
-                                          uint32_t bd) {
-#pragma unroll
-    for (int m = 0; m < kStageImg; ++m) {
-        const uint32_t u = tid + uint32_t(m) * bd;
-        if (u < kImgChunkUnits) st.img[m] = Buf2(img_r, c * (4 * kImgChunkWords) + 8 * u);
-    }
-#pragma unroll
-    for (int m = 0; m < kStageTail; ++m) {
-        const uint32_t u = tid + uint32_t(m) * bd;
-        if (u < kTailChunkUnits) st.tail[m] = Buf2(tail_r, c * kTailChunkBytes + 8 * u);
+// Issues the DMA transfers of chunk c into one LDS buffer, spread over the workgroup's waves.  Bytes
+// past a descriptor's size arrive as 0.  The caller waits (vmcnt) and barriers before reading.
+__device__ __forceinline__ void IssueChunk(uint8_t* buf, Rsrc img_r, Rsrc tail_r, uint32_t c, uint32_t wave,
+                                           uint32_t waves, uint32_t lane) {
+    for (uint32_t i = wave; i < kImgDma + kTailDma; i += waves) {
+        if (i < kImgDma) {
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(img_r, (LdsPtr)(buf + i * kDmaBytes), 16,
+                                                     c * kImgChunkBytes + i * kDmaBytes + 16 * lane, 0, 0, 0);
+        } else {
+            const uint32_t k = i - kImgDma;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(tail_r, (LdsPtr)(buf + kTailRegion + k * kDmaBytes), 16,
+                                                     c * kTailChunkBytes + k * kDmaBytes + 16 * lane, 0, 0, 0);
+        }
     }
 }
 
-__device__ __forceinline__ void StoreTailUnit(float* tail_lds, uint32_t u, v2u v) {
-    const uint32_t row = u / uint32_t(kTail / 2), q = u - row * uint32_t(kTail / 2);
-    *reinterpret_cast<v2u*>(tail_lds + row * kLdsRow + 2 * q) = v;
-}
-
-// Writes chunk c into one LDS buffer; units beyond the staged ones (workgroups of fewer than
-// kImgChunkUnits / kStageImg threads) are loaded and written here directly.
-__device__ __forceinline__ void WriteChunk(const ChunkStage& st, uint32_t* img_lds, float* tail_lds, Rsrc img_r,
-                                           Rsrc tail_r, uint32_t c, uint32_t tid, uint32_t bd) {
-#pragma unroll
-    for (int m = 0; m < kStageImg; ++m) {
-        const uint32_t u = tid + uint32_t(m) * bd;
-        if (u < kImgChunkUnits) *reinterpret_cast<v2u*>(img_lds + 2 * u) = st.img[m];
-    }
-#pragma unroll
-    for (int m = 0; m < kStageTail; ++m) {
-        const uint32_t u = tid + uint32_t(m) * bd;
-        if (u < kTailChunkUnits) StoreTailUnit(tail_lds, u, st.tail[m]);
-    }
-    for (uint32_t u = tid + kStageImg * bd; u < kImgChunkUnits; u += bd)
-        *reinterpret_cast<v2u*>(img_lds + 2 * u) = Buf2(img_r, c * (4 * kImgChunkWords) + 8 * u);
-    for (uint32_t u = tid + kStageTail * bd; u < kTailChunkUnits; u += bd)
-        StoreTailUnit(tail_lds, u, Buf2(tail_r, c * kTailChunkBytes + 8 * u));
-}
+// vmcnt(0), leaving the other counters alone: this wave's chunk transfers have landed in LDS.
+__device__ __forceinline__ void WaitChunk() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // Masked argmax state of one lane: candidates in increasing action order, the first unmasked one
 // taken whatever its score, later ones only when strictly greater (evaluation_ops.cuh:429-445).
@@ -356,7 +325,7 @@ __device__ __forceinline__ void ScoreTile(const uint32_t* img_tile, const float*
     const v4f f0 = *reinterpret_cast<const v4f*>(fx + 4 * lane);
     const v4f f1 = *reinterpret_cast<const v4f*>(fx + 256 + 4 * lane);
     const v4u cn = *reinterpret_cast<const v4u*>(img_tile + 512 + 4 * gq);
-    const float* tr = tail_tile + j * kLdsRow + kTailSteps * gq;
+    const float* tr = tail_tile + j * kTail + kTailSteps * gq;
     float ev[kScoreSteps];
     ev[0] = f0[0];
     ev[1] = f0[1];
@@ -371,6 +340,9 @@ __device__ __forceinline__ void ScoreTile(const uint32_t* img_tile, const float*
         ev[kFixedSteps + 2 * q] = v.x;
         ev[kFixedSteps + 2 * q + 1] = v.y;
     }
+    // lane group 3's last two steps are pads (tail dims 38, 39): what lies there is the next row
+    ev[kScoreSteps - 2] = gq == 3 ? 0.0f : ev[kScoreSteps - 2];
+    ev[kScoreSteps - 1] = gq == 3 ? 0.0f : ev[kScoreSteps - 1];
     v4f sc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int s = 0; s < kScoreSteps; ++s) sc = Mfma(ev[s], pol[s], sc);
@@ -394,8 +366,9 @@ __device__ __forceinline__ void ScoreTile(const uint32_t* img_tile, const float*
 
 template <bool kTrace>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void EvaluateKernel(EvalArgs p) {
-    __shared__ __align__(16) uint32_t s_img[2][kImgChunkWords];
-    __shared__ __align__(16) float s_tail[2][kChunkRows * kLdsRow];
+    // two chunk buffers as distinct objects, so a read of one never waits on transfers into the other
+    __shared__ __align__(16) uint8_t s_buf0[kChunkBufBytes];
+    __shared__ __align__(16) uint8_t s_buf1[kChunkBufBytes];
 
     const uint32_t n = blockIdx.x;
     const uint32_t slot = p.slot_indices[n];
@@ -406,13 +379,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
     const float* genome = reinterpret_cast<const float*>(p.slab + uint64_t(slot) * p.stride);
     const uint32_t tid = threadIdx.x, bd = blockDim.x;
 
-    // the two pad floats of every LDS tail row stay 0 (the chunk writes never touch them)
-    for (uint32_t idx = tid; idx < uint32_t(2 * kChunkRows); idx += bd) {
-        float* row = &s_tail[idx / kChunkRows][(idx % kChunkRows) * kLdsRow];
-        row[kTail] = 0.0f;
-        row[kTail + 1] = 0.0f;
-    }
-    __syncthreads();
     const Rsrc enc_r = MakeRsrc(p.enc_t + uint64_t(n) * (kTurnFeatures * kEncOut), uint32_t(kTurnFeatures * kEncOut * 4));
 
     const uint32_t W = p.cell_offsets[n + 1] - p.cell_offsets[n];
@@ -506,6 +472,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
         for (int step = 0; step < kMaxTurns; ++step) {
             if (!__syncthreads_or(active ? 1 : 0)) break;  // the whole workgroup leaves together
             const bool wave_active = __any(active ? 1 : 0) != 0;
+            IssueChunk(s_buf0, img_r, tail_r, 0, wave, waves, lane);  // lands behind the trunk
             float pol[kScoreSteps];
             if (wave_active) {
                 const int ttc = base_tc + step;  // turn count of every active game in this tile
@@ -527,26 +494,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
 #pragma unroll
                 for (int s = 0; s < kScoreSteps; ++s) pol[s] = 0.0f;
             }
-            ChunkStage st;
-            LoadChunk(st, img_r, tail_r, 0, tid, bd);
-            WriteChunk(st, s_img[0], s_tail[0], img_r, tail_r, 0, tid, bd);
+            WaitChunk();
             __syncthreads();
 
-            // ---- scoring: S = E . P over all actions; masked argmax
+            // ---- scoring: S = E . P over all actions; masked argmax.  Chunk c + 1 moves into the
+            // other buffer while chunk c is scored.
             ArgmaxLane am = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot};
-            for (uint32_t c = 0; c < chunks; ++c) {
-                const bool more = c + 1 < chunks;
-                if (more) LoadChunk(st, img_r, tail_r, c + 1, tid, bd);
+            auto score_chunk = [&](const uint8_t* buf, uint8_t* next, uint32_t c) {
+                if (c + 1 < chunks) IssueChunk(next, img_r, tail_r, c + 1, wave, waves, lane);
                 if (wave_active) {
-                    const uint32_t* img = s_img[c & 1];
-                    const float* tails = s_tail[c & 1];
 #pragma unroll
                     for (int k = 0; k < kChunkTiles; ++k)
-                        ScoreTile<kTrace>(img + k * kImgTileWords, tails + k * 16 * kLdsRow, lane, j, gq, pol,
-                                          (c * kChunkTiles + k) * 16 + 4 * gq, A, hist, am);
+                        ScoreTile<kTrace>(reinterpret_cast<const uint32_t*>(buf) + k * kImgTileWords,
+                                          reinterpret_cast<const float*>(buf + kTailRegion) + k * 16 * kTail, lane,
+                                          j, gq, pol, (c * kChunkTiles + k) * 16 + 4 * gq, A, hist, am);
                 }
-                if (more) WriteChunk(st, s_img[(c + 1) & 1], s_tail[(c + 1) & 1], img_r, tail_r, c + 1, tid, bd);
+                WaitChunk();
                 __syncthreads();
+            };
+            for (uint32_t c = 0; c < chunks; c += 2) {
+                score_chunk(s_buf0, s_buf1, c);
+                if (c + 1 < chunks) score_chunk(s_buf1, s_buf0, c + 1);
             }
             float best = am.best, second = am.second;
             uint32_t besta = am.besta;
