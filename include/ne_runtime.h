@@ -151,7 +151,19 @@ typedef struct ne_stage_times {
     uint64_t encoded_turns;               /* turns run through the encoder */
     uint64_t games;                       /* episodes played */
     uint64_t children;                    /* children assembled by the last advance */
+    uint64_t rescored;                    /* BF16 mode: candidates rescored in FP32 */
+    uint64_t fallbacks;                   /* BF16 mode: decisions settled by the FP32 scan */
 } ne_stage_times;
+
+/* Evaluation precision (ne_set_eval_precision).  NE_EVAL_FP32 is the reference's arithmetic.
+ * NE_EVAL_BF16_RESCORED screens every action on BF16 MFMA (FP32 accumulate), then rescores each action
+ * whose BF16 score lies within a proven error bound of the best in FP32 with the very contraction of the
+ * FP32 mode: guesses, chosen scores and fitness equal the FP32 mode's bit for bit. */
+enum {
+    NE_EVAL_FP32 = 0,
+    NE_EVAL_BF16_RESCORED = 1,
+    NE_EVAL_FORCE_FALLBACK = 0x100   /* with BF16: settle every decision by the FP32 scan (diagnostic) */
+};
 
 /* Callbacks of TryAdvanceGenerationOnDevice (slab_runtime.hpp:81-84).  Non-zero return = failure. */
 typedef struct ne_callbacks {
@@ -245,6 +257,9 @@ const char *ne_status_string(int status);
 int ne_get_stage_times(ne_runtime *rt, ne_stage_times *out);
 /* Parity mode: flags = OR of NE_TRACE_*; the next evaluate/advance records traces. */
 int ne_set_trace(ne_runtime *rt, uint32_t flags);
+/* Selects the evaluation precision for later evaluate/advance calls (no reference counterpart: the
+ * reference evaluates in its storage precision only, evaluation_ops.cuh:244-508). */
+int ne_set_eval_precision(ne_runtime *rt, uint32_t mode);
 /* Copies a recorded trace; game traces are ordered by individual, then game slot (W fresh games by
  * entry, then 2W prefilled games: slot W+q is entry q/2, variant 1+q%2). */
 int ne_download_trace(ne_runtime *rt, uint32_t kind, void *dst, uint64_t capacity_bytes, uint64_t *written);

@@ -72,6 +72,9 @@ __host__ __device__ inline int ScoreDim(int s, int g) {
 constexpr int kImgTileWords = 2 * 64 * 4 + 16;  // 528
 constexpr int kImgTileBytes = 4 * kImgTileWords;
 
+// Per-evaluation copy of a genome's encoder, transposed to [feature][out], then W0's virgin column.
+constexpr uint64_t kEncTStride = uint64_t(kTurnFeatures) * kEncOut + kH0;
+
 // Dense layers in crossover / mutation order (reference genome_ops.cuh:264-309).
 struct DenseLayerDesc {
     uint32_t w, b, in, out;

@@ -243,14 +243,17 @@ __device__ __forceinline__ void StageEncoder(const float* genome, float* s_enc, 
     __syncthreads();
 }
 
-// Writes each genome's 145->64 encoder transposed to [feature][out] (one workgroup per genome).
+// Writes each genome's 145->64 encoder transposed to [feature][out], followed by the virgin-flag
+// column of W0 (W0[:, 0], 256 values) made contiguous (one workgroup per genome).
 __global__ void TransposeEncoderKernel(const uint8_t* slab, uint64_t stride, const uint32_t* slot_indices,
                                        uint32_t slot_count, float* enc_t) {
     const uint32_t slot = slot_indices[blockIdx.x];
     if (slot >= slot_count) return;  // reported by the evaluation kernel
     __shared__ float s_w[kEncOut * kTurnFeatures];  // [out][feature], as stored in the genome
     const float* genome = reinterpret_cast<const float*>(slab + uint64_t(slot) * stride);
-    float* dst = enc_t + uint64_t(blockIdx.x) * (kTurnFeatures * kEncOut);
+    float* dst = enc_t + uint64_t(blockIdx.x) * kEncTStride;
+    for (uint32_t o = threadIdx.x; o < uint32_t(kH0); o += blockDim.x)
+        dst[kTurnFeatures * kEncOut + o] = genome[kOffW0 + o * kModelIn];
     for (uint32_t idx = threadIdx.x; idx < uint32_t(kEncOut * kTurnFeatures); idx += blockDim.x)
         s_w[idx] = genome[kOffEncW + idx];
     __syncthreads();
@@ -269,21 +272,25 @@ __global__ void TransposeEncoderKernel(const uint8_t* slab, uint64_t stride, con
 // through LDS in chunks of kChunkTiles action tiles (double-buffered, loaded by the whole workgroup
 // one chunk ahead), so a genome's embedding is read from HBM/L2 once per workgroup and decision.
 #ifndef NE_CHUNK_TILES
-#define NE_CHUNK_TILES 4  // action tiles per LDS chunk (build-time tunable)
+#define NE_CHUNK_TILES 4  // action tiles per LDS chunk of the FP32 scan (build-time tunable)
 #endif
 #ifndef NE_EVAL_WAVES_PER_EU
 #define NE_EVAL_WAVES_PER_EU 3  // register budget: 3 waves/SIMD = four 3-wave workgroups per CU
 #endif
 constexpr int kChunkTiles = NE_CHUNK_TILES;
-constexpr uint32_t kImgChunkBytes = uint32_t(kChunkTiles * kImgTileBytes);
-constexpr uint32_t kTailChunkBytes = uint32_t(kChunkTiles * 16 * kTail * 4);
+constexpr int kChunkTilesBf16 = 2;  // the BF16 mode keeps candidate lists in LDS as well
 // Chunks move by LDS DMA (buffer_load_dwordx4 ... lds): one instruction moves 64 lanes x 16 bytes to
 // consecutive LDS, so each region is a whole number of 1 KB transfers.
 constexpr uint32_t kDmaBytes = 1024;
-constexpr uint32_t kImgDma = (kImgChunkBytes + kDmaBytes - 1) / kDmaBytes;
-constexpr uint32_t kTailDma = (kTailChunkBytes + kDmaBytes - 1) / kDmaBytes;
-constexpr uint32_t kChunkBufBytes = (kImgDma + kTailDma) * kDmaBytes;
-constexpr uint32_t kTailRegion = kImgDma * kDmaBytes;  // byte offset of the tail rows in a buffer
+template <int kCT>
+struct ChunkGeom {
+    static constexpr uint32_t kImgBytes = uint32_t(kCT * kImgTileBytes);
+    static constexpr uint32_t kTailBytes = uint32_t(kCT * 16 * kTail * 4);
+    static constexpr uint32_t kImgDma = (kImgBytes + kDmaBytes - 1) / kDmaBytes;
+    static constexpr uint32_t kTailDma = (kTailBytes + kDmaBytes - 1) / kDmaBytes;
+    static constexpr uint32_t kBufBytes = (kImgDma + kTailDma) * kDmaBytes;
+    static constexpr uint32_t kTailRegion = kImgDma * kDmaBytes;  // byte offset of the tail rows
+};
 static_assert(kImgTileBytes % 16 == 0 && (16 * kTail * 4) % 16 == 0, "16-byte transfer units");
 
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
@@ -292,21 +299,50 @@ typedef __attribute__((address_space(3))) void* LdsPtr;
 // Issues the DMA transfers of chunk c into one LDS buffer, spread over the workgroup's waves.  Bytes
 // past a descriptor's size arrive as 0.  The caller waits (vmcnt) and barriers before reading.
 __device__ __forceinline__ void IssueChunk(uint8_t* buf, Rsrc img_r, Rsrc tail_r, uint32_t c, uint32_t wave,
-                                           uint32_t waves, uint32_t lane) {
-    for (uint32_t i = wave; i < kImgDma + kTailDma; i += waves) {
-        if (i < kImgDma) {
+                                           uint32_t waves, uint32_t lane, uint32_t img_bytes, uint32_t tail_bytes,
+                                           uint32_t img_dma, uint32_t tail_dma) {
+    for (uint32_t i = wave; i < img_dma + tail_dma; i += waves) {
+        if (i < img_dma) {
             __builtin_amdgcn_raw_ptr_buffer_load_lds(img_r, (LdsPtr)(buf + i * kDmaBytes), 16,
-                                                     c * kImgChunkBytes + i * kDmaBytes + 16 * lane, 0, 0, 0);
+                                                     c * img_bytes + i * kDmaBytes + 16 * lane, 0, 0, 0);
         } else {
-            const uint32_t k = i - kImgDma;
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(tail_r, (LdsPtr)(buf + kTailRegion + k * kDmaBytes), 16,
-                                                     c * kTailChunkBytes + k * kDmaBytes + 16 * lane, 0, 0, 0);
+            const uint32_t k = i - img_dma;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(tail_r, (LdsPtr)(buf + img_dma * kDmaBytes + k * kDmaBytes), 16,
+                                                     c * tail_bytes + k * kDmaBytes + 16 * lane, 0, 0, 0);
         }
     }
 }
 
 // vmcnt(0), leaving the other counters alone: this wave's chunk transfers have landed in LDS.
 __device__ __forceinline__ void WaitChunk() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Streams every chunk of the scoring image and tail rows through two LDS buffers (b0, b1: distinct
+// objects), chunk c + 1 in flight while chunk c is used.  issue(buf, c) starts the transfers of
+// chunk c; per_tile(img_tile, tail_tile, at) runs for the kCT tiles of each chunk when `compute`.
+// The whole workgroup calls this together; chunk 0 must already be in flight into b0.
+template <int kCT, typename FI, typename FT>
+__device__ __forceinline__ void StreamChunks(uint8_t* b0, uint8_t* b1, uint32_t chunks, bool compute, FI&& issue,
+                                             FT&& per_tile) {
+    using G = ChunkGeom<kCT>;
+    WaitChunk();
+    __syncthreads();
+    auto one = [&](const uint8_t* buf, uint8_t* next, uint32_t c) __device__ {
+        if (c + 1 < chunks) issue(next, c + 1);
+        if (compute) {
+#pragma unroll
+            for (int k = 0; k < kCT; ++k)
+                per_tile(reinterpret_cast<const uint32_t*>(buf) + k * kImgTileWords,
+                         reinterpret_cast<const float*>(buf + G::kTailRegion) + k * 16 * kTail,
+                         c * uint32_t(kCT) + uint32_t(k));
+        }
+        WaitChunk();
+        __syncthreads();
+    };
+    for (uint32_t c = 0; c < chunks; c += 2) {
+        one(b0, b1, c);
+        if (c + 1 < chunks) one(b1, b0, c + 1);
+    }
+}
 
 // Masked argmax state of one lane: candidates in increasing action order, the first unmasked one
 // taken whatever its score, later ones only when strictly greater (evaluation_ops.cuh:429-445).
@@ -315,18 +351,14 @@ struct ArgmaxLane {
     uint32_t besta;
 };
 
-// Scores one action tile (16 actions) against the lane group's policy slots and folds the 4 rows
-// this lane owns (actions abase..abase+3) into its argmax.
-template <bool kTrace>
-__device__ __forceinline__ void ScoreTile(const uint32_t* img_tile, const float* tail_tile, uint32_t lane, uint32_t j,
-                                          uint32_t gq, const float (&pol)[kScoreSteps], uint32_t abase, uint32_t A,
-                                          const uint32_t (&hist)[5], ArgmaxLane& am) {
+// The 17 embedding values lane (j, gq) feeds into the scoring contraction for action row j of a tile:
+// fixed steps from the two image planes, tail steps from the row (lane group 3's last two steps are
+// pads; what lies there is the next row, so they are zeroed).
+__device__ __forceinline__ void TileOperands(const uint32_t* img_tile, const float* tail_tile, uint32_t lane,
+                                             uint32_t j, uint32_t gq, float (&ev)[kScoreSteps]) {
     const float* fx = reinterpret_cast<const float*>(img_tile);
     const v4f f0 = *reinterpret_cast<const v4f*>(fx + 4 * lane);
     const v4f f1 = *reinterpret_cast<const v4f*>(fx + 256 + 4 * lane);
-    const v4u cn = *reinterpret_cast<const v4u*>(img_tile + 512 + 4 * gq);
-    const float* tr = tail_tile + j * kTail + kTailSteps * gq;
-    float ev[kScoreSteps];
     ev[0] = f0[0];
     ev[1] = f0[1];
     ev[2] = f0[2];
@@ -334,18 +366,32 @@ __device__ __forceinline__ void ScoreTile(const uint32_t* img_tile, const float*
     ev[4] = f1[0];
     ev[5] = f1[1];
     ev[6] = f1[2];
+    const float* tr = tail_tile + j * kTail + kTailSteps * gq;
 #pragma unroll
     for (int q = 0; q < kTailSteps / 2; ++q) {
         const float2 v = *reinterpret_cast<const float2*>(tr + 2 * q);
         ev[kFixedSteps + 2 * q] = v.x;
         ev[kFixedSteps + 2 * q + 1] = v.y;
     }
-    // lane group 3's last two steps are pads (tail dims 38, 39): what lies there is the next row
     ev[kScoreSteps - 2] = gq == 3 ? 0.0f : ev[kScoreSteps - 2];
     ev[kScoreSteps - 1] = gq == 3 ? 0.0f : ev[kScoreSteps - 1];
+}
+
+// The FP32 scoring contraction of one tile: 17 dependent v_mfma_f32_16x16x4_f32 steps in a fixed
+// order.  Every FP32 score of this build, streamed or rescored, comes from exactly this sequence.
+__device__ __forceinline__ v4f Fp32Scores(const float (&ev)[kScoreSteps], const float (&pol)[kScoreSteps]) {
     v4f sc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int s = 0; s < kScoreSteps; ++s) sc = Mfma(ev[s], pol[s], sc);
+    return sc;
+}
+
+// Masked argmax over the 4 rows (actions abase..abase+3) this lane owns in a tile; candidates arrive
+// in increasing action order, the first unmasked one is taken whatever its score, later ones only
+// when strictly greater (evaluation_ops.cuh:429-445).
+template <bool kTrace>
+__device__ __forceinline__ void ArgmaxUpdate(const v4f& sc, const v4u& cn, uint32_t abase, uint32_t A,
+                                             const uint32_t (&hist)[5], ArgmaxLane& am) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const uint32_t a = abase + uint32_t(r);
@@ -364,11 +410,109 @@ __device__ __forceinline__ void ScoreTile(const uint32_t* img_tile, const float*
     }
 }
 
+// Scores one action tile (16 actions) in FP32 and folds this lane's 4 rows into its argmax.
 template <bool kTrace>
+__device__ __forceinline__ void ScoreTile(const uint32_t* img_tile, const float* tail_tile, uint32_t lane, uint32_t j,
+                                          uint32_t gq, const float (&pol)[kScoreSteps], uint32_t abase, uint32_t A,
+                                          const uint32_t (&hist)[5], ArgmaxLane& am) {
+    float ev[kScoreSteps];
+    TileOperands(img_tile, tail_tile, lane, j, gq, ev);
+    const v4u cn = *reinterpret_cast<const v4u*>(img_tile + 512 + 4 * gq);
+    ArgmaxUpdate<kTrace>(Fp32Scores(ev, pol), cn, abase, A, hist, am);
+}
+
+// ---- BF16 screening -----------------------------------------------------------------------------
+// The same contraction on v_mfma_f32_16x16x32_bf16 (FP32 accumulate): lane group g feeds its 17 steps
+// as bf16 in three K=32 instructions (8 per lane each, steps 17..23 zero).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+struct PolicyBf16 {
+    bf16x8 k[3];
+};
+
+__device__ __forceinline__ PolicyBf16 PackBf16(const float (&v)[kScoreSteps]) {
+    PolicyBf16 p;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int s = 8 * m + e;
+            p.k[m][e] = __bf16(s < kScoreSteps ? v[s] : 0.0f);
+        }
+    return p;
+}
+
+__device__ __forceinline__ v4f Bf16Scores(const float (&ev)[kScoreSteps], const PolicyBf16& pb) {
+    const PolicyBf16 eb = PackBf16(ev);
+    v4f sc = {0.0f, 0.0f, 0.0f, 0.0f};
+    sc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(eb.k[0], pb.k[0], sc, 0, 0, 0);
+    sc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(eb.k[1], pb.k[1], sc, 0, 0, 0);
+    sc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(eb.k[2], pb.k[2], sc, 0, 0, 0);
+    return sc;
+}
+
+// Reduces a lane's masked-argmax state over the four lane groups sharing game column j (lower index
+// wins ties); every lane of the column ends with the game's result.
+__device__ __forceinline__ void ReduceArgmax(float& best, float& second, uint32_t& besta) {
+#pragma unroll
+    for (int off = 16; off <= 32; off <<= 1) {
+        const float ob = __shfl_xor(best, off);
+        const uint32_t oa = __shfl_xor(besta, off);
+        const float os = __shfl_xor(second, off);
+        const bool take = oa != kInvalidSlot && (besta == kInvalidSlot || ob > best || (ob == best && oa < besta));
+        if (take) {
+            second = besta == kInvalidSlot ? fmaxf(second, os) : fmaxf(fmaxf(second, best), os);
+            best = ob;
+            besta = oa;
+        } else {
+            second = oa == kInvalidSlot ? fmaxf(second, os) : fmaxf(fmaxf(second, ob), os);
+        }
+    }
+}
+
+__device__ inline float2 BufF2(Rsrc r, uint32_t voff) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
+}
+
+// The 17 FP32 operands of action a for lane group gq, read from HBM (rescoring one candidate): the
+// same values TileOperands reads from LDS for that action's row.
+__device__ __forceinline__ void ActionOperands(Rsrc img_r, Rsrc tail_r, uint32_t a, uint32_t gq,
+                                               float (&ev)[kScoreSteps]) {
+    const uint32_t base = ((a >> 4) * uint32_t(kImgTileWords) + 4 * ((a & 15u) + 16 * gq)) * 4;
+    const v4f f0 = Buf4(img_r, base, 0);
+    const v4f f1 = Buf4(img_r, base + 1024, 0);
+    ev[0] = f0[0];
+    ev[1] = f0[1];
+    ev[2] = f0[2];
+    ev[3] = f0[3];
+    ev[4] = f1[0];
+    ev[5] = f1[1];
+    ev[6] = f1[2];
+    const uint32_t toff = (a * uint32_t(kTail) + kTailSteps * gq) * 4;
+#pragma unroll
+    for (int q = 0; q < kTailSteps / 2; ++q) {
+        const float2 v = BufF2(tail_r, toff + 8 * q);
+        ev[kFixedSteps + 2 * q] = v.x;
+        ev[kFixedSteps + 2 * q + 1] = v.y;
+    }
+    ev[kScoreSteps - 2] = gq == 3 ? 0.0f : ev[kScoreSteps - 2];
+    ev[kScoreSteps - 1] = gq == 3 ? 0.0f : ev[kScoreSteps - 1];
+}
+
+// BF16 screening bound.  With round-to-nearest-even conversions |bf16(x) - x| <= 2^-9 |x|, so each
+// product is off by at most 2^-8 (1 + 2^-10) |e p|; the FP32 accumulation errors of both the BF16
+// and the FP32 contraction (<= 96 and 68 roundings of 2^-24) fit in the 3% margin.  Hence
+// |S_bf16 - S_fp32| <= eps = 1.03 * 2^-8 * sum_d M_d |p_d| (+ a subnormal floor), M_d = max_a |E_ad|.
+constexpr float kBf16RelBound = 1.03f * 0x1p-8f;
+constexpr float kBf16AbsBound = 0x1p-120f;
+constexpr uint32_t kMaxCandidates = 16;  // per game; more within the window -> FP32 scan of the decision
+
+template <bool kTrace, bool kBf16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void EvaluateKernel(EvalArgs p) {
+    constexpr int kCT = kBf16 ? kChunkTilesBf16 : kChunkTiles;
+    using G = ChunkGeom<kCT>;
     // two chunk buffers as distinct objects, so a read of one never waits on transfers into the other
-    __shared__ __align__(16) uint8_t s_buf0[kChunkBufBytes];
-    __shared__ __align__(16) uint8_t s_buf1[kChunkBufBytes];
+    __shared__ __align__(16) uint8_t s_buf0[G::kBufBytes];
+    __shared__ __align__(16) uint8_t s_buf1[G::kBufBytes];
 
     const uint32_t n = blockIdx.x;
     const uint32_t slot = p.slot_indices[n];
@@ -379,7 +523,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
     const float* genome = reinterpret_cast<const float*>(p.slab + uint64_t(slot) * p.stride);
     const uint32_t tid = threadIdx.x, bd = blockDim.x;
 
-    const Rsrc enc_r = MakeRsrc(p.enc_t + uint64_t(n) * (kTurnFeatures * kEncOut), uint32_t(kTurnFeatures * kEncOut * 4));
+    const Rsrc enc_r = MakeRsrc(p.enc_t + uint64_t(n) * kEncTStride, uint32_t(kEncTStride * 4));
 
     const uint32_t W = p.cell_offsets[n + 1] - p.cell_offsets[n];
     if (W == 0) {
@@ -400,9 +544,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
     const TrunkLane tl = MakeTrunkLane(genome, j, gq);
     const uint32_t A = p.action_count;
     const uint32_t a_tiles = (A + 15) / 16;
-    const uint32_t chunks = (a_tiles + kChunkTiles - 1) / kChunkTiles;
+    const uint32_t chunks = (a_tiles + kCT - 1) / kCT;
     const Rsrc img_r = MakeRsrc(p.img, p.img_bytes);
     const Rsrc tail_r = MakeRsrc(genome + kPolicyParams, uint32_t(p.stride - kPolicyParams * 4));
+
+    auto issue = [&](uint8_t* buf, uint32_t c) __device__ {
+        IssueChunk(buf, img_r, tail_r, c, wave, waves, lane, G::kImgBytes, G::kTailBytes, G::kImgDma, G::kTailDma);
+    };
 
     const uint32_t per_round = gridDim.y * waves;
     const uint32_t first_tile = blockIdx.y * waves;
@@ -426,6 +574,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
         bool won = false;
         bool active = false;
         uint32_t decisions = 0;
+        uint32_t rescored = 0, fallbacks = 0;
         if (valid) {
             const uint32_t entry = fresh ? game : (game >> 1);
             const uint32_t variant = fresh ? 0u : 1u + (game & 1u);
@@ -472,13 +621,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
         for (int step = 0; step < kMaxTurns; ++step) {
             if (!__syncthreads_or(active ? 1 : 0)) break;  // the whole workgroup leaves together
             const bool wave_active = __any(active ? 1 : 0) != 0;
-            IssueChunk(s_buf0, img_r, tail_r, 0, wave, waves, lane);  // lands behind the trunk
+            issue(s_buf0, 0);  // lands behind the trunk
             float pol[kScoreSteps];
             if (wave_active) {
                 const int ttc = base_tc + step;  // turn count of every active game in this tile
                 if (step == 0 || (fresh && step == 1)) {
                     InitLayer0(acc, tl);
-                    if (fresh && step == 0) AddVirgin(acc, tl, true);
+                    if (fresh && step == 0) {  // acc += W0[:, 0] (virgin flag), from the contiguous copy
+#pragma unroll
+                        for (int t = 0; t < 16; ++t)
+                            acc[t] += Buf4(enc_r, uint32_t(kTurnFeatures * kEncOut + 16 * t + 4 * gq) * 4, 0);
+                    }
                     folded = 0;
                 }
                 // ---- fold newly occupied turn slots: acc += W0[:, 1+64*slot ...] . enc(turn slot)
@@ -494,44 +647,149 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
 #pragma unroll
                 for (int s = 0; s < kScoreSteps; ++s) pol[s] = 0.0f;
             }
-            WaitChunk();
-            __syncthreads();
 
-            // ---- scoring: S = E . P over all actions; masked argmax.  Chunk c + 1 moves into the
-            // other buffer while chunk c is scored.
-            ArgmaxLane am = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot};
-            auto score_chunk = [&](const uint8_t* buf, uint8_t* next, uint32_t c) {
-                if (c + 1 < chunks) IssueChunk(next, img_r, tail_r, c + 1, wave, waves, lane);
-                if (wave_active) {
+            // ---- scoring: S = E . P over all actions, masked argmax (lowest index on ties)
+            float best = -__builtin_inff(), second = -__builtin_inff();
+            uint32_t besta = kInvalidSlot;
+            if constexpr (!kBf16) {
+                ArgmaxLane am = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot};
+                StreamChunks<kCT>(s_buf0, s_buf1, chunks, wave_active, issue,
+                                  [&](const uint32_t* img_t, const float* tail_t, uint32_t at) __device__ {
+                                      ScoreTile<kTrace>(img_t, tail_t, lane, j, gq, pol, at * 16 + 4 * gq, A, hist,
+                                                        am);
+                                  });
+                best = am.best;
+                second = am.second;
+                besta = am.besta;
+                ReduceArgmax(best, second, besta);
+            } else {
+                // BF16 screening with exact FP32 rescoring: (1) BF16 masked argmax b16 over all
+                // actions; (2) every unmasked action with S_bf16 >= b16 - 2 eps joins the game's
+                // candidate list; (3) the candidates are rescored with the FP32 contraction itself, so
+                // the FP32 argmax (which always lies in the window) and its score are reproduced bit
+                // for bit.  Decisions the window cannot settle take the FP32 scan.
+                __shared__ uint32_t s_cand[4][16][kMaxCandidates];
+                __shared__ uint32_t s_ncand[4][16];
+                __shared__ float s_mtail[4][4 * kTailSteps];
+                const PolicyBf16 pb = PackBf16(pol);
+                const bool measure = step == 0;  // per-dim tail magnitudes, once per round
+                float tmax[kTailSteps];
 #pragma unroll
-                    for (int k = 0; k < kChunkTiles; ++k)
-                        ScoreTile<kTrace>(reinterpret_cast<const uint32_t*>(buf) + k * kImgTileWords,
-                                          reinterpret_cast<const float*>(buf + kTailRegion) + k * 16 * kTail, lane,
-                                          j, gq, pol, (c * kChunkTiles + k) * 16 + 4 * gq, A, hist, am);
-                }
-                WaitChunk();
-                __syncthreads();
-            };
-            for (uint32_t c = 0; c < chunks; c += 2) {
-                score_chunk(s_buf0, s_buf1, c);
-                if (c + 1 < chunks) score_chunk(s_buf1, s_buf0, c + 1);
-            }
-            float best = am.best, second = am.second;
-            uint32_t besta = am.besta;
-            // reduce over the four lane groups that share game column j (lower index wins ties)
+                for (int t = 0; t < kTailSteps; ++t) tmax[t] = 0.0f;
+                ArgmaxLane am = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot};
+                StreamChunks<kCT>(s_buf0, s_buf1, chunks, wave_active, issue,
+                                  [&](const uint32_t* img_t, const float* tail_t, uint32_t at) __device__ {
+                                      float ev[kScoreSteps];
+                                      TileOperands(img_t, tail_t, lane, j, gq, ev);
+                                      if (measure) {
+                                          const bool row_ok = at * 16 + j < A;
 #pragma unroll
-            for (int off = 16; off <= 32; off <<= 1) {
-                const float ob = __shfl_xor(best, off);
-                const uint32_t oa = __shfl_xor(besta, off);
-                const float os = __shfl_xor(second, off);
-                const bool take = oa != kInvalidSlot && (besta == kInvalidSlot || ob > best || (ob == best && oa < besta));
-                if (take) {
-                    second = besta == kInvalidSlot ? fmaxf(second, os) : fmaxf(fmaxf(second, best), os);
-                    best = ob;
-                    besta = oa;
-                } else {
-                    second = oa == kInvalidSlot ? fmaxf(second, os) : fmaxf(fmaxf(second, ob), os);
+                                          for (int t = 0; t < kTailSteps; ++t)
+                                              tmax[t] = row_ok ? fmaxf(tmax[t], fabsf(ev[kFixedSteps + t])) : tmax[t];
+                                      }
+                                      const v4u cn = *reinterpret_cast<const v4u*>(img_t + 512 + 4 * gq);
+                                      ArgmaxUpdate<false>(Bf16Scores(ev, pb), cn, at * 16 + 4 * gq, A, hist, am);
+                                  });
+                float b16 = am.best, unused = am.second;
+                uint32_t b16a = am.besta;
+                ReduceArgmax(b16, unused, b16a);
+                if (measure && wave_active) {
+#pragma unroll
+                    for (int t = 0; t < kTailSteps; ++t)
+#pragma unroll
+                        for (int off = 1; off < 16; off <<= 1) tmax[t] = fmaxf(tmax[t], __shfl_xor(tmax[t], off));
+                    if (j == 0)
+#pragma unroll
+                        for (int t = 0; t < kTailSteps; ++t) s_mtail[wave][kTailSteps * gq + t] = tmax[t];
+                    __builtin_amdgcn_wave_barrier();
                 }
+                // the bound: sum over this lane group's steps of M_d |p_d|, then over the 4 groups
+                float esum = 0.0f, pabs = 0.0f;
+#pragma unroll
+                for (int s = 0; s < kScoreSteps; ++s) {
+                    const int d = ScoreDim(s, int(gq));
+                    const float m = d < 0 ? 0.0f : (d < kFixed ? p.fixed_absmax[d] : s_mtail[wave][d - kFixed]);
+                    esum += m * fabsf(pol[s]);
+                    pabs += m + fabsf(pol[s]);
+                }
+                esum += __shfl_xor(esum, 16);
+                esum += __shfl_xor(esum, 32);
+                pabs += __shfl_xor(pabs, 16);
+                pabs += __shfl_xor(pabs, 32);
+                const float eps = kBf16RelBound * esum + kBf16AbsBound * (pabs + 1.0f);
+                const float thr = b16 - 2.0f * eps;
+                bool fail = active && (p.force_fallback != 0 || b16a == kInvalidSlot || !(fabsf(b16) <= 3.0e38f) ||
+                                       !(eps <= 1.0e37f));
+                const bool want = active && !fail;
+                if (lane < 16) s_ncand[wave][lane] = 0u;
+                __builtin_amdgcn_wave_barrier();
+                issue(s_buf0, 0);
+                StreamChunks<kCT>(s_buf0, s_buf1, chunks, __any(want ? 1 : 0) != 0, issue,
+                                  [&](const uint32_t* img_t, const float* tail_t, uint32_t at) __device__ {
+                                      float ev[kScoreSteps];
+                                      TileOperands(img_t, tail_t, lane, j, gq, ev);
+                                      const v4f sc = Bf16Scores(ev, pb);
+                                      const float m4 = fmaxf(fmaxf(sc[0], sc[1]), fmaxf(sc[2], sc[3]));
+                                      if (__any((want && m4 >= thr) ? 1 : 0)) {
+                                          const v4u cn = *reinterpret_cast<const v4u*>(img_t + 512 + 4 * gq);
+#pragma unroll
+                                          for (int r = 0; r < 4; ++r) {
+                                              const uint32_t a = at * 16 + 4 * gq + uint32_t(r);
+                                              const uint32_t c = cn[r];
+                                              const bool masked = (c == hist[0]) | (c == hist[1]) | (c == hist[2]) |
+                                                                  (c == hist[3]) | (c == hist[4]);
+                                              if (want && sc[r] >= thr && a < A && !masked) {
+                                                  const uint32_t k = atomicAdd(&s_ncand[wave][j], 1u);
+                                                  if (k < kMaxCandidates) s_cand[wave][j][k] = a;
+                                              }
+                                          }
+                                      }
+                                  });
+                const uint32_t ncand = s_ncand[wave][j];
+                fail = fail || (want && (ncand == 0 || ncand > kMaxCandidates));
+                const bool resc = active && !fail;
+                uint32_t kmax = resc ? ncand : 0u;
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) kmax = max(kmax, uint32_t(__shfl_xor(int(kmax), off)));
+                kmax = __builtin_amdgcn_readfirstlane(kmax);
+                for (uint32_t k = 0; k < kmax; ++k) {
+                    // candidate k of game i sits in MFMA row i; the diagonal D[j][j] is game j's score
+                    const uint32_t cand = (resc && k < ncand) ? s_cand[wave][j][k] : 0u;
+                    float ev[kScoreSteps];
+                    ActionOperands(img_r, tail_r, cand, gq, ev);
+                    const v4f sc = Fp32Scores(ev, pol);
+                    const float mine = (j & 3u) == 0 ? sc[0] : ((j & 3u) == 1 ? sc[1] : ((j & 3u) == 2 ? sc[2] : sc[3]));
+                    const float v = __shfl(mine, int(j + 16 * (j >> 2)));
+                    if (resc && k < ncand) {
+                        const bool better = besta == kInvalidSlot || v > best || (v == best && cand < besta);
+                        if (kTrace) {
+                            const float demoted = besta == kInvalidSlot ? second : fmaxf(second, best);
+                            second = better ? demoted : fmaxf(second, v);
+                        }
+                        best = better ? v : best;
+                        besta = better ? cand : besta;
+                    }
+                }
+                if (resc && gq == 0) rescored += ncand;
+                const bool need = active && fail;
+                if (__syncthreads_or(need ? 1 : 0)) {
+                    issue(s_buf0, 0);
+                    ArgmaxLane af = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot};
+                    StreamChunks<kCT>(s_buf0, s_buf1, chunks, __any(need ? 1 : 0) != 0, issue,
+                                      [&](const uint32_t* img_t, const float* tail_t, uint32_t at) __device__ {
+                                          ScoreTile<kTrace>(img_t, tail_t, lane, j, gq, pol, at * 16 + 4 * gq, A, hist,
+                                                            af);
+                                      });
+                    float fb = af.best, fs = af.second;
+                    uint32_t fa = af.besta;
+                    ReduceArgmax(fb, fs, fa);
+                    if (need) {
+                        best = fb;
+                        second = fs;
+                        besta = fa;
+                    }
+                }
+                if (need && gq == 0) ++fallbacks;
             }
 
             if (kTrace && active && p.policy_trace != nullptr) {
@@ -586,11 +844,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
             score += __shfl_xor(score, off);
             dec += __shfl_xor(dec, off);
             enc += __shfl_xor(enc, off);
+            if (kBf16) {
+                rescored += __shfl_xor(rescored, off);
+                fallbacks += __shfl_xor(fallbacks, off);
+            }
         }
         if (lane == 0 && tile < tiles) {
             atomicAdd(p.raw_fitness + n, score);
             atomicAdd(p.counters + 0, (unsigned long long)dec);
             atomicAdd(p.counters + 1, (unsigned long long)enc);
+            if (kBf16) {
+                atomicAdd(p.counters + 2, (unsigned long long)rescored);
+                atomicAdd(p.counters + 3, (unsigned long long)fallbacks);
+            }
         }
     }
 }
@@ -1399,14 +1665,21 @@ void LaunchTransposeEncoder(const uint8_t* slab, uint64_t stride, const uint32_t
     hipLaunchKernelGGL(TransposeEncoderKernel, dim3(n), dim3(256), 0, s, slab, stride, slots, slot_count, enc_t);
 }
 
-void LaunchEvaluate(const EvalArgs& a, uint32_t n_genomes, uint32_t waves, uint32_t chunks, bool trace,
+void LaunchEvaluate(const EvalArgs& a, uint32_t n_genomes, uint32_t waves, uint32_t chunks, bool trace, bool bf16,
                     hipStream_t s) {
     const dim3 grid(n_genomes, chunks);
     const dim3 block(64 * waves);
-    if (trace)
-        hipLaunchKernelGGL(EvaluateKernel<true>, grid, block, 0, s, a);
-    else
-        hipLaunchKernelGGL(EvaluateKernel<false>, grid, block, 0, s, a);
+    if (bf16) {
+        if (trace)
+            hipLaunchKernelGGL((EvaluateKernel<true, true>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((EvaluateKernel<false, true>), grid, block, 0, s, a);
+    } else {
+        if (trace)
+            hipLaunchKernelGGL((EvaluateKernel<true, false>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((EvaluateKernel<false, false>), grid, block, 0, s, a);
+    }
 }
 
 void LaunchFinalize(const float* raw, const uint32_t* local_w, uint32_t n, float* fitness, uint32_t* evals,

@@ -28,10 +28,12 @@ struct EvalArgs {
     const uint32_t* canon;         // [catalog] lowest index holding the same word
     const uint32_t* img;           // [catalog tiles][kImgTileWords] scoring image
     uint32_t img_bytes;
-    const float* enc_t;            // [N][145][64] transposed encoders of the current generation
+    const float* enc_t;            // [N][kEncTStride]: transposed encoder, then W0[:, 0], per genome
+    const float* fixed_absmax;     // [26] max over the catalog of |fixed feature d| (BF16 bound)
+    uint32_t force_fallback;       // BF16 mode: settle every decision with the FP32 scan (diagnostic)
     float* raw_fitness;            // [N] summed episode scores
     uint32_t* local_w;             // [N]
-    unsigned long long* counters;  // [0] decisions, [1] encoded turns, [2] games
+    unsigned long long* counters;  // [0] decisions, [1] encoded turns, [2] BF16 candidates rescored, [3] FP32 fallbacks
     int* status;
     GameTrace* game_trace;         // optional (trace mode)
     float* policy_trace;           // optional: [games][6][64]
@@ -136,7 +138,7 @@ struct InjectArgs {
 void LaunchInject(const InjectArgs& a, hipStream_t s);
 void LaunchTransposeEncoder(const uint8_t* slab, uint64_t stride, const uint32_t* slots, uint32_t n,
                             uint32_t slot_count, float* enc_t, hipStream_t s);
-void LaunchEvaluate(const EvalArgs& a, uint32_t n_genomes, uint32_t waves, uint32_t chunks, bool trace,
+void LaunchEvaluate(const EvalArgs& a, uint32_t n_genomes, uint32_t waves, uint32_t chunks, bool trace, bool bf16,
                     hipStream_t s);
 void LaunchFinalize(const float* raw, const uint32_t* local_w, uint32_t n, float* fitness, uint32_t* evals,
                     uint8_t* has, int* status, hipStream_t s);

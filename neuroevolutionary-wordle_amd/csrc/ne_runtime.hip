@@ -81,6 +81,7 @@ struct ne_runtime {
     uint32_t* codes = nullptr;
     uint32_t* canon = nullptr;  // lowest catalog index holding the same word
     uint32_t* img = nullptr;    // scoring image, kImgTileWords per tile of 16 words
+    float* fixed_absmax = nullptr;  // [26] max over the catalog of |fixed feature|
     uint64_t img_bytes = 0;
     uint64_t catalog_count = 0;
     std::vector<uint8_t> h_catalog;
@@ -95,6 +96,7 @@ struct ne_runtime {
 
     // trace
     uint32_t trace_flags = 0;
+    uint32_t eval_mode = NE_EVAL_FP32;
     GameTrace* d_trace = nullptr;
     float* d_policy_trace = nullptr;
     uint64_t* d_game_offsets = nullptr;
@@ -389,6 +391,9 @@ static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard
     a.canon = rt->canon;
     a.img = rt->img;
     a.img_bytes = uint32_t(rt->img_bytes);
+    a.fixed_absmax = rt->fixed_absmax;
+    a.force_fallback = (rt->eval_mode & NE_EVAL_FORCE_FALLBACK) ? 1u : 0u;
+    const bool bf16 = (rt->eval_mode & 0xFFu) == NE_EVAL_BF16_RESCORED;
     a.raw_fitness = rt->raw;
     a.local_w = rt->local_w;
     a.counters = rt->counters;
@@ -414,7 +419,7 @@ static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard
         }
     }
     const uint32_t chunks = uint32_t((max_tiles + waves - 1) / waves);
-    if (rt->enc_t == nullptr && !Alloc(&rt->enc_t, rt->max_gen * uint64_t(kTurnFeatures * kEncOut)))
+    if (rt->enc_t == nullptr && !Alloc(&rt->enc_t, rt->max_gen * kEncTStride))
         return Fail(rt, NE_DEVICE_FAILURE);
     a.enc_t = rt->enc_t;
     EventBegin(rt, 8);
@@ -422,7 +427,7 @@ static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard
                            rt->enc_t, rt->stream);
     EventEnd(rt, 8);
     EventBegin(rt, 0);
-    LaunchEvaluate(a, uint32_t(rt->cur_size), waves, chunks, trace, rt->stream);
+    LaunchEvaluate(a, uint32_t(rt->cur_size), waves, chunks, trace, bf16, rt->stream);
     EventEnd(rt, 0);
     if (!Ok(hipGetLastError())) return Fail(rt, NE_DEVICE_FAILURE);
     int st = Finish(rt);
@@ -444,6 +449,8 @@ static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard
     rt->times.prepare_ms = EventMs(rt, 8);
     rt->times.finalize_ms = EventMs(rt, 1);
     rt->times.decisions = counters[0];
+    rt->times.rescored = counters[2];
+    rt->times.fallbacks = counters[3];
     rt->times.games = total_games;
     // encoded turns: a game with d >= 1 decisions encodes every turn present at its last decision
     rt->times.encoded_turns = counters[1];
@@ -803,6 +810,7 @@ void ne_destroy(ne_runtime* rt) {
     Free(rt->codes);
     Free(rt->canon);
     Free(rt->img);
+    Free(rt->fixed_absmax);
     Free(rt->owner);
     Free(rt->move_src);
     Free(rt->d_survivors);
@@ -837,6 +845,9 @@ int ne_upload_catalog(ne_runtime* rt, const uint8_t* letters, uint64_t word_coun
         // MaterializeFixedWordFeatures (output_embedding.hpp:179-205): count, or -1 when absent
         for (int l = 0; l < kAlpha; ++l) fixed[a * kFixed + l] = counts[l] == 0 ? -1.0f : float(counts[l]);
     }
+    float absmax[kFixed] = {0.0f};
+    for (uint64_t a = 0; a < word_count; ++a)
+        for (int d = 0; d < kFixed; ++d) absmax[d] = std::max(absmax[d], std::fabs(fixed[a * kFixed + d]));
     // scoring image: lane i + 16g of a tile holds steps 0..6 of action row i for lane group g
     const uint64_t tiles = (word_count + 15) / 16;
     std::vector<uint32_t> img(tiles * kImgTileWords, 0u);
@@ -856,7 +867,11 @@ int ne_upload_catalog(ne_runtime* rt, const uint8_t* letters, uint64_t word_coun
     Free(rt->codes);
     Free(rt->canon);
     Free(rt->img);
+    Free(rt->fixed_absmax);
     rt->img_bytes = tiles * kImgTileBytes;
+    if (!Alloc(&rt->fixed_absmax, kFixed) ||
+        !Ok(hipMemcpy(rt->fixed_absmax, absmax, sizeof(absmax), hipMemcpyHostToDevice)))
+        return Fail(rt, NE_DEVICE_FAILURE);
     if (!Alloc(&rt->codes, word_count) || !Alloc(&rt->canon, word_count) || !Alloc(&rt->img, img.size()) ||
         !Ok(hipMemcpy(rt->codes, codes.data(), word_count * 4, hipMemcpyHostToDevice)) ||
         !Ok(hipMemcpy(rt->canon, canon.data(), word_count * 4, hipMemcpyHostToDevice)) ||
@@ -1112,6 +1127,15 @@ const char* ne_status_string(int status) {
 int ne_get_stage_times(ne_runtime* rt, ne_stage_times* out) {
     if (!rt || !out) return NE_INVALID_ARGUMENT;
     *out = rt->times;
+    return NE_OK;
+}
+
+int ne_set_eval_precision(ne_runtime* rt, uint32_t mode) {
+    if (!rt) return NE_INVALID_ARGUMENT;
+    const uint32_t base = mode & 0xFFu;
+    if ((base != NE_EVAL_FP32 && base != NE_EVAL_BF16_RESCORED) || (mode & ~0x1FFu) != 0)
+        return Fail(rt, NE_INVALID_ARGUMENT);
+    rt->eval_mode = mode;
     return NE_OK;
 }
 

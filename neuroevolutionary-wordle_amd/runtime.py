@@ -22,6 +22,7 @@ STATUS_NAMES = {
     18: "unsupported configuration", 19: "callback failed",
 }
 
+EVAL_FP32, EVAL_BF16_RESCORED, EVAL_FORCE_FALLBACK = 0, 1, 0x100
 TRACE_GAMES = 1
 TRACE_POLICY = 2
 TRACE_XOVER = 4
@@ -87,7 +88,8 @@ class StageTimes(C.Structure):
     _fields_ = [("evaluate_ms", C.c_float), ("finalize_ms", C.c_float), ("plan_ms", C.c_float),
                 ("assemble_ms", C.c_float), ("repack_ms", C.c_float), ("inject_ms", C.c_float),
                 ("bookkeeping_ms", C.c_float), ("prepare_ms", C.c_float), ("decisions", C.c_uint64),
-                ("encoded_turns", C.c_uint64), ("games", C.c_uint64), ("children", C.c_uint64)]
+                ("encoded_turns", C.c_uint64), ("games", C.c_uint64), ("children", C.c_uint64),
+                ("rescored", C.c_uint64), ("fallbacks", C.c_uint64)]
 
 
 GAME_TRACE_DTYPE = np.dtype([("guesses", "<u4", 6), ("best", "<f4", 6), ("second", "<f4", 6),
@@ -107,7 +109,7 @@ EXPORTED_SYMBOLS = [
     "ne_evaluate", "ne_read_summary", "ne_advance_generation", "ne_get_slab_info", "ne_download_slot",
     "ne_upload_slot", "ne_download_generation", "ne_download_plan", "ne_download_local_word_counts",
     "ne_read_status", "ne_status_string", "ne_get_stage_times", "ne_set_trace", "ne_download_trace",
-    "ne_build_shard_csr",
+    "ne_build_shard_csr", "ne_set_eval_precision",
 ]
 
 _lib = None
@@ -144,6 +146,7 @@ def load_library(path=LIB_PATH):
         "ne_status_string": (C.c_char_p, [i32]),
         "ne_get_stage_times": (i32, [p, C.POINTER(StageTimes)]),
         "ne_set_trace": (i32, [p, u32]),
+        "ne_set_eval_precision": (i32, [p, u32]),
         "ne_download_trace": (i32, [p, u32, p, u64, C.POINTER(u64)]),
         "ne_build_shard_csr": (i32, [C.POINTER(WordCounts), C.POINTER(ShardReleaseHistory), u64, u64, u64, u64,
                                      p, p, u64, C.POINTER(u64)]),
@@ -317,6 +320,10 @@ class SlabRuntime:
 
     def set_trace(self, flags):
         self._check("ne_set_trace", self.lib.ne_set_trace(self.h, flags))
+
+    def set_eval_precision(self, mode):
+        """EVAL_FP32, or EVAL_BF16_RESCORED (optionally | EVAL_FORCE_FALLBACK)."""
+        self._check("ne_set_eval_precision", self.lib.ne_set_eval_precision(self.h, mode))
 
     def download_trace(self, kind, count):
         if kind == TRACE_GAMES:

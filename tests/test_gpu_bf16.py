@@ -1,0 +1,118 @@
+"""The BF16 screening form against the FP32 form on the same genomes.
+
+Bar: BF16 screening + FP32 rescoring (ne_set_eval_precision(NE_EVAL_BF16_RESCORED)) takes exactly the
+FP32 form's decisions: identical guesses, chosen scores equal bit for bit (the rescoring runs the FP32
+contraction itself), identical fitness.  The window bound that guarantees this is proven in the kernel
+(|S_bf16 - S_fp32| <= 1.03 * 2^-8 * sum_d max_a|E_ad| |p_d|); these tests check the outcome.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _runtime(pkg, side, A, catalog, offsets, ids, seed=9, var=1.0, tail=0.1):
+    rtm = pkg.runtime
+    stride = rtm.slot_stride_bytes(A)
+    n = side * side
+    rt = pkg.SlabRuntime(slab_bytes=2 * n * stride, generation_bytes=n * stride, action_count=A, grid_columns=side)
+    rt.upload_catalog(catalog)
+    rt.upload_secrets_csr(offsets, ids)
+    rt.bootstrap(n, seed=seed, variance_scale=var, tail_std=tail)
+    return rt
+
+
+def _evaluate(pkg, rt, A, games, mode):
+    rtm = pkg.runtime
+    rt.set_eval_precision(mode)
+    rt.set_trace(rtm.TRACE_GAMES)
+    rt.evaluate(rtm.word_counts(A, A))
+    _, fit, _, _ = rt.generation(0)
+    tr = rt.download_trace(rtm.TRACE_GAMES, games)
+    st = rt.stage_times()
+    return fit.copy(), tr, st
+
+
+def _same_decisions(pkg, rt, A, games, mode=None):
+    rtm = pkg.runtime
+    f32, t32, s32 = _evaluate(pkg, rt, A, games, rtm.EVAL_FP32)
+    f16, t16, s16 = _evaluate(pkg, rt, A, games, rtm.EVAL_BF16_RESCORED if mode is None else mode)
+    assert np.array_equal(t16["decisions"], t32["decisions"])
+    assert np.array_equal(t16["guesses"], t32["guesses"])
+    played = t32["guesses"] != 0xFFFFFFFF
+    assert np.array_equal(t16["best"][played].view(np.uint32), t32["best"][played].view(np.uint32))
+    assert np.array_equal(f16.view(np.uint32), f32.view(np.uint32))
+    assert s16.decisions == s32.decisions
+    return s16
+
+
+def test_bf16_form_takes_the_fp32_decisions_grid16(pkg):
+    wl = pkg.synth.workload("grid16_a512")
+    rt = _runtime(pkg, 16, 512, wl["catalog"], wl["offsets"], wl["ids"])
+    try:
+        st = _same_decisions(pkg, rt, 512, 3 * int(wl["offsets"][-1]))
+        assert st.rescored >= st.decisions - st.fallbacks
+        assert st.fallbacks <= st.decisions // 100, (st.fallbacks, st.decisions)
+        print(f"decisions {st.decisions}, candidates rescored {st.rescored} "
+              f"({st.rescored / max(1, st.decisions):.2f}/decision), fallbacks {st.fallbacks}")
+    finally:
+        rt.close()
+
+
+def test_bf16_form_takes_the_fp32_decisions_4739_actions(pkg):
+    A, side = 4739, 4
+    catalog = pkg.synth.make_catalog(A)
+    secrets = pkg.synth.sample_secrets(A, 64)
+    offsets, ids = pkg.synth.block_csr(side, side, 2, secrets)
+    rt = _runtime(pkg, side, A, catalog, offsets, ids, seed=33)
+    try:
+        st = _same_decisions(pkg, rt, A, 3 * int(offsets[-1]))
+        assert st.fallbacks <= st.decisions // 50, (st.fallbacks, st.decisions)
+        print(f"decisions {st.decisions}, rescored {st.rescored}, fallbacks {st.fallbacks}")
+    finally:
+        rt.close()
+
+
+def test_bf16_form_duplicate_words_and_ragged_cells(pkg):
+    A, side = 365, 4
+    rng = np.random.default_rng(0xD0B1E)
+    catalog = pkg.synth.make_catalog(A, seed=0xD0B1E).copy()
+    for k in range(40, A, 9):
+        catalog[k] = catalog[int(rng.integers(0, k))]
+    widths = np.array([2, 4, 5, 16, 17, 33, 70, 6, 9, 48, 2, 64, 7, 31, 8, 10])
+    offsets = np.concatenate([[0], np.cumsum(widths)]).astype(np.uint32)
+    firsts = np.array(sorted({bytes(w): i for i, w in reversed(list(enumerate(catalog)))}.values()))
+    ids = np.concatenate([rng.choice(firsts, size=w, replace=False) for w in widths]).astype(np.uint32)
+    rt = _runtime(pkg, side, A, catalog, offsets, ids, seed=21)
+    try:
+        _same_decisions(pkg, rt, A, 3 * int(offsets[-1]))
+    finally:
+        rt.close()
+
+
+def test_bf16_forced_fallback_and_all_tie_genomes(pkg):
+    """Every decision through the FP32 fallback scan; then zero genomes (every score 0, all actions
+    tie: the window holds them all and the decision must fall back) in the BF16 form."""
+    wl = pkg.synth.workload("grid16_a512")
+    rtm = pkg.runtime
+    rt = _runtime(pkg, 16, 512, wl["catalog"], wl["offsets"], wl["ids"])
+    try:
+        st = _same_decisions(pkg, rt, 512, 3 * int(wl["offsets"][-1]),
+                             mode=rtm.EVAL_BF16_RESCORED | rtm.EVAL_FORCE_FALLBACK)
+        assert st.fallbacks == st.decisions and st.rescored == 0
+        rt.bootstrap(256, seed=1, variance_scale=0.0, tail_std=0.0)
+        st = _same_decisions(pkg, rt, 512, 3 * int(wl["offsets"][-1]))
+        assert st.fallbacks == st.decisions
+    finally:
+        rt.close()
+
+
+def test_eval_precision_rejects_unknown_modes(pkg):
+    wl = pkg.synth.workload("grid16_a512")
+    rt = _runtime(pkg, 16, 512, wl["catalog"], wl["offsets"], wl["ids"])
+    try:
+        for bad in (2, 0x200, 0x101 | 0x1000):
+            with pytest.raises(pkg.runtime.RuntimeError_):
+                rt.set_eval_precision(bad)
+    finally:
+        rt.close()
