@@ -388,10 +388,17 @@ __device__ __forceinline__ v4f Fp32Scores(const float (&ev)[kScoreSteps], const 
 
 // Masked argmax over the 4 rows (actions abase..abase+3) this lane owns in a tile; candidates arrive
 // in increasing action order, the first unmasked one is taken whatever its score, later ones only
-// when strictly greater (evaluation_ops.cuh:429-445).
+// when strictly greater (evaluation_ops.cuh:429-445).  Without traces the per-row work runs only
+// when some lane of the wave can change: a lane with a candidate changes only if one of its four
+// scores beats its best (a tie never displaces the earlier, lower index).
 template <bool kTrace>
-__device__ __forceinline__ void ArgmaxUpdate(const v4f& sc, const v4u& cn, uint32_t abase, uint32_t A,
+__device__ __forceinline__ void ArgmaxUpdate(const v4f& sc, const uint32_t* cn_ptr, uint32_t abase, uint32_t A,
                                              const uint32_t (&hist)[5], ArgmaxLane& am) {
+    if (!kTrace) {
+        const float m4 = fmaxf(fmaxf(sc[0], sc[1]), fmaxf(sc[2], sc[3]));
+        if (!__any((am.besta == kInvalidSlot || m4 > am.best) ? 1 : 0)) return;
+    }
+    const v4u cn = *reinterpret_cast<const v4u*>(cn_ptr);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const uint32_t a = abase + uint32_t(r);
@@ -417,8 +424,7 @@ __device__ __forceinline__ void ScoreTile(const uint32_t* img_tile, const float*
                                           const uint32_t (&hist)[5], ArgmaxLane& am) {
     float ev[kScoreSteps];
     TileOperands(img_tile, tail_tile, lane, j, gq, ev);
-    const v4u cn = *reinterpret_cast<const v4u*>(img_tile + 512 + 4 * gq);
-    ArgmaxUpdate<kTrace>(Fp32Scores(ev, pol), cn, abase, A, hist, am);
+    ArgmaxUpdate<kTrace>(Fp32Scores(ev, pol), img_tile + 512 + 4 * gq, abase, A, hist, am);
 }
 
 // ---- BF16 screening -----------------------------------------------------------------------------
@@ -687,8 +693,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
                                           for (int t = 0; t < kTailSteps; ++t)
                                               tmax[t] = row_ok ? fmaxf(tmax[t], fabsf(ev[kFixedSteps + t])) : tmax[t];
                                       }
-                                      const v4u cn = *reinterpret_cast<const v4u*>(img_t + 512 + 4 * gq);
-                                      ArgmaxUpdate<false>(Bf16Scores(ev, pb), cn, at * 16 + 4 * gq, A, hist, am);
+                                      ArgmaxUpdate<false>(Bf16Scores(ev, pb), img_t + 512 + 4 * gq, at * 16 + 4 * gq, A,
+                                                          hist, am);
                                   });
                 float b16 = am.best, unused = am.second;
                 uint32_t b16a = am.besta;

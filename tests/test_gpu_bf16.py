@@ -116,3 +116,39 @@ def test_eval_precision_rejects_unknown_modes(pkg):
                 rt.set_eval_precision(bad)
     finally:
         rt.close()
+
+
+@pytest.mark.parametrize("dups", [False, True])
+def test_untraced_evaluations_equal_traced_ones(pkg, dups):
+    """The untraced kernels skip the per-row argmax work when no lane of a wave can change: their
+    fitness and decision counts must equal the traced kernels' (which do every row), in both forms."""
+    rtm = pkg.runtime
+    if dups:
+        A, side = 365, 4
+        rng = np.random.default_rng(5)
+        catalog = pkg.synth.make_catalog(A, seed=5).copy()
+        for k in range(30, A, 7):
+            catalog[k] = catalog[int(rng.integers(0, k))]
+        widths = np.array([2, 4, 5, 16, 17, 33, 70, 6, 9, 48, 2, 64, 7, 31, 8, 10])
+        offsets = np.concatenate([[0], np.cumsum(widths)]).astype(np.uint32)
+        firsts = np.array(sorted({bytes(w): i for i, w in reversed(list(enumerate(catalog)))}.values()))
+        ids = np.concatenate([rng.choice(firsts, size=w, replace=False) for w in widths]).astype(np.uint32)
+    else:
+        wl = pkg.synth.workload("grid16_a512")
+        A, side, catalog, offsets, ids = 512, 16, wl["catalog"], wl["offsets"], wl["ids"]
+    rt = _runtime(pkg, side, A, catalog, offsets, ids, seed=41)
+    try:
+        results = []
+        for mode in (rtm.EVAL_FP32, rtm.EVAL_BF16_RESCORED):
+            for traced in (True, False):
+                rt.set_eval_precision(mode)
+                rt.set_trace(rtm.TRACE_GAMES if traced else 0)
+                rt.evaluate(rtm.word_counts(A, A))
+                _, fit, _, _ = rt.generation(0)
+                results.append((fit.copy(), rt.stage_times().decisions))
+        f0, d0 = results[0]
+        for f, d in results[1:]:
+            assert np.array_equal(f.view(np.uint32), f0.view(np.uint32))
+            assert d == d0
+    finally:
+        rt.close()
