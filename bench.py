@@ -3,7 +3,8 @@
 A step = one full generation (ne_advance_generation): evaluate every genome's games, plan parents,
 assemble children (crossover + mutation), swap.  Workload: 64x64 toroidal grid (4,096 genomes),
 4,739 synthetic actions, 2,048 secrets in 64 block shards of 32 -> 393,216 games per generation.
-Prints one JSON line.
+The plain FP32 form is `value`; the BF16-screened form with exact FP32 rescoring (same decisions,
+same fitness) is timed over the same generations and reported beside it.  Prints one JSON line.
 """
 import argparse
 import json
@@ -24,6 +25,19 @@ WORKLOAD = "grid64_a4739"
 def algorithmic_eval_flops(decisions, encoded_turns, A):
     # per decision: 2*(321*256 + 256*128 + 128*64) + 128*A ; per encoded turn (145->64): 2*145*64
     return decisions * (246272 + 128 * A) + encoded_turns * 18560
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "evaluate_grid64_a4739_traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per evaluation launch from the committed FETCH_SIZE / WRITE_SIZE measurement of this
+    workload (scaled by the plain-copy calibration recorded with it), or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        t = json.load(f)
+    return t["fetch_bytes"] * t["fetch_scale"] + t["write_bytes"] * t["write_scale"]
 
 
 def cpu_baseline(pkg, rt, wl, A, seconds=12.0):
@@ -64,6 +78,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fp32-only", action="store_true", help="skip the BF16-screened form")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="first run one plain device-to-device copy of BYTES (a reference kernel of known "
                          "traffic for hardware byte counters)")
@@ -106,26 +121,50 @@ def main():
         print(f"[bench] cpu baseline {cpu['value']:.1f} game-evals/s on {cpu['cores']} threads", file=sys.stderr,
               flush=True)
 
-    gen = 0
-    for _ in range(args.warmup):
-        rt.advance(7 + 2 + gen, counts, cfg)
-        gen += 1
-    eval_ms, flops, decisions = [], [], 0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        rt.advance(7 + 2 + gen, counts, cfg)
-        gen += 1
-        st = rt.stage_times()
-        eval_ms.append(st.evaluate_ms)
-        flops.append(algorithmic_eval_flops(st.decisions, st.encoded_turns, A))
-        decisions += st.decisions
-    elapsed = time.perf_counter() - t0
-    st = rt.stage_times()
-    value = games_per_gen * args.steps / elapsed
-    eval_avg_ms = float(np.mean(eval_ms))
-    achieved = float(np.mean(flops)) / (eval_avg_ms * 1e-3) / 1e12
-    summary = rt.summary()
+    def run_form(mode):
+        """Bootstraps the same generation 0 and times `steps` generation steps in one form."""
+        rt.set_eval_precision(mode)
+        rt.bootstrap(n, seed=7, variance_scale=1.0, tail_std=0.1)
+        gen = 0
+        for _ in range(args.warmup):
+            rt.advance(7 + 2 + gen, counts, cfg)
+            gen += 1
+        eval_ms, flops, decisions, rescored, fallbacks = [], [], 0, 0, 0
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            rt.advance(7 + 2 + gen, counts, cfg)
+            gen += 1
+            st = rt.stage_times()
+            eval_ms.append(st.evaluate_ms)
+            flops.append(algorithmic_eval_flops(st.decisions, st.encoded_turns, A))
+            decisions += st.decisions
+            rescored += st.rescored
+            fallbacks += st.fallbacks
+        elapsed = time.perf_counter() - t0
+        _, fit, _, _ = rt.generation(0)
+        return dict(elapsed=elapsed, eval_ms=float(np.mean(eval_ms)), flops=float(np.mean(flops)),
+                    decisions=decisions, rescored=rescored, fallbacks=fallbacks, stages=rt.stage_times(),
+                    summary=rt.summary(), fitness=fit.copy())
+
+    fp32 = run_form(rtm.EVAL_FP32)
+    bf16 = run_form(rtm.EVAL_BF16_RESCORED) if not args.fp32_only else None
     rt.close()
+    st = fp32["stages"]
+    value = games_per_gen * args.steps / fp32["elapsed"]
+    eval_avg_ms = fp32["eval_ms"]
+    achieved = fp32["flops"] / (eval_avg_ms * 1e-3) / 1e12
+    summary = fp32["summary"]
+    traffic = measured_traffic()
+    bf16_out = None
+    if bf16 is not None:
+        bf16_out = {
+            "value": games_per_gen * args.steps / bf16["elapsed"], "unit": "game-evals/s",
+            "ms_per_step": bf16["elapsed"] / args.steps * 1e3, "evaluate_ms": bf16["eval_ms"],
+            "candidates_rescored_per_decision": bf16["rescored"] / max(1, bf16["decisions"]),
+            "fp32_fallback_decisions": bf16["fallbacks"],
+            "same_fitness_as_fp32": bool(np.array_equal(bf16["fitness"].view(np.uint32),
+                                                        fp32["fitness"].view(np.uint32))),
+        }
     out = {
         "metric": "game-evals/sec (plain FP32 generation step)",
         "value": value,
@@ -133,7 +172,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": fp32["elapsed"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -144,12 +183,13 @@ def main():
                    "genomes": n, "actions": A, "games_per_generation": games_per_gen,
                    "mutation_probability": 0.01, "mutation_sigma": 0.02},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_MFMA_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_MFMA_TFLOPS, "traffic": traffic,
                      "kernel": "EvaluateKernel", "kernel_ms": eval_avg_ms},
         "stages_ms": {"evaluate": st.evaluate_ms, "finalize": st.finalize_ms, "plan": st.plan_ms,
                       "assemble": st.assemble_ms, "bookkeeping": st.bookkeeping_ms},
-        "generations_per_s": args.steps / elapsed,
-        "decisions_per_generation": decisions / args.steps,
+        "generations_per_s": args.steps / fp32["elapsed"],
+        "decisions_per_generation": fp32["decisions"] / args.steps,
+        "bf16_screened_fp32_rescored": bf16_out,
         "best_fitness": summary.best_fitness,
         "cpu_baseline": cpu,
     }
