@@ -1,0 +1,138 @@
+"""Runs the measured configurations of the GA generation step on one GPU; one JSON record per run.
+
+  grid16    16x16 grid (256 genomes), 512 actions, 256 secrets in 16 block shards of 16, 20 generations
+  grid128   128x128 grid (16,384 genomes, ~41 GB slab), 4,739 actions, 8,192 secrets in 256 shards of 32
+  ragged    64x64 grid, per-cell shards of U{32..1024} secrets, actions growing 512 -> 4,739 in steps of
+            512 every 5 generations (9 widening repacks + output-embedding injections), 50 generations
+
+Every run is timed in the plain FP32 form and in the BF16-screened form (which takes the same decisions);
+`--forms fp32` skips the latter.  Usage: python run_configs.py grid16|grid128|ragged [--forms fp32,bf16]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import __graft_entry__ as ge  # noqa: E402
+
+pkg = ge.import_package()
+rtm, synth = pkg.runtime, pkg.synth
+
+
+def _run_generations(rt, gens, counts_for, injection_for, cfg, seed=7):
+    """Advances `gens` generations; returns per-generation (seconds, games, decisions, best, mean)."""
+    out = []
+    for g in range(gens):
+        counts = counts_for(g)
+        inj = injection_for(g)
+        t0 = time.perf_counter()
+        rt.advance(seed + 2 + g, counts, cfg, injection=inj)
+        dt = time.perf_counter() - t0
+        st = rt.stage_times()
+        s = rt.summary()
+        out.append((dt, int(st.games), int(st.decisions), float(s.best_fitness), float(s.average_fitness),
+                    int(st.rescored), int(st.fallbacks)))
+    return out
+
+
+def _record(name, form, rows, extra):
+    secs = sum(r[0] for r in rows)
+    games = sum(r[1] for r in rows)
+    rec = {"config": name, "form": form, "generations": len(rows), "seconds": secs,
+           "game_evals_per_s": games / secs, "generations_per_s": len(rows) / secs,
+           "decisions": sum(r[2] for r in rows), "best_fitness": [r[3] for r in rows],
+           "mean_fitness_last": rows[-1][4], "rescored": sum(r[5] for r in rows),
+           "fallbacks": sum(r[6] for r in rows)}
+    rec.update(extra)
+    print(json.dumps(rec), flush=True)
+    return rec
+
+
+def plain_config(name, gens, forms):
+    wl = synth.workload(name)
+    A, side = wl["actions"], wl["side"]
+    n = side * side
+    stride = rtm.slot_stride_bytes(A)
+    cfg = rtm.default_assembly_config(mutation_probability=0.01, mutation_sigma=0.02)
+    recs = []
+    for form in forms:
+        rt = pkg.SlabRuntime(slab_bytes=2 * n * stride + stride, generation_bytes=n * stride, action_count=A,
+                             grid_columns=side)
+        try:
+            rt.upload_catalog(wl["catalog"])
+            rt.upload_secrets_csr(wl["offsets"], wl["ids"])
+            rt.set_eval_precision(rtm.EVAL_FP32 if form == "fp32" else rtm.EVAL_BF16_RESCORED)
+            rt.bootstrap(n, seed=7, variance_scale=1.0, tail_std=0.1)
+            rows = _run_generations(rt, gens, lambda g: rtm.word_counts(A, A), lambda g: None, cfg)
+            recs.append(_record(name, form, rows, {"genomes": n, "actions": A,
+                                                   "slab_bytes": 2 * n * stride + stride}))
+        finally:
+            rt.close()
+    if len(recs) == 2:
+        print(json.dumps({"config": name, "same_best_fitness_trajectory":
+                          recs[0]["best_fitness"] == recs[1]["best_fitness"]}), flush=True)
+
+
+def ragged_config(forms, gens=50, side=64, a0=512, a_final=4739, step=512, period=5):
+    n = side * side
+    catalog = synth.make_catalog(a_final, seed=synth.CATALOG_SEED, distinct_letters=True)
+    pool = synth.sample_secrets(a_final, 1030)
+    offsets, ids = synth.ragged_csr(n, pool, 32, 1024)
+    schedule = [min(a0 + step * (g // period), a_final) for g in range(gens + 1)]
+    s_final = rtm.slot_stride_bytes(a_final)
+    cfg = rtm.default_assembly_config(mutation_probability=0.01, mutation_sigma=0.02)
+
+    def injection_for(g):
+        if schedule[g + 1] == schedule[g]:
+            return None
+        return rtm.PendingInjection(enabled=1, reserved0=0, first_catalog_word_index=schedule[g],
+                                    injection_count=schedule[g + 1] - schedule[g])
+
+    recs = []
+    for form in forms:
+        # the generation budget is sized for the final stride, so the population stays 4,096 while the
+        # action count grows; the slab holds two generations at the final stride plus repack staging
+        rt = pkg.SlabRuntime(slab_bytes=2 * n * s_final + 4 * s_final, generation_bytes=n * s_final,
+                             action_count=a0, grid_columns=side)
+        try:
+            rt.upload_catalog(catalog)
+            rt.upload_secrets_csr(offsets, ids)
+            rt.set_eval_precision(rtm.EVAL_FP32 if form == "fp32" else rtm.EVAL_BF16_RESCORED)
+            rt.bootstrap(n, seed=7, variance_scale=1.0, tail_std=0.1)
+            rows = _run_generations(rt, gens, lambda g: rtm.word_counts(schedule[g], schedule[g]), injection_for,
+                                    cfg)
+            recs.append(_record("ragged", form, rows, {
+                "genomes": n, "actions": [schedule[0], schedule[gens]], "widenings": sum(
+                    schedule[g + 1] != schedule[g] for g in range(gens)),
+                "secrets_per_cell": [int(np.min(np.diff(offsets))), int(np.max(np.diff(offsets))),
+                                     float(np.mean(np.diff(offsets)))],
+                "final_action_count": int(rt.info().action_count)}))
+        finally:
+            rt.close()
+    if len(recs) == 2:
+        print(json.dumps({"config": "ragged", "same_best_fitness_trajectory":
+                          recs[0]["best_fitness"] == recs[1]["best_fitness"]}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("config", choices=["grid16", "grid128", "ragged"])
+    ap.add_argument("--forms", default="fp32,bf16")
+    ap.add_argument("--generations", type=int, default=0)
+    args = ap.parse_args()
+    forms = [f for f in args.forms.split(",") if f]
+    if args.config == "grid16":
+        plain_config("grid16_a512", args.generations or 20, forms)
+    elif args.config == "grid128":
+        plain_config("grid128_a4739", args.generations or 3, forms)
+    else:
+        ragged_config(forms, gens=args.generations or 50)
+
+
+if __name__ == "__main__":
+    main()

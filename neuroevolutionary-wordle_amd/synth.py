@@ -13,9 +13,10 @@ def zipf_letter_probabilities(s=1.1, letters=26):
     return w / w.sum()
 
 
-def make_catalog(count, seed=CATALOG_SEED, s=1.1):
+def make_catalog(count, seed=CATALOG_SEED, s=1.1, distinct_letters=False):
     """`count` unique 5-letter words, letters i.i.d. Zipf(s) over ranks 1..26 (rank k -> letter k-1),
-    duplicates rejected in draw order.  Returns uint8 [count, 5] of letter indices 0..25."""
+    duplicates rejected in draw order (and, with `distinct_letters`, words repeating a letter, which
+    always have the hint grids output-embedding injection needs).  Returns uint8 [count, 5]."""
     rng = np.random.Generator(np.random.PCG64(seed))
     p = zipf_letter_probabilities(s)
     seen = set()
@@ -24,7 +25,7 @@ def make_catalog(count, seed=CATALOG_SEED, s=1.1):
         batch = rng.choice(26, size=(max(256, 2 * (count - len(out))), 5), p=p).astype(np.uint8)
         for w in batch:
             key = w.tobytes()
-            if key in seen:
+            if key in seen or (distinct_letters and len(set(key)) < 5):
                 continue
             seen.add(key)
             out.append(w)
@@ -80,6 +81,14 @@ WORKLOADS = {
 def workload(name):
     w = WORKLOADS[name]
     catalog = make_catalog(w["actions"])
-    secrets = sample_secrets(w["actions"], w["secrets"])
+    if w["secrets"] <= w["actions"]:
+        secrets = sample_secrets(w["actions"], w["secrets"])
+    else:
+        # more secrets than words: each shard draws its own distinct secrets, shards may share words
+        blocks = (w["side"] // w["block"]) ** 2
+        per = w["secrets"] // blocks
+        rng = np.random.Generator(np.random.PCG64(SECRETS_SEED))
+        secrets = np.concatenate([rng.choice(w["actions"], size=per, replace=False) for _ in range(blocks)])
+        secrets = secrets.astype(np.uint32)
     offsets, ids = block_csr(w["side"], w["side"], w["block"], secrets)
     return dict(w, catalog=catalog, secrets=secrets, offsets=offsets, ids=ids)
