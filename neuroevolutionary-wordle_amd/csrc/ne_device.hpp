@@ -72,6 +72,30 @@ __host__ __device__ inline int ScoreDim(int s, int g) {
 constexpr int kImgTileWords = 2 * 64 * 4 + 16;  // 528
 constexpr int kImgTileBytes = 4 * kImgTileWords;
 
+// BF16 screening operands.  Per action tile, the two A operands of v_mfma_f32_16x16x32_bf16 as
+// [m][lane][8] bf16: lane i + 16g holds, for action row i, k-slots 8g..8g+7 of instruction m.
+// Instruction 0 carries the 26 fixed dims then tail dims 32..37, instruction 1 tail dims 0..31, so the
+// 64 dims fill the 64 k-slots exactly.
+constexpr uint32_t kBf16TileBytes = 2 * 64 * 8 * 2;  // 2,048
+__host__ __device__ inline int Bf16SlotDim(int m, int slot) {  // slot = 8g + e -> policy dim
+    if (m == 0) return slot < kFixed ? slot : kFixed + 32 + (slot - kFixed);
+    return kFixed + slot;
+}
+// The inverse: policy dim d -> instruction (returned) and k-slot (*slot).
+__host__ __device__ inline int Bf16DimSlot(int d, int* slot) {
+    if (d < kFixed) {
+        *slot = d;
+        return 0;
+    }
+    const int t = d - kFixed;
+    if (t < 32) {
+        *slot = t;
+        return 1;
+    }
+    *slot = kFixed + (t - 32);
+    return 0;
+}
+
 // Per-evaluation copy of a genome's encoder, transposed to [feature][out], then W0's virgin column.
 constexpr uint64_t kEncTStride = uint64_t(kTurnFeatures) * kEncOut + kH0;
 

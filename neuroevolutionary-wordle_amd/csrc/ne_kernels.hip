@@ -14,6 +14,7 @@
 namespace ne {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 __device__ inline void SetStatus(int* status, int code) { atomicCAS(status, 0, code); }
 
@@ -263,6 +264,51 @@ __global__ void TransposeEncoderKernel(const uint8_t* slab, uint64_t stride, con
     }
 }
 
+// Packs each genome's BF16 screening operands (kBf16TileBytes per action tile, round-to-nearest-even)
+// and the max |tail value| per tail dim over its first A rows (the screening bound).  Grid: (blocks,
+// genomes), 128 threads: thread t owns instruction t / 64, lane t % 64 of every tile it visits, so its
+// 8 dims are fixed and their maxima stay in registers until one atomic per dim and workgroup row.
+__global__ __launch_bounds__(128) void PackBf16Kernel(const uint8_t* slab, uint64_t stride, const uint32_t* slot_indices,
+                                                      uint32_t slot_count, const uint32_t* img, uint32_t A,
+                                                      uint32_t a_tiles, uint8_t* out, uint64_t out_stride,
+                                                      uint32_t* tail_absmax) {
+    const uint32_t n = blockIdx.y;
+    const uint32_t slot = slot_indices[n];
+    if (slot >= slot_count) return;  // reported by the evaluation kernel
+    const float* tails = reinterpret_cast<const float*>(slab + uint64_t(slot) * stride) + kPolicyParams;
+    const uint32_t m = threadIdx.x >> 6, lane = threadIdx.x & 63u, i = lane & 15u, g = lane >> 4;
+    float mx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t at = blockIdx.x; at < a_tiles; at += gridDim.x) {
+        const uint32_t a = at * 16 + i;
+        bf16x8 v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int d = Bf16SlotDim(int(m), int(8 * g) + e);
+            float x = 0.0f;
+            if (a < A) {
+                if (d < kFixed) {  // fixed dim d sits at step d % 7 of lane group d / 7 in the FP32 image
+                    const int gg = d / kFixedSteps, st = d % kFixedSteps;
+                    x = __builtin_bit_cast(float, img[uint64_t(at) * kImgTileWords + (st / 4) * 256 +
+                                                      4 * (i + 16 * uint32_t(gg)) + (st % 4)]);
+                } else {
+                    x = tails[uint64_t(a) * kTail + uint32_t(d - kFixed)];
+                    mx[e] = fmaxf(mx[e], fabsf(x));
+                }
+            }
+            v[e] = __bf16(x);
+        }
+        *reinterpret_cast<bf16x8*>(out + uint64_t(n) * out_stride + uint64_t(at) * kBf16TileBytes + m * 1024 + lane * 16) = v;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) mx[e] = fmaxf(mx[e], __shfl_xor(mx[e], off));
+        const int d = Bf16SlotDim(int(m), int(8 * g) + e);
+        if (i == 0 && d >= kFixed)  // non-negative floats order like their bit patterns
+            atomicMax(tail_absmax + uint64_t(n) * 40 + uint32_t(d - kFixed), __builtin_bit_cast(uint32_t, mx[e]));
+    }
+}
+
 // ================================================================================================
 // Evaluation kernel.
 //
@@ -318,22 +364,19 @@ __device__ __forceinline__ void WaitChunk() { __builtin_amdgcn_s_waitcnt(0x0F70)
 
 // Streams every chunk of the scoring image and tail rows through two LDS buffers (b0, b1: distinct
 // objects), chunk c + 1 in flight while chunk c is used.  issue(buf, c) starts the transfers of
-// chunk c; per_tile(img_tile, tail_tile, at) runs for the kCT tiles of each chunk when `compute`.
+// chunk c; per_tile(buf, k, at) runs for the kCT tiles of each chunk (tile k of the buffer is action
+// tile at) when `compute`.
 // The whole workgroup calls this together; chunk 0 must already be in flight into b0.
 template <int kCT, typename FI, typename FT>
 __device__ __forceinline__ void StreamChunks(uint8_t* b0, uint8_t* b1, uint32_t chunks, bool compute, FI&& issue,
                                              FT&& per_tile) {
-    using G = ChunkGeom<kCT>;
     WaitChunk();
     __syncthreads();
     auto one = [&](const uint8_t* buf, uint8_t* next, uint32_t c) __device__ {
         if (c + 1 < chunks) issue(next, c + 1);
         if (compute) {
 #pragma unroll
-            for (int k = 0; k < kCT; ++k)
-                per_tile(reinterpret_cast<const uint32_t*>(buf) + k * kImgTileWords,
-                         reinterpret_cast<const float*>(buf + G::kTailRegion) + k * 16 * kTail,
-                         c * uint32_t(kCT) + uint32_t(k));
+            for (int k = 0; k < kCT; ++k) per_tile(buf, k, c * uint32_t(kCT) + uint32_t(k));
         }
         WaitChunk();
         __syncthreads();
@@ -427,33 +470,37 @@ __device__ __forceinline__ void ScoreTile(const uint32_t* img_tile, const float*
     ArgmaxUpdate<kTrace>(Fp32Scores(ev, pol), img_tile + 512 + 4 * gq, abase, A, hist, am);
 }
 
-// ---- BF16 screening -----------------------------------------------------------------------------
-// The same contraction on v_mfma_f32_16x16x32_bf16 (FP32 accumulate): lane group g feeds its 17 steps
-// as bf16 in three K=32 instructions (8 per lane each, steps 17..23 zero).
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-struct PolicyBf16 {
-    bf16x8 k[3];
-};
-
-__device__ __forceinline__ PolicyBf16 PackBf16(const float (&v)[kScoreSteps]) {
-    PolicyBf16 p;
-#pragma unroll
-    for (int m = 0; m < 3; ++m)
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int s = 8 * m + e;
-            p.k[m][e] = __bf16(s < kScoreSteps ? v[s] : 0.0f);
-        }
-    return p;
+// Tile k of an FP32 chunk buffer: its scoring image and its 16 tail rows.
+template <int kCT>
+__device__ __forceinline__ const uint32_t* Fp32Img(const uint8_t* buf, int k) {
+    return reinterpret_cast<const uint32_t*>(buf) + k * kImgTileWords;
+}
+template <int kCT>
+__device__ __forceinline__ const float* Fp32Tails(const uint8_t* buf, int k) {
+    return reinterpret_cast<const float*>(buf + ChunkGeom<kCT>::kTailRegion) + k * 16 * kTail;
 }
 
-__device__ __forceinline__ v4f Bf16Scores(const float (&ev)[kScoreSteps], const PolicyBf16& pb) {
-    const PolicyBf16 eb = PackBf16(ev);
+// ---- BF16 screening -----------------------------------------------------------------------------
+// The same contraction on v_mfma_f32_16x16x32_bf16 (FP32 accumulate) over the packed operands of
+// PackBf16Kernel: two instructions per tile, operands straight from LDS.
+struct PolicyBf16 {
+    bf16x8 k[2];
+};
+constexpr uint32_t kB16Tiles = 4;                                // action tiles per screening chunk
+constexpr uint32_t kB16OpsBytes = kB16Tiles * kBf16TileBytes;    // 8 transfers of 1 KB
+constexpr uint32_t kB16BufBytes = kB16OpsBytes + kDmaBytes;       // + the chunk's canonical indices
+
+__device__ __forceinline__ v4f Bf16Screen(const uint8_t* buf, int k, uint32_t lane, const PolicyBf16& pb) {
+    const uint8_t* t = buf + k * kBf16TileBytes;
+    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(t + 16 * lane);
+    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(t + 1024 + 16 * lane);
     v4f sc = {0.0f, 0.0f, 0.0f, 0.0f};
-    sc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(eb.k[0], pb.k[0], sc, 0, 0, 0);
-    sc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(eb.k[1], pb.k[1], sc, 0, 0, 0);
-    sc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(eb.k[2], pb.k[2], sc, 0, 0, 0);
+    sc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, pb.k[0], sc, 0, 0, 0);
+    sc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, pb.k[1], sc, 0, 0, 0);
     return sc;
+}
+__device__ __forceinline__ const uint32_t* Bf16Canon(const uint8_t* buf, int k, uint32_t gq) {
+    return reinterpret_cast<const uint32_t*>(buf + kB16OpsBytes) + k * 16 + 4 * gq;
 }
 
 // Reduces a lane's masked-argmax state over the four lane groups sharing game column j (lower index
@@ -514,11 +561,12 @@ constexpr uint32_t kMaxCandidates = 16;  // per game; more within the window -> 
 
 template <bool kTrace, bool kBf16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void EvaluateKernel(EvalArgs p) {
-    constexpr int kCT = kBf16 ? kChunkTilesBf16 : kChunkTiles;
+    constexpr int kCT = kBf16 ? kChunkTilesBf16 : kChunkTiles;  // FP32 stream (the BF16 form's fallback)
     using G = ChunkGeom<kCT>;
+    constexpr uint32_t kBufBytes = kBf16 && kB16BufBytes > G::kBufBytes ? kB16BufBytes : G::kBufBytes;
     // two chunk buffers as distinct objects, so a read of one never waits on transfers into the other
-    __shared__ __align__(16) uint8_t s_buf0[G::kBufBytes];
-    __shared__ __align__(16) uint8_t s_buf1[G::kBufBytes];
+    __shared__ __align__(16) uint8_t s_buf0[kBufBytes];
+    __shared__ __align__(16) uint8_t s_buf1[kBufBytes];
 
     const uint32_t n = blockIdx.x;
     const uint32_t slot = p.slot_indices[n];
@@ -556,6 +604,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
 
     auto issue = [&](uint8_t* buf, uint32_t c) __device__ {
         IssueChunk(buf, img_r, tail_r, c, wave, waves, lane, G::kImgBytes, G::kTailBytes, G::kImgDma, G::kTailDma);
+    };
+    // BF16 screening stream: 8 x 16-byte transfers of packed operands + one 4-byte transfer of the
+    // 64 canonical indices per chunk of kB16Tiles tiles
+    const uint32_t chunks16 = (a_tiles + kB16Tiles - 1) / kB16Tiles;
+    const Rsrc ops_r = MakeRsrc(kBf16 ? p.bf16_ops + uint64_t(n) * p.bf16_stride : p.slab, a_tiles * kBf16TileBytes);
+    const Rsrc canon_r = MakeRsrc(p.canon, p.canon_bytes);
+    auto issue16 = [&](uint8_t* buf, uint32_t c) __device__ {
+        for (uint32_t i = wave; i < kB16OpsBytes / kDmaBytes + 1; i += waves) {
+            if (i < kB16OpsBytes / kDmaBytes)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(ops_r, (LdsPtr)(buf + i * kDmaBytes), 16,
+                                                         c * kB16OpsBytes + i * kDmaBytes + 16 * lane, 0, 0, 0);
+            else
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(canon_r, (LdsPtr)(buf + kB16OpsBytes), 4,
+                                                         c * (kB16Tiles * 64) + 4 * lane, 0, 0, 0);
+        }
     };
 
     const uint32_t per_round = gridDim.y * waves;
@@ -627,7 +690,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
         for (int step = 0; step < kMaxTurns; ++step) {
             if (!__syncthreads_or(active ? 1 : 0)) break;  // the whole workgroup leaves together
             const bool wave_active = __any(active ? 1 : 0) != 0;
-            issue(s_buf0, 0);  // lands behind the trunk
+            if constexpr (kBf16)
+                issue16(s_buf0, 0);  // lands behind the trunk
+            else
+                issue(s_buf0, 0);
             float pol[kScoreSteps];
             if (wave_active) {
                 const int ttc = base_tc + step;  // turn count of every active game in this tile
@@ -660,9 +726,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
             if constexpr (!kBf16) {
                 ArgmaxLane am = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot};
                 StreamChunks<kCT>(s_buf0, s_buf1, chunks, wave_active, issue,
-                                  [&](const uint32_t* img_t, const float* tail_t, uint32_t at) __device__ {
-                                      ScoreTile<kTrace>(img_t, tail_t, lane, j, gq, pol, at * 16 + 4 * gq, A, hist,
-                                                        am);
+                                  [&](const uint8_t* buf, int k, uint32_t at) __device__ {
+                                      ScoreTile<kTrace>(Fp32Img<kCT>(buf, k), Fp32Tails<kCT>(buf, k), lane, j, gq, pol,
+                                                        at * 16 + 4 * gq, A, hist, am);
                                   });
                 best = am.best;
                 second = am.second;
@@ -676,45 +742,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
                 // for bit.  Decisions the window cannot settle take the FP32 scan.
                 __shared__ uint32_t s_cand[4][16][kMaxCandidates];
                 __shared__ uint32_t s_ncand[4][16];
-                __shared__ float s_mtail[4][4 * kTailSteps];
-                const PolicyBf16 pb = PackBf16(pol);
-                const bool measure = step == 0;  // per-dim tail magnitudes, once per round
-                float tmax[kTailSteps];
+                // the policy as the B operands of the two screening instructions: every lane writes its
+                // dims (bf16) at their k-slots in s_buf1, free until the stream's first barrier, and
+                // reads back its own 2 x 8
+                PolicyBf16 pb;
+                {
+                    __bf16* xb = reinterpret_cast<__bf16*>(s_buf1) + wave * 1024u;
 #pragma unroll
-                for (int t = 0; t < kTailSteps; ++t) tmax[t] = 0.0f;
+                    for (int s = 0; s < kScoreSteps; ++s) {
+                        const int d = ScoreDim(s, int(gq));
+                        if (d < 0) continue;
+                        int slot;
+                        const int m = Bf16DimSlot(d, &slot);
+                        xb[m * 512 + (int(j) + 16 * (slot >> 3)) * 8 + (slot & 7)] = __bf16(pol[s]);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    pb.k[0] = *reinterpret_cast<const bf16x8*>(xb + lane * 8);
+                    pb.k[1] = *reinterpret_cast<const bf16x8*>(xb + 512 + lane * 8);
+                }
                 ArgmaxLane am = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot};
-                StreamChunks<kCT>(s_buf0, s_buf1, chunks, wave_active, issue,
-                                  [&](const uint32_t* img_t, const float* tail_t, uint32_t at) __device__ {
-                                      float ev[kScoreSteps];
-                                      TileOperands(img_t, tail_t, lane, j, gq, ev);
-                                      if (measure) {
-                                          const bool row_ok = at * 16 + j < A;
-#pragma unroll
-                                          for (int t = 0; t < kTailSteps; ++t)
-                                              tmax[t] = row_ok ? fmaxf(tmax[t], fabsf(ev[kFixedSteps + t])) : tmax[t];
-                                      }
-                                      ArgmaxUpdate<false>(Bf16Scores(ev, pb), img_t + 512 + 4 * gq, at * 16 + 4 * gq, A,
-                                                          hist, am);
-                                  });
+                StreamChunks<kB16Tiles>(s_buf0, s_buf1, chunks16, wave_active, issue16,
+                                        [&](const uint8_t* buf, int k, uint32_t at) __device__ {
+                                            ArgmaxUpdate<false>(Bf16Screen(buf, k, lane, pb), Bf16Canon(buf, k, gq),
+                                                                at * 16 + 4 * gq, A, hist, am);
+                                        });
                 float b16 = am.best, unused = am.second;
                 uint32_t b16a = am.besta;
                 ReduceArgmax(b16, unused, b16a);
-                if (measure && wave_active) {
-#pragma unroll
-                    for (int t = 0; t < kTailSteps; ++t)
-#pragma unroll
-                        for (int off = 1; off < 16; off <<= 1) tmax[t] = fmaxf(tmax[t], __shfl_xor(tmax[t], off));
-                    if (j == 0)
-#pragma unroll
-                        for (int t = 0; t < kTailSteps; ++t) s_mtail[wave][kTailSteps * gq + t] = tmax[t];
-                    __builtin_amdgcn_wave_barrier();
-                }
                 // the bound: sum over this lane group's steps of M_d |p_d|, then over the 4 groups
                 float esum = 0.0f, pabs = 0.0f;
 #pragma unroll
                 for (int s = 0; s < kScoreSteps; ++s) {
                     const int d = ScoreDim(s, int(gq));
-                    const float m = d < 0 ? 0.0f : (d < kFixed ? p.fixed_absmax[d] : s_mtail[wave][d - kFixed]);
+                    const float m = d < 0 ? 0.0f
+                                          : (d < kFixed ? p.fixed_absmax[d]
+                                                        : p.tail_absmax[uint64_t(n) * 40 + uint32_t(d - kFixed)]);
                     esum += m * fabsf(pol[s]);
                     pabs += m + fabsf(pol[s]);
                 }
@@ -729,28 +791,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
                 const bool want = active && !fail;
                 if (lane < 16) s_ncand[wave][lane] = 0u;
                 __builtin_amdgcn_wave_barrier();
-                issue(s_buf0, 0);
-                StreamChunks<kCT>(s_buf0, s_buf1, chunks, __any(want ? 1 : 0) != 0, issue,
-                                  [&](const uint32_t* img_t, const float* tail_t, uint32_t at) __device__ {
-                                      float ev[kScoreSteps];
-                                      TileOperands(img_t, tail_t, lane, j, gq, ev);
-                                      const v4f sc = Bf16Scores(ev, pb);
-                                      const float m4 = fmaxf(fmaxf(sc[0], sc[1]), fmaxf(sc[2], sc[3]));
-                                      if (__any((want && m4 >= thr) ? 1 : 0)) {
-                                          const v4u cn = *reinterpret_cast<const v4u*>(img_t + 512 + 4 * gq);
+                issue16(s_buf0, 0);
+                StreamChunks<kB16Tiles>(
+                    s_buf0, s_buf1, chunks16, __any(want ? 1 : 0) != 0, issue16,
+                    [&](const uint8_t* buf, int k, uint32_t at) __device__ {
+                        const v4f sc = Bf16Screen(buf, k, lane, pb);
+                        const float m4 = fmaxf(fmaxf(sc[0], sc[1]), fmaxf(sc[2], sc[3]));
+                        if (__any((want && m4 >= thr) ? 1 : 0)) {
+                            const v4u cn = *reinterpret_cast<const v4u*>(Bf16Canon(buf, k, gq));
 #pragma unroll
-                                          for (int r = 0; r < 4; ++r) {
-                                              const uint32_t a = at * 16 + 4 * gq + uint32_t(r);
-                                              const uint32_t c = cn[r];
-                                              const bool masked = (c == hist[0]) | (c == hist[1]) | (c == hist[2]) |
-                                                                  (c == hist[3]) | (c == hist[4]);
-                                              if (want && sc[r] >= thr && a < A && !masked) {
-                                                  const uint32_t k = atomicAdd(&s_ncand[wave][j], 1u);
-                                                  if (k < kMaxCandidates) s_cand[wave][j][k] = a;
-                                              }
-                                          }
-                                      }
-                                  });
+                            for (int r = 0; r < 4; ++r) {
+                                const uint32_t a = at * 16 + 4 * gq + uint32_t(r);
+                                const uint32_t c = cn[r];
+                                const bool masked = (c == hist[0]) | (c == hist[1]) | (c == hist[2]) | (c == hist[3]) |
+                                                    (c == hist[4]);
+                                if (want && sc[r] >= thr && a < A && !masked) {
+                                    const uint32_t slot_k = atomicAdd(&s_ncand[wave][j], 1u);
+                                    if (slot_k < kMaxCandidates) s_cand[wave][j][slot_k] = a;
+                                }
+                            }
+                        }
+                    });
                 const uint32_t ncand = s_ncand[wave][j];
                 fail = fail || (want && (ncand == 0 || ncand > kMaxCandidates));
                 const bool resc = active && !fail;
@@ -782,9 +843,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
                     issue(s_buf0, 0);
                     ArgmaxLane af = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot};
                     StreamChunks<kCT>(s_buf0, s_buf1, chunks, __any(need ? 1 : 0) != 0, issue,
-                                      [&](const uint32_t* img_t, const float* tail_t, uint32_t at) __device__ {
-                                          ScoreTile<kTrace>(img_t, tail_t, lane, j, gq, pol, at * 16 + 4 * gq, A, hist,
-                                                            af);
+                                      [&](const uint8_t* buf, int k, uint32_t at) __device__ {
+                                          ScoreTile<kTrace>(Fp32Img<kCT>(buf, k), Fp32Tails<kCT>(buf, k), lane, j, gq,
+                                                            pol, at * 16 + 4 * gq, A, hist, af);
                                       });
                     float fb = af.best, fs = af.second;
                     uint32_t fa = af.besta;
@@ -1666,6 +1727,15 @@ __global__ __launch_bounds__(512) void InjectKernel(InjectArgs p) {
 
 // ================================================================================================
 // Launch wrappers (host side).
+void LaunchPackBf16(const uint8_t* slab, uint64_t stride, const uint32_t* slots, uint32_t n, uint32_t slot_count,
+                    const uint32_t* img, uint32_t A, uint8_t* out, uint64_t out_stride, uint32_t* tail_absmax,
+                    hipStream_t s) {
+    const uint32_t a_tiles = (A + 15) / 16;
+    const uint32_t blocks = a_tiles < 16 ? a_tiles : 16;
+    hipLaunchKernelGGL(PackBf16Kernel, dim3(blocks, n), dim3(128), 0, s, slab, stride, slots, slot_count, img, A, a_tiles,
+                       out, out_stride, tail_absmax);
+}
+
 void LaunchTransposeEncoder(const uint8_t* slab, uint64_t stride, const uint32_t* slots, uint32_t n,
                             uint32_t slot_count, float* enc_t, hipStream_t s) {
     hipLaunchKernelGGL(TransposeEncoderKernel, dim3(n), dim3(256), 0, s, slab, stride, slots, slot_count, enc_t);

@@ -25,12 +25,16 @@ struct EvalArgs {
     const uint32_t* cell_offsets;  // CSR of local secrets per cell (= individual)
     const uint32_t* secret_ids;
     const uint32_t* codes;         // packed catalog words
-    const uint32_t* canon;         // [catalog] lowest index holding the same word
+    const uint32_t* canon;         // [catalog tiles * 16] lowest index holding the same word (padded)
+    uint32_t canon_bytes;
     const uint32_t* img;           // [catalog tiles][kImgTileWords] scoring image
     uint32_t img_bytes;
     const float* enc_t;            // [N][kEncTStride]: transposed encoder, then W0[:, 0], per genome
     const float* fixed_absmax;     // [26] max over the catalog of |fixed feature d| (BF16 bound)
     uint32_t force_fallback;       // BF16 mode: settle every decision with the FP32 scan (diagnostic)
+    const uint8_t* bf16_ops;       // BF16 mode: [N][a_tiles][kBf16TileBytes] screening operands
+    uint64_t bf16_stride;          // bytes per genome in bf16_ops
+    const float* tail_absmax;      // BF16 mode: [N][40] max |tail value| per tail dim
     float* raw_fitness;            // [N] summed episode scores
     uint32_t* local_w;             // [N]
     unsigned long long* counters;  // [0] decisions, [1] encoded turns, [2] BF16 candidates rescored, [3] FP32 fallbacks
@@ -136,6 +140,9 @@ struct InjectArgs {
 };
 
 void LaunchInject(const InjectArgs& a, hipStream_t s);
+void LaunchPackBf16(const uint8_t* slab, uint64_t stride, const uint32_t* slots, uint32_t n, uint32_t slot_count,
+                    const uint32_t* img, uint32_t A, uint8_t* out, uint64_t out_stride, uint32_t* tail_absmax,
+                    hipStream_t s);
 void LaunchTransposeEncoder(const uint8_t* slab, uint64_t stride, const uint32_t* slots, uint32_t n,
                             uint32_t slot_count, float* enc_t, hipStream_t s);
 void LaunchEvaluate(const EvalArgs& a, uint32_t n_genomes, uint32_t waves, uint32_t chunks, bool trace, bool bf16,

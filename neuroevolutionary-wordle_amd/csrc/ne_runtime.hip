@@ -66,6 +66,9 @@ struct ne_runtime {
 
     // evaluation
     float* enc_t = nullptr;  // [max_gen][145][64] transposed encoders of the generation being evaluated
+    uint8_t* bf16_ops = nullptr;   // BF16 form: packed screening operands of the generation
+    uint64_t bf16_ops_bytes = 0;
+    uint32_t* tail_absmax = nullptr;  // BF16 form: [max_gen][40] max |tail value| per dim (float bits)
     float* raw = nullptr;
     uint32_t* local_w = nullptr;
     unsigned long long* counters = nullptr;
@@ -389,6 +392,7 @@ static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard
     a.secret_ids = rt->d_secrets;
     a.codes = rt->codes;
     a.canon = rt->canon;
+    a.canon_bytes = uint32_t(rt->img_bytes / kImgTileBytes * 64);
     a.img = rt->img;
     a.img_bytes = uint32_t(rt->img_bytes);
     a.fixed_absmax = rt->fixed_absmax;
@@ -422,9 +426,27 @@ static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard
     if (rt->enc_t == nullptr && !Alloc(&rt->enc_t, rt->max_gen * kEncTStride))
         return Fail(rt, NE_DEVICE_FAILURE);
     a.enc_t = rt->enc_t;
+    if (bf16) {
+        const uint64_t per_genome = uint64_t((a.action_count + 15) / 16) * kBf16TileBytes;
+        if (per_genome * rt->max_gen > rt->bf16_ops_bytes) {
+            Free(rt->bf16_ops);
+            rt->bf16_ops_bytes = 0;
+            if (!Alloc(&rt->bf16_ops, per_genome * rt->max_gen)) return Fail(rt, NE_DEVICE_FAILURE);
+            rt->bf16_ops_bytes = per_genome * rt->max_gen;
+        }
+        if (rt->tail_absmax == nullptr && !Alloc(&rt->tail_absmax, rt->max_gen * 40)) return Fail(rt, NE_DEVICE_FAILURE);
+        if (!Ok(hipMemsetAsync(rt->tail_absmax, 0, rt->cur_size * 40 * sizeof(uint32_t), rt->stream)))
+            return Fail(rt, NE_DEVICE_FAILURE);
+        a.bf16_ops = rt->bf16_ops;
+        a.bf16_stride = per_genome;
+        a.tail_absmax = reinterpret_cast<const float*>(rt->tail_absmax);
+    }
     EventBegin(rt, 8);
     LaunchTransposeEncoder(rt->slab, rt->stride, rt->cur_slots, uint32_t(rt->cur_size), uint32_t(rt->slot_count),
                            rt->enc_t, rt->stream);
+    if (bf16)
+        LaunchPackBf16(rt->slab, rt->stride, rt->cur_slots, uint32_t(rt->cur_size), uint32_t(rt->slot_count), rt->img,
+                       a.action_count, rt->bf16_ops, a.bf16_stride, rt->tail_absmax, rt->stream);
     EventEnd(rt, 8);
     EventBegin(rt, 0);
     LaunchEvaluate(a, uint32_t(rt->cur_size), waves, chunks, trace, bf16, rt->stream);
@@ -800,6 +822,8 @@ void ne_destroy(ne_runtime* rt) {
     Free(rt->xover);
     Free(rt->rank_weights);
     Free(rt->enc_t);
+    Free(rt->bf16_ops);
+    Free(rt->tail_absmax);
     Free(rt->raw);
     Free(rt->local_w);
     Free(rt->counters);
@@ -864,6 +888,7 @@ int ne_upload_catalog(ne_runtime* rt, const uint8_t* letters, uint64_t word_coun
         }
         for (int i = 0; i < 16; ++i) tile[512 + i] = t * 16 + i < word_count ? canon[t * 16 + i] : kInvalidSlot;
     }
+    canon.resize(tiles * 16, kInvalidSlot);  // whole tiles, for the screening stream's transfers
     Free(rt->codes);
     Free(rt->canon);
     Free(rt->img);
@@ -872,9 +897,9 @@ int ne_upload_catalog(ne_runtime* rt, const uint8_t* letters, uint64_t word_coun
     if (!Alloc(&rt->fixed_absmax, kFixed) ||
         !Ok(hipMemcpy(rt->fixed_absmax, absmax, sizeof(absmax), hipMemcpyHostToDevice)))
         return Fail(rt, NE_DEVICE_FAILURE);
-    if (!Alloc(&rt->codes, word_count) || !Alloc(&rt->canon, word_count) || !Alloc(&rt->img, img.size()) ||
+    if (!Alloc(&rt->codes, word_count) || !Alloc(&rt->canon, canon.size()) || !Alloc(&rt->img, img.size()) ||
         !Ok(hipMemcpy(rt->codes, codes.data(), word_count * 4, hipMemcpyHostToDevice)) ||
-        !Ok(hipMemcpy(rt->canon, canon.data(), word_count * 4, hipMemcpyHostToDevice)) ||
+        !Ok(hipMemcpy(rt->canon, canon.data(), canon.size() * 4, hipMemcpyHostToDevice)) ||
         !Ok(hipMemcpy(rt->img, img.data(), img.size() * 4, hipMemcpyHostToDevice)))
         return Fail(rt, NE_DEVICE_FAILURE);
     rt->catalog_count = word_count;
