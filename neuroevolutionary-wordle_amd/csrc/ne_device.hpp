@@ -72,11 +72,16 @@ __host__ __device__ inline int ScoreDim(int s, int g) {
 constexpr int kImgTileWords = 2 * 64 * 4 + 16;  // 528
 constexpr int kImgTileBytes = 4 * kImgTileWords;
 
-// BF16 screening operands.  Per action tile, the two A operands of v_mfma_f32_16x16x32_bf16 as
-// [m][lane][8] bf16: lane i + 16g holds, for action row i, k-slots 8g..8g+7 of instruction m.
+// BF16 screening operands.  Per action tile, the two A operands of v_mfma_f32_16x16x32_bf16, 16 bytes
+// (8 bf16) per lane: lane i + 16g holds, for action row i, k-slots 8g..8g+7 of instruction m.
 // Instruction 0 carries the 26 fixed dims then tail dims 32..37, instruction 1 tail dims 0..31, so the
-// 64 dims fill the 64 k-slots exactly.
-constexpr uint32_t kBf16TileBytes = 2 * 64 * 8 * 2;  // 2,048
+// 64 dims fill the 64 k-slots exactly.  Lanes 0..47 of instruction 0 hold fixed dims only and come
+// from one catalog-wide image; the rest is per genome:
+//   catalog tile: instruction 0, lanes 0..47                                  768 B
+//   genome tile:  instruction 0, lanes 48..63 (fixed 24, 25, tail 32..37)     256 B
+//                 instruction 1, lanes 0..63 (tail 0..31)                   1,024 B
+constexpr uint32_t kBf16CatTileBytes = 48 * 16;
+constexpr uint32_t kBf16GenTileBytes = 16 * 16 + 64 * 16;
 __host__ __device__ inline int Bf16SlotDim(int m, int slot) {  // slot = 8g + e -> policy dim
     if (m == 0) return slot < kFixed ? slot : kFixed + 32 + (slot - kFixed);
     return kFixed + slot;

@@ -264,26 +264,32 @@ __global__ void TransposeEncoderKernel(const uint8_t* slab, uint64_t stride, con
     }
 }
 
-// Packs each genome's BF16 screening operands (kBf16TileBytes per action tile, round-to-nearest-even)
-// and the max |tail value| per tail dim over its first A rows (the screening bound).  Grid: (blocks,
-// genomes), 128 threads: thread t owns instruction t / 64, lane t % 64 of every tile it visits, so its
-// 8 dims are fixed and their maxima stay in registers until one atomic per dim and workgroup row.
-__global__ __launch_bounds__(128) void PackBf16Kernel(const uint8_t* slab, uint64_t stride, const uint32_t* slot_indices,
+// Packs the per-genome part of each genome's BF16 screening operands (kBf16GenTileBytes per action
+// tile, round-to-nearest-even) and the max |tail value| per tail dim over its first A rows (the
+// screening bound).  Grid: (blocks, genomes), 320 threads = 4 tiles at a time, 80 threads per tile:
+// thread r of a tile writes bytes 16r..16r+15 of it, i.e. row i = r % 16 of operand class c = r / 16
+// (0: instruction 0 lane group 3, 1..4: instruction 1 lane group c - 1), so its 8 dims are fixed and
+// their maxima stay in registers until one reduction per workgroup.
+constexpr uint32_t kPackTiles = 4;
+__global__ __launch_bounds__(320) void PackBf16Kernel(const uint8_t* slab, uint64_t stride, const uint32_t* slot_indices,
                                                       uint32_t slot_count, const uint32_t* img, uint32_t A,
                                                       uint32_t a_tiles, uint8_t* out, uint64_t out_stride,
                                                       uint32_t* tail_absmax) {
+    __shared__ float s_mx[kPackTiles][5][8];
     const uint32_t n = blockIdx.y;
     const uint32_t slot = slot_indices[n];
     if (slot >= slot_count) return;  // reported by the evaluation kernel
     const float* tails = reinterpret_cast<const float*>(slab + uint64_t(slot) * stride) + kPolicyParams;
-    const uint32_t m = threadIdx.x >> 6, lane = threadIdx.x & 63u, i = lane & 15u, g = lane >> 4;
+    const uint32_t sub = threadIdx.x / 80, r = threadIdx.x % 80;
+    const uint32_t c = r / 16, i = r % 16;
+    const int m = c == 0 ? 0 : 1, g = c == 0 ? 3 : int(c) - 1;
     float mx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t at = blockIdx.x; at < a_tiles; at += gridDim.x) {
+    for (uint32_t at = blockIdx.x * kPackTiles + sub; at < a_tiles; at += gridDim.x * kPackTiles) {
         const uint32_t a = at * 16 + i;
         bf16x8 v;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const int d = Bf16SlotDim(int(m), int(8 * g) + e);
+            const int d = Bf16SlotDim(m, 8 * g + e);
             float x = 0.0f;
             if (a < A) {
                 if (d < kFixed) {  // fixed dim d sits at step d % 7 of lane group d / 7 in the FP32 image
@@ -297,15 +303,24 @@ __global__ __launch_bounds__(128) void PackBf16Kernel(const uint8_t* slab, uint6
             }
             v[e] = __bf16(x);
         }
-        *reinterpret_cast<bf16x8*>(out + uint64_t(n) * out_stride + uint64_t(at) * kBf16TileBytes + m * 1024 + lane * 16) = v;
+        *reinterpret_cast<bf16x8*>(out + uint64_t(n) * out_stride + uint64_t(at) * kBf16GenTileBytes + 16 * r) = v;
     }
+    // the 16 threads of one (tile, class) are 16 aligned lanes of one wave (80 and 16 divide evenly)
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
 #pragma unroll
         for (int off = 1; off < 16; off <<= 1) mx[e] = fmaxf(mx[e], __shfl_xor(mx[e], off));
-        const int d = Bf16SlotDim(int(m), int(8 * g) + e);
-        if (i == 0 && d >= kFixed)  // non-negative floats order like their bit patterns
-            atomicMax(tail_absmax + uint64_t(n) * 40 + uint32_t(d - kFixed), __builtin_bit_cast(uint32_t, mx[e]));
+        if (i == 0) s_mx[sub][c][e] = mx[e];
+    }
+    __syncthreads();
+    if (threadIdx.x < uint32_t(kTail)) {
+        const uint32_t t = threadIdx.x;
+        const uint32_t cc = t < 32 ? 1 + t / 8 : 0, ee = t < 32 ? t % 8 : t - 32 + (kFixed - 24);
+        float v = 0.0f;
+#pragma unroll
+        for (uint32_t q = 0; q < kPackTiles; ++q) v = fmaxf(v, s_mx[q][cc][ee]);
+        // non-negative floats order like their bit patterns
+        atomicMax(tail_absmax + uint64_t(n) * 40 + t, __builtin_bit_cast(uint32_t, v));
     }
 }
 
@@ -486,21 +501,29 @@ __device__ __forceinline__ const float* Fp32Tails(const uint8_t* buf, int k) {
 struct PolicyBf16 {
     bf16x8 k[2];
 };
-constexpr uint32_t kB16Tiles = 4;                                // action tiles per screening chunk
-constexpr uint32_t kB16OpsBytes = kB16Tiles * kBf16TileBytes;    // 8 transfers of 1 KB
-constexpr uint32_t kB16BufBytes = kB16OpsBytes + kDmaBytes;       // + the chunk's canonical indices
+// A screening chunk of kB16Tiles tiles in LDS: the catalog parts (3 transfers of 1 KB), the genome
+// parts (5 transfers), then the tiles' canonical indices (one 4-byte transfer).
+constexpr uint32_t kB16Tiles = 4;
+constexpr uint32_t kB16CatBytes = kB16Tiles * kBf16CatTileBytes;
+constexpr uint32_t kB16GenBytes = kB16Tiles * kBf16GenTileBytes;
+constexpr uint32_t kB16OpsDma = (kB16CatBytes + kB16GenBytes) / kDmaBytes;
+constexpr uint32_t kB16BufBytes = kB16CatBytes + kB16GenBytes + kB16Tiles * 64;
+static_assert(kB16CatBytes % kDmaBytes == 0 && kB16GenBytes % kDmaBytes == 0, "whole transfers");
 
 __device__ __forceinline__ v4f Bf16Screen(const uint8_t* buf, int k, uint32_t lane, const PolicyBf16& pb) {
-    const uint8_t* t = buf + k * kBf16TileBytes;
-    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(t + 16 * lane);
-    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(t + 1024 + 16 * lane);
+    // one base pointer and an integer select keep the read attributable to its buffer (a pointer
+    // select would make every read wait for the transfers into the other buffer)
+    const uint32_t off0 = lane < 48 ? k * kBf16CatTileBytes + 16 * lane
+                                    : kB16CatBytes + k * kBf16GenTileBytes + 16 * (lane - 48);
+    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(buf + off0);
+    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(buf + kB16CatBytes + k * kBf16GenTileBytes + 256 + 16 * lane);
     v4f sc = {0.0f, 0.0f, 0.0f, 0.0f};
     sc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, pb.k[0], sc, 0, 0, 0);
     sc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, pb.k[1], sc, 0, 0, 0);
     return sc;
 }
 __device__ __forceinline__ const uint32_t* Bf16Canon(const uint8_t* buf, int k, uint32_t gq) {
-    return reinterpret_cast<const uint32_t*>(buf + kB16OpsBytes) + k * 16 + 4 * gq;
+    return reinterpret_cast<const uint32_t*>(buf + kB16CatBytes + kB16GenBytes) + k * 16 + 4 * gq;
 }
 
 // Reduces a lane's masked-argmax state over the four lane groups sharing game column j (lower index
@@ -557,7 +580,8 @@ __device__ __forceinline__ void ActionOperands(Rsrc img_r, Rsrc tail_r, uint32_t
 // |S_bf16 - S_fp32| <= eps = 1.03 * 2^-8 * sum_d M_d |p_d| (+ a subnormal floor), M_d = max_a |E_ad|.
 constexpr float kBf16RelBound = 1.03f * 0x1p-8f;
 constexpr float kBf16AbsBound = 0x1p-120f;
-constexpr uint32_t kMaxCandidates = 16;  // per game; more within the window -> FP32 scan of the decision
+constexpr uint32_t kMaxCandidates = 32;  // per game; more within the window -> FP32 scan of the decision
+constexpr uint32_t kLaneCandidates = 8;  // per lane during the screening pass (compacted when full)
 
 template <bool kTrace, bool kBf16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void EvaluateKernel(EvalArgs p) {
@@ -605,18 +629,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
     auto issue = [&](uint8_t* buf, uint32_t c) __device__ {
         IssueChunk(buf, img_r, tail_r, c, wave, waves, lane, G::kImgBytes, G::kTailBytes, G::kImgDma, G::kTailDma);
     };
-    // BF16 screening stream: 8 x 16-byte transfers of packed operands + one 4-byte transfer of the
-    // 64 canonical indices per chunk of kB16Tiles tiles
+    // BF16 screening stream, per chunk of kB16Tiles tiles: 3 x 16-byte transfers of catalog operands,
+    // 5 of the genome's packed operands, one 4-byte transfer of the 64 canonical indices
     const uint32_t chunks16 = (a_tiles + kB16Tiles - 1) / kB16Tiles;
-    const Rsrc ops_r = MakeRsrc(kBf16 ? p.bf16_ops + uint64_t(n) * p.bf16_stride : p.slab, a_tiles * kBf16TileBytes);
+    const Rsrc cat_r = MakeRsrc(p.bf16_cat, p.bf16_cat_bytes);
+    const Rsrc ops_r = MakeRsrc(kBf16 ? p.bf16_ops + uint64_t(n) * p.bf16_stride : p.slab, a_tiles * kBf16GenTileBytes);
     const Rsrc canon_r = MakeRsrc(p.canon, p.canon_bytes);
     auto issue16 = [&](uint8_t* buf, uint32_t c) __device__ {
-        for (uint32_t i = wave; i < kB16OpsBytes / kDmaBytes + 1; i += waves) {
-            if (i < kB16OpsBytes / kDmaBytes)
+        constexpr uint32_t kCatDma = kB16CatBytes / kDmaBytes;
+        for (uint32_t i = wave; i < kB16OpsDma + 1; i += waves) {
+            if (i < kCatDma)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(cat_r, (LdsPtr)(buf + i * kDmaBytes), 16,
+                                                         c * kB16CatBytes + i * kDmaBytes + 16 * lane, 0, 0, 0);
+            else if (i < kB16OpsDma)
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(ops_r, (LdsPtr)(buf + i * kDmaBytes), 16,
-                                                         c * kB16OpsBytes + i * kDmaBytes + 16 * lane, 0, 0, 0);
+                                                         c * kB16GenBytes + (i - kCatDma) * kDmaBytes + 16 * lane, 0,
+                                                         0, 0);
             else
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(canon_r, (LdsPtr)(buf + kB16OpsBytes), 4,
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(canon_r, (LdsPtr)(buf + kB16OpsDma * kDmaBytes), 4,
                                                          c * (kB16Tiles * 64) + 4 * lane, 0, 0, 0);
         }
     };
@@ -735,19 +765,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
                 besta = am.besta;
                 ReduceArgmax(best, second, besta);
             } else {
-                // BF16 screening with exact FP32 rescoring: (1) BF16 masked argmax b16 over all
-                // actions; (2) every unmasked action with S_bf16 >= b16 - 2 eps joins the game's
-                // candidate list; (3) the candidates are rescored with the FP32 contraction itself, so
-                // the FP32 argmax (which always lies in the window) and its score are reproduced bit
-                // for bit.  Decisions the window cannot settle take the FP32 scan.
-                __shared__ uint32_t s_cand[4][16][kMaxCandidates];
+                // BF16 screening with exact FP32 rescoring.  With |S_bf16 - S_fp32| <= eps for every
+                // action, the FP32 argmax (and every action tying with it) satisfies S_bf16 >= b16 - 2 eps,
+                // b16 = the BF16 masked maximum.  One screening pass keeps, per lane, every unmasked row
+                // with S_bf16 >= (its running maximum) - 2 eps, a superset of that window since the
+                // running maximum never exceeds b16; the window's members are then rescored with the
+                // FP32 contraction itself, so the FP32 argmax and its score are reproduced bit for bit.
+                // Decisions the window cannot settle (overflowing lists, non-finite scores) take the
+                // FP32 scan.
+                // each wave's candidate table lives in s_buf0, free from the end of the stream until the
+                // fallback's or the next decision's first transfer
+                static_assert(sizeof(uint32_t) * 4 * 16 * kMaxCandidates <= kBufBytes, "candidate table");
+                auto s_cand = reinterpret_cast<uint32_t (*)[16][kMaxCandidates]>(s_buf0);
                 __shared__ uint32_t s_ncand[4][16];
+                __shared__ float s_lsc[4][kLaneCandidates][64];
+                __shared__ uint32_t s_lac[4][kLaneCandidates][64];
                 // the policy as the B operands of the two screening instructions: every lane writes its
-                // dims (bf16) at their k-slots in s_buf1, free until the stream's first barrier, and
-                // reads back its own 2 x 8
+                // dims (bf16) at their k-slots in the wave's (still unused) list scores, and reads back
+                // its own 2 x 8
                 PolicyBf16 pb;
                 {
-                    __bf16* xb = reinterpret_cast<__bf16*>(s_buf1) + wave * 1024u;
+                    static_assert(sizeof(float) * kLaneCandidates * 64 >= 2 * 1024, "policy exchange");
+                    __bf16* xb = reinterpret_cast<__bf16*>(&s_lsc[wave][0][0]);
 #pragma unroll
                     for (int s = 0; s < kScoreSteps; ++s) {
                         const int d = ScoreDim(s, int(gq));
@@ -760,15 +799,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
                     pb.k[0] = *reinterpret_cast<const bf16x8*>(xb + lane * 8);
                     pb.k[1] = *reinterpret_cast<const bf16x8*>(xb + 512 + lane * 8);
                 }
-                ArgmaxLane am = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot};
-                StreamChunks<kB16Tiles>(s_buf0, s_buf1, chunks16, wave_active, issue16,
-                                        [&](const uint8_t* buf, int k, uint32_t at) __device__ {
-                                            ArgmaxUpdate<false>(Bf16Screen(buf, k, lane, pb), Bf16Canon(buf, k, gq),
-                                                                at * 16 + 4 * gq, A, hist, am);
-                                        });
-                float b16 = am.best, unused = am.second;
-                uint32_t b16a = am.besta;
-                ReduceArgmax(b16, unused, b16a);
                 // the bound: sum over this lane group's steps of M_d |p_d|, then over the 4 groups
                 float esum = 0.0f, pabs = 0.0f;
 #pragma unroll
@@ -785,33 +815,93 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
                 pabs += __shfl_xor(pabs, 16);
                 pabs += __shfl_xor(pabs, 32);
                 const float eps = kBf16RelBound * esum + kBf16AbsBound * (pabs + 1.0f);
-                const float thr = b16 - 2.0f * eps;
-                bool fail = active && (p.force_fallback != 0 || b16a == kInvalidSlot || !(fabsf(b16) <= 3.0e38f) ||
-                                       !(eps <= 1.0e37f));
+                const float eps2 = 2.0f * eps;
+                bool fail = active && (p.force_fallback != 0 || !(eps <= 1.0e37f));
                 const bool want = active && !fail;
-                if (lane < 16) s_ncand[wave][lane] = 0u;
-                __builtin_amdgcn_wave_barrier();
-                issue16(s_buf0, 0);
+                // lane state: running masked maximum, its threshold, list length, overflow / non-finite
+                float lbest = -__builtin_inff(), lthr = -__builtin_inff();
+                uint32_t lcnt = 0;
+                bool lfail = false;
+                float llost = -__builtin_inff();  // best score dropped from a full list
                 StreamChunks<kB16Tiles>(
                     s_buf0, s_buf1, chunks16, __any(want ? 1 : 0) != 0, issue16,
                     [&](const uint8_t* buf, int k, uint32_t at) __device__ {
                         const v4f sc = Bf16Screen(buf, k, lane, pb);
                         const float m4 = fmaxf(fmaxf(sc[0], sc[1]), fmaxf(sc[2], sc[3]));
-                        if (__any((want && m4 >= thr) ? 1 : 0)) {
-                            const v4u cn = *reinterpret_cast<const v4u*>(Bf16Canon(buf, k, gq));
+                        // a NaN or infinite score anywhere in the row quad (or an overflowing sum of
+                        // four) sends the decision to the FP32 scan
+                        lfail = lfail || !(fabsf((sc[0] + sc[1]) + (sc[2] + sc[3])) <= 3.0e38f);
+                        if (!__any((want && m4 >= lthr) ? 1 : 0)) return;
+                        const v4u cn = *reinterpret_cast<const v4u*>(Bf16Canon(buf, k, gq));
 #pragma unroll
-                            for (int r = 0; r < 4; ++r) {
-                                const uint32_t a = at * 16 + 4 * gq + uint32_t(r);
-                                const uint32_t c = cn[r];
-                                const bool masked = (c == hist[0]) | (c == hist[1]) | (c == hist[2]) | (c == hist[3]) |
-                                                    (c == hist[4]);
-                                if (want && sc[r] >= thr && a < A && !masked) {
-                                    const uint32_t slot_k = atomicAdd(&s_ncand[wave][j], 1u);
-                                    if (slot_k < kMaxCandidates) s_cand[wave][j][slot_k] = a;
+                        for (int r = 0; r < 4; ++r) {
+                            const uint32_t a = at * 16 + 4 * gq + uint32_t(r);
+                            const uint32_t c = cn[r];
+                            const bool masked = (c == hist[0]) | (c == hist[1]) | (c == hist[2]) | (c == hist[3]) |
+                                                (c == hist[4]);
+                            const float v = sc[r];
+                            if (!(want && a < A && !masked && v >= lthr)) continue;
+                            if (v > lbest) {
+                                lbest = v;
+                                lthr = v - eps2;
+                            }
+                            if (lcnt == kLaneCandidates) {  // drop what the risen threshold excludes
+                                uint32_t w = 0;
+                                for (uint32_t q = 0; q < kLaneCandidates; ++q) {
+                                    const float sq = s_lsc[wave][q][lane];
+                                    if (sq >= lthr) {
+                                        s_lsc[wave][w][lane] = sq;
+                                        s_lac[wave][w][lane] = s_lac[wave][q][lane];
+                                        ++w;
+                                    }
+                                }
+                                lcnt = w;
+                            }
+                            if (lcnt < kLaneCandidates) {
+                                s_lsc[wave][lcnt][lane] = v;
+                                s_lac[wave][lcnt][lane] = a;
+                                ++lcnt;
+                            } else {  // full: keep the best kLaneCandidates, remember the best score dropped
+                                uint32_t qmin = 0;
+                                float smin = s_lsc[wave][0][lane];
+                                for (uint32_t q = 1; q < kLaneCandidates; ++q) {
+                                    const float sq = s_lsc[wave][q][lane];
+                                    if (sq < smin) {
+                                        smin = sq;
+                                        qmin = q;
+                                    }
+                                }
+                                if (v > smin) {
+                                    s_lsc[wave][qmin][lane] = v;
+                                    s_lac[wave][qmin][lane] = a;
+                                    llost = fmaxf(llost, smin);
+                                } else {
+                                    llost = fmaxf(llost, v);
                                 }
                             }
                         }
                     });
+                // the game's window: b16 over its 4 lane groups, then every listed row within 2 eps
+                float b16 = lbest;
+                b16 = fmaxf(b16, __shfl_xor(b16, 16));
+                b16 = fmaxf(b16, __shfl_xor(b16, 32));
+                const float thr = b16 - eps2;
+                // a dropped score inside the window means the lists may miss a window member
+                int gfail = (lfail || llost >= thr) ? 1 : 0;
+                gfail |= __shfl_xor(gfail, 16);
+                gfail |= __shfl_xor(gfail, 32);
+                fail = fail || (want && (gfail != 0 || !(fabsf(b16) <= 3.0e38f)));
+                if (lane < 16) s_ncand[wave][lane] = 0u;
+                __builtin_amdgcn_wave_barrier();
+                if (want && !fail) {
+                    for (uint32_t q = 0; q < lcnt; ++q) {
+                        if (s_lsc[wave][q][lane] >= thr) {
+                            const uint32_t slot_k = atomicAdd(&s_ncand[wave][j], 1u);
+                            if (slot_k < kMaxCandidates) s_cand[wave][j][slot_k] = s_lac[wave][q][lane];
+                        }
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
                 const uint32_t ncand = s_ncand[wave][j];
                 fail = fail || (want && (ncand == 0 || ncand > kMaxCandidates));
                 const bool resc = active && !fail;
@@ -1731,8 +1821,9 @@ void LaunchPackBf16(const uint8_t* slab, uint64_t stride, const uint32_t* slots,
                     const uint32_t* img, uint32_t A, uint8_t* out, uint64_t out_stride, uint32_t* tail_absmax,
                     hipStream_t s) {
     const uint32_t a_tiles = (A + 15) / 16;
-    const uint32_t blocks = a_tiles < 16 ? a_tiles : 16;
-    hipLaunchKernelGGL(PackBf16Kernel, dim3(blocks, n), dim3(128), 0, s, slab, stride, slots, slot_count, img, A, a_tiles,
+    const uint32_t groups = (a_tiles + kPackTiles - 1) / kPackTiles;
+    const uint32_t blocks = groups < 16 ? groups : 16;
+    hipLaunchKernelGGL(PackBf16Kernel, dim3(blocks, n), dim3(320), 0, s, slab, stride, slots, slot_count, img, A, a_tiles,
                        out, out_stride, tail_absmax);
 }
 

@@ -32,7 +32,9 @@ struct EvalArgs {
     const float* enc_t;            // [N][kEncTStride]: transposed encoder, then W0[:, 0], per genome
     const float* fixed_absmax;     // [26] max over the catalog of |fixed feature d| (BF16 bound)
     uint32_t force_fallback;       // BF16 mode: settle every decision with the FP32 scan (diagnostic)
-    const uint8_t* bf16_ops;       // BF16 mode: [N][a_tiles][kBf16TileBytes] screening operands
+    const uint8_t* bf16_cat;       // BF16 mode: [catalog tiles][kBf16CatTileBytes] catalog screening operands
+    uint32_t bf16_cat_bytes;
+    const uint8_t* bf16_ops;       // BF16 mode: [N][a_tiles][kBf16GenTileBytes] per-genome screening operands
     uint64_t bf16_stride;          // bytes per genome in bf16_ops
     const float* tail_absmax;      // BF16 mode: [N][40] max |tail value| per tail dim
     float* raw_fitness;            // [N] summed episode scores

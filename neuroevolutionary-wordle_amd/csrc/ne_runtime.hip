@@ -85,6 +85,8 @@ struct ne_runtime {
     uint32_t* canon = nullptr;  // lowest catalog index holding the same word
     uint32_t* img = nullptr;    // scoring image, kImgTileWords per tile of 16 words
     float* fixed_absmax = nullptr;  // [26] max over the catalog of |fixed feature|
+    uint8_t* bf16_cat = nullptr;    // [tiles][kBf16CatTileBytes] BF16 screening operands of fixed dims 0..23
+    uint64_t bf16_cat_bytes = 0;
     uint64_t img_bytes = 0;
     uint64_t catalog_count = 0;
     std::vector<uint8_t> h_catalog;
@@ -129,6 +131,15 @@ template <typename T>
 void Free(T*& p) {
     if (p) (void)hipFree(p);
     p = nullptr;
+}
+
+// float -> bfloat16 bits, round to nearest even (NaN stays a quiet NaN)
+uint16_t Bf16Bits(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return uint16_t((u >> 16) | 0x40u);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
 }
 
 int Fail(ne_runtime* rt, int code) {
@@ -396,6 +407,8 @@ static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard
     a.img = rt->img;
     a.img_bytes = uint32_t(rt->img_bytes);
     a.fixed_absmax = rt->fixed_absmax;
+    a.bf16_cat = rt->bf16_cat;
+    a.bf16_cat_bytes = uint32_t(rt->bf16_cat_bytes);
     a.force_fallback = (rt->eval_mode & NE_EVAL_FORCE_FALLBACK) ? 1u : 0u;
     const bool bf16 = (rt->eval_mode & 0xFFu) == NE_EVAL_BF16_RESCORED;
     a.raw_fitness = rt->raw;
@@ -427,7 +440,7 @@ static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard
         return Fail(rt, NE_DEVICE_FAILURE);
     a.enc_t = rt->enc_t;
     if (bf16) {
-        const uint64_t per_genome = uint64_t((a.action_count + 15) / 16) * kBf16TileBytes;
+        const uint64_t per_genome = uint64_t((a.action_count + 15) / 16) * kBf16GenTileBytes;
         if (per_genome * rt->max_gen > rt->bf16_ops_bytes) {
             Free(rt->bf16_ops);
             rt->bf16_ops_bytes = 0;
@@ -835,6 +848,7 @@ void ne_destroy(ne_runtime* rt) {
     Free(rt->canon);
     Free(rt->img);
     Free(rt->fixed_absmax);
+    Free(rt->bf16_cat);
     Free(rt->owner);
     Free(rt->move_src);
     Free(rt->d_survivors);
@@ -889,18 +903,31 @@ int ne_upload_catalog(ne_runtime* rt, const uint8_t* letters, uint64_t word_coun
         for (int i = 0; i < 16; ++i) tile[512 + i] = t * 16 + i < word_count ? canon[t * 16 + i] : kInvalidSlot;
     }
     canon.resize(tiles * 16, kInvalidSlot);  // whole tiles, for the screening stream's transfers
+    // BF16 screening operands of instruction 0, lanes 0..47 (fixed dims 0..23), round-to-nearest-even
+    std::vector<uint16_t> bf16_cat(tiles * kBf16CatTileBytes / 2, 0);
+    for (uint64_t t = 0; t < tiles; ++t)
+        for (int lane = 0; lane < 48; ++lane) {
+            const uint64_t a = t * 16 + (lane & 15);
+            if (a >= word_count) continue;
+            for (int e = 0; e < 8; ++e)
+                bf16_cat[(t * 48 + lane) * 8 + e] = Bf16Bits(fixed[a * kFixed + Bf16SlotDim(0, 8 * (lane >> 4) + e)]);
+        }
     Free(rt->codes);
     Free(rt->canon);
     Free(rt->img);
     Free(rt->fixed_absmax);
+    Free(rt->bf16_cat);
     rt->img_bytes = tiles * kImgTileBytes;
+    rt->bf16_cat_bytes = bf16_cat.size() * 2;
     if (!Alloc(&rt->fixed_absmax, kFixed) ||
         !Ok(hipMemcpy(rt->fixed_absmax, absmax, sizeof(absmax), hipMemcpyHostToDevice)))
         return Fail(rt, NE_DEVICE_FAILURE);
     if (!Alloc(&rt->codes, word_count) || !Alloc(&rt->canon, canon.size()) || !Alloc(&rt->img, img.size()) ||
         !Ok(hipMemcpy(rt->codes, codes.data(), word_count * 4, hipMemcpyHostToDevice)) ||
         !Ok(hipMemcpy(rt->canon, canon.data(), canon.size() * 4, hipMemcpyHostToDevice)) ||
-        !Ok(hipMemcpy(rt->img, img.data(), img.size() * 4, hipMemcpyHostToDevice)))
+        !Ok(hipMemcpy(rt->img, img.data(), img.size() * 4, hipMemcpyHostToDevice)) ||
+        !Alloc(&rt->bf16_cat, rt->bf16_cat_bytes) ||
+        !Ok(hipMemcpy(rt->bf16_cat, bf16_cat.data(), rt->bf16_cat_bytes, hipMemcpyHostToDevice)))
         return Fail(rt, NE_DEVICE_FAILURE);
     rt->catalog_count = word_count;
     rt->h_catalog.assign(letters, letters + 5 * word_count);
