@@ -37,6 +37,11 @@ __device__ inline v4f Buf4(Rsrc r, uint32_t voff, uint32_t soff) {
     return __builtin_bit_cast(v4f, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
 }
 
+// LDS DMA (buffer_load ... lds): an LDS destination pointer, and the wait for a wave's transfers.
+typedef __attribute__((address_space(3))) void* LdsPtr;
+// vmcnt(0), leaving the other counters alone: this wave's chunk transfers have landed in LDS.
+__device__ __forceinline__ void WaitChunk() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Register-only Wordle feedback (no per-letter count array): a non-green position p of letter L is
 // yellow iff the number of earlier non-green positions of letter L is below the number of
 // non-green solution positions holding L -- the same result as the reference's two passes
@@ -234,6 +239,126 @@ __device__ __forceinline__ void TrunkToPolicy(const v4f (&acc)[16], const TrunkL
     for (int s = 0; s < kScoreSteps; ++s) pol[s] = ScoreDim(s, int(tl.gq)) < 0 ? 0.0f : pv[s / 4][s % 4];
 }
 
+// ---- layers 1 and 2 with their weights streamed through LDS ----------------------------------
+// The weights of layers 1 and 2 are the same for every wave of a workgroup, so the workgroup moves
+// them once per decision by LDS DMA, chunk i + 1 in flight while chunk i is used, instead of every
+// wave loading them into registers.  Chunk t < 16 is W1[0..128)[16t..16t+16) as [128][16] (8 KB);
+// chunk 16 + c is W2[0..64)[32c..32c+32) as [64][32] (8 KB).  A transfer (one instruction, 64 lanes
+// x 16 bytes) covers 16 rows of a W1 chunk or 8 rows of a W2 chunk; its per-lane byte offset is
+// fixed and the chunk / transfer part is uniform (an SGPR), so issuing needs no per-lane arithmetic.
+// The MFMA order is exactly that of TrunkToPolicy, so the results are identical.
+constexpr uint32_t kTrunkDma = 8;                       // transfers per chunk
+constexpr uint32_t kTrunkW1Chunks = kH0 / 16;           // 16
+constexpr uint32_t kTrunkW2Chunks = kH1 / 32;           // 4
+constexpr uint32_t kTrunkChunkBytes = kTrunkDma * 1024;
+static_assert(kH1 * 16 * 4 == kTrunkChunkBytes && kPol * 32 * 4 == kTrunkChunkBytes, "8 KB chunks");
+
+// Per-lane byte offsets of the two transfer shapes (lane l: W1 row l/4, inputs 4(l%4)..+3; W2 row
+// l/8, inputs 4(l%8)..+3, relative to the transfer's first row and the chunk's first input).
+__device__ __forceinline__ uint32_t TrunkLaneW1(uint32_t lane) { return ((lane >> 2) * kH0 + 4 * (lane & 3u)) * 4; }
+__device__ __forceinline__ uint32_t TrunkLaneW2(uint32_t lane) { return ((lane >> 3) * kH1 + 4 * (lane & 7u)) * 4; }
+
+// Starts the transfers of trunk chunk c (0 .. kTrunkW1Chunks + kTrunkW2Chunks - 1) into buf; the
+// waves of the workgroup take transfers round robin.
+__device__ __forceinline__ void IssueTrunkChunk(uint8_t* buf, Rsrc gr, uint32_t c, uint32_t wave, uint32_t waves,
+                                                uint32_t lw1, uint32_t lw2) {
+    const bool w1 = c < kTrunkW1Chunks;
+    const uint32_t voff = w1 ? lw1 : lw2;
+    const uint32_t base = w1 ? uint32_t(kOffW1 + 16 * c) * 4 : uint32_t(kOffW2 + 32 * (c - kTrunkW1Chunks)) * 4;
+    const uint32_t row_bytes = w1 ? 16u * kH0 * 4 : 8u * kH1 * 4;  // rows per transfer x row length
+    for (uint32_t i = wave; i < kTrunkDma; i += waves)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(gr, (LdsPtr)(buf + i * 1024), 16, voff, base + i * row_bytes, 0, 0);
+}
+
+// Buffers handed to TrunkStream hold its largest layer-2 read: a pad row (19 + 30 + 3 + 16), last 16 bytes.
+constexpr uint32_t kTrunkReadBytes = (19 + 30 + 3 + 16) * 32 * 4 + 64 + 48 + 16;
+
+// Layers 1 and 2 for the whole workgroup (every wave calls it; `compute` = this wave has games).
+// Chunk 0 must be in flight into b0; chunk i uses b0 when i is even, b1 when odd.  During the last
+// chunk (in b1) next_stage(b0) starts whatever follows (the scoring stream's first chunk).
+template <typename FN>
+__device__ __forceinline__ void TrunkStream(const v4f (&acc)[16], const TrunkLane& tl, float (&pol)[kScoreSteps],
+                                            bool compute, uint8_t* b0, uint8_t* b1, uint32_t wave, uint32_t waves,
+                                            uint32_t lane, FN&& next_stage) {
+    static_assert(kTrunkW1Chunks % 2 == 0 && kTrunkW2Chunks % 2 == 0, "the last chunk sits in b1");
+    const uint32_t lw1 = TrunkLaneW1(lane), lw2 = TrunkLaneW2(lane);
+    v4f h1[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) h1[u] = Buf4(tl.gr, tl.off_b1, 64 * u);
+    constexpr int kRowTiles = (kScoreSteps + 3) / 4;
+    v4f pv[kRowTiles];
+    WaitChunk();
+    __syncthreads();
+    // layer 1 (the two loops are each unrolled whole, so every accumulator index is static)
+    const uint32_t rd1 = 64 * tl.j + 16 * tl.gq;  // row 16u + j, inputs 4gq..4gq+3 of a W1 chunk
+#pragma unroll
+    for (uint32_t t = 0; t < kTrunkW1Chunks; ++t) {
+        uint8_t* cur = (t & 1u) ? b1 : b0;
+        uint8_t* nxt = (t & 1u) ? b0 : b1;
+        IssueTrunkChunk(nxt, tl.gr, t + 1, wave, waves, lw1, lw2);
+        if (t == kTrunkW1Chunks - 1) {  // layer-2 biases, landing during the last layer-1 chunk
+            const uint32_t bias_f = uint32_t(kOffB2 + 7 * tl.gq) * 4, bias_t = uint32_t(kOffB2 + 19 + 10 * tl.gq) * 4;
+#pragma unroll
+            for (int u = 0; u < kRowTiles; ++u)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int st = 4 * u + r < kScoreSteps ? 4 * u + r : 0;
+                    pv[u][r] = BufF(tl.gr, st < kFixedSteps ? bias_f : bias_t, uint32_t(4 * st));
+                }
+        }
+        if (compute) {
+            float hv[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) hv[r] = fmaxf(acc[t][r], 0.0f);
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) {
+                const v4f w = *reinterpret_cast<const v4f*>(cur + u * 1024 + rd1);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) h1[u] = Mfma(w[r], hv[r], h1[u]);
+            }
+        }
+        WaitChunk();
+        __syncthreads();
+    }
+    // layer 2: chunk c holds inputs 32c..32c+31, i.e. input tiles t = 2c, 2c + 1.  Lane (j, gq) of row
+    // tile u reads row ScoreDim(4u + j%4, j/4): 7jg + jr + 4u while that is a fixed dim (u = 0, and
+    // u = 1 for jr < 3), else 19 + 10jg + jr + 4u -- three base addresses plus immediates.  Rows past
+    // 63 are pads (their results are zeroed below); they read bytes of the buffer past the chunk.
+    const uint32_t jr = tl.j & 3u, jg = tl.j >> 2;
+    const uint32_t rd_f = (7 * jg + jr) * (32 * 4) + 16 * tl.gq;
+    const uint32_t rd_t = (19 + 10 * jg + jr) * (32 * 4) + 16 * tl.gq;
+    const uint32_t rd_1 = jr + 4 < uint32_t(kFixedSteps) ? rd_f + 512 : rd_t + 512;
+#pragma unroll
+    for (uint32_t c = 0; c < kTrunkW2Chunks; ++c) {
+        uint8_t* cur = (c & 1u) ? b1 : b0;
+        uint8_t* nxt = (c & 1u) ? b0 : b1;
+        if (c + 1 < kTrunkW2Chunks)
+            IssueTrunkChunk(nxt, tl.gr, kTrunkW1Chunks + c + 1, wave, waves, lw1, lw2);
+        else
+            next_stage(nxt);
+        if (compute) {
+#pragma unroll
+            for (uint32_t h = 0; h < 2; ++h) {
+                const uint32_t t = 2 * c + h;
+                float hv[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) hv[r] = fmaxf(h1[t][r], 0.0f);
+#pragma unroll
+                for (int u = 0; u < kRowTiles; ++u) {
+                    const uint32_t rd = u == 0 ? rd_f : (u == 1 ? rd_1 : rd_t + 512 * uint32_t(u));
+                    const v4f w = *reinterpret_cast<const v4f*>(cur + rd + 64 * h);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) pv[u] = Mfma(w[r], hv[r], pv[u]);
+                }
+            }
+        }
+        WaitChunk();
+        __syncthreads();
+    }
+#pragma unroll
+    for (int s = 0; s < kScoreSteps; ++s) pol[s] = ScoreDim(s, int(tl.gq)) < 0 ? 0.0f : pv[s / 4][s % 4];
+}
+
 // Stages a genome's 145->64 encoder into LDS, transposed to [feature][out].
 __device__ __forceinline__ void StageEncoder(const float* genome, float* s_enc, float* s_encb) {
     for (uint32_t idx = threadIdx.x; idx < uint32_t(kEncOut * kTurnFeatures); idx += blockDim.x) {
@@ -355,7 +480,6 @@ struct ChunkGeom {
 static_assert(kImgTileBytes % 16 == 0 && (16 * kTail * 4) % 16 == 0, "16-byte transfer units");
 
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
-typedef __attribute__((address_space(3))) void* LdsPtr;
 
 // Issues the DMA transfers of chunk c into one LDS buffer, spread over the workgroup's waves.  Bytes
 // past a descriptor's size arrive as 0.  The caller waits (vmcnt) and barriers before reading.
@@ -374,8 +498,6 @@ __device__ __forceinline__ void IssueChunk(uint8_t* buf, Rsrc img_r, Rsrc tail_r
     }
 }
 
-// vmcnt(0), leaving the other counters alone: this wave's chunk transfers have landed in LDS.
-__device__ __forceinline__ void WaitChunk() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // Streams every chunk of the scoring image and tail rows through two LDS buffers (b0, b1: distinct
 // objects), chunk c + 1 in flight while chunk c is used.  issue(buf, c) starts the transfers of
@@ -720,10 +842,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
         for (int step = 0; step < kMaxTurns; ++step) {
             if (!__syncthreads_or(active ? 1 : 0)) break;  // the whole workgroup leaves together
             const bool wave_active = __any(active ? 1 : 0) != 0;
-            if constexpr (kBf16)
-                issue16(s_buf0, 0);  // lands behind the trunk
-            else
-                issue(s_buf0, 0);
+            IssueTrunkChunk(s_buf0, tl.gr, 0, wave, waves, TrunkLaneW1(lane), TrunkLaneW2(lane));  // lands behind the fold
             float pol[kScoreSteps];
             if (wave_active) {
                 const int ttc = base_tc + step;  // turn count of every active game in this tile
@@ -744,8 +863,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
                     FoldSlot(acc, tl, ts, e);
                 }
                 folded = ttc;
-                TrunkToPolicy(acc, tl, pol);
-            } else {
+            }
+            // layers 1 and 2; the scoring stream's first chunk goes into s_buf0 during the last chunk
+            static_assert(kTrunkChunkBytes <= kBufBytes && kTrunkReadBytes <= kBufBytes, "trunk chunk");
+            TrunkStream(acc, tl, pol, wave_active, s_buf0, s_buf1, wave, waves, lane, [&](uint8_t* b) __device__ {
+                if constexpr (kBf16)
+                    issue16(b, 0);
+                else
+                    issue(b, 0);
+            });
+            if (!wave_active) {
 #pragma unroll
                 for (int s = 0; s < kScoreSteps; ++s) pol[s] = 0.0f;
             }
