@@ -163,17 +163,29 @@ __device__ __forceinline__ void EncodeTurnGlobal(Rsrc enc_t, const TrunkLane& tl
     }
 }
 
-// acc += W0[:, 1+64*slot .. 1+64*slot+63] . e
+// acc += W0[:, 1+64*slot .. 1+64*slot+63] . e.  Lane (j, gq)'s 16 weights of neuron tile t are
+// contiguous (row 16t + j, inputs 16gq..16gq+15 of the slot): four 128-bit loads (buffer loads need
+// only dword alignment, and rows of 321 weights start at every dword phase), issued one tile ahead.
 __device__ __forceinline__ void FoldSlot(v4f (&acc)[16], const TrunkLane& tl, int slot, const float (&e)[16]) {
     const uint32_t soff_slot = uint32_t(64 * slot) * 4;
+    v4f a[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a[q] = Buf4(tl.gr, tl.off_fold, soff_slot + 16 * q);
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
-        float a[16];
+        v4f nx[4];
+        if (t + 1 < 16) {
 #pragma unroll
-        for (int s = 0; s < 16; ++s) a[s] = BufF(tl.gr, tl.off_fold, soff_slot + uint32_t(16 * t * kModelIn + s) * 4);
+            for (int q = 0; q < 4; ++q)
+                nx[q] = Buf4(tl.gr, tl.off_fold, soff_slot + uint32_t(16 * (t + 1) * kModelIn) * 4 + 16 * q);
+        }
 #pragma unroll
-        for (int s = 0; s < 16; ++s) acc[t] = Mfma(a[s], e[s], acc[t]);
+        for (int s = 0; s < 16; ++s) acc[t] = Mfma(a[s / 4][s % 4], e[s], acc[t]);
         __builtin_amdgcn_sched_barrier(0);
+        if (t + 1 < 16) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a[q] = nx[q];
+        }
     }
 }
 
