@@ -264,6 +264,72 @@ int ne_set_eval_precision(ne_runtime *rt, uint32_t mode);
  * entry, then 2W prefilled games: slot W+q is entry q/2, variant 1+q%2). */
 int ne_download_trace(ne_runtime *rt, uint32_t kind, void *dst, uint64_t capacity_bytes, uint64_t *written);
 
+/* ---- surrogate-screened evolution ---------------------------------------------------------------
+ * No counterpart in the reference (which evaluates every child, slab_runtime.cu:1554-1661): an
+ * oversampled brood per cell is assembled, every child is fingerprinted (its guesses on 64 fixed probe
+ * states) and scored by a small learned surrogate, and only each cell's best-predicted child plus a
+ * random audit slice of the brood is evaluated in full.  The surrogate (MLP 192 -> 128 -> 64 -> 1)
+ * trains online with Adam on the (fingerprint, fitness) pairs of every fully evaluated child.  While
+ * the audit's rank correlation is below the threshold, or too few pairs have been seen, generations
+ * run plain (one child per cell, all evaluated) and keep feeding the surrogate. */
+typedef struct ne_screen_config {
+    uint32_t brood_factor;       /* children per cell while screening (4: the best-predicted 25% go on) */
+    uint32_t replay_capacity;    /* (fingerprint, fitness) pairs kept for training (65,536) */
+    float audit_fraction;        /* random slice of the brood always evaluated in full (0.05) */
+    float spearman_threshold;    /* screening pauses while the audit's rank correlation is below (0.5) */
+    float learning_rate;         /* Adam step size (1e-3) */
+    uint32_t train_steps;        /* minibatch steps per generation (32) */
+    uint32_t batch_size;         /* minibatch size, 1..1024 (256) */
+    uint32_t min_replay;         /* pairs seen before screening may start (4,096) */
+    uint32_t seed;               /* probe states, surrogate initialisation, audit and minibatch draws */
+    uint32_t reserved0;
+} ne_screen_config;
+
+typedef struct ne_screen_report {
+    uint32_t screened;           /* 1: this step screened a brood; 0: plain step (warm-up or fallback) */
+    uint32_t next_screened;      /* what the next step will do */
+    uint32_t brood;              /* children assembled */
+    uint32_t evaluated;          /* children evaluated in full */
+    uint32_t audited;            /* brood members in the audit slice */
+    float audit_spearman;        /* rank correlation of prediction and fitness over the audit slice
+                                    (predictions made before training on them; NaN below 3 members) */
+    float train_loss;            /* 1/2 mean squared error of the last minibatch (standardised fitness) */
+    float fingerprint_ms;        /* encoder transpose + fingerprint kernel */
+    float surrogate_ms;          /* prediction + training */
+    float best_fitness;          /* of the new generation */
+    float average_fitness;
+    uint32_t reserved0;
+    uint64_t replay_count;       /* pairs in the replay buffer */
+} ne_screen_report;
+
+/* Allocates the surrogate, its Adam state, the replay buffer and the 64 probe states (drawn from the
+ * first action_count catalog words).  The generation budget must hold brood_factor x population
+ * children and the slab one more population.  Calling it again resets the surrogate. */
+int ne_enable_screening(ne_runtime *rt, const ne_screen_config *config);
+/* One screened generation step (no widening or injection).  When the current generation has no
+ * fitness of its own evaluation yet it is evaluated first; on return the new generation is evaluated
+ * (its fitness is that of the full evaluation of each selected child). */
+int ne_advance_generation_screened(ne_runtime *rt, uint32_t generation_seed, const ne_word_counts *counts,
+                                   const ne_assembly_config *config, const ne_shard_release_history *history,
+                                   ne_screen_report *report);
+/* Fingerprints of `count` genomes (slot indices) with the given parent fitness pairs ([count][2]):
+ * out[count][192], per probe p = 16 d + k (board k at depth d): [3p] (2 greens + yellows) / 10 of the
+ * genome's guess against the probe secret, [3p+1] d >= 1: 1 if the guess fits every hint on the board,
+ * d = 0: 10 x the fitness of parent k % 2, [3p+2] 1 if the guess is the secret. */
+int ne_fingerprint(ne_runtime *rt, const uint32_t *slot_indices, const float *parent_fitness, uint64_t count,
+                   float *out);
+/* The probe states: per board k (16), the secret's catalog index then its 3 turns' guess indices. */
+int ne_get_probe_states(ne_runtime *rt, uint32_t *dst, uint64_t capacity, uint64_t *count);
+/* Surrogate access (host arrays): predictions, one Adam step on (x, t), parameters (33,025 floats:
+ * W1T [192][128], b1 [128], W2T [128][64], b2 [64], w3 [64], b3).  Setting parameters zeroes the Adam
+ * state. */
+int ne_surrogate_predict(ne_runtime *rt, const float *x, uint64_t count, float *y);
+int ne_surrogate_train_step(ne_runtime *rt, const float *x, const float *t, uint64_t count);
+int ne_surrogate_get_params(ne_runtime *rt, float *dst, uint64_t capacity, uint64_t *count);
+int ne_surrogate_set_params(ne_runtime *rt, const float *src, uint64_t count);
+/* Host only: Spearman rank correlation (average ranks for ties); NaN when n < 3 or a side is constant. */
+double ne_rank_correlation(const float *a, const float *b, uint64_t n);
+
 #ifdef __cplusplus
 }
 #endif

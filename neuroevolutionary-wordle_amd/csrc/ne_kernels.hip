@@ -1157,6 +1157,99 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
     }
 }
 
+// ================================================================================================
+// Child fingerprint for surrogate screening.  One workgroup of 4 waves per child; wave d plays the
+// 16 probe boards of depth d (lane column k = probe secret k): the turns are folded into layer 0
+// exactly as in a game, layers 1 and 2 stream through LDS, all actions are scored in FP32 with the
+// masked argmax of a decision.  Per probe p = 16 d + k the child's guess g yields 3 inputs:
+//   [3p]     quality: (2 greens + yellows) / 10 of g against probe secret k;
+//   [3p + 1] depth >= 1: 1 if g is consistent with every turn on the board (it would have produced the
+//            same feedback as the secret), else 0; depth 0 (no hints): the fitness of parent k % 2,
+//            as (f - shift) * scale;
+//   [3p + 2] 1 if g is the secret.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void FingerprintKernel(FingerprintArgs p) {
+    constexpr int kCT = kChunkTiles;
+    using G = ChunkGeom<kCT>;
+    __shared__ __align__(16) uint8_t s_buf0[G::kBufBytes];
+    __shared__ __align__(16) uint8_t s_buf1[G::kBufBytes];
+    static_assert(kTrunkChunkBytes <= G::kBufBytes && kTrunkReadBytes <= G::kBufBytes, "trunk chunk");
+    const uint32_t n = blockIdx.x;
+    const uint32_t slot = p.slots[n];
+    if (slot >= p.slot_count) {
+        if (threadIdx.x == 0) SetStatus(p.status, 4);  // kInvalidGeneration
+        return;
+    }
+    const float* genome = reinterpret_cast<const float*>(p.slab + uint64_t(slot) * p.stride);
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, j = lane & 15u, gq = lane >> 4;
+    const uint32_t waves = blockDim.x >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t d = wave;  // turns on the board
+    const TrunkLane tl = MakeTrunkLane(genome, j, gq);
+    const Rsrc enc_r = MakeRsrc(p.enc_t + uint64_t(n) * kEncTStride, uint32_t(kEncTStride * 4));
+    const uint32_t A = p.action_count;
+    const uint32_t a_tiles = (A + 15) / 16;
+    const uint32_t chunks = (a_tiles + kCT - 1) / kCT;
+    const Rsrc img_r = MakeRsrc(p.img, p.img_bytes);
+    const Rsrc tail_r = MakeRsrc(genome + kPolicyParams, uint32_t(p.stride - kPolicyParams * 4));
+    auto issue = [&](uint8_t* buf, uint32_t c) __device__ {
+        IssueChunk(buf, img_r, tail_r, c, wave, waves, lane, G::kImgBytes, G::kTailBytes, G::kImgDma, G::kTailDma);
+    };
+    const ProbeSet& ps = *p.probes;
+    uint32_t hist[5] = {kInvalidSlot, kInvalidSlot, kInvalidSlot, kInvalidSlot, kInvalidSlot};
+    for (uint32_t t = 0; t < d; ++t) hist[t] = ps.turn_canon[j][t];
+
+    IssueTrunkChunk(s_buf0, tl.gr, 0, wave, waves, TrunkLaneW1(lane), TrunkLaneW2(lane));  // lands behind the fold
+    v4f acc[16];
+    InitLayer0(acc, tl);
+    if (d == 0) {  // virgin flag: the board is empty
+#pragma unroll
+        for (int t = 0; t < 16; ++t) acc[t] += Buf4(enc_r, uint32_t(kTurnFeatures * kEncOut + 16 * t + 4 * gq) * 4, 0);
+    }
+    for (uint32_t ts = 0; ts < d; ++ts) {
+        float e[16];
+        EncodeTurnGlobal(enc_r, tl, ps.turn_code[j][ts], ps.turn_fb[j][ts], e);
+        FoldSlot(acc, tl, int(ts), e);
+    }
+    float pol[kScoreSteps];
+    TrunkStream(acc, tl, pol, true, s_buf0, s_buf1, wave, waves, lane, [&](uint8_t* b) __device__ { issue(b, 0); });
+    ArgmaxLane am = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot};
+    StreamChunks<kCT>(s_buf0, s_buf1, chunks, true, issue, [&](const uint8_t* buf, int k, uint32_t at) __device__ {
+        ScoreTile<false>(Fp32Img<kCT>(buf, k), Fp32Tails<kCT>(buf, k), lane, j, gq, pol, at * 16 + 4 * gq, A, hist, am);
+    });
+    float best = am.best, second = am.second;
+    uint32_t besta = am.besta;
+    ReduceArgmax(best, second, besta);
+    if (gq != 0) return;
+    float* out = p.out + uint64_t(n) * kFingerprintDims + 3 * (d * kProbeSecrets + j);
+    if (besta == kInvalidSlot) {  // nothing selectable (every action masked)
+        out[0] = 0.0f;
+        out[1] = 0.0f;
+        out[2] = 0.0f;
+        return;
+    }
+    const uint32_t g = p.codes[besta], sc = ps.secret_code[j];
+    const uint32_t fb = FeedbackRegs(g, sc);
+    int greens = 0, yellows = 0;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        const uint32_t tv = (fb >> (2 * q)) & 3u;
+        greens += tv == 0u ? 1 : 0;
+        yellows += tv == 1u ? 1 : 0;
+    }
+    float c;
+    if (d == 0) {
+        const uint32_t parent = p.pairs[2 * n + (j & 1u)];
+        c = (p.parent_fitness[parent] - p.parent_shift) * p.parent_scale;
+    } else {
+        bool consistent = true;
+        for (uint32_t t = 0; t < d; ++t) consistent = consistent && FeedbackRegs(ps.turn_code[j][t], g) == ps.turn_fb[j][t];
+        c = consistent ? 1.0f : 0.0f;
+    }
+    out[0] = float(2 * greens + yellows) * 0.1f;
+    out[1] = c;
+    out[2] = g == sc ? 1.0f : 0.0f;
+}
+
 // Fitness normalisation (FinalizeSlabGenerationFitnessKernel, slab_runtime.cu:551-586 and
 // NormalizeFitnessForSelection, evaluation_ops.cuh:97-111).
 __global__ void FinalizeKernel(const float* raw, const uint32_t* local_w, uint32_t n_genomes, float* fitness,
@@ -1964,6 +2057,10 @@ void LaunchPackBf16(const uint8_t* slab, uint64_t stride, const uint32_t* slots,
     const uint32_t blocks = groups < 16 ? groups : 16;
     hipLaunchKernelGGL(PackBf16Kernel, dim3(blocks, n), dim3(320), 0, s, slab, stride, slots, slot_count, img, A, a_tiles,
                        out, out_stride, tail_absmax);
+}
+
+void LaunchFingerprint(const FingerprintArgs& a, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(FingerprintKernel, dim3(n), dim3(64 * kProbeDepths), 0, s, a);
 }
 
 void LaunchTransposeEncoder(const uint8_t* slab, uint64_t stride, const uint32_t* slots, uint32_t n,

@@ -141,6 +141,69 @@ struct InjectArgs {
     int* status;
 };
 
+// ---- surrogate screening --------------------------------------------------------------------
+// The 64 probe game states of a child's fingerprint: 16 secrets x boards of 0..3 turns (probe
+// p = 16 d + k holds the first d turns of board k).
+constexpr int kProbeSecrets = 16;
+constexpr int kProbeDepths = 4;
+constexpr int kProbes = kProbeSecrets * kProbeDepths;
+constexpr int kFingerprintDims = 3 * kProbes;  // 192
+struct ProbeSet {
+    uint32_t secret_code[kProbeSecrets];
+    uint32_t turn_code[kProbeSecrets][kProbeDepths - 1];
+    uint32_t turn_fb[kProbeSecrets][kProbeDepths - 1];
+    uint32_t turn_canon[kProbeSecrets][kProbeDepths - 1];
+};
+
+struct FingerprintArgs {
+    const uint8_t* slab;
+    uint64_t stride;
+    uint32_t slot_count;
+    uint32_t action_count;
+    const uint32_t* slots;         // [n] children
+    const uint32_t* codes;
+    const uint32_t* img;
+    uint32_t img_bytes;
+    const float* enc_t;            // [n][kEncTStride] (TransposeEncoderKernel over `slots`)
+    const ProbeSet* probes;
+    const uint32_t* pairs;         // [n][2] parent positions in the parent generation
+    const float* parent_fitness;   // [parents]
+    float parent_shift, parent_scale;  // parent fitness input = (f - shift) * scale
+    float* out;                    // [n][kFingerprintDims]
+    int* status;
+};
+
+void LaunchFingerprint(const FingerprintArgs& a, uint32_t n, hipStream_t s);
+
+// Surrogate MLP 192 -> 128 -> 64 -> 1 (ReLU, ReLU, linear), fp32.  Parameters in one array: W1T
+// [192][128] (input-major), b1 [128], W2T [128][64], b2 [64], w3 [64], b3.
+constexpr uint32_t kSgIn = kFingerprintDims, kSgH1 = 128, kSgH2 = 64;
+constexpr uint32_t kSgOffW1 = 0;
+constexpr uint32_t kSgOffB1 = kSgOffW1 + kSgIn * kSgH1;
+constexpr uint32_t kSgOffW2 = kSgOffB1 + kSgH1;
+constexpr uint32_t kSgOffB2 = kSgOffW2 + kSgH1 * kSgH2;
+constexpr uint32_t kSgOffW3 = kSgOffB2 + kSgH2;
+constexpr uint32_t kSgOffB3 = kSgOffW3 + kSgH2;
+constexpr uint32_t kSgParams = kSgOffB3 + 1;  // 33,025
+
+void LaunchSurrogateInit(float* params, float* adam_m, float* adam_v, uint32_t seed, hipStream_t s);
+// y[n] = f(x[n][192]); h1 [n][128] and h2 [n][64] (post-ReLU) are kept when non-null.
+void LaunchSurrogateForward(const float* params, const float* x, uint32_t n, float* y, float* h1, float* h2,
+                            hipStream_t s);
+// Loss 1/(2n) sum (y - t)^2: dy [n], dh2 [n][64], dh1 [n][128].
+void LaunchSurrogateBackprop(const float* params, const float* h1, const float* h2, const float* y, const float* t,
+                             uint32_t n, float* dy, float* dh1, float* dh2, hipStream_t s);
+// Gradient of every parameter over the batch, then one Adam step (beta 0.9 / 0.999, eps 1e-8);
+// c1 = 1 / (1 - 0.9^step), c2 = 1 / (1 - 0.999^step).
+void LaunchSurrogateAdam(float* params, float* adam_m, float* adam_v, const float* x, const float* h1, const float* h2,
+                         const float* dy, const float* dh1, const float* dh2, uint32_t n, float lr, float c1, float c2,
+                         hipStream_t s);
+// Replay buffer: rows (head + e) % cap <- fingerprint rows src_row[e] of fp, fitness[e].
+void LaunchReplayAppend(const float* fp, const uint32_t* src_row, const float* fitness, uint32_t count, float* rx,
+                        float* rf, uint32_t head, uint32_t cap, hipStream_t s);
+// Minibatch: x[b] = rx[idx[b]], t[b] = (rf[idx[b]] - mu) * inv_sigma.
+void LaunchReplayGather(const float* rx, const float* rf, const uint32_t* idx, uint32_t n, float mu, float inv_sigma,
+                        float* x, float* t, hipStream_t s);
 void LaunchInject(const InjectArgs& a, hipStream_t s);
 void LaunchPackBf16(const uint8_t* slab, uint64_t stride, const uint32_t* slots, uint32_t n, uint32_t slot_count,
                     const uint32_t* img, uint32_t A, uint8_t* out, uint64_t out_stride, uint32_t* tail_absmax,

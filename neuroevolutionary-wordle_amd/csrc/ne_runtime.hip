@@ -109,11 +109,38 @@ struct ne_runtime {
     uint64_t trace_capacity = 0;
 
     // timing
-    StageEvents ev[9];
+    StageEvents ev[11];
     ne_stage_times times{};
     ne_fitness_summary last_summary{};
     bool have_summary = false;
     int last_status = NE_OK;
+    bool cur_evaluated = false;  // the current generation's fitness is that of its own evaluation
+
+    // surrogate screening
+    bool screen_on = false;
+    bool screen_active = false;  // the next screened step assembles a brood
+    ne_screen_config screen{};
+    ProbeSet h_probes{};
+    uint32_t h_probe_words[kProbeSecrets][kProbeDepths] = {};  // secret index, then the 3 guesses' indices
+    ProbeSet* d_probes = nullptr;
+    float* sg_params = nullptr;
+    float* sg_m = nullptr;
+    float* sg_v = nullptr;
+    uint64_t sg_step = 0;
+    float* fp = nullptr;    // [max_gen][192] fingerprints of the brood
+    float* pred = nullptr;  // [max_gen]
+    float* e_fit = nullptr;
+    uint32_t* e_evals = nullptr;
+    uint8_t* e_has = nullptr;
+    uint32_t* e_slots = nullptr;
+    uint32_t* e_rows = nullptr;
+    float* replay_x = nullptr;
+    float* replay_f = nullptr;
+    std::vector<float> h_replay_f;
+    uint64_t replay_count = 0, replay_head = 0;
+    float *bx = nullptr, *bt = nullptr, *by = nullptr, *bh1 = nullptr, *bh2 = nullptr;
+    float *bdy = nullptr, *bdh1 = nullptr, *bdh2 = nullptr;
+    uint32_t* bidx = nullptr;
 };
 
 namespace {
@@ -331,43 +358,30 @@ int EnsureTrace(ne_runtime* rt, uint64_t games) {
 
 // TryEvaluateCurrentGenerationFitnessOnDevice (slab_runtime.cu:1086-1175): per-cell secrets, one
 // game-play kernel, then fitness normalisation and the population summary.
-static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard_release_history* hist) {
-    const uint64_t cols = rt->config.grid_column_count;
-    if (!wc) return Fail(rt, NE_INVALID_ARGUMENT);
-    if (rt->cur_size == 0 || cols == 0 || rt->cur_size % cols != 0) return Fail(rt, NE_INVALID_GENERATION);
-    if (rt->catalog_count == 0) return Fail(rt, NE_INVALID_TRAINING_SHARD);
-    if (wc->training_word_count > rt->catalog_count || wc->action_space_word_count > rt->catalog_count ||
-        wc->training_word_count > wc->action_space_word_count)
-        return Fail(rt, NE_INVALID_TRAINING_SHARD);
-    const uint64_t rows = rt->cur_size / cols;
-    std::vector<uint32_t> built_offsets, built_secrets;
-    const std::vector<uint32_t>* offs = &rt->h_offsets;
-    const std::vector<uint32_t>* secs = &rt->h_secrets;
-    if (!rt->csr_from_caller) {
-        const int st = BuildShardCsr(rt, wc, hist, rows, cols, built_offsets, built_secrets);
-        if (st != NE_OK) return Fail(rt, st);
-        offs = &built_offsets;
-        secs = &built_secrets;
-    } else if (rt->h_offsets.size() < rt->cur_size + 1) {
-        return Fail(rt, NE_INVALID_TRAINING_SHARD);
-    }
-    for (uint64_t i = 0; i < (*offs)[rt->cur_size]; ++i)
-        if ((*secs)[i] >= rt->catalog_count) return Fail(rt, NE_INVALID_TRAINING_SHARD);
-    const uint64_t n_secrets = (*offs)[rt->cur_size];
+// Evaluates n genomes: slots[e] (device) plays the 3 episode variants of every secret in
+// secs[offs[e] .. offs[e+1]).  Fitness, evaluation counts and has-fitness go to out_* (device, n each);
+// rt->times and rt->last_summary describe this evaluation.
+static int EvaluateCore(ne_runtime* rt, uint64_t action_space, const uint32_t* slots, uint64_t n,
+                        const std::vector<uint32_t>& offs, const std::vector<uint32_t>& secs, float* out_fit,
+                        uint32_t* out_evals, uint8_t* out_has, uint64_t generation_index) {
+    if (n == 0 || n > rt->max_gen || offs.size() < n + 1) return Fail(rt, NE_INVALID_GENERATION);
+    for (uint64_t i = 0; i < offs[n]; ++i)
+        if (secs[i] >= rt->catalog_count) return Fail(rt, NE_INVALID_TRAINING_SHARD);
+    const uint64_t n_secrets = offs[n];
     if (n_secrets > rt->d_secrets_cap) {
         Free(rt->d_secrets);
         if (!Alloc(&rt->d_secrets, n_secrets)) return Fail(rt, NE_DEVICE_FAILURE);
         rt->d_secrets_cap = n_secrets;
     }
-    bool ok = Ok(hipMemcpyAsync(rt->d_offsets, offs->data(), (rt->cur_size + 1) * sizeof(uint32_t),
+    bool ok = Ok(hipMemcpyAsync(rt->d_offsets, offs.data(), (n + 1) * sizeof(uint32_t),
                                 hipMemcpyHostToDevice, rt->stream));
     if (n_secrets)
-        ok &= Ok(hipMemcpyAsync(rt->d_secrets, secs->data(), n_secrets * sizeof(uint32_t), hipMemcpyHostToDevice,
+        ok &= Ok(hipMemcpyAsync(rt->d_secrets, secs.data(), n_secrets * sizeof(uint32_t), hipMemcpyHostToDevice,
                                 rt->stream));
     uint64_t max_tiles = 0, total_games = 0;
-    std::vector<uint64_t> game_offsets(rt->cur_size, 0);
-    for (uint64_t c = 0; c < rt->cur_size; ++c) {
-        const uint64_t W = (*offs)[c + 1] - (*offs)[c];
+    std::vector<uint64_t> game_offsets(n, 0);
+    for (uint64_t c = 0; c < n; ++c) {
+        const uint64_t W = offs[c + 1] - offs[c];
         if (W == 0) return Fail(rt, NE_INVALID_TRAINING_SHARD);
         max_tiles = std::max(max_tiles, (W + 15) / 16 + (2 * W + 15) / 16);
         game_offsets[c] = total_games;
@@ -377,15 +391,15 @@ static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard
     if (trace) {
         if (EnsureTrace(rt, total_games) != NE_OK) return rt->last_status;
         Free(rt->d_game_offsets);
-        if (!Alloc(&rt->d_game_offsets, rt->cur_size)) return Fail(rt, NE_DEVICE_FAILURE);
-        ok &= Ok(hipMemcpyAsync(rt->d_game_offsets, game_offsets.data(), rt->cur_size * sizeof(uint64_t),
+        if (!Alloc(&rt->d_game_offsets, n)) return Fail(rt, NE_DEVICE_FAILURE);
+        ok &= Ok(hipMemcpyAsync(rt->d_game_offsets, game_offsets.data(), n * sizeof(uint64_t),
                                 hipMemcpyHostToDevice, rt->stream));
         ok &= Ok(hipMemsetAsync(rt->d_trace, 0xFF, total_games * sizeof(GameTrace), rt->stream));
         rt->trace_games = total_games;
     }
-    ok &= Ok(hipMemsetAsync(rt->cur_fit, 0, rt->max_gen * sizeof(float), rt->stream));
-    ok &= Ok(hipMemsetAsync(rt->cur_evals, 0, rt->max_gen * sizeof(uint32_t), rt->stream));
-    ok &= Ok(hipMemsetAsync(rt->cur_has, 0, rt->max_gen, rt->stream));
+    ok &= Ok(hipMemsetAsync(out_fit, 0, n * sizeof(float), rt->stream));
+    ok &= Ok(hipMemsetAsync(out_evals, 0, n * sizeof(uint32_t), rt->stream));
+    ok &= Ok(hipMemsetAsync(out_has, 0, n, rt->stream));
     ok &= Ok(hipMemsetAsync(rt->raw, 0, rt->max_gen * sizeof(float), rt->stream));
     ok &= Ok(hipMemsetAsync(rt->local_w, 0, rt->max_gen * sizeof(uint32_t), rt->stream));
     ok &= Ok(hipMemsetAsync(rt->counters, 0, 4 * sizeof(unsigned long long), rt->stream));
@@ -396,9 +410,9 @@ static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard
     a.slab = rt->slab;
     a.stride = rt->stride;
     a.slot_count = uint32_t(rt->slot_count);
-    a.action_count = uint32_t(std::min(rt->action_count, wc->action_space_word_count));
+    a.action_count = uint32_t(std::min(rt->action_count, action_space));
     if (a.action_count == 0) return Fail(rt, NE_INVALID_TRAINING_SHARD);
-    a.slot_indices = rt->cur_slots;
+    a.slot_indices = slots;
     a.cell_offsets = rt->d_offsets;
     a.secret_ids = rt->d_secrets;
     a.codes = rt->codes;
@@ -425,8 +439,8 @@ static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard
     uint64_t best_cost = ~0ull;
     for (uint32_t w = 4; w >= 1; --w) {
         uint64_t cost = 0;
-        for (uint64_t c = 0; c < rt->cur_size; ++c) {
-            const uint64_t W = (*offs)[c + 1] - (*offs)[c];
+        for (uint64_t c = 0; c < n; ++c) {
+            const uint64_t W = offs[c + 1] - offs[c];
             const uint64_t t = (W + 15) / 16 + (2 * W + 15) / 16;
             cost += (t + w - 1) / w * w;
         }
@@ -448,30 +462,27 @@ static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard
             rt->bf16_ops_bytes = per_genome * rt->max_gen;
         }
         if (rt->tail_absmax == nullptr && !Alloc(&rt->tail_absmax, rt->max_gen * 40)) return Fail(rt, NE_DEVICE_FAILURE);
-        if (!Ok(hipMemsetAsync(rt->tail_absmax, 0, rt->cur_size * 40 * sizeof(uint32_t), rt->stream)))
+        if (!Ok(hipMemsetAsync(rt->tail_absmax, 0, n * 40 * sizeof(uint32_t), rt->stream)))
             return Fail(rt, NE_DEVICE_FAILURE);
         a.bf16_ops = rt->bf16_ops;
         a.bf16_stride = per_genome;
         a.tail_absmax = reinterpret_cast<const float*>(rt->tail_absmax);
     }
     EventBegin(rt, 8);
-    LaunchTransposeEncoder(rt->slab, rt->stride, rt->cur_slots, uint32_t(rt->cur_size), uint32_t(rt->slot_count),
-                           rt->enc_t, rt->stream);
+    LaunchTransposeEncoder(rt->slab, rt->stride, slots, uint32_t(n), uint32_t(rt->slot_count), rt->enc_t, rt->stream);
     if (bf16)
-        LaunchPackBf16(rt->slab, rt->stride, rt->cur_slots, uint32_t(rt->cur_size), uint32_t(rt->slot_count), rt->img,
+        LaunchPackBf16(rt->slab, rt->stride, slots, uint32_t(n), uint32_t(rt->slot_count), rt->img,
                        a.action_count, rt->bf16_ops, a.bf16_stride, rt->tail_absmax, rt->stream);
     EventEnd(rt, 8);
     EventBegin(rt, 0);
-    LaunchEvaluate(a, uint32_t(rt->cur_size), waves, chunks, trace, bf16, rt->stream);
+    LaunchEvaluate(a, uint32_t(n), waves, chunks, trace, bf16, rt->stream);
     EventEnd(rt, 0);
     if (!Ok(hipGetLastError())) return Fail(rt, NE_DEVICE_FAILURE);
     int st = Finish(rt);
     if (st != NE_OK) return st;
     EventBegin(rt, 1);
-    LaunchFinalize(rt->raw, rt->local_w, uint32_t(rt->cur_size), rt->cur_fit, rt->cur_evals, rt->cur_has, rt->status,
-                   rt->stream);
-    LaunchSummary(rt->cur_fit, rt->cur_has, rt->cur_slots, uint32_t(rt->cur_size), rt->summary, rt->status,
-                  rt->stream);
+    LaunchFinalize(rt->raw, rt->local_w, uint32_t(n), out_fit, out_evals, out_has, rt->status, rt->stream);
+    LaunchSummary(out_fit, out_has, slots, uint32_t(n), rt->summary, rt->status, rt->stream);
     EventEnd(rt, 1);
     st = Finish(rt);
     if (st != NE_OK) return st;
@@ -493,12 +504,40 @@ static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard
     rt->last_summary.average_fitness = sr.average_fitness;
     rt->last_summary.best_index = sr.best_index;
     rt->last_summary.best_slot_index = sr.best_slot;
-    rt->last_summary.generation_index = rt->cur_index;
+    rt->last_summary.generation_index = generation_index;
     rt->last_summary.action_count = rt->action_count;
-    rt->last_summary.population_size = rt->cur_size;
+    rt->last_summary.population_size = n;
     rt->have_summary = true;
     rt->last_status = NE_OK;
     return NE_OK;
+}
+
+
+static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard_release_history* hist) {
+    const uint64_t cols = rt->config.grid_column_count;
+    if (!wc) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (rt->cur_size == 0 || cols == 0 || rt->cur_size % cols != 0) return Fail(rt, NE_INVALID_GENERATION);
+    if (rt->catalog_count == 0) return Fail(rt, NE_INVALID_TRAINING_SHARD);
+    if (wc->training_word_count > rt->catalog_count || wc->action_space_word_count > rt->catalog_count ||
+        wc->training_word_count > wc->action_space_word_count)
+        return Fail(rt, NE_INVALID_TRAINING_SHARD);
+    const uint64_t rows = rt->cur_size / cols;
+    std::vector<uint32_t> built_offsets, built_secrets;
+    const std::vector<uint32_t>* offs = &rt->h_offsets;
+    const std::vector<uint32_t>* secs = &rt->h_secrets;
+    if (!rt->csr_from_caller) {
+        const int st = BuildShardCsr(rt, wc, hist, rows, cols, built_offsets, built_secrets);
+        if (st != NE_OK) return Fail(rt, st);
+        offs = &built_offsets;
+        secs = &built_secrets;
+    } else if (rt->h_offsets.size() < rt->cur_size + 1) {
+        return Fail(rt, NE_INVALID_TRAINING_SHARD);
+    }
+    if (offs->size() < rt->cur_size + 1) return Fail(rt, NE_INVALID_TRAINING_SHARD);
+    const int st = EvaluateCore(rt, wc->action_space_word_count, rt->cur_slots, rt->cur_size, *offs, *secs, rt->cur_fit,
+                                rt->cur_evals, rt->cur_has, rt->cur_index);
+    if (st == NE_OK) rt->cur_evaluated = true;
+    return st;
 }
 
 // TryPlanNextGenerationShape (slab_runtime.cu:919-939).
@@ -750,6 +789,170 @@ static void SwapGenerations(ne_runtime* rt) {
     rt->next_index = 0;
     rt->next_size = 0;
     rt->planned_children = 0;
+    rt->cur_evaluated = false;
+}
+
+// ---- surrogate screening (host side) ----------------------------------------------------------
+constexpr uint32_t kTagProbe = 0x50524F42u;
+constexpr uint32_t kTagAudit = 0x41554454u;
+constexpr uint32_t kTagBatch = 0x42415443u;
+constexpr float kParentFitnessScale = 10.0f;
+
+// Average ranks (ties share the mean of their positions), 1-based.
+static void AverageRanks(const float* v, uint64_t n, std::vector<double>& r) {
+    std::vector<uint64_t> idx(n);
+    for (uint64_t i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) { return v[a] < v[b]; });
+    r.assign(n, 0.0);
+    for (uint64_t i = 0; i < n;) {
+        uint64_t k = i;
+        while (k + 1 < n && v[idx[k + 1]] == v[idx[i]]) ++k;
+        const double rank = 0.5 * double(i + k) + 1.0;
+        for (uint64_t q = i; q <= k; ++q) r[idx[q]] = rank;
+        i = k + 1;
+    }
+}
+
+static double RankCorrelation(const float* a, const float* b, uint64_t n) {
+    if (n < 3) return std::nan("");
+    for (uint64_t i = 0; i < n; ++i)
+        if (!std::isfinite(a[i]) || !std::isfinite(b[i])) return std::nan("");
+    std::vector<double> ra, rb;
+    AverageRanks(a, n, ra);
+    AverageRanks(b, n, rb);
+    double ma = 0.0, mb = 0.0;
+    for (uint64_t i = 0; i < n; ++i) {
+        ma += ra[i];
+        mb += rb[i];
+    }
+    ma /= double(n);
+    mb /= double(n);
+    double sab = 0.0, saa = 0.0, sbb = 0.0;
+    for (uint64_t i = 0; i < n; ++i) {
+        sab += (ra[i] - ma) * (rb[i] - mb);
+        saa += (ra[i] - ma) * (ra[i] - ma);
+        sbb += (rb[i] - mb) * (rb[i] - mb);
+    }
+    if (saa == 0.0 || sbb == 0.0) return std::nan("");
+    return sab / std::sqrt(saa * sbb);
+}
+
+// 16 probe secrets and, per secret, 3 opening guesses (distinct words, none the secret), drawn from
+// the first `words` catalog words with Philox(k, purpose, attempt; seed, tag).
+static bool DrawProbes(ne_runtime* rt, uint64_t words, uint32_t seed) {
+    if (words < kProbeSecrets + kProbeDepths) return false;
+    std::vector<uint32_t> code(rt->catalog_count), canon(rt->catalog_count);
+    std::unordered_map<uint32_t, uint32_t> first;
+    for (uint64_t a = 0; a < rt->catalog_count; ++a) {
+        uint32_t c = 0;
+        for (int q = 0; q < 5; ++q) c |= uint32_t(rt->h_catalog[5 * a + q]) << (5 * q);
+        code[a] = c;
+        canon[a] = first.emplace(c, uint32_t(a)).first->second;
+    }
+    ProbeSet ps{};
+    std::vector<uint32_t> secret_codes;
+    for (int k = 0; k < kProbeSecrets; ++k) {
+        uint32_t idx = 0;
+        for (uint32_t attempt = 0;; ++attempt) {
+            if (attempt > 100000) return false;
+            idx = Philox4x32(uint32_t(k), 0, attempt, 0, seed, kTagProbe).x % uint32_t(words);
+            if (std::find(secret_codes.begin(), secret_codes.end(), code[idx]) == secret_codes.end()) break;
+        }
+        secret_codes.push_back(code[idx]);
+        ps.secret_code[k] = code[idx];
+        rt->h_probe_words[k][0] = idx;
+        for (int t = 0; t < kProbeDepths - 1; ++t) {
+            uint32_t g = 0;
+            for (uint32_t attempt = 0;; ++attempt) {
+                if (attempt > 100000) return false;
+                g = Philox4x32(uint32_t(k), uint32_t(1 + t), attempt, 0, seed, kTagProbe).x % uint32_t(words);
+                bool clash = code[g] == code[idx];
+                for (int q = 0; q < t; ++q) clash = clash || code[g] == ps.turn_code[k][q];
+                if (!clash) break;
+            }
+            ps.turn_code[k][t] = code[g];
+            ps.turn_fb[k][t] = FeedbackPacked(code[g], code[idx]);
+            ps.turn_canon[k][t] = canon[g];
+            rt->h_probe_words[k][1 + t] = g;
+        }
+    }
+    rt->h_probes = ps;
+    return true;
+}
+
+// One Adam step of the surrogate on a device minibatch (x [n][192], t [n]) in the batch buffers' room.
+static void SurrogateStep(ne_runtime* rt, const float* x, const float* t, uint32_t n) {
+    ++rt->sg_step;
+    const float c1 = float(1.0 / (1.0 - std::pow(0.9, double(rt->sg_step))));
+    const float c2 = float(1.0 / (1.0 - std::pow(0.999, double(rt->sg_step))));
+    LaunchSurrogateForward(rt->sg_params, x, n, rt->by, rt->bh1, rt->bh2, rt->stream);
+    LaunchSurrogateBackprop(rt->sg_params, rt->bh1, rt->bh2, rt->by, t, n, rt->bdy, rt->bdh1, rt->bdh2, rt->stream);
+    LaunchSurrogateAdam(rt->sg_params, rt->sg_m, rt->sg_v, x, rt->bh1, rt->bh2, rt->bdy, rt->bdh1, rt->bdh2, n,
+                        rt->screen.learning_rate, c1, c2, rt->stream);
+}
+
+// Mean and standard deviation of the replay buffer's fitness (the training targets' standardisation).
+static void ReplayStats(const ne_runtime* rt, float* mu, float* sigma) {
+    double m = 0.0, q = 0.0;
+    const uint64_t n = rt->replay_count;
+    for (uint64_t i = 0; i < n; ++i) m += rt->h_replay_f[i];
+    m = n ? m / double(n) : 0.0;
+    for (uint64_t i = 0; i < n; ++i) q += (rt->h_replay_f[i] - m) * (rt->h_replay_f[i] - m);
+    const double sd = n > 1 ? std::sqrt(q / double(n - 1)) : 0.0;
+    *mu = float(m);
+    *sigma = sd > 1e-12 ? float(sd) : 1.0f;
+}
+
+// train_steps minibatches drawn uniformly from the replay buffer; returns 1/2 MSE of the last one.
+static int TrainFromReplay(ne_runtime* rt, float* loss) {
+    *loss = std::nanf("");
+    const uint32_t bs = rt->screen.batch_size;
+    if (rt->replay_count < bs) return NE_OK;
+    float mu = 0.0f, sigma = 1.0f;
+    ReplayStats(rt, &mu, &sigma);
+    std::vector<uint32_t> idx(bs);
+    for (uint32_t st = 0; st < rt->screen.train_steps; ++st) {
+        for (uint32_t b = 0; b < bs; ++b)
+            idx[b] = Philox4x32(uint32_t(rt->sg_step + 1), b, uint32_t((rt->sg_step + 1) >> 32), 0, rt->screen.seed,
+                                kTagBatch).x % uint32_t(rt->replay_count);
+        if (!Ok(hipMemcpyAsync(rt->bidx, idx.data(), bs * 4, hipMemcpyHostToDevice, rt->stream)))
+            return Fail(rt, NE_DEVICE_FAILURE);
+        LaunchReplayGather(rt->replay_x, rt->replay_f, rt->bidx, bs, mu, 1.0f / sigma, rt->bx, rt->bt, rt->stream);
+        SurrogateStep(rt, rt->bx, rt->bt, bs);
+    }
+    std::vector<float> y(bs), t(bs);
+    if (!Ok(hipMemcpyAsync(y.data(), rt->by, bs * 4, hipMemcpyDeviceToHost, rt->stream)) ||
+        !Ok(hipMemcpyAsync(t.data(), rt->bt, bs * 4, hipMemcpyDeviceToHost, rt->stream)) ||
+        !Ok(hipStreamSynchronize(rt->stream)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    double l = 0.0;
+    for (uint32_t b = 0; b < bs; ++b) l += 0.5 * double(y[b] - t[b]) * double(y[b] - t[b]);
+    *loss = float(l / double(bs));
+    return NE_OK;
+}
+
+// Fingerprints of n genomes (device slot list) into out (device [n][192]).
+static void LaunchFingerprints(ne_runtime* rt, const uint32_t* slots, uint32_t n, const uint32_t* pairs,
+                               const float* parent_fitness, float* out) {
+    LaunchTransposeEncoder(rt->slab, rt->stride, slots, n, uint32_t(rt->slot_count), rt->enc_t, rt->stream);
+    FingerprintArgs f{};
+    f.slab = rt->slab;
+    f.stride = rt->stride;
+    f.slot_count = uint32_t(rt->slot_count);
+    f.action_count = uint32_t(rt->action_count);
+    f.slots = slots;
+    f.codes = rt->codes;
+    f.img = rt->img;
+    f.img_bytes = uint32_t(rt->img_bytes);
+    f.enc_t = rt->enc_t;
+    f.probes = rt->d_probes;
+    f.pairs = pairs;
+    f.parent_fitness = parent_fitness;
+    f.parent_shift = 0.0f;
+    f.parent_scale = kParentFitnessScale;
+    f.out = out;
+    f.status = rt->status;
+    LaunchFingerprint(f, n, rt->stream);
 }
 
 // ================================================================================================
@@ -849,6 +1052,28 @@ void ne_destroy(ne_runtime* rt) {
     Free(rt->img);
     Free(rt->fixed_absmax);
     Free(rt->bf16_cat);
+    Free(rt->d_probes);
+    Free(rt->sg_params);
+    Free(rt->sg_m);
+    Free(rt->sg_v);
+    Free(rt->fp);
+    Free(rt->pred);
+    Free(rt->e_fit);
+    Free(rt->e_evals);
+    Free(rt->e_has);
+    Free(rt->e_slots);
+    Free(rt->e_rows);
+    Free(rt->replay_x);
+    Free(rt->replay_f);
+    Free(rt->bx);
+    Free(rt->bt);
+    Free(rt->by);
+    Free(rt->bh1);
+    Free(rt->bh2);
+    Free(rt->bdy);
+    Free(rt->bdh1);
+    Free(rt->bdh2);
+    Free(rt->bidx);
     Free(rt->owner);
     Free(rt->move_src);
     Free(rt->d_survivors);
@@ -989,6 +1214,7 @@ int ne_bootstrap(ne_runtime* rt, uint64_t generation_size, uint32_t generation_s
     if (st != NE_OK) return st;
     rt->cur_index = generation_index;
     rt->cur_size = generation_size;
+    rt->cur_evaluated = false;
     rt->next_index = rt->next_size = rt->planned_children = 0;
     rt->have_summary = false;
     return NE_OK;
@@ -1223,6 +1449,340 @@ int ne_download_trace(ne_runtime* rt, uint32_t kind, void* dst, uint64_t capacit
     if (!Ok(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost))) return Fail(rt, NE_DEVICE_FAILURE);
     if (written) *written = bytes;
     return NE_OK;
+}
+
+
+// ---- surrogate screening ----------------------------------------------------------------------
+int ne_enable_screening(ne_runtime* rt, const ne_screen_config* config) {
+    if (!rt || !config) return Fail(rt, NE_INVALID_ARGUMENT);
+    const ne_screen_config& c = *config;
+    if (c.brood_factor == 0 || c.brood_factor > 16 || c.batch_size == 0 || c.batch_size > 1024 ||
+        c.replay_capacity < c.batch_size || !(c.audit_fraction >= 0.0f && c.audit_fraction <= 1.0f) ||
+        !(c.learning_rate > 0.0f) || !(c.spearman_threshold >= -1.0f && c.spearman_threshold <= 1.0f))
+        return Fail(rt, NE_INVALID_ARGUMENT);
+    if (rt->catalog_count == 0) return Fail(rt, NE_INVALID_TRAINING_SHARD);
+    if (!DrawProbes(rt, std::min<uint64_t>(rt->action_count, rt->catalog_count), c.seed))
+        return Fail(rt, NE_INVALID_TRAINING_SHARD);
+    const bool realloc = !rt->screen_on || rt->screen.replay_capacity != c.replay_capacity;
+    const uint64_t bmax = std::max<uint64_t>(c.batch_size, 1024);
+    if (realloc) {
+        Free(rt->replay_x);
+        Free(rt->replay_f);
+        if (!Alloc(&rt->replay_x, uint64_t(c.replay_capacity) * kSgIn) || !Alloc(&rt->replay_f, c.replay_capacity))
+            return Fail(rt, NE_DEVICE_FAILURE);
+    }
+    if (!rt->screen_on) {
+        bool ok = Alloc(&rt->d_probes, 1) && Alloc(&rt->sg_params, kSgParams) && Alloc(&rt->sg_m, kSgParams) &&
+                  Alloc(&rt->sg_v, kSgParams) && Alloc(&rt->fp, rt->max_gen * kSgIn) && Alloc(&rt->pred, rt->max_gen) &&
+                  Alloc(&rt->e_fit, rt->max_gen) && Alloc(&rt->e_evals, rt->max_gen) && Alloc(&rt->e_has, rt->max_gen) &&
+                  Alloc(&rt->e_slots, rt->max_gen) && Alloc(&rt->e_rows, rt->max_gen) && Alloc(&rt->bx, bmax * kSgIn) &&
+                  Alloc(&rt->bt, bmax) && Alloc(&rt->by, bmax) && Alloc(&rt->bh1, bmax * kSgH1) &&
+                  Alloc(&rt->bh2, bmax * kSgH2) && Alloc(&rt->bdy, bmax) && Alloc(&rt->bdh1, bmax * kSgH1) &&
+                  Alloc(&rt->bdh2, bmax * kSgH2) && Alloc(&rt->bidx, bmax);
+        if (!ok) return Fail(rt, NE_DEVICE_FAILURE);
+    }
+    if (rt->enc_t == nullptr && !Alloc(&rt->enc_t, rt->max_gen * kEncTStride)) return Fail(rt, NE_DEVICE_FAILURE);
+    rt->screen = c;
+    rt->screen_on = true;
+    rt->screen_active = false;
+    rt->sg_step = 0;
+    rt->replay_count = 0;
+    rt->replay_head = 0;
+    rt->h_replay_f.assign(c.replay_capacity, 0.0f);
+    LaunchSurrogateInit(rt->sg_params, rt->sg_m, rt->sg_v, c.seed, rt->stream);
+    if (!Ok(hipMemcpyAsync(rt->d_probes, &rt->h_probes, sizeof(ProbeSet), hipMemcpyHostToDevice, rt->stream)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    return Finish(rt);
+}
+
+int ne_advance_generation_screened(ne_runtime* rt, uint32_t generation_seed, const ne_word_counts* counts,
+                                   const ne_assembly_config* config, const ne_shard_release_history* history,
+                                   ne_screen_report* report) {
+    if (!rt || !counts) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (!rt->screen_on) return Fail(rt, NE_UNSUPPORTED);
+    const ne_assembly_config cfg = config ? *config : DefaultAssembly();
+    if (!ValidAssembly(cfg)) return Fail(rt, NE_INVALID_ASSEMBLY_CONFIG);
+    ne_screen_report rep{};
+    rt->times = ne_stage_times{};
+    int st = NE_OK;
+    if (!rt->cur_evaluated) {
+        st = EvaluateFull(rt, counts, history);
+        if (st != NE_OK) return st;
+    }
+    const uint64_t N = rt->cur_size, cols = rt->config.grid_column_count, rows = N / cols;
+    const bool screened = rt->screen_active;
+    const uint64_t B = screened ? rt->screen.brood_factor : 1;
+    const uint64_t BN = B * N;
+    if (BN > rt->max_gen) return Fail(rt, NE_INVALID_ASSEMBLY_CONFIG);
+    // the cells' secrets, for the children (generation cur_index + 1)
+    std::vector<uint32_t> built_o, built_s;
+    const std::vector<uint32_t>* offs = &rt->h_offsets;
+    const std::vector<uint32_t>* secs = &rt->h_secrets;
+    if (!rt->csr_from_caller) {
+        st = BuildShardCsrCore(counts, history, rt->cur_index + 1, rows, cols, rt->max_gen / cols, rt->catalog_count,
+                               built_o, built_s);
+        if (st != NE_OK) return Fail(rt, st);
+        offs = &built_o;
+        secs = &built_s;
+    }
+    if (offs->size() < N + 1) return Fail(rt, NE_INVALID_TRAINING_SHARD);
+
+    // plan: child b*N + c belongs to cell c (its draws are keyed by b*N + c, so b = 0 is the plain plan)
+    st = EnsureRankWeights(rt, cfg.rank_exponent, std::max<uint64_t>(N, 25));
+    if (st != NE_OK) return st;
+    PlanArgs pa{};
+    pa.fitness = rt->cur_fit;
+    pa.has_fitness = rt->cur_has;
+    pa.cur_size = uint32_t(N);
+    pa.child_count = uint32_t(BN);
+    pa.cols = uint32_t(cols);
+    pa.gen_index = uint32_t(rt->cur_index);
+    pa.radius = uint32_t(std::min<uint64_t>(cfg.cellular_breeding_radius, 0xFFFFFFFFu));
+    pa.generation_seed = generation_seed;
+    pa.rank_weights = rt->rank_weights;
+    pa.pairs = rt->pairs;
+    pa.status = rt->status;
+    EventBegin(rt, 2);
+    LaunchPlan(pa, rt->stream);
+    EventEnd(rt, 2);
+    st = Finish(rt);
+    if (st != NE_OK) return st;
+    const float plan_ms = EventMs(rt, 2);
+    rt->planned_children = BN;
+    st = Assemble(rt, generation_seed, cfg, rt->action_count);
+    if (st != NE_OK) return st;
+    const float assemble_ms = rt->times.assemble_ms, book_ms = rt->times.bookkeeping_ms;
+
+    // fingerprints of the whole brood, then the surrogate's predictions (parent fitness per position
+    // stays in cur_fit after the parents' slots are released)
+    EventBegin(rt, 9);
+    LaunchFingerprints(rt, rt->nxt_slots, uint32_t(BN), rt->pairs, rt->cur_fit, rt->fp);
+    EventEnd(rt, 9);
+    EventBegin(rt, 10);
+    LaunchSurrogateForward(rt->sg_params, rt->fp, uint32_t(BN), rt->pred, nullptr, nullptr, rt->stream);
+    EventEnd(rt, 10);
+    st = Finish(rt);
+    if (st != NE_OK) return st;
+    rep.fingerprint_ms = EventMs(rt, 9);
+    float sg_ms = EventMs(rt, 10);
+    std::vector<float> pred(BN);
+    std::vector<uint32_t> brood_slots(BN);
+    if (!Ok(hipMemcpy(pred.data(), rt->pred, BN * 4, hipMemcpyDeviceToHost)) ||
+        !Ok(hipMemcpy(brood_slots.data(), rt->nxt_slots, BN * 4, hipMemcpyDeviceToHost)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+
+    // selection: each cell's best-predicted child (lowest brood index on ties); audit: a Philox draw
+    // per brood member (keyed by member and parent generation)
+    std::vector<uint32_t> rows_e;
+    std::vector<int64_t> entry_of(BN, -1);
+    rows_e.reserve(BN);
+    for (uint64_t c = 0; c < N; ++c) {
+        uint64_t best = c;
+        for (uint64_t b = 1; b < B; ++b)
+            if (pred[b * N + c] > pred[best]) best = b * N + c;
+        entry_of[best] = int64_t(rows_e.size());
+        rows_e.push_back(uint32_t(best));
+    }
+    std::vector<uint32_t> audit;
+    for (uint64_t i = 0; i < BN; ++i) {
+        const U4 w = Philox4x32(uint32_t(i), uint32_t(rt->cur_index), uint32_t(rt->cur_index >> 32), 0,
+                                rt->screen.seed, kTagAudit);
+        if (!(UniformOpenClosed(w.x) <= rt->screen.audit_fraction)) continue;
+        audit.push_back(uint32_t(i));
+        if (entry_of[i] < 0) {
+            entry_of[i] = int64_t(rows_e.size());
+            rows_e.push_back(uint32_t(i));
+        }
+    }
+    const uint64_t E = rows_e.size();
+    std::vector<uint32_t> e_slots(E), eo(E + 1, 0), es;
+    for (uint64_t e = 0; e < E; ++e) {
+        e_slots[e] = brood_slots[rows_e[e]];
+        const uint64_t cell = rows_e[e] % N;
+        es.insert(es.end(), secs->begin() + (*offs)[cell], secs->begin() + (*offs)[cell + 1]);
+        eo[e + 1] = uint32_t(es.size());
+    }
+    if (!Ok(hipMemcpyAsync(rt->e_slots, e_slots.data(), E * 4, hipMemcpyHostToDevice, rt->stream)) ||
+        !Ok(hipMemcpyAsync(rt->e_rows, rows_e.data(), E * 4, hipMemcpyHostToDevice, rt->stream)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    st = EvaluateCore(rt, counts->action_space_word_count, rt->e_slots, E, eo, es, rt->e_fit, rt->e_evals, rt->e_has,
+                      rt->cur_index + 1);
+    if (st != NE_OK) return st;
+    std::vector<float> e_fit(E);
+    if (!Ok(hipMemcpy(e_fit.data(), rt->e_fit, E * 4, hipMemcpyDeviceToHost))) return Fail(rt, NE_DEVICE_FAILURE);
+
+    // the audit's rank correlation, from predictions made before any training on these children
+    std::vector<float> ap, af;
+    for (const uint32_t r : audit) {
+        ap.push_back(pred[r]);
+        af.push_back(e_fit[size_t(entry_of[r])]);
+    }
+    const double rho = RankCorrelation(ap.data(), af.data(), ap.size());
+
+    // every evaluated child joins the replay buffer (the last `capacity` when more), then training
+    const uint64_t cap = rt->screen.replay_capacity;
+    const uint64_t first = E > cap ? E - cap : 0;
+    LaunchReplayAppend(rt->fp, rt->e_rows + first, rt->e_fit + first, uint32_t(E - first), rt->replay_x, rt->replay_f,
+                       uint32_t(rt->replay_head), uint32_t(cap), rt->stream);
+    for (uint64_t e = first; e < E; ++e) rt->h_replay_f[(rt->replay_head + e - first) % cap] = e_fit[e];
+    rt->replay_head = (rt->replay_head + (E - first)) % cap;
+    rt->replay_count = std::min<uint64_t>(rt->replay_count + (E - first), cap);
+    EventBegin(rt, 10);
+    float loss = std::nanf("");
+    st = TrainFromReplay(rt, &loss);
+    if (st != NE_OK) return st;
+    EventEnd(rt, 10);
+
+    // the new generation: the selected children; every other brood member goes back to the free stack
+    std::vector<uint32_t> keep(BN, 0), sel_slots(N);
+    for (uint64_t c = 0; c < N; ++c) {
+        keep[rows_e[c]] = 1;
+        sel_slots[c] = e_slots[c];
+    }
+    bool ok = Ok(hipMemcpyAsync(rt->refcounts, keep.data(), BN * 4, hipMemcpyHostToDevice, rt->stream));
+    LaunchCollectReleased(rt->nxt_slots, uint32_t(BN), rt->refcounts, rt->slot_state, rt->free_stack, rt->free_count,
+                          uint32_t(rt->slot_count), rt->status, rt->stream);
+    LaunchFillU32(rt->nxt_slots, kInvalidSlot, BN, rt->stream);
+    ok &= Ok(hipMemcpyAsync(rt->nxt_slots, sel_slots.data(), N * 4, hipMemcpyHostToDevice, rt->stream));
+    ok &= Ok(hipMemcpyAsync(rt->nxt_fit, rt->e_fit, N * sizeof(float), hipMemcpyDeviceToDevice, rt->stream));
+    ok &= Ok(hipMemcpyAsync(rt->nxt_evals, rt->e_evals, N * sizeof(uint32_t), hipMemcpyDeviceToDevice, rt->stream));
+    ok &= Ok(hipMemcpyAsync(rt->nxt_has, rt->e_has, N, hipMemcpyDeviceToDevice, rt->stream));
+    if (!ok) return Fail(rt, NE_DEVICE_FAILURE);
+    st = Finish(rt);
+    if (st != NE_OK) return st;
+    sg_ms += EventMs(rt, 10);
+    rt->next_size = N;
+    const ne_stage_times eval_times = rt->times;
+    SwapGenerations(rt);
+    rt->cur_evaluated = true;
+    LaunchSummary(rt->cur_fit, rt->cur_has, rt->cur_slots, uint32_t(N), rt->summary, rt->status, rt->stream);
+    st = Finish(rt);
+    if (st != NE_OK) return st;
+    SummaryRecord sr{};
+    if (!Ok(hipMemcpy(&sr, rt->summary, sizeof(sr), hipMemcpyDeviceToHost))) return Fail(rt, NE_DEVICE_FAILURE);
+    rt->last_summary.best_fitness = sr.best_fitness;
+    rt->last_summary.average_fitness = sr.average_fitness;
+    rt->last_summary.best_index = sr.best_index;
+    rt->last_summary.best_slot_index = sr.best_slot;
+    rt->last_summary.generation_index = rt->cur_index;
+    rt->last_summary.action_count = rt->action_count;
+    rt->last_summary.population_size = N;
+    rt->have_summary = true;
+
+    rt->times = eval_times;
+    rt->times.plan_ms = plan_ms;
+    rt->times.assemble_ms = assemble_ms;
+    rt->times.bookkeeping_ms = book_ms;
+    rt->times.children = BN;
+    rt->screen_active = rt->replay_count >= rt->screen.min_replay && std::isfinite(rho) &&
+                        rho >= double(rt->screen.spearman_threshold);
+    rep.screened = screened ? 1u : 0u;
+    rep.next_screened = rt->screen_active ? 1u : 0u;
+    rep.brood = uint32_t(BN);
+    rep.evaluated = uint32_t(E);
+    rep.audited = uint32_t(audit.size());
+    rep.audit_spearman = float(rho);
+    rep.train_loss = loss;
+    rep.surrogate_ms = sg_ms;
+    rep.best_fitness = sr.best_fitness;
+    rep.average_fitness = sr.average_fitness;
+    rep.replay_count = rt->replay_count;
+    if (report) *report = rep;
+    rt->last_status = NE_OK;
+    return NE_OK;
+}
+
+int ne_fingerprint(ne_runtime* rt, const uint32_t* slot_indices, const float* parent_fitness, uint64_t count,
+                   float* out) {
+    if (!rt || !slot_indices || !parent_fitness || !out || count == 0) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (!rt->screen_on) return Fail(rt, NE_UNSUPPORTED);
+    if (count > rt->max_gen) return Fail(rt, NE_INVALID_ARGUMENT);
+    for (uint64_t i = 0; i < count; ++i)
+        if (slot_indices[i] >= rt->slot_count) return Fail(rt, NE_INVALID_ARGUMENT);
+    uint32_t* d_pairs = nullptr;
+    float* d_pf = nullptr;
+    std::vector<uint32_t> pairs(2 * count);
+    for (uint64_t i = 0; i < 2 * count; ++i) pairs[i] = uint32_t(i);
+    if (!Alloc(&d_pairs, 2 * count) || !Alloc(&d_pf, 2 * count)) {
+        Free(d_pairs);
+        return Fail(rt, NE_DEVICE_FAILURE);
+    }
+    bool ok = Ok(hipMemcpyAsync(rt->e_slots, slot_indices, count * 4, hipMemcpyHostToDevice, rt->stream));
+    ok &= Ok(hipMemcpyAsync(d_pairs, pairs.data(), 2 * count * 4, hipMemcpyHostToDevice, rt->stream));
+    ok &= Ok(hipMemcpyAsync(d_pf, parent_fitness, 2 * count * 4, hipMemcpyHostToDevice, rt->stream));
+    if (ok) LaunchFingerprints(rt, rt->e_slots, uint32_t(count), d_pairs, d_pf, rt->fp);
+    int st = ok ? Finish(rt) : Fail(rt, NE_DEVICE_FAILURE);
+    if (st == NE_OK && !Ok(hipMemcpy(out, rt->fp, count * kSgIn * 4, hipMemcpyDeviceToHost))) st = Fail(rt, NE_DEVICE_FAILURE);
+    Free(d_pairs);
+    Free(d_pf);
+    return st;
+}
+
+int ne_get_probe_states(ne_runtime* rt, uint32_t* dst, uint64_t capacity, uint64_t* count) {
+    if (!rt || !dst) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (!rt->screen_on) return Fail(rt, NE_UNSUPPORTED);
+    constexpr uint64_t n = uint64_t(kProbeSecrets) * kProbeDepths;
+    if (count) *count = n;
+    if (capacity < n) return Fail(rt, NE_INVALID_ARGUMENT);
+    std::memcpy(dst, rt->h_probe_words, n * sizeof(uint32_t));
+    return NE_OK;
+}
+
+int ne_surrogate_predict(ne_runtime* rt, const float* x, uint64_t count, float* y) {
+    if (!rt || !x || !y || count == 0 || count > 0xFFFFFFFFull / kSgIn) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (!rt->screen_on) return Fail(rt, NE_UNSUPPORTED);
+    float *dx = nullptr, *dy = nullptr;
+    if (!Alloc(&dx, count * kSgIn) || !Alloc(&dy, count)) {
+        Free(dx);
+        return Fail(rt, NE_DEVICE_FAILURE);
+    }
+    int st = Ok(hipMemcpyAsync(dx, x, count * kSgIn * 4, hipMemcpyHostToDevice, rt->stream)) ? NE_OK
+                                                                                             : Fail(rt, NE_DEVICE_FAILURE);
+    if (st == NE_OK) {
+        LaunchSurrogateForward(rt->sg_params, dx, uint32_t(count), dy, nullptr, nullptr, rt->stream);
+        st = Finish(rt);
+    }
+    if (st == NE_OK && !Ok(hipMemcpy(y, dy, count * 4, hipMemcpyDeviceToHost))) st = Fail(rt, NE_DEVICE_FAILURE);
+    Free(dx);
+    Free(dy);
+    return st;
+}
+
+int ne_surrogate_train_step(ne_runtime* rt, const float* x, const float* t, uint64_t count) {
+    if (!rt || !x || !t || count == 0) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (!rt->screen_on) return Fail(rt, NE_UNSUPPORTED);
+    if (count > std::max<uint64_t>(rt->screen.batch_size, 1024)) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (!Ok(hipMemcpyAsync(rt->bx, x, count * kSgIn * 4, hipMemcpyHostToDevice, rt->stream)) ||
+        !Ok(hipMemcpyAsync(rt->bt, t, count * 4, hipMemcpyHostToDevice, rt->stream)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    SurrogateStep(rt, rt->bx, rt->bt, uint32_t(count));
+    return Finish(rt);
+}
+
+int ne_surrogate_get_params(ne_runtime* rt, float* dst, uint64_t capacity, uint64_t* count) {
+    if (!rt || !dst) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (!rt->screen_on) return Fail(rt, NE_UNSUPPORTED);
+    if (count) *count = kSgParams;
+    if (capacity < kSgParams) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (Sync(rt) != NE_OK || !Ok(hipMemcpy(dst, rt->sg_params, kSgParams * 4, hipMemcpyDeviceToHost)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    return NE_OK;
+}
+
+int ne_surrogate_set_params(ne_runtime* rt, const float* src, uint64_t count) {
+    if (!rt || !src || count != kSgParams) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (!rt->screen_on) return Fail(rt, NE_UNSUPPORTED);
+    bool ok = Ok(hipMemcpyAsync(rt->sg_params, src, kSgParams * 4, hipMemcpyHostToDevice, rt->stream));
+    ok &= Ok(hipMemsetAsync(rt->sg_m, 0, kSgParams * 4, rt->stream));
+    ok &= Ok(hipMemsetAsync(rt->sg_v, 0, kSgParams * 4, rt->stream));
+    if (!ok) return Fail(rt, NE_DEVICE_FAILURE);
+    rt->sg_step = 0;
+    return Finish(rt);
+}
+
+double ne_rank_correlation(const float* a, const float* b, uint64_t n) {
+    if (!a || !b) return std::nan("");
+    return RankCorrelation(a, b, n);
 }
 
 }  // extern "C"

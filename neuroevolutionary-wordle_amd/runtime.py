@@ -99,6 +99,35 @@ POST_FITNESS_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(WordCounts), C.c_vo
 SHARD_RELEASE_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(WordCounts), C.POINTER(PendingInjection), C.c_void_p)
 
 
+class ScreenConfig(C.Structure):
+    _fields_ = [("brood_factor", C.c_uint32), ("replay_capacity", C.c_uint32), ("audit_fraction", C.c_float),
+                ("spearman_threshold", C.c_float), ("learning_rate", C.c_float), ("train_steps", C.c_uint32),
+                ("batch_size", C.c_uint32), ("min_replay", C.c_uint32), ("seed", C.c_uint32),
+                ("reserved0", C.c_uint32)]
+
+
+class ScreenReport(C.Structure):
+    _fields_ = [("screened", C.c_uint32), ("next_screened", C.c_uint32), ("brood", C.c_uint32),
+                ("evaluated", C.c_uint32), ("audited", C.c_uint32), ("audit_spearman", C.c_float),
+                ("train_loss", C.c_float), ("fingerprint_ms", C.c_float), ("surrogate_ms", C.c_float),
+                ("best_fitness", C.c_float), ("average_fitness", C.c_float), ("reserved0", C.c_uint32),
+                ("replay_count", C.c_uint64)]
+
+
+SURROGATE_PARAMS = 33025
+FINGERPRINT_DIMS = 192
+
+
+def default_screen_config(**kw):
+    """The surrogate-screened configuration: brood 4x, top 25% (each cell's best-predicted child) plus a
+    5% random audit, Adam lr 1e-3 on a replay buffer of 65,536 pairs, fallback below Spearman 0.5."""
+    c = ScreenConfig(brood_factor=4, replay_capacity=65536, audit_fraction=0.05, spearman_threshold=0.5,
+                     learning_rate=1e-3, train_steps=32, batch_size=256, min_replay=4096, seed=0x5C2EE7, reserved0=0)
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
 class Callbacks(C.Structure):
     _fields_ = [("post_fitness_evaluation", POST_FITNESS_CB), ("training_shard_release", SHARD_RELEASE_CB),
                 ("user", C.c_void_p)]
@@ -109,7 +138,9 @@ EXPORTED_SYMBOLS = [
     "ne_evaluate", "ne_read_summary", "ne_advance_generation", "ne_get_slab_info", "ne_download_slot",
     "ne_upload_slot", "ne_download_generation", "ne_download_plan", "ne_download_local_word_counts",
     "ne_read_status", "ne_status_string", "ne_get_stage_times", "ne_set_trace", "ne_download_trace",
-    "ne_build_shard_csr", "ne_set_eval_precision",
+    "ne_build_shard_csr", "ne_set_eval_precision", "ne_enable_screening", "ne_advance_generation_screened",
+    "ne_fingerprint", "ne_get_probe_states", "ne_surrogate_predict", "ne_surrogate_train_step",
+    "ne_surrogate_get_params", "ne_surrogate_set_params", "ne_rank_correlation",
 ]
 
 _lib = None
@@ -150,6 +181,16 @@ def load_library(path=LIB_PATH):
         "ne_download_trace": (i32, [p, u32, p, u64, C.POINTER(u64)]),
         "ne_build_shard_csr": (i32, [C.POINTER(WordCounts), C.POINTER(ShardReleaseHistory), u64, u64, u64, u64,
                                      p, p, u64, C.POINTER(u64)]),
+        "ne_enable_screening": (i32, [p, C.POINTER(ScreenConfig)]),
+        "ne_advance_generation_screened": (i32, [p, u32, C.POINTER(WordCounts), C.POINTER(AssemblyConfig),
+                                                 C.POINTER(ShardReleaseHistory), C.POINTER(ScreenReport)]),
+        "ne_fingerprint": (i32, [p, p, p, u64, p]),
+        "ne_get_probe_states": (i32, [p, p, u64, C.POINTER(u64)]),
+        "ne_surrogate_predict": (i32, [p, p, u64, p]),
+        "ne_surrogate_train_step": (i32, [p, p, p, u64]),
+        "ne_surrogate_get_params": (i32, [p, p, u64, C.POINTER(u64)]),
+        "ne_surrogate_set_params": (i32, [p, p, u64]),
+        "ne_rank_correlation": (C.c_double, [p, p, u64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -339,6 +380,67 @@ class SlabRuntime:
 
     def status(self):
         return self.lib.ne_read_status(self.h)
+
+    # --- surrogate screening
+    def enable_screening(self, config=None):
+        cfg = config if config is not None else default_screen_config()
+        self._check("ne_enable_screening", self.lib.ne_enable_screening(self.h, C.byref(cfg)))
+
+    def advance_screened(self, seed, counts, config=None, history=None):
+        cfg = C.byref(config) if config is not None else None
+        hist = None
+        if history is not None:
+            arr = np.ascontiguousarray(history, dtype=np.uint64)
+            self._hist_keep = arr
+            hist = C.byref(ShardReleaseHistory(arr.ctypes.data_as(C.POINTER(C.c_uint64)), arr.size))
+        rep = ScreenReport()
+        self._check("ne_advance_generation_screened",
+                    self.lib.ne_advance_generation_screened(self.h, seed, C.byref(counts), cfg, hist, C.byref(rep)))
+        return rep
+
+    def fingerprint(self, slots, parent_fitness):
+        slots = np.ascontiguousarray(slots, dtype=np.uint32)
+        pf = np.ascontiguousarray(parent_fitness, dtype=np.float32).reshape(-1, 2)
+        out = np.zeros((slots.size, FINGERPRINT_DIMS), np.float32)
+        self._check("ne_fingerprint", self.lib.ne_fingerprint(self.h, _ptr(slots), _ptr(pf), slots.size, _ptr(out)))
+        return out
+
+    def probe_states(self):
+        """[16][4]: per probe board, the secret's catalog index then its 3 turns' guess indices."""
+        out = np.zeros((16, 4), np.uint32)
+        n = C.c_uint64()
+        self._check("ne_get_probe_states", self.lib.ne_get_probe_states(self.h, _ptr(out), out.size, C.byref(n)))
+        return out
+
+    def surrogate_predict(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, FINGERPRINT_DIMS)
+        y = np.zeros(x.shape[0], np.float32)
+        self._check("ne_surrogate_predict", self.lib.ne_surrogate_predict(self.h, _ptr(x), x.shape[0], _ptr(y)))
+        return y
+
+    def surrogate_train_step(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, FINGERPRINT_DIMS)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        self._check("ne_surrogate_train_step",
+                    self.lib.ne_surrogate_train_step(self.h, _ptr(x), _ptr(t), x.shape[0]))
+
+    def surrogate_params(self):
+        out = np.zeros(SURROGATE_PARAMS, np.float32)
+        n = C.c_uint64()
+        self._check("ne_surrogate_get_params",
+                    self.lib.ne_surrogate_get_params(self.h, _ptr(out), out.size, C.byref(n)))
+        return out
+
+    def set_surrogate_params(self, params):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        self._check("ne_surrogate_set_params", self.lib.ne_surrogate_set_params(self.h, _ptr(params), params.size))
+
+
+def rank_correlation(a, b):
+    """Spearman rank correlation through the library's host routine (average ranks for ties)."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    return float(load_library().ne_rank_correlation(_ptr(a), _ptr(b), a.size))
 
 
 def build_shard_csr(counts, generation_index, rows, cols, epicenter_rows=None, history=None):
