@@ -1582,6 +1582,42 @@ __global__ void XoverDecisionKernel(XoverArgs p) {
         r.splice_point = 1u + d9.x % uint32_t(kTail - 1);
     }
     p.records[c] = r;
+    // the donor-change positions, ascending (a breeding wave whose 256-value window holds none of them
+    // takes one donor for the whole window)
+    uint32_t* out = p.breaks + uint64_t(c) * kBreakStride;
+    static_assert(7 * kDenseLayers + 6 <= int(kBreakStride) - 1, "break list");
+    uint32_t nb = 0;
+    uint32_t tmp[kBreakStride - 1];
+    auto push = [&](uint32_t b) {
+        uint32_t k = nb++;
+        for (; k > 0 && tmp[k - 1] > b; --k) tmp[k] = tmp[k - 1];
+        tmp[k] = b;
+    };
+    for (int l = 0; l < kDenseLayers; ++l) {
+        const DenseLayerDesc d = DenseLayerAt(l);
+        push(d.w);
+        push(d.b);
+        push(d.b + d.out);
+        if (r.neuron[l] != kInvalidSlot) {
+            push(d.w + r.neuron[l] * d.in);
+            push(d.w + (r.neuron[l] + 1) * d.in);
+            push(d.b + r.neuron[l]);
+            push(d.b + r.neuron[l] + 1);
+        }
+    }
+    const uint32_t P = uint32_t(kPolicyParams);
+    if (r.row2 != kInvalidSlot) {
+        push(P + r.row2 * kTail);
+        push(P + (r.row2 + 1) * kTail);
+    }
+    if (r.splice_row != kInvalidSlot) {
+        push(P + r.splice_row * kTail);
+        push(P + r.splice_row * kTail + r.splice_point);
+        push(P + (r.splice_row + 1) * kTail);
+    }
+    push(P + p.parent_action_count * kTail);
+    out[0] = nb;
+    for (uint32_t i = 0; i < nb; ++i) out[1 + i] = tmp[i];
 }
 
 __device__ inline bool ElementDonorFirst(const XoverRecord& r, uint32_t e) {
@@ -1612,6 +1648,13 @@ __device__ inline bool ElementDonorFirst(const XoverRecord& r, uint32_t e) {
     return src;
 }
 
+// The mutated value of element e (taken for about mutation_p of the elements): a normal draw of its own.
+__device__ __forceinline__ float MutatedValue(float v, uint32_t e, uint32_t child, uint32_t gen,
+                                                          uint32_t seed, float sigma) {
+    const U4 z = Philox4x32(e, child, gen, 0, seed, kTagNorm);
+    return __builtin_fmaf(sigma, NormalFromWords(z.x, z.y), v);
+}
+
 // Crossover + mutation, element-parallel: each thread builds 4 consecutive values of one child
 // (BreedAndMutateGenome, genome_ops.cuh:386-407, on counter-based draws).  Writes the full slot.
 __global__ __launch_bounds__(256) void BreedKernel(BreedArgs p) {
@@ -1622,11 +1665,8 @@ __global__ __launch_bounds__(256) void BreedKernel(BreedArgs p) {
     __shared__ const float* s_p2;
     __shared__ float* s_child;
     __shared__ int s_bad;
-    // Positions where the donor can change (layer, neuron-row, bias, tail-row and splice bounds):
-    // a wave whose 256-value window holds none of them takes one donor for the whole window.
-    constexpr int kMaxBreaks = 48;
-    __shared__ uint32_t s_breaks[kMaxBreaks];
-    __shared__ uint32_t s_nbreaks;
+    // the child's donor-change positions (count, then ascending), from the crossover-decision kernel
+    __shared__ uint32_t s_brk[kBreakStride];
     if (threadIdx.x == 0) {
         s_rec = p.records[child];
         const uint32_t pa = p.pairs[2 * child], pb = p.pairs[2 * child + 1];
@@ -1643,80 +1683,51 @@ __global__ __launch_bounds__(256) void BreedKernel(BreedArgs p) {
         }
     }
     const uint32_t n_parent = uint32_t(kPolicyParams) + p.parent_action_count * kTail;
-    if (threadIdx.x == 0) {
-        const XoverRecord& r = s_rec;
-        uint32_t nb = 0;
-        auto add = [&](uint32_t b) { s_breaks[nb++] = b; };
-        for (int l = 0; l < kDenseLayers; ++l) {
-            const DenseLayerDesc d = DenseLayerAt(l);
-            add(d.w);
-            add(d.b);
-            add(d.b + d.out);
-            if (r.neuron[l] != kInvalidSlot) {
-                add(d.w + r.neuron[l] * d.in);
-                add(d.w + (r.neuron[l] + 1) * d.in);
-                add(d.b + r.neuron[l]);
-                add(d.b + r.neuron[l] + 1);
-            }
-        }
-        const uint32_t P = uint32_t(kPolicyParams);
-        if (r.row2 != kInvalidSlot) {
-            add(P + r.row2 * kTail);
-            add(P + (r.row2 + 1) * kTail);
-        }
-        if (r.splice_row != kInvalidSlot) {
-            add(P + r.splice_row * kTail);
-            add(P + r.splice_row * kTail + r.splice_point);
-            add(P + (r.splice_row + 1) * kTail);
-        }
-        add(n_parent);
-        s_nbreaks = nb;
-    }
+    if (threadIdx.x < kBreakStride) s_brk[threadIdx.x] = p.breaks[uint64_t(child) * kBreakStride + threadIdx.x];
     __syncthreads();
     if (s_bad) return;
     const XoverRecord r = s_rec;
     const float* p1 = s_p1;
     const float* p2 = s_p2;
     float* out = s_child;
-    const uint32_t nbreaks = s_nbreaks;
+    const uint32_t nbreaks = s_brk[0];
+    const uint32_t* s_breaks = s_brk + 1;
     const uint32_t n_slot = uint32_t(p.child_stride / 4);
     const bool mutate = p.mutation_p > 0.0f && p.mutation_sigma > 0.0f;
     const uint32_t quads = n_slot / 4;
     const uint32_t lane = threadIdx.x & 63u;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += gridDim.x * blockDim.x) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t kb = 0;  // first break above the previous window's start
+    // crossover: the donor values of quad q (called in increasing q, as the cursor requires)
+    auto load_quad = [&](uint32_t q) __device__ -> v4f {
         const uint32_t e0 = 4 * q;
-        // the wave's window [w0, w0 + 256): uniform donor unless a break lies strictly inside
-        const uint32_t w0 = __builtin_amdgcn_readfirstlane(e0 - 4 * lane);
-        bool uniform = w0 + 256 <= n_parent;
-        for (uint32_t k = 0; k < nbreaks && uniform; ++k) {
-            const uint32_t b = s_breaks[k];
-            uniform = !(b > w0 && b < w0 + 256);
-        }
         v4f v = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (e0 < n_parent) {
-            if (uniform) {
-                v = Load4((ElementDonorFirst(r, w0) ? p1 : p2) + e0);
-            } else {
-                const bool d0 = ElementDonorFirst(r, e0), d1 = ElementDonorFirst(r, e0 + 1);
-                const bool d2 = ElementDonorFirst(r, e0 + 2), d3 = ElementDonorFirst(r, e0 + 3);
-                if (d0 == d1 && d1 == d2 && d2 == d3 && e0 + 4 <= n_parent) {
-                    v = Load4((d0 ? p1 : p2) + e0);
-                } else {
-                    const bool dd[4] = {d0, d1, d2, d3};
+        if (e0 >= n_parent) return v;
+        // the wave's window [w0, w0 + 256): uniform donor unless a break lies strictly inside, i.e. unless
+        // the first break above w0 is below w0 + 256
+        const uint32_t w0 = __builtin_amdgcn_readfirstlane(e0 - 4 * lane);
+        while (kb < nbreaks && s_breaks[kb] <= w0) ++kb;
+        const bool uniform = w0 + 256 <= n_parent && (kb == nbreaks || s_breaks[kb] >= w0 + 256);
+        if (uniform) return Load4((ElementDonorFirst(r, w0) ? p1 : p2) + e0);
+        const bool d0 = ElementDonorFirst(r, e0), d1 = ElementDonorFirst(r, e0 + 1);
+        const bool d2 = ElementDonorFirst(r, e0 + 2), d3 = ElementDonorFirst(r, e0 + 3);
+        if (d0 == d1 && d1 == d2 && d2 == d3 && e0 + 4 <= n_parent) return Load4((d0 ? p1 : p2) + e0);
+        const bool dd[4] = {d0, d1, d2, d3};
 #pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        if (e0 + i < n_parent) v[i] = (dd[i] ? p1 : p2)[e0 + i];
-                }
-            }
+        for (int i = 0; i < 4; ++i)
+            if (e0 + i < n_parent) v[i] = (dd[i] ? p1 : p2)[e0 + i];
+        return v;
+    };
+    // mutation and tail-row scaling of quad q, then the store
+    auto finish_quad = [&](uint32_t q, v4f v) __device__ {
+        const uint32_t e0 = 4 * q;
+        if (e0 < n_parent) {
             if (mutate) {
                 const U4 w = Philox4x32(q, child, p.gen_index, 0, p.seed, kTagMuta);
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    if (e0 + i < n_parent && BernoulliDraw(p.mutation_p, Lane(w, i))) {
-                        const U4 z = Philox4x32(e0 + i, child, p.gen_index, 0, p.seed, kTagNorm);
-                        v[i] = __builtin_fmaf(p.mutation_sigma, NormalFromWords(z.x, z.y), v[i]);
-                    }
-                }
+                for (int i = 0; i < 4; ++i)
+                    if (e0 + i < n_parent && BernoulliDraw(p.mutation_p, Lane(w, i)))
+                        v[i] = MutatedValue(v[i], e0 + i, child, p.gen_index, p.seed, p.mutation_sigma);
             }
             if (p.row_scale_p > 0.0f) {
 #pragma unroll
@@ -1725,12 +1736,20 @@ __global__ __launch_bounds__(256) void BreedKernel(BreedArgs p) {
                     if (e < uint32_t(kPolicyParams) || e >= n_parent) continue;
                     const uint32_t row = (e - uint32_t(kPolicyParams)) / kTail;
                     const U4 w = Philox4x32(row, child, p.gen_index, 0, p.seed, kTagRscl);
-                    if (BernoulliDraw(p.row_scale_p, w.x))
-                        v[i] = v[i] * (BernoulliDraw(0.5f, w.y) ? 1.02f : 0.98f);
+                    if (BernoulliDraw(p.row_scale_p, w.x)) v[i] = v[i] * (BernoulliDraw(0.5f, w.y) ? 1.02f : 0.98f);
                 }
             }
         }
         *reinterpret_cast<v4f*>(out + e0) = v;
+    };
+    // one quad ahead: the next quad's load is in flight while this one's draws are computed
+    uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    v4f cur = q < quads ? load_quad(q) : v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    for (; q < quads; q += stride) {
+        const uint32_t qn = q + stride;
+        const v4f nxt = qn < quads ? load_quad(qn) : v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        finish_quad(q, cur);
+        cur = nxt;
     }
 }
 
@@ -2110,8 +2129,12 @@ void LaunchXover(const XoverArgs& a, hipStream_t s) {
 
 void LaunchBreed(const BreedArgs& a, uint32_t batch, hipStream_t s) {
     const uint32_t quads = uint32_t(a.child_stride / 16);
-    uint32_t blocks_x = (quads + 255) / 256;
-    if (blocks_x > 64) blocks_x = 64;
+    // about 16k workgroups in all (measured at 4,096 children: 4 per child 2.66 ms, 16 2.78 ms, 64 3.14 ms
+    // -- each workgroup's long forward walk through its share of the child beats many short ones), at
+    // least 4 and at most 64 per child, so that small generations still fill the GPU
+    uint32_t blocks_x = (16384 + batch - 1) / batch;
+    blocks_x = blocks_x < 4 ? 4 : (blocks_x > 64 ? 64 : blocks_x);
+    if (blocks_x > (quads + 255) / 256) blocks_x = (quads + 255) / 256;
     hipLaunchKernelGGL(BreedKernel, dim3(blocks_x, batch), dim3(256), 0, s, a);
 }
 

@@ -60,6 +60,10 @@ struct PlanArgs {
     int* status;
 };
 
+// Per child, the element positions where the donor parent can change, ascending: [0] = count, then
+// the positions (layer, neuron-row, bias, tail-row and splice bounds, and the genome end).
+constexpr uint32_t kBreakStride = 64;
+
 struct XoverArgs {
     uint32_t child_count;
     uint32_t gen_index;
@@ -67,6 +71,7 @@ struct XoverArgs {
     uint32_t parent_action_count;
     float t1, t2, t3;
     XoverRecord* records;
+    uint32_t* breaks;              // [child_count][kBreakStride]
 };
 
 struct BreedArgs {
@@ -76,6 +81,7 @@ struct BreedArgs {
     const uint32_t* pairs;
     const uint32_t* parent_slots;
     const XoverRecord* records;
+    const uint32_t* breaks;        // [children][kBreakStride] from the crossover-decision kernel
     const uint32_t* free_stack;
     uint32_t free_count_before;
     uint32_t child_offset;

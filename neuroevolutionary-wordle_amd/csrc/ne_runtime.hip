@@ -60,6 +60,7 @@ struct ne_runtime {
     uint32_t* pairs = nullptr;
     uint32_t* refcounts = nullptr;
     XoverRecord* xover = nullptr;
+    uint32_t* breaks = nullptr;  // [max_gen][kBreakStride] donor-change positions per child
     float* rank_weights = nullptr;
     uint64_t rank_weights_len = 0;
     float rank_weights_exponent = -1.0f;
@@ -637,6 +638,7 @@ static int Assemble(ne_runtime* rt, uint32_t seed, const ne_assembly_config& cfg
     xa.t2 = cfg.crossover_temperature_level2;
     xa.t3 = cfg.crossover_temperature_level3;
     xa.records = rt->xover;
+    xa.breaks = rt->breaks;
     LaunchXover(xa, rt->stream);
     EventEnd(rt, 6);
     st = Finish(rt);
@@ -655,6 +657,7 @@ static int Assemble(ne_runtime* rt, uint32_t seed, const ne_assembly_config& cfg
         b.pairs = rt->pairs;
         b.parent_slots = rt->cur_slots;
         b.records = rt->xover;
+        b.breaks = rt->breaks;
         b.free_stack = rt->free_stack;
         b.free_count_before = fc;
         b.child_offset = uint32_t(offset);
@@ -997,7 +1000,8 @@ int ne_create(const ne_runtime_config* config, ne_runtime** out_runtime) {
           Alloc(&rt->cur_has, max_gen);
     ok &= Alloc(&rt->nxt_slots, max_gen) && Alloc(&rt->nxt_fit, max_gen) && Alloc(&rt->nxt_evals, max_gen) &&
           Alloc(&rt->nxt_has, max_gen);
-    ok &= Alloc(&rt->pairs, 2 * max_gen) && Alloc(&rt->refcounts, max_gen) && Alloc(&rt->xover, max_gen);
+    ok &= Alloc(&rt->pairs, 2 * max_gen) && Alloc(&rt->refcounts, max_gen) && Alloc(&rt->xover, max_gen) &&
+          Alloc(&rt->breaks, max_gen * kBreakStride);
     ok &= Alloc(&rt->raw, max_gen) && Alloc(&rt->local_w, max_gen) && Alloc(&rt->counters, 4) &&
           Alloc(&rt->status, 1) && Alloc(&rt->summary, 1) && Alloc(&rt->d_offsets, max_gen + 1);
     for (auto& e : rt->ev) ok &= Ok(hipEventCreate(&e.begin)) && Ok(hipEventCreate(&e.end));
@@ -1036,6 +1040,7 @@ void ne_destroy(ne_runtime* rt) {
     Free(rt->pairs);
     Free(rt->refcounts);
     Free(rt->xover);
+    Free(rt->breaks);
     Free(rt->rank_weights);
     Free(rt->enc_t);
     Free(rt->bf16_ops);

@@ -4,9 +4,13 @@
   grid128   128x128 grid (16,384 genomes, ~41 GB slab), 4,739 actions, 8,192 secrets in 256 shards of 32
   ragged    64x64 grid, per-cell shards of U{32..1024} secrets, actions growing 512 -> 4,739 in steps of
             512 every 5 generations (9 widening repacks + output-embedding injections), 50 generations
+  target    64x64 grid, 4,739 actions: wall-clock to the plain FP32 run's best fitness (over G generations,
+            default 200) for the BF16-screened form and the surrogate-screened form (brood 4x, best-predicted
+            25% + 5% audit, BF16-screened evaluation), per seed (default 3 seeds)
 
 Every run is timed in the plain FP32 form and in the BF16-screened form (which takes the same decisions);
-`--forms fp32` skips the latter.  Usage: python run_configs.py grid16|grid128|ragged [--forms fp32,bf16]
+`--forms fp32` skips the latter.  Usage: python run_configs.py grid16|grid128|ragged|target [--forms fp32,bf16]
+[--generations G] [--seeds S]
 """
 import argparse
 import json
@@ -119,17 +123,96 @@ def ragged_config(forms, gens=50, side=64, a0=512, a_final=4739, step=512, perio
                           recs[0]["best_fitness"] == recs[1]["best_fitness"]}), flush=True)
 
 
+def target_config(seeds, gens, cap_factor=2):
+    """Wall-clock to target: the plain FP32 run's best fitness over `gens` generations is the target; the
+    BF16-screened form (same decisions) and the surrogate-screened form run until their best-so-far
+    reaches it (at most cap_factor * gens generations).  Host wall-clock per step, steps only."""
+    wl = synth.workload("grid64_a4739")
+    A, side = wl["actions"], wl["side"]
+    n = side * side
+    stride = rtm.slot_stride_bytes(A)
+    counts = rtm.word_counts(A, A)
+    cfg = rtm.default_assembly_config(mutation_probability=0.01, mutation_sigma=0.02)
+    scfg = rtm.default_screen_config()
+    out = []
+    for seed in seeds:
+        def make(mode, brood):
+            rt = pkg.SlabRuntime(slab_bytes=(brood + 2) * n * stride, generation_bytes=brood * n * stride,
+                                 action_count=A, grid_columns=side)
+            rt.upload_catalog(wl["catalog"])
+            rt.upload_secrets_csr(wl["offsets"], wl["ids"])
+            rt.set_eval_precision(mode)
+            rt.bootstrap(n, seed=seed, variance_scale=1.0, tail_std=0.1)
+            return rt
+
+        def run(rt, step, limit, target=None):
+            t, best, trace, screened, rhos = 0.0, -1.0, [], 0, []
+            for g in range(limit):
+                t0 = time.perf_counter()
+                rep = step(rt, g)
+                t += time.perf_counter() - t0
+                best = max(best, float(rt.summary().best_fitness))
+                trace.append((round(t, 4), best))
+                if rep is not None:
+                    screened += rep.screened
+                    if np.isfinite(rep.audit_spearman):
+                        rhos.append(float(rep.audit_spearman))
+                if (g + 1) % 50 == 0:
+                    print(f"[target] seed {seed} generation {g + 1} best {best:.5f} {t:.2f} s", file=sys.stderr,
+                          flush=True)
+                if target is not None and best >= target:
+                    break
+            return trace, screened, rhos
+
+        plain = lambda rt, g: rt.advance(seed + 2 + g, counts, cfg)  # noqa: E731
+        screened_step = lambda rt, g: rt.advance_screened(seed + 2 + g, counts, cfg)  # noqa: E731
+        rt = make(rtm.EVAL_FP32, 1)
+        try:
+            trace_p, _, _ = run(rt, plain, gens)
+        finally:
+            rt.close()
+        target = trace_p[-1][1]
+        t_plain = next(tt for tt, b in trace_p if b >= target)
+        g_plain = next(i + 1 for i, (tt, b) in enumerate(trace_p) if b >= target)
+        rec = {"config": "target", "seed": seed, "genomes": n, "actions": A, "plain_generations": gens,
+               "target_fitness": target, "forms": {}}
+        rec["forms"]["fp32"] = {"seconds_to_target": t_plain, "generations_to_target": g_plain, "reached": True,
+                                "seconds_per_generation": trace_p[-1][0] / len(trace_p)}
+        for form in ("bf16", "surrogate"):
+            rt = make(rtm.EVAL_BF16_RESCORED, 4 if form == "surrogate" else 1)
+            try:
+                if form == "surrogate":
+                    rt.enable_screening(scfg)
+                tr, scr, rhos = run(rt, plain if form == "bf16" else screened_step, cap_factor * gens, target)
+            finally:
+                rt.close()
+            reached = tr[-1][1] >= target
+            rec["forms"][form] = {"seconds_to_target": tr[-1][0] if reached else None,
+                                  "generations_to_target": len(tr) if reached else None, "reached": reached,
+                                  "seconds_per_generation": tr[-1][0] / len(tr),
+                                  "speedup_vs_fp32": t_plain / tr[-1][0] if reached else None}
+            if form == "surrogate":
+                rec["forms"][form].update({"screened_generations": scr, "generations": len(tr),
+                                           "audit_spearman_mean": float(np.mean(rhos)) if rhos else None})
+        print(json.dumps(rec), flush=True)
+        out.append(rec)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("config", choices=["grid16", "grid128", "ragged"])
+    ap.add_argument("config", choices=["grid16", "grid128", "ragged", "target"])
     ap.add_argument("--forms", default="fp32,bf16")
     ap.add_argument("--generations", type=int, default=0)
+    ap.add_argument("--seeds", type=int, default=3)
     args = ap.parse_args()
     forms = [f for f in args.forms.split(",") if f]
     if args.config == "grid16":
         plain_config("grid16_a512", args.generations or 20, forms)
     elif args.config == "grid128":
         plain_config("grid128_a4739", args.generations or 3, forms)
+    elif args.config == "target":
+        target_config([7 + 10 * i for i in range(args.seeds)], args.generations or 200)
     else:
         ragged_config(forms, gens=args.generations or 50)
 
