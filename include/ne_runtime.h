@@ -302,8 +302,8 @@ typedef struct ne_screen_report {
     uint64_t replay_count;       /* pairs in the replay buffer */
 } ne_screen_report;
 
-/* Allocates the surrogate, its Adam state, the replay buffer and the 64 probe states (drawn from the
- * first action_count catalog words).  The generation budget must hold brood_factor x population
+/* Allocates the surrogate, its Adam state, the replay buffer and the 64 probe states.  Probe boards use,
+ * and are answered from, the probe vocabulary: the first min(action_count, 512) catalog words.  The generation budget must hold brood_factor x population
  * children and the slab one more population.  Calling it again resets the surrogate. */
 int ne_enable_screening(ne_runtime *rt, const ne_screen_config *config);
 /* One screened generation step (no widening or injection).  When the current generation has no
@@ -314,7 +314,7 @@ int ne_advance_generation_screened(ne_runtime *rt, uint32_t generation_seed, con
                                    ne_screen_report *report);
 /* Fingerprints of `count` genomes (slot indices) with the given parent fitness pairs ([count][2]):
  * out[count][192], per probe p = 16 d + k (board k at depth d): [3p] (2 greens + yellows) / 10 of the
- * genome's guess against the probe secret, [3p+1] d >= 1: 1 if the guess fits every hint on the board,
+ * genome's guess (its masked argmax over the probe vocabulary) against the probe secret, [3p+1] d >= 1: 1 if the guess fits every hint on the board,
  * d = 0: 10 x the fitness of parent k % 2, [3p+2] 1 if the guess is the secret. */
 int ne_fingerprint(ne_runtime *rt, const uint32_t *slot_indices, const float *parent_fitness, uint64_t count,
                    float *out);

@@ -800,6 +800,9 @@ constexpr uint32_t kTagProbe = 0x50524F42u;
 constexpr uint32_t kTagAudit = 0x41554454u;
 constexpr uint32_t kTagBatch = 0x42415443u;
 constexpr float kParentFitnessScale = 10.0f;
+// Probe boards are answered from the first kProbeVocabulary catalog words (the whole action space when
+// smaller): the guess characterises the policy at a ninth of the scoring cost of 4,739 actions.
+constexpr uint64_t kProbeVocabulary = 512;
 
 // Average ranks (ties share the mean of their positions), 1-based.
 static void AverageRanks(const float* v, uint64_t n, std::vector<double>& r) {
@@ -913,14 +916,21 @@ static int TrainFromReplay(ne_runtime* rt, float* loss) {
     if (rt->replay_count < bs) return NE_OK;
     float mu = 0.0f, sigma = 1.0f;
     ReplayStats(rt, &mu, &sigma);
-    std::vector<uint32_t> idx(bs);
-    for (uint32_t st = 0; st < rt->screen.train_steps; ++st) {
+    // every step's minibatch indices in one upload (draw of row b of step s: Philox(s, b; seed, tag)),
+    // then the steps back to back on the stream
+    const uint32_t steps = rt->screen.train_steps;
+    std::vector<uint32_t> idx(uint64_t(steps) * bs);
+    for (uint32_t st = 0; st < steps; ++st) {
+        const uint64_t s_abs = rt->sg_step + 1 + st;
         for (uint32_t b = 0; b < bs; ++b)
-            idx[b] = Philox4x32(uint32_t(rt->sg_step + 1), b, uint32_t((rt->sg_step + 1) >> 32), 0, rt->screen.seed,
-                                kTagBatch).x % uint32_t(rt->replay_count);
-        if (!Ok(hipMemcpyAsync(rt->bidx, idx.data(), bs * 4, hipMemcpyHostToDevice, rt->stream)))
-            return Fail(rt, NE_DEVICE_FAILURE);
-        LaunchReplayGather(rt->replay_x, rt->replay_f, rt->bidx, bs, mu, 1.0f / sigma, rt->bx, rt->bt, rt->stream);
+            idx[uint64_t(st) * bs + b] = Philox4x32(uint32_t(s_abs), b, uint32_t(s_abs >> 32), 0, rt->screen.seed,
+                                                    kTagBatch).x % uint32_t(rt->replay_count);
+    }
+    if (!Ok(hipMemcpyAsync(rt->bidx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, rt->stream)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    for (uint32_t st = 0; st < steps; ++st) {
+        LaunchReplayGather(rt->replay_x, rt->replay_f, rt->bidx + uint64_t(st) * bs, bs, mu, 1.0f / sigma, rt->bx,
+                           rt->bt, rt->stream);
         SurrogateStep(rt, rt->bx, rt->bt, bs);
     }
     std::vector<float> y(bs), t(bs);
@@ -942,7 +952,7 @@ static void LaunchFingerprints(ne_runtime* rt, const uint32_t* slots, uint32_t n
     f.slab = rt->slab;
     f.stride = rt->stride;
     f.slot_count = uint32_t(rt->slot_count);
-    f.action_count = uint32_t(rt->action_count);
+    f.action_count = uint32_t(std::min<uint64_t>(rt->action_count, kProbeVocabulary));
     f.slots = slots;
     f.codes = rt->codes;
     f.img = rt->img;
@@ -1466,7 +1476,7 @@ int ne_enable_screening(ne_runtime* rt, const ne_screen_config* config) {
         !(c.learning_rate > 0.0f) || !(c.spearman_threshold >= -1.0f && c.spearman_threshold <= 1.0f))
         return Fail(rt, NE_INVALID_ARGUMENT);
     if (rt->catalog_count == 0) return Fail(rt, NE_INVALID_TRAINING_SHARD);
-    if (!DrawProbes(rt, std::min<uint64_t>(rt->action_count, rt->catalog_count), c.seed))
+    if (!DrawProbes(rt, std::min<uint64_t>({rt->action_count, rt->catalog_count, kProbeVocabulary}), c.seed))
         return Fail(rt, NE_INVALID_TRAINING_SHARD);
     const bool realloc = !rt->screen_on || rt->screen.replay_capacity != c.replay_capacity;
     const uint64_t bmax = std::max<uint64_t>(c.batch_size, 1024);
@@ -1483,10 +1493,13 @@ int ne_enable_screening(ne_runtime* rt, const ne_screen_config* config) {
                   Alloc(&rt->e_slots, rt->max_gen) && Alloc(&rt->e_rows, rt->max_gen) && Alloc(&rt->bx, bmax * kSgIn) &&
                   Alloc(&rt->bt, bmax) && Alloc(&rt->by, bmax) && Alloc(&rt->bh1, bmax * kSgH1) &&
                   Alloc(&rt->bh2, bmax * kSgH2) && Alloc(&rt->bdy, bmax) && Alloc(&rt->bdh1, bmax * kSgH1) &&
-                  Alloc(&rt->bdh2, bmax * kSgH2) && Alloc(&rt->bidx, bmax);
+                  Alloc(&rt->bdh2, bmax * kSgH2);
         if (!ok) return Fail(rt, NE_DEVICE_FAILURE);
     }
     if (rt->enc_t == nullptr && !Alloc(&rt->enc_t, rt->max_gen * kEncTStride)) return Fail(rt, NE_DEVICE_FAILURE);
+    Free(rt->bidx);
+    if (!Alloc(&rt->bidx, std::max<uint64_t>(uint64_t(c.train_steps) * c.batch_size, 1)))
+        return Fail(rt, NE_DEVICE_FAILURE);
     rt->screen = c;
     rt->screen_on = true;
     rt->screen_active = false;
