@@ -1658,8 +1658,8 @@ __device__ __forceinline__ float MutatedValue(float v, uint32_t e, uint32_t chil
 // Crossover + mutation, element-parallel: each thread builds 4 consecutive values of one child
 // (BreedAndMutateGenome, genome_ops.cuh:386-407, on counter-based draws).  Writes the full slot.
 __global__ __launch_bounds__(256) void BreedKernel(BreedArgs p) {
-    const uint32_t bc = blockIdx.y;  // child within the batch
-    const uint32_t child = p.child_offset + bc;
+    const uint32_t bc = blockIdx.y;  // entry of the batch
+    const uint32_t child = p.child_list != nullptr ? p.child_list[bc] : p.child_offset + bc;
     __shared__ XoverRecord s_rec;
     __shared__ const float* s_p1;
     __shared__ const float* s_p2;
@@ -1671,13 +1671,14 @@ __global__ __launch_bounds__(256) void BreedKernel(BreedArgs p) {
         s_rec = p.records[child];
         const uint32_t pa = p.pairs[2 * child], pb = p.pairs[2 * child + 1];
         const uint32_t sa = p.parent_slots[pa], sb = p.parent_slots[pb];
-        const uint32_t cs = p.free_stack[p.free_count_before - 1 - bc];
+        const uint32_t cs =
+            p.given_slots != nullptr ? p.given_slots[child] : p.free_stack[p.free_count_before - 1 - bc];
         s_bad = (sa >= p.slot_count || sb >= p.slot_count || cs >= p.slot_count) ? 1 : 0;
         if (s_bad) SetStatus(p.status, 13);  // kInvalidParentIndex
         s_p1 = reinterpret_cast<const float*>(p.slab + uint64_t(sa) * p.parent_stride);
         s_p2 = reinterpret_cast<const float*>(p.slab + uint64_t(sb) * p.parent_stride);
         s_child = reinterpret_cast<float*>(p.slab + uint64_t(cs) * p.child_stride);
-        if (blockIdx.x == 0 && !s_bad) {
+        if (blockIdx.x == 0 && !s_bad && p.given_slots == nullptr) {
             p.child_slots[child] = cs;
             p.slot_state[cs] = 1u;
         }
@@ -1692,9 +1693,8 @@ __global__ __launch_bounds__(256) void BreedKernel(BreedArgs p) {
     float* out = s_child;
     const uint32_t nbreaks = s_brk[0];
     const uint32_t* s_breaks = s_brk + 1;
-    const uint32_t n_slot = uint32_t(p.child_stride / 4);
     const bool mutate = p.mutation_p > 0.0f && p.mutation_sigma > 0.0f;
-    const uint32_t quads = n_slot / 4;
+    const uint32_t quads = p.elem_end / 4;  // the loop covers quads [elem_begin / 4, elem_end / 4)
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t stride = gridDim.x * blockDim.x;
     uint32_t kb = 0;  // first break above the previous window's start
@@ -1743,7 +1743,7 @@ __global__ __launch_bounds__(256) void BreedKernel(BreedArgs p) {
         *reinterpret_cast<v4f*>(out + e0) = v;
     };
     // one quad ahead: the next quad's load is in flight while this one's draws are computed
-    uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t q = p.elem_begin / 4 + blockIdx.x * blockDim.x + threadIdx.x;
     v4f cur = q < quads ? load_quad(q) : v4f{0.0f, 0.0f, 0.0f, 0.0f};
     for (; q < quads; q += stride) {
         const uint32_t qn = q + stride;
@@ -2128,7 +2128,7 @@ void LaunchXover(const XoverArgs& a, hipStream_t s) {
 }
 
 void LaunchBreed(const BreedArgs& a, uint32_t batch, hipStream_t s) {
-    const uint32_t quads = uint32_t(a.child_stride / 16);
+    const uint32_t quads = (a.elem_end - a.elem_begin) / 4;
     // about 16k workgroups in all (measured at 4,096 children: 4 per child 2.66 ms, 16 2.78 ms, 64 3.14 ms
     // -- each workgroup's long forward walk through its share of the child beats many short ones), at
     // least 4 and at most 64 per child, so that small generations still fill the GPU

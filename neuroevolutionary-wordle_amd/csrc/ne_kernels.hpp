@@ -95,6 +95,10 @@ struct BreedArgs {
     float row_scale_p;
     uint32_t slot_count;
     int* status;
+    uint32_t elem_begin;           // child elements written: [elem_begin, elem_end), multiples of 4
+    uint32_t elem_end;
+    const uint32_t* child_list;    // optional: child index of each batch entry (else child_offset + entry)
+    const uint32_t* given_slots;   // optional: [child] slots already holding the child (completing it)
 };
 
 struct BootArgs {

@@ -671,6 +671,8 @@ static int Assemble(ne_runtime* rt, uint32_t seed, const ne_assembly_config& cfg
         b.row_scale_p = cfg.output_tail_row_scale_mutation_probability;
         b.slot_count = uint32_t(rt->slot_count);
         b.status = rt->status;
+        b.elem_begin = 0;
+        b.elem_end = uint32_t(rt->stride / 4);
         EventBegin(rt, 3);
         LaunchBreed(b, uint32_t(batch), rt->stream);
         EventEnd(rt, 3);
@@ -941,6 +943,93 @@ static int TrainFromReplay(ne_runtime* rt, float* loss) {
     double l = 0.0;
     for (uint32_t b = 0; b < bs; ++b) l += 0.5 * double(y[b] - t[b]) * double(y[b] - t[b]);
     *loss = float(l / double(bs));
+    return NE_OK;
+}
+
+// Brood assembly of screened evolution, in two phases over the planned children (rt->pairs): phase 1
+// breeds every child's policy and probe-vocabulary tail rows (all the fingerprint reads) into newly
+// popped slots; phase 2 (BreedRest) completes the children that will be evaluated.  Draws are keyed by
+// (element, child), so a completed child equals a one-pass breed.  The parents stay alive until phase 2.
+static BreedArgs BroodBreedArgs(ne_runtime* rt, uint32_t seed, const ne_assembly_config& cfg) {
+    BreedArgs b{};
+    b.slab = rt->slab;
+    b.parent_stride = rt->stride;
+    b.child_stride = rt->stride;
+    b.pairs = rt->pairs;
+    b.parent_slots = rt->cur_slots;
+    b.records = rt->xover;
+    b.breaks = rt->breaks;
+    b.free_stack = rt->free_stack;
+    b.child_slots = rt->nxt_slots;
+    b.slot_state = rt->slot_state;
+    b.parent_action_count = uint32_t(rt->action_count);
+    b.gen_index = uint32_t(rt->cur_index);
+    b.seed = seed;
+    b.mutation_p = cfg.mutation_probability;
+    b.mutation_sigma = cfg.mutation_sigma;
+    b.row_scale_p = cfg.output_tail_row_scale_mutation_probability;
+    b.slot_count = uint32_t(rt->slot_count);
+    b.status = rt->status;
+    return b;
+}
+
+static uint32_t BroodPhase1End(const ne_runtime* rt) {
+    const uint64_t e = kPolicyParams + std::min<uint64_t>(rt->action_count, kProbeVocabulary) * kTail;
+    return uint32_t(std::min<uint64_t>((e + 3) & ~uint64_t(3), rt->stride / 4));
+}
+
+static int AssembleBrood(ne_runtime* rt, uint32_t seed, const ne_assembly_config& cfg, uint64_t BN) {
+    rt->next_index = rt->cur_index + 1;
+    rt->next_size = BN;
+    int st = ClearGenerationBuffers(rt, true);
+    if (st != NE_OK) return st;
+    XoverArgs xa{};
+    xa.child_count = uint32_t(BN);
+    xa.gen_index = uint32_t(rt->cur_index);
+    xa.seed = seed;
+    xa.parent_action_count = uint32_t(rt->action_count);
+    xa.t1 = cfg.crossover_temperature_level1;
+    xa.t2 = cfg.crossover_temperature_level2;
+    xa.t3 = cfg.crossover_temperature_level3;
+    xa.records = rt->xover;
+    xa.breaks = rt->breaks;
+    LaunchXover(xa, rt->stream);
+    st = Finish(rt);
+    if (st != NE_OK) return st;
+    const uint32_t fc = HostFreeCount(rt);
+    if (fc < BN) return Fail(rt, NE_SLAB_FULL);  // parents and the whole brood are alive together
+    BreedArgs b = BroodBreedArgs(rt, seed, cfg);
+    b.free_count_before = fc;
+    b.child_offset = 0;
+    b.elem_begin = 0;
+    b.elem_end = BroodPhase1End(rt);
+    EventBegin(rt, 3);
+    LaunchBreed(b, uint32_t(BN), rt->stream);
+    EventEnd(rt, 3);
+    LaunchPopSlots(rt->free_count, uint32_t(BN), rt->stream);
+    st = Finish(rt);
+    if (st != NE_OK) return st;
+    rt->times.assemble_ms = EventMs(rt, 3);
+    rt->times.children = BN;
+    return NE_OK;
+}
+
+// Phase 2: the rest of each listed child (device list of `count` brood rows), in its own slot.
+static int BreedRest(ne_runtime* rt, uint32_t seed, const ne_assembly_config& cfg, const uint32_t* rows,
+                     uint64_t count) {
+    const uint32_t begin = BroodPhase1End(rt), end = uint32_t(rt->stride / 4);
+    if (begin >= end || count == 0) return NE_OK;
+    BreedArgs b = BroodBreedArgs(rt, seed, cfg);
+    b.elem_begin = begin;
+    b.elem_end = end;
+    b.child_list = rows;
+    b.given_slots = rt->nxt_slots;
+    EventBegin(rt, 3);
+    LaunchBreed(b, uint32_t(count), rt->stream);
+    EventEnd(rt, 3);
+    const int st = Finish(rt);
+    if (st != NE_OK) return st;
+    rt->times.assemble_ms += EventMs(rt, 3);
     return NE_OK;
 }
 
@@ -1567,9 +1656,10 @@ int ne_advance_generation_screened(ne_runtime* rt, uint32_t generation_seed, con
     if (st != NE_OK) return st;
     const float plan_ms = EventMs(rt, 2);
     rt->planned_children = BN;
-    st = Assemble(rt, generation_seed, cfg, rt->action_count);
+    // screening: the brood's first phase only (every child's policy and probe-vocabulary rows); otherwise
+    // the plain assembly (which also releases the parents as their children are done)
+    st = screened ? AssembleBrood(rt, generation_seed, cfg, BN) : Assemble(rt, generation_seed, cfg, rt->action_count);
     if (st != NE_OK) return st;
-    const float assemble_ms = rt->times.assemble_ms, book_ms = rt->times.bookkeeping_ms;
 
     // fingerprints of the whole brood, then the surrogate's predictions (parent fitness per position
     // stays in cur_fit after the parents' slots are released)
@@ -1623,6 +1713,17 @@ int ne_advance_generation_screened(ne_runtime* rt, uint32_t generation_seed, con
     if (!Ok(hipMemcpyAsync(rt->e_slots, e_slots.data(), E * 4, hipMemcpyHostToDevice, rt->stream)) ||
         !Ok(hipMemcpyAsync(rt->e_rows, rows_e.data(), E * 4, hipMemcpyHostToDevice, rt->stream)))
         return Fail(rt, NE_DEVICE_FAILURE);
+    if (screened) {
+        // complete the children that are evaluated, then every parent is free
+        st = BreedRest(rt, generation_seed, cfg, rt->e_rows, E);
+        if (st != NE_OK) return st;
+        if (!Ok(hipMemsetAsync(rt->refcounts, 0, N * sizeof(uint32_t), rt->stream))) return Fail(rt, NE_DEVICE_FAILURE);
+        LaunchCollectReleased(rt->cur_slots, uint32_t(N), rt->refcounts, rt->slot_state, rt->free_stack,
+                              rt->free_count, uint32_t(rt->slot_count), rt->status, rt->stream);
+        st = Finish(rt);
+        if (st != NE_OK) return st;
+    }
+    const float assemble_ms = rt->times.assemble_ms, book_ms = rt->times.bookkeeping_ms;
     st = EvaluateCore(rt, counts->action_space_word_count, rt->e_slots, E, eo, es, rt->e_fit, rt->e_evals, rt->e_has,
                       rt->cur_index + 1);
     if (st != NE_OK) return st;

@@ -843,15 +843,14 @@ uint64_t oro_chebyshev(uint64_t rows, uint64_t cols, uint64_t a, uint64_t b) { r
 
 // Parent plan (BuildSlabAssemblyPlanKernel, slab_runtime.cu:633-675, and
 // TrySelectCellularParentPairDevice, selection_ops.cuh:218-263).  wtable[r] = weight of rank r.
-int oro_plan(const float* fit, const uint8_t* has, uint64_t cur_size, uint64_t child_count, uint64_t cols,
-             uint64_t gen_index, uint64_t radius, const float* wtable, uint32_t generation_seed, uint32_t* pairs) {
-    if (cols == 0 || cur_size % cols || child_count % cols || child_count > cur_size || radius == 0)
-        return kInvalidAssemblyConfig;
+static int PlanChildren(const float* fit, const uint8_t* has, uint64_t cur_size, uint64_t child_count, uint64_t cols,
+                        uint64_t gen_index, uint64_t radius, const float* wtable, uint32_t generation_seed,
+                        uint32_t* pairs) {
     const uint64_t rows = cur_size / cols;
     const uint32_t seed = generation_seed ^ kPlanSeedXor;
     std::vector<uint64_t> cand(cur_size > 25 ? cur_size : 25);
     for (uint64_t c = 0; c < child_count; ++c) {
-        const uint64_t focal = (c / cols) * cols + (c % cols);
+        const uint64_t focal = c % cur_size;
         uint64_t ncand = 0;
         if (radius <= 2) {
             ncand = uint64_t(MooreNeighbors(rows, cols, focal, radius, cand.data()));
@@ -872,6 +871,24 @@ int oro_plan(const float* fit, const uint8_t* has, uint64_t cur_size, uint64_t c
         pairs[2 * c + 1] = uint32_t(chosen[1]);
     }
     return kOk;
+}
+
+int oro_plan(const float* fit, const uint8_t* has, uint64_t cur_size, uint64_t child_count, uint64_t cols,
+             uint64_t gen_index, uint64_t radius, const float* wtable, uint32_t generation_seed, uint32_t* pairs) {
+    if (cols == 0 || cur_size % cols || child_count % cols || child_count > cur_size || radius == 0)
+        return kInvalidAssemblyConfig;
+    return PlanChildren(fit, has, cur_size, child_count, cols, gen_index, radius, wtable, generation_seed, pairs);
+}
+
+// The brood of screened evolution (this build's extension, no reference counterpart): brood x cur_size
+// children, child c belonging to cell c % cur_size and drawing with child index c -- for c < cur_size
+// exactly oro_plan's child c.
+int oro_plan_brood(const float* fit, const uint8_t* has, uint64_t cur_size, uint64_t brood, uint64_t cols,
+                   uint64_t gen_index, uint64_t radius, const float* wtable, uint32_t generation_seed,
+                   uint32_t* pairs) {
+    if (cols == 0 || cur_size == 0 || cur_size % cols || brood == 0 || radius == 0) return kInvalidAssemblyConfig;
+    return PlanChildren(fit, has, cur_size, brood * cur_size, cols, gen_index, radius, wtable, generation_seed,
+                        pairs);
 }
 
 // Slot bookkeeping of one assembly (device_runtime.cu:1434-1557 with TryAllocateSlabSlot /

@@ -38,6 +38,7 @@ _SIGS = {
     "oro_moore": (I32, [U64, U64, U64, U64, P]),
     "oro_chebyshev": (U64, [U64, U64, U64, U64]),
     "oro_plan": (I32, [P, P, U64, U64, U64, U64, U64, P, U32, P]),
+    "oro_plan_brood": (I32, [P, P, U64, U64, U64, U64, U64, P, U32, P]),
     "oro_xover_decisions": (None, [I32, U64, U32, U32, U32, F32, F32, F32, P]),
     "oro_xover_record_words": (U64, []),
     "oro_assemble_slots": (I32, [U32, P, P, P, P, U32, P, U32, P, P]),
@@ -191,6 +192,17 @@ def plan(fitness, has, cur_size, child_count, cols, gen_index, radius, wtable, g
     pairs = np.zeros((child_count, 2), np.uint32)
     st = lib().oro_plan(ptr(f), ptr(h), cur_size, child_count, cols, gen_index, radius, ptr(w), generation_seed,
                         ptr(pairs))
+    return st, pairs
+
+
+def plan_brood(fitness, has, cur_size, brood, cols, gen_index, radius, wtable, generation_seed):
+    """Parent pairs of a screened step's brood: brood x cur_size children, child c of cell c % cur_size."""
+    f = np.ascontiguousarray(fitness, np.float32)
+    h = np.ascontiguousarray(has, np.uint8)
+    w = np.ascontiguousarray(wtable, np.float32)
+    pairs = np.zeros((brood * cur_size, 2), np.uint32)
+    st = lib().oro_plan_brood(ptr(f), ptr(h), cur_size, brood, cols, gen_index, radius, ptr(w), generation_seed,
+                              ptr(pairs))
     return st, pairs
 
 

@@ -175,3 +175,46 @@ def test_unscreened_steps_equal_the_plain_step(pkg):
         assert np.array_equal(rt_p.download_slot(sp[i]).view(np.uint32), rt_s.download_slot(ss[i]).view(np.uint32))
     rt_s.close()
     rt_p.close()
+
+
+def test_two_phase_brood_children_equal_one_pass_breeding(pkg, orc):
+    """A screened step breeds every brood member's policy and probe-vocabulary rows first and completes only
+    the evaluated ones; each cell's selected child must equal the oracle's one-pass breeding of one of the
+    cell's brood members (700 actions: 188 rows past the 512-word probe vocabulary are completed later)."""
+    rtm = pkg.runtime
+    A, side = 700, 8
+    cat = pkg.synth.make_catalog(A)
+    sec = pkg.synth.sample_secrets(A, 32)
+    off, ids = pkg.synth.block_csr(side, side, 4, sec)
+    wl = {"actions": A, "side": side, "catalog": cat, "offsets": off, "ids": ids}
+    rt = _runtime(pkg, wl, seed=5)
+    # a large audit slice so the 64-cell generation yields a finite rank correlation and screening starts
+    rt.enable_screening(rtm.default_screen_config(min_replay=1, spearman_threshold=-1.0, audit_fraction=0.5))
+    counts = rtm.word_counts(A, A)
+    cfg = rtm.default_assembly_config(mutation_probability=0.01, mutation_sigma=0.02)
+    assert rt.advance_screened(101, counts, cfg).screened == 0
+    slots, fit, _, has = rt.generation(0)
+    n = side * side
+    n_vals = orc.policy_param_count() + 38 * A
+    parents = [rt.download_slot(s)[:n_vals] for s in slots]
+    seed = 202
+    rep = rt.advance_screened(seed, counts, cfg)
+    assert rep.screened == 1 and rep.brood == 4 * n
+    children, _, _, _ = rt.generation(0)
+    st, pairs = orc.plan_brood(fit, has, n, 4, side, 1, 2, orc.rank_weights(0.5, max(n, 25)), seed)
+    assert st == 0
+    st, plain = orc.plan(fit, has, n, n, side, 1, 2, orc.rank_weights(0.5, max(n, 25)), seed)
+    assert st == 0 and np.array_equal(pairs[:n], plain), "brood member 0 is the plain plan's child"
+    temps = (cfg.crossover_temperature_level1, cfg.crossover_temperature_level2, cfg.crossover_temperature_level3)
+    picked = []
+    for c in range(n):
+        got = rt.download_slot(children[c])
+        assert not got[n_vals:].any()
+        match = [b for b in range(4)
+                 if np.array_equal(got[:n_vals].view(np.uint32), orc.breed_child(
+                     parents[pairs[b * n + c][0]], parents[pairs[b * n + c][1]], A, A, seed, 1, b * n + c, *temps,
+                     cfg.mutation_probability, cfg.mutation_sigma, 0.0)[0].view(np.uint32))]
+        assert match, c
+        picked.append(match[0])
+    assert len(set(picked)) > 1, "the surrogate picks different brood members in different cells"
+    rt.close()
