@@ -51,6 +51,7 @@ __global__ __launch_bounds__(128) void SurrogateForwardKernel(const float* th, c
         float acc[kSgRows];
 #pragma unroll
         for (uint32_t r = 0; r < kSgRows; ++r) acc[r] = th[kSgOffB1 + h];
+#pragma unroll 8
         for (uint32_t i = 0; i < kSgIn; ++i) {
             const float w = th[kSgOffW1 + i * kSgH1 + h];
 #pragma unroll
@@ -70,6 +71,7 @@ __global__ __launch_bounds__(128) void SurrogateForwardKernel(const float* th, c
         float acc[kHalf];
 #pragma unroll
         for (uint32_t r = 0; r < kHalf; ++r) acc[r] = th[kSgOffB2 + o];
+#pragma unroll 8
         for (uint32_t h = 0; h < kSgH1; ++h) {
             const float w = th[kSgOffW2 + h * kSgH2 + o];
 #pragma unroll
@@ -110,7 +112,8 @@ __global__ __launch_bounds__(128) void SurrogateBackpropKernel(const float* th, 
     dh1[uint64_t(m) * kSgH1 + tid] = h1[uint64_t(m) * kSgH1 + tid] > 0.0f ? acc : 0.0f;
 }
 
-// One thread per parameter: its gradient summed over the batch in sample order, then Adam.
+// One thread per parameter: its gradient summed over the batch in sample order (loads issued 8 samples
+// ahead of the accumulation), then Adam.
 __global__ __launch_bounds__(256) void SurrogateAdamKernel(float* th, float* am, float* av, const float* x,
                                                            const float* h1, const float* h2, const float* dy,
                                                            const float* dh1, const float* dh2, uint32_t n, float lr,
@@ -120,20 +123,26 @@ __global__ __launch_bounds__(256) void SurrogateAdamKernel(float* th, float* am,
     float g = 0.0f;
     if (p < kSgOffB1) {
         const uint32_t i = p / kSgH1, h = p - i * kSgH1;
+#pragma unroll 8
         for (uint32_t m = 0; m < n; ++m) g = fmaf(x[uint64_t(m) * kSgIn + i], dh1[uint64_t(m) * kSgH1 + h], g);
     } else if (p < kSgOffW2) {
         const uint32_t h = p - kSgOffB1;
+#pragma unroll 8
         for (uint32_t m = 0; m < n; ++m) g += dh1[uint64_t(m) * kSgH1 + h];
     } else if (p < kSgOffB2) {
         const uint32_t q = p - kSgOffW2, h = q / kSgH2, o = q - h * kSgH2;
+#pragma unroll 8
         for (uint32_t m = 0; m < n; ++m) g = fmaf(h1[uint64_t(m) * kSgH1 + h], dh2[uint64_t(m) * kSgH2 + o], g);
     } else if (p < kSgOffW3) {
         const uint32_t o = p - kSgOffB2;
+#pragma unroll 8
         for (uint32_t m = 0; m < n; ++m) g += dh2[uint64_t(m) * kSgH2 + o];
     } else if (p < kSgOffB3) {
         const uint32_t o = p - kSgOffW3;
+#pragma unroll 8
         for (uint32_t m = 0; m < n; ++m) g = fmaf(h2[uint64_t(m) * kSgH2 + o], dy[m], g);
     } else {
+#pragma unroll 8
         for (uint32_t m = 0; m < n; ++m) g += dy[m];
     }
     const float m1 = 0.9f * am[p] + 0.1f * g;
