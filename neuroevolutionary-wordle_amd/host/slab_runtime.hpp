@@ -233,14 +233,8 @@ inline bool TryReadPopulationFitnessSummaryFromDevice(const DeviceSlabGARuntimeB
     return true;
 }
 
-// TryAdvanceGenerationOnDevice (slab_runtime.hpp:144-150)
-inline bool TryAdvanceGenerationOnDevice(DeviceSlabGARuntimeBuffers& buffers, std::uint32_t generation_seed,
-                                         const RuntimeWordCounts& counts, const GenerationAssemblyConfig& config = {},
-                                         const PendingOutputEmbeddingInjection& injection = {}, bool verbose = false,
-                                         const PostFitnessEvaluationCallback& post_fitness_evaluation_callback = {},
-                                         const TrainingShardReleaseCallback& training_shard_release_callback = {},
-                                         const TrainingDataShardReleaseHistory* history = nullptr) {
-    const ne_word_counts c = detail::ToC(counts);
+namespace detail {
+inline ne_assembly_config ToC(const GenerationAssemblyConfig& config) {
     ne_assembly_config a{};
     a.cellular_breeding_radius = config.parent_selection.cellular_breeding_radius;
     a.rank_exponent = config.parent_selection.rank_exponent;
@@ -250,6 +244,19 @@ inline bool TryAdvanceGenerationOnDevice(DeviceSlabGARuntimeBuffers& buffers, st
     a.mutation_probability = config.mutation.mutation_probability;
     a.mutation_sigma = config.mutation.mutation_sigma;
     a.output_tail_row_scale_mutation_probability = config.mutation.output_tail_row_scale_mutation_probability;
+    return a;
+}
+}  // namespace detail
+
+// TryAdvanceGenerationOnDevice (slab_runtime.hpp:144-150)
+inline bool TryAdvanceGenerationOnDevice(DeviceSlabGARuntimeBuffers& buffers, std::uint32_t generation_seed,
+                                         const RuntimeWordCounts& counts, const GenerationAssemblyConfig& config = {},
+                                         const PendingOutputEmbeddingInjection& injection = {}, bool verbose = false,
+                                         const PostFitnessEvaluationCallback& post_fitness_evaluation_callback = {},
+                                         const TrainingShardReleaseCallback& training_shard_release_callback = {},
+                                         const TrainingDataShardReleaseHistory* history = nullptr) {
+    const ne_word_counts c = detail::ToC(counts);
+    const ne_assembly_config a = detail::ToC(config);
     ne_pending_injection inj{};
     inj.enabled = injection.enabled ? 1u : 0u;
     inj.first_catalog_word_index = injection.first_catalog_word_index;
@@ -295,6 +302,51 @@ inline bool TryDownloadCurrentGenerationFromDevice(const DeviceSlabGARuntimeBuff
 }
 inline bool TryDownloadNextGenerationFromDevice(const DeviceSlabGARuntimeBuffers& b, SlabGeneration& g) {
     return TryDownloadGenerationFromDevice(b, 1, g);
+}
+
+// Surrogate-screened evolution (no counterpart in the reference, whose step is TryAdvanceGenerationOnDevice
+// above): an oversampled brood per cell, ranked by a surrogate trained online, only the best-predicted
+// child of each cell plus a random audit slice evaluated in full.
+struct SurrogateScreeningConfig {
+    std::uint32_t brood_factor = 4;
+    std::uint32_t replay_capacity = 65536;
+    float audit_fraction = 0.05f;
+    float spearman_threshold = 0.5f;
+    float learning_rate = 1e-3f;
+    std::uint32_t train_steps = 32;
+    std::uint32_t batch_size = 256;
+    std::uint32_t min_replay = 4096;
+    std::uint32_t seed = 0x5C2EE7u;
+};
+
+inline bool TryEnableSurrogateScreening(DeviceSlabGARuntimeBuffers& buffers, const SurrogateScreeningConfig& config = {}) {
+    ne_screen_config c{};
+    c.brood_factor = config.brood_factor;
+    c.replay_capacity = config.replay_capacity;
+    c.audit_fraction = config.audit_fraction;
+    c.spearman_threshold = config.spearman_threshold;
+    c.learning_rate = config.learning_rate;
+    c.train_steps = config.train_steps;
+    c.batch_size = config.batch_size;
+    c.min_replay = config.min_replay;
+    c.seed = config.seed;
+    return ne_enable_screening(buffers.handle, &c) == NE_OK;
+}
+
+inline bool TryAdvanceScreenedGenerationOnDevice(DeviceSlabGARuntimeBuffers& buffers, std::uint32_t generation_seed,
+                                                 const RuntimeWordCounts& counts,
+                                                 const GenerationAssemblyConfig& config = {},
+                                                 const TrainingDataShardReleaseHistory* history = nullptr,
+                                                 ne_screen_report* report = nullptr) {
+    const ne_word_counts c = detail::ToC(counts);
+    const ne_assembly_config a = detail::ToC(config);
+    ne_shard_release_history h{};
+    if (history) {
+        h.release_generations = history->release_generations.data();
+        h.release_count = history->release_generations.size();
+    }
+    return ne_advance_generation_screened(buffers.handle, generation_seed, &c, &a, history ? &h : nullptr, report) ==
+           NE_OK;
 }
 
 // TryDownloadSlabSlotBytesFromDevice (slab_runtime.hpp:128-129)
