@@ -726,432 +726,442 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
     __shared__ __align__(16) uint8_t s_buf0[kBufBytes];
     __shared__ __align__(16) uint8_t s_buf1[kBufBytes];
 
-    const uint32_t n = blockIdx.x;
-    const uint32_t slot = p.slot_indices[n];
-    if (slot >= p.slot_count) {
-        if (threadIdx.x == 0) SetStatus(p.status, 4);  // kInvalidGeneration
-        return;
-    }
-    const float* genome = reinterpret_cast<const float*>(p.slab + uint64_t(slot) * p.stride);
-    const uint32_t tid = threadIdx.x, bd = blockDim.x;
-
-    const Rsrc enc_r = MakeRsrc(p.enc_t + uint64_t(n) * kEncTStride, uint32_t(kEncTStride * 4));
-
-    const uint32_t W = p.cell_offsets[n + 1] - p.cell_offsets[n];
-    if (W == 0) {
-        if (tid == 0) SetStatus(p.status, 5);  // kInvalidTrainingShard
-        return;
-    }
-    if (tid == 0 && blockIdx.y == 0) p.local_w[n] = W;
-    const uint32_t* local = p.secret_ids + p.cell_offsets[n];
-    const uint32_t tiles_fresh = (W + 15) / 16;
-    const uint32_t tiles = tiles_fresh + (2 * W + 15) / 16;
-
-    const uint32_t lane = tid & 63u;
-    const uint32_t j = lane & 15u;   // A-operand row / B-operand column within a 16-tile
-    const uint32_t gq = lane >> 4;   // k group (0..3)
-    const uint32_t waves = bd >> 6;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-
-    const TrunkLane tl = MakeTrunkLane(genome, j, gq);
-    const uint32_t A = p.action_count;
-    const uint32_t a_tiles = (A + 15) / 16;
-    const uint32_t chunks = (a_tiles + kCT - 1) / kCT;
-    const Rsrc img_r = MakeRsrc(p.img, p.img_bytes);
-    const Rsrc tail_r = MakeRsrc(genome + kPolicyParams, uint32_t(p.stride - kPolicyParams * 4));
-
-    auto issue = [&](uint8_t* buf, uint32_t c) __device__ {
-        IssueChunk(buf, img_r, tail_r, c, wave, waves, lane, G::kImgBytes, G::kTailBytes, G::kImgDma, G::kTailDma);
-    };
-    // BF16 screening stream, per chunk of kB16Tiles tiles: 3 x 16-byte transfers of catalog operands,
-    // 5 of the genome's packed operands, one 4-byte transfer of the 64 canonical indices
-    const uint32_t chunks16 = (a_tiles + kB16Tiles - 1) / kB16Tiles;
-    const Rsrc cat_r = MakeRsrc(p.bf16_cat, p.bf16_cat_bytes);
-    const Rsrc ops_r = MakeRsrc(kBf16 ? p.bf16_ops + uint64_t(n) * p.bf16_stride : p.slab, a_tiles * kBf16GenTileBytes);
-    const Rsrc canon_r = MakeRsrc(p.canon, p.canon_bytes);
-    auto issue16 = [&](uint8_t* buf, uint32_t c) __device__ {
-        constexpr uint32_t kCatDma = kB16CatBytes / kDmaBytes;
-        for (uint32_t i = wave; i < kB16OpsDma + 1; i += waves) {
-            if (i < kCatDma)
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(cat_r, (LdsPtr)(buf + i * kDmaBytes), 16,
-                                                         c * kB16CatBytes + i * kDmaBytes + 16 * lane, 0, 0, 0);
-            else if (i < kB16OpsDma)
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(ops_r, (LdsPtr)(buf + i * kDmaBytes), 16,
-                                                         c * kB16GenBytes + (i - kCatDma) * kDmaBytes + 16 * lane, 0,
-                                                         0, 0);
-            else
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(canon_r, (LdsPtr)(buf + kB16OpsDma * kDmaBytes), 4,
-                                                         c * (kB16Tiles * 64) + 4 * lane, 0, 0, 0);
+    // Persistent: the workgroup takes (genome, split) items from the work list until it is empty; split s
+    // of genome n plays tiles s*waves + r*splits*waves + wave (r = 0, 1, ...) of n's game tiles.
+    __shared__ uint32_t s_item;
+    for (;;) {
+        __syncthreads();  // the previous item is done with the LDS buffers
+        if (threadIdx.x == 0) s_item = atomicAdd(p.work_counter, 1u);
+        __syncthreads();
+        const uint32_t item = s_item;
+        if (item >= p.work_count) break;
+        const uint32_t n = p.work[2 * item], split = p.work[2 * item + 1];
+        const uint32_t slot = p.slot_indices[n];
+        if (slot >= p.slot_count) {
+            if (threadIdx.x == 0) SetStatus(p.status, 4);  // kInvalidGeneration
+            continue;
         }
-    };
+        const float* genome = reinterpret_cast<const float*>(p.slab + uint64_t(slot) * p.stride);
+        const uint32_t tid = threadIdx.x, bd = blockDim.x;
 
-    const uint32_t per_round = gridDim.y * waves;
-    const uint32_t first_tile = blockIdx.y * waves;
-    const uint32_t rounds = tiles > first_tile ? (tiles - first_tile + per_round - 1) / per_round : 0u;
-    for (uint32_t round = 0; round < rounds; ++round) {
-        const uint32_t tile = first_tile + round * per_round + wave;
-        const bool fresh = tile < tiles_fresh;
-        const uint32_t game = fresh ? tile * 16 + j : (tile - tiles_fresh) * 16 + j;
-        const bool valid = tile < tiles && (fresh ? game < W : game < 2 * W);
-        const uint32_t game_slot = fresh ? game : W + game;
+        const Rsrc enc_r = MakeRsrc(p.enc_t + uint64_t(n) * kEncTStride, uint32_t(kEncTStride * 4));
 
-        // ---- episode initialisation (evaluation_ops.cuh:171-237)
-        // hist[0..tc) are the canonical indices of the guessed words, most recent first (unused
-        // entries never match); (last, prev) are the two most recent turns with their feedback,
-        // which is all the turn-slot fold ever needs
-        uint32_t hist[5] = {kInvalidSlot, kInvalidSlot, kInvalidSlot, kInvalidSlot, kInvalidSlot};
-        uint32_t last_code = 0, last_fb = 0, prev_code = 0, prev_fb = 0;
-        GameTrace* tr = nullptr;
-        uint32_t sol = 0;
-        int tc = 0;
-        bool won = false;
-        bool active = false;
-        uint32_t decisions = 0;
-        uint32_t rescored = 0, fallbacks = 0;
-        if (valid) {
-            const uint32_t entry = fresh ? game : (game >> 1);
-            const uint32_t variant = fresh ? 0u : 1u + (game & 1u);
-            sol = p.codes[local[entry]];
-            bool ok = true;
-            if (variant != 0) {
-                const uint32_t first = variant == 1 ? entry + 1 : entry + 3;
-                const uint32_t w1 = local[first % W], w2 = local[(first + 1) % W];
-                const uint32_t c1 = p.codes[w1];
-                const uint32_t c2 = p.codes[w2];
-                tc = 1;
-                won = c1 == sol;
-                if (won) {
-                    ok = false;  // the second prefill would be appended to a finished grid
-                } else {
-                    prev_code = c1;
-                    prev_fb = FeedbackRegs(c1, sol);
-                    last_code = c2;
-                    last_fb = FeedbackRegs(c2, sol);
-                    hist[0] = p.canon[w2];
-                    hist[1] = p.canon[w1];
-                    tc = 2;
-                    won = c2 == sol;
-                }
-            }
-            if (!ok) {
-                if (gq == 0) SetStatus(p.status, 6);  // kGuessAppendFailed
-            } else {
-                active = !won && tc < kMaxTurns;
-            }
-            if (kTrace && p.game_trace != nullptr && gq == 0) {
-                tr = p.game_trace + p.game_offsets[n] + game_slot;
-                for (int d = 0; d < 6; ++d) {
-                    tr->guesses[d] = kInvalidSlot;
-                    tr->best[d] = 0.0f;
-                    tr->second[d] = 0.0f;
-                }
-            }
+        const uint32_t W = p.cell_offsets[n + 1] - p.cell_offsets[n];
+        if (W == 0) {
+            if (tid == 0) SetStatus(p.status, 5);  // kInvalidTrainingShard
+            continue;
         }
+        if (tid == 0 && split == 0) p.local_w[n] = W;
+        const uint32_t* local = p.secret_ids + p.cell_offsets[n];
+        const uint32_t tiles_fresh = (W + 15) / 16;
+        const uint32_t tiles = tiles_fresh + (2 * W + 15) / 16;
 
-        const int base_tc = fresh ? 0 : 2;
-        v4f acc[16];  // layer-0 pre-activations, neuron tile t rows 4*gq+r, game column j
-        int folded = 0;
-        for (int step = 0; step < kMaxTurns; ++step) {
-            if (!__syncthreads_or(active ? 1 : 0)) break;  // the whole workgroup leaves together
-            const bool wave_active = __any(active ? 1 : 0) != 0;
-            IssueTrunkChunk(s_buf0, tl.gr, 0, wave, waves, TrunkLaneW1(lane), TrunkLaneW2(lane));  // lands behind the fold
-            float pol[kScoreSteps];
-            if (wave_active) {
-                const int ttc = base_tc + step;  // turn count of every active game in this tile
-                if (step == 0 || (fresh && step == 1)) {
-                    InitLayer0(acc, tl);
-                    if (fresh && step == 0) {  // acc += W0[:, 0] (virgin flag), from the contiguous copy
-#pragma unroll
-                        for (int t = 0; t < 16; ++t)
-                            acc[t] += Buf4(enc_r, uint32_t(kTurnFeatures * kEncOut + 16 * t + 4 * gq) * 4, 0);
-                    }
-                    folded = 0;
-                }
-                // ---- fold newly occupied turn slots: acc += W0[:, 1+64*slot ...] . enc(turn slot)
-                for (int ts = folded; ts < ttc; ++ts) {
-                    const bool is_last = ts == ttc - 1;
-                    float e[16];
-                    EncodeTurnGlobal(enc_r, tl, is_last ? last_code : prev_code, is_last ? last_fb : prev_fb, e);
-                    FoldSlot(acc, tl, ts, e);
-                }
-                folded = ttc;
-            }
-            // layers 1 and 2; the scoring stream's first chunk goes into s_buf0 during the last chunk
-            static_assert(kTrunkChunkBytes <= kBufBytes && kTrunkReadBytes <= kBufBytes, "trunk chunk");
-            TrunkStream(acc, tl, pol, wave_active, s_buf0, s_buf1, wave, waves, lane, [&](uint8_t* b) __device__ {
-                if constexpr (kBf16)
-                    issue16(b, 0);
+        const uint32_t lane = tid & 63u;
+        const uint32_t j = lane & 15u;   // A-operand row / B-operand column within a 16-tile
+        const uint32_t gq = lane >> 4;   // k group (0..3)
+        const uint32_t waves = bd >> 6;
+        const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+
+        const TrunkLane tl = MakeTrunkLane(genome, j, gq);
+        const uint32_t A = p.action_count;
+        const uint32_t a_tiles = (A + 15) / 16;
+        const uint32_t chunks = (a_tiles + kCT - 1) / kCT;
+        const Rsrc img_r = MakeRsrc(p.img, p.img_bytes);
+        const Rsrc tail_r = MakeRsrc(genome + kPolicyParams, uint32_t(p.stride - kPolicyParams * 4));
+
+        auto issue = [&](uint8_t* buf, uint32_t c) __device__ {
+            IssueChunk(buf, img_r, tail_r, c, wave, waves, lane, G::kImgBytes, G::kTailBytes, G::kImgDma, G::kTailDma);
+        };
+        // BF16 screening stream, per chunk of kB16Tiles tiles: 3 x 16-byte transfers of catalog operands,
+        // 5 of the genome's packed operands, one 4-byte transfer of the 64 canonical indices
+        const uint32_t chunks16 = (a_tiles + kB16Tiles - 1) / kB16Tiles;
+        const Rsrc cat_r = MakeRsrc(p.bf16_cat, p.bf16_cat_bytes);
+        const Rsrc ops_r = MakeRsrc(kBf16 ? p.bf16_ops + uint64_t(n) * p.bf16_stride : p.slab, a_tiles * kBf16GenTileBytes);
+        const Rsrc canon_r = MakeRsrc(p.canon, p.canon_bytes);
+        auto issue16 = [&](uint8_t* buf, uint32_t c) __device__ {
+            constexpr uint32_t kCatDma = kB16CatBytes / kDmaBytes;
+            for (uint32_t i = wave; i < kB16OpsDma + 1; i += waves) {
+                if (i < kCatDma)
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(cat_r, (LdsPtr)(buf + i * kDmaBytes), 16,
+                                                             c * kB16CatBytes + i * kDmaBytes + 16 * lane, 0, 0, 0);
+                else if (i < kB16OpsDma)
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(ops_r, (LdsPtr)(buf + i * kDmaBytes), 16,
+                                                             c * kB16GenBytes + (i - kCatDma) * kDmaBytes + 16 * lane, 0,
+                                                             0, 0);
                 else
-                    issue(b, 0);
-            });
-            if (!wave_active) {
-#pragma unroll
-                for (int s = 0; s < kScoreSteps; ++s) pol[s] = 0.0f;
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(canon_r, (LdsPtr)(buf + kB16OpsDma * kDmaBytes), 4,
+                                                             c * (kB16Tiles * 64) + 4 * lane, 0, 0, 0);
             }
+        };
 
-            // ---- scoring: S = E . P over all actions, masked argmax (lowest index on ties)
-            float best = -__builtin_inff(), second = -__builtin_inff();
-            uint32_t besta = kInvalidSlot;
-            if constexpr (!kBf16) {
-                ArgmaxLane am = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot};
-                StreamChunks<kCT>(s_buf0, s_buf1, chunks, wave_active, issue,
-                                  [&](const uint8_t* buf, int k, uint32_t at) __device__ {
-                                      ScoreTile<kTrace>(Fp32Img<kCT>(buf, k), Fp32Tails<kCT>(buf, k), lane, j, gq, pol,
-                                                        at * 16 + 4 * gq, A, hist, am);
-                                  });
-                best = am.best;
-                second = am.second;
-                besta = am.besta;
-                ReduceArgmax(best, second, besta);
-            } else {
-                // BF16 screening with exact FP32 rescoring.  With |S_bf16 - S_fp32| <= eps for every
-                // action, the FP32 argmax (and every action tying with it) satisfies S_bf16 >= b16 - 2 eps,
-                // b16 = the BF16 masked maximum.  One screening pass keeps, per lane, every unmasked row
-                // with S_bf16 >= (its running maximum) - 2 eps, a superset of that window since the
-                // running maximum never exceeds b16; the window's members are then rescored with the
-                // FP32 contraction itself, so the FP32 argmax and its score are reproduced bit for bit.
-                // Decisions the window cannot settle (overflowing lists, non-finite scores) take the
-                // FP32 scan.
-                // each wave's candidate table lives in s_buf0, free from the end of the stream until the
-                // fallback's or the next decision's first transfer
-                static_assert(sizeof(uint32_t) * 4 * 16 * kMaxCandidates <= kBufBytes, "candidate table");
-                auto s_cand = reinterpret_cast<uint32_t (*)[16][kMaxCandidates]>(s_buf0);
-                __shared__ uint32_t s_ncand[4][16];
-                __shared__ float s_lsc[4][kLaneCandidates][64];
-                __shared__ uint32_t s_lac[4][kLaneCandidates][64];
-                // the policy as the B operands of the two screening instructions: every lane writes its
-                // dims (bf16) at their k-slots in the wave's (still unused) list scores, and reads back
-                // its own 2 x 8
-                PolicyBf16 pb;
-                {
-                    static_assert(sizeof(float) * kLaneCandidates * 64 >= 2 * 1024, "policy exchange");
-                    __bf16* xb = reinterpret_cast<__bf16*>(&s_lsc[wave][0][0]);
-#pragma unroll
-                    for (int s = 0; s < kScoreSteps; ++s) {
-                        const int d = ScoreDim(s, int(gq));
-                        if (d < 0) continue;
-                        int slot;
-                        const int m = Bf16DimSlot(d, &slot);
-                        xb[m * 512 + (int(j) + 16 * (slot >> 3)) * 8 + (slot & 7)] = __bf16(pol[s]);
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                    pb.k[0] = *reinterpret_cast<const bf16x8*>(xb + lane * 8);
-                    pb.k[1] = *reinterpret_cast<const bf16x8*>(xb + 512 + lane * 8);
-                }
-                // the bound: sum over this lane group's steps of M_d |p_d|, then over the 4 groups
-                float esum = 0.0f, pabs = 0.0f;
-#pragma unroll
-                for (int s = 0; s < kScoreSteps; ++s) {
-                    const int d = ScoreDim(s, int(gq));
-                    const float m = d < 0 ? 0.0f
-                                          : (d < kFixed ? p.fixed_absmax[d]
-                                                        : p.tail_absmax[uint64_t(n) * 40 + uint32_t(d - kFixed)]);
-                    esum += m * fabsf(pol[s]);
-                    pabs += m + fabsf(pol[s]);
-                }
-                esum += __shfl_xor(esum, 16);
-                esum += __shfl_xor(esum, 32);
-                pabs += __shfl_xor(pabs, 16);
-                pabs += __shfl_xor(pabs, 32);
-                const float eps = kBf16RelBound * esum + kBf16AbsBound * (pabs + 1.0f);
-                const float eps2 = 2.0f * eps;
-                bool fail = active && (p.force_fallback != 0 || !(eps <= 1.0e37f));
-                const bool want = active && !fail;
-                // lane state: running masked maximum, its threshold, list length, overflow / non-finite
-                float lbest = -__builtin_inff(), lthr = -__builtin_inff();
-                uint32_t lcnt = 0;
-                bool lfail = false;
-                float llost = -__builtin_inff();  // best score dropped from a full list
-                StreamChunks<kB16Tiles>(
-                    s_buf0, s_buf1, chunks16, __any(want ? 1 : 0) != 0, issue16,
-                    [&](const uint8_t* buf, int k, uint32_t at) __device__ {
-                        const v4f sc = Bf16Screen(buf, k, lane, pb);
-                        const float m4 = fmaxf(fmaxf(sc[0], sc[1]), fmaxf(sc[2], sc[3]));
-                        // a NaN or infinite score anywhere in the row quad (or an overflowing sum of
-                        // four) sends the decision to the FP32 scan
-                        lfail = lfail || !(fabsf((sc[0] + sc[1]) + (sc[2] + sc[3])) <= 3.0e38f);
-                        if (!__any((want && m4 >= lthr) ? 1 : 0)) return;
-                        const v4u cn = *reinterpret_cast<const v4u*>(Bf16Canon(buf, k, gq));
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const uint32_t a = at * 16 + 4 * gq + uint32_t(r);
-                            const uint32_t c = cn[r];
-                            const bool masked = (c == hist[0]) | (c == hist[1]) | (c == hist[2]) | (c == hist[3]) |
-                                                (c == hist[4]);
-                            const float v = sc[r];
-                            if (!(want && a < A && !masked && v >= lthr)) continue;
-                            if (v > lbest) {
-                                lbest = v;
-                                lthr = v - eps2;
-                            }
-                            if (lcnt == kLaneCandidates) {  // drop what the risen threshold excludes
-                                uint32_t w = 0;
-                                for (uint32_t q = 0; q < kLaneCandidates; ++q) {
-                                    const float sq = s_lsc[wave][q][lane];
-                                    if (sq >= lthr) {
-                                        s_lsc[wave][w][lane] = sq;
-                                        s_lac[wave][w][lane] = s_lac[wave][q][lane];
-                                        ++w;
-                                    }
-                                }
-                                lcnt = w;
-                            }
-                            if (lcnt < kLaneCandidates) {
-                                s_lsc[wave][lcnt][lane] = v;
-                                s_lac[wave][lcnt][lane] = a;
-                                ++lcnt;
-                            } else {  // full: keep the best kLaneCandidates, remember the best score dropped
-                                uint32_t qmin = 0;
-                                float smin = s_lsc[wave][0][lane];
-                                for (uint32_t q = 1; q < kLaneCandidates; ++q) {
-                                    const float sq = s_lsc[wave][q][lane];
-                                    if (sq < smin) {
-                                        smin = sq;
-                                        qmin = q;
-                                    }
-                                }
-                                if (v > smin) {
-                                    s_lsc[wave][qmin][lane] = v;
-                                    s_lac[wave][qmin][lane] = a;
-                                    llost = fmaxf(llost, smin);
-                                } else {
-                                    llost = fmaxf(llost, v);
-                                }
-                            }
-                        }
-                    });
-                // the game's window: b16 over its 4 lane groups, then every listed row within 2 eps
-                float b16 = lbest;
-                b16 = fmaxf(b16, __shfl_xor(b16, 16));
-                b16 = fmaxf(b16, __shfl_xor(b16, 32));
-                const float thr = b16 - eps2;
-                // a dropped score inside the window means the lists may miss a window member
-                int gfail = (lfail || llost >= thr) ? 1 : 0;
-                gfail |= __shfl_xor(gfail, 16);
-                gfail |= __shfl_xor(gfail, 32);
-                fail = fail || (want && (gfail != 0 || !(fabsf(b16) <= 3.0e38f)));
-                if (lane < 16) s_ncand[wave][lane] = 0u;
-                __builtin_amdgcn_wave_barrier();
-                if (want && !fail) {
-                    for (uint32_t q = 0; q < lcnt; ++q) {
-                        if (s_lsc[wave][q][lane] >= thr) {
-                            const uint32_t slot_k = atomicAdd(&s_ncand[wave][j], 1u);
-                            if (slot_k < kMaxCandidates) s_cand[wave][j][slot_k] = s_lac[wave][q][lane];
-                        }
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();
-                const uint32_t ncand = s_ncand[wave][j];
-                fail = fail || (want && (ncand == 0 || ncand > kMaxCandidates));
-                const bool resc = active && !fail;
-                uint32_t kmax = resc ? ncand : 0u;
-#pragma unroll
-                for (int off = 1; off < 64; off <<= 1) kmax = max(kmax, uint32_t(__shfl_xor(int(kmax), off)));
-                kmax = __builtin_amdgcn_readfirstlane(kmax);
-                for (uint32_t k = 0; k < kmax; ++k) {
-                    // candidate k of game i sits in MFMA row i; the diagonal D[j][j] is game j's score
-                    const uint32_t cand = (resc && k < ncand) ? s_cand[wave][j][k] : 0u;
-                    float ev[kScoreSteps];
-                    ActionOperands(img_r, tail_r, cand, gq, ev);
-                    const v4f sc = Fp32Scores(ev, pol);
-                    const float mine = (j & 3u) == 0 ? sc[0] : ((j & 3u) == 1 ? sc[1] : ((j & 3u) == 2 ? sc[2] : sc[3]));
-                    const float v = __shfl(mine, int(j + 16 * (j >> 2)));
-                    if (resc && k < ncand) {
-                        const bool better = besta == kInvalidSlot || v > best || (v == best && cand < besta);
-                        if (kTrace) {
-                            const float demoted = besta == kInvalidSlot ? second : fmaxf(second, best);
-                            second = better ? demoted : fmaxf(second, v);
-                        }
-                        best = better ? v : best;
-                        besta = better ? cand : besta;
-                    }
-                }
-                if (resc && gq == 0) rescored += ncand;
-                const bool need = active && fail;
-                if (__syncthreads_or(need ? 1 : 0)) {
-                    issue(s_buf0, 0);
-                    ArgmaxLane af = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot};
-                    StreamChunks<kCT>(s_buf0, s_buf1, chunks, __any(need ? 1 : 0) != 0, issue,
-                                      [&](const uint8_t* buf, int k, uint32_t at) __device__ {
-                                          ScoreTile<kTrace>(Fp32Img<kCT>(buf, k), Fp32Tails<kCT>(buf, k), lane, j, gq,
-                                                            pol, at * 16 + 4 * gq, A, hist, af);
-                                      });
-                    float fb = af.best, fs = af.second;
-                    uint32_t fa = af.besta;
-                    ReduceArgmax(fb, fs, fa);
-                    if (need) {
-                        best = fb;
-                        second = fs;
-                        besta = fa;
-                    }
-                }
-                if (need && gq == 0) ++fallbacks;
-            }
+        const uint32_t per_round = p.splits * waves;
+        const uint32_t first_tile = split * waves;
+        const uint32_t rounds = tiles > first_tile ? (tiles - first_tile + per_round - 1) / per_round : 0u;
+        for (uint32_t round = 0; round < rounds; ++round) {
+            const uint32_t tile = first_tile + round * per_round + wave;
+            const bool fresh = tile < tiles_fresh;
+            const uint32_t game = fresh ? tile * 16 + j : (tile - tiles_fresh) * 16 + j;
+            const bool valid = tile < tiles && (fresh ? game < W : game < 2 * W);
+            const uint32_t game_slot = fresh ? game : W + game;
 
-            if (kTrace && active && p.policy_trace != nullptr) {
-                float* dst = p.policy_trace + ((p.game_offsets[n] + game_slot) * uint64_t(kMaxTurns) + decisions) * kPol;
-#pragma unroll
-                for (int s = 0; s < kScoreSteps; ++s) {
-                    const int d = ScoreDim(s, int(gq));
-                    if (d >= 0) dst[d] = pol[s];
+            // ---- episode initialisation (evaluation_ops.cuh:171-237)
+            // hist[0..tc) are the canonical indices of the guessed words, most recent first (unused
+            // entries never match); (last, prev) are the two most recent turns with their feedback,
+            // which is all the turn-slot fold ever needs
+            uint32_t hist[5] = {kInvalidSlot, kInvalidSlot, kInvalidSlot, kInvalidSlot, kInvalidSlot};
+            uint32_t last_code = 0, last_fb = 0, prev_code = 0, prev_fb = 0;
+            GameTrace* tr = nullptr;
+            uint32_t sol = 0;
+            int tc = 0;
+            bool won = false;
+            bool active = false;
+            uint32_t decisions = 0;
+            uint32_t rescored = 0, fallbacks = 0;
+            if (valid) {
+                const uint32_t entry = fresh ? game : (game >> 1);
+                const uint32_t variant = fresh ? 0u : 1u + (game & 1u);
+                sol = p.codes[local[entry]];
+                bool ok = true;
+                if (variant != 0) {
+                    const uint32_t first = variant == 1 ? entry + 1 : entry + 3;
+                    const uint32_t w1 = local[first % W], w2 = local[(first + 1) % W];
+                    const uint32_t c1 = p.codes[w1];
+                    const uint32_t c2 = p.codes[w2];
+                    tc = 1;
+                    won = c1 == sol;
+                    if (won) {
+                        ok = false;  // the second prefill would be appended to a finished grid
+                    } else {
+                        prev_code = c1;
+                        prev_fb = FeedbackRegs(c1, sol);
+                        last_code = c2;
+                        last_fb = FeedbackRegs(c2, sol);
+                        hist[0] = p.canon[w2];
+                        hist[1] = p.canon[w1];
+                        tc = 2;
+                        won = c2 == sol;
+                    }
                 }
-            }
-            // ---- apply the guess (wordle_grid.hpp:308-316)
-            if (active) {
-                if (besta == kInvalidSlot) {
-                    if (gq == 0) SetStatus(p.status, 8);  // kActionSelectionFailed
-                    active = false;
+                if (!ok) {
+                    if (gq == 0) SetStatus(p.status, 6);  // kGuessAppendFailed
                 } else {
-                    const uint32_t code = p.codes[besta];
-#pragma unroll
-                    for (int t = 4; t > 0; --t) hist[t] = hist[t - 1];
-                    hist[0] = p.canon[besta];
-                    prev_code = last_code;
-                    prev_fb = last_fb;
-                    last_code = code;
-                    last_fb = FeedbackRegs(code, sol);
-                    if (kTrace && tr != nullptr) {  // every active game has taken `step` decisions
-                        tr->guesses[step] = besta;
-                        tr->best[step] = best;
-                        tr->second[step] = second;
-                    }
-                    ++tc;
-                    ++decisions;
-                    won = code == sol;
                     active = !won && tc < kMaxTurns;
                 }
+                if (kTrace && p.game_trace != nullptr && gq == 0) {
+                    tr = p.game_trace + p.game_offsets[n] + game_slot;
+                    for (int d = 0; d < 6; ++d) {
+                        tr->guesses[d] = kInvalidSlot;
+                        tr->best[d] = 0.0f;
+                        tr->second[d] = 0.0f;
+                    }
+                }
             }
-        }
 
-        // ---- score the finished episodes (evaluation_ops.cuh:239-242) and sum over the tile
-        float score = (valid && won) ? 10.0f + float(kMaxTurns - tc) : 0.0f;
-        uint32_t dec = (valid && gq == 0) ? decisions : 0u;
-        // turns that went through the encoder: every turn present at the game's last decision
-        uint32_t enc = (valid && gq == 0 && decisions > 0) ? uint32_t(base_tc) + decisions - 1u : 0u;
-        if (gq != 0) score = 0.0f;
-        if (kTrace && tr != nullptr) {
-            tr->decisions = decisions;
-            tr->turn_count = uint32_t(tc);
-            tr->score = score;
-            tr->won = won ? 1u : 0u;
-        }
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            score += __shfl_xor(score, off);
-            dec += __shfl_xor(dec, off);
-            enc += __shfl_xor(enc, off);
-            if (kBf16) {
-                rescored += __shfl_xor(rescored, off);
-                fallbacks += __shfl_xor(fallbacks, off);
+            const int base_tc = fresh ? 0 : 2;
+            v4f acc[16];  // layer-0 pre-activations, neuron tile t rows 4*gq+r, game column j
+            int folded = 0;
+            for (int step = 0; step < kMaxTurns; ++step) {
+                if (!__syncthreads_or(active ? 1 : 0)) break;  // the whole workgroup leaves together
+                const bool wave_active = __any(active ? 1 : 0) != 0;
+                IssueTrunkChunk(s_buf0, tl.gr, 0, wave, waves, TrunkLaneW1(lane), TrunkLaneW2(lane));  // lands behind the fold
+                float pol[kScoreSteps];
+                if (wave_active) {
+                    const int ttc = base_tc + step;  // turn count of every active game in this tile
+                    if (step == 0 || (fresh && step == 1)) {
+                        InitLayer0(acc, tl);
+                        if (fresh && step == 0) {  // acc += W0[:, 0] (virgin flag), from the contiguous copy
+    #pragma unroll
+                            for (int t = 0; t < 16; ++t)
+                                acc[t] += Buf4(enc_r, uint32_t(kTurnFeatures * kEncOut + 16 * t + 4 * gq) * 4, 0);
+                        }
+                        folded = 0;
+                    }
+                    // ---- fold newly occupied turn slots: acc += W0[:, 1+64*slot ...] . enc(turn slot)
+                    for (int ts = folded; ts < ttc; ++ts) {
+                        const bool is_last = ts == ttc - 1;
+                        float e[16];
+                        EncodeTurnGlobal(enc_r, tl, is_last ? last_code : prev_code, is_last ? last_fb : prev_fb, e);
+                        FoldSlot(acc, tl, ts, e);
+                    }
+                    folded = ttc;
+                }
+                // layers 1 and 2; the scoring stream's first chunk goes into s_buf0 during the last chunk
+                static_assert(kTrunkChunkBytes <= kBufBytes && kTrunkReadBytes <= kBufBytes, "trunk chunk");
+                TrunkStream(acc, tl, pol, wave_active, s_buf0, s_buf1, wave, waves, lane, [&](uint8_t* b) __device__ {
+                    if constexpr (kBf16)
+                        issue16(b, 0);
+                    else
+                        issue(b, 0);
+                });
+                if (!wave_active) {
+    #pragma unroll
+                    for (int s = 0; s < kScoreSteps; ++s) pol[s] = 0.0f;
+                }
+
+                // ---- scoring: S = E . P over all actions, masked argmax (lowest index on ties)
+                float best = -__builtin_inff(), second = -__builtin_inff();
+                uint32_t besta = kInvalidSlot;
+                if constexpr (!kBf16) {
+                    ArgmaxLane am = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot};
+                    StreamChunks<kCT>(s_buf0, s_buf1, chunks, wave_active, issue,
+                                      [&](const uint8_t* buf, int k, uint32_t at) __device__ {
+                                          ScoreTile<kTrace>(Fp32Img<kCT>(buf, k), Fp32Tails<kCT>(buf, k), lane, j, gq, pol,
+                                                            at * 16 + 4 * gq, A, hist, am);
+                                      });
+                    best = am.best;
+                    second = am.second;
+                    besta = am.besta;
+                    ReduceArgmax(best, second, besta);
+                } else {
+                    // BF16 screening with exact FP32 rescoring.  With |S_bf16 - S_fp32| <= eps for every
+                    // action, the FP32 argmax (and every action tying with it) satisfies S_bf16 >= b16 - 2 eps,
+                    // b16 = the BF16 masked maximum.  One screening pass keeps, per lane, every unmasked row
+                    // with S_bf16 >= (its running maximum) - 2 eps, a superset of that window since the
+                    // running maximum never exceeds b16; the window's members are then rescored with the
+                    // FP32 contraction itself, so the FP32 argmax and its score are reproduced bit for bit.
+                    // Decisions the window cannot settle (overflowing lists, non-finite scores) take the
+                    // FP32 scan.
+                    // each wave's candidate table lives in s_buf0, free from the end of the stream until the
+                    // fallback's or the next decision's first transfer
+                    static_assert(sizeof(uint32_t) * 4 * 16 * kMaxCandidates <= kBufBytes, "candidate table");
+                    auto s_cand = reinterpret_cast<uint32_t (*)[16][kMaxCandidates]>(s_buf0);
+                    __shared__ uint32_t s_ncand[4][16];
+                    __shared__ float s_lsc[4][kLaneCandidates][64];
+                    __shared__ uint32_t s_lac[4][kLaneCandidates][64];
+                    // the policy as the B operands of the two screening instructions: every lane writes its
+                    // dims (bf16) at their k-slots in the wave's (still unused) list scores, and reads back
+                    // its own 2 x 8
+                    PolicyBf16 pb;
+                    {
+                        static_assert(sizeof(float) * kLaneCandidates * 64 >= 2 * 1024, "policy exchange");
+                        __bf16* xb = reinterpret_cast<__bf16*>(&s_lsc[wave][0][0]);
+    #pragma unroll
+                        for (int s = 0; s < kScoreSteps; ++s) {
+                            const int d = ScoreDim(s, int(gq));
+                            if (d < 0) continue;
+                            int slot;
+                            const int m = Bf16DimSlot(d, &slot);
+                            xb[m * 512 + (int(j) + 16 * (slot >> 3)) * 8 + (slot & 7)] = __bf16(pol[s]);
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        pb.k[0] = *reinterpret_cast<const bf16x8*>(xb + lane * 8);
+                        pb.k[1] = *reinterpret_cast<const bf16x8*>(xb + 512 + lane * 8);
+                    }
+                    // the bound: sum over this lane group's steps of M_d |p_d|, then over the 4 groups
+                    float esum = 0.0f, pabs = 0.0f;
+    #pragma unroll
+                    for (int s = 0; s < kScoreSteps; ++s) {
+                        const int d = ScoreDim(s, int(gq));
+                        const float m = d < 0 ? 0.0f
+                                              : (d < kFixed ? p.fixed_absmax[d]
+                                                            : p.tail_absmax[uint64_t(n) * 40 + uint32_t(d - kFixed)]);
+                        esum += m * fabsf(pol[s]);
+                        pabs += m + fabsf(pol[s]);
+                    }
+                    esum += __shfl_xor(esum, 16);
+                    esum += __shfl_xor(esum, 32);
+                    pabs += __shfl_xor(pabs, 16);
+                    pabs += __shfl_xor(pabs, 32);
+                    const float eps = kBf16RelBound * esum + kBf16AbsBound * (pabs + 1.0f);
+                    const float eps2 = 2.0f * eps;
+                    bool fail = active && (p.force_fallback != 0 || !(eps <= 1.0e37f));
+                    const bool want = active && !fail;
+                    // lane state: running masked maximum, its threshold, list length, overflow / non-finite
+                    float lbest = -__builtin_inff(), lthr = -__builtin_inff();
+                    uint32_t lcnt = 0;
+                    bool lfail = false;
+                    float llost = -__builtin_inff();  // best score dropped from a full list
+                    StreamChunks<kB16Tiles>(
+                        s_buf0, s_buf1, chunks16, __any(want ? 1 : 0) != 0, issue16,
+                        [&](const uint8_t* buf, int k, uint32_t at) __device__ {
+                            const v4f sc = Bf16Screen(buf, k, lane, pb);
+                            const float m4 = fmaxf(fmaxf(sc[0], sc[1]), fmaxf(sc[2], sc[3]));
+                            // a NaN or infinite score anywhere in the row quad (or an overflowing sum of
+                            // four) sends the decision to the FP32 scan
+                            lfail = lfail || !(fabsf((sc[0] + sc[1]) + (sc[2] + sc[3])) <= 3.0e38f);
+                            if (!__any((want && m4 >= lthr) ? 1 : 0)) return;
+                            const v4u cn = *reinterpret_cast<const v4u*>(Bf16Canon(buf, k, gq));
+    #pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const uint32_t a = at * 16 + 4 * gq + uint32_t(r);
+                                const uint32_t c = cn[r];
+                                const bool masked = (c == hist[0]) | (c == hist[1]) | (c == hist[2]) | (c == hist[3]) |
+                                                    (c == hist[4]);
+                                const float v = sc[r];
+                                if (!(want && a < A && !masked && v >= lthr)) continue;
+                                if (v > lbest) {
+                                    lbest = v;
+                                    lthr = v - eps2;
+                                }
+                                if (lcnt == kLaneCandidates) {  // drop what the risen threshold excludes
+                                    uint32_t w = 0;
+                                    for (uint32_t q = 0; q < kLaneCandidates; ++q) {
+                                        const float sq = s_lsc[wave][q][lane];
+                                        if (sq >= lthr) {
+                                            s_lsc[wave][w][lane] = sq;
+                                            s_lac[wave][w][lane] = s_lac[wave][q][lane];
+                                            ++w;
+                                        }
+                                    }
+                                    lcnt = w;
+                                }
+                                if (lcnt < kLaneCandidates) {
+                                    s_lsc[wave][lcnt][lane] = v;
+                                    s_lac[wave][lcnt][lane] = a;
+                                    ++lcnt;
+                                } else {  // full: keep the best kLaneCandidates, remember the best score dropped
+                                    uint32_t qmin = 0;
+                                    float smin = s_lsc[wave][0][lane];
+                                    for (uint32_t q = 1; q < kLaneCandidates; ++q) {
+                                        const float sq = s_lsc[wave][q][lane];
+                                        if (sq < smin) {
+                                            smin = sq;
+                                            qmin = q;
+                                        }
+                                    }
+                                    if (v > smin) {
+                                        s_lsc[wave][qmin][lane] = v;
+                                        s_lac[wave][qmin][lane] = a;
+                                        llost = fmaxf(llost, smin);
+                                    } else {
+                                        llost = fmaxf(llost, v);
+                                    }
+                                }
+                            }
+                        });
+                    // the game's window: b16 over its 4 lane groups, then every listed row within 2 eps
+                    float b16 = lbest;
+                    b16 = fmaxf(b16, __shfl_xor(b16, 16));
+                    b16 = fmaxf(b16, __shfl_xor(b16, 32));
+                    const float thr = b16 - eps2;
+                    // a dropped score inside the window means the lists may miss a window member
+                    int gfail = (lfail || llost >= thr) ? 1 : 0;
+                    gfail |= __shfl_xor(gfail, 16);
+                    gfail |= __shfl_xor(gfail, 32);
+                    fail = fail || (want && (gfail != 0 || !(fabsf(b16) <= 3.0e38f)));
+                    if (lane < 16) s_ncand[wave][lane] = 0u;
+                    __builtin_amdgcn_wave_barrier();
+                    if (want && !fail) {
+                        for (uint32_t q = 0; q < lcnt; ++q) {
+                            if (s_lsc[wave][q][lane] >= thr) {
+                                const uint32_t slot_k = atomicAdd(&s_ncand[wave][j], 1u);
+                                if (slot_k < kMaxCandidates) s_cand[wave][j][slot_k] = s_lac[wave][q][lane];
+                            }
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    const uint32_t ncand = s_ncand[wave][j];
+                    fail = fail || (want && (ncand == 0 || ncand > kMaxCandidates));
+                    const bool resc = active && !fail;
+                    uint32_t kmax = resc ? ncand : 0u;
+    #pragma unroll
+                    for (int off = 1; off < 64; off <<= 1) kmax = max(kmax, uint32_t(__shfl_xor(int(kmax), off)));
+                    kmax = __builtin_amdgcn_readfirstlane(kmax);
+                    for (uint32_t k = 0; k < kmax; ++k) {
+                        // candidate k of game i sits in MFMA row i; the diagonal D[j][j] is game j's score
+                        const uint32_t cand = (resc && k < ncand) ? s_cand[wave][j][k] : 0u;
+                        float ev[kScoreSteps];
+                        ActionOperands(img_r, tail_r, cand, gq, ev);
+                        const v4f sc = Fp32Scores(ev, pol);
+                        const float mine = (j & 3u) == 0 ? sc[0] : ((j & 3u) == 1 ? sc[1] : ((j & 3u) == 2 ? sc[2] : sc[3]));
+                        const float v = __shfl(mine, int(j + 16 * (j >> 2)));
+                        if (resc && k < ncand) {
+                            const bool better = besta == kInvalidSlot || v > best || (v == best && cand < besta);
+                            if (kTrace) {
+                                const float demoted = besta == kInvalidSlot ? second : fmaxf(second, best);
+                                second = better ? demoted : fmaxf(second, v);
+                            }
+                            best = better ? v : best;
+                            besta = better ? cand : besta;
+                        }
+                    }
+                    if (resc && gq == 0) rescored += ncand;
+                    const bool need = active && fail;
+                    if (__syncthreads_or(need ? 1 : 0)) {
+                        issue(s_buf0, 0);
+                        ArgmaxLane af = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot};
+                        StreamChunks<kCT>(s_buf0, s_buf1, chunks, __any(need ? 1 : 0) != 0, issue,
+                                          [&](const uint8_t* buf, int k, uint32_t at) __device__ {
+                                              ScoreTile<kTrace>(Fp32Img<kCT>(buf, k), Fp32Tails<kCT>(buf, k), lane, j, gq,
+                                                                pol, at * 16 + 4 * gq, A, hist, af);
+                                          });
+                        float fb = af.best, fs = af.second;
+                        uint32_t fa = af.besta;
+                        ReduceArgmax(fb, fs, fa);
+                        if (need) {
+                            best = fb;
+                            second = fs;
+                            besta = fa;
+                        }
+                    }
+                    if (need && gq == 0) ++fallbacks;
+                }
+
+                if (kTrace && active && p.policy_trace != nullptr) {
+                    float* dst = p.policy_trace + ((p.game_offsets[n] + game_slot) * uint64_t(kMaxTurns) + decisions) * kPol;
+    #pragma unroll
+                    for (int s = 0; s < kScoreSteps; ++s) {
+                        const int d = ScoreDim(s, int(gq));
+                        if (d >= 0) dst[d] = pol[s];
+                    }
+                }
+                // ---- apply the guess (wordle_grid.hpp:308-316)
+                if (active) {
+                    if (besta == kInvalidSlot) {
+                        if (gq == 0) SetStatus(p.status, 8);  // kActionSelectionFailed
+                        active = false;
+                    } else {
+                        const uint32_t code = p.codes[besta];
+    #pragma unroll
+                        for (int t = 4; t > 0; --t) hist[t] = hist[t - 1];
+                        hist[0] = p.canon[besta];
+                        prev_code = last_code;
+                        prev_fb = last_fb;
+                        last_code = code;
+                        last_fb = FeedbackRegs(code, sol);
+                        if (kTrace && tr != nullptr) {  // every active game has taken `step` decisions
+                            tr->guesses[step] = besta;
+                            tr->best[step] = best;
+                            tr->second[step] = second;
+                        }
+                        ++tc;
+                        ++decisions;
+                        won = code == sol;
+                        active = !won && tc < kMaxTurns;
+                    }
+                }
             }
-        }
-        if (lane == 0 && tile < tiles) {
-            atomicAdd(p.raw_fitness + n, score);
-            atomicAdd(p.counters + 0, (unsigned long long)dec);
-            atomicAdd(p.counters + 1, (unsigned long long)enc);
-            if (kBf16) {
-                atomicAdd(p.counters + 2, (unsigned long long)rescored);
-                atomicAdd(p.counters + 3, (unsigned long long)fallbacks);
+
+            // ---- score the finished episodes (evaluation_ops.cuh:239-242) and sum over the tile
+            float score = (valid && won) ? 10.0f + float(kMaxTurns - tc) : 0.0f;
+            uint32_t dec = (valid && gq == 0) ? decisions : 0u;
+            // turns that went through the encoder: every turn present at the game's last decision
+            uint32_t enc = (valid && gq == 0 && decisions > 0) ? uint32_t(base_tc) + decisions - 1u : 0u;
+            if (gq != 0) score = 0.0f;
+            if (kTrace && tr != nullptr) {
+                tr->decisions = decisions;
+                tr->turn_count = uint32_t(tc);
+                tr->score = score;
+                tr->won = won ? 1u : 0u;
+            }
+    #pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                score += __shfl_xor(score, off);
+                dec += __shfl_xor(dec, off);
+                enc += __shfl_xor(enc, off);
+                if (kBf16) {
+                    rescored += __shfl_xor(rescored, off);
+                    fallbacks += __shfl_xor(fallbacks, off);
+                }
+            }
+            if (lane == 0 && tile < tiles) {
+                atomicAdd(p.raw_fitness + n, score);
+                atomicAdd(p.counters + 0, (unsigned long long)dec);
+                atomicAdd(p.counters + 1, (unsigned long long)enc);
+                if (kBf16) {
+                    atomicAdd(p.counters + 2, (unsigned long long)rescored);
+                    atomicAdd(p.counters + 3, (unsigned long long)fallbacks);
+                }
             }
         }
     }
@@ -2087,9 +2097,8 @@ void LaunchTransposeEncoder(const uint8_t* slab, uint64_t stride, const uint32_t
     hipLaunchKernelGGL(TransposeEncoderKernel, dim3(n), dim3(256), 0, s, slab, stride, slots, slot_count, enc_t);
 }
 
-void LaunchEvaluate(const EvalArgs& a, uint32_t n_genomes, uint32_t waves, uint32_t chunks, bool trace, bool bf16,
-                    hipStream_t s) {
-    const dim3 grid(n_genomes, chunks);
+void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, bool trace, bool bf16, hipStream_t s) {
+    const dim3 grid(blocks);
     const dim3 block(64 * waves);
     if (bf16) {
         if (trace)

@@ -44,6 +44,10 @@ struct EvalArgs {
     GameTrace* game_trace;         // optional (trace mode)
     float* policy_trace;           // optional: [games][6][64]
     const uint64_t* game_offsets;  // [N] first game index of each individual (trace mode)
+    const uint32_t* work;          // [work_count][2] (genome, split) items, taken in order
+    uint32_t work_count;
+    uint32_t* work_counter;        // next item (0 at launch)
+    uint32_t splits;               // the largest split count of a genome
 };
 
 struct PlanArgs {
@@ -220,8 +224,8 @@ void LaunchPackBf16(const uint8_t* slab, uint64_t stride, const uint32_t* slots,
                     hipStream_t s);
 void LaunchTransposeEncoder(const uint8_t* slab, uint64_t stride, const uint32_t* slots, uint32_t n,
                             uint32_t slot_count, float* enc_t, hipStream_t s);
-void LaunchEvaluate(const EvalArgs& a, uint32_t n_genomes, uint32_t waves, uint32_t chunks, bool trace, bool bf16,
-                    hipStream_t s);
+// Persistent evaluation: `blocks` workgroups of `waves` waves drain the work list.
+void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, bool trace, bool bf16, hipStream_t s);
 void LaunchFinalize(const float* raw, const uint32_t* local_w, uint32_t n, float* fitness, uint32_t* evals,
                     uint8_t* has, int* status, hipStream_t s);
 void LaunchSummary(const float* fitness, const uint8_t* has, const uint32_t* slots, uint32_t n, SummaryRecord* out,

@@ -116,6 +116,11 @@ struct ne_runtime {
     bool have_summary = false;
     int last_status = NE_OK;
     bool cur_evaluated = false;  // the current generation's fitness is that of its own evaluation
+    uint32_t* d_work = nullptr;         // persistent evaluation's work list
+    uint64_t work_cap = 0;
+    uint32_t* d_work_counter = nullptr;
+    uint64_t cu_count = 256;
+    uint32_t tuned_waves = 0;           // 0: the idle-slot heuristic picks the workgroup size
 
     // surrogate screening
     bool screen_on = false;
@@ -436,21 +441,53 @@ static int EvaluateCore(ne_runtime* rt, uint64_t action_space, const uint32_t* s
     // Waves per workgroup: 6-wave workgroups of this kernel's register footprint fit once per CU, 2-4
     // wave ones fill a CU (12 waves); pick the size that leaves the fewest idle wave slots over all
     // genomes (ties: the larger).  A genome's tiles then spread over ceil(max_tiles / waves) workgroups.
+    std::vector<uint32_t> tiles(n);
+    for (uint64_t c = 0; c < n; ++c) {
+        const uint64_t W = offs[c + 1] - offs[c];
+        tiles[c] = uint32_t((W + 15) / 16 + (2 * W + 15) / 16);
+    }
     uint32_t waves = 1;
     uint64_t best_cost = ~0ull;
     for (uint32_t w = 4; w >= 1; --w) {
         uint64_t cost = 0;
-        for (uint64_t c = 0; c < n; ++c) {
-            const uint64_t W = offs[c + 1] - offs[c];
-            const uint64_t t = (W + 15) / 16 + (2 * W + 15) / 16;
-            cost += (t + w - 1) / w * w;
-        }
+        for (uint64_t c = 0; c < n; ++c) cost += (tiles[c] + w - 1) / w * w;
         if (cost < best_cost) {
             best_cost = cost;
             waves = w;
         }
     }
-    const uint32_t chunks = uint32_t((max_tiles + waves - 1) / waves);
+    if (rt->tuned_waves != 0) waves = rt->tuned_waves;  // the autotuner's choice (ne_autotune_evaluation)
+    // Work list for the persistent kernel: split-major (every genome's first `waves` tiles, then the next
+    // ...), genomes by descending tile count within a split, so no workgroup slot is spent on an empty
+    // split and the longest genomes start first.
+    const uint32_t splits = uint32_t((max_tiles + waves - 1) / waves);
+    std::vector<uint32_t> order(n);
+    for (uint64_t c = 0; c < n; ++c) order[c] = uint32_t(c);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return tiles[x] > tiles[y]; });
+    std::vector<uint32_t> work;
+    for (uint32_t sp = 0; sp < splits; ++sp)
+        for (const uint32_t g : order) {
+            if (uint64_t(sp) * waves >= tiles[g]) break;
+            work.push_back(g);
+            work.push_back(sp);
+        }
+    const uint64_t items = work.size() / 2;
+    if (items > rt->work_cap) {
+        Free(rt->d_work);
+        rt->work_cap = 0;
+        if (!Alloc(&rt->d_work, 2 * items)) return Fail(rt, NE_DEVICE_FAILURE);
+        rt->work_cap = items;
+    }
+    if (!Ok(hipMemcpyAsync(rt->d_work, work.data(), work.size() * 4, hipMemcpyHostToDevice, rt->stream)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    LaunchFillU32(rt->d_work_counter, 0u, 1, rt->stream);
+    a.work = rt->d_work;
+    a.work_count = uint32_t(items);
+    a.work_counter = rt->d_work_counter;
+    a.splits = splits;
+    // resident workgroups: 3 waves per SIMD (168 VGPRs) and at most 4 workgroups per CU (LDS)
+    const uint64_t per_cu = std::min<uint64_t>(12 / waves, 4);
+    const uint32_t blocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(items, rt->cu_count * per_cu)));
     if (rt->enc_t == nullptr && !Alloc(&rt->enc_t, rt->max_gen * kEncTStride))
         return Fail(rt, NE_DEVICE_FAILURE);
     a.enc_t = rt->enc_t;
@@ -476,7 +513,7 @@ static int EvaluateCore(ne_runtime* rt, uint64_t action_space, const uint32_t* s
                        a.action_count, rt->bf16_ops, a.bf16_stride, rt->tail_absmax, rt->stream);
     EventEnd(rt, 8);
     EventBegin(rt, 0);
-    LaunchEvaluate(a, uint32_t(n), waves, chunks, trace, bf16, rt->stream);
+    LaunchEvaluate(a, blocks, waves, trace, bf16, rt->stream);
     EventEnd(rt, 0);
     if (!Ok(hipGetLastError())) return Fail(rt, NE_DEVICE_FAILURE);
     int st = Finish(rt);
@@ -1078,6 +1115,11 @@ int ne_create(const ne_runtime_config* config, ne_runtime** out_runtime) {
     if (!rt) return NE_DEVICE_FAILURE;
     rt->config = c;
     (void)hipGetDevice(&rt->device);
+    {
+        int cus = 0;
+        if (Ok(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, rt->device)) && cus > 0)
+            rt->cu_count = uint64_t(cus);
+    }
     if (c.stream) {
         rt->stream = reinterpret_cast<hipStream_t>(c.stream);
     } else {
@@ -1102,7 +1144,8 @@ int ne_create(const ne_runtime_config* config, ne_runtime** out_runtime) {
     ok &= Alloc(&rt->pairs, 2 * max_gen) && Alloc(&rt->refcounts, max_gen) && Alloc(&rt->xover, max_gen) &&
           Alloc(&rt->breaks, max_gen * kBreakStride);
     ok &= Alloc(&rt->raw, max_gen) && Alloc(&rt->local_w, max_gen) && Alloc(&rt->counters, 4) &&
-          Alloc(&rt->status, 1) && Alloc(&rt->summary, 1) && Alloc(&rt->d_offsets, max_gen + 1);
+          Alloc(&rt->status, 1) && Alloc(&rt->summary, 1) && Alloc(&rt->d_offsets, max_gen + 1) &&
+          Alloc(&rt->d_work_counter, 1);
     for (auto& e : rt->ev) ok &= Ok(hipEventCreate(&e.begin)) && Ok(hipEventCreate(&e.end));
     if (ok) {
         ok &= Ok(hipMemsetAsync(rt->slab, 0, slots * rt->stride, rt->stream));
@@ -1148,6 +1191,8 @@ void ne_destroy(ne_runtime* rt) {
     Free(rt->local_w);
     Free(rt->counters);
     Free(rt->status);
+    Free(rt->d_work_counter);
+    Free(rt->d_work);
     Free(rt->summary);
     Free(rt->d_offsets);
     Free(rt->d_secrets);
