@@ -260,6 +260,16 @@ int ne_set_trace(ne_runtime *rt, uint32_t flags);
 /* Selects the evaluation precision for later evaluate/advance calls (no reference counterpart: the
  * reference evaluates in its storage precision only, evaluation_ops.cuh:244-508). */
 int ne_set_eval_precision(ne_runtime *rt, uint32_t mode);
+/* Autotuner of the evaluation's workgroup shape (no reference counterpart): evaluates the current
+ * generation once per shape -- 1, 2, 3 and 4 waves, i.e. 16 to 64 games per workgroup of the persistent
+ * evaluation kernel -- times each with HIP events, and keeps the fastest for later evaluations of this
+ * handle.  Results do not depend on the shape, so the generation is left evaluated as by ne_evaluate.
+ * timings_ms (optional) receives the 4 measured times. */
+int ne_autotune_evaluation(ne_runtime *rt, const ne_word_counts *counts, const ne_shard_release_history *history,
+                           uint32_t *chosen_waves, float *timings_ms);
+/* Sets the evaluation's waves per workgroup (1..4), or 0 for the built-in choice (the size leaving the
+ * fewest idle wave slots). */
+int ne_set_eval_shape(ne_runtime *rt, uint32_t waves);
 /* Copies a recorded trace; game traces are ordered by individual, then game slot (W fresh games by
  * entry, then 2W prefilled games: slot W+q is entry q/2, variant 1+q%2). */
 int ne_download_trace(ne_runtime *rt, uint32_t kind, void *dst, uint64_t capacity_bytes, uint64_t *written);
@@ -282,8 +292,12 @@ typedef struct ne_screen_config {
     uint32_t batch_size;         /* minibatch size, 1..1024 (256) */
     uint32_t min_replay;         /* pairs seen before screening may start (4,096) */
     uint32_t seed;               /* probe states, surrogate initialisation, audit and minibatch draws */
-    uint32_t reserved0;
+    uint32_t flags;              /* NE_SCREEN_*; 0: a generation's training steps replay as one hipGraph */
 } ne_screen_config;
+
+enum {
+    NE_SCREEN_EAGER_TRAINING = 1   /* launch the training steps one by one instead of replaying a graph */
+};
 
 typedef struct ne_screen_report {
     uint32_t screened;           /* 1: this step screened a brood; 0: plain step (warm-up or fallback) */

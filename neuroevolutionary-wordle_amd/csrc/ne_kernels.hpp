@@ -207,16 +207,16 @@ void LaunchSurrogateForward(const float* params, const float* x, uint32_t n, flo
 // Loss 1/(2n) sum (y - t)^2: dy [n], dh2 [n][64], dh1 [n][128].
 void LaunchSurrogateBackprop(const float* params, const float* h1, const float* h2, const float* y, const float* t,
                              uint32_t n, float* dy, float* dh1, float* dh2, hipStream_t s);
-// Gradient of every parameter over the batch, then one Adam step (beta 0.9 / 0.999, eps 1e-8);
-// c1 = 1 / (1 - 0.9^step), c2 = 1 / (1 - 0.999^step).
+// Gradient of every parameter over the batch, then one Adam step (beta 0.9 / 0.999, eps 1e-8); coef (device)
+// = {1 / (1 - 0.9^step), 1 / (1 - 0.999^step)}, so a captured graph replays with new steps.
 void LaunchSurrogateAdam(float* params, float* adam_m, float* adam_v, const float* x, const float* h1, const float* h2,
-                         const float* dy, const float* dh1, const float* dh2, uint32_t n, float lr, float c1, float c2,
+                         const float* dy, const float* dh1, const float* dh2, uint32_t n, float lr, const float* coef,
                          hipStream_t s);
 // Replay buffer: rows (head + e) % cap <- fingerprint rows src_row[e] of fp, fitness[e].
 void LaunchReplayAppend(const float* fp, const uint32_t* src_row, const float* fitness, uint32_t count, float* rx,
                         float* rf, uint32_t head, uint32_t cap, hipStream_t s);
-// Minibatch: x[b] = rx[idx[b]], t[b] = (rf[idx[b]] - mu) * inv_sigma.
-void LaunchReplayGather(const float* rx, const float* rf, const uint32_t* idx, uint32_t n, float mu, float inv_sigma,
+// Minibatch: x[b] = rx[idx[b]], t[b] = (rf[idx[b]] - stats[0]) * stats[1] (stats on the device).
+void LaunchReplayGather(const float* rx, const float* rf, const uint32_t* idx, uint32_t n, const float* stats,
                         float* x, float* t, hipStream_t s);
 void LaunchInject(const InjectArgs& a, hipStream_t s);
 void LaunchPackBf16(const uint8_t* slab, uint64_t stride, const uint32_t* slots, uint32_t n, uint32_t slot_count,

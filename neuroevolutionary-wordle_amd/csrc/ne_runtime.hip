@@ -147,6 +147,10 @@ struct ne_runtime {
     float *bx = nullptr, *bt = nullptr, *by = nullptr, *bh1 = nullptr, *bh2 = nullptr;
     float *bdy = nullptr, *bdh1 = nullptr, *bdh2 = nullptr;
     uint32_t* bidx = nullptr;
+    float* sg_coef = nullptr;   // [0..1] target standardisation, then per training step {c1, c2}
+    float* sg_coef1 = nullptr;  // {c1, c2} of a direct ne_surrogate_train_step
+    hipGraph_t train_graph = nullptr;       // one generation's training steps, captured once
+    hipGraphExec_t train_exec = nullptr;
 };
 
 namespace {
@@ -925,15 +929,30 @@ static bool DrawProbes(ne_runtime* rt, uint64_t words, uint32_t seed) {
     return true;
 }
 
-// One Adam step of the surrogate on a device minibatch (x [n][192], t [n]) in the batch buffers' room.
-static void SurrogateStep(ne_runtime* rt, const float* x, const float* t, uint32_t n) {
-    ++rt->sg_step;
-    const float c1 = float(1.0 / (1.0 - std::pow(0.9, double(rt->sg_step))));
-    const float c2 = float(1.0 / (1.0 - std::pow(0.999, double(rt->sg_step))));
+// Adam's bias corrections of step t (counted from 1).
+static void AdamCoef(uint64_t t, float* c) {
+    c[0] = float(1.0 / (1.0 - std::pow(0.9, double(t))));
+    c[1] = float(1.0 / (1.0 - std::pow(0.999, double(t))));
+}
+
+// One Adam step of the surrogate on a device minibatch (x [n][192], t [n]); coef (device) holds its bias
+// corrections.  Launches only, so it can be captured.
+static void SurrogateStep(ne_runtime* rt, const float* x, const float* t, uint32_t n, const float* coef) {
     LaunchSurrogateForward(rt->sg_params, x, n, rt->by, rt->bh1, rt->bh2, rt->stream);
     LaunchSurrogateBackprop(rt->sg_params, rt->bh1, rt->bh2, rt->by, t, n, rt->bdy, rt->bdh1, rt->bdh2, rt->stream);
     LaunchSurrogateAdam(rt->sg_params, rt->sg_m, rt->sg_v, x, rt->bh1, rt->bh2, rt->bdy, rt->bdh1, rt->bdh2, n,
-                        rt->screen.learning_rate, c1, c2, rt->stream);
+                        rt->screen.learning_rate, coef, rt->stream);
+}
+
+// The launches of one generation's training (steps x {gather, forward, backprop, Adam}), reading their
+// indices, standardisation and bias corrections from device buffers filled before each run.
+static void TrainingLaunches(ne_runtime* rt) {
+    const uint32_t bs = rt->screen.batch_size;
+    for (uint32_t st = 0; st < rt->screen.train_steps; ++st) {
+        LaunchReplayGather(rt->replay_x, rt->replay_f, rt->bidx + uint64_t(st) * bs, bs, rt->sg_coef, rt->bx, rt->bt,
+                           rt->stream);
+        SurrogateStep(rt, rt->bx, rt->bt, bs, rt->sg_coef + 2 + 2 * uint64_t(st));
+    }
 }
 
 // Mean and standard deviation of the replay buffer's fitness (the training targets' standardisation).
@@ -965,13 +984,27 @@ static int TrainFromReplay(ne_runtime* rt, float* loss) {
             idx[uint64_t(st) * bs + b] = Philox4x32(uint32_t(s_abs), b, uint32_t(s_abs >> 32), 0, rt->screen.seed,
                                                     kTagBatch).x % uint32_t(rt->replay_count);
     }
-    if (!Ok(hipMemcpyAsync(rt->bidx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, rt->stream)))
+    std::vector<float> coef(2 + 2 * uint64_t(steps));
+    coef[0] = mu;
+    coef[1] = 1.0f / sigma;
+    for (uint32_t st = 0; st < steps; ++st) AdamCoef(rt->sg_step + 1 + st, &coef[2 + 2 * st]);
+    if (!Ok(hipMemcpyAsync(rt->bidx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, rt->stream)) ||
+        !Ok(hipMemcpyAsync(rt->sg_coef, coef.data(), coef.size() * 4, hipMemcpyHostToDevice, rt->stream)))
         return Fail(rt, NE_DEVICE_FAILURE);
-    for (uint32_t st = 0; st < steps; ++st) {
-        LaunchReplayGather(rt->replay_x, rt->replay_f, rt->bidx + uint64_t(st) * bs, bs, mu, 1.0f / sigma, rt->bx,
-                           rt->bt, rt->stream);
-        SurrogateStep(rt, rt->bx, rt->bt, bs);
+    if (rt->screen.flags & NE_SCREEN_EAGER_TRAINING) {
+        TrainingLaunches(rt);
+    } else {
+        // the steps as one hipGraph: captured on first use, replayed every generation (kernels only)
+        if (!rt->train_exec) {
+            if (!Ok(hipStreamBeginCapture(rt->stream, hipStreamCaptureModeThreadLocal))) return Fail(rt, NE_DEVICE_FAILURE);
+            TrainingLaunches(rt);
+            if (!Ok(hipStreamEndCapture(rt->stream, &rt->train_graph)) ||
+                !Ok(hipGraphInstantiate(&rt->train_exec, rt->train_graph, nullptr, nullptr, 0)))
+                return Fail(rt, NE_DEVICE_FAILURE);
+        }
+        if (!Ok(hipGraphLaunch(rt->train_exec, rt->stream))) return Fail(rt, NE_DEVICE_FAILURE);
     }
+    rt->sg_step += steps;
     std::vector<float> y(bs), t(bs);
     if (!Ok(hipMemcpyAsync(y.data(), rt->by, bs * 4, hipMemcpyDeviceToHost, rt->stream)) ||
         !Ok(hipMemcpyAsync(t.data(), rt->bt, bs * 4, hipMemcpyDeviceToHost, rt->stream)) ||
@@ -1223,6 +1256,10 @@ void ne_destroy(ne_runtime* rt) {
     Free(rt->bdh1);
     Free(rt->bdh2);
     Free(rt->bidx);
+    Free(rt->sg_coef);
+    Free(rt->sg_coef1);
+    if (rt->train_exec) (void)hipGraphExecDestroy(rt->train_exec);
+    if (rt->train_graph) (void)hipGraphDestroy(rt->train_graph);
     Free(rt->owner);
     Free(rt->move_src);
     Free(rt->d_survivors);
@@ -1601,6 +1638,38 @@ int ne_download_trace(ne_runtime* rt, uint32_t kind, void* dst, uint64_t capacit
 }
 
 
+// ---- evaluation shape ---------------------------------------------------------------------------
+int ne_set_eval_shape(ne_runtime* rt, uint32_t waves) {
+    if (!rt || waves > 4) return Fail(rt, NE_INVALID_ARGUMENT);
+    rt->tuned_waves = waves;
+    return NE_OK;
+}
+
+int ne_autotune_evaluation(ne_runtime* rt, const ne_word_counts* counts, const ne_shard_release_history* history,
+                           uint32_t* chosen_waves, float* timings_ms) {
+    if (!rt || !counts) return Fail(rt, NE_INVALID_ARGUMENT);
+    uint32_t best = 0;
+    float best_ms = 0.0f;
+    for (uint32_t w = 1; w <= 4; ++w) {
+        rt->tuned_waves = w;
+        rt->times = ne_stage_times{};
+        const int st = EvaluateFull(rt, counts, history);
+        if (st != NE_OK) {
+            rt->tuned_waves = 0;
+            return st;
+        }
+        const float ms = rt->times.evaluate_ms;
+        if (timings_ms) timings_ms[w - 1] = ms;
+        if (best == 0 || ms < best_ms) {
+            best = w;
+            best_ms = ms;
+        }
+    }
+    rt->tuned_waves = best;
+    if (chosen_waves) *chosen_waves = best;
+    return NE_OK;
+}
+
 // ---- surrogate screening ----------------------------------------------------------------------
 int ne_enable_screening(ne_runtime* rt, const ne_screen_config* config) {
     if (!rt || !config) return Fail(rt, NE_INVALID_ARGUMENT);
@@ -1632,7 +1701,13 @@ int ne_enable_screening(ne_runtime* rt, const ne_screen_config* config) {
     }
     if (rt->enc_t == nullptr && !Alloc(&rt->enc_t, rt->max_gen * kEncTStride)) return Fail(rt, NE_DEVICE_FAILURE);
     Free(rt->bidx);
-    if (!Alloc(&rt->bidx, std::max<uint64_t>(uint64_t(c.train_steps) * c.batch_size, 1)))
+    Free(rt->sg_coef);
+    if (rt->train_exec) (void)hipGraphExecDestroy(rt->train_exec);
+    if (rt->train_graph) (void)hipGraphDestroy(rt->train_graph);
+    rt->train_exec = nullptr;
+    rt->train_graph = nullptr;
+    if (!Alloc(&rt->bidx, std::max<uint64_t>(uint64_t(c.train_steps) * c.batch_size, 1)) ||
+        !Alloc(&rt->sg_coef, 2 + 2 * uint64_t(c.train_steps)) || (!rt->sg_coef1 && !Alloc(&rt->sg_coef1, 2)))
         return Fail(rt, NE_DEVICE_FAILURE);
     rt->screen = c;
     rt->screen_on = true;
@@ -1916,10 +1991,13 @@ int ne_surrogate_train_step(ne_runtime* rt, const float* x, const float* t, uint
     if (!rt || !x || !t || count == 0) return Fail(rt, NE_INVALID_ARGUMENT);
     if (!rt->screen_on) return Fail(rt, NE_UNSUPPORTED);
     if (count > std::max<uint64_t>(rt->screen.batch_size, 1024)) return Fail(rt, NE_INVALID_ARGUMENT);
+    float coef[2];
+    AdamCoef(++rt->sg_step, coef);
     if (!Ok(hipMemcpyAsync(rt->bx, x, count * kSgIn * 4, hipMemcpyHostToDevice, rt->stream)) ||
-        !Ok(hipMemcpyAsync(rt->bt, t, count * 4, hipMemcpyHostToDevice, rt->stream)))
+        !Ok(hipMemcpyAsync(rt->bt, t, count * 4, hipMemcpyHostToDevice, rt->stream)) ||
+        !Ok(hipMemcpyAsync(rt->sg_coef1, coef, sizeof(coef), hipMemcpyHostToDevice, rt->stream)))
         return Fail(rt, NE_DEVICE_FAILURE);
-    SurrogateStep(rt, rt->bx, rt->bt, uint32_t(count));
+    SurrogateStep(rt, rt->bx, rt->bt, uint32_t(count), rt->sg_coef1);
     return Finish(rt);
 }
 

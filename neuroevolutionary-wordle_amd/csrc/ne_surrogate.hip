@@ -117,9 +117,10 @@ __global__ __launch_bounds__(128) void SurrogateBackpropKernel(const float* th, 
 __global__ __launch_bounds__(256) void SurrogateAdamKernel(float* th, float* am, float* av, const float* x,
                                                            const float* h1, const float* h2, const float* dy,
                                                            const float* dh1, const float* dh2, uint32_t n, float lr,
-                                                           float c1, float c2) {
+                                                           const float* coef) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= kSgParams) return;
+    const float c1 = coef[0], c2 = coef[1];
     float g = 0.0f;
     if (p < kSgOffB1) {
         const uint32_t i = p / kSgH1, h = p - i * kSgH1;
@@ -162,13 +163,13 @@ __global__ void ReplayAppendKernel(const float* fp, const uint32_t* src_row, con
     if (threadIdx.x == 0) rf[dst] = fitness[e];
 }
 
-__global__ void ReplayGatherKernel(const float* rx, const float* rf, const uint32_t* idx, uint32_t n, float mu,
-                                   float inv_sigma, float* x, float* t) {
+__global__ void ReplayGatherKernel(const float* rx, const float* rf, const uint32_t* idx, uint32_t n,
+                                   const float* stats, float* x, float* t) {
     const uint32_t b = blockIdx.x;
     if (b >= n) return;
     const uint32_t r = idx[b];
     for (uint32_t c = threadIdx.x; c < kSgIn; c += blockDim.x) x[uint64_t(b) * kSgIn + c] = rx[uint64_t(r) * kSgIn + c];
-    if (threadIdx.x == 0) t[b] = (rf[r] - mu) * inv_sigma;
+    if (threadIdx.x == 0) t[b] = (rf[r] - stats[0]) * stats[1];
 }
 
 void LaunchSurrogateInit(float* params, float* adam_m, float* adam_v, uint32_t seed, hipStream_t s) {
@@ -190,10 +191,10 @@ void LaunchSurrogateBackprop(const float* params, const float* h1, const float* 
 }
 
 void LaunchSurrogateAdam(float* params, float* adam_m, float* adam_v, const float* x, const float* h1, const float* h2,
-                         const float* dy, const float* dh1, const float* dh2, uint32_t n, float lr, float c1, float c2,
+                         const float* dy, const float* dh1, const float* dh2, uint32_t n, float lr, const float* coef,
                          hipStream_t s) {
     hipLaunchKernelGGL(SurrogateAdamKernel, dim3((kSgParams + 255) / 256), dim3(256), 0, s, params, adam_m, adam_v,
-                       x, h1, h2, dy, dh1, dh2, n, lr, c1, c2);
+                       x, h1, h2, dy, dh1, dh2, n, lr, coef);
 }
 
 void LaunchReplayAppend(const float* fp, const uint32_t* src_row, const float* fitness, uint32_t count, float* rx,
@@ -202,10 +203,10 @@ void LaunchReplayAppend(const float* fp, const uint32_t* src_row, const float* f
     hipLaunchKernelGGL(ReplayAppendKernel, dim3(count), dim3(64), 0, s, fp, src_row, fitness, count, rx, rf, head, cap);
 }
 
-void LaunchReplayGather(const float* rx, const float* rf, const uint32_t* idx, uint32_t n, float mu, float inv_sigma,
+void LaunchReplayGather(const float* rx, const float* rf, const uint32_t* idx, uint32_t n, const float* stats,
                         float* x, float* t, hipStream_t s) {
     if (n == 0) return;
-    hipLaunchKernelGGL(ReplayGatherKernel, dim3(n), dim3(64), 0, s, rx, rf, idx, n, mu, inv_sigma, x, t);
+    hipLaunchKernelGGL(ReplayGatherKernel, dim3(n), dim3(64), 0, s, rx, rf, idx, n, stats, x, t);
 }
 
 }  // namespace ne

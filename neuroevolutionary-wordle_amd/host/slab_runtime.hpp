@@ -317,6 +317,7 @@ struct SurrogateScreeningConfig {
     std::uint32_t batch_size = 256;
     std::uint32_t min_replay = 4096;
     std::uint32_t seed = 0x5C2EE7u;
+    bool eager_training = false;  // true: launch the training steps one by one instead of a hipGraph replay
 };
 
 inline bool TryEnableSurrogateScreening(DeviceSlabGARuntimeBuffers& buffers, const SurrogateScreeningConfig& config = {}) {
@@ -330,6 +331,7 @@ inline bool TryEnableSurrogateScreening(DeviceSlabGARuntimeBuffers& buffers, con
     c.batch_size = config.batch_size;
     c.min_replay = config.min_replay;
     c.seed = config.seed;
+    c.flags = config.eager_training ? NE_SCREEN_EAGER_TRAINING : 0u;
     return ne_enable_screening(buffers.handle, &c) == NE_OK;
 }
 

@@ -103,7 +103,7 @@ class ScreenConfig(C.Structure):
     _fields_ = [("brood_factor", C.c_uint32), ("replay_capacity", C.c_uint32), ("audit_fraction", C.c_float),
                 ("spearman_threshold", C.c_float), ("learning_rate", C.c_float), ("train_steps", C.c_uint32),
                 ("batch_size", C.c_uint32), ("min_replay", C.c_uint32), ("seed", C.c_uint32),
-                ("reserved0", C.c_uint32)]
+                ("flags", C.c_uint32)]
 
 
 class ScreenReport(C.Structure):
@@ -115,6 +115,7 @@ class ScreenReport(C.Structure):
 
 
 SURROGATE_PARAMS = 33025
+SCREEN_EAGER_TRAINING = 1
 FINGERPRINT_DIMS = 192
 
 
@@ -122,7 +123,7 @@ def default_screen_config(**kw):
     """The surrogate-screened configuration: brood 4x, top 25% (each cell's best-predicted child) plus a
     5% random audit, Adam lr 1e-3 on a replay buffer of 65,536 pairs, fallback below Spearman 0.5."""
     c = ScreenConfig(brood_factor=4, replay_capacity=65536, audit_fraction=0.05, spearman_threshold=0.5,
-                     learning_rate=1e-3, train_steps=32, batch_size=256, min_replay=4096, seed=0x5C2EE7, reserved0=0)
+                     learning_rate=1e-3, train_steps=32, batch_size=256, min_replay=4096, seed=0x5C2EE7, flags=0)
     for k, v in kw.items():
         setattr(c, k, v)
     return c
@@ -140,7 +141,8 @@ EXPORTED_SYMBOLS = [
     "ne_read_status", "ne_status_string", "ne_get_stage_times", "ne_set_trace", "ne_download_trace",
     "ne_build_shard_csr", "ne_set_eval_precision", "ne_enable_screening", "ne_advance_generation_screened",
     "ne_fingerprint", "ne_get_probe_states", "ne_surrogate_predict", "ne_surrogate_train_step",
-    "ne_surrogate_get_params", "ne_surrogate_set_params", "ne_rank_correlation",
+    "ne_surrogate_get_params", "ne_surrogate_set_params", "ne_rank_correlation", "ne_autotune_evaluation",
+    "ne_set_eval_shape",
 ]
 
 _lib = None
@@ -191,6 +193,9 @@ def load_library(path=LIB_PATH):
         "ne_surrogate_get_params": (i32, [p, p, u64, C.POINTER(u64)]),
         "ne_surrogate_set_params": (i32, [p, p, u64]),
         "ne_rank_correlation": (C.c_double, [p, p, u64]),
+        "ne_autotune_evaluation": (i32, [p, C.POINTER(WordCounts), C.POINTER(ShardReleaseHistory), C.POINTER(u32),
+                                         p]),
+        "ne_set_eval_shape": (i32, [p, u32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -380,6 +385,19 @@ class SlabRuntime:
 
     def status(self):
         return self.lib.ne_read_status(self.h)
+
+    # --- evaluation shape
+    def autotune_evaluation(self, counts):
+        """Times the current generation's evaluation at 1..4 waves per workgroup and keeps the fastest;
+        returns (waves, [ms for 1..4 waves])."""
+        w = C.c_uint32()
+        t = np.zeros(4, np.float32)
+        self._check("ne_autotune_evaluation",
+                    self.lib.ne_autotune_evaluation(self.h, C.byref(counts), None, C.byref(w), _ptr(t)))
+        return w.value, t.tolist()
+
+    def set_eval_shape(self, waves):
+        self._check("ne_set_eval_shape", self.lib.ne_set_eval_shape(self.h, waves))
 
     # --- surrogate screening
     def enable_screening(self, config=None):

@@ -218,3 +218,17 @@ def test_two_phase_brood_children_equal_one_pass_breeding(pkg, orc):
         picked.append(match[0])
     assert len(set(picked)) > 1, "the surrogate picks different brood members in different cells"
     rt.close()
+
+
+def test_graph_replayed_training_equals_eager_launches(pkg):
+    """A generation's training steps replay as one captured hipGraph; over several generations (eager work
+    and synchronisations between the replays) every replay must leave what eager launches leave."""
+    wl = pkg.synth.workload("grid16_a512")
+    kw = dict(min_replay=256, spearman_threshold=-1.0)
+    rt_g, reps_g, fits_g = _screened_run(pkg, wl, 5, kw)
+    rt_e, reps_e, fits_e = _screened_run(pkg, wl, 5, dict(kw, flags=pkg.runtime.SCREEN_EAGER_TRAINING))
+    assert reps_g == reps_e
+    assert all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(fits_g, fits_e))
+    assert np.array_equal(rt_g.surrogate_params().view(np.uint32), rt_e.surrogate_params().view(np.uint32))
+    rt_g.close()
+    rt_e.close()
