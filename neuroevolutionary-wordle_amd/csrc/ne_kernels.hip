@@ -413,6 +413,9 @@ __global__ __launch_bounds__(320) void PackBf16Kernel(const uint8_t* slab, uint6
                                                       uint32_t a_tiles, uint8_t* out, uint64_t out_stride,
                                                       uint32_t* tail_absmax) {
     __shared__ float s_mx[kPackTiles][5][8];
+    // the kPackTiles tiles' tail rows (16 x 38 contiguous floats each, 16-byte aligned), staged with
+    // coalesced 128-bit loads
+    __shared__ __align__(16) float s_rows[kPackTiles * 16 * kTail];
     const uint32_t n = blockIdx.y;
     const uint32_t slot = slot_indices[n];
     if (slot >= slot_count) return;  // reported by the evaluation kernel
@@ -420,9 +423,27 @@ __global__ __launch_bounds__(320) void PackBf16Kernel(const uint8_t* slab, uint6
     const uint32_t sub = threadIdx.x / 80, r = threadIdx.x % 80;
     const uint32_t c = r / 16, i = r % 16;
     const int m = c == 0 ? 0 : 1, g = c == 0 ? 3 : int(c) - 1;
+    const uint32_t n_tail = A * kTail;  // valid tail floats
     float mx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t at = blockIdx.x * kPackTiles + sub; at < a_tiles; at += gridDim.x * kPackTiles) {
+    for (uint32_t at0 = blockIdx.x * kPackTiles; at0 < a_tiles; at0 += gridDim.x * kPackTiles) {
+        __syncthreads();  // the previous tiles are done with s_rows
+        const uint32_t f0 = at0 * 16 * kTail;  // first float of the staged rows
+        for (uint32_t q = threadIdx.x; q < kPackTiles * 16 * kTail / 4; q += blockDim.x) {
+            const uint32_t f = f0 + 4 * q;
+            v4f v = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (f + 4 <= n_tail) {
+                v = *reinterpret_cast<const v4f*>(tails + f);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) v[k] = f + k < n_tail ? tails[f + k] : 0.0f;
+            }
+            *reinterpret_cast<v4f*>(s_rows + 4 * q) = v;
+        }
+        __syncthreads();
+        const uint32_t at = at0 + sub;
+        if (at >= a_tiles) continue;
         const uint32_t a = at * 16 + i;
+        const float* row = s_rows + (sub * 16 + i) * kTail;
         bf16x8 v;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -434,7 +455,7 @@ __global__ __launch_bounds__(320) void PackBf16Kernel(const uint8_t* slab, uint6
                     x = __builtin_bit_cast(float, img[uint64_t(at) * kImgTileWords + (st / 4) * 256 +
                                                       4 * (i + 16 * uint32_t(gg)) + (st % 4)]);
                 } else {
-                    x = tails[uint64_t(a) * kTail + uint32_t(d - kFixed)];
+                    x = row[d - kFixed];
                     mx[e] = fmaxf(mx[e], fabsf(x));
                 }
             }
