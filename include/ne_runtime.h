@@ -39,7 +39,9 @@ typedef enum ne_status {
     NE_SLAB_REPACK_FAILED = 16,
     NE_INVALID_ARGUMENT = 17,         /* null pointer, short buffer, size out of range */
     NE_UNSUPPORTED = 18,              /* a configuration this build does not implement */
-    NE_CALLBACK_FAILED = 19           /* a caller callback returned non-zero */
+    NE_CALLBACK_FAILED = 19,          /* a caller callback returned non-zero */
+    NE_CHECKPOINT_INVALID = 20,       /* a checkpoint failed validation (magic, versions, checksum, content) */
+    NE_IO_FAILURE = 21                /* a checkpoint file could not be opened, read or written */
 } ne_status;
 
 typedef struct ne_runtime ne_runtime;
@@ -73,10 +75,12 @@ typedef struct ne_word_counts {
     uint64_t shard_radius_growth_period_generations;
 } ne_word_counts;
 
-/* TrainingDataShardReleaseHistory (training_data.hpp:47-51): release generation per shard. */
+/* TrainingDataShardReleaseHistory (training_data.hpp:49-53): release generation per shard and the
+ * fitness p99 at its release (kept in checkpoints only; NULL reads as zeros). */
 typedef struct ne_shard_release_history {
     const uint64_t *release_generations;
     uint64_t release_count;
+    const float *release_fitness_p99;
 } ne_shard_release_history;
 
 /* ParentSelectionConfig / BreedingConfig / MutationConfig
@@ -343,6 +347,79 @@ int ne_surrogate_get_params(ne_runtime *rt, float *dst, uint64_t capacity, uint6
 int ne_surrogate_set_params(ne_runtime *rt, const float *src, uint64_t count);
 /* Host only: Spearman rank correlation (average ranks for ties); NaN when n < 3 or a side is constant. */
 double ne_rank_correlation(const float *a, const float *b, uint64_t n);
+
+/* ---- checkpoint and resume ---------------------------------------------------------------
+ * A checkpoint is the logical pre-recombination snapshot of RuntimeCheckpoint (slab_runtime.hpp:86-117):
+ * the current generation (slots, fitness, evaluation counts), the parent plan of the next one, every
+ * live genome's slot bytes, and the configuration and seed that finish the step.  Files are
+ * byte-compatible with TryWriteRuntimeCheckpointAtomically / TryReadRuntimeCheckpoint
+ * (slab_runtime.cu:1934-2081): little-endian, schema 3, genome layout 1, FNV-1a 64 checksum. */
+typedef struct ne_checkpoint ne_checkpoint;
+
+typedef struct ne_checkpoint_info {
+    uint32_t schema_version;
+    uint32_t genome_layout_version;
+    uint32_t resume_phase;               /* 1: pre-recombination, pre-mutation */
+    uint32_t generation_seed;            /* seed of the step the checkpoint interrupts */
+    uint64_t checksum;
+    uint64_t training_data_identity_hash;
+    uint64_t generation_index;           /* of the current (parent) generation */
+    uint64_t population;                 /* current generation size */
+    uint64_t child_count;                /* children planned for the next generation */
+    uint64_t live_genotypes;             /* genome records */
+    uint64_t payload_bytes;              /* CheckpointPayloadByteCount (run_genetic_algorithm.cu:1360-1372) */
+    uint64_t action_count;               /* slab layout after any widening of this step */
+    uint64_t slot_stride_bytes;
+    uint64_t slot_count;
+    uint64_t slab_bytes;
+    uint64_t generation_byte_budget;
+    uint64_t grid_column_count;
+    uint64_t release_count;              /* shard release history length */
+    ne_word_counts word_counts;
+    ne_assembly_config assembly_config;
+    ne_pending_injection injection;
+} ne_checkpoint_info;
+
+/* TryCreatePrebreedingCheckpointOnDevice (slab_runtime.hpp:152-158; slab_runtime.cu:1707-1731): the first
+ * half of ne_advance_generation (evaluation, callbacks, plan, widening), then the snapshot.  The runtime
+ * is left at the boundary: ne_resume_generation_from_checkpoint finishes the step.  catalog_letters
+ * (5 letter indices per word) may be NULL; when given, the identity hash covers the catalog. */
+int ne_create_prebreeding_checkpoint(ne_runtime *rt, uint32_t generation_seed, const ne_word_counts *counts,
+                                     const ne_assembly_config *config, const ne_pending_injection *injection,
+                                     const uint8_t *catalog_letters, uint64_t catalog_word_count, uint32_t verbose,
+                                     const ne_callbacks *callbacks, const ne_shard_release_history *history,
+                                     ne_checkpoint **out_checkpoint);
+/* TryRestorePrebreedingCheckpointToDevice (slab_runtime.hpp:160; slab_runtime.cu:1733-1866): a new
+ * runtime holding the checkpoint's boundary state.  The k-th genome record takes slot k of a fresh slab
+ * of checkpoint slot_count slots.  stream: hipStream_t or NULL (own stream).  The caller uploads the
+ * catalog and secrets before resuming, as for a new runtime. */
+int ne_restore_prebreeding_checkpoint(const ne_checkpoint *checkpoint, void *stream, ne_runtime **out_runtime);
+/* TryResumeGenerationFromCheckpointOnDevice (slab_runtime.hpp:162-163; slab_runtime.cu:1868-1880): the
+ * second half of the step (assembly, injection, swap) with the checkpoint's seed and configuration.
+ * The runtime must stand at that checkpoint's boundary (same generation index and planned children). */
+int ne_resume_generation_from_checkpoint(ne_runtime *rt, const ne_checkpoint *checkpoint, uint32_t verbose);
+/* TryWriteRuntimeCheckpointAtomically (slab_runtime.hpp:165-166): writes <path>.tmp, then renames it
+ * over path; creates missing parent directories. */
+int ne_write_checkpoint(const ne_checkpoint *checkpoint, const char *path);
+/* TryReadRuntimeCheckpoint (slab_runtime.hpp:168): reads and validates (magic, versions, checksum). */
+int ne_read_checkpoint(const char *path, ne_checkpoint **out_checkpoint);
+/* TryCloneRuntimeCheckpoint (run_genetic_algorithm.cu:1374-1415): a deep copy, e.g. for a background
+ * write while the original resumes. */
+int ne_clone_checkpoint(const ne_checkpoint *checkpoint, ne_checkpoint **out_checkpoint);
+int ne_get_checkpoint_info(const ne_checkpoint *checkpoint, ne_checkpoint_info *out);
+/* Current generation arrays (each may be NULL; capacity entries) and the plan ([child][2] parent
+ * indices) of a checkpoint. */
+int ne_get_checkpoint_generation(const ne_checkpoint *checkpoint, uint32_t *slot_indices, float *fitness,
+                                 uint32_t *evaluation_counts, uint8_t *has_fitness, uint64_t capacity);
+int ne_get_checkpoint_plan(const ne_checkpoint *checkpoint, uint32_t *pairs, uint64_t capacity_pairs);
+/* Genome record `record`: its organism index and slot bytes (capacity >= slot_stride_bytes). */
+int ne_get_checkpoint_genome(const ne_checkpoint *checkpoint, uint64_t record, uint32_t *organism_index,
+                             void *dst, uint64_t capacity);
+void ne_free_checkpoint(ne_checkpoint *checkpoint);
+/* The training-data identity hash of TryDownloadBoundaryCheckpointPayload (slab_runtime.cu:1282-1296):
+ * FNV-1a 64 over the word counts, the release history and, when given, the catalog.  Host only. */
+uint64_t ne_training_data_identity_hash(const ne_word_counts *counts, const ne_shard_release_history *history,
+                                        const uint8_t *catalog_letters, uint64_t catalog_word_count);
 
 #ifdef __cplusplus
 }

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/ne_runtime.h"
+#include "ne_checkpoint.hpp"
 #include "ne_device.hpp"
 #include "ne_kernels.hpp"
 
@@ -45,6 +46,7 @@ struct ne_runtime {
 
     // generations
     uint64_t max_gen = 0;
+    uint64_t epi_rows = 0;  // rows of the epicenter grid the shard centres are drawn on
     uint64_t generation_byte_budget = 0;
     uint64_t cur_index = 0, cur_size = 0, next_index = 0, next_size = 0, planned_children = 0;
     uint32_t* cur_slots = nullptr;
@@ -304,8 +306,7 @@ int BuildShardCsrCore(const ne_word_counts* wc, const ne_shard_release_history* 
 
 int BuildShardCsr(ne_runtime* rt, const ne_word_counts* wc, const ne_shard_release_history* hist, uint64_t rows,
                   uint64_t cols, std::vector<uint32_t>& offsets, std::vector<uint32_t>& secrets) {
-    return BuildShardCsrCore(wc, hist, rt->cur_index, rows, cols, rt->max_gen / cols, rt->catalog_count, offsets,
-                             secrets);
+    return BuildShardCsrCore(wc, hist, rt->cur_index, rows, cols, rt->epi_rows, rt->catalog_count, offsets, secrets);
 }
 
 int EnsureRankWeights(ne_runtime* rt, float exponent, uint64_t n) {
@@ -1130,20 +1131,13 @@ static void LaunchFingerprints(ne_runtime* rt, const uint32_t* slots, uint32_t n
 // ================================================================================================
 extern "C" {
 
-int ne_create(const ne_runtime_config* config, ne_runtime** out_runtime) {
-    if (!config || !out_runtime) return NE_INVALID_ARGUMENT;
-    *out_runtime = nullptr;
-    const ne_runtime_config& c = *config;
-    if (c.encoder_hidden != 0) return NE_UNSUPPORTED;
-    if (c.host_spillover_byte_budget != 0) return NE_UNSUPPORTED;
-    // IsValidDeviceSlabGARuntimeConfig (slab_runtime.hpp:50-60) on this build's stride.
-    if (c.genotype_slab_byte_budget < c.generation_byte_budget || c.action_count == 0 || c.grid_column_count == 0 ||
-        (c.population_size_ceiling != 0 && c.population_size_ceiling % c.grid_column_count != 0))
-        return NE_INVALID_RUNTIME_CONFIG;
-    const uint64_t slots = SlotCountForBudget(c.genotype_slab_byte_budget, c.action_count, 0);
-    uint64_t max_gen = SlotCountForBudget(c.generation_byte_budget, c.action_count, c.population_size_ceiling);
-    max_gen = (max_gen / c.grid_column_count) * c.grid_column_count;
-    if (slots == 0 || max_gen == 0 || slots < max_gen || slots >= 0xFFFFFFFFull) return NE_INVALID_RUNTIME_CONFIG;
+}  // extern "C"
+
+// The runtime over a slab of `slots` slots of config.action_count actions and generation buffers of
+// max_gen entries (TryCreateDeviceSlabGARuntimeBuffers, slab_runtime.cu:943-1000: the slab layout's
+// byte size is slots x stride).
+static int CreateCore(const ne_runtime_config& c, uint64_t slots, uint64_t max_gen, uint64_t epi_rows,
+                      ne_runtime** out_runtime) {
     ne_runtime* rt = new (std::nothrow) ne_runtime();
     if (!rt) return NE_DEVICE_FAILURE;
     rt->config = c;
@@ -1165,9 +1159,10 @@ int ne_create(const ne_runtime_config* config, ne_runtime** out_runtime) {
     rt->action_count = c.action_count;
     rt->stride = SlotStrideBytes(c.action_count);
     rt->slot_count = slots;
-    rt->slab_bytes = c.genotype_slab_byte_budget;
+    rt->slab_bytes = slots * rt->stride;
     rt->max_gen = max_gen;
-    rt->generation_byte_budget = max_gen * rt->stride;
+    rt->epi_rows = epi_rows;
+    rt->generation_byte_budget = c.generation_byte_budget;
     bool ok = Alloc(&rt->slab, slots * rt->stride);
     ok &= Alloc(&rt->slot_state, slots) && Alloc(&rt->free_stack, slots) && Alloc(&rt->free_count, 1);
     ok &= Alloc(&rt->cur_slots, max_gen) && Alloc(&rt->cur_fit, max_gen) && Alloc(&rt->cur_evals, max_gen) &&
@@ -1195,6 +1190,27 @@ int ne_create(const ne_runtime_config* config, ne_runtime** out_runtime) {
     }
     *out_runtime = rt;
     return NE_OK;
+}
+
+extern "C" {
+
+int ne_create(const ne_runtime_config* config, ne_runtime** out_runtime) {
+    if (!config || !out_runtime) return NE_INVALID_ARGUMENT;
+    *out_runtime = nullptr;
+    ne_runtime_config c = *config;
+    if (c.encoder_hidden != 0) return NE_UNSUPPORTED;
+    if (c.host_spillover_byte_budget != 0) return NE_UNSUPPORTED;
+    // IsValidDeviceSlabGARuntimeConfig (slab_runtime.hpp:50-60) on this build's stride.
+    if (c.genotype_slab_byte_budget < c.generation_byte_budget || c.action_count == 0 || c.grid_column_count == 0 ||
+        (c.population_size_ceiling != 0 && c.population_size_ceiling % c.grid_column_count != 0))
+        return NE_INVALID_RUNTIME_CONFIG;
+    const uint64_t slots = SlotCountForBudget(c.genotype_slab_byte_budget, c.action_count, 0);
+    uint64_t max_gen = SlotCountForBudget(c.generation_byte_budget, c.action_count, c.population_size_ceiling);
+    max_gen = (max_gen / c.grid_column_count) * c.grid_column_count;
+    if (slots == 0 || max_gen == 0 || slots < max_gen || slots >= 0xFFFFFFFFull) return NE_INVALID_RUNTIME_CONFIG;
+    // TryComputeGenerationByteBudgetBytes (slab_runtime.cu:683-696): capacity x stride
+    c.generation_byte_budget = max_gen * SlotStrideBytes(c.action_count);
+    return CreateCore(c, slots, max_gen, max_gen / c.grid_column_count, out_runtime);
 }
 
 void ne_destroy(ne_runtime* rt) {
@@ -1418,15 +1434,16 @@ int ne_read_summary(ne_runtime* rt, ne_fitness_summary* out) {
     return NE_OK;
 }
 
-int ne_advance_generation(ne_runtime* rt, uint32_t generation_seed, const ne_word_counts* counts,
-                          const ne_assembly_config* config, const ne_pending_injection* injection, uint32_t verbose,
-                          const ne_callbacks* callbacks, const ne_shard_release_history* history) {
-    (void)verbose;
-    if (!rt || !counts) return Fail(rt, NE_INVALID_ARGUMENT);
-    const ne_assembly_config cfg = config ? *config : DefaultAssembly();
-    if (!ValidAssembly(cfg)) return Fail(rt, NE_INVALID_ASSEMBLY_CONFIG);
-    ne_pending_injection inj{};
-    if (injection) inj = *injection;
+}  // extern "C"
+
+// The first half of a generation step, up to the pre-recombination boundary
+// (TryPreparePrebreedingBoundaryOnDevice, slab_runtime.cu:1363-1510): evaluation, the shard-release
+// callback (which may set the injection), the next generation's shape, the parent plan, the
+// post-fitness callback, and the widening repack when an injection is pending.  *parent_a receives
+// the action count the parents were bred with.
+static int PrepareBoundary(ne_runtime* rt, uint32_t generation_seed, const ne_word_counts* counts,
+                           const ne_assembly_config& cfg, ne_pending_injection& inj, const ne_callbacks* callbacks,
+                           const ne_shard_release_history* history, uint64_t* parent_a) {
     rt->times = ne_stage_times{};
     int st = EvaluateFull(rt, counts, history);
     if (st != NE_OK) return st;
@@ -1462,12 +1479,19 @@ int ne_advance_generation(ne_runtime* rt, uint32_t generation_seed, const ne_wor
     if (callbacks && callbacks->post_fitness_evaluation &&
         callbacks->post_fitness_evaluation(rt, counts, callbacks->user) != 0)
         return Fail(rt, NE_CALLBACK_FAILED);
-    const uint64_t parent_a = rt->action_count;
+    *parent_a = rt->action_count;
     if (inj.enabled) {
         st = Widen(rt, next_a);
         if (st != NE_OK) return st;
     }
-    st = Assemble(rt, generation_seed, cfg, parent_a);
+    return NE_OK;
+}
+
+// The second half (TryAssemblePreparedGenerationOnDevice, slab_runtime.cu:1512-1686): assembly in
+// batches, the output-embedding injection, and the generation swap.
+static int AssemblePrepared(ne_runtime* rt, uint32_t generation_seed, const ne_assembly_config& cfg,
+                            uint64_t parent_a, const ne_pending_injection& inj) {
+    int st = Assemble(rt, generation_seed, cfg, parent_a);
     if (st != NE_OK) return st;
     if (inj.enabled) {
         st = Inject(rt, parent_a, inj);
@@ -1476,6 +1500,23 @@ int ne_advance_generation(ne_runtime* rt, uint32_t generation_seed, const ne_wor
     SwapGenerations(rt);
     rt->last_status = NE_OK;
     return NE_OK;
+}
+
+extern "C" {
+
+int ne_advance_generation(ne_runtime* rt, uint32_t generation_seed, const ne_word_counts* counts,
+                          const ne_assembly_config* config, const ne_pending_injection* injection, uint32_t verbose,
+                          const ne_callbacks* callbacks, const ne_shard_release_history* history) {
+    (void)verbose;
+    if (!rt || !counts) return Fail(rt, NE_INVALID_ARGUMENT);
+    const ne_assembly_config cfg = config ? *config : DefaultAssembly();
+    if (!ValidAssembly(cfg)) return Fail(rt, NE_INVALID_ASSEMBLY_CONFIG);
+    ne_pending_injection inj{};
+    if (injection) inj = *injection;
+    uint64_t parent_a = 0;
+    const int st = PrepareBoundary(rt, generation_seed, counts, cfg, inj, callbacks, history, &parent_a);
+    if (st != NE_OK) return st;
+    return AssemblePrepared(rt, generation_seed, cfg, parent_a, inj);
 }
 
 int ne_get_slab_info(ne_runtime* rt, ne_slab_info* out) {
@@ -1746,7 +1787,7 @@ int ne_advance_generation_screened(ne_runtime* rt, uint32_t generation_seed, con
     const std::vector<uint32_t>* offs = &rt->h_offsets;
     const std::vector<uint32_t>* secs = &rt->h_secrets;
     if (!rt->csr_from_caller) {
-        st = BuildShardCsrCore(counts, history, rt->cur_index + 1, rows, cols, rt->max_gen / cols, rt->catalog_count,
+        st = BuildShardCsrCore(counts, history, rt->cur_index + 1, rows, cols, rt->epi_rows, rt->catalog_count,
                                built_o, built_s);
         if (st != NE_OK) return Fail(rt, st);
         offs = &built_o;
@@ -2025,6 +2066,230 @@ int ne_surrogate_set_params(ne_runtime* rt, const float* src, uint64_t count) {
 double ne_rank_correlation(const float* a, const float* b, uint64_t n) {
     if (!a || !b) return std::nan("");
     return RankCorrelation(a, b, n);
+}
+
+}  // extern "C"
+
+// ---- checkpoint and resume (device side; the file format is in ne_checkpoint.cpp) ---------------
+
+// Copies the genomes of `slots` to dst[n][stride] through two pinned staging buffers of up to 64 MiB:
+// the DMA into one overlaps the host copy out of the other.
+static int DownloadGenomes(ne_runtime* rt, const uint32_t* slots, uint64_t n, uint8_t* dst) {
+    if (n == 0) return NE_OK;
+    const uint64_t S = rt->stride;
+    const uint64_t per = std::max<uint64_t>(1, std::min<uint64_t>(n, (64ull << 20) / S));
+    uint8_t* pin[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    uint64_t first[2] = {0, 0}, cnt[2] = {0, 0};
+    bool ok = true;
+    for (int i = 0; i < 2; ++i) {
+        ok = ok && Ok(hipHostMalloc(reinterpret_cast<void**>(&pin[i]), per * S, hipHostMallocDefault));
+        ok = ok && Ok(hipEventCreateWithFlags(&done[i], hipEventDisableTiming));
+    }
+    auto drain = [&](int i) {
+        if (!ok || cnt[i] == 0) return;
+        ok = Ok(hipEventSynchronize(done[i]));
+        if (ok) std::memcpy(dst + first[i] * S, pin[i], cnt[i] * S);
+        cnt[i] = 0;
+    };
+    for (uint64_t b = 0, c = 0; ok && b < n; b += per, ++c) {
+        const int i = int(c & 1);
+        drain(i);
+        const uint64_t k = std::min(per, n - b);
+        for (uint64_t j = 0; ok && j < k; ++j)
+            ok = Ok(hipMemcpyAsync(pin[i] + j * S, rt->slab + uint64_t(slots[b + j]) * S, S, hipMemcpyDeviceToHost,
+                                   rt->stream));
+        ok = ok && Ok(hipEventRecord(done[i], rt->stream));
+        first[i] = b;
+        cnt[i] = k;
+    }
+    drain(0);
+    drain(1);
+    (void)hipStreamSynchronize(rt->stream);  // nothing may still write the staging buffers
+    for (int i = 0; i < 2; ++i) {
+        if (pin[i]) (void)hipHostFree(pin[i]);
+        if (done[i]) (void)hipEventDestroy(done[i]);
+    }
+    return ok ? NE_OK : Fail(rt, NE_DEVICE_FAILURE);
+}
+
+static void StoreShape(uint8_t* out, uint64_t rows, uint64_t cols) {
+    ne_ckpt::StoreU64(out, rows);
+    ne_ckpt::StoreU64(out + 8, cols);
+    ne_ckpt::StoreU64(out + 16, rows * cols);
+}
+
+extern "C" {
+
+int ne_create_prebreeding_checkpoint(ne_runtime* rt, uint32_t generation_seed, const ne_word_counts* counts,
+                                     const ne_assembly_config* config, const ne_pending_injection* injection,
+                                     const uint8_t* catalog_letters, uint64_t catalog_word_count, uint32_t verbose,
+                                     const ne_callbacks* callbacks, const ne_shard_release_history* history,
+                                     ne_checkpoint** out_checkpoint) {
+    (void)verbose;
+    if (!rt || !counts || !out_checkpoint) return Fail(rt, NE_INVALID_ARGUMENT);
+    *out_checkpoint = nullptr;
+    if (history && (history->release_count > ne_ckpt::kReleaseCapacity ||
+                    (history->release_count && !history->release_generations)))
+        return Fail(rt, NE_INVALID_ARGUMENT);
+    const ne_assembly_config cfg = config ? *config : DefaultAssembly();
+    if (!ValidAssembly(cfg)) return Fail(rt, NE_INVALID_ASSEMBLY_CONFIG);
+    ne_pending_injection inj{};
+    if (injection) inj = *injection;
+    uint64_t parent_a = 0;
+    int st = PrepareBoundary(rt, generation_seed, counts, cfg, inj, callbacks, history, &parent_a);
+    if (st != NE_OK) return st;
+
+    // TryDownloadBoundaryCheckpointPayload (slab_runtime.cu:1260-1356)
+    ne_checkpoint* c = new (std::nothrow) ne_checkpoint();
+    if (!c) return Fail(rt, NE_DEVICE_FAILURE);
+    const uint64_t n = rt->cur_size, children = rt->planned_children, cols = rt->config.grid_column_count;
+    try {
+        c->generation_seed = generation_seed;
+        ne_ckpt::EncodeWordCounts(*counts, c->word_counts);
+        if (history && history->release_count) {
+            c->release_generations.assign(history->release_generations,
+                                          history->release_generations + history->release_count);
+            c->release_p99.assign(history->release_count, 0.0f);
+            if (history->release_fitness_p99)
+                c->release_p99.assign(history->release_fitness_p99,
+                                      history->release_fitness_p99 + history->release_count);
+        }
+        c->identity_hash = ne_ckpt::IdentityHash(c->word_counts, history != nullptr, c->release_generations,
+                                                 c->release_p99, catalog_letters, catalog_word_count);
+        ne_ckpt::EncodeAssembly(cfg, c->assembly);
+        ne_ckpt::EncodeInjection(inj, c->injection);
+        const uint64_t rc[6] = {rt->slab_bytes, rt->generation_byte_budget, 0, rt->action_count, rt->max_gen, cols};
+        std::memcpy(c->runtime_config, rc, sizeof(rc));
+        const uint64_t layout[4] = {rt->action_count, rt->stride, rt->slot_count, rt->slab_bytes};
+        std::memcpy(c->slab_layout, layout, sizeof(layout));
+        StoreShape(c->shapes[0], n / cols, cols);
+        StoreShape(c->shapes[1], children / cols, cols);
+        StoreShape(c->shapes[2], rt->epi_rows, cols);
+        c->generation_index = rt->cur_index;
+        c->slots.resize(n);
+        c->fitness.resize(n);
+        c->evals.resize(n);
+        c->has.resize(n);
+        c->pairs.resize(2 * children);
+    } catch (const std::bad_alloc&) {
+        delete c;
+        return Fail(rt, NE_DEVICE_FAILURE);
+    }
+    bool ok = Ok(hipMemcpyAsync(c->slots.data(), rt->cur_slots, n * 4, hipMemcpyDeviceToHost, rt->stream));
+    ok = ok && Ok(hipMemcpyAsync(c->fitness.data(), rt->cur_fit, n * 4, hipMemcpyDeviceToHost, rt->stream));
+    ok = ok && Ok(hipMemcpyAsync(c->evals.data(), rt->cur_evals, n * 4, hipMemcpyDeviceToHost, rt->stream));
+    ok = ok && Ok(hipMemcpyAsync(c->has.data(), rt->cur_has, n, hipMemcpyDeviceToHost, rt->stream));
+    ok = ok && Ok(hipMemcpyAsync(c->pairs.data(), rt->pairs, children * 8, hipMemcpyDeviceToHost, rt->stream));
+    ok = ok && Ok(hipStreamSynchronize(rt->stream));
+    if (!ok) {
+        delete c;
+        return Fail(rt, NE_DEVICE_FAILURE);
+    }
+    // one record per organism holding a slot, in organism order
+    std::vector<uint32_t> live_slots;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (c->slots[i] == kInvalidSlot) continue;
+        if (c->slots[i] >= rt->slot_count) {
+            delete c;
+            return Fail(rt, NE_INVALID_SLAB);
+        }
+        c->organisms.push_back(uint32_t(i));
+        live_slots.push_back(c->slots[i]);
+    }
+    c->genome_bytes = rt->stride;
+    try {
+        c->genomes.resize(live_slots.size() * rt->stride);
+    } catch (const std::bad_alloc&) {
+        delete c;
+        return Fail(rt, NE_DEVICE_FAILURE);
+    }
+    st = DownloadGenomes(rt, live_slots.data(), live_slots.size(), c->genomes.data());
+    if (st != NE_OK) {
+        delete c;
+        return st;
+    }
+    c->checksum = ne_ckpt::Checksum(*c);
+    *out_checkpoint = c;
+    rt->last_status = NE_OK;
+    return NE_OK;
+}
+
+int ne_restore_prebreeding_checkpoint(const ne_checkpoint* c, void* stream, ne_runtime** out_runtime) {
+    if (!c || !out_runtime) return NE_INVALID_ARGUMENT;
+    *out_runtime = nullptr;
+    // the validation of TryRestorePrebreedingCheckpointToDevice (slab_runtime.cu:1734-1742, 1790-1813)
+    if (c->schema_version != ne_ckpt::kSchemaVersion || c->genome_layout_version != ne_ckpt::kGenomeLayoutVersion ||
+        c->resume_phase != ne_ckpt::kPhasePreRecombination || c->checksum != ne_ckpt::Checksum(*c))
+        return NE_CHECKPOINT_INVALID;
+    const uint64_t A = c->ActionCount(), S = c->Stride(), slots = c->SlotCount();
+    if (A == 0 || slots == 0 || slots >= 0xFFFFFFFFull || S != SlotStrideBytes(A) || c->SlabBytes() / S != slots ||
+        c->genome_bytes != S)
+        return NE_CHECKPOINT_INVALID;
+    const uint64_t n = c->slots.size(), children = c->ChildCount(), cols = c->RuntimeConfigField(5);
+    const uint64_t epi_rows = c->ShapeField(2, 0), L = c->Records();
+    if (n == 0 || children == 0 || cols == 0 || n % cols != 0 || children % cols != 0 || epi_rows == 0 ||
+        L > slots || n > 0xFFFFFFFFull)
+        return NE_CHECKPOINT_INVALID;
+    if (c->RuntimeConfigField(2) != 0) return NE_UNSUPPORTED;  // host spill budget
+    std::vector<uint8_t> restored(n, 0);
+    std::vector<uint32_t> cur_slots(n, kInvalidSlot);
+    for (uint64_t k = 0; k < L; ++k) {
+        const uint32_t org = c->organisms[k];
+        if (org >= n || restored[org] || c->slots[org] == kInvalidSlot) return NE_CHECKPOINT_INVALID;
+        restored[org] = 1;
+        cur_slots[org] = uint32_t(k);  // TryAllocateSlabSlot on a fresh slab hands out slots 0, 1, 2, ...
+    }
+    for (uint64_t i = 0; i < 2 * children; ++i)
+        if (c->pairs[i] >= n || !restored[c->pairs[i]]) return NE_CHECKPOINT_INVALID;
+
+    ne_runtime_config rc{};
+    rc.genotype_slab_byte_budget = slots * S;
+    rc.generation_byte_budget = c->RuntimeConfigField(1);
+    rc.action_count = A;
+    rc.grid_column_count = cols;
+    rc.stream = stream;
+    const uint64_t max_gen = std::max(n, children);
+    rc.population_size_ceiling = max_gen;
+    ne_runtime* rt = nullptr;
+    int st = CreateCore(rc, slots, max_gen, epi_rows, &rt);
+    if (st != NE_OK) return st;
+    LaunchFillU32(rt->slot_state, 1u, L, rt->stream);
+    LaunchPopSlots(rt->free_count, uint32_t(L), rt->stream);
+    bool ok = L == 0 || Ok(hipMemcpyAsync(rt->slab, c->genomes.data(), L * S, hipMemcpyHostToDevice, rt->stream));
+    ok = ok && Ok(hipMemcpyAsync(rt->cur_slots, cur_slots.data(), n * 4, hipMemcpyHostToDevice, rt->stream));
+    ok = ok && Ok(hipMemcpyAsync(rt->cur_fit, c->fitness.data(), n * 4, hipMemcpyHostToDevice, rt->stream));
+    ok = ok && Ok(hipMemcpyAsync(rt->cur_evals, c->evals.data(), n * 4, hipMemcpyHostToDevice, rt->stream));
+    ok = ok && Ok(hipMemcpyAsync(rt->cur_has, c->has.data(), n, hipMemcpyHostToDevice, rt->stream));
+    ok = ok && Ok(hipMemcpyAsync(rt->pairs, c->pairs.data(), children * 8, hipMemcpyHostToDevice, rt->stream));
+    ok = ok && Sync(rt) == NE_OK;
+    if (!ok) {
+        ne_destroy(rt);
+        return NE_DEVICE_FAILURE;
+    }
+    rt->cur_index = c->generation_index;
+    rt->cur_size = n;
+    rt->planned_children = children;
+    rt->cur_evaluated = true;
+    *out_runtime = rt;
+    return NE_OK;
+}
+
+int ne_resume_generation_from_checkpoint(ne_runtime* rt, const ne_checkpoint* c, uint32_t verbose) {
+    (void)verbose;
+    if (!rt || !c) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (c->Stride() != rt->stride || c->ActionCount() != rt->action_count || c->generation_index != rt->cur_index ||
+        c->slots.size() != rt->cur_size || c->ChildCount() != rt->planned_children || rt->planned_children == 0)
+        return Fail(rt, NE_INVALID_GENERATION);
+    const ne_assembly_config cfg = ne_ckpt::DecodeAssembly(c->assembly);
+    if (!ValidAssembly(cfg)) return Fail(rt, NE_INVALID_ASSEMBLY_CONFIG);
+    const ne_pending_injection inj = ne_ckpt::DecodeInjection(c->injection);
+    // SlabDeviceAssemblyConfig.parent_action_count of the resume (slab_runtime.cu:1871-1874)
+    const uint64_t parent_a = inj.enabled ? inj.first_catalog_word_index : c->ActionCount();
+    if (inj.enabled && (inj.injection_count == 0 || parent_a + inj.injection_count != rt->action_count ||
+                        parent_a + inj.injection_count > rt->catalog_count))
+        return Fail(rt, NE_OUTPUT_EMBEDDING_INJECTION_FAILED);
+    return AssemblePrepared(rt, c->generation_seed, cfg, parent_a, inj);
 }
 
 }  // extern "C"

@@ -1,9 +1,14 @@
 // run_ga.cpp -- thin C++ driver of the GA generation step through the host API of slab_runtime.hpp:
 // bootstrap, then `generations` calls of TryAdvanceGenerationOnDevice with the reference CLI's
 // generation-seed convention (seed + 2 + g, src/cli/run_genetic_algorithm.cu:1883-1884), then a
-// final evaluation, printing one summary line per generation.
+// final evaluation, printing one summary line per generation.  With --checkpoint-every K the steps
+// from generations K, 2K, ... go through TryCreatePrebreedingCheckpointOnDevice, a background write to
+// --checkpoint PATH and TryResumeGenerationFromCheckpointOnDevice; --resume PATH restores a checkpoint
+// into a new runtime and continues from it (the reference CLI's flow, run_genetic_algorithm.cu:1605-1660,
+// 1900-1967).
 //
 // usage: run_ga <workload_dir> <grid_columns> <generations> <seed> [mutation_probability]
+//               [--checkpoint PATH --checkpoint-every K] [--resume PATH]
 // workload_dir holds catalog.u8 (5 letter indices per word), offsets.u32 (cells+1) and secrets.u32.
 #include <chrono>
 #include <cstdio>
@@ -35,14 +40,38 @@ static int Fail(const DeviceSlabGARuntimeBuffers& b, const char* what) {
 
 int main(int argc, char** argv) {
     if (argc < 5) {
-        std::fprintf(stderr, "usage: %s <workload_dir> <grid_columns> <generations> <seed> [mutation_p]\n", argv[0]);
+        std::fprintf(stderr,
+                     "usage: %s <workload_dir> <grid_columns> <generations> <seed> [mutation_p] "
+                     "[--checkpoint PATH --checkpoint-every K] [--resume PATH]\n",
+                     argv[0]);
         return 2;
     }
     const std::string dir = argv[1];
     const std::size_t cols = std::strtoull(argv[2], nullptr, 10);
     const std::size_t generations = std::strtoull(argv[3], nullptr, 10);
     const std::uint32_t seed = static_cast<std::uint32_t>(std::strtoul(argv[4], nullptr, 10));
-    const float mutation_p = argc > 5 ? std::strtof(argv[5], nullptr) : 0.01f;
+    float mutation_p = 0.01f;
+    std::string checkpoint_path, resume_path;
+    std::size_t checkpoint_every = 0;
+    for (int a = 5; a < argc; ++a) {
+        const std::string arg = argv[a];
+        if (arg == "--checkpoint" && a + 1 < argc) {
+            checkpoint_path = argv[++a];
+        } else if (arg == "--checkpoint-every" && a + 1 < argc) {
+            checkpoint_every = std::strtoull(argv[++a], nullptr, 10);
+        } else if (arg == "--resume" && a + 1 < argc) {
+            resume_path = argv[++a];
+        } else if (a == 5) {
+            mutation_p = std::strtof(argv[a], nullptr);
+        } else {
+            std::fprintf(stderr, "unknown argument %s\n", argv[a]);
+            return 2;
+        }
+    }
+    if (checkpoint_every != 0 && checkpoint_path.empty()) {
+        std::fprintf(stderr, "--checkpoint-every needs --checkpoint PATH\n");
+        return 2;
+    }
     std::vector<std::uint8_t> catalog;
     std::vector<std::uint32_t> offsets, secrets;
     if (!ReadAll(dir + "/catalog.u8", catalog) || !ReadAll(dir + "/offsets.u32", offsets) ||
@@ -60,14 +89,32 @@ int main(int argc, char** argv) {
     config.action_count = actions;
     config.grid_column_count = cols;
     DeviceSlabGARuntimeBuffers buffers{};
-    if (!TryCreateDeviceSlabGARuntimeBuffers(buffers, config)) {
+    RuntimeCheckpoint resumed{};
+    if (!resume_path.empty()) {
+        if (!TryReadRuntimeCheckpoint(resume_path, resumed)) {
+            std::fprintf(stderr, "could not read or validate checkpoint %s\n", resume_path.c_str());
+            return 1;
+        }
+        if (!TryRestorePrebreedingCheckpointToDevice(resumed, buffers)) {
+            std::fprintf(stderr, "could not restore checkpoint %s\n", resume_path.c_str());
+            return 1;
+        }
+    } else if (!TryCreateDeviceSlabGARuntimeBuffers(buffers, config)) {
         std::fprintf(stderr, "runtime creation failed\n");
         return 1;
     }
     if (!TryUploadTrainingWordCatalog(buffers, catalog) || !TryUploadTrainingSecrets(buffers, offsets, secrets))
         return Fail(buffers, "input upload");
-    const ne_init_config init{1.0f, 1.0f, 0.1f, 0.0f};
-    if (!TryBootstrapRandomCurrentGenerationOnDevice(buffers, cells, seed, 0, &init)) return Fail(buffers, "bootstrap");
+    std::size_t first_generation = 0;
+    if (resumed.Valid()) {
+        if (!TryResumeGenerationFromCheckpointOnDevice(buffers, resumed)) return Fail(buffers, "checkpoint resume");
+        first_generation = resumed.Info().generation_index + 1;
+        std::printf("resumed generation %zu from %s\n", first_generation, resume_path.c_str());
+    } else {
+        const ne_init_config init{1.0f, 1.0f, 0.1f, 0.0f};
+        if (!TryBootstrapRandomCurrentGenerationOnDevice(buffers, cells, seed, 0, &init))
+            return Fail(buffers, "bootstrap");
+    }
 
     RuntimeWordCounts counts{};
     counts.training_word_count = actions;
@@ -81,15 +128,33 @@ int main(int argc, char** argv) {
         ++evaluated;
         return true;
     };
+    RuntimeCheckpointAsyncWriter writer{};
     const auto start = std::chrono::steady_clock::now();
-    for (std::size_t g = 0; g < generations; ++g) {
+    for (std::size_t g = first_generation; g < generations; ++g) {
         const std::uint32_t generation_seed = seed + 2u + static_cast<std::uint32_t>(g);
-        if (!TryAdvanceGenerationOnDevice(buffers, generation_seed, counts, assembly, {}, false, count_evaluations))
+        if (checkpoint_every != 0 && g != 0 && g % checkpoint_every == 0) {
+            RuntimeCheckpoint checkpoint{}, for_write{};
+            if (!TryCreatePrebreedingCheckpointOnDevice(buffers, generation_seed, counts, assembly, {}, checkpoint,
+                                                        &catalog, false, count_evaluations))
+                return Fail(buffers, "checkpoint creation");
+            if (!writer.TryWaitForWrite() || !TryCloneRuntimeCheckpoint(checkpoint, for_write) ||
+                !writer.TryStartWrite(std::move(for_write), checkpoint_path)) {
+                std::fprintf(stderr, "checkpoint write failed\n");
+                return 1;
+            }
+            if (!TryResumeGenerationFromCheckpointOnDevice(buffers, checkpoint)) return Fail(buffers, "resume");
+        } else if (!TryAdvanceGenerationOnDevice(buffers, generation_seed, counts, assembly, {}, false,
+                                                 count_evaluations)) {
             return Fail(buffers, "advance");
+        }
         PopulationFitnessSummary s{};
         if (!TryReadPopulationFitnessSummaryFromDevice(buffers, s)) return Fail(buffers, "summary");
         std::printf("generation %zu: best %.6f mean %.6f best_index %zu population %zu\n", s.generation_index,
                     s.best_fitness, s.average_fitness, s.best_index, s.population_size);
+    }
+    if (!writer.TryWaitForWrite()) {
+        std::fprintf(stderr, "checkpoint write failed\n");
+        return 1;
     }
     if (!TryEvaluateCurrentGenerationFitnessOnDevice(buffers, counts)) return Fail(buffers, "final evaluation");
     PopulationFitnessSummary s{};
@@ -101,5 +166,5 @@ int main(int argc, char** argv) {
     if (!TryDownloadCurrentGenerationFromDevice(buffers, current) || current.slot_indices.size() != cells)
         return Fail(buffers, "download");
     DestroyDeviceSlabGARuntimeBuffers(buffers);
-    return evaluated == generations ? 0 : 1;
+    return evaluated == generations - first_generation ? 0 : 1;
 }

@@ -7,9 +7,13 @@
 // every call returns false with the runtime status readable through TryReadDeviceSlabGARuntimeStatus.
 #pragma once
 
+#include <chrono>
 #include <cstddef>
 #include <cstdint>
+#include <filesystem>
 #include <functional>
+#include <future>
+#include <utility>
 #include <vector>
 
 #include "../../include/ne_runtime.h"
@@ -108,6 +112,7 @@ struct SlabGeneration {
 
 struct TrainingDataShardReleaseHistory {
     std::vector<std::uint64_t> release_generations;
+    std::vector<float> release_fitness_p99;  // empty, or one per release generation
 };
 
 struct DeviceSlabGARuntimeBuffers {
@@ -143,6 +148,18 @@ inline RuntimeWordCounts FromC(const ne_word_counts& c) {
     w.shard_initial_radius = c.shard_initial_radius;
     w.shard_radius_growth_period_generations = c.shard_radius_growth_period_generations;
     return w;
+}
+
+// The C view of a history (null when there is none).
+inline const ne_shard_release_history* HistoryView(const TrainingDataShardReleaseHistory* history,
+                                                   ne_shard_release_history& view) {
+    if (!history) return nullptr;
+    view = {};
+    view.release_generations = history->release_generations.data();
+    view.release_count = history->release_generations.size();
+    if (history->release_fitness_p99.size() == history->release_generations.size())
+        view.release_fitness_p99 = history->release_fitness_p99.data();
+    return &view;
 }
 
 struct CallbackContext {
@@ -216,11 +233,7 @@ inline bool TryEvaluateCurrentGenerationFitnessOnDevice(DeviceSlabGARuntimeBuffe
                                                         const TrainingDataShardReleaseHistory* history = nullptr) {
     const ne_word_counts c = detail::ToC(counts);
     ne_shard_release_history h{};
-    if (history) {
-        h.release_generations = history->release_generations.data();
-        h.release_count = history->release_generations.size();
-    }
-    return ne_evaluate(buffers.handle, &c, history ? &h : nullptr) == NE_OK;
+    return ne_evaluate(buffers.handle, &c, detail::HistoryView(history, h)) == NE_OK;
 }
 
 // TryReadPopulationFitnessSummaryFromDevice (slab_runtime.hpp:141-142)
@@ -267,12 +280,8 @@ inline bool TryAdvanceGenerationOnDevice(DeviceSlabGARuntimeBuffers& buffers, st
     cbs.training_shard_release = training_shard_release_callback ? detail::ReleaseTrampoline : nullptr;
     cbs.user = &ctx;
     ne_shard_release_history h{};
-    if (history) {
-        h.release_generations = history->release_generations.data();
-        h.release_count = history->release_generations.size();
-    }
     return ne_advance_generation(buffers.handle, generation_seed, &c, &a, &inj, verbose ? 1u : 0u, &cbs,
-                                 history ? &h : nullptr) == NE_OK;
+                                 detail::HistoryView(history, h)) == NE_OK;
 }
 
 inline bool TryDownloadGenerationFromDevice(const DeviceSlabGARuntimeBuffers& buffers, int which,
@@ -343,13 +352,143 @@ inline bool TryAdvanceScreenedGenerationOnDevice(DeviceSlabGARuntimeBuffers& buf
     const ne_word_counts c = detail::ToC(counts);
     const ne_assembly_config a = detail::ToC(config);
     ne_shard_release_history h{};
-    if (history) {
-        h.release_generations = history->release_generations.data();
-        h.release_count = history->release_generations.size();
-    }
-    return ne_advance_generation_screened(buffers.handle, generation_seed, &c, &a, history ? &h : nullptr, report) ==
-           NE_OK;
+    return ne_advance_generation_screened(buffers.handle, generation_seed, &c, &a, detail::HistoryView(history, h),
+                                          report) == NE_OK;
 }
+
+// ---- checkpoint and resume (slab_runtime.hpp:86-178) ------------------------------------------
+
+// RuntimeCheckpoint (slab_runtime.hpp:98-117): owns the library's checkpoint; move-only.  Info()
+// carries the header fields, the layout and the decoded configuration.
+class RuntimeCheckpoint {
+  public:
+    RuntimeCheckpoint() = default;
+    explicit RuntimeCheckpoint(ne_checkpoint* handle) : handle_(handle) {}
+    RuntimeCheckpoint(const RuntimeCheckpoint&) = delete;
+    RuntimeCheckpoint& operator=(const RuntimeCheckpoint&) = delete;
+    RuntimeCheckpoint(RuntimeCheckpoint&& other) noexcept : handle_(std::exchange(other.handle_, nullptr)) {}
+    RuntimeCheckpoint& operator=(RuntimeCheckpoint&& other) noexcept {
+        if (this != &other) {
+            ne_free_checkpoint(handle_);
+            handle_ = std::exchange(other.handle_, nullptr);
+        }
+        return *this;
+    }
+    ~RuntimeCheckpoint() { ne_free_checkpoint(handle_); }
+
+    bool Valid() const { return handle_ != nullptr; }
+    ne_checkpoint* Handle() const { return handle_; }
+    ne_checkpoint_info Info() const {
+        ne_checkpoint_info i{};
+        if (handle_) (void)ne_get_checkpoint_info(handle_, &i);
+        return i;
+    }
+    void Reset(ne_checkpoint* handle = nullptr) {
+        ne_free_checkpoint(handle_);
+        handle_ = handle;
+    }
+
+  private:
+    ne_checkpoint* handle_ = nullptr;
+};
+
+// TryCreatePrebreedingCheckpointOnDevice (slab_runtime.hpp:152-158)
+inline bool TryCreatePrebreedingCheckpointOnDevice(
+    DeviceSlabGARuntimeBuffers& buffers, std::uint32_t generation_seed, const RuntimeWordCounts& counts,
+    const GenerationAssemblyConfig& config, const PendingOutputEmbeddingInjection& injection,
+    RuntimeCheckpoint& checkpoint_out, const std::vector<std::uint8_t>* host_training_word_catalog = nullptr,
+    bool verbose = false, const PostFitnessEvaluationCallback& post_fitness_evaluation_callback = {},
+    const TrainingShardReleaseCallback& training_shard_release_callback = {},
+    const TrainingDataShardReleaseHistory* history = nullptr) {
+    checkpoint_out.Reset();
+    const ne_word_counts c = detail::ToC(counts);
+    const ne_assembly_config a = detail::ToC(config);
+    ne_pending_injection inj{};
+    inj.enabled = injection.enabled ? 1u : 0u;
+    inj.first_catalog_word_index = injection.first_catalog_word_index;
+    inj.injection_count = injection.injection_count;
+    detail::CallbackContext ctx{&buffers, &post_fitness_evaluation_callback, &training_shard_release_callback};
+    ne_callbacks cbs{};
+    cbs.post_fitness_evaluation = post_fitness_evaluation_callback ? detail::PostTrampoline : nullptr;
+    cbs.training_shard_release = training_shard_release_callback ? detail::ReleaseTrampoline : nullptr;
+    cbs.user = &ctx;
+    ne_shard_release_history h{};
+    ne_checkpoint* out = nullptr;
+    const std::uint8_t* letters = host_training_word_catalog ? host_training_word_catalog->data() : nullptr;
+    const std::uint64_t words = host_training_word_catalog ? host_training_word_catalog->size() / 5 : 0;
+    if (ne_create_prebreeding_checkpoint(buffers.handle, generation_seed, &c, &a, &inj, letters, words,
+                                         verbose ? 1u : 0u, &cbs, detail::HistoryView(history, h), &out) != NE_OK)
+        return false;
+    checkpoint_out.Reset(out);
+    return true;
+}
+
+// TryRestorePrebreedingCheckpointToDevice (slab_runtime.hpp:160): `buffers` receives a new runtime.
+inline bool TryRestorePrebreedingCheckpointToDevice(const RuntimeCheckpoint& checkpoint,
+                                                    DeviceSlabGARuntimeBuffers& buffers) {
+    buffers = {};
+    if (ne_restore_prebreeding_checkpoint(checkpoint.Handle(), nullptr, &buffers.handle) != NE_OK) return false;
+    buffers.grid_column_count = checkpoint.Info().grid_column_count;
+    return true;
+}
+
+// TryResumeGenerationFromCheckpointOnDevice (slab_runtime.hpp:162-163)
+inline bool TryResumeGenerationFromCheckpointOnDevice(DeviceSlabGARuntimeBuffers& buffers,
+                                                      const RuntimeCheckpoint& checkpoint, bool verbose = false) {
+    return ne_resume_generation_from_checkpoint(buffers.handle, checkpoint.Handle(), verbose ? 1u : 0u) == NE_OK;
+}
+
+// TryWriteRuntimeCheckpointAtomically / TryReadRuntimeCheckpoint (slab_runtime.hpp:165-168)
+inline bool TryWriteRuntimeCheckpointAtomically(const RuntimeCheckpoint& checkpoint,
+                                                const std::filesystem::path& checkpoint_path) {
+    return ne_write_checkpoint(checkpoint.Handle(), checkpoint_path.c_str()) == NE_OK;
+}
+
+inline bool TryReadRuntimeCheckpoint(const std::filesystem::path& checkpoint_path, RuntimeCheckpoint& checkpoint_out) {
+    ne_checkpoint* out = nullptr;
+    checkpoint_out.Reset();
+    if (ne_read_checkpoint(checkpoint_path.c_str(), &out) != NE_OK) return false;
+    checkpoint_out.Reset(out);
+    return true;
+}
+
+// TryCloneRuntimeCheckpoint (run_genetic_algorithm.cu:1374-1415)
+inline bool TryCloneRuntimeCheckpoint(const RuntimeCheckpoint& source, RuntimeCheckpoint& clone_out) {
+    ne_checkpoint* out = nullptr;
+    clone_out.Reset();
+    if (ne_clone_checkpoint(source.Handle(), &out) != NE_OK) return false;
+    clone_out.Reset(out);
+    return true;
+}
+
+// RuntimeCheckpointAsyncWriter (slab_runtime.hpp:170-178; slab_runtime.cu:2083-2126): one background
+// write at a time; a finished write's result is collected before the next starts.
+class RuntimeCheckpointAsyncWriter {
+  public:
+    bool TryStartWrite(RuntimeCheckpoint checkpoint, std::filesystem::path checkpoint_path) {
+        if (IsWriteInProgress()) return false;
+        if (pending_.valid() && !pending_.get()) return false;
+        pending_ = std::async(std::launch::async, [c = std::move(checkpoint), p = std::move(checkpoint_path)]() {
+            return TryWriteRuntimeCheckpointAtomically(c, p);
+        });
+        return true;
+    }
+    bool IsWriteInProgress() {
+        return pending_.valid() && pending_.wait_for(std::chrono::seconds(0)) != std::future_status::ready;
+    }
+    bool TryCollectFinishedWrite(bool& write_finished_out, bool& write_succeeded_out) {
+        write_finished_out = false;
+        write_succeeded_out = true;
+        if (!pending_.valid() || IsWriteInProgress()) return true;
+        write_finished_out = true;
+        write_succeeded_out = pending_.get();
+        return write_succeeded_out;
+    }
+    bool TryWaitForWrite() { return pending_.valid() ? pending_.get() : true; }
+
+  private:
+    std::future<bool> pending_{};
+};
 
 // TryDownloadSlabSlotBytesFromDevice (slab_runtime.hpp:128-129)
 inline bool TryDownloadSlabSlotBytesFromDevice(const DeviceSlabGARuntimeBuffers& buffers, std::uint32_t slot_index,

@@ -19,7 +19,7 @@ STATUS_NAMES = {
     8: "action selection failed", 9: "population not evaluated", 10: "invalid assembly config",
     11: "parent selection failed", 12: "invalid assembly plan", 13: "invalid parent index", 14: "slab full",
     15: "output embedding injection failed", 16: "slab repack failed", 17: "invalid argument",
-    18: "unsupported configuration", 19: "callback failed",
+    18: "unsupported configuration", 19: "callback failed", 20: "checkpoint invalid", 21: "checkpoint I/O failure",
 }
 
 EVAL_FP32, EVAL_BF16_RESCORED, EVAL_FORCE_FALLBACK = 0, 1, 0x100
@@ -50,7 +50,8 @@ class WordCounts(C.Structure):
 
 
 class ShardReleaseHistory(C.Structure):
-    _fields_ = [("release_generations", C.POINTER(C.c_uint64)), ("release_count", C.c_uint64)]
+    _fields_ = [("release_generations", C.POINTER(C.c_uint64)), ("release_count", C.c_uint64),
+                ("release_fitness_p99", C.POINTER(C.c_float))]
 
 
 class AssemblyConfig(C.Structure):
@@ -129,6 +130,16 @@ def default_screen_config(**kw):
     return c
 
 
+class CheckpointInfo(C.Structure):
+    _fields_ = ([(n, C.c_uint32) for n in ("schema_version", "genome_layout_version", "resume_phase",
+                                          "generation_seed")] +
+                [(n, C.c_uint64) for n in ("checksum", "training_data_identity_hash", "generation_index",
+                                          "population", "child_count", "live_genotypes", "payload_bytes",
+                                          "action_count", "slot_stride_bytes", "slot_count", "slab_bytes",
+                                          "generation_byte_budget", "grid_column_count", "release_count")] +
+                [("word_counts", WordCounts), ("assembly_config", AssemblyConfig), ("injection", PendingInjection)])
+
+
 class Callbacks(C.Structure):
     _fields_ = [("post_fitness_evaluation", POST_FITNESS_CB), ("training_shard_release", SHARD_RELEASE_CB),
                 ("user", C.c_void_p)]
@@ -142,7 +153,10 @@ EXPORTED_SYMBOLS = [
     "ne_build_shard_csr", "ne_set_eval_precision", "ne_enable_screening", "ne_advance_generation_screened",
     "ne_fingerprint", "ne_get_probe_states", "ne_surrogate_predict", "ne_surrogate_train_step",
     "ne_surrogate_get_params", "ne_surrogate_set_params", "ne_rank_correlation", "ne_autotune_evaluation",
-    "ne_set_eval_shape",
+    "ne_set_eval_shape", "ne_create_prebreeding_checkpoint", "ne_restore_prebreeding_checkpoint",
+    "ne_resume_generation_from_checkpoint", "ne_write_checkpoint", "ne_read_checkpoint", "ne_clone_checkpoint",
+    "ne_get_checkpoint_info", "ne_get_checkpoint_generation", "ne_get_checkpoint_plan", "ne_get_checkpoint_genome",
+    "ne_free_checkpoint", "ne_training_data_identity_hash",
 ]
 
 _lib = None
@@ -196,6 +210,20 @@ def load_library(path=LIB_PATH):
         "ne_autotune_evaluation": (i32, [p, C.POINTER(WordCounts), C.POINTER(ShardReleaseHistory), C.POINTER(u32),
                                          p]),
         "ne_set_eval_shape": (i32, [p, u32]),
+        "ne_create_prebreeding_checkpoint": (i32, [p, u32, C.POINTER(WordCounts), C.POINTER(AssemblyConfig),
+                                                   C.POINTER(PendingInjection), p, u64, u32, C.POINTER(Callbacks),
+                                                   C.POINTER(ShardReleaseHistory), C.POINTER(p)]),
+        "ne_restore_prebreeding_checkpoint": (i32, [p, p, C.POINTER(p)]),
+        "ne_resume_generation_from_checkpoint": (i32, [p, p, u32]),
+        "ne_write_checkpoint": (i32, [p, C.c_char_p]),
+        "ne_read_checkpoint": (i32, [C.c_char_p, C.POINTER(p)]),
+        "ne_clone_checkpoint": (i32, [p, C.POINTER(p)]),
+        "ne_get_checkpoint_info": (i32, [p, C.POINTER(CheckpointInfo)]),
+        "ne_get_checkpoint_generation": (i32, [p, p, p, p, p, u64]),
+        "ne_get_checkpoint_plan": (i32, [p, p, u64]),
+        "ne_get_checkpoint_genome": (i32, [p, u64, C.POINTER(u32), p, u64]),
+        "ne_free_checkpoint": (None, [p]),
+        "ne_training_data_identity_hash": (u64, [C.POINTER(WordCounts), C.POINTER(ShardReleaseHistory), p, u64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -213,6 +241,20 @@ class RuntimeError_(RuntimeError):
 
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def _history(history, fitness_p99=None):
+    """(pointer to an ne_shard_release_history or None, arrays to keep alive) for release generations and,
+    optionally, the fitness p99 at each release."""
+    if history is None:
+        return None, None
+    gens = np.ascontiguousarray(history, dtype=np.uint64)
+    p99 = None if fitness_p99 is None else np.ascontiguousarray(fitness_p99, dtype=np.float32)
+    if p99 is not None and p99.size != gens.size:
+        raise ValueError("one fitness p99 per release generation")
+    h = ShardReleaseHistory(gens.ctypes.data_as(C.POINTER(C.c_uint64)), gens.size,
+                            None if p99 is None else p99.ctypes.data_as(C.POINTER(C.c_float)))
+    return C.byref(h), (h, gens, p99)
 
 
 def default_assembly_config(**kw):
@@ -288,24 +330,55 @@ class SlabRuntime:
                                                           C.byref(ic)))
 
     def evaluate(self, counts, history=None):
-        hist = None
-        if history is not None:
-            arr = np.ascontiguousarray(history, dtype=np.uint64)
-            self._hist_keep = arr
-            hist = C.byref(ShardReleaseHistory(arr.ctypes.data_as(C.POINTER(C.c_uint64)), arr.size))
+        hist, self._hist_keep = _history(history)
         self._check("ne_evaluate", self.lib.ne_evaluate(self.h, C.byref(counts), hist))
 
     def advance(self, seed, counts, config=None, injection=None, callbacks=None, history=None):
         cfg = C.byref(config) if config is not None else None
         inj = C.byref(injection) if injection is not None else None
         cbs = C.byref(callbacks) if callbacks is not None else None
-        hist = None
-        if history is not None:
-            arr = np.ascontiguousarray(history, dtype=np.uint64)
-            self._hist_keep = arr
-            hist = C.byref(ShardReleaseHistory(arr.ctypes.data_as(C.POINTER(C.c_uint64)), arr.size))
+        hist, self._hist_keep = _history(history)
         self._check("ne_advance_generation",
                     self.lib.ne_advance_generation(self.h, seed, C.byref(counts), cfg, inj, 0, cbs, hist))
+
+    # --- checkpoint and resume
+    def create_checkpoint(self, seed, counts, config=None, injection=None, catalog=None, callbacks=None,
+                          history=None, fitness_p99=None):
+        """TryCreatePrebreedingCheckpointOnDevice: the first half of a generation step (evaluation, plan,
+        widening) and the snapshot; `resume` finishes the step."""
+        cfg = C.byref(config) if config is not None else None
+        inj = C.byref(injection) if injection is not None else None
+        cbs = C.byref(callbacks) if callbacks is not None else None
+        hist, self._hist_keep = _history(history, fitness_p99)
+        cat = None
+        n = 0
+        if catalog is not None:
+            cat = np.ascontiguousarray(catalog, dtype=np.uint8).reshape(-1, 5)
+            n = cat.shape[0]
+        h = C.c_void_p()
+        self._check("ne_create_prebreeding_checkpoint", self.lib.ne_create_prebreeding_checkpoint(
+            self.h, seed, C.byref(counts), cfg, inj, None if cat is None else _ptr(cat), n, 0, cbs, hist,
+            C.byref(h)))
+        return Checkpoint(h)
+
+    def resume(self, checkpoint):
+        """TryResumeGenerationFromCheckpointOnDevice: assembly, injection and swap from the boundary."""
+        self._check("ne_resume_generation_from_checkpoint",
+                    self.lib.ne_resume_generation_from_checkpoint(self.h, checkpoint.h, 0))
+
+    @classmethod
+    def restore(cls, checkpoint):
+        """TryRestorePrebreedingCheckpointToDevice: a new runtime at the checkpoint's boundary."""
+        self = cls.__new__(cls)
+        self.lib = load_library()
+        self.h = None
+        h = C.c_void_p()
+        st = self.lib.ne_restore_prebreeding_checkpoint(checkpoint.h, None, C.byref(h))
+        if st != 0:
+            raise RuntimeError_("ne_restore_prebreeding_checkpoint", st)
+        self.h = h
+        self.grid_columns = checkpoint.info().grid_column_count
+        return self
 
     def summary(self):
         s = FitnessSummary()
@@ -406,11 +479,7 @@ class SlabRuntime:
 
     def advance_screened(self, seed, counts, config=None, history=None):
         cfg = C.byref(config) if config is not None else None
-        hist = None
-        if history is not None:
-            arr = np.ascontiguousarray(history, dtype=np.uint64)
-            self._hist_keep = arr
-            hist = C.byref(ShardReleaseHistory(arr.ctypes.data_as(C.POINTER(C.c_uint64)), arr.size))
+        hist, self._hist_keep = _history(history)
         rep = ScreenReport()
         self._check("ne_advance_generation_screened",
                     self.lib.ne_advance_generation_screened(self.h, seed, C.byref(counts), cfg, hist, C.byref(rep)))
@@ -454,6 +523,96 @@ class SlabRuntime:
         self._check("ne_surrogate_set_params", self.lib.ne_surrogate_set_params(self.h, _ptr(params), params.size))
 
 
+class Checkpoint:
+    """A pre-recombination checkpoint (RuntimeCheckpoint, slab_runtime.hpp:98-117), host memory only.
+
+    Files are those of TryWriteRuntimeCheckpointAtomically / TryReadRuntimeCheckpoint
+    (slab_runtime.cu:1934-2081)."""
+
+    def __init__(self, handle):
+        self.lib = load_library()
+        self.h = handle
+
+    @classmethod
+    def read(cls, path):
+        lib = load_library()
+        h = C.c_void_p()
+        st = lib.ne_read_checkpoint(os.fsencode(path), C.byref(h))
+        if st != 0:
+            raise RuntimeError_("ne_read_checkpoint", st)
+        return cls(h)
+
+    def write(self, path):
+        st = self.lib.ne_write_checkpoint(self.h, os.fsencode(path))
+        if st != 0:
+            raise RuntimeError_("ne_write_checkpoint", st)
+
+    def clone(self):
+        h = C.c_void_p()
+        st = self.lib.ne_clone_checkpoint(self.h, C.byref(h))
+        if st != 0:
+            raise RuntimeError_("ne_clone_checkpoint", st)
+        return Checkpoint(h)
+
+    def info(self):
+        i = CheckpointInfo()
+        st = self.lib.ne_get_checkpoint_info(self.h, C.byref(i))
+        if st != 0:
+            raise RuntimeError_("ne_get_checkpoint_info", st)
+        return i
+
+    def generation(self):
+        """(slot_indices, fitness, evaluation_counts, has_fitness) of the current generation."""
+        n = self.info().population
+        slots, fit = np.zeros(n, np.uint32), np.zeros(n, np.float32)
+        evals, has = np.zeros(n, np.uint32), np.zeros(n, np.uint8)
+        st = self.lib.ne_get_checkpoint_generation(self.h, _ptr(slots), _ptr(fit), _ptr(evals), _ptr(has), n)
+        if st != 0:
+            raise RuntimeError_("ne_get_checkpoint_generation", st)
+        return slots, fit, evals, has
+
+    def plan(self):
+        n = self.info().child_count
+        pairs = np.zeros((n, 2), np.uint32)
+        st = self.lib.ne_get_checkpoint_plan(self.h, _ptr(pairs), n)
+        if st != 0:
+            raise RuntimeError_("ne_get_checkpoint_plan", st)
+        return pairs
+
+    def genome(self, record):
+        """(organism index, fp32 view of the record's slot bytes)."""
+        stride = self.info().slot_stride_bytes
+        out = np.zeros(stride // 4, np.float32)
+        org = C.c_uint32()
+        st = self.lib.ne_get_checkpoint_genome(self.h, record, C.byref(org), _ptr(out), stride)
+        if st != 0:
+            raise RuntimeError_("ne_get_checkpoint_genome", st)
+        return org.value, out
+
+    def close(self):
+        if self.h:
+            self.lib.ne_free_checkpoint(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def training_data_identity_hash(counts, history=None, fitness_p99=None, catalog=None):
+    """The identity hash a checkpoint records for its training data (host only)."""
+    lib = load_library()
+    hist, keep = _history(history, fitness_p99)
+    cat = None
+    n = 0
+    if catalog is not None:
+        cat = np.ascontiguousarray(catalog, dtype=np.uint8).reshape(-1, 5)
+        n = cat.shape[0]
+    return int(lib.ne_training_data_identity_hash(C.byref(counts), hist, None if cat is None else _ptr(cat), n))
+
+
 def rank_correlation(a, b):
     """Spearman rank correlation through the library's host routine (average ranks for ties)."""
     a = np.ascontiguousarray(a, dtype=np.float32)
@@ -467,10 +626,7 @@ def build_shard_csr(counts, generation_index, rows, cols, epicenter_rows=None, h
     epi = rows if epicenter_rows is None else epicenter_rows
     offsets = np.zeros(rows * cols + 1, np.uint32)
     n = C.c_uint64()
-    hist = None
-    if history is not None:
-        arr = np.ascontiguousarray(history, dtype=np.uint64)
-        hist = C.byref(ShardReleaseHistory(arr.ctypes.data_as(C.POINTER(C.c_uint64)), arr.size))
+    hist, keep = _history(history)
     st = lib.ne_build_shard_csr(C.byref(counts), hist, generation_index, rows, cols, epi, _ptr(offsets), None, 0,
                                 C.byref(n))
     if st not in (0, 17):

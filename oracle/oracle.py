@@ -83,6 +83,16 @@ def ref():
         R.ref_chebyshev.restype = U64
         R.ref_shard_csr.argtypes = [U64] * 10 + [P, U64, P, P, U64]
         R.ref_shard_csr.restype = I32
+        R.ref_word_counts_image.argtypes = [P, P]
+        R.ref_word_counts_image.restype = I32
+        R.ref_assembly_image.argtypes = [U64, P, P]
+        R.ref_assembly_image.restype = I32
+        R.ref_assembly_defaults.argtypes = [P, P]
+        R.ref_assembly_defaults.restype = None
+        R.ref_grid_shape_image.argtypes = [U64, U64, P]
+        R.ref_grid_shape_image.restype = I32
+        R.ref_checkpoint_sizes.argtypes = [P]
+        R.ref_checkpoint_sizes.restype = None
         _ref = R
     return _ref
 

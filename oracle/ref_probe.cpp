@@ -6,6 +6,8 @@
 #include <cstdint>
 #include <cstring>
 
+#include "genetic_algorithm/device/runtime_common.hpp"
+#include "genetic_algorithm/generation_assembly.hpp"
 #include "spatial/grid.hpp"
 #include "training_folder/training_data.hpp"
 #include "wordle/feedback.hpp"
@@ -98,6 +100,66 @@ int ref_shard_csr(uint64_t initial, uint64_t step, uint64_t period, uint64_t int
     }
     offsets[grid.cell_count] = static_cast<uint32_t>(n);
     return static_cast<int>(set.shard_count);
+}
+
+// Byte images of the configuration structs the checkpoint writer stores whole
+// (slab_runtime.cu:1962-1971), built from the reference's own types.
+int ref_word_counts_image(const uint64_t* v, uint8_t* out) {
+    neuroevolution::genetic_algorithm::device_common::RuntimeWordCounts w{};
+    w.training_word_count = v[0];
+    w.action_space_word_count = v[1];
+    w.training_word_schedule.initial_word_count = v[2];
+    w.training_word_schedule.word_count_step = v[3];
+    w.training_word_schedule.word_count_step_period_generations = v[4];
+    w.shard_initial_radius = v[5];
+    w.shard_radius_growth_period_generations = v[6];
+    std::memcpy(out, &w, sizeof(w));
+    return static_cast<int>(sizeof(w));
+}
+
+int ref_assembly_image(uint64_t radius, const float* f, uint8_t* out) {
+    neuroevolution::genetic_algorithm::GenerationAssemblyConfig c{};
+    std::memset(&c, 0, sizeof(c));
+    c.parent_selection.cellular_breeding_radius = radius;
+    c.parent_selection.rank_exponent = f[0];
+    c.breeding.crossover_temperature_level1 = f[1];
+    c.breeding.crossover_temperature_level2 = f[2];
+    c.breeding.crossover_temperature_level3 = f[3];
+    c.mutation.mutation_probability = f[4];
+    c.mutation.mutation_sigma = f[5];
+    c.mutation.output_tail_row_scale_mutation_probability = f[6];
+    std::memcpy(out, &c, sizeof(c));
+    return static_cast<int>(sizeof(c));
+}
+
+// The defaults of a value-initialised GenerationAssemblyConfig: radius, then the 7 floats.
+void ref_assembly_defaults(uint64_t* radius, float* f) {
+    const neuroevolution::genetic_algorithm::GenerationAssemblyConfig c{};
+    *radius = c.parent_selection.cellular_breeding_radius;
+    f[0] = c.parent_selection.rank_exponent;
+    f[1] = c.breeding.crossover_temperature_level1;
+    f[2] = c.breeding.crossover_temperature_level2;
+    f[3] = c.breeding.crossover_temperature_level3;
+    f[4] = c.mutation.mutation_probability;
+    f[5] = c.mutation.mutation_sigma;
+    f[6] = c.mutation.output_tail_row_scale_mutation_probability;
+}
+
+int ref_grid_shape_image(uint64_t rows, uint64_t cols, uint8_t* out) {
+    sp::CellularGridShape g{};
+    if (!sp::TryMakeRectangularCellularGridShape(rows, cols, g)) return -1;
+    std::memcpy(out, &g, sizeof(g));
+    return static_cast<int>(sizeof(g));
+}
+
+// sizeof(wordle::Word), of a release-history entry (generation, p99), and the history capacity.
+void ref_checkpoint_sizes(uint64_t* out) {
+    const tf::TrainingDataShardReleaseHistory h{};
+    out[0] = sizeof(wd::Word);
+    out[1] = sizeof(h.release_generations[0]);
+    out[2] = sizeof(h.release_fitness_p99[0]);
+    out[3] = sizeof(h.release_count);
+    out[4] = tf::kTrainingWordCatalogCapacity;
 }
 
 }  // extern "C"

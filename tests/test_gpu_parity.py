@@ -358,13 +358,16 @@ def test_widening_repack_and_injection(pkg, orc):
     rt.advance(606, rtm.word_counts(A0, A0), cfg, injection=injection)
     info = rt.info()
     assert info.action_count == A0 + inj and info.slot_stride_bytes == s1
+    # the slab layout's byte size is slot count x stride (device_runtime.cu:1047-1050) and the widened
+    # layout keeps it (TryCreateExpandedSlabLayout, layout.hpp:55-62)
+    slots1 = (slab_bytes // s0 * s0) // s1
+    assert info.slot_count == slots1 and info.slab_bytes == slab_bytes // s0 * s0
     pairs = rt.plan()
     children, _, _, _ = rt.generation(0)
     # slot bookkeeping: repack then assembly
-    rep = orc.repack_slots(slab_bytes // s1, slots0, pairs)
+    rep = orc.repack_slots(slots1, slots0, pairs)
     assert rep["status"] == 0
-    asm = orc.assemble_slots(slab_bytes // s1, rep["stack"], rep["free_count"], rep["state"], rep["parent_slots"],
-                             pairs)
+    asm = orc.assemble_slots(slots1, rep["stack"], rep["free_count"], rep["state"], rep["parent_slots"], pairs)
     assert asm["status"] == 0 and children.tolist() == asm["child_slots"].tolist()
     # the generation byte budget was fixed at the old stride, so the population shrinks to whole
     # rows that fit at the new stride (TryPlanNextGenerationShape, slab_runtime.cu:919-939)
