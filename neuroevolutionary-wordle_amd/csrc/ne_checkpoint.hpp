@@ -1,9 +1,10 @@
 // ne_checkpoint.hpp -- the in-memory checkpoint and its file format (host only).
 //
-// The file is byte-compatible with the reference's RuntimeCheckpoint writer and reader
-// (src/genetic_algorithm/device/slab_runtime.cu:1934-2081): the same field order, little-endian
-// scalars, and the LP64 images of the reference's configuration structs written whole (padding
-// included), so a checkpoint written by either side reads on the other.
+// The container is the reference's RuntimeCheckpoint file (src/genetic_algorithm/device/
+// slab_runtime.cu:1934-2081): the same header, field order, little-endian scalars, LP64 images of the
+// reference's configuration structs written whole (padding included) and the same checksum.  The genome
+// records differ: they hold this build's fp32 slots (the reference stores fp16 parameters), so the file
+// carries its own genome layout version and each side's reader refuses the other's genomes.
 #pragma once
 
 #include <cstdint>
@@ -16,7 +17,9 @@ namespace ne_ckpt {
 
 constexpr uint64_t kMagic = 0x4B4350474147524Eull;  // slab_runtime.cu:1184 ("NRGAGPCK" as bytes)
 constexpr uint32_t kSchemaVersion = 3;              // kRuntimeCheckpointSchemaVersion (slab_runtime.hpp:86)
-constexpr uint32_t kGenomeLayoutVersion = 1;        // kRuntimeCheckpointGenomeLayoutVersion (slab_runtime.hpp:87)
+// kRuntimeCheckpointGenomeLayoutVersion (slab_runtime.hpp:87) is 1 for the reference's fp16 genomes; fp32
+// slots with the 145->64 encoder are "FP32" (these four bytes in file order).
+constexpr uint32_t kGenomeLayoutVersion = 0x32335046u;
 constexpr uint32_t kPhasePreRecombination = 1;      // RuntimeCheckpointResumePhase (slab_runtime.hpp:89-91)
 constexpr uint64_t kReleaseCapacity = 4739;         // kTrainingWordCatalogCapacity (training_data.hpp:22)
 

@@ -5,13 +5,15 @@ src/genetic_algorithm/device/slab_runtime.cu: the checksum ComputeRuntimeCheckpo
 the identity hash of TryDownloadBoundaryCheckpointPayload (:1282-1296), the writer
 TryWriteRuntimeCheckpointAtomically (:1934-2007) and the reader TryReadRuntimeCheckpoint (:2009-2081).
 The configuration structs are written whole as their LP64 images; their sizes and offsets are pinned
-against the reference headers compiled into oracle/_ref (tests/test_checkpoint_format.py).
+against the reference headers compiled into oracle/_ref (tests/test_checkpoint_format.py).  The genome
+records hold fp32 slots, which the genome layout version says ("FP32"; the reference's fp16 genomes are 1).
 """
 import struct
 
 MAGIC = 0x4B4350474147524E          # slab_runtime.cu:1184
 SCHEMA_VERSION = 3                  # slab_runtime.hpp:86
-GENOME_LAYOUT_VERSION = 1           # slab_runtime.hpp:87
+GENOME_LAYOUT_VERSION = 0x32335046  # b"FP32": fp32 genome slots (the reference's fp16 layout is 1, slab_runtime.hpp:87)
+REFERENCE_GENOME_LAYOUT_VERSION = 1
 PHASE_PRE_RECOMBINATION = 1         # slab_runtime.hpp:89-91
 RELEASE_CAPACITY = 4739             # training_data.hpp:22
 FNV_PRIME = 1099511628211

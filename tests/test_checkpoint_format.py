@@ -58,7 +58,8 @@ def _synthetic(seed=1, n=12, children=8, releases=3, holes=(3,)):
     for h in holes:
         slots[h] = 0xFFFFFFFF  # an organism without a slot has no genome record
     c = {
-        "schema_version": 3, "genome_layout_version": 1, "resume_phase": 1, "generation_seed": 0xDEADBEEF,
+        "schema_version": 3, "genome_layout_version": cko.GENOME_LAYOUT_VERSION, "resume_phase": 1,
+        "generation_seed": 0xDEADBEEF,
         "word_counts": cko.pack_word_counts(40, A, 32, 8, 5, 1, 2),
         "release_generations": list(range(5, 5 + 5 * releases, 5)),
         "release_p99": np.float32(rng.random(releases)).tolist(),
@@ -93,7 +94,8 @@ def test_library_reads_the_oracle_file_and_writes_it_back(pkg, tmp_path):
     (tmp_path / "in.ckpt").write_bytes(data)
     ck = rtm.Checkpoint.read(str(tmp_path / "in.ckpt"))
     i = ck.info()
-    assert (i.schema_version, i.genome_layout_version, i.resume_phase, i.generation_seed) == (3, 1, 1, 0xDEADBEEF)
+    assert (i.schema_version, i.genome_layout_version, i.resume_phase, i.generation_seed) == (
+        3, cko.GENOME_LAYOUT_VERSION, 1, 0xDEADBEEF)
     assert i.checksum == c["checksum"] and i.training_data_identity_hash == c["identity_hash"]
     assert (i.generation_index, i.population, i.child_count, i.live_genotypes) == (7, 12, 8, 11)
     assert (i.action_count, i.slot_stride_bytes, i.slot_count, i.slab_bytes) == (A, STRIDE, 40, 40 * STRIDE)
@@ -151,7 +153,7 @@ def _corruptions():
         "truncated": good[:-100],
         "trailing header only": good[:40],
         "schema 2": field(base, schema_version=2),
-        "genome layout 2": field(base, genome_layout_version=2),
+        "the reference's fp16 genome layout": field(base, genome_layout_version=cko.REFERENCE_GENOME_LAYOUT_VERSION),
         "release history over capacity": cko.write(too_many),
         "empty generation": field(base, slots=[], fitness=[], evals=[], has=[], records=[]),
         "empty plan": field(base, pairs=[]),

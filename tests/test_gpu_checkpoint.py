@@ -207,7 +207,8 @@ def test_restore_rejects_inconsistent_checkpoints(pkg, tmp_path):
 
     def make(**kw):
         n, cols = 4, 2
-        c = {"schema_version": 3, "genome_layout_version": 1, "resume_phase": 1, "generation_seed": 1,
+        c = {"schema_version": 3, "genome_layout_version": cko.GENOME_LAYOUT_VERSION, "resume_phase": 1,
+             "generation_seed": 1,
              "identity_hash": 0, "word_counts": cko.pack_word_counts(A, A, A, 0, 1, 0, 2),
              "release_generations": [], "release_p99": [],
              "assembly": cko.pack_assembly(2, 0.5, 0.02, 0.01, 0.005, 1e-4, 0.02, 0.0),
