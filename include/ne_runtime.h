@@ -421,6 +421,85 @@ void ne_free_checkpoint(ne_checkpoint *checkpoint);
 uint64_t ne_training_data_identity_hash(const ne_word_counts *counts, const ne_shard_release_history *history,
                                         const uint8_t *catalog_letters, uint64_t catalog_word_count);
 
+/* ---- generation telemetry and the curriculum controller ------------------------------------
+ * The statistics the reference's driver logs and releases training shards by
+ * (src/cli/run_genetic_algorithm.cu:594-624, 941-1286), computed where the generation lives. */
+
+/* GenerationFitnessTelemetryRow (run_genetic_algorithm.cu:594-610). */
+typedef struct ne_fitness_telemetry {
+    uint64_t generation;
+    uint64_t population_size;
+    uint64_t training_word_count;
+    float fitness_min;
+    float fitness_mean;
+    float fitness_median;
+    float fitness_p90;                   /* nearest rank */
+    float fitness_p99;                   /* nearest rank */
+    float fitness_max;
+    float fitness_stddev;                /* population standard deviation */
+    uint32_t has_breeding_metrics;
+    uint64_t distinct_fitness_count;
+    uint64_t parent_childless_count;     /* parents the plan gives no child */
+    uint64_t parent_one_child_count;
+    uint64_t parent_multiple_children_count;
+} ne_fitness_telemetry;
+
+/* TryCollectGenerationFitnessTelemetry (run_genetic_algorithm.cu:1080-1186) on the current generation,
+ * which must be evaluated.  Breeding metrics need the next generation's plan (from the post-fitness
+ * callback on).  At most 16,384 organisms (NE_UNSUPPORTED above). */
+int ne_collect_fitness_telemetry(ne_runtime *rt, const ne_word_counts *counts, uint32_t collect_breeding_metrics,
+                                 ne_fitness_telemetry *out);
+
+/* GeneticConvergenceTelemetryRow (run_genetic_algorithm.cu:612-624). */
+typedef struct ne_convergence_telemetry {
+    uint64_t generation;
+    uint64_t sample_organisms;
+    uint64_t sample_weights;
+    uint64_t pair_count;
+    float centroid_distance_mean;
+    float centroid_distance_min;
+    float centroid_distance_max;
+    float pairwise_distance_mean;
+    float pairwise_distance_min;
+    float pairwise_distance_max;
+    float elapsed_ms;
+    float reserved0;
+} ne_convergence_telemetry;
+
+/* TryCollectGeneticConvergenceTelemetry (run_genetic_algorithm.cu:941-1052): up to 256 organisms x 8,192
+ * trainable scalars x 512 organism pairs sampled by MakeSamplePlan (genetic_convergence.hpp:87-117) from
+ * the run seed and the generation index; centroid and pairwise L2 distances over the sample. */
+int ne_collect_convergence_telemetry(ne_runtime *rt, uint32_t run_seed, ne_convergence_telemetry *out);
+
+/* MakeSamplePlan (genetic_convergence.hpp:87-117), host only: organism indices (<= 256), weight indices
+ * (<= 8,192) and organism pairs ([512][2], indices into the organism sample).  Any output may be NULL;
+ * counts receive the three lengths. */
+int ne_convergence_sample_plan(uint64_t population, uint64_t trainable_scalars, uint32_t run_seed,
+                               uint64_t generation, uint32_t *organisms, uint32_t *weights, uint32_t *pairs,
+                               uint64_t counts[3]);
+
+/* The adaptive shard-release thresholds of the reference CLI (training_data.hpp:17-20 defaults). */
+typedef struct ne_shard_release_policy {
+    uint64_t min_gap_generations;        /* 3: generations between releases */
+    uint64_t first_new_release_generation; /* 10: the first non-foundation release comes no earlier */
+    float centroid_distance_threshold;   /* 6.0: release when the centroid distance mean is <= this ... */
+    float fitness_p99_threshold;         /* 0.20: ... and the fitness p99 is > this */
+    uint32_t run_seed;                   /* seeds the convergence sample */
+    uint32_t reserved0;
+} ne_shard_release_policy;
+
+/* TryMaybeReleaseTrainingDataShard (run_genetic_algorithm.cu:1200-1286), for the training-shard release
+ * callback: on the evaluated current generation, decides whether the next catalog words are released for
+ * the next generation.  On a release *injection is set for them, (next generation, fitness p99) is
+ * appended to the caller's history arrays (capacity entries; the foundation record must be there) and
+ * *released is 1; otherwise *injection is cleared.  *next_word_count is the action count afterwards. */
+int ne_maybe_release_training_shard(ne_runtime *rt, const ne_word_counts *counts,
+                                    const ne_shard_release_policy *policy, uint64_t catalog_word_count,
+                                    uint64_t *release_generations, float *release_fitness_p99,
+                                    uint64_t *release_count, uint64_t release_capacity,
+                                    ne_pending_injection *injection, uint64_t *next_word_count,
+                                    uint32_t *released);
+
 #ifdef __cplusplus
 }
 #endif

@@ -249,4 +249,18 @@ void LaunchRepackCopies(uint8_t* slab, uint64_t old_stride, uint64_t new_stride,
                         uint32_t survivors, uint32_t base, uint8_t* staging, hipStream_t s);
 void LaunchFillU32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t s);
 
+// ---- generation telemetry (ne_telemetry.hip) ----
+constexpr uint32_t kFitnessStatsMax = 16384;  // generation sizes the LDS sort holds (64 KB)
+struct FitnessStatsRecord {
+    double sum;             // ascending-order double sum of the fitness
+    double squared_deltas;  // ascending-order double sum of (fitness - mean)^2
+    float min, max, median_low, median_high, p90, p99;
+    uint32_t distinct, missing, childless, one_child, multiple_children;
+};
+void LaunchFitnessStats(const float* fit, const uint8_t* has, uint32_t n, uint32_t pow2, uint32_t i90, uint32_t i99,
+                        const uint32_t* refcounts, FitnessStatsRecord* out, hipStream_t s);
+void LaunchConvergence(const uint8_t* slab, uint64_t stride, const uint32_t* slots, uint32_t n_org,
+                       const uint32_t* weights, uint32_t n_w, uint64_t scalars, const uint32_t* pairs, uint32_t n_pairs,
+                       float inverse, float* by_org, float* by_weight, float* centroid, float* squared, hipStream_t s);
+
 }  // namespace ne

@@ -3,12 +3,18 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <numeric>
+#include <random>
 #include <unordered_map>
+#include <unordered_set>
+#include <utility>
 #include <vector>
 
 #include "../../include/ne_runtime.h"
@@ -153,6 +159,14 @@ struct ne_runtime {
     float* sg_coef1 = nullptr;  // {c1, c2} of a direct ne_surrogate_train_step
     hipGraph_t train_graph = nullptr;       // one generation's training steps, captured once
     hipGraphExec_t train_exec = nullptr;
+
+    // telemetry
+    FitnessStatsRecord* d_fstats = nullptr;
+    float* cv_by_org = nullptr;     // [256][8192] sampled scalars, organism-major
+    float* cv_by_weight = nullptr;  // the same, weight-major
+    float* cv_centroid = nullptr;   // [8192]
+    float* cv_squared = nullptr;    // [256 + 512] squared distances
+    uint32_t* cv_index = nullptr;   // [256] slots, [8192] scalar indices, [512][2] pairs
 };
 
 namespace {
@@ -1274,6 +1288,12 @@ void ne_destroy(ne_runtime* rt) {
     Free(rt->bidx);
     Free(rt->sg_coef);
     Free(rt->sg_coef1);
+    Free(rt->d_fstats);
+    Free(rt->cv_by_org);
+    Free(rt->cv_by_weight);
+    Free(rt->cv_centroid);
+    Free(rt->cv_squared);
+    Free(rt->cv_index);
     if (rt->train_exec) (void)hipGraphExecDestroy(rt->train_exec);
     if (rt->train_graph) (void)hipGraphDestroy(rt->train_graph);
     Free(rt->owner);
@@ -2290,6 +2310,264 @@ int ne_resume_generation_from_checkpoint(ne_runtime* rt, const ne_checkpoint* c,
                         parent_a + inj.injection_count > rt->catalog_count))
         return Fail(rt, NE_OUTPUT_EMBEDDING_INJECTION_FAILED);
     return AssemblePrepared(rt, c->generation_seed, cfg, parent_a, inj);
+}
+
+}  // extern "C"
+
+// ---- generation telemetry and the curriculum controller (host side) ----------------------------
+
+constexpr uint64_t kSampleOrganisms = 256;  // kDefaultSampleOrganismCount (genetic_convergence.hpp:18)
+constexpr uint64_t kSampleWeights = 8192;   // kDefaultSampleWeightCount
+constexpr uint64_t kSamplePairs = 512;      // kDefaultPairCount
+
+// MixSamplingSeed (genetic_convergence.hpp:48-57): a hash-combine of the generation into the run seed,
+// then the SplitMix64 finaliser.
+static uint64_t MixSamplingSeed(uint32_t run_seed, uint64_t generation) {
+    uint64_t v = 0x9E3779B97F4A7C15ull ^ uint64_t(run_seed);
+    v ^= generation + 0xBF58476D1CE4E5B9ull + (v << 6) + (v >> 2);
+    v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
+    v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
+    return v ^ (v >> 31);
+}
+
+// SampleUniqueIndices (genetic_convergence.hpp:59-83): Floyd's sampling of `requested` distinct indices
+// below `universe` in draw order (all of them, in order, when requested >= universe).
+static std::vector<uint64_t> SampleUnique(uint64_t universe, uint64_t requested, std::mt19937_64& rng) {
+    const uint64_t k = std::min(requested, universe);
+    std::vector<uint64_t> out;
+    out.reserve(k);
+    if (k == universe) {
+        out.resize(k);
+        std::iota(out.begin(), out.end(), uint64_t{0});
+        return out;
+    }
+    std::unordered_set<uint64_t> taken;
+    taken.reserve(2 * k);
+    for (uint64_t cursor = universe - k; cursor < universe; ++cursor) {
+        std::uniform_int_distribution<uint64_t> draw(0, cursor);
+        const uint64_t c = draw(rng);
+        const uint64_t v = taken.count(c) ? cursor : c;
+        taken.insert(v);
+        out.push_back(v);
+    }
+    return out;
+}
+
+struct SamplePlan {
+    std::vector<uint64_t> organisms, weights;
+    std::vector<uint32_t> pairs;  // [pair][2] indices into `organisms`
+};
+
+// MakeSamplePlan (genetic_convergence.hpp:87-117); the draws come from std::mt19937_64 and
+// std::uniform_int_distribution, so the plan is the reference's under the same C++ standard library.
+static SamplePlan MakeSamplePlan(uint64_t population, uint64_t scalars, uint32_t run_seed, uint64_t generation) {
+    SamplePlan p;
+    if (population == 0 || scalars == 0) return p;
+    std::mt19937_64 rng(MixSamplingSeed(run_seed, generation));
+    p.organisms = SampleUnique(population, kSampleOrganisms, rng);
+    p.weights = SampleUnique(scalars, kSampleWeights, rng);
+    if (p.organisms.size() < 2) return p;
+    std::uniform_int_distribution<uint64_t> first(0, p.organisms.size() - 1), second(0, p.organisms.size() - 2);
+    p.pairs.reserve(2 * kSamplePairs);
+    for (uint64_t i = 0; i < kSamplePairs; ++i) {
+        const uint64_t a = first(rng);
+        uint64_t b = second(rng);
+        if (b >= a) ++b;
+        p.pairs.push_back(uint32_t(a));
+        p.pairs.push_back(uint32_t(b));
+    }
+    return p;
+}
+
+extern "C" {
+
+int ne_convergence_sample_plan(uint64_t population, uint64_t trainable_scalars, uint32_t run_seed,
+                               uint64_t generation, uint32_t* organisms, uint32_t* weights, uint32_t* pairs,
+                               uint64_t counts[3]) {
+    if (!counts || trainable_scalars > 0xFFFFFFFFull || population > 0xFFFFFFFFull) return NE_INVALID_ARGUMENT;
+    const SamplePlan p = MakeSamplePlan(population, trainable_scalars, run_seed, generation);
+    counts[0] = p.organisms.size();
+    counts[1] = p.weights.size();
+    counts[2] = p.pairs.size() / 2;
+    if (organisms)
+        for (size_t i = 0; i < p.organisms.size(); ++i) organisms[i] = uint32_t(p.organisms[i]);
+    if (weights)
+        for (size_t i = 0; i < p.weights.size(); ++i) weights[i] = uint32_t(p.weights[i]);
+    if (pairs) std::copy(p.pairs.begin(), p.pairs.end(), pairs);
+    return NE_OK;
+}
+
+int ne_collect_fitness_telemetry(ne_runtime* rt, const ne_word_counts* counts, uint32_t collect_breeding_metrics,
+                                 ne_fitness_telemetry* out) {
+    if (!rt || !counts || !out) return Fail(rt, NE_INVALID_ARGUMENT);
+    const uint64_t n = rt->cur_size;
+    if (n == 0) return Fail(rt, NE_INVALID_GENERATION);
+    if (n > kFitnessStatsMax) return Fail(rt, NE_UNSUPPORTED);
+    if (collect_breeding_metrics && rt->planned_children == 0) return Fail(rt, NE_INVALID_ASSEMBLY_PLAN);
+    if (!rt->d_fstats && !Alloc(&rt->d_fstats, 1)) return Fail(rt, NE_DEVICE_FAILURE);
+    uint32_t pow2 = 2;
+    while (pow2 < n) pow2 <<= 1;
+    // nearest-rank percentile (run_genetic_algorithm.cu:1127-1132)
+    auto rank_index = [n](double percentile) {
+        const uint64_t rank = uint64_t(std::ceil((percentile / 100.0) * double(n)));
+        const uint64_t index = rank == 0 ? 0 : rank - 1;
+        return uint32_t(index < n ? index : n - 1);
+    };
+    const uint32_t* refcounts = nullptr;
+    if (collect_breeding_metrics) {
+        if (!Ok(hipMemsetAsync(rt->refcounts, 0, rt->max_gen * sizeof(uint32_t), rt->stream)))
+            return Fail(rt, NE_DEVICE_FAILURE);
+        LaunchRefcount(rt->pairs, uint32_t(rt->planned_children), uint32_t(n), rt->refcounts, rt->status, rt->stream);
+        refcounts = rt->refcounts;
+    }
+    LaunchFitnessStats(rt->cur_fit, rt->cur_has, uint32_t(n), pow2, rank_index(90.0), rank_index(99.0), refcounts,
+                       rt->d_fstats, rt->stream);
+    FitnessStatsRecord r{};
+    if (!Ok(hipMemcpyAsync(&r, rt->d_fstats, sizeof(r), hipMemcpyDeviceToHost, rt->stream)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    const int st = Finish(rt);
+    if (st != NE_OK) return st;
+    if (r.missing != 0) return Fail(rt, NE_POPULATION_NOT_EVALUATED);
+    ne_fitness_telemetry t{};
+    t.generation = rt->cur_index;
+    t.population_size = n;
+    t.training_word_count = counts->training_word_count;
+    t.fitness_min = r.min;
+    t.fitness_mean = float(r.sum / double(n));
+    t.fitness_median = n % 2 == 0 ? (r.median_low + r.median_high) * 0.5f : r.median_high;
+    t.fitness_p90 = r.p90;
+    t.fitness_p99 = r.p99;
+    t.fitness_max = r.max;
+    t.fitness_stddev = float(std::sqrt(r.squared_deltas / double(n)));
+    t.distinct_fitness_count = r.distinct;
+    if (collect_breeding_metrics) {
+        t.has_breeding_metrics = 1;
+        t.parent_childless_count = r.childless;
+        t.parent_one_child_count = r.one_child;
+        t.parent_multiple_children_count = r.multiple_children;
+    }
+    *out = t;
+    return NE_OK;
+}
+
+int ne_collect_convergence_telemetry(ne_runtime* rt, uint32_t run_seed, ne_convergence_telemetry* out) {
+    if (!rt || !out) return Fail(rt, NE_INVALID_ARGUMENT);
+    const auto start = std::chrono::steady_clock::now();
+    const uint64_t n = rt->cur_size;
+    if (n == 0) return Fail(rt, NE_INVALID_GENERATION);
+    const uint64_t scalars = GenomeValueCount(rt->action_count);
+    const SamplePlan plan = MakeSamplePlan(n, scalars, run_seed, rt->cur_index);
+    if (plan.organisms.empty() || plan.weights.empty()) return Fail(rt, NE_INVALID_GENERATION);
+    const uint32_t n_org = uint32_t(plan.organisms.size()), n_w = uint32_t(plan.weights.size());
+    const uint32_t n_pairs = uint32_t(plan.pairs.size() / 2);
+    std::vector<uint32_t> slots(n);
+    if (!Ok(hipMemcpyAsync(slots.data(), rt->cur_slots, n * 4, hipMemcpyDeviceToHost, rt->stream)) ||
+        !Ok(hipStreamSynchronize(rt->stream)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    // one host block: sampled slots, scalar indices, pairs
+    std::vector<uint32_t> index(kSampleOrganisms + kSampleWeights + 2 * kSamplePairs, 0u);
+    for (uint32_t i = 0; i < n_org; ++i) {
+        const uint32_t slot = slots[plan.organisms[i]];
+        if (slot == kInvalidSlot || slot >= rt->slot_count) return Fail(rt, NE_INVALID_SLAB);
+        index[i] = slot;
+    }
+    for (uint32_t i = 0; i < n_w; ++i) index[kSampleOrganisms + i] = uint32_t(plan.weights[i]);
+    std::copy(plan.pairs.begin(), plan.pairs.end(), index.begin() + kSampleOrganisms + kSampleWeights);
+    if (!rt->cv_index) {
+        bool ok = Alloc(&rt->cv_index, index.size()) && Alloc(&rt->cv_by_org, kSampleOrganisms * kSampleWeights) &&
+                  Alloc(&rt->cv_by_weight, kSampleOrganisms * kSampleWeights) &&
+                  Alloc(&rt->cv_centroid, kSampleWeights) && Alloc(&rt->cv_squared, kSampleOrganisms + kSamplePairs);
+        if (!ok) return Fail(rt, NE_DEVICE_FAILURE);
+    }
+    if (!Ok(hipMemcpyAsync(rt->cv_index, index.data(), index.size() * 4, hipMemcpyHostToDevice, rt->stream)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    const float inverse = 1.0f / float(n_org);
+    LaunchConvergence(rt->slab, rt->stride, rt->cv_index, n_org, rt->cv_index + kSampleOrganisms, n_w, scalars,
+                      rt->cv_index + kSampleOrganisms + kSampleWeights, n_pairs, inverse, rt->cv_by_org,
+                      rt->cv_by_weight, rt->cv_centroid, rt->cv_squared, rt->stream);
+    std::vector<float> squared(n_org + n_pairs);
+    if (!Ok(hipMemcpyAsync(squared.data(), rt->cv_squared, squared.size() * 4, hipMemcpyDeviceToHost, rt->stream)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    const int st = Finish(rt);
+    if (st != NE_OK) return st;
+    // the rest of TryComputeMetrics (genetic_convergence.hpp:146-191): square roots, float sums in order
+    ne_convergence_telemetry t{};
+    t.generation = rt->cur_index;
+    t.sample_organisms = n_org;
+    t.sample_weights = n_w;
+    t.pair_count = n_pairs;
+    float sum = 0.0f;
+    t.centroid_distance_min = FLT_MAX;
+    for (uint32_t o = 0; o < n_org; ++o) {
+        const float d = std::sqrt(squared[o]);
+        sum += d;
+        t.centroid_distance_min = std::min(t.centroid_distance_min, d);
+        t.centroid_distance_max = std::max(t.centroid_distance_max, d);
+    }
+    t.centroid_distance_mean = sum * inverse;
+    if (n_pairs == 0) {
+        t.pairwise_distance_min = 0.0f;
+    } else {
+        sum = 0.0f;
+        t.pairwise_distance_min = FLT_MAX;
+        for (uint32_t p = 0; p < n_pairs; ++p) {
+            const float d = std::sqrt(squared[n_org + p]);
+            sum += d;
+            t.pairwise_distance_min = std::min(t.pairwise_distance_min, d);
+            t.pairwise_distance_max = std::max(t.pairwise_distance_max, d);
+        }
+        t.pairwise_distance_mean = sum / float(n_pairs);
+    }
+    t.elapsed_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - start).count();
+    *out = t;
+    return NE_OK;
+}
+
+int ne_maybe_release_training_shard(ne_runtime* rt, const ne_word_counts* counts,
+                                    const ne_shard_release_policy* policy, uint64_t catalog_word_count,
+                                    uint64_t* release_generations, float* release_fitness_p99,
+                                    uint64_t* release_count, uint64_t release_capacity,
+                                    ne_pending_injection* injection, uint64_t* next_word_count,
+                                    uint32_t* released) {
+    if (!rt || !counts || !policy || !release_count || !injection || !next_word_count || !released)
+        return Fail(rt, NE_INVALID_ARGUMENT);
+    *injection = ne_pending_injection{};
+    *next_word_count = counts->action_space_word_count;
+    *released = 0;
+    if (*release_count == 0 || !release_generations || !release_fitness_p99)
+        return Fail(rt, NE_INVALID_TRAINING_SHARD);  // no foundation release record
+    if (counts->action_space_word_count >= catalog_word_count || counts->training_word_schedule.word_count_step == 0)
+        return NE_OK;
+    const uint64_t latest = *release_count - 1;
+    const uint64_t release_generation = rt->cur_index + 1;
+    const uint64_t previous = release_generations[latest];
+    const uint64_t since = release_generation >= previous ? release_generation - previous : 0;
+    if (*release_count == 1 && release_generation < policy->first_new_release_generation) return NE_OK;
+    if (since < policy->min_gap_generations) return NE_OK;
+    ne_fitness_telemetry fitness{};
+    int st = ne_collect_fitness_telemetry(rt, counts, 0, &fitness);
+    if (st != NE_OK) return st;
+    ne_convergence_telemetry convergence{};
+    st = ne_collect_convergence_telemetry(rt, policy->run_seed, &convergence);
+    if (st != NE_OK) return st;
+    const bool centroid_triggered = convergence.centroid_distance_mean <= policy->centroid_distance_threshold;
+    const bool fitness_triggered = fitness.fitness_p99 > policy->fitness_p99_threshold;
+    if (!centroid_triggered || !fitness_triggered) return NE_OK;
+    const uint64_t first = counts->action_space_word_count;
+    const uint64_t count = std::min(catalog_word_count - first, counts->training_word_schedule.word_count_step);
+    injection->enabled = 1;
+    injection->first_catalog_word_index = first;
+    injection->injection_count = count;
+    *next_word_count = first + count;
+    // TryRecordTrainingDataShardRelease (training_data.hpp:119-133)
+    if (*release_count >= std::min<uint64_t>(release_capacity, ne_ckpt::kReleaseCapacity) ||
+        release_generation <= release_generations[latest])
+        return Fail(rt, NE_INVALID_TRAINING_SHARD);
+    release_generations[*release_count] = release_generation;
+    release_fitness_p99[*release_count] = fitness.fitness_p99;
+    ++*release_count;
+    *released = 1;
+    return NE_OK;
 }
 
 }  // extern "C"

@@ -490,6 +490,67 @@ class RuntimeCheckpointAsyncWriter {
     std::future<bool> pending_{};
 };
 
+// ---- generation telemetry and the adaptive shard release (run_genetic_algorithm.cu:594-624, 941-1286) ----
+using GenerationFitnessTelemetryRow = ne_fitness_telemetry;
+using GeneticConvergenceTelemetryRow = ne_convergence_telemetry;
+
+// TryCollectGenerationFitnessTelemetry (run_genetic_algorithm.cu:1080-1186)
+inline bool TryCollectGenerationFitnessTelemetry(const DeviceSlabGARuntimeBuffers& buffers,
+                                                 const RuntimeWordCounts& counts, bool collect_breeding_metrics,
+                                                 GenerationFitnessTelemetryRow& row_out) {
+    const ne_word_counts c = detail::ToC(counts);
+    row_out = {};
+    return ne_collect_fitness_telemetry(buffers.handle, &c, collect_breeding_metrics ? 1u : 0u, &row_out) == NE_OK;
+}
+
+// TryCollectGeneticConvergenceTelemetry (run_genetic_algorithm.cu:941-1052)
+inline bool TryCollectGeneticConvergenceTelemetry(const DeviceSlabGARuntimeBuffers& buffers, std::uint32_t run_seed,
+                                                  GeneticConvergenceTelemetryRow& row_out) {
+    row_out = {};
+    return ne_collect_convergence_telemetry(buffers.handle, run_seed, &row_out) == NE_OK;
+}
+
+// The CLI thresholds of TryMaybeReleaseTrainingDataShard (training_data.hpp:17-20 defaults).
+struct ShardReleasePolicy {
+    std::size_t shard_release_min_gap_generations = 3;
+    std::size_t first_new_shard_release_generation = 10;
+    float shard_release_centroid_distance_threshold = 6.0f;
+    float shard_release_fitness_p99_threshold = 0.20f;
+    std::uint32_t seed = 0;
+};
+
+// TryMaybeReleaseTrainingDataShard (run_genetic_algorithm.cu:1200-1286), for the training-shard release
+// callback; `history` must hold the foundation record and grows by one entry per release.
+inline bool TryMaybeReleaseTrainingDataShard(const DeviceSlabGARuntimeBuffers& buffers,
+                                             const RuntimeWordCounts& counts, const ShardReleasePolicy& policy,
+                                             std::size_t catalog_word_count, TrainingDataShardReleaseHistory& history,
+                                             PendingOutputEmbeddingInjection& injection,
+                                             std::size_t& next_word_count_out) {
+    const ne_word_counts c = detail::ToC(counts);
+    ne_shard_release_policy p{};
+    p.min_gap_generations = policy.shard_release_min_gap_generations;
+    p.first_new_release_generation = policy.first_new_shard_release_generation;
+    p.centroid_distance_threshold = policy.shard_release_centroid_distance_threshold;
+    p.fitness_p99_threshold = policy.shard_release_fitness_p99_threshold;
+    p.run_seed = policy.seed;
+    history.release_fitness_p99.resize(history.release_generations.size(), 0.0f);
+    std::uint64_t count = history.release_generations.size();
+    history.release_generations.resize(count + 1);
+    history.release_fitness_p99.resize(count + 1);
+    ne_pending_injection inj{};
+    std::uint64_t next = 0;
+    std::uint32_t released = 0;
+    const int st = ne_maybe_release_training_shard(buffers.handle, &c, &p, catalog_word_count,
+                                                   history.release_generations.data(),
+                                                   history.release_fitness_p99.data(), &count, count + 1, &inj,
+                                                   &next, &released);
+    history.release_generations.resize(count);
+    history.release_fitness_p99.resize(count);
+    injection = {inj.enabled != 0, inj.first_catalog_word_index, inj.injection_count};
+    next_word_count_out = next;
+    return st == NE_OK;
+}
+
 // TryDownloadSlabSlotBytesFromDevice (slab_runtime.hpp:128-129)
 inline bool TryDownloadSlabSlotBytesFromDevice(const DeviceSlabGARuntimeBuffers& buffers, std::uint32_t slot_index,
                                                std::vector<std::uint8_t>& slot_bytes) {

@@ -140,6 +140,37 @@ class CheckpointInfo(C.Structure):
                 [("word_counts", WordCounts), ("assembly_config", AssemblyConfig), ("injection", PendingInjection)])
 
 
+class FitnessTelemetry(C.Structure):
+    _fields_ = ([(n, C.c_uint64) for n in ("generation", "population_size", "training_word_count")] +
+                [(n, C.c_float) for n in ("fitness_min", "fitness_mean", "fitness_median", "fitness_p90",
+                                          "fitness_p99", "fitness_max", "fitness_stddev")] +
+                [("has_breeding_metrics", C.c_uint32)] +
+                [(n, C.c_uint64) for n in ("distinct_fitness_count", "parent_childless_count",
+                                          "parent_one_child_count", "parent_multiple_children_count")])
+
+
+class ConvergenceTelemetry(C.Structure):
+    _fields_ = ([(n, C.c_uint64) for n in ("generation", "sample_organisms", "sample_weights", "pair_count")] +
+                [(n, C.c_float) for n in ("centroid_distance_mean", "centroid_distance_min", "centroid_distance_max",
+                                          "pairwise_distance_mean", "pairwise_distance_min", "pairwise_distance_max",
+                                          "elapsed_ms", "reserved0")])
+
+
+class ShardReleasePolicy(C.Structure):
+    _fields_ = [("min_gap_generations", C.c_uint64), ("first_new_release_generation", C.c_uint64),
+                ("centroid_distance_threshold", C.c_float), ("fitness_p99_threshold", C.c_float),
+                ("run_seed", C.c_uint32), ("reserved0", C.c_uint32)]
+
+
+def default_release_policy(**kw):
+    """The reference CLI's defaults (training_data.hpp:17-20)."""
+    p = ShardReleasePolicy(min_gap_generations=3, first_new_release_generation=10, centroid_distance_threshold=6.0,
+                           fitness_p99_threshold=0.20, run_seed=7, reserved0=0)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
 class Callbacks(C.Structure):
     _fields_ = [("post_fitness_evaluation", POST_FITNESS_CB), ("training_shard_release", SHARD_RELEASE_CB),
                 ("user", C.c_void_p)]
@@ -156,7 +187,8 @@ EXPORTED_SYMBOLS = [
     "ne_set_eval_shape", "ne_create_prebreeding_checkpoint", "ne_restore_prebreeding_checkpoint",
     "ne_resume_generation_from_checkpoint", "ne_write_checkpoint", "ne_read_checkpoint", "ne_clone_checkpoint",
     "ne_get_checkpoint_info", "ne_get_checkpoint_generation", "ne_get_checkpoint_plan", "ne_get_checkpoint_genome",
-    "ne_free_checkpoint", "ne_training_data_identity_hash",
+    "ne_free_checkpoint", "ne_training_data_identity_hash", "ne_collect_fitness_telemetry",
+    "ne_collect_convergence_telemetry", "ne_convergence_sample_plan", "ne_maybe_release_training_shard",
 ]
 
 _lib = None
@@ -224,6 +256,12 @@ def load_library(path=LIB_PATH):
         "ne_get_checkpoint_genome": (i32, [p, u64, C.POINTER(u32), p, u64]),
         "ne_free_checkpoint": (None, [p]),
         "ne_training_data_identity_hash": (u64, [C.POINTER(WordCounts), C.POINTER(ShardReleaseHistory), p, u64]),
+        "ne_collect_fitness_telemetry": (i32, [p, C.POINTER(WordCounts), u32, C.POINTER(FitnessTelemetry)]),
+        "ne_collect_convergence_telemetry": (i32, [p, u32, C.POINTER(ConvergenceTelemetry)]),
+        "ne_convergence_sample_plan": (i32, [u64, u64, u32, u64, p, p, p, p]),
+        "ne_maybe_release_training_shard": (i32, [p, C.POINTER(WordCounts), C.POINTER(ShardReleasePolicy), u64, p, p,
+                                                  C.POINTER(u64), u64, C.POINTER(PendingInjection),
+                                                  C.POINTER(u64), C.POINTER(u32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -340,6 +378,30 @@ class SlabRuntime:
         hist, self._hist_keep = _history(history)
         self._check("ne_advance_generation",
                     self.lib.ne_advance_generation(self.h, seed, C.byref(counts), cfg, inj, 0, cbs, hist))
+
+    # --- telemetry and the curriculum controller
+    def fitness_telemetry(self, counts, breeding=False):
+        out = FitnessTelemetry()
+        self._check("ne_collect_fitness_telemetry",
+                    self.lib.ne_collect_fitness_telemetry(self.h, C.byref(counts), 1 if breeding else 0,
+                                                          C.byref(out)))
+        return out
+
+    def convergence_telemetry(self, run_seed):
+        out = ConvergenceTelemetry()
+        self._check("ne_collect_convergence_telemetry",
+                    self.lib.ne_collect_convergence_telemetry(self.h, run_seed, C.byref(out)))
+        return out
+
+    def maybe_release_shard(self, counts, policy, catalog_words, history):
+        """TryMaybeReleaseTrainingDataShard: (released, injection, next_word_count); appends to `history`."""
+        inj = PendingInjection()
+        nxt = C.c_uint64()
+        rel = C.c_uint32()
+        self._check("ne_maybe_release_training_shard", self.lib.ne_maybe_release_training_shard(
+            self.h, C.byref(counts), C.byref(policy), catalog_words, _ptr(history.generations), _ptr(history.p99),
+            C.byref(history.count), history.generations.size, C.byref(inj), C.byref(nxt), C.byref(rel)))
+        return bool(rel.value), inj, nxt.value
 
     # --- checkpoint and resume
     def create_checkpoint(self, seed, counts, config=None, injection=None, catalog=None, callbacks=None,
@@ -611,6 +673,38 @@ def training_data_identity_hash(counts, history=None, fitness_p99=None, catalog=
         cat = np.ascontiguousarray(catalog, dtype=np.uint8).reshape(-1, 5)
         n = cat.shape[0]
     return int(lib.ne_training_data_identity_hash(C.byref(counts), hist, None if cat is None else _ptr(cat), n))
+
+
+def convergence_sample_plan(population, trainable_scalars, run_seed, generation):
+    """MakeSamplePlan through the library (host only): (organisms, scalar indices, [pairs][2])."""
+    lib = load_library()
+    org = np.zeros(256, np.uint32)
+    w = np.zeros(8192, np.uint32)
+    pr = np.zeros((512, 2), np.uint32)
+    counts = np.zeros(3, np.uint64)
+    st = lib.ne_convergence_sample_plan(population, trainable_scalars, run_seed, generation, _ptr(org), _ptr(w),
+                                        _ptr(pr), _ptr(counts))
+    if st != 0:
+        raise RuntimeError_("ne_convergence_sample_plan", st)
+    return org[:int(counts[0])], w[:int(counts[1])], pr[:int(counts[2])]
+
+
+class ReleaseHistory:
+    """A caller-owned shard release history (generations and fitness p99), as the reference CLI keeps it."""
+
+    def __init__(self, capacity=4739):
+        self.generations = np.zeros(capacity, np.uint64)
+        self.p99 = np.zeros(capacity, np.float32)
+        self.count = C.c_uint64(0)
+
+    def record(self, generation, p99):
+        self.generations[self.count.value] = generation
+        self.p99[self.count.value] = p99
+        self.count.value += 1
+
+    def releases(self):
+        n = self.count.value
+        return self.generations[:n].tolist(), self.p99[:n].tolist()
 
 
 def rank_correlation(a, b):

@@ -18,7 +18,10 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <numeric>
+#include <random>
 #include <thread>
+#include <unordered_set>
 #include <vector>
 
 namespace {
@@ -1072,6 +1075,56 @@ void oro_bootstrap_genome(int enc_hidden, float* genome, uint64_t A, uint32_t se
         const float z = (e & 1) ? NormalFromWords(w[2], w[3]) : NormalFromWords(w[0], w[1]);
         genome[e] = std * z;
     }
+}
+
+// The n-th output (1-based) of std::mt19937_64 seeded with `seed` (the engine MakeSamplePlan draws from,
+// genetic_convergence.hpp:93).
+uint64_t oro_mt19937_64_nth(uint64_t seed, uint64_t n) {
+    std::mt19937_64 e(seed);
+    uint64_t v = 0;
+    for (uint64_t i = 0; i < n; ++i) v = e();
+    return v;
+}
+
+// MakeSamplePlan after its seed mixing (genetic_convergence.hpp:59-117): the engine seeded with
+// engine_seed, Floyd's sampling of the organism and the scalar indices (all of them, in order, when the
+// request covers the universe), then `pair_count` ordered pairs of distinct sampled organisms.
+int oro_sample_plan(uint64_t population, uint64_t scalars, uint64_t engine_seed, uint64_t organism_count,
+                    uint64_t weight_count, uint64_t pair_count, uint32_t* organisms, uint32_t* weights,
+                    uint32_t* pairs, uint64_t* counts) {
+    std::mt19937_64 e(engine_seed);
+    auto sample = [&e](uint64_t universe, uint64_t requested) {
+        const uint64_t k = requested < universe ? requested : universe;
+        std::vector<uint64_t> out(k);
+        if (k == universe) {
+            for (uint64_t i = 0; i < k; ++i) out[i] = i;
+            return out;
+        }
+        std::unordered_set<uint64_t> seen;
+        for (uint64_t j = 0, cursor = universe - k; cursor < universe; ++cursor, ++j) {
+            const uint64_t c = std::uniform_int_distribution<uint64_t>(0, cursor)(e);
+            out[j] = seen.count(c) != 0 ? cursor : c;
+            seen.insert(out[j]);
+        }
+        return out;
+    };
+    const std::vector<uint64_t> org = sample(population, organism_count);
+    const std::vector<uint64_t> w = sample(scalars, weight_count);
+    counts[0] = org.size();
+    counts[1] = w.size();
+    counts[2] = 0;
+    for (size_t i = 0; i < org.size(); ++i) organisms[i] = uint32_t(org[i]);
+    for (size_t i = 0; i < w.size(); ++i) weights[i] = uint32_t(w[i]);
+    if (org.size() < 2) return 0;
+    std::uniform_int_distribution<uint64_t> d1(0, org.size() - 1), d2(0, org.size() - 2);
+    for (uint64_t p = 0; p < pair_count; ++p) {
+        const uint64_t a = d1(e);
+        const uint64_t b0 = d2(e);
+        pairs[2 * p] = uint32_t(a);
+        pairs[2 * p + 1] = uint32_t(b0 >= a ? b0 + 1 : b0);
+    }
+    counts[2] = pair_count;
+    return 0;
 }
 
 }  // extern "C"

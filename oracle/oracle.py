@@ -39,6 +39,8 @@ _SIGS = {
     "oro_chebyshev": (U64, [U64, U64, U64, U64]),
     "oro_plan": (I32, [P, P, U64, U64, U64, U64, U64, P, U32, P]),
     "oro_plan_brood": (I32, [P, P, U64, U64, U64, U64, U64, P, U32, P]),
+    "oro_mt19937_64_nth": (U64, [U64, U64]),
+    "oro_sample_plan": (I32, [U64, U64, U64, U64, U64, U64, P, P, P, P]),
     "oro_xover_decisions": (None, [I32, U64, U32, U32, U32, F32, F32, F32, P]),
     "oro_xover_record_words": (U64, []),
     "oro_assemble_slots": (I32, [U32, P, P, P, P, U32, P, U32, P, P]),
