@@ -50,7 +50,7 @@ typedef struct ne_runtime ne_runtime;
 typedef struct ne_runtime_config {
     uint64_t genotype_slab_byte_budget;   /* slab bytes; slot count = budget / slot stride */
     uint64_t generation_byte_budget;      /* bytes one generation may occupy */
-    uint64_t host_spillover_byte_budget;  /* must be 0: host spill is not built */
+    uint64_t host_spillover_byte_budget;  /* host bytes for assembled children when the slab fills (0: none) */
     uint64_t action_count;                /* A: output-embedding rows per genome */
     uint64_t population_size_ceiling;     /* 0 = none; else a multiple of grid_column_count */
     uint64_t grid_column_count;           /* columns of the toroidal grid */
@@ -157,6 +157,7 @@ typedef struct ne_stage_times {
     uint64_t children;                    /* children assembled by the last advance */
     uint64_t rescored;                    /* BF16 mode: candidates rescored in FP32 */
     uint64_t fallbacks;                   /* BF16 mode: decisions settled by the FP32 scan */
+    uint64_t spilled;                     /* children the last advance held in host memory (slab full) */
 } ne_stage_times;
 
 /* Evaluation precision (ne_set_eval_precision).  NE_EVAL_FP32 is the reference's arithmetic.

@@ -1904,6 +1904,38 @@ __global__ void ReleaseBatchKernel(const uint32_t* pairs, uint32_t child_offset,
     if (atomicSub(refcounts + a, 1u) == 0u || atomicSub(refcounts + b, 1u) == 0u) SetStatus(status, 3);
 }
 
+// Host spill of an assembled prefix (ReleaseNextGenerationPrefixKernel / AllocateRestoredChildPrefixKernel,
+// slab_runtime.cu:733-807), in a fixed order: the slots of children 0..k-1 are pushed onto the free stack
+// at positions fc..fc+k-1; on the way back child i takes free_stack[fc-1-i], as the breeding batches do.
+__global__ void SpillReleaseKernel(uint32_t* child_slots, uint32_t k, uint32_t fc, uint32_t* slot_state,
+                                   uint32_t* free_stack, uint32_t* free_count, uint32_t slot_count, int* status) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *free_count = fc + k;
+    if (i >= k) return;
+    const uint32_t s = child_slots[i];
+    if (s >= slot_count || slot_state[s] == 0 || fc + i >= slot_count) {
+        SetStatus(status, 3);  // kInvalidSlab
+        return;
+    }
+    slot_state[s] = 0;
+    free_stack[fc + i] = s;
+    child_slots[i] = kInvalidSlot;
+}
+
+__global__ void SpillRestoreKernel(uint32_t* child_slots, uint32_t k, uint32_t fc, uint32_t* slot_state,
+                                   const uint32_t* free_stack, uint32_t* free_count, int* status) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *free_count = fc - k;
+    if (i >= k) return;
+    const uint32_t s = free_stack[fc - 1 - i];
+    if (child_slots[i] != kInvalidSlot || slot_state[s] != 0) {
+        SetStatus(status, 3);
+        return;
+    }
+    slot_state[s] = 1;
+    child_slots[i] = s;
+}
+
 __global__ void PopSlotsKernel(uint32_t* free_count, uint32_t count) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *free_count -= count;
 }
@@ -2192,6 +2224,18 @@ void LaunchReleaseBatch(const uint32_t* pairs, uint32_t child_offset, uint32_t b
                         int* status, hipStream_t s) {
     hipLaunchKernelGGL(ReleaseBatchKernel, dim3((batch + 255) / 256), dim3(256), 0, s, pairs, child_offset, batch,
                        refcounts, status);
+}
+
+void LaunchSpillRelease(uint32_t* child_slots, uint32_t k, uint32_t fc, uint32_t* slot_state, uint32_t* free_stack,
+                        uint32_t* free_count, uint32_t slot_count, int* status, hipStream_t s) {
+    hipLaunchKernelGGL(SpillReleaseKernel, dim3((k + 255) / 256), dim3(256), 0, s, child_slots, k, fc, slot_state,
+                       free_stack, free_count, slot_count, status);
+}
+
+void LaunchSpillRestore(uint32_t* child_slots, uint32_t k, uint32_t fc, uint32_t* slot_state,
+                        const uint32_t* free_stack, uint32_t* free_count, int* status, hipStream_t s) {
+    hipLaunchKernelGGL(SpillRestoreKernel, dim3((k + 255) / 256), dim3(256), 0, s, child_slots, k, fc, slot_state,
+                       free_stack, free_count, status);
 }
 
 void LaunchPopSlots(uint32_t* free_count, uint32_t count, hipStream_t s) {

@@ -242,6 +242,10 @@ void LaunchCollectReleased(uint32_t* cur_slots, uint32_t cur_size, const uint32_
 void LaunchReleaseBatch(const uint32_t* pairs, uint32_t child_offset, uint32_t batch, uint32_t* refcounts,
                         int* status, hipStream_t s);
 void LaunchPopSlots(uint32_t* free_count, uint32_t count, hipStream_t s);
+void LaunchSpillRelease(uint32_t* child_slots, uint32_t k, uint32_t fc, uint32_t* slot_state, uint32_t* free_stack,
+                        uint32_t* free_count, uint32_t slot_count, int* status, hipStream_t s);
+void LaunchSpillRestore(uint32_t* child_slots, uint32_t k, uint32_t fc, uint32_t* slot_state,
+                        const uint32_t* free_stack, uint32_t* free_count, int* status, hipStream_t s);
 void LaunchInitSlabMetadata(uint32_t* slot_state, uint32_t* free_stack, uint32_t* free_count, uint32_t slot_count,
                             hipStream_t s);
 void LaunchRepackPlan(const RepackArgs& a, hipStream_t s);

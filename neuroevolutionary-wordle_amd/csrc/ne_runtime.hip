@@ -673,8 +673,50 @@ static int Widen(ne_runtime* rt, uint64_t next_a) {
     return NE_OK;
 }
 
+// Host spill (TrySpillAssembledChildrenToHost / TryRestoreSpilledChildrenToDevice, slab_runtime.cu:
+// 809-917): the assembled children 0..k-1 go to pinned host memory and their slots back onto the free
+// stack; after the last batch they take slots again and their bytes return.
+static int SpillPrefix(ne_runtime* rt, uint64_t k, std::vector<uint8_t*>& chunks) {
+    std::vector<uint32_t> slots(k);
+    if (!Ok(hipMemcpyAsync(slots.data(), rt->nxt_slots, k * 4, hipMemcpyDeviceToHost, rt->stream)) ||
+        !Ok(hipStreamSynchronize(rt->stream)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    uint8_t* host = nullptr;
+    if (!Ok(hipHostMalloc(reinterpret_cast<void**>(&host), k * rt->stride, hipHostMallocDefault)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    chunks.push_back(host);
+    bool ok = true;
+    for (uint64_t i = 0; ok && i < k; ++i) {
+        if (slots[i] >= rt->slot_count) return Fail(rt, NE_INVALID_GENERATION);
+        ok = Ok(hipMemcpyAsync(host + i * rt->stride, rt->slab + uint64_t(slots[i]) * rt->stride, rt->stride,
+                               hipMemcpyDeviceToHost, rt->stream));
+    }
+    if (!ok) return Fail(rt, NE_DEVICE_FAILURE);
+    LaunchSpillRelease(rt->nxt_slots, uint32_t(k), HostFreeCount(rt), rt->slot_state, rt->free_stack, rt->free_count,
+                       uint32_t(rt->slot_count), rt->status, rt->stream);
+    return Finish(rt);
+}
+
+static int RestorePrefix(ne_runtime* rt, uint64_t k, const uint8_t* host) {
+    const uint32_t fc = HostFreeCount(rt);
+    if (fc < k) return Fail(rt, NE_DEVICE_FAILURE);  // as the reference reports a failed restore
+    LaunchSpillRestore(rt->nxt_slots, uint32_t(k), fc, rt->slot_state, rt->free_stack, rt->free_count, rt->status,
+                       rt->stream);
+    int st = Finish(rt);
+    if (st != NE_OK) return st;
+    std::vector<uint32_t> slots(k);
+    bool ok = Ok(hipMemcpyAsync(slots.data(), rt->nxt_slots, k * 4, hipMemcpyDeviceToHost, rt->stream)) &&
+              Ok(hipStreamSynchronize(rt->stream));
+    for (uint64_t i = 0; ok && i < k; ++i)
+        ok = Ok(hipMemcpyAsync(rt->slab + uint64_t(slots[i]) * rt->stride, host + i * rt->stride, rt->stride,
+                               hipMemcpyHostToDevice, rt->stream));
+    if (!ok) return Fail(rt, NE_DEVICE_FAILURE);
+    return Finish(rt);
+}
+
 // Assembly: validate, reference counts, GC of unreferenced parents, batches of children bounded by
-// free slots, parent release after each batch (device_runtime.cu:1434-1557).
+// free slots, parent release after each batch (device_runtime.cu:1434-1557); one host spill of the
+// assembled prefix when the slab fills (slab_runtime.cu:1554-1661).
 static int Assemble(ne_runtime* rt, uint32_t seed, const ne_assembly_config& cfg, uint64_t parent_a) {
     rt->next_index = rt->cur_index + 1;
     rt->next_size = rt->planned_children;
@@ -701,10 +743,29 @@ static int Assemble(ne_runtime* rt, uint32_t seed, const ne_assembly_config& cfg
     if (st != NE_OK) return st;
     float book_ms = EventMs(rt, 6);
     float breed_ms = 0.0f;
-    uint64_t offset = 0;
+    uint64_t offset = 0, spilled = 0;
+    std::vector<uint8_t*> spill;  // the pinned host copy of a spilled prefix
+    struct FreeSpill {
+        std::vector<uint8_t*>& v;
+        hipStream_t stream;
+        ~FreeSpill() {
+            if (v.empty()) return;
+            (void)hipStreamSynchronize(stream);  // no copy may still use the pinned memory
+            for (uint8_t* p : v) (void)hipHostFree(p);
+        }
+    } free_spill{spill, rt->stream};
     while (offset < rt->next_size) {
         const uint32_t fc = HostFreeCount(rt);
-        if (fc == 0) return Fail(rt, NE_SLAB_FULL);
+        if (fc == 0) {
+            if (spilled != 0 || offset == 0 || rt->config.host_spillover_byte_budget == 0)
+                return Fail(rt, NE_SLAB_FULL);
+            if (offset > rt->config.host_spillover_byte_budget / rt->stride)
+                return Fail(rt, NE_DEVICE_FAILURE);  // the reference's outcome when the spill slab is too small
+            st = SpillPrefix(rt, offset, spill);
+            if (st != NE_OK) return st;
+            spilled = offset;
+            continue;
+        }
         const uint64_t batch = std::min<uint64_t>(rt->next_size - offset, fc);
         BreedArgs b{};
         b.slab = rt->slab;
@@ -744,6 +805,11 @@ static int Assemble(ne_runtime* rt, uint32_t seed, const ne_assembly_config& cfg
         book_ms += EventMs(rt, 7);
         offset += batch;
     }
+    if (spilled != 0) {
+        st = RestorePrefix(rt, spilled, spill.front());
+        if (st != NE_OK) return st;
+    }
+    rt->times.spilled = spilled;
     rt->times.assemble_ms = breed_ms;
     rt->times.bookkeeping_ms = book_ms;
     rt->times.children = rt->next_size;
@@ -1213,7 +1279,6 @@ int ne_create(const ne_runtime_config* config, ne_runtime** out_runtime) {
     *out_runtime = nullptr;
     ne_runtime_config c = *config;
     if (c.encoder_hidden != 0) return NE_UNSUPPORTED;
-    if (c.host_spillover_byte_budget != 0) return NE_UNSUPPORTED;
     // IsValidDeviceSlabGARuntimeConfig (slab_runtime.hpp:50-60) on this build's stride.
     if (c.genotype_slab_byte_budget < c.generation_byte_budget || c.action_count == 0 || c.grid_column_count == 0 ||
         (c.population_size_ceiling != 0 && c.population_size_ceiling % c.grid_column_count != 0))
@@ -2179,7 +2244,8 @@ int ne_create_prebreeding_checkpoint(ne_runtime* rt, uint32_t generation_seed, c
                                                  c->release_p99, catalog_letters, catalog_word_count);
         ne_ckpt::EncodeAssembly(cfg, c->assembly);
         ne_ckpt::EncodeInjection(inj, c->injection);
-        const uint64_t rc[6] = {rt->slab_bytes, rt->generation_byte_budget, 0, rt->action_count, rt->max_gen, cols};
+        const uint64_t rc[6] = {rt->slab_bytes, rt->generation_byte_budget, rt->config.host_spillover_byte_budget,
+                                rt->action_count, rt->max_gen, cols};
         std::memcpy(c->runtime_config, rc, sizeof(rc));
         const uint64_t layout[4] = {rt->action_count, rt->stride, rt->slot_count, rt->slab_bytes};
         std::memcpy(c->slab_layout, layout, sizeof(layout));
@@ -2251,7 +2317,6 @@ int ne_restore_prebreeding_checkpoint(const ne_checkpoint* c, void* stream, ne_r
     if (n == 0 || children == 0 || cols == 0 || n % cols != 0 || children % cols != 0 || epi_rows == 0 ||
         L > slots || n > 0xFFFFFFFFull)
         return NE_CHECKPOINT_INVALID;
-    if (c->RuntimeConfigField(2) != 0) return NE_UNSUPPORTED;  // host spill budget
     std::vector<uint8_t> restored(n, 0);
     std::vector<uint32_t> cur_slots(n, kInvalidSlot);
     for (uint64_t k = 0; k < L; ++k) {
@@ -2266,6 +2331,7 @@ int ne_restore_prebreeding_checkpoint(const ne_checkpoint* c, void* stream, ne_r
     ne_runtime_config rc{};
     rc.genotype_slab_byte_budget = slots * S;
     rc.generation_byte_budget = c->RuntimeConfigField(1);
+    rc.host_spillover_byte_budget = c->RuntimeConfigField(2);
     rc.action_count = A;
     rc.grid_column_count = cols;
     rc.stream = stream;

@@ -90,7 +90,7 @@ class StageTimes(C.Structure):
                 ("assemble_ms", C.c_float), ("repack_ms", C.c_float), ("inject_ms", C.c_float),
                 ("bookkeeping_ms", C.c_float), ("prepare_ms", C.c_float), ("decisions", C.c_uint64),
                 ("encoded_turns", C.c_uint64), ("games", C.c_uint64), ("children", C.c_uint64),
-                ("rescored", C.c_uint64), ("fallbacks", C.c_uint64)]
+                ("rescored", C.c_uint64), ("fallbacks", C.c_uint64), ("spilled", C.c_uint64)]
 
 
 GAME_TRACE_DTYPE = np.dtype([("guesses", "<u4", 6), ("best", "<f4", 6), ("second", "<f4", 6),
@@ -319,10 +319,11 @@ def slot_stride_bytes(action_count):
 class SlabRuntime:
     """One runtime handle: TryCreateDeviceSlabGARuntimeBuffers ... DestroyDeviceSlabGARuntimeBuffers."""
 
-    def __init__(self, slab_bytes, generation_bytes, action_count, grid_columns, population_ceiling=0):
+    def __init__(self, slab_bytes, generation_bytes, action_count, grid_columns, population_ceiling=0,
+                 host_spill_bytes=0):
         self.lib = load_library()
         cfg = RuntimeConfig(genotype_slab_byte_budget=slab_bytes, generation_byte_budget=generation_bytes,
-                            host_spillover_byte_budget=0, action_count=action_count,
+                            host_spillover_byte_budget=host_spill_bytes, action_count=action_count,
                             population_size_ceiling=population_ceiling, grid_column_count=grid_columns,
                             encoder_hidden=0, reserved0=0, stream=None)
         h = C.c_void_p()

@@ -902,7 +902,7 @@ int oro_plan_brood(const float* fit, const uint8_t* has, uint64_t cur_size, uint
 // number of batches.
 int oro_assemble_slots(uint32_t slot_count, uint32_t* free_stack, uint32_t* free_count, uint32_t* slot_state,
                        uint32_t* cur_slots, uint32_t cur_size, const uint32_t* pairs, uint32_t child_count,
-                       uint32_t* child_slots, uint32_t* batches_out) {
+                       uint32_t* child_slots, uint32_t* batches_out, uint32_t spill_capacity, uint32_t* spilled_out) {
     std::vector<uint32_t> ref(cur_size, 0);
     for (uint32_t c = 0; c < child_count; ++c) {
         if (pairs[2 * c] >= cur_size || pairs[2 * c + 1] >= cur_size) return kInvalidParentIndex;
@@ -922,10 +922,22 @@ int oro_assemble_slots(uint32_t slot_count, uint32_t* free_stack, uint32_t* free
     };
     int st = collect();
     if (st != kOk) return st;
-    uint32_t offset = 0, batches = 0;
+    uint32_t offset = 0, batches = 0, spilled = 0;
     while (offset < child_count) {
         const uint32_t fc = *free_count;
-        if (fc == 0) return kSlabFull;
+        if (fc == 0) {
+            // one host spill of the assembled prefix (slab_runtime.cu:1554-1640): its slots go back onto
+            // the free stack in child order
+            if (spilled != 0 || offset == 0 || spill_capacity == 0) return kSlabFull;
+            if (offset > spill_capacity) return 1;  // kCudaFailure
+            for (uint32_t i = 0; i < offset; ++i) {
+                slot_state[child_slots[i]] = 0;
+                free_stack[(*free_count)++] = child_slots[i];
+                child_slots[i] = 0xFFFFFFFFu;
+            }
+            spilled = offset;
+            continue;
+        }
         const uint32_t batch = std::min(child_count - offset, fc);
         for (uint32_t i = 0; i < batch; ++i) {
             const uint32_t sl = free_stack[fc - 1 - i];
@@ -942,7 +954,17 @@ int oro_assemble_slots(uint32_t slot_count, uint32_t* free_stack, uint32_t* free
         offset += batch;
         ++batches;
     }
+    if (spilled != 0) {  // the spilled prefix takes slots again, child i from the top of the stack down
+        const uint32_t fc = *free_count;
+        if (fc < spilled) return 1;
+        for (uint32_t i = 0; i < spilled; ++i) {
+            child_slots[i] = free_stack[fc - 1 - i];
+            slot_state[child_slots[i]] = 1;
+        }
+        *free_count = fc - spilled;
+    }
     if (batches_out) *batches_out = batches;
+    if (spilled_out) *spilled_out = spilled;
     return kOk;
 }
 

@@ -43,7 +43,7 @@ _SIGS = {
     "oro_sample_plan": (I32, [U64, U64, U64, U64, U64, U64, P, P, P, P]),
     "oro_xover_decisions": (None, [I32, U64, U32, U32, U32, F32, F32, F32, P]),
     "oro_xover_record_words": (U64, []),
-    "oro_assemble_slots": (I32, [U32, P, P, P, P, U32, P, U32, P, P]),
+    "oro_assemble_slots": (I32, [U32, P, P, P, P, U32, P, U32, P, P, U32, P]),
     "oro_hint_group": (I32, [P, P, P, P]),
     "oro_repack_slots": (I32, [U32, P, U32, P, U32, P, P, P, P, P]),
     "oro_median_tail_norm": (F32, [P, U64]),
@@ -279,7 +279,7 @@ def slab_after_bootstrap(slot_count, n):
     return stack, np.uint32(slot_count - n), state, np.arange(n, dtype=np.uint32)
 
 
-def assemble_slots(slot_count, stack, free_count, state, cur_slots, pairs):
+def assemble_slots(slot_count, stack, free_count, state, cur_slots, pairs, spill_capacity=0):
     stack = stack.copy()
     state = state.copy()
     cur = np.ascontiguousarray(cur_slots, np.uint32).copy()
@@ -287,10 +287,11 @@ def assemble_slots(slot_count, stack, free_count, state, cur_slots, pairs):
     fc = np.array([free_count], np.uint32)
     child = np.zeros(len(pr), np.uint32)
     nb = np.zeros(1, np.uint32)
+    sp = np.zeros(1, np.uint32)
     st = lib().oro_assemble_slots(slot_count, ptr(stack), ptr(fc), ptr(state), ptr(cur), cur.size, ptr(pr), len(pr),
-                                  ptr(child), ptr(nb))
+                                  ptr(child), ptr(nb), spill_capacity, ptr(sp))
     return dict(status=st, stack=stack, free_count=int(fc[0]), state=state, parent_slots=cur, child_slots=child,
-                batches=int(nb[0]))
+                batches=int(nb[0]), spilled=int(sp[0]))
 
 
 def hint_group(word_letters):
