@@ -40,7 +40,7 @@ typedef enum ne_status {
     NE_INVALID_ARGUMENT = 17,         /* null pointer, short buffer, size out of range */
     NE_UNSUPPORTED = 18,              /* a configuration this build does not implement */
     NE_CALLBACK_FAILED = 19,          /* a caller callback returned non-zero */
-    NE_CHECKPOINT_INVALID = 20,       /* a checkpoint failed validation (magic, versions, checksum, content) */
+    NE_CHECKPOINT_INVALID = 20,       /* a checkpoint or winner artifact failed validation */
     NE_IO_FAILURE = 21                /* a checkpoint file could not be opened, read or written */
 } ne_status;
 
@@ -500,6 +500,74 @@ int ne_maybe_release_training_shard(ne_runtime *rt, const ne_word_counts *counts
                                     uint64_t *release_count, uint64_t release_capacity,
                                     ne_pending_injection *injection, uint64_t *next_word_count,
                                     uint32_t *released);
+
+/* ---- single-model inference and the winner artifact ------------------------------------------
+ * One genome serving next-guess requests (src/inference/single_model_device_runtime.hpp:9-49), and the
+ * artifact that carries a run's best genome (src/model_artifact/winner_artifact*.hpp). */
+typedef struct ne_model ne_model;
+
+/* A played Wordle board (wordle/wordle_grid.hpp:13-36): turn_count turns of a guess and its tiles. */
+typedef struct ne_board {
+    uint32_t turn_count;                 /* 0..6 */
+    uint8_t guesses[6][5];               /* letter indices 0..25 */
+    uint8_t tiles[6][5];                 /* 0 green, 1 yellow, 2 grey */
+} ne_board;
+
+/* SelectedAction (output_embedding.hpp) plus the runner-up score. */
+typedef struct ne_selected_action {
+    uint32_t status;                     /* NE_OK; NE_POLICY_FORWARD_FAILED: invalid or finished board;
+                                            NE_ACTION_SELECTION_FAILED: every action already guessed */
+    uint32_t action_index;
+    float score;
+    float second_score;                  /* best score among the other unguessed actions; -inf if none */
+    uint8_t word[5];
+    uint8_t reserved[3];
+} ne_selected_action;
+
+/* TryCreateSingleModelDeviceRuntime (single_model_device_runtime.hpp:38-40): genome_byte_count must be
+ * the slot stride of action_count actions; action_letters holds action_count words of 5 letter indices.
+ * stream: hipStream_t or NULL (own stream). */
+int ne_model_create(const void *genome_bytes, uint64_t genome_byte_count, const uint8_t *action_letters,
+                    uint64_t action_count, void *stream, ne_model **out_model);
+void ne_model_destroy(ne_model *model);
+/* TrySelectNextGuessWithSingleModelDeviceRuntime (single_model_device_runtime.hpp:44-47): the policy
+ * forward on the board, the masked argmax over the actions (a guessed word, any copy of it, is never
+ * guessed again; ties go to the lowest index).  Returns the board's status. */
+int ne_model_select_guess(ne_model *model, const ne_board *board, ne_selected_action *out);
+/* No counterpart (the reference serves one board per call): `count` boards in one launch, one status each
+ * in out[i].status; the call's return is NE_OK unless the call itself failed. */
+int ne_model_select_guesses(ne_model *model, const ne_board *boards, uint64_t count, ne_selected_action *out);
+
+/* WinnerArtifactMetadata (winner_artifact.hpp:12-21). */
+typedef struct ne_winner_metadata {
+    float best_fitness;
+    uint32_t best_slot_index;
+    uint64_t generation_index;
+    uint64_t best_index;
+    uint64_t action_count;
+    uint64_t genome_byte_count;
+    uint32_t seed;
+    uint32_t reserved0;
+    char action_space_path[1024];        /* NUL-terminated */
+    char timestamp_local[32];            /* set by the writer and the reader: YYYY-MM-DDTHH:MM:SS */
+} ne_winner_metadata;
+
+/* TryWriteWinnerArtifactToPaths (winner_artifact.hpp:32-35): the genome bytes to binary_path and the JSON
+ * metadata (format_version 2, the first action_count words) to metadata_path, each through <path>.tmp. */
+int ne_write_winner_artifact(const char *binary_path, const char *metadata_path, const void *genome_bytes,
+                             const uint8_t *action_letters, uint64_t action_word_count,
+                             ne_winner_metadata *metadata);
+/* TryWriteWinnerArtifact (winner_artifact.hpp:28-30): a new winner-<timestamp>-g<generation>-seed<seed>
+ * pair in directory (a -2, -3, ... suffix when the name exists); the chosen paths are returned. */
+int ne_write_winner_artifact_in_directory(const char *directory, const void *genome_bytes,
+                                          const uint8_t *action_letters, uint64_t action_word_count,
+                                          ne_winner_metadata *metadata, char *binary_path_out,
+                                          char *metadata_path_out, uint64_t path_capacity);
+/* TryReadWinnerArtifact (winner_artifact_reader.hpp:20-21): fills metadata; genome_dst (capacity bytes)
+ * and letters_dst (capacity words) may be NULL to learn the sizes first. */
+int ne_read_winner_artifact(const char *binary_path, const char *metadata_path, ne_winner_metadata *metadata,
+                            void *genome_dst, uint64_t genome_capacity, uint8_t *letters_dst,
+                            uint64_t letters_capacity_words);
 
 #ifdef __cplusplus
 }

@@ -22,7 +22,8 @@ HIPCC_FLAGS = [
     "-Wno-cuda-compat",  # device lambdas carry __device__ after the parameter list
 ]
 
-SOURCES = ["ne_kernels.hip", "ne_surrogate.hip", "ne_telemetry.hip", "ne_runtime.hip", "ne_checkpoint.cpp"]
+SOURCES = ["ne_kernels.hip", "ne_surrogate.hip", "ne_telemetry.hip", "ne_runtime.hip", "ne_model.hip",
+           "ne_checkpoint.cpp", "ne_artifact.cpp"]
 
 
 DRIVER = os.path.join(HERE, "run_ga")

@@ -5,10 +5,10 @@
 // from generations K, 2K, ... go through TryCreatePrebreedingCheckpointOnDevice, a background write to
 // --checkpoint PATH and TryResumeGenerationFromCheckpointOnDevice; --resume PATH restores a checkpoint
 // into a new runtime and continues from it (the reference CLI's flow, run_genetic_algorithm.cu:1605-1660,
-// 1900-1967).
+// 1900-1967); --winner DIR writes the final generation's best genome as a winner artifact.
 //
 // usage: run_ga <workload_dir> <grid_columns> <generations> <seed> [mutation_probability]
-//               [--checkpoint PATH --checkpoint-every K] [--resume PATH]
+//               [--checkpoint PATH --checkpoint-every K] [--resume PATH] [--winner DIR]
 // workload_dir holds catalog.u8 (5 letter indices per word), offsets.u32 (cells+1) and secrets.u32.
 #include <chrono>
 #include <cstdio>
@@ -42,7 +42,7 @@ int main(int argc, char** argv) {
     if (argc < 5) {
         std::fprintf(stderr,
                      "usage: %s <workload_dir> <grid_columns> <generations> <seed> [mutation_p] "
-                     "[--checkpoint PATH --checkpoint-every K] [--resume PATH]\n",
+                     "[--checkpoint PATH --checkpoint-every K] [--resume PATH] [--winner DIR]\n",
                      argv[0]);
         return 2;
     }
@@ -51,7 +51,7 @@ int main(int argc, char** argv) {
     const std::size_t generations = std::strtoull(argv[3], nullptr, 10);
     const std::uint32_t seed = static_cast<std::uint32_t>(std::strtoul(argv[4], nullptr, 10));
     float mutation_p = 0.01f;
-    std::string checkpoint_path, resume_path;
+    std::string checkpoint_path, resume_path, winner_dir;
     std::size_t checkpoint_every = 0;
     for (int a = 5; a < argc; ++a) {
         const std::string arg = argv[a];
@@ -61,6 +61,8 @@ int main(int argc, char** argv) {
             checkpoint_every = std::strtoull(argv[++a], nullptr, 10);
         } else if (arg == "--resume" && a + 1 < argc) {
             resume_path = argv[++a];
+        } else if (arg == "--winner" && a + 1 < argc) {
+            winner_dir = argv[++a];
         } else if (a == 5) {
             mutation_p = std::strtof(argv[a], nullptr);
         } else {
@@ -165,6 +167,25 @@ int main(int argc, char** argv) {
     SlabGeneration current{};
     if (!TryDownloadCurrentGenerationFromDevice(buffers, current) || current.slot_indices.size() != cells)
         return Fail(buffers, "download");
+    if (!winner_dir.empty()) {  // TryPersistCheckpointWinningGenome's artifact (run_genetic_algorithm.cu:1925-1934)
+        std::vector<std::uint8_t> genome;
+        if (!TryDownloadSlabSlotBytesFromDevice(buffers, s.best_slot_index, genome)) return Fail(buffers, "winner");
+        ne_amd::model_artifact::WinnerArtifactMetadata meta{};
+        meta.best_fitness = s.best_fitness;
+        meta.generation_index = s.generation_index;
+        meta.best_index = s.best_index;
+        meta.best_slot_index = s.best_slot_index;
+        meta.action_count = actions;
+        meta.genome_byte_count = genome.size();
+        meta.seed = seed;
+        std::snprintf(meta.action_space_path, sizeof(meta.action_space_path), "%s/catalog.u8", dir.c_str());
+        ne_amd::model_artifact::WinnerArtifactPaths paths{};
+        if (!ne_amd::model_artifact::TryWriteWinnerArtifact(winner_dir, genome.data(), catalog, meta, paths)) {
+            std::fprintf(stderr, "could not write the winner artifact into %s\n", winner_dir.c_str());
+            return 1;
+        }
+        std::printf("winner artifact %s\n", paths.binary_path.c_str());
+    }
     DestroyDeviceSlabGARuntimeBuffers(buffers);
     return evaluated == generations - first_generation ? 0 : 1;
 }

@@ -13,6 +13,7 @@
 #include <filesystem>
 #include <functional>
 #include <future>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -575,3 +576,123 @@ inline const char* DeviceSlabGARuntimeStatusCodeString(DeviceSlabGARuntimeStatus
 }
 
 }  // namespace ne_amd::slab_device
+
+// ---- single-model inference (src/inference/single_model_device_runtime.hpp:9-49) -------------------
+namespace ne_amd::inference {
+
+enum class SingleModelDeviceRuntimeStatusCode : int {
+    kOk = 0,
+    kInvalidRuntime = 1,
+    kCudaMemcpyFailed = 2,
+    kKernelLaunchFailed = 3,
+    kKernelExecutionFailed = 4,
+    kPolicyForwardFailed = 5,
+    kActionSelectionFailed = 6,
+};
+
+struct SingleModelDeviceRuntime {
+    ne_model* handle = nullptr;
+    std::size_t action_count = 0;
+};
+
+// TryCreateSingleModelDeviceRuntime (single_model_device_runtime.hpp:38-40); action_letters holds 5 letter
+// indices per word.
+inline bool TryCreateSingleModelDeviceRuntime(const std::uint8_t* host_genome_bytes, std::size_t genome_byte_count,
+                                              const std::vector<std::uint8_t>& action_letters,
+                                              SingleModelDeviceRuntime& runtime_out) {
+    runtime_out = {};
+    ne_model* m = nullptr;
+    if (ne_model_create(host_genome_bytes, genome_byte_count, action_letters.data(), action_letters.size() / 5, nullptr,
+                        &m) != NE_OK)
+        return false;
+    runtime_out = {m, action_letters.size() / 5};
+    return true;
+}
+
+inline void DestroySingleModelDeviceRuntime(SingleModelDeviceRuntime& runtime) noexcept {
+    ne_model_destroy(runtime.handle);
+    runtime = {};
+}
+
+// TrySelectNextGuessWithSingleModelDeviceRuntime (single_model_device_runtime.hpp:44-47)
+inline bool TrySelectNextGuessWithSingleModelDeviceRuntime(const SingleModelDeviceRuntime& runtime,
+                                                           const ne_board& grid, ne_selected_action& selected_out,
+                                                           SingleModelDeviceRuntimeStatusCode* status_out = nullptr) {
+    using S = SingleModelDeviceRuntimeStatusCode;
+    S st = S::kInvalidRuntime;
+    if (runtime.handle) {
+        switch (ne_model_select_guess(runtime.handle, &grid, &selected_out)) {
+            case NE_OK: st = S::kOk; break;
+            case NE_POLICY_FORWARD_FAILED: st = S::kPolicyForwardFailed; break;
+            case NE_ACTION_SELECTION_FAILED: st = S::kActionSelectionFailed; break;
+            default: st = S::kKernelExecutionFailed;
+        }
+    }
+    if (status_out) *status_out = st;
+    return st == S::kOk;
+}
+
+}  // namespace ne_amd::inference
+
+// ---- the winner artifact (src/model_artifact/winner_artifact*.hpp) --------------------------------
+namespace ne_amd::model_artifact {
+
+using WinnerArtifactMetadata = ne_winner_metadata;
+
+struct WinnerArtifactPaths {
+    std::filesystem::path binary_path{};
+    std::filesystem::path metadata_path{};
+    std::string timestamp_local{};
+};
+
+struct LoadedWinnerArtifact {
+    WinnerArtifactMetadata metadata{};
+    std::vector<std::uint8_t> action_letters{};
+    std::vector<std::uint8_t> genome_bytes{};
+};
+
+// TryWriteWinnerArtifact (winner_artifact.hpp:28-30)
+inline bool TryWriteWinnerArtifact(const std::filesystem::path& directory, const std::uint8_t* genome_bytes,
+                                   const std::vector<std::uint8_t>& action_letters, WinnerArtifactMetadata metadata,
+                                   WinnerArtifactPaths& paths_out) {
+    paths_out = {};
+    char bin[4096], meta[4096];
+    if (ne_write_winner_artifact_in_directory(directory.c_str(), genome_bytes, action_letters.data(),
+                                              action_letters.size() / 5, &metadata, bin, meta, sizeof(bin)) != NE_OK)
+        return false;
+    paths_out = {bin, meta, metadata.timestamp_local};
+    return true;
+}
+
+// TryWriteWinnerArtifactToPaths (winner_artifact.hpp:32-35)
+inline bool TryWriteWinnerArtifactToPaths(const std::filesystem::path& binary_path,
+                                          const std::filesystem::path& metadata_path, const std::uint8_t* genome_bytes,
+                                          const std::vector<std::uint8_t>& action_letters,
+                                          WinnerArtifactMetadata metadata, WinnerArtifactPaths& paths_out) {
+    paths_out = {};
+    if (ne_write_winner_artifact(binary_path.c_str(), metadata_path.c_str(), genome_bytes, action_letters.data(),
+                                 action_letters.size() / 5, &metadata) != NE_OK)
+        return false;
+    paths_out = {binary_path, metadata_path, metadata.timestamp_local};
+    return true;
+}
+
+// TryReadWinnerArtifact (winner_artifact_reader.hpp:20-21)
+inline bool TryReadWinnerArtifact(const std::filesystem::path& binary_path, const std::filesystem::path& metadata_path,
+                                  LoadedWinnerArtifact& artifact_out) {
+    artifact_out = {};
+    WinnerArtifactMetadata m{};
+    if (ne_read_winner_artifact(binary_path.c_str(), metadata_path.c_str(), &m, nullptr, 0, nullptr, 0) != NE_OK)
+        return false;
+    artifact_out.genome_bytes.resize(m.genome_byte_count);
+    artifact_out.action_letters.resize(5 * m.action_count);
+    if (ne_read_winner_artifact(binary_path.c_str(), metadata_path.c_str(), &m, artifact_out.genome_bytes.data(),
+                                m.genome_byte_count, artifact_out.action_letters.data(), m.action_count) != NE_OK) {
+        artifact_out = {};
+        return false;
+    }
+    artifact_out.metadata = m;
+    return true;
+}
+
+}  // namespace ne_amd::model_artifact

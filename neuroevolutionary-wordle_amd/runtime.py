@@ -171,6 +171,33 @@ def default_release_policy(**kw):
     return p
 
 
+class Board(C.Structure):
+    _fields_ = [("turn_count", C.c_uint32), ("guesses", (C.c_uint8 * 5) * 6), ("tiles", (C.c_uint8 * 5) * 6)]
+
+
+class SelectedAction(C.Structure):
+    _fields_ = [("status", C.c_uint32), ("action_index", C.c_uint32), ("score", C.c_float),
+                ("second_score", C.c_float), ("word", C.c_uint8 * 5), ("reserved", C.c_uint8 * 3)]
+
+
+class WinnerMetadata(C.Structure):
+    _fields_ = [("best_fitness", C.c_float), ("best_slot_index", C.c_uint32), ("generation_index", C.c_uint64),
+                ("best_index", C.c_uint64), ("action_count", C.c_uint64), ("genome_byte_count", C.c_uint64),
+                ("seed", C.c_uint32), ("reserved0", C.c_uint32), ("action_space_path", C.c_char * 1024),
+                ("timestamp_local", C.c_char * 32)]
+
+
+def make_board(turns):
+    """An ne_board from [(word letters[5], tiles[5]), ...]."""
+    b = Board()
+    b.turn_count = len(turns)
+    for t, (w, tl) in enumerate(turns):
+        for p in range(5):
+            b.guesses[t][p] = int(w[p])
+            b.tiles[t][p] = int(tl[p])
+    return b
+
+
 class Callbacks(C.Structure):
     _fields_ = [("post_fitness_evaluation", POST_FITNESS_CB), ("training_shard_release", SHARD_RELEASE_CB),
                 ("user", C.c_void_p)]
@@ -189,6 +216,8 @@ EXPORTED_SYMBOLS = [
     "ne_get_checkpoint_info", "ne_get_checkpoint_generation", "ne_get_checkpoint_plan", "ne_get_checkpoint_genome",
     "ne_free_checkpoint", "ne_training_data_identity_hash", "ne_collect_fitness_telemetry",
     "ne_collect_convergence_telemetry", "ne_convergence_sample_plan", "ne_maybe_release_training_shard",
+    "ne_model_create", "ne_model_destroy", "ne_model_select_guess", "ne_model_select_guesses",
+    "ne_write_winner_artifact", "ne_write_winner_artifact_in_directory", "ne_read_winner_artifact",
 ]
 
 _lib = None
@@ -262,6 +291,14 @@ def load_library(path=LIB_PATH):
         "ne_maybe_release_training_shard": (i32, [p, C.POINTER(WordCounts), C.POINTER(ShardReleasePolicy), u64, p, p,
                                                   C.POINTER(u64), u64, C.POINTER(PendingInjection),
                                                   C.POINTER(u64), C.POINTER(u32)]),
+        "ne_model_create": (i32, [p, u64, p, u64, p, C.POINTER(p)]),
+        "ne_model_destroy": (None, [p]),
+        "ne_model_select_guess": (i32, [p, C.POINTER(Board), C.POINTER(SelectedAction)]),
+        "ne_model_select_guesses": (i32, [p, p, u64, p]),
+        "ne_write_winner_artifact": (i32, [C.c_char_p, C.c_char_p, p, p, u64, C.POINTER(WinnerMetadata)]),
+        "ne_write_winner_artifact_in_directory": (i32, [C.c_char_p, p, p, u64, C.POINTER(WinnerMetadata), p, p,
+                                                        u64]),
+        "ne_read_winner_artifact": (i32, [C.c_char_p, C.c_char_p, C.POINTER(WinnerMetadata), p, u64, p, u64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -706,6 +743,99 @@ class ReleaseHistory:
     def releases(self):
         n = self.count.value
         return self.generations[:n].tolist(), self.p99[:n].tolist()
+
+
+class Model:
+    """TryCreateSingleModelDeviceRuntime ... DestroySingleModelDeviceRuntime: one genome serving guesses."""
+
+    def __init__(self, genome, action_letters):
+        self.lib = load_library()
+        self.h = None
+        g = np.ascontiguousarray(genome).view(np.uint8)
+        w = np.ascontiguousarray(action_letters, dtype=np.uint8).reshape(-1, 5)
+        h = C.c_void_p()
+        st = self.lib.ne_model_create(_ptr(g), g.size, _ptr(w), w.shape[0], None, C.byref(h))
+        if st != 0:
+            raise RuntimeError_("ne_model_create", st)
+        self.h = h
+        self.action_count = w.shape[0]
+
+    def select(self, turns):
+        """TrySelectNextGuessWithSingleModelDeviceRuntime on one board: an ne_selected_action."""
+        out = SelectedAction()
+        self.lib.ne_model_select_guess(self.h, C.byref(make_board(turns)), C.byref(out))
+        return out
+
+    def select_many(self, boards):
+        """Many boards in one launch (a list of ne_board or of turn lists): a SelectedAction array."""
+        arr = (Board * len(boards))(*[b if isinstance(b, Board) else make_board(b) for b in boards])
+        out = (SelectedAction * len(boards))()
+        st = self.lib.ne_model_select_guesses(self.h, C.cast(arr, C.c_void_p), len(boards), C.cast(out, C.c_void_p))
+        if st != 0:
+            raise RuntimeError_("ne_model_select_guesses", st)
+        return out
+
+    def close(self):
+        if self.h:
+            self.lib.ne_model_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def winner_metadata(best_fitness, generation_index, best_index, best_slot_index, action_count, seed,
+                    action_space_path=""):
+    m = WinnerMetadata(best_fitness=best_fitness, best_slot_index=best_slot_index, generation_index=generation_index,
+                       best_index=best_index, action_count=action_count,
+                       genome_byte_count=slot_stride_bytes(action_count), seed=seed)
+    m.action_space_path = action_space_path.encode()
+    return m
+
+
+def write_winner_artifact(binary_path, metadata_path, genome, action_letters, metadata):
+    """TryWriteWinnerArtifactToPaths; returns the metadata with its timestamp."""
+    lib = load_library()
+    g = np.ascontiguousarray(genome).view(np.uint8)
+    w = np.ascontiguousarray(action_letters, dtype=np.uint8).reshape(-1, 5)
+    st = lib.ne_write_winner_artifact(os.fsencode(binary_path), os.fsencode(metadata_path), _ptr(g), _ptr(w),
+                                      w.shape[0], C.byref(metadata))
+    if st != 0:
+        raise RuntimeError_("ne_write_winner_artifact", st)
+    return metadata
+
+
+def write_winner_artifact_in_directory(directory, genome, action_letters, metadata):
+    """TryWriteWinnerArtifact: (binary path, metadata path) of a new winner-... pair in directory."""
+    lib = load_library()
+    g = np.ascontiguousarray(genome).view(np.uint8)
+    w = np.ascontiguousarray(action_letters, dtype=np.uint8).reshape(-1, 5)
+    b, m = C.create_string_buffer(4096), C.create_string_buffer(4096)
+    st = lib.ne_write_winner_artifact_in_directory(os.fsencode(directory), _ptr(g), _ptr(w), w.shape[0],
+                                                   C.byref(metadata), b, m, 4096)
+    if st != 0:
+        raise RuntimeError_("ne_write_winner_artifact_in_directory", st)
+    return b.value.decode(), m.value.decode()
+
+
+def read_winner_artifact(binary_path, metadata_path):
+    """TryReadWinnerArtifact: (metadata, fp32 genome, action letters [n][5])."""
+    lib = load_library()
+    meta = WinnerMetadata()
+    b, m = os.fsencode(binary_path), os.fsencode(metadata_path)
+    st = lib.ne_read_winner_artifact(b, m, C.byref(meta), None, 0, None, 0)
+    if st != 0:
+        raise RuntimeError_("ne_read_winner_artifact", st)
+    genome = np.zeros(meta.genome_byte_count // 4, np.float32)
+    letters = np.zeros((meta.action_count, 5), np.uint8)
+    st = lib.ne_read_winner_artifact(b, m, C.byref(meta), _ptr(genome), genome.nbytes, _ptr(letters),
+                                     meta.action_count)
+    if st != 0:
+        raise RuntimeError_("ne_read_winner_artifact", st)
+    return meta, genome, letters
 
 
 def rank_correlation(a, b):

@@ -350,7 +350,9 @@ bool SelectAction(const T* policy, const float* tails, const uint8_t* catalog, c
     return true;
 }
 
-// Episode initialisation (evaluation_ops.cuh:171-237): episode e -> entry e/3, variant e%3.
+// Episode initialisation (evaluation_ops.cuh:171-237): variant 0 starts empty; variants 1 and 2 start with
+// the cell's next two secrets (entry+1, entry+2) or the two after them (entry+3, entry+4) already played.
+// A genome's 3W games are the W fresh ones (slot = entry), then the prefilled pairs (slot W + 2 entry + v-1).
 int InitEpisode(const uint8_t* catalog, const uint32_t* local, uint64_t W, uint64_t entry, int variant, Grid& g) {
     std::memset(&g, 0, sizeof(g));
     std::memcpy(g.sol, catalog + 5 * size_t(local[entry]), 5);
@@ -670,27 +672,64 @@ void oro_feedback(const uint8_t* guess, const uint8_t* sol, uint8_t* fb) { Feedb
 
 // Forward one grid state.  letters/tiles: tc*5 bytes.  Writes fp32 sequential policy to out32
 // and/or fp64 to out64 (either may be null).  Returns 0 or a status code.
-int oro_policy_forward(const float* genome, int enc_hidden, const uint8_t* letters, const uint8_t* tiles, int tc,
-                       float* out32, double* out64) {
-    const Layout L = MakeLayout(enc_hidden);
+// A grid from a played board: the solution is the guess of an all-green turn when there is one, else a
+// word none of the guesses equals (so the board counts as won exactly when a turn is all green).
+Grid BoardGrid(const uint8_t* letters, const uint8_t* tiles, int tc) {
     Grid g{};
     g.tc = tc;
     for (int t = 0; t < tc; ++t) {
         std::memcpy(g.guess[t], letters + 5 * t, 5);
         std::memcpy(g.tile[t], tiles + 5 * t, 5);
     }
-    std::memset(g.sol, 25, 5);
+    for (uint8_t l = 25;; --l) {  // 26 candidate words, at most 6 guesses: one is free
+        std::memset(g.sol, l, 5);
+        bool used = false;
+        for (int t = 0; t < tc; ++t) used |= SameWord(g.guess[t], g.sol);
+        if (!used) break;
+    }
     for (int t = 0; t < tc; ++t) {
         bool all_green = true;
         for (int p = 0; p < 5; ++p) all_green &= tiles[5 * t + p] == 0;
         if (all_green) std::memcpy(g.sol, letters + 5 * t, 5);
     }
+    return g;
+}
+
+int oro_policy_forward(const float* genome, int enc_hidden, const uint8_t* letters, const uint8_t* tiles, int tc,
+                       float* out32, double* out64) {
+    const Layout L = MakeLayout(enc_hidden);
+    const Grid g = BoardGrid(letters, tiles, tc);
     if (out32 && !PolicyForward<float>(genome, L, g, out32)) return kPolicyForwardFailed;
     if (out64 && !PolicyForward<double>(genome, L, g, out64)) return kPolicyForwardFailed;
     return kOk;
 }
 
 void oro_fixed_features(const uint8_t* word, float* out) { FixedFeatures(word, out); }
+
+// The next guess for a played board (TrySelectNextGuessWithSingleModelDeviceRuntime,
+// single_model_device_runtime.cu:122-181; dynamic_policy.hpp:62-150): the policy forward and the masked
+// argmax over the first A catalog words, in fp32 and in fp64.  Returns kPolicyForwardFailed for an
+// invalid or finished board, kActionSelectionFailed when every action was guessed.
+int oro_select_guess(const float* genome, uint64_t A, const uint8_t* catalog, const uint8_t* letters,
+                     const uint8_t* tiles, int tc, uint64_t* index32, float* best32, float* second32,
+                     uint64_t* index64, double* best64, double* second64) {
+    if (tc < 0 || tc > kMaxTurns) return kPolicyForwardFailed;
+    for (int i = 0; i < 5 * tc; ++i)
+        if (letters[i] >= kAlpha || tiles[i] > 2) return kPolicyForwardFailed;
+    const Layout L = MakeLayout(0);
+    const Grid g = BoardGrid(letters, tiles, tc);
+    float pol32[kPol];
+    double pol64[kPol];
+    if (!PolicyForward<float>(genome, L, g, pol32) || !PolicyForward<double>(genome, L, g, pol64))
+        return kPolicyForwardFailed;
+    std::vector<float> fixed(kFixed * A);
+    for (uint64_t a = 0; a < A; ++a) FixedFeatures(catalog + 5 * a, fixed.data() + kFixed * a);
+    const float* tails = genome + L.P;
+    if (!SelectAction<float>(pol32, tails, catalog, fixed.data(), A, g, index32, best32, second32) ||
+        !SelectAction<double>(pol64, tails, catalog, fixed.data(), A, g, index64, best64, second64))
+        return kActionSelectionFailed;
+    return kOk;
+}
 
 // One turn through the shared encoder (shared_encoder.hpp:186-207).
 void oro_encode_turn(const float* genome, int enc_hidden, const uint8_t* letters, const uint8_t* tiles, float* out32,

@@ -28,6 +28,7 @@ _SIGS = {
     "oro_feedback": (None, [P, P, P]),
     "oro_policy_forward": (I32, [P, I32, P, P, I32, P, P]),
     "oro_fixed_features": (None, [P, P]),
+    "oro_select_guess": (I32, [P, U64, P, P, P, I32, P, P, P, P, P, P]),
     "oro_encode_turn": (None, [P, I32, P, P, P, P]),
     "oro_score_action": (F32, [P, P, P]),
     "oro_evaluate": (I32, [P, U64, I32, U64, U64, P, U64, P, P, I32, I32, P, P, P, P, P, P]),
@@ -142,6 +143,26 @@ def policy_forward(genome, turns, enc_hidden=0):
     if st != 0:
         raise RuntimeError(f"oro_policy_forward status {st}")
     return o32, o64
+
+
+def select_guess(genome, A, catalog, turns):
+    """The next guess for a board of (word letters[5], tiles[5]) turns: dict with status, the fp32 choice
+    (index, best, second) and the fp64 one."""
+    g = np.ascontiguousarray(genome, np.float32)
+    cat = np.ascontiguousarray(catalog, np.uint8)
+    tc = len(turns)
+    letters = np.zeros((max(tc, 1), 5), np.uint8)
+    tiles = np.zeros((max(tc, 1), 5), np.uint8)
+    for i, (w, t) in enumerate(turns):
+        letters[i] = w
+        tiles[i] = t
+    i32, i64 = C.c_uint64(), C.c_uint64()
+    b32, s32 = C.c_float(), C.c_float()
+    b64, s64 = C.c_double(), C.c_double()
+    st = lib().oro_select_guess(ptr(g), A, ptr(cat), ptr(letters), ptr(tiles), tc, C.byref(i32), C.byref(b32),
+                                C.byref(s32), C.byref(i64), C.byref(b64), C.byref(s64))
+    return dict(status=st, index=i32.value, best=np.float32(b32.value), second=np.float32(s32.value),
+                index64=i64.value, best64=b64.value, second64=s64.value)
 
 
 def evaluate(genomes, enc_hidden, A_genome, A_runtime, catalog, offsets, ids, mode=0, threads=1, trace=False):
