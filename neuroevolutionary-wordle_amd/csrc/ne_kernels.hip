@@ -1686,6 +1686,14 @@ __device__ __forceinline__ float MutatedValue(float v, uint32_t e, uint32_t chil
     return __builtin_fmaf(sigma, NormalFromWords(z.x, z.y), v);
 }
 
+#ifndef NE_BREED_BATCH
+#define NE_BREED_BATCH 4
+#endif
+// quads per lane per batch (at most 8: 32 pending bits); measured on 4,096 children of 313,010 elements at
+// mutation p = 0.01: 1 quad 2.12 ms, 4 quads 1.87 ms, 8 quads 1.95 ms, without mutation 1.79 ms
+constexpr int kBreedBatch = NE_BREED_BATCH;
+static_assert(kBreedBatch >= 1 && kBreedBatch <= 8, "breed batch");
+
 // Crossover + mutation, element-parallel: each thread builds 4 consecutive values of one child
 // (BreedAndMutateGenome, genome_ops.cuh:386-407, on counter-based draws).  Writes the full slot.
 __global__ __launch_bounds__(256) void BreedKernel(BreedArgs p) {
@@ -1749,38 +1757,67 @@ __global__ __launch_bounds__(256) void BreedKernel(BreedArgs p) {
             if (e0 + i < n_parent) v[i] = (dd[i] ? p1 : p2)[e0 + i];
         return v;
     };
-    // mutation and tail-row scaling of quad q, then the store
-    auto finish_quad = [&](uint32_t q, v4f v) __device__ {
+    // tail-row scaling of quad q (element draws per row), then the store
+    auto store_quad = [&](uint32_t q, v4f v) __device__ {
         const uint32_t e0 = 4 * q;
-        if (e0 < n_parent) {
-            if (mutate) {
-                const U4 w = Philox4x32(q, child, p.gen_index, 0, p.seed, kTagMuta);
+        if (e0 < n_parent && p.row_scale_p > 0.0f) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    if (e0 + i < n_parent && BernoulliDraw(p.mutation_p, Lane(w, i)))
-                        v[i] = MutatedValue(v[i], e0 + i, child, p.gen_index, p.seed, p.mutation_sigma);
-            }
-            if (p.row_scale_p > 0.0f) {
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const uint32_t e = e0 + i;
-                    if (e < uint32_t(kPolicyParams) || e >= n_parent) continue;
-                    const uint32_t row = (e - uint32_t(kPolicyParams)) / kTail;
-                    const U4 w = Philox4x32(row, child, p.gen_index, 0, p.seed, kTagRscl);
-                    if (BernoulliDraw(p.row_scale_p, w.x)) v[i] = v[i] * (BernoulliDraw(0.5f, w.y) ? 1.02f : 0.98f);
-                }
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t e = e0 + i;
+                if (e < uint32_t(kPolicyParams) || e >= n_parent) continue;
+                const uint32_t row = (e - uint32_t(kPolicyParams)) / kTail;
+                const U4 w = Philox4x32(row, child, p.gen_index, 0, p.seed, kTagRscl);
+                if (BernoulliDraw(p.row_scale_p, w.x)) v[i] = v[i] * (BernoulliDraw(0.5f, w.y) ? 1.02f : 0.98f);
             }
         }
         *reinterpret_cast<v4f*>(out + e0) = v;
     };
-    // one quad ahead: the next quad's load is in flight while this one's draws are computed
-    uint32_t q = p.elem_begin / 4 + blockIdx.x * blockDim.x + threadIdx.x;
-    v4f cur = q < quads ? load_quad(q) : v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    for (; q < quads; q += stride) {
-        const uint32_t qn = q + stride;
-        const v4f nxt = qn < quads ? load_quad(qn) : v4f{0.0f, 0.0f, 0.0f, 0.0f};
-        finish_quad(q, cur);
-        cur = nxt;
+    // Batches of kBreedBatch quads per lane: the batch's loads are in flight while its Bernoulli draws are
+    // computed, and the elements drawn for mutation are then mutated one per pass across the whole batch,
+    // so the wave runs the normal draw as often as its busiest lane needs over kBreedBatch quads (about
+    // once per 4 quads at p = 0.01) instead of once for every element index some lane drew.
+    constexpr int kB = kBreedBatch;
+    const v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t base = p.elem_begin / 4 + blockIdx.x * blockDim.x + threadIdx.x; base < quads;
+         base += uint32_t(kB) * stride) {
+        v4f v[kB];
+#pragma unroll
+        for (int k = 0; k < kB; ++k) {
+            const uint32_t qk = base + uint32_t(k) * stride;
+            v[k] = qk < quads ? load_quad(qk) : zero;
+        }
+        if (mutate) {
+            uint32_t pending = 0;  // bit 4k+i: element i of quad k drawn for mutation
+#pragma unroll
+            for (int k = 0; k < kB; ++k) {
+                const uint32_t qk = base + uint32_t(k) * stride;
+                if (qk >= quads || 4 * qk >= n_parent) continue;
+                const U4 w = Philox4x32(qk, child, p.gen_index, 0, p.seed, kTagMuta);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (4 * qk + i < n_parent && BernoulliDraw(p.mutation_p, Lane(w, i))) pending |= 1u << (4 * k + i);
+            }
+            while (pending != 0) {
+                const uint32_t b = uint32_t(__builtin_ctz(pending));
+                pending &= pending - 1u;
+                float x = 0.0f;
+#pragma unroll
+                for (int k = 0; k < kB; ++k)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) x = b == uint32_t(4 * k + i) ? v[k][i] : x;
+                const uint32_t e = 4 * (base + (b >> 2) * stride) + (b & 3u);
+                const float m = MutatedValue(x, e, child, p.gen_index, p.seed, p.mutation_sigma);
+#pragma unroll
+                for (int k = 0; k < kB; ++k)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) v[k][i] = b == uint32_t(4 * k + i) ? m : v[k][i];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kB; ++k) {
+            const uint32_t qk = base + uint32_t(k) * stride;
+            if (qk < quads) store_quad(qk, v[k]);
+        }
     }
 }
 
