@@ -4,7 +4,10 @@ A step = one full generation (ne_advance_generation): evaluate every genome's ga
 assemble children (crossover + mutation), swap.  Workload: 64x64 toroidal grid (4,096 genomes),
 4,739 synthetic actions, 2,048 secrets in 64 block shards of 32 -> 393,216 games per generation.
 The plain FP32 form is `value`; the BF16-screened form with exact FP32 rescoring (same decisions,
-same fitness) is timed over the same generations and reported beside it.  Prints one JSON line.
+same fitness) is timed over the same generations and reported beside it, and so are both forms with every
+step replayed as one captured hipGraph (ne_set_step_graph: same results; only the whole replay can be
+timed, so `value` and the roofline stay on the eager steps, whose kernels are timed with HIP events).
+Prints one JSON line.
 """
 import argparse
 import json
@@ -121,9 +124,11 @@ def main():
         print(f"[bench] cpu baseline {cpu['value']:.1f} game-evals/s on {cpu['cores']} threads", file=sys.stderr,
               flush=True)
 
-    def run_form(mode):
-        """Bootstraps the same generation 0 and times `steps` generation steps in one form."""
+    def run_form(mode, graph=False):
+        """Bootstraps the same generation 0 and times `steps` generation steps in one form (eagerly, or as
+        captured step graphs: then only the whole replay is timed)."""
         rt.set_eval_precision(mode)
+        rt.set_step_graph(graph)
         rt.bootstrap(n, seed=7, variance_scale=1.0, tail_std=0.1)
         gen = 0
         for _ in range(args.warmup):
@@ -142,12 +147,15 @@ def main():
             fallbacks += st.fallbacks
         elapsed = time.perf_counter() - t0
         _, fit, _, _ = rt.generation(0)
+        rt.set_step_graph(False)
         return dict(elapsed=elapsed, eval_ms=float(np.mean(eval_ms)), flops=float(np.mean(flops)),
                     decisions=decisions, rescored=rescored, fallbacks=fallbacks, stages=rt.stage_times(),
                     summary=rt.summary(), fitness=fit.copy())
 
     fp32 = run_form(rtm.EVAL_FP32)
     bf16 = run_form(rtm.EVAL_BF16_RESCORED) if not args.fp32_only else None
+    fp32_graph = run_form(rtm.EVAL_FP32, graph=True)
+    bf16_graph = run_form(rtm.EVAL_BF16_RESCORED, graph=True) if not args.fp32_only else None
     rt.close()
     st = fp32["stages"]
     value = games_per_gen * args.steps / fp32["elapsed"]
@@ -165,6 +173,15 @@ def main():
             "same_fitness_as_fp32": bool(np.array_equal(bf16["fitness"].view(np.uint32),
                                                         fp32["fitness"].view(np.uint32))),
         }
+    def graph_record(rec, eager):
+        return {"value": games_per_gen * args.steps / rec["elapsed"], "unit": "game-evals/s",
+                "ms_per_step": rec["elapsed"] / args.steps * 1e3,
+                "replay_ms": float(rec["stages"].step_graph_ms),
+                "same_fitness_as_eager": bool(np.array_equal(rec["fitness"].view(np.uint32),
+                                                             eager["fitness"].view(np.uint32)))}
+    graph_out = {"fp32": graph_record(fp32_graph, fp32)}
+    if bf16 is not None:
+        graph_out["bf16_screened_fp32_rescored"] = graph_record(bf16_graph, bf16)
     out = {
         "metric": "game-evals/sec (plain FP32 generation step)",
         "value": value,
@@ -190,6 +207,7 @@ def main():
         "generations_per_s": args.steps / fp32["elapsed"],
         "decisions_per_generation": fp32["decisions"] / args.steps,
         "bf16_screened_fp32_rescored": bf16_out,
+        "captured_step_graph": graph_out,
         "best_fitness": summary.best_fitness,
         "cpu_baseline": cpu,
     }

@@ -1398,6 +1398,10 @@ constexpr int kMaxMoore = 25;
 
 __global__ __launch_bounds__(kPlanBlock) void PlanKernel(PlanArgs p) {
     __shared__ uint32_t s_cand[kPlanBlock][kMaxMoore];
+    if (p.step != nullptr) {
+        p.gen_index = p.step[0];
+        p.generation_seed = p.step[1];
+    }
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= p.child_count) return;
     const uint32_t cols = p.cols, rows = p.cur_size / cols;
@@ -1485,6 +1489,10 @@ __global__ __launch_bounds__(64 * kPlanWaves) void PlanListedKernel(PlanArgs p) 
     __shared__ uint64_t s_key[kPlanWaves][32];
     __shared__ uint32_t s_elig[kPlanWaves][32];
     __shared__ float s_wt[kPlanWaves][32];
+    if (p.step != nullptr) {
+        p.gen_index = p.step[0];
+        p.generation_seed = p.step[1];
+    }
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t c = blockIdx.x * kPlanWaves + wave;
     if (c >= p.child_count) return;  // whole wave
@@ -1585,6 +1593,10 @@ __global__ __launch_bounds__(64 * kPlanWaves) void PlanListedKernel(PlanArgs p) 
 __global__ void XoverDecisionKernel(XoverArgs p) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= p.child_count) return;
+    if (p.step != nullptr) {
+        p.gen_index = p.step[0];
+        p.seed = p.step[1];
+    }
     XoverRecord r;
     const U4 s0 = Philox4x32(c, p.gen_index, 0, 0, p.seed, kTagXovr);
     const bool enc_first = !BernoulliDraw(p.t1, s0.x);
@@ -1706,14 +1718,23 @@ __global__ __launch_bounds__(256) void BreedKernel(BreedArgs p) {
     __shared__ int s_bad;
     // the child's donor-change positions (count, then ascending), from the crossover-decision kernel
     __shared__ uint32_t s_brk[kBreakStride];
+    if (p.step != nullptr) {
+        p.gen_index = p.step[0];
+        p.seed = p.step[1];
+    }
     if (threadIdx.x == 0) {
         s_rec = p.records[child];
         const uint32_t pa = p.pairs[2 * child], pb = p.pairs[2 * child + 1];
         const uint32_t sa = p.parent_slots[pa], sb = p.parent_slots[pb];
-        const uint32_t cs =
-            p.given_slots != nullptr ? p.given_slots[child] : p.free_stack[p.free_count_before - 1 - bc];
-        s_bad = (sa >= p.slot_count || sb >= p.slot_count || cs >= p.slot_count) ? 1 : 0;
-        if (s_bad) SetStatus(p.status, 13);  // kInvalidParentIndex
+        const uint32_t fcb = p.free_count_dev != nullptr ? *p.free_count_dev : p.free_count_before;
+        const bool room = p.given_slots != nullptr || fcb >= gridDim.y;
+        const uint32_t cs = p.given_slots != nullptr ? p.given_slots[child]
+                                                     : (room ? p.free_stack[fcb - 1 - bc] : kInvalidSlot);
+        s_bad = (!room || sa >= p.slot_count || sb >= p.slot_count || cs >= p.slot_count) ? 1 : 0;
+        if (!room)
+            SetStatus(p.status, 14);  // kSlabFull
+        else if (s_bad)
+            SetStatus(p.status, 13);  // kInvalidParentIndex
         s_p1 = reinterpret_cast<const float*>(p.slab + uint64_t(sa) * p.parent_stride);
         s_p2 = reinterpret_cast<const float*>(p.slab + uint64_t(sb) * p.parent_stride);
         s_child = reinterpret_cast<float*>(p.slab + uint64_t(cs) * p.child_stride);
@@ -1974,7 +1995,8 @@ __global__ void SpillRestoreKernel(uint32_t* child_slots, uint32_t k, uint32_t f
 }
 
 __global__ void PopSlotsKernel(uint32_t* free_count, uint32_t count) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *free_count -= count;
+    // a batch larger than the free count was refused by the assembly kernel (slab full): nothing popped
+    if (threadIdx.x == 0 && blockIdx.x == 0 && *free_count >= count) *free_count -= count;
 }
 
 __global__ void InitSlabMetadataKernel(uint32_t* slot_state, uint32_t* free_stack, uint32_t* free_count,
@@ -2051,6 +2073,20 @@ __global__ void RepackScatterKernel(uint8_t* slab, uint64_t new_stride, uint64_t
 __global__ void FillU32Kernel(uint32_t* dst, uint32_t value, uint64_t count) {
     for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < count; i += uint64_t(gridDim.x) * blockDim.x)
         dst[i] = value;
+}
+
+__global__ void FillU8Kernel(uint8_t* dst, uint8_t value, uint64_t count) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < count; i += uint64_t(gridDim.x) * blockDim.x)
+        dst[i] = value;
+}
+
+__global__ void StepReadbackKernel(const int* status, const uint32_t* free_count, const SummaryRecord* summary,
+                                   const unsigned long long* counters, StepReadback* out) {
+    if (threadIdx.x != 0) return;
+    out->status = *status;
+    out->free_count = *free_count;
+    out->summary = *summary;
+    for (int i = 0; i < 4; ++i) out->counters[i] = counters[i];
 }
 
 // ================================================================================================
@@ -2303,6 +2339,18 @@ void LaunchFillU32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t s)
     uint64_t blocks = (count + 255) / 256;
     if (blocks > 1024) blocks = 1024;
     hipLaunchKernelGGL(FillU32Kernel, dim3(uint32_t(blocks)), dim3(256), 0, s, dst, value, count);
+}
+
+void LaunchFillU8(uint8_t* dst, uint8_t value, uint64_t count, hipStream_t s) {
+    if (count == 0) return;
+    uint64_t blocks = (count + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(FillU8Kernel, dim3(uint32_t(blocks)), dim3(256), 0, s, dst, value, count);
+}
+
+void LaunchStepReadback(const int* status, const uint32_t* free_count, const SummaryRecord* summary,
+                        const unsigned long long* counters, StepReadback* out, hipStream_t s) {
+    hipLaunchKernelGGL(StepReadbackKernel, dim3(1), dim3(64), 0, s, status, free_count, summary, counters, out);
 }
 
 }  // namespace ne

@@ -62,6 +62,7 @@ struct PlanArgs {
     const float* rank_weights;     // [rank] -> weight, rank 1..cur_size
     uint32_t* pairs;
     int* status;
+    const uint32_t* step;          // optional: device {gen_index, generation_seed} in place of the fields
 };
 
 // Per child, the element positions where the donor parent can change, ascending: [0] = count, then
@@ -76,6 +77,7 @@ struct XoverArgs {
     float t1, t2, t3;
     XoverRecord* records;
     uint32_t* breaks;              // [child_count][kBreakStride]
+    const uint32_t* step;          // optional: device {gen_index, seed} in place of the fields
 };
 
 struct BreedArgs {
@@ -103,6 +105,17 @@ struct BreedArgs {
     uint32_t elem_end;
     const uint32_t* child_list;    // optional: child index of each batch entry (else child_offset + entry)
     const uint32_t* given_slots;   // optional: [child] slots already holding the child (completing it)
+    const uint32_t* step;          // optional: device {gen_index, seed} in place of the fields
+    const uint32_t* free_count_dev;  // optional: the free count read on the device in place of
+                                     // free_count_before; fewer free slots than the batch -> slab full
+};
+
+// What a captured generation step leaves for one host read after its replay.
+struct StepReadback {
+    int status;
+    uint32_t free_count;
+    SummaryRecord summary;
+    unsigned long long counters[4];
 };
 
 struct BootArgs {
@@ -252,6 +265,9 @@ void LaunchRepackPlan(const RepackArgs& a, hipStream_t s);
 void LaunchRepackCopies(uint8_t* slab, uint64_t old_stride, uint64_t new_stride, const uint32_t* move_src,
                         uint32_t survivors, uint32_t base, uint8_t* staging, hipStream_t s);
 void LaunchFillU32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t s);
+void LaunchFillU8(uint8_t* dst, uint8_t value, uint64_t count, hipStream_t s);
+void LaunchStepReadback(const int* status, const uint32_t* free_count, const SummaryRecord* summary,
+                        const unsigned long long* counters, StepReadback* out, hipStream_t s);
 
 // ---- generation telemetry (ne_telemetry.hip) ----
 constexpr uint32_t kFitnessStatsMax = 16384;  // generation sizes the LDS sort holds (64 KB)

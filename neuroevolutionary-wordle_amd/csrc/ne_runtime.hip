@@ -32,6 +32,34 @@ struct StageEvents {
     hipEvent_t begin = nullptr, end = nullptr;
 };
 
+// One evaluation's launch description: the kernel arguments, the workgroup shape and the games played.
+struct EvalSetup {
+    EvalArgs a;
+    uint32_t blocks, waves;
+    bool bf16, trace;
+    uint64_t total_games;
+};
+
+// Everything a captured generation step bakes in: the kernel arguments (with the per-step values left
+// to the device block) and the generation buffers the clears and the summary use.  Compared byte for
+// byte: the structures are zero-filled before their fields are set.
+struct StepGraphKey {
+    EvalArgs a;
+    PlanArgs pa;
+    XoverArgs xa;
+    BreedArgs b;
+    uint64_t n, next_n, max_gen;
+    uint32_t blocks, waves, bf16, pad;
+    const void* buffers[8];
+};
+
+struct StepGraph {
+    StepGraphKey key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t last_use = 0;
+};
+
 }  // namespace
 
 struct ne_runtime {
@@ -118,7 +146,7 @@ struct ne_runtime {
     uint64_t trace_capacity = 0;
 
     // timing
-    StageEvents ev[11];
+    StageEvents ev[12];
     ne_stage_times times{};
     ne_fitness_summary last_summary{};
     bool have_summary = false;
@@ -129,6 +157,19 @@ struct ne_runtime {
     uint32_t* d_work_counter = nullptr;
     uint64_t cu_count = 256;
     uint32_t tuned_waves = 0;           // 0: the idle-slot heuristic picks the workgroup size
+
+    // captured generation steps (ne_set_step_graph): one graph per buffer parity, the evaluation's host
+    // preparation cached while its inputs stay the same
+    bool step_graph_on = false;
+    StepGraph step_graphs[2];
+    uint64_t step_graph_uses = 0;
+    uint32_t* d_step = nullptr;             // {gen_index, generation_seed} of the step being replayed
+    uint32_t* h_step = nullptr;             // pinned
+    StepReadback* d_readback = nullptr;
+    StepReadback* h_readback = nullptr;     // pinned
+    uint64_t input_epoch = 1;               // bumped by every catalog or secrets upload
+    EvalSetup cached_eval{};
+    uint64_t cached_eval_key[10] = {};
 
     // surrogate screening
     bool screen_on = false;
@@ -193,6 +234,13 @@ uint16_t Bf16Bits(float x) {
     if ((u & 0x7FFFFFFFu) > 0x7F800000u) return uint16_t((u >> 16) | 0x40u);
     u += 0x7FFFu + ((u >> 16) & 1u);
     return uint16_t(u >> 16);
+}
+
+static void DestroyStepGraph(StepGraph& g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g.exec = nullptr;
+    g.graph = nullptr;
 }
 
 int Fail(ne_runtime* rt, int code) {
@@ -359,13 +407,14 @@ bool ValidAssembly(const ne_assembly_config& c) {
            unit(c.output_tail_row_scale_mutation_probability);
 }
 
+// Fill kernels rather than memsets, so that a captured step holds no memset node.
 int ClearGenerationBuffers(ne_runtime* rt, bool next) {
     uint32_t* slots = next ? rt->nxt_slots : rt->cur_slots;
     LaunchFillU32(slots, kInvalidSlot, rt->max_gen, rt->stream);
-    bool ok = Ok(hipMemsetAsync(next ? rt->nxt_fit : rt->cur_fit, 0, rt->max_gen * sizeof(float), rt->stream));
-    ok &= Ok(hipMemsetAsync(next ? rt->nxt_evals : rt->cur_evals, 0, rt->max_gen * sizeof(uint32_t), rt->stream));
-    ok &= Ok(hipMemsetAsync(next ? rt->nxt_has : rt->cur_has, 0, rt->max_gen, rt->stream));
-    return ok ? NE_OK : Fail(rt, NE_DEVICE_FAILURE);
+    LaunchFillU32(reinterpret_cast<uint32_t*>(next ? rt->nxt_fit : rt->cur_fit), 0u, rt->max_gen, rt->stream);
+    LaunchFillU32(next ? rt->nxt_evals : rt->cur_evals, 0u, rt->max_gen, rt->stream);
+    LaunchFillU8(next ? rt->nxt_has : rt->cur_has, 0, rt->max_gen, rt->stream);
+    return Ok(hipGetLastError()) ? NE_OK : Fail(rt, NE_DEVICE_FAILURE);
 }
 
 int EnsureTrace(ne_runtime* rt, uint64_t games) {
@@ -386,9 +435,11 @@ int EnsureTrace(ne_runtime* rt, uint64_t games) {
 // Evaluates n genomes: slots[e] (device) plays the 3 episode variants of every secret in
 // secs[offs[e] .. offs[e+1]).  Fitness, evaluation counts and has-fitness go to out_* (device, n each);
 // rt->times and rt->last_summary describe this evaluation.
-static int EvaluateCore(ne_runtime* rt, uint64_t action_space, const uint32_t* slots, uint64_t n,
-                        const std::vector<uint32_t>& offs, const std::vector<uint32_t>& secs, float* out_fit,
-                        uint32_t* out_evals, uint8_t* out_has, uint64_t generation_index) {
+// Host half of an evaluation: validation, the CSR and work-list uploads, buffers and the kernel arguments.
+// Nothing here is captured into a graph.
+static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32_t* slots, uint64_t n,
+                             const std::vector<uint32_t>& offs, const std::vector<uint32_t>& secs, EvalSetup* out) {
+    ++rt->input_epoch;  // the CSR and work-list buffers are about to hold this evaluation's lists
     if (n == 0 || n > rt->max_gen || offs.size() < n + 1) return Fail(rt, NE_INVALID_GENERATION);
     for (uint64_t i = 0; i < offs[n]; ++i)
         if (secs[i] >= rt->catalog_count) return Fail(rt, NE_INVALID_TRAINING_SHARD);
@@ -422,16 +473,10 @@ static int EvaluateCore(ne_runtime* rt, uint64_t action_space, const uint32_t* s
         ok &= Ok(hipMemsetAsync(rt->d_trace, 0xFF, total_games * sizeof(GameTrace), rt->stream));
         rt->trace_games = total_games;
     }
-    ok &= Ok(hipMemsetAsync(out_fit, 0, n * sizeof(float), rt->stream));
-    ok &= Ok(hipMemsetAsync(out_evals, 0, n * sizeof(uint32_t), rt->stream));
-    ok &= Ok(hipMemsetAsync(out_has, 0, n, rt->stream));
-    ok &= Ok(hipMemsetAsync(rt->raw, 0, rt->max_gen * sizeof(float), rt->stream));
-    ok &= Ok(hipMemsetAsync(rt->local_w, 0, rt->max_gen * sizeof(uint32_t), rt->stream));
-    ok &= Ok(hipMemsetAsync(rt->counters, 0, 4 * sizeof(unsigned long long), rt->stream));
-    ok &= Ok(hipMemsetAsync(rt->status, 0, sizeof(int), rt->stream));
     if (!ok) return Fail(rt, NE_DEVICE_FAILURE);
 
-    EvalArgs a{};
+    EvalArgs a;
+    std::memset(&a, 0, sizeof(a));  // the arguments are compared byte for byte by the step graph
     a.slab = rt->slab;
     a.stride = rt->stride;
     a.slot_count = uint32_t(rt->slot_count);
@@ -499,7 +544,6 @@ static int EvaluateCore(ne_runtime* rt, uint64_t action_space, const uint32_t* s
     }
     if (!Ok(hipMemcpyAsync(rt->d_work, work.data(), work.size() * 4, hipMemcpyHostToDevice, rt->stream)))
         return Fail(rt, NE_DEVICE_FAILURE);
-    LaunchFillU32(rt->d_work_counter, 0u, 1, rt->stream);
     a.work = rt->d_work;
     a.work_count = uint32_t(items);
     a.work_counter = rt->d_work_counter;
@@ -519,38 +563,56 @@ static int EvaluateCore(ne_runtime* rt, uint64_t action_space, const uint32_t* s
             rt->bf16_ops_bytes = per_genome * rt->max_gen;
         }
         if (rt->tail_absmax == nullptr && !Alloc(&rt->tail_absmax, rt->max_gen * 40)) return Fail(rt, NE_DEVICE_FAILURE);
-        if (!Ok(hipMemsetAsync(rt->tail_absmax, 0, n * 40 * sizeof(uint32_t), rt->stream)))
-            return Fail(rt, NE_DEVICE_FAILURE);
         a.bf16_ops = rt->bf16_ops;
         a.bf16_stride = per_genome;
         a.tail_absmax = reinterpret_cast<const float*>(rt->tail_absmax);
     }
-    EventBegin(rt, 8);
-    LaunchTransposeEncoder(rt->slab, rt->stride, slots, uint32_t(n), uint32_t(rt->slot_count), rt->enc_t, rt->stream);
-    if (bf16)
-        LaunchPackBf16(rt->slab, rt->stride, slots, uint32_t(n), uint32_t(rt->slot_count), rt->img,
-                       a.action_count, rt->bf16_ops, a.bf16_stride, rt->tail_absmax, rt->stream);
-    EventEnd(rt, 8);
-    EventBegin(rt, 0);
-    LaunchEvaluate(a, blocks, waves, trace, bf16, rt->stream);
-    EventEnd(rt, 0);
-    if (!Ok(hipGetLastError())) return Fail(rt, NE_DEVICE_FAILURE);
-    int st = Finish(rt);
-    if (st != NE_OK) return st;
-    EventBegin(rt, 1);
+    out->a = a;
+    out->blocks = blocks;
+    out->waves = waves;
+    out->bf16 = bf16;
+    out->trace = trace;
+    out->total_games = total_games;
+    return NE_OK;
+}
+
+// Device half, part 1: clears (fill kernels, no memset nodes, so the sequence can be captured), the
+// encoder transpose (+ BF16 pack) and the game-play kernel.  `timed`: stage events around the launches.
+static void LaunchEvaluation(ne_runtime* rt, const EvalSetup& s, uint64_t n, float* out_fit, uint32_t* out_evals,
+                             uint8_t* out_has, bool timed) {
+    LaunchFillU32(reinterpret_cast<uint32_t*>(out_fit), 0u, n, rt->stream);
+    LaunchFillU32(out_evals, 0u, n, rt->stream);
+    LaunchFillU8(out_has, 0, n, rt->stream);
+    LaunchFillU32(reinterpret_cast<uint32_t*>(rt->raw), 0u, rt->max_gen, rt->stream);
+    LaunchFillU32(rt->local_w, 0u, rt->max_gen, rt->stream);
+    LaunchFillU32(reinterpret_cast<uint32_t*>(rt->counters), 0u, 8, rt->stream);
+    LaunchFillU32(reinterpret_cast<uint32_t*>(rt->status), 0u, 1, rt->stream);
+    LaunchFillU32(rt->d_work_counter, 0u, 1, rt->stream);
+    if (s.bf16) LaunchFillU32(rt->tail_absmax, 0u, n * 40, rt->stream);
+    if (timed) EventBegin(rt, 8);
+    LaunchTransposeEncoder(rt->slab, rt->stride, s.a.slot_indices, uint32_t(n), uint32_t(rt->slot_count), rt->enc_t,
+                           rt->stream);
+    if (s.bf16)
+        LaunchPackBf16(rt->slab, rt->stride, s.a.slot_indices, uint32_t(n), uint32_t(rt->slot_count), rt->img,
+                       s.a.action_count, rt->bf16_ops, s.a.bf16_stride, rt->tail_absmax, rt->stream);
+    if (timed) EventEnd(rt, 8);
+    if (timed) EventBegin(rt, 0);
+    LaunchEvaluate(s.a, s.blocks, s.waves, s.trace, s.bf16, rt->stream);
+    if (timed) EventEnd(rt, 0);
+}
+
+// Device half, part 2: fitness normalisation and the generation summary.
+static void LaunchFinalizeSummary(ne_runtime* rt, const uint32_t* slots, uint64_t n, float* out_fit,
+                                  uint32_t* out_evals, uint8_t* out_has, bool timed) {
+    if (timed) EventBegin(rt, 1);
     LaunchFinalize(rt->raw, rt->local_w, uint32_t(n), out_fit, out_evals, out_has, rt->status, rt->stream);
     LaunchSummary(out_fit, out_has, slots, uint32_t(n), rt->summary, rt->status, rt->stream);
-    EventEnd(rt, 1);
-    st = Finish(rt);
-    if (st != NE_OK) return st;
-    unsigned long long counters[4] = {0, 0, 0, 0};
-    SummaryRecord sr{};
-    if (!Ok(hipMemcpy(counters, rt->counters, sizeof(counters), hipMemcpyDeviceToHost)) ||
-        !Ok(hipMemcpy(&sr, rt->summary, sizeof(sr), hipMemcpyDeviceToHost)))
-        return Fail(rt, NE_DEVICE_FAILURE);
-    rt->times.evaluate_ms = EventMs(rt, 0);
-    rt->times.prepare_ms = EventMs(rt, 8);
-    rt->times.finalize_ms = EventMs(rt, 1);
+    if (timed) EventEnd(rt, 1);
+}
+
+// The host record of an evaluation: summary, counters, stage times.
+static void RecordEvaluation(ne_runtime* rt, const SummaryRecord& sr, const unsigned long long (&counters)[4],
+                             uint64_t total_games, uint64_t n, uint64_t generation_index) {
     rt->times.decisions = counters[0];
     rt->times.rescored = counters[2];
     rt->times.fallbacks = counters[3];
@@ -565,6 +627,30 @@ static int EvaluateCore(ne_runtime* rt, uint64_t action_space, const uint32_t* s
     rt->last_summary.action_count = rt->action_count;
     rt->last_summary.population_size = n;
     rt->have_summary = true;
+}
+
+static int EvaluateCore(ne_runtime* rt, uint64_t action_space, const uint32_t* slots, uint64_t n,
+                        const std::vector<uint32_t>& offs, const std::vector<uint32_t>& secs, float* out_fit,
+                        uint32_t* out_evals, uint8_t* out_has, uint64_t generation_index) {
+    EvalSetup s;
+    int st = PrepareEvaluation(rt, action_space, slots, n, offs, secs, &s);
+    if (st != NE_OK) return st;
+    LaunchEvaluation(rt, s, n, out_fit, out_evals, out_has, true);
+    if (!Ok(hipGetLastError())) return Fail(rt, NE_DEVICE_FAILURE);
+    st = Finish(rt);
+    if (st != NE_OK) return st;
+    LaunchFinalizeSummary(rt, slots, n, out_fit, out_evals, out_has, true);
+    st = Finish(rt);
+    if (st != NE_OK) return st;
+    unsigned long long counters[4] = {0, 0, 0, 0};
+    SummaryRecord sr{};
+    if (!Ok(hipMemcpy(counters, rt->counters, sizeof(counters), hipMemcpyDeviceToHost)) ||
+        !Ok(hipMemcpy(&sr, rt->summary, sizeof(sr), hipMemcpyDeviceToHost)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    rt->times.evaluate_ms = EventMs(rt, 0);
+    rt->times.prepare_ms = EventMs(rt, 8);
+    rt->times.finalize_ms = EventMs(rt, 1);
+    RecordEvaluation(rt, sr, counters, s.total_games, n, generation_index);
     rt->last_status = NE_OK;
     return NE_OK;
 }
@@ -1295,6 +1381,11 @@ int ne_create(const ne_runtime_config* config, ne_runtime** out_runtime) {
 void ne_destroy(ne_runtime* rt) {
     if (!rt) return;
     if (rt->stream) (void)hipStreamSynchronize(rt->stream);
+    for (StepGraph& g : rt->step_graphs) DestroyStepGraph(g);
+    Free(rt->d_step);
+    Free(rt->d_readback);
+    if (rt->h_step) (void)hipHostFree(rt->h_step);
+    if (rt->h_readback) (void)hipHostFree(rt->h_readback);
     Free(rt->slab);
     Free(rt->slot_state);
     Free(rt->free_stack);
@@ -1378,6 +1469,7 @@ void ne_destroy(ne_runtime* rt) {
 
 int ne_upload_catalog(ne_runtime* rt, const uint8_t* letters, uint64_t word_count) {
     if (!rt || !letters || word_count == 0 || word_count > kCatalogCapacity) return Fail(rt, NE_INVALID_ARGUMENT);
+    ++rt->input_epoch;
     std::vector<uint32_t> codes(word_count), canon(word_count);
     std::vector<float> fixed(word_count * kFixed, 0.0f);
     std::unordered_map<uint32_t, uint32_t> first_index;
@@ -1456,6 +1548,7 @@ int ne_upload_secrets_csr(ne_runtime* rt, const uint32_t* cell_offsets, uint64_t
     rt->h_offsets.assign(cell_offsets, cell_offsets + cell_count + 1);
     rt->h_secrets.assign(secret_ids, secret_ids + secret_count);
     rt->csr_from_caller = true;
+    ++rt->input_epoch;
     return NE_OK;
 }
 
@@ -1464,6 +1557,7 @@ int ne_clear_secrets(ne_runtime* rt) {
     rt->h_offsets.clear();
     rt->h_secrets.clear();
     rt->csr_from_caller = false;
+    ++rt->input_epoch;
     return NE_OK;
 }
 
@@ -1587,6 +1681,185 @@ static int AssemblePrepared(ne_runtime* rt, uint32_t generation_seed, const ne_a
     return NE_OK;
 }
 
+// ---- captured generation step ---------------------------------------------------------------
+// A plain step that needs no host decision between its stages -- no injection or widening, the same
+// population size, every child fitting the free slots in one assembly batch, the secrets given as a CSR,
+// no trace and no callbacks -- is one fixed kernel sequence: clears, encoder transpose (+ BF16 pack),
+// evaluation, finalize + summary, parent selection, refcounts and the release of unreferenced parents,
+// crossover decisions, one assembly batch, the slot pop, the batch's releases, and a gather of what the
+// host reads back.  It is captured once per buffer parity (the current and next generation buffers
+// swap every step) and replayed with the step's generation index and seed in a device block, so a step
+// costs one graph launch and one host read instead of ~25 launches and 6 synchronisations.  The kernels
+// and their arguments are those of the eager step; only the per-step values come from the device block
+// and the assembly batch takes its free count on the device.
+
+static void LaunchStepSequence(ne_runtime* rt, const EvalSetup& es, const PlanArgs& pa, const XoverArgs& xa,
+                               const BreedArgs& b, uint64_t n, uint64_t next_n) {
+    LaunchEvaluation(rt, es, n, rt->cur_fit, rt->cur_evals, rt->cur_has, false);
+    LaunchFinalizeSummary(rt, rt->cur_slots, n, rt->cur_fit, rt->cur_evals, rt->cur_has, false);
+    LaunchPlan(pa, rt->stream);
+    (void)ClearGenerationBuffers(rt, true);
+    LaunchFillU32(rt->refcounts, 0u, rt->max_gen, rt->stream);
+    LaunchRefcount(rt->pairs, uint32_t(next_n), uint32_t(n), rt->refcounts, rt->status, rt->stream);
+    LaunchCollectReleased(rt->cur_slots, uint32_t(n), rt->refcounts, rt->slot_state, rt->free_stack, rt->free_count,
+                          uint32_t(rt->slot_count), rt->status, rt->stream);
+    LaunchXover(xa, rt->stream);
+    LaunchBreed(b, uint32_t(next_n), rt->stream);
+    LaunchPopSlots(rt->free_count, uint32_t(next_n), rt->stream);
+    LaunchReleaseBatch(rt->pairs, 0u, uint32_t(next_n), rt->refcounts, rt->status, rt->stream);
+    LaunchCollectReleased(rt->cur_slots, uint32_t(n), rt->refcounts, rt->slot_state, rt->free_stack, rt->free_count,
+                          uint32_t(rt->slot_count), rt->status, rt->stream);
+    LaunchStepReadback(rt->status, rt->free_count, rt->summary, rt->counters, rt->d_readback, rt->stream);
+}
+
+// Runs the step as a graph when it qualifies (*done = true, the step's status returned); otherwise
+// leaves everything untouched for the eager step (*done = false).
+static int TryGraphStep(ne_runtime* rt, uint32_t seed, const ne_word_counts* counts, const ne_assembly_config& cfg,
+                        bool* done) {
+    *done = false;
+    const uint64_t cols = rt->config.grid_column_count, n = rt->cur_size;
+    if (!rt->step_graph_on || rt->trace_flags != 0 || !rt->csr_from_caller || n == 0 || cols == 0 || n % cols != 0 ||
+        rt->catalog_count == 0 || rt->h_offsets.size() < n + 1)
+        return NE_OK;
+    if (counts->training_word_count > rt->catalog_count || counts->action_space_word_count > rt->catalog_count ||
+        counts->training_word_count > counts->action_space_word_count)
+        return NE_OK;  // the eager step reports it
+    const ne_pending_injection none{};
+    uint64_t next_a = 0, next_n = 0;
+    if (!PlanShape(rt, none, &next_a, &next_n) || next_n != n || next_a != rt->action_count) return NE_OK;
+    // every child must find a free slot in one batch: the free count at the start of the step (it only
+    // grows before the batch, as unreferenced parents are released) covers the whole generation
+    if (HostFreeCount(rt) < next_n) return NE_OK;
+    if (!rt->d_step && (!Alloc(&rt->d_step, 4) || !Ok(hipHostMalloc((void**)&rt->h_step, 16))))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    if (!rt->d_readback && (!Alloc(&rt->d_readback, 1) ||
+                            !Ok(hipHostMalloc((void**)&rt->h_readback, sizeof(StepReadback)))))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    *done = true;
+    rt->times = ne_stage_times{};
+
+    // the evaluation's host preparation (the CSR and work-list uploads), kept while nothing else has
+    // prepared an evaluation or uploaded inputs since (input_epoch) and its shape is the same; only the
+    // slot list changes with the buffer parity
+    auto eval_key = [&](uint64_t (&k)[10]) {
+        const uint64_t v[10] = {rt->input_epoch, n, std::min(rt->action_count, counts->action_space_word_count),
+                                rt->eval_mode, rt->tuned_waves, uint64_t(uintptr_t(rt->slab)), rt->stride,
+                                rt->slot_count, rt->action_count, rt->max_gen};
+        std::memcpy(k, v, sizeof(v));
+    };
+    uint64_t ek[10];
+    eval_key(ek);
+    if (std::memcmp(ek, rt->cached_eval_key, sizeof(ek)) != 0) {
+        std::memset(rt->cached_eval_key, 0, sizeof(rt->cached_eval_key));
+        const int st = PrepareEvaluation(rt, counts->action_space_word_count, rt->cur_slots, n, rt->h_offsets,
+                                         rt->h_secrets, &rt->cached_eval);
+        if (st != NE_OK) return st;
+        eval_key(rt->cached_eval_key);  // after the preparation's own epoch bump
+    }
+    EvalSetup es = rt->cached_eval;
+    es.a.slot_indices = rt->cur_slots;
+    int st = EnsureRankWeights(rt, cfg.rank_exponent, std::max<uint64_t>(n, 25));
+    if (st != NE_OK) return st;
+
+    StepGraphKey key;
+    std::memset(&key, 0, sizeof(key));
+    key.a = es.a;
+    key.blocks = es.blocks;
+    key.waves = es.waves;
+    key.bf16 = es.bf16 ? 1u : 0u;
+    PlanArgs& pa = key.pa;
+    pa.fitness = rt->cur_fit;
+    pa.has_fitness = rt->cur_has;
+    pa.cur_size = uint32_t(n);
+    pa.child_count = uint32_t(next_n);
+    pa.cols = uint32_t(cols);
+    pa.radius = uint32_t(std::min<uint64_t>(cfg.cellular_breeding_radius, 0xFFFFFFFFu));
+    pa.rank_weights = rt->rank_weights;
+    pa.pairs = rt->pairs;
+    pa.status = rt->status;
+    pa.step = rt->d_step;
+    XoverArgs& xa = key.xa;
+    xa.child_count = uint32_t(next_n);
+    xa.parent_action_count = uint32_t(rt->action_count);
+    xa.t1 = cfg.crossover_temperature_level1;
+    xa.t2 = cfg.crossover_temperature_level2;
+    xa.t3 = cfg.crossover_temperature_level3;
+    xa.records = rt->xover;
+    xa.breaks = rt->breaks;
+    xa.step = rt->d_step;
+    BreedArgs& b = key.b;
+    b.slab = rt->slab;
+    b.parent_stride = rt->stride;
+    b.child_stride = rt->stride;
+    b.pairs = rt->pairs;
+    b.parent_slots = rt->cur_slots;
+    b.records = rt->xover;
+    b.breaks = rt->breaks;
+    b.free_stack = rt->free_stack;
+    b.child_slots = rt->nxt_slots;
+    b.slot_state = rt->slot_state;
+    b.parent_action_count = uint32_t(rt->action_count);
+    b.mutation_p = cfg.mutation_probability;
+    b.mutation_sigma = cfg.mutation_sigma;
+    b.row_scale_p = cfg.output_tail_row_scale_mutation_probability;
+    b.slot_count = uint32_t(rt->slot_count);
+    b.status = rt->status;
+    b.elem_begin = 0;
+    b.elem_end = uint32_t(rt->stride / 4);
+    b.step = rt->d_step;
+    b.free_count_dev = rt->free_count;
+    key.n = n;
+    key.next_n = next_n;
+    key.max_gen = rt->max_gen;
+    const void* bufs[8] = {rt->cur_evals, rt->nxt_fit, rt->nxt_evals, rt->nxt_has, rt->refcounts, rt->summary,
+                           rt->d_readback, rt->enc_t};
+    std::memcpy(key.buffers, bufs, sizeof(bufs));
+
+    StepGraph* g = nullptr;
+    for (StepGraph& c : rt->step_graphs)
+        if (c.exec && std::memcmp(&c.key, &key, sizeof(key)) == 0) g = &c;
+    const bool capture = g == nullptr;
+    if (capture) {
+        g = rt->step_graphs[0].last_use <= rt->step_graphs[1].last_use ? &rt->step_graphs[0] : &rt->step_graphs[1];
+        DestroyStepGraph(*g);
+        if (!Ok(hipStreamBeginCapture(rt->stream, hipStreamCaptureModeThreadLocal))) return Fail(rt, NE_DEVICE_FAILURE);
+        LaunchStepSequence(rt, es, key.pa, key.xa, key.b, n, next_n);
+        const bool ended = Ok(hipStreamEndCapture(rt->stream, &g->graph));
+        if (!ended || !Ok(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0))) {
+            DestroyStepGraph(*g);
+            return Fail(rt, NE_DEVICE_FAILURE);
+        }
+        g->key = key;
+    }
+    g->last_use = ++rt->step_graph_uses;
+    rt->h_step[0] = uint32_t(rt->cur_index);
+    rt->h_step[1] = seed;
+    bool ok = Ok(hipMemcpyAsync(rt->d_step, rt->h_step, 8, hipMemcpyHostToDevice, rt->stream));
+    EventBegin(rt, 11);
+    ok &= Ok(hipGraphLaunch(g->exec, rt->stream));
+    EventEnd(rt, 11);
+    ok &= Ok(hipMemcpyAsync(rt->h_readback, rt->d_readback, sizeof(StepReadback), hipMemcpyDeviceToHost, rt->stream));
+    if (!ok) return Fail(rt, NE_DEVICE_FAILURE);
+    st = Sync(rt);
+    if (st != NE_OK) return st;
+    const StepReadback rb = *rt->h_readback;
+    if (rb.status != 0) {
+        (void)hipMemset(rt->status, 0, sizeof(int));
+        return Fail(rt, rb.status);
+    }
+    RecordEvaluation(rt, rb.summary, rb.counters, es.total_games, n, rt->cur_index);
+    rt->times.step_graph_ms = EventMs(rt, 11);
+    rt->times.step_graph = capture ? 2u : 1u;
+    rt->times.children = next_n;
+    rt->cur_evaluated = true;
+    rt->planned_children = next_n;
+    rt->next_index = rt->cur_index + 1;
+    rt->next_size = next_n;
+    SwapGenerations(rt);
+    rt->last_status = NE_OK;
+    return NE_OK;
+}
+
 extern "C" {
 
 int ne_advance_generation(ne_runtime* rt, uint32_t generation_seed, const ne_word_counts* counts,
@@ -1598,6 +1871,13 @@ int ne_advance_generation(ne_runtime* rt, uint32_t generation_seed, const ne_wor
     if (!ValidAssembly(cfg)) return Fail(rt, NE_INVALID_ASSEMBLY_CONFIG);
     ne_pending_injection inj{};
     if (injection) inj = *injection;
+    const bool plain = !inj.enabled && (!callbacks || (!callbacks->training_shard_release &&
+                                                       !callbacks->post_fitness_evaluation));
+    if (plain && rt->step_graph_on) {
+        bool done = false;
+        const int gst = TryGraphStep(rt, generation_seed, counts, cfg, &done);
+        if (done || gst != NE_OK) return gst;
+    }
     uint64_t parent_a = 0;
     const int st = PrepareBoundary(rt, generation_seed, counts, cfg, inj, callbacks, history, &parent_a);
     if (st != NE_OK) return st;
@@ -1768,6 +2048,14 @@ int ne_download_trace(ne_runtime* rt, uint32_t kind, void* dst, uint64_t capacit
 int ne_set_eval_shape(ne_runtime* rt, uint32_t waves) {
     if (!rt || waves > 4) return Fail(rt, NE_INVALID_ARGUMENT);
     rt->tuned_waves = waves;
+    return NE_OK;
+}
+
+int ne_set_step_graph(ne_runtime* rt, uint32_t enabled) {
+    if (!rt || enabled > 1) return Fail(rt, NE_INVALID_ARGUMENT);
+    rt->step_graph_on = enabled != 0;
+    if (!rt->step_graph_on)
+        for (StepGraph& g : rt->step_graphs) DestroyStepGraph(g);
     return NE_OK;
 }
 

@@ -5,10 +5,12 @@
 // from generations K, 2K, ... go through TryCreatePrebreedingCheckpointOnDevice, a background write to
 // --checkpoint PATH and TryResumeGenerationFromCheckpointOnDevice; --resume PATH restores a checkpoint
 // into a new runtime and continues from it (the reference CLI's flow, run_genetic_algorithm.cu:1605-1660,
-// 1900-1967); --winner DIR writes the final generation's best genome as a winner artifact.
+// 1900-1967); --winner DIR writes the final generation's best genome as a winner artifact; --graph runs
+// the plain steps as captured hipGraphs (TrySetCapturedGenerationSteps; the steps then carry no
+// post-fitness callback, which would keep them eager, and evaluations are counted per step instead).
 //
 // usage: run_ga <workload_dir> <grid_columns> <generations> <seed> [mutation_probability]
-//               [--checkpoint PATH --checkpoint-every K] [--resume PATH] [--winner DIR]
+//               [--checkpoint PATH --checkpoint-every K] [--resume PATH] [--winner DIR] [--graph]
 // workload_dir holds catalog.u8 (5 letter indices per word), offsets.u32 (cells+1) and secrets.u32.
 #include <chrono>
 #include <cstdio>
@@ -42,7 +44,7 @@ int main(int argc, char** argv) {
     if (argc < 5) {
         std::fprintf(stderr,
                      "usage: %s <workload_dir> <grid_columns> <generations> <seed> [mutation_p] "
-                     "[--checkpoint PATH --checkpoint-every K] [--resume PATH] [--winner DIR]\n",
+                     "[--checkpoint PATH --checkpoint-every K] [--resume PATH] [--winner DIR] [--graph]\n",
                      argv[0]);
         return 2;
     }
@@ -53,9 +55,12 @@ int main(int argc, char** argv) {
     float mutation_p = 0.01f;
     std::string checkpoint_path, resume_path, winner_dir;
     std::size_t checkpoint_every = 0;
+    bool graph = false;
     for (int a = 5; a < argc; ++a) {
         const std::string arg = argv[a];
-        if (arg == "--checkpoint" && a + 1 < argc) {
+        if (arg == "--graph") {
+            graph = true;
+        } else if (arg == "--checkpoint" && a + 1 < argc) {
             checkpoint_path = argv[++a];
         } else if (arg == "--checkpoint-every" && a + 1 < argc) {
             checkpoint_every = std::strtoull(argv[++a], nullptr, 10);
@@ -107,6 +112,7 @@ int main(int argc, char** argv) {
     }
     if (!TryUploadTrainingWordCatalog(buffers, catalog) || !TryUploadTrainingSecrets(buffers, offsets, secrets))
         return Fail(buffers, "input upload");
+    if (graph && !TrySetCapturedGenerationSteps(buffers, true)) return Fail(buffers, "captured steps");
     std::size_t first_generation = 0;
     if (resumed.Valid()) {
         if (!TryResumeGenerationFromCheckpointOnDevice(buffers, resumed)) return Fail(buffers, "checkpoint resume");
@@ -146,8 +152,10 @@ int main(int argc, char** argv) {
             }
             if (!TryResumeGenerationFromCheckpointOnDevice(buffers, checkpoint)) return Fail(buffers, "resume");
         } else if (!TryAdvanceGenerationOnDevice(buffers, generation_seed, counts, assembly, {}, false,
-                                                 count_evaluations)) {
+                                                 graph ? PostFitnessEvaluationCallback{} : count_evaluations)) {
             return Fail(buffers, "advance");
+        } else if (graph) {
+            ++evaluated;
         }
         PopulationFitnessSummary s{};
         if (!TryReadPopulationFitnessSummaryFromDevice(buffers, s)) return Fail(buffers, "summary");

@@ -321,7 +321,7 @@ struct SurrogateScreeningConfig {
     std::uint32_t brood_factor = 4;
     std::uint32_t replay_capacity = 65536;
     float audit_fraction = 0.05f;
-    float spearman_threshold = 0.5f;
+    float spearman_threshold = 0.0f;
     float learning_rate = 1e-3f;
     std::uint32_t train_steps = 32;
     std::uint32_t batch_size = 256;
@@ -329,6 +329,12 @@ struct SurrogateScreeningConfig {
     std::uint32_t seed = 0x5C2EE7u;
     bool eager_training = false;  // true: launch the training steps one by one instead of a hipGraph replay
 };
+
+// No counterpart in the reference (which launches every stage from the host): qualifying plain steps of
+// TryAdvanceGenerationOnDevice run as one captured hipGraph each, with results equal to the eager step's.
+inline bool TrySetCapturedGenerationSteps(DeviceSlabGARuntimeBuffers& buffers, bool enabled) {
+    return ne_set_step_graph(buffers.handle, enabled ? 1u : 0u) == NE_OK;
+}
 
 inline bool TryEnableSurrogateScreening(DeviceSlabGARuntimeBuffers& buffers, const SurrogateScreeningConfig& config = {}) {
     ne_screen_config c{};

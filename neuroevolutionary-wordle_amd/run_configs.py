@@ -6,10 +6,13 @@
             512 every 5 generations (9 widening repacks + output-embedding injections), 50 generations
   target    64x64 grid, 4,739 actions: wall-clock to the plain FP32 run's best fitness (over G generations,
             default 200) for the BF16-screened form and the surrogate-screened form (brood 4x, best-predicted
-            25% + 5% audit, BF16-screened evaluation), per seed (default 3 seeds)
+            25% + 5% audit, BF16-screened evaluation), per seed (default 10 seeds), then the geometric mean,
+            median and total-time ratio of the speedups
 
 Every run is timed in the plain FP32 form and in the BF16-screened form (which takes the same decisions);
-`--forms fp32` skips the latter.  Usage: python run_configs.py grid16|grid128|ragged|target [--forms fp32,bf16]
+`--forms fp32` skips the latter; grid16 and grid128 also take the forms fp32-graph and bf16-graph (every step
+one captured hipGraph, same results), and target times the BF16-screened form both ways.
+Usage: python run_configs.py grid16|grid128|ragged|target [--forms fp32,bf16,fp32-graph,bf16-graph]
 [--generations G] [--seeds S]
 """
 import argparse
@@ -70,16 +73,17 @@ def plain_config(name, gens, forms):
         try:
             rt.upload_catalog(wl["catalog"])
             rt.upload_secrets_csr(wl["offsets"], wl["ids"])
-            rt.set_eval_precision(rtm.EVAL_FP32 if form == "fp32" else rtm.EVAL_BF16_RESCORED)
+            rt.set_eval_precision(rtm.EVAL_FP32 if form.startswith("fp32") else rtm.EVAL_BF16_RESCORED)
+            rt.set_step_graph(form.endswith("-graph"))
             rt.bootstrap(n, seed=7, variance_scale=1.0, tail_std=0.1)
             rows = _run_generations(rt, gens, lambda g: rtm.word_counts(A, A), lambda g: None, cfg)
             recs.append(_record(name, form, rows, {"genomes": n, "actions": A,
                                                    "slab_bytes": 2 * n * stride + stride}))
         finally:
             rt.close()
-    if len(recs) == 2:
+    if len(recs) >= 2:
         print(json.dumps({"config": name, "same_best_fitness_trajectory":
-                          recs[0]["best_fitness"] == recs[1]["best_fitness"]}), flush=True)
+                          all(r["best_fitness"] == recs[0]["best_fitness"] for r in recs)}), flush=True)
 
 
 def ragged_config(forms, gens=50, side=64, a0=512, a_final=4739, step=512, period=5):
@@ -178,12 +182,13 @@ def target_config(seeds, gens, cap_factor=2):
                "target_fitness": target, "forms": {}}
         rec["forms"]["fp32"] = {"seconds_to_target": t_plain, "generations_to_target": g_plain, "reached": True,
                                 "seconds_per_generation": trace_p[-1][0] / len(trace_p)}
-        for form in ("bf16", "surrogate"):
+        for form in ("bf16", "bf16-graph", "surrogate"):
             rt = make(rtm.EVAL_BF16_RESCORED, 4 if form == "surrogate" else 1)
             try:
                 if form == "surrogate":
                     rt.enable_screening(scfg)
-                tr, scr, rhos = run(rt, plain if form == "bf16" else screened_step, cap_factor * gens, target)
+                rt.set_step_graph(form == "bf16-graph")
+                tr, scr, rhos = run(rt, plain if form != "surrogate" else screened_step, cap_factor * gens, target)
             finally:
                 rt.close()
             reached = tr[-1][1] >= target
@@ -196,6 +201,15 @@ def target_config(seeds, gens, cap_factor=2):
                                            "audit_spearman_mean": float(np.mean(rhos)) if rhos else None})
         print(json.dumps(rec), flush=True)
         out.append(rec)
+    agg = {"config": "target", "seeds": [r["seed"] for r in out]}
+    for form in ("bf16", "bf16-graph", "surrogate"):
+        sp = [r["forms"][form]["speedup_vs_fp32"] for r in out if r["forms"][form]["reached"]]
+        t_form = sum(r["forms"][form]["seconds_to_target"] for r in out if r["forms"][form]["reached"])
+        t_fp32 = sum(r["forms"]["fp32"]["seconds_to_target"] for r in out if r["forms"][form]["reached"])
+        agg[form] = {"reached": len(sp), "geometric_mean_speedup": float(np.exp(np.mean(np.log(sp)))) if sp else None,
+                     "median_speedup": float(np.median(sp)) if sp else None,
+                     "total_time_ratio": t_fp32 / t_form if sp else None}
+    print(json.dumps(agg), flush=True)
     return out
 
 
@@ -204,7 +218,7 @@ def main():
     ap.add_argument("config", choices=["grid16", "grid128", "ragged", "target"])
     ap.add_argument("--forms", default="fp32,bf16")
     ap.add_argument("--generations", type=int, default=0)
-    ap.add_argument("--seeds", type=int, default=3)
+    ap.add_argument("--seeds", type=int, default=10)
     args = ap.parse_args()
     forms = [f for f in args.forms.split(",") if f]
     if args.config == "grid16":

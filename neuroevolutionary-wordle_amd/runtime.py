@@ -90,7 +90,8 @@ class StageTimes(C.Structure):
                 ("assemble_ms", C.c_float), ("repack_ms", C.c_float), ("inject_ms", C.c_float),
                 ("bookkeeping_ms", C.c_float), ("prepare_ms", C.c_float), ("decisions", C.c_uint64),
                 ("encoded_turns", C.c_uint64), ("games", C.c_uint64), ("children", C.c_uint64),
-                ("rescored", C.c_uint64), ("fallbacks", C.c_uint64), ("spilled", C.c_uint64)]
+                ("rescored", C.c_uint64), ("fallbacks", C.c_uint64), ("spilled", C.c_uint64),
+                ("step_graph_ms", C.c_float), ("step_graph", C.c_uint32)]
 
 
 GAME_TRACE_DTYPE = np.dtype([("guesses", "<u4", 6), ("best", "<f4", 6), ("second", "<f4", 6),
@@ -122,8 +123,9 @@ FINGERPRINT_DIMS = 192
 
 def default_screen_config(**kw):
     """The surrogate-screened configuration: brood 4x, top 25% (each cell's best-predicted child) plus a
-    5% random audit, Adam lr 1e-3 on a replay buffer of 65,536 pairs, fallback below Spearman 0.5."""
-    c = ScreenConfig(brood_factor=4, replay_capacity=65536, audit_fraction=0.05, spearman_threshold=0.5,
+    5% random audit, Adam lr 1e-3 on a replay buffer of 65,536 pairs, plain steps while the audit Spearman
+    rank correlation is below 0 (an uninformative or inverted surrogate)."""
+    c = ScreenConfig(brood_factor=4, replay_capacity=65536, audit_fraction=0.05, spearman_threshold=0.0,
                      learning_rate=1e-3, train_steps=32, batch_size=256, min_replay=4096, seed=0x5C2EE7, flags=0)
     for k, v in kw.items():
         setattr(c, k, v)
@@ -211,7 +213,7 @@ EXPORTED_SYMBOLS = [
     "ne_build_shard_csr", "ne_set_eval_precision", "ne_enable_screening", "ne_advance_generation_screened",
     "ne_fingerprint", "ne_get_probe_states", "ne_surrogate_predict", "ne_surrogate_train_step",
     "ne_surrogate_get_params", "ne_surrogate_set_params", "ne_rank_correlation", "ne_autotune_evaluation",
-    "ne_set_eval_shape", "ne_create_prebreeding_checkpoint", "ne_restore_prebreeding_checkpoint",
+    "ne_set_eval_shape", "ne_set_step_graph", "ne_create_prebreeding_checkpoint", "ne_restore_prebreeding_checkpoint",
     "ne_resume_generation_from_checkpoint", "ne_write_checkpoint", "ne_read_checkpoint", "ne_clone_checkpoint",
     "ne_get_checkpoint_info", "ne_get_checkpoint_generation", "ne_get_checkpoint_plan", "ne_get_checkpoint_genome",
     "ne_free_checkpoint", "ne_training_data_identity_hash", "ne_collect_fitness_telemetry",
@@ -271,6 +273,7 @@ def load_library(path=LIB_PATH):
         "ne_autotune_evaluation": (i32, [p, C.POINTER(WordCounts), C.POINTER(ShardReleaseHistory), C.POINTER(u32),
                                          p]),
         "ne_set_eval_shape": (i32, [p, u32]),
+        "ne_set_step_graph": (i32, [p, u32]),
         "ne_create_prebreeding_checkpoint": (i32, [p, u32, C.POINTER(WordCounts), C.POINTER(AssemblyConfig),
                                                    C.POINTER(PendingInjection), p, u64, u32, C.POINTER(Callbacks),
                                                    C.POINTER(ShardReleaseHistory), C.POINTER(p)]),
@@ -571,6 +574,10 @@ class SlabRuntime:
 
     def set_eval_shape(self, waves):
         self._check("ne_set_eval_shape", self.lib.ne_set_eval_shape(self.h, waves))
+
+    def set_step_graph(self, enabled=True):
+        """Plain steps as one captured hipGraph each (bit-identical to the eager step)."""
+        self._check("ne_set_step_graph", self.lib.ne_set_step_graph(self.h, 1 if enabled else 0))
 
     # --- surrogate screening
     def enable_screening(self, config=None):
