@@ -658,11 +658,18 @@ struct PolicyBf16 {
 };
 // A screening chunk of kB16Tiles tiles in LDS: the catalog parts (3 transfers of 1 KB), the genome
 // parts (5 transfers), then the tiles' canonical indices (one 4-byte transfer).
-constexpr uint32_t kB16Tiles = 4;
+// Tiles per screening chunk, a multiple of 4.  Measured at 64x64 / 4,739 actions: 4 tiles 11.0 ms per
+// evaluation; 8 tiles 14.2 ms (50.7 KB of LDS per workgroup: three workgroups per CU instead of four).
+#ifndef NE_B16_TILES
+#define NE_B16_TILES 4
+#endif
+constexpr uint32_t kB16Tiles = NE_B16_TILES;
 constexpr uint32_t kB16CatBytes = kB16Tiles * kBf16CatTileBytes;
 constexpr uint32_t kB16GenBytes = kB16Tiles * kBf16GenTileBytes;
 constexpr uint32_t kB16OpsDma = (kB16CatBytes + kB16GenBytes) / kDmaBytes;
+constexpr uint32_t kB16CanonDma = kB16Tiles / 4;  // 4-byte transfers: 16 indices x 4 tiles each
 constexpr uint32_t kB16BufBytes = kB16CatBytes + kB16GenBytes + kB16Tiles * 64;
+static_assert(kB16Tiles % 4 == 0, "whole canonical-index transfers");
 static_assert(kB16CatBytes % kDmaBytes == 0 && kB16GenBytes % kDmaBytes == 0, "whole transfers");
 
 __device__ __forceinline__ v4f Bf16Screen(const uint8_t* buf, int k, uint32_t lane, const PolicyBf16& pb) {
@@ -801,7 +808,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
         const Rsrc canon_r = MakeRsrc(p.canon, p.canon_bytes);
         auto issue16 = [&](uint8_t* buf, uint32_t c) __device__ {
             constexpr uint32_t kCatDma = kB16CatBytes / kDmaBytes;
-            for (uint32_t i = wave; i < kB16OpsDma + 1; i += waves) {
+            for (uint32_t i = wave; i < kB16OpsDma + kB16CanonDma; i += waves) {
                 if (i < kCatDma)
                     __builtin_amdgcn_raw_ptr_buffer_load_lds(cat_r, (LdsPtr)(buf + i * kDmaBytes), 16,
                                                              c * kB16CatBytes + i * kDmaBytes + 16 * lane, 0, 0, 0);
@@ -810,8 +817,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
                                                              c * kB16GenBytes + (i - kCatDma) * kDmaBytes + 16 * lane, 0,
                                                              0, 0);
                 else
-                    __builtin_amdgcn_raw_ptr_buffer_load_lds(canon_r, (LdsPtr)(buf + kB16OpsDma * kDmaBytes), 4,
-                                                             c * (kB16Tiles * 64) + 4 * lane, 0, 0, 0);
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(canon_r,
+                                                             (LdsPtr)(buf + kB16OpsDma * kDmaBytes + (i - kB16OpsDma) * 256),
+                                                             4, c * (kB16Tiles * 64) + (i - kB16OpsDma) * 256 + 4 * lane,
+                                                             0, 0, 0);
             }
         };
 
