@@ -4,8 +4,9 @@ Bar: a runtime that replays its plain steps as hipGraphs ends every step in the 
 runs them eagerly, bit for bit -- summary, generation slots, fitness and evaluation counts, the plan, the
 free-slot count and every genome of the new generation -- in the FP32 and the BF16-screened forms; the
 first step of each buffer parity captures, later ones replay; a changed configuration captures anew; an
-eager evaluation between steps (which rewrites the work-list buffers) and a step that does not qualify
-(a slab without room for a whole generation) leave the results unchanged.
+eager evaluation between steps (which rewrites the work-list buffers), a step that does not qualify (a slab
+without room for a whole generation), a widening step with injection (new stride: both parities capture
+anew) and surrogate-screened steps in between leave the results unchanged.
 """
 import numpy as np
 import pytest
@@ -150,3 +151,79 @@ def test_cpp_driver_with_captured_steps(pkg, tmp_path):
         assert res.returncode == 0, res.stdout + res.stderr
         out.append([x for x in res.stdout.splitlines() if x.startswith("generation ")])
     assert len(out[0]) == 6 and out[0] == out[1]
+
+
+def test_graph_steps_across_a_widening(pkg):
+    """A widening step (injection, eager) between captured steps: the new stride captures anew."""
+    rtm = pkg.runtime
+    side, a0, a1 = 16, 512, 1024
+    n = side * side
+    catalog = pkg.synth.make_catalog(a1, seed=pkg.synth.CATALOG_SEED, distinct_letters=True)
+    off, ids = pkg.synth.block_csr(side, side, 4, pkg.synth.sample_secrets(a0, 256))
+    s1 = rtm.slot_stride_bytes(a1)
+    cfg = rtm.default_assembly_config(mutation_probability=0.02, mutation_sigma=0.03)
+    inj = rtm.PendingInjection(enabled=1, reserved0=0, first_catalog_word_index=a0, injection_count=a1 - a0)
+
+    def make(graph):
+        rt = pkg.SlabRuntime(slab_bytes=2 * n * s1 + 4 * s1, generation_bytes=n * s1, action_count=a0,
+                             grid_columns=side)
+        rt.upload_catalog(catalog)
+        rt.upload_secrets_csr(off, ids)
+        rt.bootstrap(n, seed=21, variance_scale=1.0, tail_std=0.1)
+        rt.set_step_graph(graph)
+        return rt
+
+    eager, graph = make(False), make(True)
+    try:
+        kinds = []
+        for g in range(6):
+            a = a0 if g <= 2 else a1
+            counts = rtm.word_counts(a, a)
+            for rt in (eager, graph):
+                rt.advance(60 + g, counts, cfg, injection=inj if g == 2 else None)
+            kinds.append(int(graph.stage_times().step_graph))
+            _compare(_state(eager), _state(graph), f"step {g}")
+        assert graph.info().action_count == a1
+        assert kinds == [2, 2, 0, 2, 2, 1], kinds
+    finally:
+        eager.close()
+        graph.close()
+
+
+def test_graph_steps_between_screened_steps(pkg):
+    """Surrogate-screened steps (eager) interleaved with captured plain steps."""
+    rtm = pkg.runtime
+    wl = pkg.synth.workload("grid16_a512")
+    A, side = wl["actions"], wl["side"]
+    n = side * side
+    stride = rtm.slot_stride_bytes(A)
+    counts = rtm.word_counts(A, A)
+    cfg = rtm.default_assembly_config(mutation_probability=0.02, mutation_sigma=0.03)
+    scfg = rtm.default_screen_config(brood_factor=2, min_replay=256)
+
+    def make(graph):
+        rt = pkg.SlabRuntime(slab_bytes=4 * n * stride, generation_bytes=2 * n * stride, action_count=A,
+                             grid_columns=side)
+        rt.upload_catalog(wl["catalog"])
+        rt.upload_secrets_csr(wl["offsets"], wl["ids"])
+        rt.set_eval_precision(rtm.EVAL_BF16_RESCORED)
+        rt.bootstrap(n, seed=33, variance_scale=1.0, tail_std=0.1)
+        rt.enable_screening(scfg)
+        rt.set_step_graph(graph)
+        return rt
+
+    eager, graph = make(False), make(True)
+    try:
+        kinds = []
+        for g, screened in enumerate([False, True, False, True, True, False, False]):
+            for rt in (eager, graph):
+                if screened:
+                    rt.advance_screened(80 + g, counts, cfg)
+                else:
+                    rt.advance(80 + g, counts, cfg)
+            kinds.append(None if screened else int(graph.stage_times().step_graph))
+            _compare(_state(eager), _state(graph), f"step {g}")
+        assert all(k in (1, 2) for k in kinds if k is not None), kinds
+    finally:
+        eager.close()
+        graph.close()
