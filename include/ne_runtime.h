@@ -287,7 +287,8 @@ int ne_set_eval_shape(ne_runtime *rt, uint32_t waves);
  * replayed with the generation index and seed in a device block, then one host read of the status,
  * summary and counters.  Results equal the eager step's bit for bit; ne_get_stage_times reports the
  * graph's replay time only (step_graph_ms).  Any other step, or a change of configuration, shape or
- * buffers, runs eagerly or captures anew.  enabled = 0 (the default) releases the graphs. */
+ * buffers, runs eagerly or captures anew; a capture that cannot be made runs the step eagerly and turns
+ * the mode off.  enabled = 0 (the default) releases the graphs. */
 int ne_set_step_graph(ne_runtime *rt, uint32_t enabled);
 /* Copies a recorded trace; game traces are ordered by individual, then game slot (W fresh games by
  * entry, then 2W prefilled games: slot W+q is entry q/2, variant 1+q%2). */

@@ -1822,12 +1822,21 @@ static int TryGraphStep(ne_runtime* rt, uint32_t seed, const ne_word_counts* cou
     if (capture) {
         g = rt->step_graphs[0].last_use <= rt->step_graphs[1].last_use ? &rt->step_graphs[0] : &rt->step_graphs[1];
         DestroyStepGraph(*g);
-        if (!Ok(hipStreamBeginCapture(rt->stream, hipStreamCaptureModeThreadLocal))) return Fail(rt, NE_DEVICE_FAILURE);
-        LaunchStepSequence(rt, es, key.pa, key.xa, key.b, n, next_n);
-        const bool ended = Ok(hipStreamEndCapture(rt->stream, &g->graph));
-        if (!ended || !Ok(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0))) {
+        // nothing runs during a capture: when it cannot be made, the step runs eagerly and later steps
+        // stay eager (ne_stage_times.step_graph reads 0)
+        bool made = Ok(hipStreamBeginCapture(rt->stream, hipStreamCaptureModeThreadLocal));
+        if (made) {
+            LaunchStepSequence(rt, es, key.pa, key.xa, key.b, n, next_n);
+            made = Ok(hipStreamEndCapture(rt->stream, &g->graph));
+            made = made && Ok(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+        }
+        if (!made) {
             DestroyStepGraph(*g);
-            return Fail(rt, NE_DEVICE_FAILURE);
+            (void)hipGetLastError();
+            rt->step_graph_on = false;
+            rt->times = ne_stage_times{};
+            *done = false;
+            return NE_OK;
         }
         g->key = key;
     }
