@@ -4,6 +4,8 @@
   grid128   128x128 grid (16,384 genomes, ~41 GB slab), 4,739 actions, 8,192 secrets in 256 shards of 32
   ragged    64x64 grid, per-cell shards of U{32..1024} secrets, actions growing 512 -> 4,739 in steps of
             512 every 5 generations (9 widening repacks + output-embedding injections), 50 generations
+  published the reference's two published evaluation workloads (1,024 genomes / 20 actions / 20 words per cell;
+            672 genomes / 2,000 actions / 2,000 words per cell), evaluation only, against its published times
   target    64x64 grid, 4,739 actions: wall-clock to the plain FP32 run's best fitness (over G generations,
             default 200) for the BF16-screened form and the surrogate-screened form (brood 4x, best-predicted
             25% + 5% audit, BF16-screened evaluation), per seed (default 10 seeds), then the geometric mean,
@@ -127,6 +129,47 @@ def ragged_config(forms, gens=50, side=64, a0=512, a_final=4739, step=512, perio
                           recs[0]["best_fitness"] == recs[1]["best_fitness"]}), flush=True)
 
 
+def published_config(forms, reps=5):
+    """The reference's published evaluation workloads (docs/genetic-algorithm/fitness-evaluation-concurrency-
+    results.md:8-35): generation 0 with 1,024 genomes on a 32-column grid, 20 actions and every cell playing
+    all 20 words (61,440 games), and generation 10 with 672 genomes (21 x 32), 2,000 actions and all 2,000
+    words per cell (4,032,000 games).  Synthetic Zipf catalogs, random-init genomes; evaluation only,
+    timed with HIP events (median of `reps` evaluations of the same generation)."""
+    published_ms = {"gen0_a20": 676.9, "gen10_a2000": 98116.9}
+    for name, rows, cols, A in (("gen0_a20", 32, 32, 20), ("gen10_a2000", 21, 32, 2000)):
+        n = rows * cols
+        stride = rtm.slot_stride_bytes(A)
+        catalog = synth.make_catalog(A)
+        offsets = (np.arange(n + 1, dtype=np.uint64) * A).astype(np.uint32)
+        ids = np.tile(np.arange(A, dtype=np.uint32), n)
+        for form in forms:
+            rt = pkg.SlabRuntime(slab_bytes=2 * n * stride + stride, generation_bytes=n * stride, action_count=A,
+                                 grid_columns=cols)
+            try:
+                rt.upload_catalog(catalog)
+                rt.upload_secrets_csr(offsets, ids)
+                rt.set_eval_precision(rtm.EVAL_FP32 if form == "fp32" else rtm.EVAL_BF16_RESCORED)
+                rt.bootstrap(n, seed=7, variance_scale=1.0, tail_std=0.1)
+                counts = rtm.word_counts(A, A)
+                ms, wall = [], []
+                for _ in range(reps):
+                    t0 = time.perf_counter()
+                    rt.evaluate(counts)
+                    wall.append(time.perf_counter() - t0)
+                    st = rt.stage_times()
+                    ms.append(st.prepare_ms + st.evaluate_ms + st.finalize_ms)
+                games = 3 * n * A
+                eval_ms = float(np.median(ms))
+                print(json.dumps({"config": "published", "workload": name, "form": form, "genomes": n, "actions": A,
+                                  "words_per_cell": A, "games": games, "evaluation_ms": eval_ms,
+                                  "wall_ms": 1e3 * float(np.median(wall)), "game_evals_per_s": games / eval_ms * 1e3,
+                                  "decisions": int(st.decisions), "best_fitness": float(rt.summary().best_fitness),
+                                  "reference_rtx5050_laptop_ms": published_ms[name],
+                                  "speedup_vs_published": published_ms[name] / eval_ms}), flush=True)
+            finally:
+                rt.close()
+
+
 def target_config(seeds, gens, cap_factor=2):
     """Wall-clock to target: the plain FP32 run's best fitness over `gens` generations is the target; the
     BF16-screened form (same decisions) and the surrogate-screened form run until their best-so-far
@@ -215,7 +258,7 @@ def target_config(seeds, gens, cap_factor=2):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("config", choices=["grid16", "grid128", "ragged", "target"])
+    ap.add_argument("config", choices=["grid16", "grid128", "ragged", "target", "published"])
     ap.add_argument("--forms", default="fp32,bf16")
     ap.add_argument("--generations", type=int, default=0)
     ap.add_argument("--seeds", type=int, default=10)
@@ -225,6 +268,8 @@ def main():
         plain_config("grid16_a512", args.generations or 20, forms)
     elif args.config == "grid128":
         plain_config("grid128_a4739", args.generations or 3, forms)
+    elif args.config == "published":
+        published_config(forms)
     elif args.config == "target":
         target_config([7 + 10 * i for i in range(args.seeds)], args.generations or 200)
     else:
