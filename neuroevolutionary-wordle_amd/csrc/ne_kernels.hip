@@ -663,6 +663,7 @@ struct PolicyBf16 {
 #ifndef NE_B16_TILES
 #define NE_B16_TILES 4
 #endif
+
 constexpr uint32_t kB16Tiles = NE_B16_TILES;
 constexpr uint32_t kB16CatBytes = kB16Tiles * kBf16CatTileBytes;
 constexpr uint32_t kB16GenBytes = kB16Tiles * kBf16GenTileBytes;
@@ -1001,16 +1002,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
                     uint32_t lcnt = 0;
                     bool lfail = false;
                     float llost = -__builtin_inff();  // best score dropped from a full list
-                    StreamChunks<kB16Tiles>(
-                        s_buf0, s_buf1, chunks16, __any(want ? 1 : 0) != 0, issue16,
-                        [&](const uint8_t* buf, int k, uint32_t at) __device__ {
-                            const v4f sc = Bf16Screen(buf, k, lane, pb);
+                    // one tile's screening scores: running maximum, window threshold, candidate lists
+                    auto screen = [&](const v4f& sc, uint32_t at, auto&& canon_of) __device__ {
                             const float m4 = fmaxf(fmaxf(sc[0], sc[1]), fmaxf(sc[2], sc[3]));
                             // a NaN or infinite score anywhere in the row quad (or an overflowing sum of
                             // four) sends the decision to the FP32 scan
                             lfail = lfail || !(fabsf((sc[0] + sc[1]) + (sc[2] + sc[3])) <= 3.0e38f);
                             if (!__any((want && m4 >= lthr) ? 1 : 0)) return;
-                            const v4u cn = *reinterpret_cast<const v4u*>(Bf16Canon(buf, k, gq));
+                            const v4u cn = canon_of();
     #pragma unroll
                             for (int r = 0; r < 4; ++r) {
                                 const uint32_t a = at * 16 + 4 * gq + uint32_t(r);
@@ -1058,6 +1057,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
                                     }
                                 }
                             }
+                    };
+                    StreamChunks<kB16Tiles>(
+                        s_buf0, s_buf1, chunks16, __any(want ? 1 : 0) != 0, issue16,
+                        [&](const uint8_t* buf, int k, uint32_t at) __device__ {
+                            screen(Bf16Screen(buf, k, lane, pb), at, [&]() __device__ {
+                                return *reinterpret_cast<const v4u*>(Bf16Canon(buf, k, gq));
+                            });
                         });
                     // the game's window: b16 over its 4 lane groups, then every listed row within 2 eps
                     float b16 = lbest;
