@@ -407,7 +407,12 @@ __global__ void TransposeEncoderKernel(const uint8_t* slab, uint64_t stride, con
 // thread r of a tile writes bytes 16r..16r+15 of it, i.e. row i = r % 16 of operand class c = r / 16
 // (0: instruction 0 lane group 3, 1..4: instruction 1 lane group c - 1), so its 8 dims are fixed and
 // their maxima stay in registers until one reduction per workgroup.
-constexpr uint32_t kPackTiles = 4;
+// Workgroups per genome: measured at 64x64 / 4,739 actions (encoder transpose + pack): 16 -> 1.61 ms,
+// 8 -> 1.50, 4 -> 1.42, 2 -> 1.40 without the register prefetch; 4 -> 1.05 ms with it (8 -> 1.10)
+#ifndef NE_PACK_BLOCKS
+#define NE_PACK_BLOCKS 4
+#endif
+constexpr uint32_t kPackTiles = 4;  // = blockDim.x / 80: 5 classes x 16 rows per tile
 __global__ __launch_bounds__(320) void PackBf16Kernel(const uint8_t* slab, uint64_t stride, const uint32_t* slot_indices,
                                                       uint32_t slot_count, const uint32_t* img, uint32_t A,
                                                       uint32_t a_tiles, uint8_t* out, uint64_t out_stride,
@@ -425,21 +430,39 @@ __global__ __launch_bounds__(320) void PackBf16Kernel(const uint8_t* slab, uint6
     const int m = c == 0 ? 0 : 1, g = c == 0 ? 3 : int(c) - 1;
     const uint32_t n_tail = A * kTail;  // valid tail floats
     float mx[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t at0 = blockIdx.x * kPackTiles; at0 < a_tiles; at0 += gridDim.x * kPackTiles) {
-        __syncthreads();  // the previous tiles are done with s_rows
-        const uint32_t f0 = at0 * 16 * kTail;  // first float of the staged rows
-        for (uint32_t q = threadIdx.x; q < kPackTiles * 16 * kTail / 4; q += blockDim.x) {
+    // the next group's rows are loaded into registers while the current group is packed
+    constexpr uint32_t kQuads = kPackTiles * 16 * kTail / 4;  // float4s per group (2,432)
+    constexpr uint32_t kPer = (kQuads + 319) / 320;            // per thread (blockDim.x = 320)
+    v4f pre[kPer];
+    auto load_group = [&](uint32_t at0) __device__ {
+        const uint32_t f0 = at0 * 16 * kTail;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            const uint32_t q = threadIdx.x + k * 320;
             const uint32_t f = f0 + 4 * q;
             v4f v = {0.0f, 0.0f, 0.0f, 0.0f};
-            if (f + 4 <= n_tail) {
-                v = *reinterpret_cast<const v4f*>(tails + f);
-            } else {
+            if (q < kQuads && at0 < a_tiles) {
+                if (f + 4 <= n_tail) {
+                    v = *reinterpret_cast<const v4f*>(tails + f);
+                } else {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) v[k] = f + k < n_tail ? tails[f + k] : 0.0f;
+                    for (int e = 0; e < 4; ++e) v[e] = f + e < n_tail ? tails[f + e] : 0.0f;
+                }
             }
-            *reinterpret_cast<v4f*>(s_rows + 4 * q) = v;
+            pre[k] = v;
+        }
+    };
+    load_group(blockIdx.x * kPackTiles);
+    for (uint32_t at0 = blockIdx.x * kPackTiles; at0 < a_tiles; at0 += gridDim.x * kPackTiles) {
+        __syncthreads();  // the previous tiles are done with s_rows
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            const uint32_t q = threadIdx.x + k * 320;
+            if (q < kQuads) *reinterpret_cast<v4f*>(s_rows + 4 * q) = pre[k];
         }
         __syncthreads();
+        load_group(at0 + gridDim.x * kPackTiles);
+
         const uint32_t at = at0 + sub;
         if (at >= a_tiles) continue;
         const uint32_t a = at * 16 + i;
@@ -2251,7 +2274,7 @@ void LaunchPackBf16(const uint8_t* slab, uint64_t stride, const uint32_t* slots,
                     hipStream_t s) {
     const uint32_t a_tiles = (A + 15) / 16;
     const uint32_t groups = (a_tiles + kPackTiles - 1) / kPackTiles;
-    const uint32_t blocks = groups < 16 ? groups : 16;
+    const uint32_t blocks = groups < NE_PACK_BLOCKS ? groups : NE_PACK_BLOCKS;
     hipLaunchKernelGGL(PackBf16Kernel, dim3(blocks, n), dim3(320), 0, s, slab, stride, slots, slot_count, img, A, a_tiles,
                        out, out_stride, tail_absmax);
 }
