@@ -167,7 +167,8 @@ def main():
     if bf16 is not None:
         bf16_out = {
             "value": games_per_gen * args.steps / bf16["elapsed"], "unit": "game-evals/s",
-            "ms_per_step": bf16["elapsed"] / args.steps * 1e3, "evaluate_ms": bf16["eval_ms"],
+            "ms_per_step": bf16["elapsed"] / args.steps * 1e3, "generations_per_s": args.steps / bf16["elapsed"],
+            "evaluate_ms": bf16["eval_ms"],
             "candidates_rescored_per_decision": bf16["rescored"] / max(1, bf16["decisions"]),
             "fp32_fallback_decisions": bf16["fallbacks"],
             "same_fitness_as_fp32": bool(np.array_equal(bf16["fitness"].view(np.uint32),
@@ -175,7 +176,7 @@ def main():
         }
     def graph_record(rec, eager):
         return {"value": games_per_gen * args.steps / rec["elapsed"], "unit": "game-evals/s",
-                "ms_per_step": rec["elapsed"] / args.steps * 1e3,
+                "ms_per_step": rec["elapsed"] / args.steps * 1e3, "generations_per_s": args.steps / rec["elapsed"],
                 "replay_ms": float(rec["stages"].step_graph_ms),
                 "same_fitness_as_eager": bool(np.array_equal(rec["fitness"].view(np.uint32),
                                                              eager["fitness"].view(np.uint32)))}
