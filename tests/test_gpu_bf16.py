@@ -118,11 +118,14 @@ def test_eval_precision_rejects_unknown_modes(pkg):
         rt.close()
 
 
-@pytest.mark.parametrize("dups", [False, True])
-def test_untraced_evaluations_equal_traced_ones(pkg, dups):
-    """The untraced kernels skip the per-row argmax work when no lane of a wave can change: their
-    fitness and decision counts must equal the traced kernels' (which do every row), in both forms."""
+@pytest.mark.parametrize("case", ["plain", "dups", "zero"])
+def test_untraced_evaluations_equal_traced_ones(pkg, case):
+    """The untraced kernels skip the per-row argmax work when no lane of a wave can change, gated by the
+    best score another lane of the game already holds: their fitness and decision counts must equal the
+    traced kernels' (which do every row), in both forms.  "zero": all-zero genomes, so every score is
+    exactly 0 and every decision is a tie that the lowest unmasked index must win."""
     rtm = pkg.runtime
+    dups = case == "dups"
     if dups:
         A, side = 365, 4
         rng = np.random.default_rng(5)
@@ -136,7 +139,8 @@ def test_untraced_evaluations_equal_traced_ones(pkg, dups):
     else:
         wl = pkg.synth.workload("grid16_a512")
         A, side, catalog, offsets, ids = 512, 16, wl["catalog"], wl["offsets"], wl["ids"]
-    rt = _runtime(pkg, side, A, catalog, offsets, ids, seed=41)
+    rt = _runtime(pkg, side, A, catalog, offsets, ids, seed=41, var=0.0 if case == "zero" else 1.0,
+                  tail=0.0 if case == "zero" else 0.1)
     try:
         results = []
         for mode in (rtm.EVAL_FP32, rtm.EVAL_BF16_RESCORED):
