@@ -778,7 +778,9 @@ __device__ __forceinline__ void ActionOperands(Rsrc img_r, Rsrc tail_r, uint32_t
 constexpr float kBf16RelBound = 1.03f * 0x1p-8f;
 constexpr float kBf16AbsBound = 0x1p-120f;
 constexpr uint32_t kMaxCandidates = 32;  // per game; more within the window -> FP32 scan of the decision
-constexpr uint32_t kLaneCandidates = 8;  // per lane during the screening pass (compacted when full)
+// Per lane during the screening pass (compacted when full).  4 measured: lists overflow, 10,037 instead of
+// 68 FP32 fallbacks per evaluation at 64x64 / 4,739 actions, 11.8 instead of 9.5 ms.
+constexpr uint32_t kLaneCandidates = 8;
 
 template <bool kTrace, bool kBf16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void EvaluateKernel(EvalArgs p) {
@@ -1000,8 +1002,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
                     // its own 2 x 8
                     PolicyBf16 pb;
                     {
-                        static_assert(sizeof(float) * kLaneCandidates * 64 >= 2 * 1024, "policy exchange");
-                        __bf16* xb = reinterpret_cast<__bf16*>(&s_lsc[wave][0][0]);
+                        // s_buf1 is free here: the trunk's last chunk is consumed and the screening stream
+                        // first writes it after its opening barrier
+                        static_assert(4 * 2048 <= kBufBytes, "policy exchange");
+                        __bf16* xb = reinterpret_cast<__bf16*>(s_buf1 + wave * 2048);
     #pragma unroll
                         for (int s = 0; s < kScoreSteps; ++s) {
                             const int d = ScoreDim(s, int(gq));
@@ -2302,6 +2306,18 @@ void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, bool tra
         else
             hipLaunchKernelGGL((EvaluateKernel<false, false>), grid, block, 0, s, a);
     }
+}
+
+uint32_t EvaluateResidentPerCu(uint32_t waves, bool trace, bool bf16) {
+    // what the registers and LDS of the kernel built here allow, as the runtime reports it
+    const void* k = bf16 ? (trace ? reinterpret_cast<const void*>(&EvaluateKernel<true, true>)
+                                  : reinterpret_cast<const void*>(&EvaluateKernel<false, true>))
+                         : (trace ? reinterpret_cast<const void*>(&EvaluateKernel<true, false>)
+                                  : reinterpret_cast<const void*>(&EvaluateKernel<false, false>));
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, int(64 * waves), 0) != hipSuccess || blocks <= 0)
+        return 0;
+    return uint32_t(blocks);
 }
 
 void LaunchFinalize(const float* raw, const uint32_t* local_w, uint32_t n, float* fitness, uint32_t* evals,

@@ -548,8 +548,10 @@ static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32
     a.work_count = uint32_t(items);
     a.work_counter = rt->d_work_counter;
     a.splits = splits;
-    // resident workgroups: 3 waves per SIMD (168 VGPRs) and at most 4 workgroups per CU (LDS)
-    const uint64_t per_cu = std::min<uint64_t>(12 / waves, 4);
+    // resident workgroups per CU, as the kernel's registers and LDS allow (3 waves per SIMD at 168 VGPRs
+    // and 4 workgroups per CU by LDS in the FP32 form)
+    const uint32_t resident = EvaluateResidentPerCu(waves, trace, bf16);
+    const uint64_t per_cu = resident != 0 ? resident : std::min<uint64_t>(12 / waves, 4);
     const uint32_t blocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(items, rt->cu_count * per_cu)));
     if (rt->enc_t == nullptr && !Alloc(&rt->enc_t, rt->max_gen * kEncTStride))
         return Fail(rt, NE_DEVICE_FAILURE);
