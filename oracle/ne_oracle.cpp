@@ -877,6 +877,17 @@ void oro_rank_weights(float exponent, uint64_t n, float* out) {
     for (uint64_t r = 1; r <= n; ++r) out[r] = float(std::pow(double(r), double(exponent)));
 }
 
+// One rank-roulette draw (TrySelectRankWeightedIndexFromCandidates, selection_ops.cuh:115-161) with the
+// tie-break seed and the uniform given directly, so tests can place the threshold exactly.
+// excluded = UINT64_MAX for none.  Returns 1 and the chosen cell, or 0 when no candidate is eligible.
+int oro_select_one(const float* fit, const uint8_t* has, const uint64_t* cand, uint64_t ncand, const float* wtable,
+                   uint64_t tie_seed, float u, uint64_t excluded, uint64_t* chosen) {
+    return SelectOne(fit, has, cand, ncand, wtable, tie_seed, u, excluded, chosen) ? 1 : 0;
+}
+
+// SampleBernoulli (genome_ops.cuh:35-44) on one word of the counter-based stream.
+int oro_bernoulli(float p, uint32_t word) { return Bernoulli(p, word) ? 1 : 0; }
+
 int oro_moore(uint64_t rows, uint64_t cols, uint64_t focal, uint64_t radius, uint64_t* out) {
     return MooreNeighbors(rows, cols, focal, radius, out);
 }

@@ -36,6 +36,8 @@ _SIGS = {
     "oro_summary": (None, [P, U64, P, P, P]),
     "oro_normalize_fitness": (F32, [F32, U64]),
     "oro_rank_weights": (None, [F32, U64, P]),
+    "oro_select_one": (I32, [P, P, P, U64, P, U64, F32, U64, P]),
+    "oro_bernoulli": (I32, [F32, U32]),
     "oro_moore": (I32, [U64, U64, U64, U64, P]),
     "oro_chebyshev": (U64, [U64, U64, U64, U64]),
     "oro_plan": (I32, [P, P, U64, U64, U64, U64, U64, P, U32, P]),
@@ -216,6 +218,20 @@ def rank_weights(exponent, n):
     w = np.zeros(n + 1, np.float32)
     lib().oro_rank_weights(exponent, n, ptr(w))
     return w
+
+
+NO_INDEX = 0xFFFFFFFFFFFFFFFF
+
+
+def select_one(fitness, has, candidates, wtable, tie_seed, u, excluded=NO_INDEX):
+    """One roulette draw over `candidates` (cell indices, in list order).  Returns the chosen cell or None."""
+    f = np.ascontiguousarray(fitness, np.float32)
+    h = np.ascontiguousarray(has, np.uint8)
+    c = np.ascontiguousarray(candidates, np.uint64)
+    w = np.ascontiguousarray(wtable, np.float32)
+    out = C.c_uint64()
+    ok = lib().oro_select_one(ptr(f), ptr(h), ptr(c), c.size, ptr(w), tie_seed, u, excluded, C.byref(out))
+    return out.value if ok else None
 
 
 def plan(fitness, has, cur_size, child_count, cols, gen_index, radius, wtable, generation_seed):
