@@ -36,7 +36,15 @@ std::string Escape(std::string_view in) {
             case '\n': out += "\\n"; break;
             case '\r': out += "\\r"; break;
             case '\t': out += "\\t"; break;
-            default: out += c;
+            default:
+                if (static_cast<unsigned char>(c) < 0x20) {  // other control bytes: \u00XX (RFC 8259 7)
+                    static const char kHex[] = "0123456789abcdef";
+                    out += "\\u00";
+                    out += kHex[static_cast<unsigned char>(c) >> 4];
+                    out += kHex[static_cast<unsigned char>(c) & 15];
+                } else {
+                    out += c;
+                }
         }
     }
     return out;
@@ -133,6 +141,21 @@ bool ReadString(const std::string& json, std::string_view key, std::string& out)
                 case 'n': c = '\n'; break;
                 case 'r': c = '\r'; break;
                 case 't': c = '\t'; break;
+                case 'u': {  // \u00XX: the single-byte code points the writer emits for control bytes
+                    if (pos + 4 >= json.size() || json[pos + 1] != '0' || json[pos + 2] != '0') return false;
+                    int v = 0;
+                    for (int k = 3; k <= 4; ++k) {
+                        const char h = json[pos + k];
+                        const int d = (h >= '0' && h <= '9') ? h - '0'
+                                      : (h >= 'a' && h <= 'f') ? h - 'a' + 10
+                                      : (h >= 'A' && h <= 'F') ? h - 'A' + 10 : -1;
+                        if (d < 0) return false;
+                        v = 16 * v + d;
+                    }
+                    c = char(v);
+                    pos += 4;
+                    break;
+                }
                 default: return false;
             }
         }

@@ -318,11 +318,19 @@ int ne_read_checkpoint(const char* path, ne_checkpoint** out) {
     uint64_t live = 0;
     r.Value(live);
     r.ok &= live <= count;
+    // the stride comes from the file: bounded by the bytes left before anything is sized from it, so a
+    // corrupt stride can neither wrap `stride + 12` or `live * stride` nor divide by zero (whether it is
+    // the stride of the file's action count is checked at restore, as TryRestorePrebreedingCheckpoint-
+    // ToDevice does, slab_runtime.cu:1790-1813)
     c->genome_bytes = c->Stride();
-    r.ok &= live == 0 || (c->genome_bytes > 0 && r.Fits(live, c->genome_bytes + 12));
+    r.ok &= live == 0 || (c->genome_bytes > 0 && c->genome_bytes <= r.left && r.Fits(live, c->genome_bytes + 12));
     if (r.ok) {
-        c->organisms.resize(live);
-        c->genomes.resize(live * c->genome_bytes);
+        try {
+            c->organisms.resize(live);
+            c->genomes.resize(live * c->genome_bytes);  // live * stride <= bytes left in the file
+        } catch (const std::exception&) {
+            r.ok = false;
+        }
     }
     for (uint64_t k = 0; r.ok && k < live; ++k) {
         uint64_t bytes = 0;

@@ -998,12 +998,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
                     __shared__ float s_lsc[4][kLaneCandidates][64];
                     __shared__ uint32_t s_lac[4][kLaneCandidates][64];
                     // the policy as the B operands of the two screening instructions: every lane writes its
-                    // dims (bf16) at their k-slots in the wave's (still unused) list scores, and reads back
-                    // its own 2 x 8
+                    // dims (bf16) at their k-slots in this wave's 2 KB of s_buf1, and reads back its own 2 x 8
                     PolicyBf16 pb;
                     {
-                        // s_buf1 is free here: the trunk's last chunk is consumed and the screening stream
-                        // first writes it after its opening barrier
+                        // s_buf1 is free between two barriers: TrunkStream's closing barrier (every wave has
+                        // consumed the last trunk chunk, which sat in s_buf1) and StreamChunks' opening
+                        // barrier (after which the screening stream's second chunk is issued into s_buf1)
                         static_assert(4 * 2048 <= kBufBytes, "policy exchange");
                         __bf16* xb = reinterpret_cast<__bf16*>(s_buf1 + wave * 2048);
     #pragma unroll

@@ -152,6 +152,11 @@ struct ne_runtime {
     bool have_summary = false;
     int last_status = NE_OK;
     bool cur_evaluated = false;  // the current generation's fitness is that of its own evaluation
+    // what that fitness was measured on: a screened step reuses it only while these still hold
+    uint64_t upload_epoch = 1;          // bumped by every catalog or secrets upload (not by evaluations)
+    uint64_t fit_upload_epoch = 0;      // upload_epoch when the fitness was measured (0: unknown)
+    uint64_t fit_action_space = 0;
+    std::vector<uint32_t> fit_offsets, fit_secrets;  // the per-cell secrets it was measured on
     uint32_t* d_work = nullptr;         // persistent evaluation's work list
     uint64_t work_cap = 0;
     uint32_t* d_work_counter = nullptr;
@@ -658,6 +663,41 @@ static int EvaluateCore(ne_runtime* rt, uint64_t action_space, const uint32_t* s
 }
 
 
+// Records the inputs the current generation's fitness was measured on (n cells of a per-cell CSR).
+static void RecordFitnessInputs(ne_runtime* rt, uint64_t action_space, const std::vector<uint32_t>& offs,
+                                const std::vector<uint32_t>& secs, uint64_t n) {
+    rt->fit_upload_epoch = rt->upload_epoch;
+    rt->fit_action_space = action_space;
+    rt->fit_offsets.assign(offs.begin(), offs.begin() + n + 1);
+    rt->fit_secrets.assign(secs.begin() + offs[0], secs.begin() + offs[n]);
+}
+
+// The current generation's fitness still describes it for this step's inputs: no catalog or secrets
+// upload since it was measured, the same action space and the same per-cell secrets (the caller's CSR,
+// or the shard schedule's lists for this generation under `wc` and `hist`).  The reference evaluates at
+// every generation boundary (slab_runtime.cu:1374-1424); a reuse is valid only when that evaluation
+// would repeat the recorded one.
+static bool FitnessInputsHold(ne_runtime* rt, const ne_word_counts* wc, const ne_shard_release_history* hist) {
+    if (!rt->cur_evaluated || rt->fit_upload_epoch != rt->upload_epoch || !wc ||
+        rt->fit_action_space != wc->action_space_word_count)
+        return false;
+    const uint64_t cols = rt->config.grid_column_count, n = rt->cur_size;
+    if (cols == 0 || n == 0 || n % cols != 0 || rt->fit_offsets.size() != n + 1) return false;
+    std::vector<uint32_t> built_offsets, built_secrets;
+    const std::vector<uint32_t>* offs = &rt->h_offsets;
+    const std::vector<uint32_t>* secs = &rt->h_secrets;
+    if (!rt->csr_from_caller) {
+        if (BuildShardCsr(rt, wc, hist, n / cols, cols, built_offsets, built_secrets) != NE_OK) return false;
+        offs = &built_offsets;
+        secs = &built_secrets;
+    }
+    if (offs->size() < n + 1) return false;
+    if (!std::equal(offs->begin(), offs->begin() + n + 1, rt->fit_offsets.begin())) return false;
+    const uint64_t k = (*offs)[n] - (*offs)[0];
+    return k == rt->fit_secrets.size() &&
+           std::equal(secs->begin() + (*offs)[0], secs->begin() + (*offs)[n], rt->fit_secrets.begin());
+}
+
 static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard_release_history* hist) {
     const uint64_t cols = rt->config.grid_column_count;
     if (!wc) return Fail(rt, NE_INVALID_ARGUMENT);
@@ -681,7 +721,10 @@ static int EvaluateFull(ne_runtime* rt, const ne_word_counts* wc, const ne_shard
     if (offs->size() < rt->cur_size + 1) return Fail(rt, NE_INVALID_TRAINING_SHARD);
     const int st = EvaluateCore(rt, wc->action_space_word_count, rt->cur_slots, rt->cur_size, *offs, *secs, rt->cur_fit,
                                 rt->cur_evals, rt->cur_has, rt->cur_index);
-    if (st == NE_OK) rt->cur_evaluated = true;
+    if (st == NE_OK) {
+        rt->cur_evaluated = true;
+        RecordFitnessInputs(rt, wc->action_space_word_count, *offs, *secs, rt->cur_size);
+    }
     return st;
 }
 
@@ -1472,6 +1515,7 @@ void ne_destroy(ne_runtime* rt) {
 int ne_upload_catalog(ne_runtime* rt, const uint8_t* letters, uint64_t word_count) {
     if (!rt || !letters || word_count == 0 || word_count > kCatalogCapacity) return Fail(rt, NE_INVALID_ARGUMENT);
     ++rt->input_epoch;
+    ++rt->upload_epoch;
     std::vector<uint32_t> codes(word_count), canon(word_count);
     std::vector<float> fixed(word_count * kFixed, 0.0f);
     std::unordered_map<uint32_t, uint32_t> first_index;
@@ -1551,6 +1595,7 @@ int ne_upload_secrets_csr(ne_runtime* rt, const uint32_t* cell_offsets, uint64_t
     rt->h_secrets.assign(secret_ids, secret_ids + secret_count);
     rt->csr_from_caller = true;
     ++rt->input_epoch;
+    ++rt->upload_epoch;
     return NE_OK;
 }
 
@@ -1560,6 +1605,7 @@ int ne_clear_secrets(ne_runtime* rt) {
     rt->h_secrets.clear();
     rt->csr_from_caller = false;
     ++rt->input_epoch;
+    ++rt->upload_epoch;
     return NE_OK;
 }
 
@@ -2157,7 +2203,7 @@ int ne_advance_generation_screened(ne_runtime* rt, uint32_t generation_seed, con
     ne_screen_report rep{};
     rt->times = ne_stage_times{};
     int st = NE_OK;
-    if (!rt->cur_evaluated) {
+    if (!FitnessInputsHold(rt, counts, history)) {
         st = EvaluateFull(rt, counts, history);
         if (st != NE_OK) return st;
     }
@@ -2319,6 +2365,7 @@ int ne_advance_generation_screened(ne_runtime* rt, uint32_t generation_seed, con
     const ne_stage_times eval_times = rt->times;
     SwapGenerations(rt);
     rt->cur_evaluated = true;
+    RecordFitnessInputs(rt, counts->action_space_word_count, *offs, *secs, N);
     LaunchSummary(rt->cur_fit, rt->cur_has, rt->cur_slots, uint32_t(N), rt->summary, rt->status, rt->stream);
     st = Finish(rt);
     if (st != NE_OK) return st;

@@ -105,9 +105,12 @@ def ragged_config(forms, gens=50, side=64, a0=512, a_final=4739, step=512, perio
 
     recs = []
     for form in forms:
-        # the generation budget is sized for the final stride, so the population stays 4,096 while the
-        # action count grows; the slab holds two generations at the final stride plus repack staging
-        rt = pkg.SlabRuntime(slab_bytes=2 * n * s_final + 4 * s_final, generation_bytes=n * s_final,
+        # the generation budget becomes capacity x initial stride (slab_runtime.cu:685-696) and every
+        # widening takes whole rows of it at the new stride (:918-938); one row beyond n at the final stride
+        # keeps all 64 rows, so the population stays 4,096 while the action count grows (n x final stride
+        # alone would give 63 rows at 4,739 actions); the slab holds two generations at the final stride
+        # plus repack staging
+        rt = pkg.SlabRuntime(slab_bytes=2 * n * s_final + 4 * s_final, generation_bytes=(n + side) * s_final,
                              action_count=a0, grid_columns=side)
         try:
             rt.upload_catalog(catalog)
@@ -121,7 +124,8 @@ def ragged_config(forms, gens=50, side=64, a0=512, a_final=4739, step=512, perio
                     schedule[g + 1] != schedule[g] for g in range(gens)),
                 "secrets_per_cell": [int(np.min(np.diff(offsets))), int(np.max(np.diff(offsets))),
                                      float(np.mean(np.diff(offsets)))],
-                "final_action_count": int(rt.info().action_count)}))
+                "final_action_count": int(rt.info().action_count),
+                "final_population": int(rt.info().current_generation_size)}))
         finally:
             rt.close()
     if len(recs) == 2:

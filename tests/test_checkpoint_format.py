@@ -200,3 +200,19 @@ def test_status_names_cover_the_checkpoint_codes(pkg):
     names = pkg.runtime.STATUS_NAMES
     assert names[20] == "checkpoint invalid" and names[21] == "checkpoint I/O failure"
     assert os.path.exists(pkg.runtime.LIB_PATH)
+
+
+@pytest.mark.parametrize("stride", [(1 << 64) - 8, (1 << 64) - 12, 0, 1 << 40])
+def test_corrupt_stride_is_rejected_without_sizing_from_it(pkg, tmp_path, stride):
+    """The slot stride is read from the file; the library must refuse one that cannot describe the genome
+    records that follow before sizing anything from it (a stride near 2^64 would wrap `stride + 12` and
+    `live * stride`, zero would divide by zero, 2^40 is larger than the file)."""
+    c = _synthetic()
+    A_, _, slots_total, _ = cko.slab_layout(c)
+    c["slab_layout"] = struct.pack("<4Q", A_, stride, slots_total, (slots_total * stride) % (1 << 64))
+    c["checksum"] = cko.checksum(c)
+    path = tmp_path / "stride.ckpt"
+    path.write_bytes(cko.write(c))
+    with pytest.raises(RuntimeError) as err:
+        pkg.runtime.Checkpoint.read(str(path))
+    assert err.value.status == 20

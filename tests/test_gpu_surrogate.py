@@ -232,3 +232,39 @@ def test_graph_replayed_training_equals_eager_launches(pkg):
     assert np.array_equal(rt_g.surrogate_params().view(np.uint32), rt_e.surrogate_params().view(np.uint32))
     rt_g.close()
     rt_e.close()
+
+
+def test_screened_step_reevaluates_after_new_secrets(pkg):
+    """A screened step plans from the current generation's fitness.  When the secrets change between steps
+    (here: a different per-cell CSR), that fitness must be measured again on the new secrets, as the
+    reference evaluates at every generation boundary (slab_runtime.cu:1374-1424): a runtime that re-uploads
+    and steps must end exactly like one that also evaluates explicitly before stepping."""
+    rtm = pkg.runtime
+    wl = pkg.synth.workload("grid16_a512")
+    kw = dict(min_replay=256, spearman_threshold=-1.0)
+    counts = rtm.word_counts(512, 512)
+    cfg = rtm.default_assembly_config(mutation_probability=0.01, mutation_sigma=0.02)
+    # the same block shards with the secret lists rotated within each cell: another CSR of equal shape
+    offs = wl["offsets"]
+    ids2 = np.concatenate([np.roll(wl["ids"][offs[c]:offs[c + 1]], 3) for c in range(len(offs) - 1)])
+    ids2 = (ids2 + 1) % 512
+    runs = []
+    for explicit in (False, True):
+        rt = _runtime(pkg, wl)
+        rt.enable_screening(rtm.default_screen_config(**kw))
+        for g in range(2):
+            rt.advance_screened(9 + g, counts, cfg)
+        rt.upload_secrets_csr(offs, ids2.astype(np.uint32))
+        if explicit:
+            rt.evaluate(counts)
+        stale = rt.generation(0)[1].copy()
+        rt.advance_screened(11, counts, cfg)
+        pairs = rt.plan().copy()
+        slots, fit, _, _ = rt.generation(0)
+        runs.append((stale, pairs, slots.copy(), fit.copy()))
+        rt.close()
+    (s0, p0, sl0, f0), (s1, p1, sl1, f1) = runs
+    assert not np.array_equal(s0, s1), "the new secrets change the current generation's fitness"
+    assert np.array_equal(p0, p1)
+    assert np.array_equal(sl0, sl1)
+    assert np.array_equal(f0.view(np.uint32), f1.view(np.uint32))

@@ -65,6 +65,20 @@ def test_directory_names_do_not_collide(pkg, tmp_path):
         assert rtm.read_winner_artifact(b, m)[1].tobytes() == genome.tobytes()
 
 
+def test_control_bytes_in_the_path_stay_valid_json(pkg, tmp_path):
+    """Control bytes below 0x20 other than \\n \\r \\t are written as \\u00XX (RFC 8259 section 7)."""
+    rtm = pkg.runtime
+    genome, letters, meta = _inputs(pkg)
+    path = "dir\x01name\x1f/words\x07.txt\ttab"
+    meta.action_space_path = path.encode()
+    b, m = tmp_path / "c.bin", tmp_path / "c.json"
+    rtm.write_winner_artifact(str(b), str(m), genome, letters, meta)
+    text = m.read_text()
+    assert "\\u0001" in text and "\\u001f" in text and "\\u0007" in text
+    assert json.loads(text)["action_space_path"] == path
+    assert rtm.read_winner_artifact(str(b), str(m))[0].action_space_path == path.encode()
+
+
 def _rewrite(path, **changes):
     doc = json.loads(path.read_text())
     for k, v in changes.items():
