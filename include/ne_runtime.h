@@ -555,6 +555,15 @@ int ne_model_select_guess(ne_model *model, const ne_board *board, ne_selected_ac
 /* No counterpart (the reference serves one board per call): `count` boards in one launch, one status each
  * in out[i].status; the call's return is NE_OK unless the call itself failed. */
 int ne_model_select_guesses(ne_model *model, const ne_board *boards, uint64_t count, ne_selected_action *out);
+/* No counterpart in the reference: ne_model_select_guesses for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  It allocates nothing: the
+ * boards and results are staged in one static device array per GPU (NE_NOALLOC_MAX_BOARDS boards,
+ * 704 KB, held until the process ends; its address is looked up by ne_model_create).  count above
+ * NE_NOALLOC_MAX_BOARDS returns NE_INVALID_ARGUMENT.  Calls that overlap on one GPU (other threads or
+ * streams, any model) share the array and race. */
+#define NE_NOALLOC_MAX_BOARDS 8192
+int ne_model_select_guesses_noalloc(ne_model *model, const ne_board *boards, uint64_t count,
+                                    ne_selected_action *out);
 
 /* WinnerArtifactMetadata (winner_artifact.hpp:12-21). */
 typedef struct ne_winner_metadata {

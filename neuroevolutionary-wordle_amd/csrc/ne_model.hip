@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <new>
 
 #include "../../include/ne_runtime.h"
@@ -187,6 +188,34 @@ __global__ __launch_bounds__(kInferThreads) void SelectGuessKernel(const float* 
     }
 }
 
+// ---- staging of ne_model_select_guesses_noalloc ------------------------------------------------
+// One array per GPU for the life of the process: kNoAllocMaxBoards boards, then their results.  On the
+// host the name `scratch` is a stand-in, not a device address: host code uses only the address
+// hipGetSymbolAddress returns for each device (looked up once per device, with that device current).
+constexpr uint64_t kNoAllocMaxBoards = NE_NOALLOC_MAX_BOARDS;
+constexpr uint64_t kScratchResultsOffset = kNoAllocMaxBoards * sizeof(ne_board);
+constexpr uint64_t kScratchBytes = kScratchResultsOffset + kNoAllocMaxBoards * sizeof(ne_selected_action);
+static_assert(kScratchResultsOffset % 256 == 0, "results start on a 256-byte boundary");
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// This device's `scratch` address; the caller has made `device` current.
+int ScratchAddress(int device, void** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxScratchDevices) return NE_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return NE_DEVICE_FAILURE;
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return NE_OK;
+}
+
 }  // namespace
 
 struct ne_model {
@@ -198,6 +227,7 @@ struct ne_model {
     ne_board* boards = nullptr;
     ne_selected_action* results = nullptr;
     uint64_t capacity = 0;
+    void* scratch = nullptr;  // this model's GPU's `scratch` (ne_model_select_guesses_noalloc)
 };
 
 extern "C" {
@@ -221,6 +251,8 @@ int ne_model_create(const void* genome_bytes, uint64_t genome_byte_count, const 
         m->own_stream = ok;
     }
     m->action_count = action_count;
+    int device = -1;
+    ok = ok && hipGetDevice(&device) == hipSuccess && ScratchAddress(device, &m->scratch) == NE_OK;
     ok = ok && hipMalloc(reinterpret_cast<void**>(&m->genome), genome_byte_count) == hipSuccess;
     ok = ok && hipMalloc(reinterpret_cast<void**>(&m->words), 5 * action_count) == hipSuccess;
     ok = ok && hipMemcpyAsync(m->genome, genome_bytes, genome_byte_count, hipMemcpyHostToDevice, m->stream) ==
@@ -268,6 +300,29 @@ int ne_model_select_guesses(ne_model* m, const ne_board* boards, uint64_t count,
         ok = hipGetLastError() == hipSuccess;
     }
     ok = ok && hipMemcpyAsync(out, m->results, count * sizeof(ne_selected_action), hipMemcpyDeviceToHost,
+                              m->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(m->stream) == hipSuccess;
+    return ok ? NE_OK : NE_DEVICE_FAILURE;
+}
+
+// ne_model_select_guesses through the GPU's static staging array: no allocation of any kind.  Calls that
+// overlap on one GPU (other threads or streams) share the array and race; every byte a call reads back
+// is written by that call first (its boards by the upload, every result by the kernel), so nothing
+// needs clearing between calls.
+int ne_model_select_guesses_noalloc(ne_model* m, const ne_board* boards, uint64_t count, ne_selected_action* out) {
+    if (!m || !boards || !out || count == 0 || count > kNoAllocMaxBoards) return NE_INVALID_ARGUMENT;
+    if (!m->scratch) return NE_DEVICE_FAILURE;
+    ne_board* d_boards = static_cast<ne_board*>(m->scratch);
+    ne_selected_action* d_results =
+        reinterpret_cast<ne_selected_action*>(static_cast<uint8_t*>(m->scratch) + kScratchResultsOffset);
+    bool ok = hipMemcpyAsync(d_boards, boards, count * sizeof(ne_board), hipMemcpyHostToDevice, m->stream) ==
+              hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(SelectGuessKernel, dim3(uint32_t(count)), dim3(kInferThreads), 0, m->stream, m->genome,
+                           m->words, uint32_t(m->action_count), d_boards, d_results);
+        ok = hipGetLastError() == hipSuccess;
+    }
+    ok = ok && hipMemcpyAsync(out, d_results, count * sizeof(ne_selected_action), hipMemcpyDeviceToHost,
                               m->stream) == hipSuccess;
     ok = ok && hipStreamSynchronize(m->stream) == hipSuccess;
     return ok ? NE_OK : NE_DEVICE_FAILURE;

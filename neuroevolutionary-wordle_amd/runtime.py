@@ -219,6 +219,7 @@ EXPORTED_SYMBOLS = [
     "ne_free_checkpoint", "ne_training_data_identity_hash", "ne_collect_fitness_telemetry",
     "ne_collect_convergence_telemetry", "ne_convergence_sample_plan", "ne_maybe_release_training_shard",
     "ne_model_create", "ne_model_destroy", "ne_model_select_guess", "ne_model_select_guesses",
+    "ne_model_select_guesses_noalloc",
     "ne_write_winner_artifact", "ne_write_winner_artifact_in_directory", "ne_read_winner_artifact",
 ]
 
@@ -298,6 +299,7 @@ def load_library(path=LIB_PATH):
         "ne_model_destroy": (None, [p]),
         "ne_model_select_guess": (i32, [p, C.POINTER(Board), C.POINTER(SelectedAction)]),
         "ne_model_select_guesses": (i32, [p, p, u64, p]),
+        "ne_model_select_guesses_noalloc": (i32, [p, p, u64, p]),
         "ne_write_winner_artifact": (i32, [C.c_char_p, C.c_char_p, p, p, u64, C.POINTER(WinnerMetadata)]),
         "ne_write_winner_artifact_in_directory": (i32, [C.c_char_p, p, p, u64, C.POINTER(WinnerMetadata), p, p,
                                                         u64]),
@@ -752,6 +754,9 @@ class ReleaseHistory:
         return self.generations[:n].tolist(), self.p99[:n].tolist()
 
 
+NOALLOC_MAX_BOARDS = 8192  # NE_NOALLOC_MAX_BOARDS
+
+
 class Model:
     """TryCreateSingleModelDeviceRuntime ... DestroySingleModelDeviceRuntime: one genome serving guesses."""
 
@@ -780,6 +785,17 @@ class Model:
         st = self.lib.ne_model_select_guesses(self.h, C.cast(arr, C.c_void_p), len(boards), C.cast(out, C.c_void_p))
         if st != 0:
             raise RuntimeError_("ne_model_select_guesses", st)
+        return out
+
+    def select_many_noalloc(self, boards):
+        """select_many through ne_model_select_guesses_noalloc (no device allocation; at most
+        NOALLOC_MAX_BOARDS boards per call)."""
+        arr = (Board * len(boards))(*[b if isinstance(b, Board) else make_board(b) for b in boards])
+        out = (SelectedAction * len(boards))()
+        st = self.lib.ne_model_select_guesses_noalloc(self.h, C.cast(arr, C.c_void_p), len(boards),
+                                                      C.cast(out, C.c_void_p))
+        if st != 0:
+            raise RuntimeError_("ne_model_select_guesses_noalloc", st)
         return out
 
     def close(self):
