@@ -10,6 +10,7 @@ timed, so `value` and the roofline stay on the eager steps, whose kernels are ti
 Prints one JSON line.
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -19,6 +20,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "neuroevolutionary-wordle_amd"))
 
 PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
@@ -31,48 +33,87 @@ def algorithmic_eval_flops(decisions, encoded_turns, A):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "evaluate_grid64_a4739_traffic.json")
+KERNEL_SOURCES = [os.path.join(ROOT, "neuroevolutionary-wordle_amd", "csrc", f) for f in ("ne_kernels.hip", "ne_device.hpp")]
+
+
+def kernel_source_hash():
+    """sha256 over the evaluation kernel's sources (what a traffic measurement is tied to)."""
+    h = hashlib.sha256()
+    for path in KERNEL_SOURCES:
+        with open(path, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
 
 
 def measured_traffic():
     """HBM bytes per evaluation launch from the committed FETCH_SIZE / WRITE_SIZE measurement of this
-    workload (scaled by the plain-copy calibration recorded with it), or None."""
+    workload (scaled by the plain-copy calibration recorded with it) and whether it was measured on the
+    kernel sources of this tree: (bytes or None, stale)."""
     if not os.path.exists(TRAFFIC_FILE):
-        return None
+        return None, True
     with open(TRAFFIC_FILE) as f:
         t = json.load(f)
-    return t["fetch_bytes"] * t["fetch_scale"] + t["write_bytes"] * t["write_scale"]
+    stale = t.get("kernel_source_sha256") != kernel_source_hash()
+    return t["fetch_bytes"] * t["fetch_scale"] + t["write_bytes"] * t["write_scale"], stale
 
 
-def cpu_baseline(pkg, rt, wl, A, seconds=12.0):
-    """The oracle (fp32, scalar C++, all host threads) on a bounded sample of the same workload."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(pkg, rt, wl, A, seconds=3.0, reps=5):
+    """The oracle (fp32, scalar C++) on bounded samples of the same workload: the host's cores for one GPU
+    (at most 16, the box's share) and 1 thread, each the median of `reps` timed runs of about `seconds`
+    after a warm-up run."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(threads, 16))  # the box's CPU share for one GPU
+    share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    share = max(1, min(share, 16))
     slots, _, _, _ = rt.generation(0)
     offsets, ids = wl["offsets"], wl["ids"]
-    # probe with a few genomes to size the sample to about `seconds` of CPU work
-    probe = max(threads, 8)
-    genomes = [rt.download_slot(s) for s in slots[:probe]]
-    t0 = time.perf_counter()
-    r = orc.evaluate(genomes, 0, A, A, wl["catalog"], offsets[:probe + 1], ids, mode=0, threads=threads)
-    dt = time.perf_counter() - t0
-    games = 3 * int(offsets[probe])
-    rate = games / dt
-    n = int(min(len(slots), max(probe, rate * seconds / (games / probe))))
-    n = max(probe, (n // threads) * threads)
-    if n > probe:
-        genomes = [rt.download_slot(s) for s in slots[:n]]
+    pool = []
+
+    def timed(n, threads):
+        while len(pool) < n:
+            pool.append(rt.download_slot(slots[len(pool)]))
         t0 = time.perf_counter()
-        r = orc.evaluate(genomes, 0, A, A, wl["catalog"], offsets[:n + 1], ids, mode=0, threads=threads)
+        r = orc.evaluate(pool[:n], 0, A, A, wl["catalog"], offsets[:n + 1], ids, mode=0, threads=threads)
         dt = time.perf_counter() - t0
-        games = 3 * int(offsets[n])
-        rate = games / dt
-    assert r["status"] == 0
-    return {"value": rate, "unit": "game-evals/s", "cores": threads, "kind": "port",
-            "sample": f"fitness evaluation of the first {n} genomes of generation 0 ({games} games, "
-                      f"{r['decisions']} decisions) at A={A}, {dt:.1f} s"}
+        assert r["status"] == 0
+        return 3 * int(offsets[n]) / dt, r["decisions"], dt
+
+    def measure(threads):
+        probe = max(threads, 2)
+        rate, _, _ = timed(probe, threads)  # warm-up and sizing
+        per_genome = 3 * int(offsets[probe]) / probe
+        n = int(min(len(slots), max(probe, rate * seconds / per_genome)))
+        n = max(probe, (n // threads) * threads)
+        runs = [timed(n, threads) for _ in range(reps)]
+        rates = sorted(r[0] for r in runs)
+        return {"value": float(np.median(rates)), "threads": threads, "genomes": n, "games": 3 * int(offsets[n]),
+                "decisions": int(runs[0][1]), "seconds_per_run": float(np.median([r[2] for r in runs])),
+                "runs": [float(x) for x in rates]}
+
+    many = measure(share)
+    one = measure(1)
+    return {"value": many["value"], "unit": "game-evals/s", "cores": share, "kind": "port",
+            "sample": f"fitness evaluation of the first {many['genomes']} genomes of generation 0 ({many['games']} "
+                      f"games, {many['decisions']} decisions) at A={A}: median of {reps} runs of "
+                      f"{many['seconds_per_run']:.1f} s on {share} threads",
+            "one_thread": {"value": one["value"], "unit": "game-evals/s",
+                           "sample": f"first {one['genomes']} genomes ({one['games']} games): median of {reps} runs "
+                                     f"of {one['seconds_per_run']:.1f} s"},
+            "runs": many["runs"], "one_thread_runs": one["runs"],
+            "hardware_concurrency": os.cpu_count(), "cpu_model": cpu_model(),
+            "build": "g++ -O3 -march=x86-64-v3 -ffp-contract=off, no fast-math (oracle/build_oracle.py)"}
 
 
 def main():
@@ -81,6 +122,11 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-wallclock", action="store_true", help="skip the wall-clock-to-target leg")
+    ap.add_argument("--wallclock-seeds", default="207,217,227",
+                    help="seeds of the wall-clock-to-target leg (none of them was used to choose a setting)")
+    ap.add_argument("--wallclock-generations", type=int, default=200,
+                    help="plain FP32 generations that define the target (other forms stop at twice as many)")
     ap.add_argument("--fp32-only", action="store_true", help="skip the BF16-screened form")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="first run one plain device-to-device copy of BYTES (a reference kernel of known "
@@ -162,7 +208,7 @@ def main():
     eval_avg_ms = fp32["eval_ms"]
     achieved = fp32["flops"] / (eval_avg_ms * 1e-3) / 1e12
     summary = fp32["summary"]
-    traffic = measured_traffic()
+    traffic, traffic_stale = measured_traffic()
     bf16_out = None
     if bf16 is not None:
         bf16_out = {
@@ -201,8 +247,11 @@ def main():
                    "genomes": n, "actions": A, "games_per_generation": games_per_gen,
                    "mutation_probability": 0.01, "mutation_sigma": 0.02},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_MFMA_TFLOPS, "traffic": traffic,
+                     "frac": achieved / PEAK_FP32_MFMA_TFLOPS, "traffic": None if traffic_stale else traffic,
                      "kernel": "EvaluateKernel", "kernel_ms": eval_avg_ms},
+        "traffic_measurement": {"bytes_per_launch": traffic, "stale": traffic_stale, "file": os.path.relpath(
+            TRAFFIC_FILE, ROOT), "note": "FETCH_SIZE + WRITE_SIZE passes; stale when the kernel sources changed "
+                                         "since (then roofline.traffic is null)"},
         "stages_ms": {"evaluate": st.evaluate_ms, "finalize": st.finalize_ms, "plan": st.plan_ms,
                       "assemble": st.assemble_ms, "bookkeeping": st.bookkeeping_ms},
         "generations_per_s": args.steps / fp32["elapsed"],
@@ -212,6 +261,18 @@ def main():
         "best_fitness": summary.best_fitness,
         "cpu_baseline": cpu,
     }
+    if not args.no_wallclock:
+        import run_configs  # the configuration runner beside the package (imports the same package)
+        seeds = [int(x) for x in args.wallclock_seeds.split(",") if x]
+        print(f"[bench] wall-clock to target: seeds {seeds}, {args.wallclock_generations} plain FP32 generations",
+              file=sys.stderr, flush=True)
+        recs, agg = run_configs.wallclock_to_target(seeds, args.wallclock_generations, cap_factor=2)
+        out["wallclock_to_target"] = dict(agg, workload="64x64 grid, 4739 actions (the bench workload)",
+                                          metric="speedup = plain FP32 seconds to its own best fitness after "
+                                                 "the plain generations / the form's seconds to reach it",
+                                          target_fitness=[r["target_fitness"] for r in recs],
+                                          fp32_seconds_to_target=[r["forms"]["fp32"]["seconds_to_target"]
+                                                                  for r in recs])
     print(json.dumps(out))
 
 

@@ -236,7 +236,18 @@ int ne_advance_generation(ne_runtime *rt, uint32_t generation_seed, const ne_wor
 int ne_get_slab_info(ne_runtime *rt, ne_slab_info *out);
 /* TryDownloadSlabSlotBytesFromDevice (slab_runtime.hpp:128-129). */
 int ne_download_slot(ne_runtime *rt, uint32_t slot_index, void *dst, uint64_t capacity, uint64_t *written);
-/* TryUploadSlabToDevice for one slot (device_runtime.hpp:415); the slot must be occupied. */
+/* TryDownloadSlabFromDevice (slab_runtime.hpp:126; genotype_slab/device_runtime.hpp:91): the whole slab,
+ * any part of it -- slot_count x slot_stride_bytes of slot bytes into `storage` (storage_capacity bytes),
+ * each slot's liveness count (slot_states; 0 = free), the free-slot stack (free_stack; the next slot handed
+ * out is free_stack[*free_count - 1]) and its count.  slot_states and free_stack hold slot_capacity
+ * entries; any output pointer may be NULL. */
+int ne_download_slab(ne_runtime *rt, void *storage, uint64_t storage_capacity, uint32_t *slot_states,
+                     uint32_t *free_stack, uint64_t slot_capacity, uint32_t *free_count);
+/* SwapDeviceSlabGenerationBuffers (slab_runtime.hpp:181; slab_runtime.cu:2128-2135): the current and next
+ * generation exchange slot handles, fitness, evaluation counts, has-fitness flags, indices and sizes (a
+ * generation step ends with this swap; calling it again exchanges them back). */
+int ne_swap_generations(ne_runtime *rt);
+/* TryUploadSlabToDevice for one slot (genotype_slab/device_runtime.hpp:89); the slot must be occupied. */
 int ne_upload_slot(ne_runtime *rt, uint32_t slot_index, const void *src, uint64_t byte_count);
 /* TryDownloadCurrent/NextGenerationFromDevice (slab_runtime.hpp:131-135); which: 0 current,
  * 1 next.  Any output pointer may be NULL.  Arrays hold capacity entries. */
@@ -306,10 +317,9 @@ typedef struct ne_screen_config {
     uint32_t brood_factor;       /* children per cell while screening (4: the best-predicted 25% go on) */
     uint32_t replay_capacity;    /* (fingerprint, fitness) pairs kept for training (65,536) */
     float audit_fraction;        /* random slice of the brood always evaluated in full (0.05) */
-    float spearman_threshold;    /* screening pauses while the audit's rank correlation is below (0: any
-                                    positive correlation raises the expected fitness of the chosen child;
-                                    over 10 seeds at 64x64 / 4,739 actions, 0 reached the target 1.79x
-                                    faster than plain FP32 and 0.5 1.40x) */
+    float spearman_threshold;    /* screening pauses while the audit's rank correlation is below (0.5;
+                                    0 screens whenever the correlation is positive, which raises the
+                                    expected fitness of the chosen child; bench.py times both) */
     float learning_rate;         /* Adam step size (1e-3) */
     uint32_t train_steps;        /* minibatch steps per generation (32) */
     uint32_t batch_size;         /* minibatch size, 1..1024 (256) */

@@ -519,6 +519,10 @@ __global__ __launch_bounds__(320) void PackBf16Kernel(const uint8_t* slab, uint6
 #ifndef NE_EVAL_WAVES_PER_EU
 #define NE_EVAL_WAVES_PER_EU 3  // register budget: 3 waves/SIMD = four 3-wave workgroups per CU
 #endif
+#ifndef NE_EVAL_MAX_WAVES
+#define NE_EVAL_MAX_WAVES 4  // waves per evaluation workgroup the kernel's LDS tables are sized for
+#endif
+constexpr uint32_t kEvalMaxWaves = NE_EVAL_MAX_WAVES;
 constexpr int kChunkTiles = NE_CHUNK_TILES;
 constexpr int kChunkTilesBf16 = 2;  // the BF16 mode keeps candidate lists in LDS as well
 // Chunks move by LDS DMA (buffer_load_dwordx4 ... lds): one instruction moves 64 lanes x 16 bytes to
@@ -783,7 +787,7 @@ constexpr uint32_t kMaxCandidates = 32;  // per game; more within the window -> 
 constexpr uint32_t kLaneCandidates = 8;
 
 template <bool kTrace, bool kBf16>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void EvaluateKernel(EvalArgs p) {
+__global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void EvaluateKernel(EvalArgs p) {
     constexpr int kCT = kBf16 ? kChunkTilesBf16 : kChunkTiles;  // FP32 stream (the BF16 form's fallback)
     using G = ChunkGeom<kCT>;
     constexpr uint32_t kBufBytes = kBf16 && kB16BufBytes > G::kBufBytes ? kB16BufBytes : G::kBufBytes;
@@ -992,11 +996,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
                     // FP32 scan.
                     // each wave's candidate table lives in s_buf0, free from the end of the stream until the
                     // fallback's or the next decision's first transfer
-                    static_assert(sizeof(uint32_t) * 4 * 16 * kMaxCandidates <= kBufBytes, "candidate table");
+                    static_assert(sizeof(uint32_t) * kEvalMaxWaves * 16 * kMaxCandidates <= kBufBytes, "candidate table");
                     auto s_cand = reinterpret_cast<uint32_t (*)[16][kMaxCandidates]>(s_buf0);
-                    __shared__ uint32_t s_ncand[4][16];
-                    __shared__ float s_lsc[4][kLaneCandidates][64];
-                    __shared__ uint32_t s_lac[4][kLaneCandidates][64];
+                    __shared__ uint32_t s_ncand[kEvalMaxWaves][16];
+                    __shared__ float s_lsc[kEvalMaxWaves][kLaneCandidates][64];
+                    __shared__ uint32_t s_lac[kEvalMaxWaves][kLaneCandidates][64];
                     // the policy as the B operands of the two screening instructions: every lane writes its
                     // dims (bf16) at their k-slots in this wave's 2 KB of s_buf1, and reads back its own 2 x 8
                     PolicyBf16 pb;
@@ -1004,7 +1008,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
                         // s_buf1 is free between two barriers: TrunkStream's closing barrier (every wave has
                         // consumed the last trunk chunk, which sat in s_buf1) and StreamChunks' opening
                         // barrier (after which the screening stream's second chunk is issued into s_buf1)
-                        static_assert(4 * 2048 <= kBufBytes, "policy exchange");
+                        static_assert(kEvalMaxWaves * 2048 <= kBufBytes, "policy exchange");
                         __bf16* xb = reinterpret_cast<__bf16*>(s_buf1 + wave * 2048);
     #pragma unroll
                         for (int s = 0; s < kScoreSteps; ++s) {
@@ -2307,6 +2311,8 @@ void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, bool tra
             hipLaunchKernelGGL((EvaluateKernel<false, false>), grid, block, 0, s, a);
     }
 }
+
+uint32_t EvaluateMaxWaves() { return kEvalMaxWaves; }
 
 uint32_t EvaluateResidentPerCu(uint32_t waves, bool trace, bool bf16) {
     // what the registers and LDS of the kernel built here allow, as the runtime reports it

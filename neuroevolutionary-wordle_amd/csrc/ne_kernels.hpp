@@ -241,6 +241,8 @@ void LaunchTransposeEncoder(const uint8_t* slab, uint64_t stride, const uint32_t
 void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, bool trace, bool bf16, hipStream_t s);
 // Workgroups of `waves` waves of the evaluation kernel that fit on one CU at once (0 if unknown).
 uint32_t EvaluateResidentPerCu(uint32_t waves, bool trace, bool bf16);
+// The most waves per evaluation workgroup this build supports (its LDS tables are sized for them).
+uint32_t EvaluateMaxWaves();
 void LaunchFinalize(const float* raw, const uint32_t* local_w, uint32_t n, float* fitness, uint32_t* evals,
                     uint8_t* has, int* status, hipStream_t s);
 void LaunchSummary(const float* fitness, const uint8_t* has, const uint32_t* slots, uint32_t n, SummaryRecord* out,

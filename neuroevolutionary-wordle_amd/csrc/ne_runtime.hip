@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <ctime>
+#include <cstdarg>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -152,6 +154,7 @@ struct ne_runtime {
     bool have_summary = false;
     int last_status = NE_OK;
     bool cur_evaluated = false;  // the current generation's fitness is that of its own evaluation
+    bool verbose = false;        // progress lines of the step being run (the caller's `verbose`)
     // what that fitness was measured on: a screened step reuses it only while these still hold
     uint64_t upload_epoch = 1;          // bumped by every catalog or secrets upload (not by evaluations)
     uint64_t fit_upload_epoch = 0;      // upload_epoch when the fitness was measured (0: unknown)
@@ -282,6 +285,30 @@ uint64_t SlotCountForBudget(uint64_t budget, uint64_t action_count, uint64_t cei
     if (stride == 0 || budget < stride) return 0;
     const uint64_t n = budget / stride;
     return (ceiling == 0 || n < ceiling) ? n : ceiling;
+}
+
+// Verbose progress lines in the reference's form (src/common/progress_log.hpp:16-48): "[local time]
+// message" on stdout, a stage's duration appended as " in X.X ms".
+using ProgressClock = std::chrono::steady_clock;
+
+void Progress(const ne_runtime* rt, const char* fmt, ...) {
+    if (!rt->verbose) return;
+    char msg[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(msg, sizeof(msg), fmt, ap);
+    va_end(ap);
+    const std::time_t now = std::time(nullptr);
+    std::tm local{};
+    localtime_r(&now, &local);
+    char stamp[32];
+    std::strftime(stamp, sizeof(stamp), "%Y-%m-%d %H:%M:%S", &local);
+    std::printf("[%s] %s\n", stamp, msg);
+    std::fflush(stdout);
+}
+
+double MsSince(ProgressClock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(ProgressClock::now() - t0).count();
 }
 
 void EventBegin(ne_runtime* rt, int i) { (void)hipEventRecord(rt->ev[i].begin, rt->stream); }
@@ -892,12 +919,20 @@ static int Assemble(ne_runtime* rt, uint32_t seed, const ne_assembly_config& cfg
                 return Fail(rt, NE_SLAB_FULL);
             if (offset > rt->config.host_spillover_byte_budget / rt->stride)
                 return Fail(rt, NE_DEVICE_FAILURE);  // the reference's outcome when the spill slab is too small
+            Progress(rt, "Generation %llu: slab full, spilling %llu assembled children to host memory",
+                     (unsigned long long)rt->next_index, (unsigned long long)offset);
+            const auto ts = ProgressClock::now();
             st = SpillPrefix(rt, offset, spill);
             if (st != NE_OK) return st;
+            Progress(rt, "Generation %llu: spilled %llu children in %.1f ms", (unsigned long long)rt->next_index,
+                     (unsigned long long)offset, MsSince(ts));
             spilled = offset;
             continue;
         }
         const uint64_t batch = std::min<uint64_t>(rt->next_size - offset, fc);
+        const auto tb = ProgressClock::now();
+        Progress(rt, "Generation %llu: assembly batch from child %llu, %llu children (%u free slots)",
+                 (unsigned long long)rt->next_index, (unsigned long long)offset, (unsigned long long)batch, fc);
         BreedArgs b{};
         b.slab = rt->slab;
         b.parent_stride = rt->stride;
@@ -932,6 +967,8 @@ static int Assemble(ne_runtime* rt, uint32_t seed, const ne_assembly_config& cfg
         EventEnd(rt, 7);
         st = Finish(rt);
         if (st != NE_OK) return st;
+        Progress(rt, "Generation %llu: assembly batch from child %llu done in %.1f ms", (unsigned long long)rt->next_index,
+                 (unsigned long long)offset, MsSince(tb));
         breed_ms += EventMs(rt, 3);
         book_ms += EventMs(rt, 7);
         offset += batch;
@@ -1672,13 +1709,25 @@ static int PrepareBoundary(ne_runtime* rt, uint32_t generation_seed, const ne_wo
                            const ne_assembly_config& cfg, ne_pending_injection& inj, const ne_callbacks* callbacks,
                            const ne_shard_release_history* history, uint64_t* parent_a) {
     rt->times = ne_stage_times{};
+    const uint64_t g = rt->cur_index + 1;
+    Progress(rt, "Generation %llu: starting advancement from generation %llu (%llu genomes, %llu actions)",
+             (unsigned long long)g, (unsigned long long)rt->cur_index, (unsigned long long)rt->cur_size,
+             (unsigned long long)rt->action_count);
+    Progress(rt, "Generation %llu: evaluating current generation fitness", (unsigned long long)g);
+    auto t0 = ProgressClock::now();
     int st = EvaluateFull(rt, counts, history);
     if (st != NE_OK) return st;
+    Progress(rt, "Generation %llu: current generation fitness evaluated (%llu games, %llu decisions, best %.6f) in %.1f ms",
+             (unsigned long long)g, (unsigned long long)rt->times.games, (unsigned long long)rt->times.decisions,
+             double(rt->last_summary.best_fitness), MsSince(t0));
     if (callbacks && callbacks->training_shard_release &&
         callbacks->training_shard_release(rt, counts, &inj, callbacks->user) != 0)
         return Fail(rt, NE_CALLBACK_FAILED);
+    Progress(rt, "Generation %llu: planning next generation shape", (unsigned long long)g);
     uint64_t next_a = 0, next_n = 0;
     if (!PlanShape(rt, inj, &next_a, &next_n)) return Fail(rt, NE_INVALID_ASSEMBLY_CONFIG);
+    Progress(rt, "Generation %llu: next generation shape planned (%llu genomes, %llu actions%s)", (unsigned long long)g,
+             (unsigned long long)next_n, (unsigned long long)next_a, inj.enabled ? ", widening" : "");
     if (inj.enabled && inj.first_catalog_word_index + inj.injection_count > rt->catalog_count)
         return Fail(rt, NE_OUTPUT_EMBEDDING_INJECTION_FAILED);
     const uint64_t radius = cfg.cellular_breeding_radius;
@@ -1696,6 +1745,8 @@ static int PrepareBoundary(ne_runtime* rt, uint32_t generation_seed, const ne_wo
     pa.rank_weights = rt->rank_weights;
     pa.pairs = rt->pairs;
     pa.status = rt->status;
+    Progress(rt, "Generation %llu: building next-generation parent plan", (unsigned long long)g);
+    t0 = ProgressClock::now();
     EventBegin(rt, 2);
     LaunchPlan(pa, rt->stream);
     EventEnd(rt, 2);
@@ -1703,13 +1754,20 @@ static int PrepareBoundary(ne_runtime* rt, uint32_t generation_seed, const ne_wo
     if (st != NE_OK) return st;
     rt->times.plan_ms = EventMs(rt, 2);
     rt->planned_children = next_n;
+    Progress(rt, "Generation %llu: parent plan built (%llu children) in %.1f ms", (unsigned long long)g,
+             (unsigned long long)next_n, MsSince(t0));
     if (callbacks && callbacks->post_fitness_evaluation &&
         callbacks->post_fitness_evaluation(rt, counts, callbacks->user) != 0)
         return Fail(rt, NE_CALLBACK_FAILED);
     *parent_a = rt->action_count;
     if (inj.enabled) {
+        Progress(rt, "Generation %llu: widening the slab from %llu to %llu actions", (unsigned long long)g,
+                 (unsigned long long)rt->action_count, (unsigned long long)next_a);
+        t0 = ProgressClock::now();
         st = Widen(rt, next_a);
         if (st != NE_OK) return st;
+        Progress(rt, "Generation %llu: slab widened (%llu slots of %llu bytes) in %.1f ms", (unsigned long long)g,
+                 (unsigned long long)rt->slot_count, (unsigned long long)rt->stride, MsSince(t0));
     }
     return NE_OK;
 }
@@ -1718,14 +1776,24 @@ static int PrepareBoundary(ne_runtime* rt, uint32_t generation_seed, const ne_wo
 // batches, the output-embedding injection, and the generation swap.
 static int AssemblePrepared(ne_runtime* rt, uint32_t generation_seed, const ne_assembly_config& cfg,
                             uint64_t parent_a, const ne_pending_injection& inj) {
+    const uint64_t g = rt->cur_index + 1;
+    const auto t0 = ProgressClock::now();
+    Progress(rt, "Generation %llu: assembling %llu children", (unsigned long long)g,
+             (unsigned long long)rt->planned_children);
     int st = Assemble(rt, generation_seed, cfg, parent_a);
     if (st != NE_OK) return st;
     if (inj.enabled) {
+        Progress(rt, "Generation %llu: injecting %llu output-embedding rows", (unsigned long long)g,
+                 (unsigned long long)inj.injection_count);
+        const auto ti = ProgressClock::now();
         st = Inject(rt, parent_a, inj);
         if (st != NE_OK) return st;
+        Progress(rt, "Generation %llu: output-embedding rows injected in %.1f ms", (unsigned long long)g, MsSince(ti));
     }
     SwapGenerations(rt);
     rt->last_status = NE_OK;
+    Progress(rt, "Generation %llu: advancement finished in %.1f ms (assembly and swap)", (unsigned long long)g,
+             MsSince(t0));
     return NE_OK;
 }
 
@@ -1922,17 +1990,27 @@ extern "C" {
 int ne_advance_generation(ne_runtime* rt, uint32_t generation_seed, const ne_word_counts* counts,
                           const ne_assembly_config* config, const ne_pending_injection* injection, uint32_t verbose,
                           const ne_callbacks* callbacks, const ne_shard_release_history* history) {
-    (void)verbose;
     if (!rt || !counts) return Fail(rt, NE_INVALID_ARGUMENT);
     const ne_assembly_config cfg = config ? *config : DefaultAssembly();
     if (!ValidAssembly(cfg)) return Fail(rt, NE_INVALID_ASSEMBLY_CONFIG);
     ne_pending_injection inj{};
     if (injection) inj = *injection;
+    struct VerboseScope {
+        ne_runtime* rt;
+        ~VerboseScope() { rt->verbose = false; }
+    } verbose_scope{rt};
+    rt->verbose = verbose != 0;
     const bool plain = !inj.enabled && (!callbacks || (!callbacks->training_shard_release &&
                                                        !callbacks->post_fitness_evaluation));
     if (plain && rt->step_graph_on) {
         bool done = false;
+        const uint64_t g = rt->cur_index + 1;
+        const auto t0 = ProgressClock::now();
         const int gst = TryGraphStep(rt, generation_seed, counts, cfg, &done);
+        if (done && gst == NE_OK)
+            Progress(rt, "Generation %llu: advanced as one captured step (%llu games, best %.6f) in %.1f ms",
+                     (unsigned long long)g, (unsigned long long)rt->times.games,
+                     double(rt->last_summary.best_fitness), MsSince(t0));
         if (done || gst != NE_OK) return gst;
     }
     uint64_t parent_a = 0;
@@ -1964,6 +2042,37 @@ int ne_download_slot(ne_runtime* rt, uint32_t slot_index, void* dst, uint64_t ca
     if (!Ok(hipMemcpy(dst, rt->slab + uint64_t(slot_index) * rt->stride, rt->stride, hipMemcpyDeviceToHost)))
         return Fail(rt, NE_DEVICE_FAILURE);
     if (written) *written = rt->stride;
+    return NE_OK;
+}
+
+int ne_download_slab(ne_runtime* rt, void* storage, uint64_t storage_capacity, uint32_t* slot_states,
+                     uint32_t* free_stack, uint64_t slot_capacity, uint32_t* free_count) {
+    if (!rt) return NE_INVALID_ARGUMENT;
+    if (storage && storage_capacity < rt->slot_count * rt->stride) return Fail(rt, NE_INVALID_ARGUMENT);
+    if ((slot_states || free_stack) && slot_capacity < rt->slot_count) return Fail(rt, NE_INVALID_ARGUMENT);
+    bool ok = Ok(hipStreamSynchronize(rt->stream));
+    if (ok && storage) ok = Ok(hipMemcpy(storage, rt->slab, rt->slot_count * rt->stride, hipMemcpyDeviceToHost));
+    if (ok && slot_states)
+        ok = Ok(hipMemcpy(slot_states, rt->slot_state, rt->slot_count * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (ok && free_stack)
+        ok = Ok(hipMemcpy(free_stack, rt->free_stack, rt->slot_count * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (ok && free_count) ok = Ok(hipMemcpy(free_count, rt->free_count, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return ok ? NE_OK : Fail(rt, NE_DEVICE_FAILURE);
+}
+
+// SwapDeviceSlabGenerationBuffers (slab_runtime.cu:2128-2135): the current and next generation exchange
+// their arrays, indices and sizes.  The new current generation's fitness is not taken as measured for the
+// screened step (it re-evaluates).
+int ne_swap_generations(ne_runtime* rt) {
+    if (!rt) return NE_INVALID_ARGUMENT;
+    std::swap(rt->cur_slots, rt->nxt_slots);
+    std::swap(rt->cur_fit, rt->nxt_fit);
+    std::swap(rt->cur_evals, rt->nxt_evals);
+    std::swap(rt->cur_has, rt->nxt_has);
+    std::swap(rt->cur_index, rt->next_index);
+    std::swap(rt->cur_size, rt->next_size);
+    rt->cur_evaluated = false;
+    rt->have_summary = false;
     return NE_OK;
 }
 
@@ -2103,7 +2212,7 @@ int ne_download_trace(ne_runtime* rt, uint32_t kind, void* dst, uint64_t capacit
 
 // ---- evaluation shape ---------------------------------------------------------------------------
 int ne_set_eval_shape(ne_runtime* rt, uint32_t waves) {
-    if (!rt || waves > 4) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (!rt || waves > EvaluateMaxWaves()) return Fail(rt, NE_INVALID_ARGUMENT);
     rt->tuned_waves = waves;
     return NE_OK;
 }
@@ -2557,8 +2666,12 @@ int ne_create_prebreeding_checkpoint(ne_runtime* rt, uint32_t generation_seed, c
                                      const uint8_t* catalog_letters, uint64_t catalog_word_count, uint32_t verbose,
                                      const ne_callbacks* callbacks, const ne_shard_release_history* history,
                                      ne_checkpoint** out_checkpoint) {
-    (void)verbose;
     if (!rt || !counts || !out_checkpoint) return Fail(rt, NE_INVALID_ARGUMENT);
+    struct VerboseScope {
+        ne_runtime* rt;
+        ~VerboseScope() { rt->verbose = false; }
+    } verbose_scope{rt};
+    rt->verbose = verbose != 0;
     *out_checkpoint = nullptr;
     if (history && (history->release_count > ne_ckpt::kReleaseCapacity ||
                     (history->release_count && !history->release_generations)))
@@ -2708,8 +2821,14 @@ int ne_restore_prebreeding_checkpoint(const ne_checkpoint* c, void* stream, ne_r
 }
 
 int ne_resume_generation_from_checkpoint(ne_runtime* rt, const ne_checkpoint* c, uint32_t verbose) {
-    (void)verbose;
     if (!rt || !c) return Fail(rt, NE_INVALID_ARGUMENT);
+    struct VerboseScope {
+        ne_runtime* rt;
+        ~VerboseScope() { rt->verbose = false; }
+    } verbose_scope{rt};
+    rt->verbose = verbose != 0;
+    Progress(rt, "Generation %llu: resuming from the pre-recombination checkpoint of generation %llu",
+             (unsigned long long)(rt->cur_index + 1), (unsigned long long)rt->cur_index);
     if (c->Stride() != rt->stride || c->ActionCount() != rt->action_count || c->generation_index != rt->cur_index ||
         c->slots.size() != rt->cur_size || c->ChildCount() != rt->planned_children || rt->planned_children == 0)
         return Fail(rt, NE_INVALID_GENERATION);

@@ -321,7 +321,7 @@ struct SurrogateScreeningConfig {
     std::uint32_t brood_factor = 4;
     std::uint32_t replay_capacity = 65536;
     float audit_fraction = 0.05f;
-    float spearman_threshold = 0.0f;
+    float spearman_threshold = 0.5f;
     float learning_rate = 1e-3f;
     std::uint32_t train_steps = 32;
     std::uint32_t batch_size = 256;
@@ -566,6 +566,37 @@ inline bool TryDownloadSlabSlotBytesFromDevice(const DeviceSlabGARuntimeBuffers&
     slot_bytes.assign(info.slot_stride_bytes, 0);
     uint64_t written = 0;
     return ne_download_slot(buffers.handle, slot_index, slot_bytes.data(), slot_bytes.size(), &written) == NE_OK;
+}
+
+// HostGenotypeSlab (genotype_slab/slab_allocator.hpp:23-30) as this build downloads it: the layout, every
+// slot's bytes, each slot's liveness count (0 = free) and the free-slot stack with its count.
+struct HostGenotypeSlab {
+    ne_slab_info layout{};
+    std::vector<std::uint8_t> storage;
+    std::vector<std::uint32_t> slot_liveness;
+    std::vector<std::uint32_t> free_slot_stack;
+    std::uint32_t free_slot_count = 0;
+    const std::uint8_t* SlotBytes(std::uint32_t slot) const {
+        return storage.data() + std::size_t(slot) * layout.slot_stride_bytes;
+    }
+};
+
+// TryDownloadSlabFromDevice (slab_runtime.hpp:126)
+inline bool TryDownloadSlabFromDevice(const DeviceSlabGARuntimeBuffers& buffers, HostGenotypeSlab& host_buffer) {
+    host_buffer = {};
+    if (ne_get_slab_info(buffers.handle, &host_buffer.layout) != NE_OK) return false;
+    const std::uint64_t slots = host_buffer.layout.slot_count;
+    host_buffer.storage.assign(slots * host_buffer.layout.slot_stride_bytes, 0);
+    host_buffer.slot_liveness.assign(slots, 0);
+    host_buffer.free_slot_stack.assign(slots, 0);
+    return ne_download_slab(buffers.handle, host_buffer.storage.data(), host_buffer.storage.size(),
+                            host_buffer.slot_liveness.data(), host_buffer.free_slot_stack.data(), slots,
+                            &host_buffer.free_slot_count) == NE_OK;
+}
+
+// SwapDeviceSlabGenerationBuffers (slab_runtime.hpp:181)
+inline void SwapDeviceSlabGenerationBuffers(DeviceSlabGARuntimeBuffers& buffers) noexcept {
+    (void)ne_swap_generations(buffers.handle);
 }
 
 // TryReadDeviceSlabGARuntimeStatus (slab_runtime.hpp:183-184)

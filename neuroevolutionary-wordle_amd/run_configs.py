@@ -7,9 +7,10 @@
   published the reference's two published evaluation workloads (1,024 genomes / 20 actions / 20 words per cell;
             672 genomes / 2,000 actions / 2,000 words per cell), evaluation only, against its published times
   target    64x64 grid, 4,739 actions: wall-clock to the plain FP32 run's best fitness (over G generations,
-            default 200) for the BF16-screened form and the surrogate-screened form (brood 4x, best-predicted
-            25% + 5% audit, BF16-screened evaluation), per seed (default 10 seeds), then the geometric mean,
-            median and total-time ratio of the speedups
+            default 200) for the BF16-screened form (eager and captured) and the surrogate-screened form
+            (brood 4x, best-predicted 25% + 5% audit, BF16-screened evaluation; rank-correlation gate 0.5
+            and 0), per seed (default 10 seeds), then the geometric mean, median and total-time ratio of the
+            speedups
 
 Every run is timed in the plain FP32 form and in the BF16-screened form (which takes the same decisions);
 `--forms fp32` skips the latter; grid16 and grid128 also take the forms fp32-graph and bf16-graph (every step
@@ -174,17 +175,24 @@ def published_config(forms, reps=5):
                 rt.close()
 
 
-def target_config(seeds, gens, cap_factor=2):
-    """Wall-clock to target: the plain FP32 run's best fitness over `gens` generations is the target; the
-    BF16-screened form (same decisions) and the surrogate-screened form run until their best-so-far
-    reaches it (at most cap_factor * gens generations).  Host wall-clock per step, steps only."""
+TARGET_FORMS = ("bf16", "bf16-graph", "surrogate", "surrogate-rho0")
+
+
+def wallclock_to_target(seeds, gens, cap_factor=2, forms=TARGET_FORMS, log=sys.stderr):
+    """Wall-clock to target at 64x64 / 4,739 actions: per seed, the plain FP32 run's best fitness over `gens`
+    generations is the target; every other form runs until its best-so-far reaches it (at most
+    cap_factor * gens generations).  Forms: "bf16" (BF16-screened evaluation, the FP32 form's decisions),
+    "bf16-graph" (the same, every step one captured hipGraph), "surrogate" (surrogate-screened evolution,
+    brood 4x, best-predicted 25% + 5% audit, BF16-screened evaluation, plain steps while the audit's rank
+    correlation is below 0.5) and "surrogate-rho0" (the same gated at 0).  Host wall-clock per step, steps
+    only.  Returns (per-seed records, aggregate)."""
     wl = synth.workload("grid64_a4739")
     A, side = wl["actions"], wl["side"]
     n = side * side
     stride = rtm.slot_stride_bytes(A)
     counts = rtm.word_counts(A, A)
     cfg = rtm.default_assembly_config(mutation_probability=0.01, mutation_sigma=0.02)
-    scfg = rtm.default_screen_config()
+    gates = {"surrogate": 0.5, "surrogate-rho0": 0.0}
     out = []
     for seed in seeds:
         def make(mode, brood):
@@ -196,7 +204,7 @@ def target_config(seeds, gens, cap_factor=2):
             rt.bootstrap(n, seed=seed, variance_scale=1.0, tail_std=0.1)
             return rt
 
-        def run(rt, step, limit, target=None):
+        def run(rt, step, limit, label, target=None):
             t, best, trace, screened, rhos = 0.0, -1.0, [], 0, []
             for g in range(limit):
                 t0 = time.perf_counter()
@@ -208,8 +216,8 @@ def target_config(seeds, gens, cap_factor=2):
                     screened += rep.screened
                     if np.isfinite(rep.audit_spearman):
                         rhos.append(float(rep.audit_spearman))
-                if (g + 1) % 50 == 0:
-                    print(f"[target] seed {seed} generation {g + 1} best {best:.5f} {t:.2f} s", file=sys.stderr,
+                if (g + 1) % 100 == 0 and log is not None:
+                    print(f"[target] seed {seed} {label} generation {g + 1} best {best:.5f} {t:.2f} s", file=log,
                           flush=True)
                 if target is not None and best >= target:
                     break
@@ -219,7 +227,7 @@ def target_config(seeds, gens, cap_factor=2):
         screened_step = lambda rt, g: rt.advance_screened(seed + 2 + g, counts, cfg)  # noqa: E731
         rt = make(rtm.EVAL_FP32, 1)
         try:
-            trace_p, _, _ = run(rt, plain, gens)
+            trace_p, _, _ = run(rt, plain, gens, "fp32")
         finally:
             rt.close()
         target = trace_p[-1][1]
@@ -229,13 +237,14 @@ def target_config(seeds, gens, cap_factor=2):
                "target_fitness": target, "forms": {}}
         rec["forms"]["fp32"] = {"seconds_to_target": t_plain, "generations_to_target": g_plain, "reached": True,
                                 "seconds_per_generation": trace_p[-1][0] / len(trace_p)}
-        for form in ("bf16", "bf16-graph", "surrogate"):
-            rt = make(rtm.EVAL_BF16_RESCORED, 4 if form == "surrogate" else 1)
+        for form in forms:
+            surrogate = form in gates
+            rt = make(rtm.EVAL_BF16_RESCORED, 4 if surrogate else 1)
             try:
-                if form == "surrogate":
-                    rt.enable_screening(scfg)
+                if surrogate:
+                    rt.enable_screening(rtm.default_screen_config(spearman_threshold=gates[form]))
                 rt.set_step_graph(form == "bf16-graph")
-                tr, scr, rhos = run(rt, plain if form != "surrogate" else screened_step, cap_factor * gens, target)
+                tr, scr, rhos = run(rt, screened_step if surrogate else plain, cap_factor * gens, form, target)
             finally:
                 rt.close()
             reached = tr[-1][1] >= target
@@ -243,21 +252,29 @@ def target_config(seeds, gens, cap_factor=2):
                                   "generations_to_target": len(tr) if reached else None, "reached": reached,
                                   "seconds_per_generation": tr[-1][0] / len(tr),
                                   "speedup_vs_fp32": t_plain / tr[-1][0] if reached else None}
-            if form == "surrogate":
-                rec["forms"][form].update({"screened_generations": scr, "generations": len(tr),
+            if surrogate:
+                rec["forms"][form].update({"spearman_threshold": gates[form], "screened_generations": scr,
+                                           "generations": len(tr),
                                            "audit_spearman_mean": float(np.mean(rhos)) if rhos else None})
-        print(json.dumps(rec), flush=True)
         out.append(rec)
-    agg = {"config": "target", "seeds": [r["seed"] for r in out]}
-    for form in ("bf16", "bf16-graph", "surrogate"):
+    agg = {"seeds": [r["seed"] for r in out], "plain_generations": gens, "cap_generations": cap_factor * gens}
+    for form in forms:
         sp = [r["forms"][form]["speedup_vs_fp32"] for r in out if r["forms"][form]["reached"]]
         t_form = sum(r["forms"][form]["seconds_to_target"] for r in out if r["forms"][form]["reached"])
         t_fp32 = sum(r["forms"]["fp32"]["seconds_to_target"] for r in out if r["forms"][form]["reached"])
         agg[form] = {"reached": len(sp), "geometric_mean_speedup": float(np.exp(np.mean(np.log(sp)))) if sp else None,
                      "median_speedup": float(np.median(sp)) if sp else None,
-                     "total_time_ratio": t_fp32 / t_form if sp else None}
-    print(json.dumps(agg), flush=True)
-    return out
+                     "total_time_ratio": t_fp32 / t_form if sp else None,
+                     "per_seed": [r["forms"][form]["speedup_vs_fp32"] for r in out]}
+    return out, agg
+
+
+def target_config(seeds, gens, cap_factor=2):
+    recs, agg = wallclock_to_target(seeds, gens, cap_factor)
+    for r in recs:
+        print(json.dumps(r), flush=True)
+    print(json.dumps(dict(agg, config="target")), flush=True)
+    return recs
 
 
 def main():

@@ -124,8 +124,8 @@ FINGERPRINT_DIMS = 192
 def default_screen_config(**kw):
     """The surrogate-screened configuration: brood 4x, top 25% (each cell's best-predicted child) plus a
     5% random audit, Adam lr 1e-3 on a replay buffer of 65,536 pairs, plain steps while the audit Spearman
-    rank correlation is below 0 (an uninformative or inverted surrogate)."""
-    c = ScreenConfig(brood_factor=4, replay_capacity=65536, audit_fraction=0.05, spearman_threshold=0.0,
+    rank correlation is below 0.5."""
+    c = ScreenConfig(brood_factor=4, replay_capacity=65536, audit_fraction=0.05, spearman_threshold=0.5,
                      learning_rate=1e-3, train_steps=32, batch_size=256, min_replay=4096, seed=0x5C2EE7, flags=0)
     for k, v in kw.items():
         setattr(c, k, v)
@@ -208,7 +208,7 @@ class Callbacks(C.Structure):
 EXPORTED_SYMBOLS = [
     "ne_create", "ne_destroy", "ne_upload_catalog", "ne_upload_secrets_csr", "ne_clear_secrets", "ne_bootstrap",
     "ne_evaluate", "ne_read_summary", "ne_advance_generation", "ne_get_slab_info", "ne_download_slot",
-    "ne_upload_slot", "ne_download_generation", "ne_download_plan", "ne_download_local_word_counts",
+    "ne_download_slab", "ne_swap_generations", "ne_upload_slot", "ne_download_generation", "ne_download_plan", "ne_download_local_word_counts",
     "ne_read_status", "ne_status_string", "ne_get_stage_times", "ne_set_trace", "ne_download_trace",
     "ne_build_shard_csr", "ne_set_eval_precision", "ne_enable_screening", "ne_advance_generation_screened",
     "ne_fingerprint", "ne_get_probe_states", "ne_surrogate_predict", "ne_surrogate_train_step",
@@ -249,6 +249,8 @@ def load_library(path=LIB_PATH):
                                         C.POINTER(ShardReleaseHistory)]),
         "ne_get_slab_info": (i32, [p, C.POINTER(SlabInfo)]),
         "ne_download_slot": (i32, [p, u32, p, u64, C.POINTER(u64)]),
+        "ne_download_slab": (i32, [p, p, u64, p, p, u64, p]),
+        "ne_swap_generations": (i32, [p]),
         "ne_upload_slot": (i32, [p, u32, p, u64]),
         "ne_download_generation": (i32, [p, i32, p, p, p, p, u64, C.POINTER(u64)]),
         "ne_download_plan": (i32, [p, p, u64, C.POINTER(u64)]),
@@ -414,13 +416,14 @@ class SlabRuntime:
         hist, self._hist_keep = _history(history)
         self._check("ne_evaluate", self.lib.ne_evaluate(self.h, C.byref(counts), hist))
 
-    def advance(self, seed, counts, config=None, injection=None, callbacks=None, history=None):
+    def advance(self, seed, counts, config=None, injection=None, callbacks=None, history=None, verbose=False):
         cfg = C.byref(config) if config is not None else None
         inj = C.byref(injection) if injection is not None else None
         cbs = C.byref(callbacks) if callbacks is not None else None
         hist, self._hist_keep = _history(history)
         self._check("ne_advance_generation",
-                    self.lib.ne_advance_generation(self.h, seed, C.byref(counts), cfg, inj, 0, cbs, hist))
+                    self.lib.ne_advance_generation(self.h, seed, C.byref(counts), cfg, inj, 1 if verbose else 0, cbs,
+                                                   hist))
 
     # --- telemetry and the curriculum controller
     def fitness_telemetry(self, counts, breeding=False):
@@ -516,6 +519,24 @@ class SlabRuntime:
         self._check("ne_download_slot", self.lib.ne_download_slot(self.h, int(slot), _ptr(buf), buf.nbytes,
                                                                   C.byref(w)))
         return buf
+
+    def download_slab(self, storage=True):
+        """TryDownloadSlabFromDevice: (slot bytes [slot_count, stride] as uint8 or None, slot liveness counts,
+        free-slot stack, free count)."""
+        info = self.info()
+        S = info.slot_count
+        data = np.zeros((S, info.slot_stride_bytes), np.uint8) if storage else None
+        states = np.zeros(S, np.uint32)
+        stack = np.zeros(S, np.uint32)
+        fc = C.c_uint32()
+        self._check("ne_download_slab", self.lib.ne_download_slab(
+            self.h, _ptr(data) if storage else None, data.nbytes if storage else 0, _ptr(states), _ptr(stack), S,
+            C.byref(fc)))
+        return data, states, stack, int(fc.value)
+
+    def swap_generations(self):
+        """SwapDeviceSlabGenerationBuffers."""
+        self._check("ne_swap_generations", self.lib.ne_swap_generations(self.h))
 
     def upload_slot(self, slot, values):
         values = np.ascontiguousarray(values, dtype=np.float32)

@@ -199,7 +199,7 @@ def test_graph_steps_between_screened_steps(pkg):
     stride = rtm.slot_stride_bytes(A)
     counts = rtm.word_counts(A, A)
     cfg = rtm.default_assembly_config(mutation_probability=0.02, mutation_sigma=0.03)
-    scfg = rtm.default_screen_config(brood_factor=2, min_replay=256)
+    scfg = rtm.default_screen_config(brood_factor=2, min_replay=256, spearman_threshold=0.0)
 
     def make(graph):
         rt = pkg.SlabRuntime(slab_bytes=4 * n * stride, generation_bytes=2 * n * stride, action_count=A,
