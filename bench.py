@@ -123,6 +123,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-wallclock", action="store_true", help="skip the wall-clock-to-target leg")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="time every form at the built-in launch shape instead of the autotuner's")
     ap.add_argument("--wallclock-seeds", default="207,217,227",
                     help="seeds of the wall-clock-to-target leg (none of them was used to choose a setting)")
     ap.add_argument("--wallclock-generations", type=int, default=200,
@@ -170,12 +172,21 @@ def main():
         print(f"[bench] cpu baseline {cpu['value']:.1f} game-evals/s on {cpu['cores']} threads", file=sys.stderr,
               flush=True)
 
+    shapes = {}
+
     def run_form(mode, graph=False):
         """Bootstraps the same generation 0 and times `steps` generation steps in one form (eagerly, or as
         captured step graphs: then only the whole replay is timed)."""
         rt.set_eval_precision(mode)
-        rt.set_step_graph(graph)
+        rt.set_step_graph(False)
         rt.bootstrap(n, seed=7, variance_scale=1.0, tail_std=0.1)
+        if args.autotune:
+            # the evaluation's launch shape for this form, chosen on generation 0 (results do not depend on it)
+            _, wave_ms, shape = rt.autotune_evaluation(counts)
+            shapes[(mode, graph)] = {"waves": shape.waves, "rounds_per_item": shape.rounds_per_item,
+                                     "workgroups_per_cu": shape.workgroups_per_cu,
+                                     "waves_sweep_ms": [round(x, 3) for x in wave_ms]}
+        rt.set_step_graph(graph)
         gen = 0
         for _ in range(args.warmup):
             rt.advance(7 + 2 + gen, counts, cfg)
@@ -259,6 +270,8 @@ def main():
         "bf16_screened_fp32_rescored": bf16_out,
         "captured_step_graph": graph_out,
         "best_fitness": summary.best_fitness,
+        "launch_shapes": {("fp32" if m == rtm.EVAL_FP32 else "bf16") + ("_graph" if g else ""): v
+                          for (m, g), v in shapes.items()},
         "cpu_baseline": cpu,
     }
     if not args.no_wallclock:

@@ -161,6 +161,10 @@ typedef struct ne_stage_times {
     float step_graph_ms;                  /* captured step (ne_set_step_graph): the whole graph's replay;
                                              the per-stage fields above are then 0 */
     uint32_t step_graph;                  /* 0: eager step, 1: a captured step replayed, 2: captured, then replayed */
+    uint64_t repack_survivors;            /* widening: parents moved to the wider layout */
+    uint64_t repack_old_stride;           /* widening: their slot stride before the repack (bytes) */
+    float repack_copy_ms;                 /* widening: the gather and scatter kernels alone */
+    uint32_t reserved0;
 } ne_stage_times;
 
 /* Evaluation precision (ne_set_eval_precision).  NE_EVAL_FP32 is the reference's arithmetic.
@@ -279,16 +283,31 @@ int ne_set_trace(ne_runtime *rt, uint32_t flags);
 /* Selects the evaluation precision for later evaluate/advance calls (no reference counterpart: the
  * reference evaluates in its storage precision only, evaluation_ops.cuh:244-508). */
 int ne_set_eval_precision(ne_runtime *rt, uint32_t mode);
-/* Autotuner of the evaluation's workgroup shape (no reference counterpart): evaluates the current
- * generation once per shape -- 1, 2, 3 and 4 waves, i.e. 16 to 64 games per workgroup of the persistent
- * evaluation kernel -- times each with HIP events, and keeps the fastest for later evaluations of this
- * handle.  Results do not depend on the shape, so the generation is left evaluated as by ne_evaluate.
- * timings_ms (optional) receives the 4 measured times. */
+/* Shape of the persistent evaluation kernel's launch (no reference counterpart). */
+typedef struct ne_eval_shape {
+    uint32_t waves;              /* waves (tiles of 16 games) per workgroup, 1..4; 0: the built-in choice
+                                    (the size leaving the fewest idle wave slots) */
+    uint32_t rounds_per_item;    /* rounds of `waves` tiles one work item plays (1: a genome's tiles spread
+                                    over ceil(tiles / waves) items; r: over ceil(tiles / (r x waves))) */
+    uint32_t workgroups_per_cu;  /* persistent workgroups per CU; 0: as many as the kernel's registers and
+                                    LDS allow */
+    uint32_t reserved0;
+} ne_eval_shape;
+/* Autotuner of the evaluation's launch shape (no reference counterpart): a coordinate search over the
+ * three knobs of ne_eval_shape, each candidate timed with HIP events on one evaluation of the current
+ * generation -- waves 1..4 (rounds 1, all workgroups), then rounds 2 and 4, then one or two workgroups per
+ * CU fewer than fit -- keeping the fastest for later evaluations of this handle.  Results do not depend on
+ * the shape; the generation is left evaluated (with the chosen shape) as by ne_evaluate.  chosen_waves
+ * (optional) receives the chosen waves, timings_ms (optional) the 4 times of the waves sweep;
+ * ne_get_eval_shape reports the whole choice. */
 int ne_autotune_evaluation(ne_runtime *rt, const ne_word_counts *counts, const ne_shard_release_history *history,
                            uint32_t *chosen_waves, float *timings_ms);
-/* Sets the evaluation's waves per workgroup (1..4), or 0 for the built-in choice (the size leaving the
- * fewest idle wave slots). */
+/* Sets the evaluation's waves per workgroup (1..4), or 0 for the built-in choice; the other knobs keep their
+ * values. */
 int ne_set_eval_shape(ne_runtime *rt, uint32_t waves);
+/* Sets or reads every knob of the evaluation's launch shape. */
+int ne_set_eval_shape_ex(ne_runtime *rt, const ne_eval_shape *shape);
+int ne_get_eval_shape(ne_runtime *rt, ne_eval_shape *out);
 
 /* Captured generation steps (no counterpart in the reference, which launches every stage from the
  * host).  With enabled = 1, ne_advance_generation runs a plain step -- no injection, no callbacks, no

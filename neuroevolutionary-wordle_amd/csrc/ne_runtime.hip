@@ -148,7 +148,7 @@ struct ne_runtime {
     uint64_t trace_capacity = 0;
 
     // timing
-    StageEvents ev[12];
+    StageEvents ev[13];  // 12: the repack's gather and scatter copies
     ne_stage_times times{};
     ne_fitness_summary last_summary{};
     bool have_summary = false;
@@ -165,6 +165,8 @@ struct ne_runtime {
     uint32_t* d_work_counter = nullptr;
     uint64_t cu_count = 256;
     uint32_t tuned_waves = 0;           // 0: the idle-slot heuristic picks the workgroup size
+    uint32_t tuned_rounds = 1;          // rounds of `waves` tiles one work item plays
+    uint32_t tuned_wg_per_cu = 0;       // persistent workgroups per CU (0: as many as fit)
 
     // captured generation steps (ne_set_step_graph): one graph per buffer parity, the evaluation's host
     // preparation cached while its inputs stay the same
@@ -555,8 +557,11 @@ static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32
     if (rt->tuned_waves != 0) waves = rt->tuned_waves;  // the autotuner's choice (ne_autotune_evaluation)
     // Work list for the persistent kernel: split-major (every genome's first `waves` tiles, then the next
     // ...), genomes by descending tile count within a split, so no workgroup slot is spent on an empty
-    // split and the longest genomes start first.
-    const uint32_t splits = uint32_t((max_tiles + waves - 1) / waves);
+    // split and the longest genomes start first.  An item plays `rounds` rounds of `waves` tiles (split s of
+    // genome n: tiles s*waves + r*splits*waves + wave), so a genome's tiles spread over
+    // ceil(tiles / (rounds x waves)) items.
+    const uint32_t rounds = std::max<uint32_t>(1, rt->tuned_rounds);
+    const uint32_t splits = uint32_t((max_tiles + uint64_t(waves) * rounds - 1) / (uint64_t(waves) * rounds));
     std::vector<uint32_t> order(n);
     for (uint64_t c = 0; c < n; ++c) order[c] = uint32_t(c);
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return tiles[x] > tiles[y]; });
@@ -583,7 +588,8 @@ static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32
     // resident workgroups per CU, as the kernel's registers and LDS allow (3 waves per SIMD at 168 VGPRs
     // and 4 workgroups per CU by LDS in the FP32 form)
     const uint32_t resident = EvaluateResidentPerCu(waves, trace, bf16);
-    const uint64_t per_cu = resident != 0 ? resident : std::min<uint64_t>(12 / waves, 4);
+    uint64_t per_cu = resident != 0 ? resident : std::min<uint64_t>(12 / waves, 4);
+    if (rt->tuned_wg_per_cu != 0) per_cu = std::min<uint64_t>(per_cu, rt->tuned_wg_per_cu);
     const uint32_t blocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(items, rt->cu_count * per_cu)));
     if (rt->enc_t == nullptr && !Alloc(&rt->enc_t, rt->max_gen * kEncTStride))
         return Fail(rt, NE_DEVICE_FAILURE);
@@ -819,12 +825,17 @@ static int Widen(ne_runtime* rt, uint64_t next_a) {
     uint32_t survivors = 0;
     if (!Ok(hipMemcpy(&survivors, rt->d_survivors, sizeof(uint32_t), hipMemcpyDeviceToHost)))
         return Fail(rt, NE_DEVICE_FAILURE);
+    EventBegin(rt, 12);
     LaunchRepackCopies(rt->slab, rt->stride, new_stride, rt->move_src, survivors, uint32_t(new_slots - survivors),
                        rt->staging, rt->stream);
+    EventEnd(rt, 12);
     EventEnd(rt, 4);
     st = Finish(rt);
     if (st != NE_OK) return st;
     rt->times.repack_ms = EventMs(rt, 4);
+    rt->times.repack_copy_ms = EventMs(rt, 12);
+    rt->times.repack_survivors = survivors;
+    rt->times.repack_old_stride = rt->stride;
     rt->action_count = next_a;
     rt->stride = new_stride;
     rt->slot_count = new_slots;
@@ -1859,7 +1870,9 @@ static int TryGraphStep(ne_runtime* rt, uint32_t seed, const ne_word_counts* cou
     // slot list changes with the buffer parity
     auto eval_key = [&](uint64_t (&k)[10]) {
         const uint64_t v[10] = {rt->input_epoch, n, std::min(rt->action_count, counts->action_space_word_count),
-                                rt->eval_mode, rt->tuned_waves, uint64_t(uintptr_t(rt->slab)), rt->stride,
+                                rt->eval_mode,
+                                rt->tuned_waves | (uint64_t(rt->tuned_rounds) << 16) | (uint64_t(rt->tuned_wg_per_cu) << 32),
+                                uint64_t(uintptr_t(rt->slab)), rt->stride,
                                 rt->slot_count, rt->action_count, rt->max_gen};
         std::memcpy(k, v, sizeof(v));
     };
@@ -2228,25 +2241,80 @@ int ne_set_step_graph(ne_runtime* rt, uint32_t enabled) {
 int ne_autotune_evaluation(ne_runtime* rt, const ne_word_counts* counts, const ne_shard_release_history* history,
                            uint32_t* chosen_waves, float* timings_ms) {
     if (!rt || !counts) return Fail(rt, NE_INVALID_ARGUMENT);
-    uint32_t best = 0;
-    float best_ms = 0.0f;
-    for (uint32_t w = 1; w <= 4; ++w) {
+    // coordinate search, one knob at a time, each candidate timed on one evaluation of the current generation
+    // (results do not depend on the shape): waves per workgroup, then rounds per work item, then workgroups
+    // per CU
+    const uint32_t saved[3] = {rt->tuned_waves, rt->tuned_rounds, rt->tuned_wg_per_cu};
+    auto time_shape = [&](uint32_t w, uint32_t r, uint32_t g, float* ms) -> int {
         rt->tuned_waves = w;
+        rt->tuned_rounds = r;
+        rt->tuned_wg_per_cu = g;
         rt->times = ne_stage_times{};
         const int st = EvaluateFull(rt, counts, history);
         if (st != NE_OK) {
-            rt->tuned_waves = 0;
+            rt->tuned_waves = saved[0];
+            rt->tuned_rounds = saved[1];
+            rt->tuned_wg_per_cu = saved[2];
             return st;
         }
-        const float ms = rt->times.evaluate_ms;
+        *ms = rt->times.evaluate_ms;
+        return NE_OK;
+    };
+    uint32_t bw = 0, br = 1, bg = 0;
+    float best_ms = 0.0f;
+    for (uint32_t w = 1; w <= 4; ++w) {
+        float ms = 0.0f;
+        const int st = time_shape(w, 1, 0, &ms);
+        if (st != NE_OK) return st;
         if (timings_ms) timings_ms[w - 1] = ms;
-        if (best == 0 || ms < best_ms) {
-            best = w;
+        if (bw == 0 || ms < best_ms) {
+            bw = w;
             best_ms = ms;
         }
     }
-    rt->tuned_waves = best;
-    if (chosen_waves) *chosen_waves = best;
+    for (uint32_t r = 2; r <= 4; r *= 2) {
+        float ms = 0.0f;
+        const int st = time_shape(bw, r, 0, &ms);
+        if (st != NE_OK) return st;
+        if (ms < best_ms) {
+            br = r;
+            best_ms = ms;
+        }
+    }
+    const uint32_t resident = std::max<uint32_t>(1, EvaluateResidentPerCu(bw, rt->trace_flags != 0,
+                                                                         (rt->eval_mode & 0xFFu) == NE_EVAL_BF16_RESCORED));
+    for (uint32_t g = resident - 1; g >= 1 && g + 2 >= resident; --g) {
+        float ms = 0.0f;
+        const int st = time_shape(bw, br, g, &ms);
+        if (st != NE_OK) return st;
+        if (ms < best_ms) {
+            bg = g;
+            best_ms = ms;
+        }
+    }
+    rt->tuned_waves = bw;
+    rt->tuned_rounds = br;
+    rt->tuned_wg_per_cu = bg;
+    if (chosen_waves) *chosen_waves = bw;
+    return EvaluateFull(rt, counts, history);  // leave the generation evaluated with the chosen shape
+}
+
+int ne_get_eval_shape(ne_runtime* rt, ne_eval_shape* out) {
+    if (!rt || !out) return NE_INVALID_ARGUMENT;
+    out->waves = rt->tuned_waves;
+    out->rounds_per_item = rt->tuned_rounds;
+    out->workgroups_per_cu = rt->tuned_wg_per_cu;
+    out->reserved0 = 0;
+    return NE_OK;
+}
+
+int ne_set_eval_shape_ex(ne_runtime* rt, const ne_eval_shape* shape) {
+    if (!rt || !shape || shape->waves > EvaluateMaxWaves() || shape->rounds_per_item > 64 ||
+        shape->workgroups_per_cu > 64)
+        return Fail(rt, NE_INVALID_ARGUMENT);
+    rt->tuned_waves = shape->waves;
+    rt->tuned_rounds = std::max<uint32_t>(1, shape->rounds_per_item);
+    rt->tuned_wg_per_cu = shape->workgroups_per_cu;
     return NE_OK;
 }
 

@@ -13,8 +13,10 @@
             speedups
 
 Every run is timed in the plain FP32 form and in the BF16-screened form (which takes the same decisions);
-`--forms fp32` skips the latter; grid16 and grid128 also take the forms fp32-graph and bf16-graph (every step
-one captured hipGraph, same results), and target times the BF16-screened form both ways.
+`--forms fp32` skips the latter; grid16, grid128 and ragged also take the forms fp32-graph and bf16-graph
+(every step that needs no host decision replayed as one captured hipGraph -- in the ragged run every step
+but the widening ones -- same results), and target times the BF16-screened form both ways.  The ragged run
+autotunes the evaluation's launch shape on generation 0 and reports each widening's repack throughput.
 Usage: python run_configs.py grid16|grid128|ragged|target [--forms fp32,bf16,fp32-graph,bf16-graph]
 [--generations G] [--seeds S]
 """
@@ -34,8 +36,10 @@ pkg = ge.import_package()
 rtm, synth = pkg.runtime, pkg.synth
 
 
-def _run_generations(rt, gens, counts_for, injection_for, cfg, seed=7):
-    """Advances `gens` generations; returns per-generation (seconds, games, decisions, best, mean)."""
+def _run_generations(rt, gens, counts_for, injection_for, cfg, seed=7, repacks=None):
+    """Advances `gens` generations; returns per-generation (seconds, games, decisions, best, mean, rescored,
+    fallbacks, step kind: 0 eager / 1 replayed / 2 captured).  Widening steps append (survivors, old stride,
+    copy ms, whole repack ms) to `repacks`."""
     out = []
     for g in range(gens):
         counts = counts_for(g)
@@ -45,8 +49,11 @@ def _run_generations(rt, gens, counts_for, injection_for, cfg, seed=7):
         dt = time.perf_counter() - t0
         st = rt.stage_times()
         s = rt.summary()
+        if inj is not None and repacks is not None:
+            repacks.append((int(st.repack_survivors), int(st.repack_old_stride), float(st.repack_copy_ms),
+                            float(st.repack_ms)))
         out.append((dt, int(st.games), int(st.decisions), float(s.best_fitness), float(s.average_fitness),
-                    int(st.rescored), int(st.fallbacks)))
+                    int(st.rescored), int(st.fallbacks), int(st.step_graph)))
     return out
 
 
@@ -57,7 +64,8 @@ def _record(name, form, rows, extra):
            "game_evals_per_s": games / secs, "generations_per_s": len(rows) / secs,
            "decisions": sum(r[2] for r in rows), "best_fitness": [r[3] for r in rows],
            "mean_fitness_last": rows[-1][4], "rescored": sum(r[5] for r in rows),
-           "fallbacks": sum(r[6] for r in rows)}
+           "fallbacks": sum(r[6] for r in rows),
+           "steps_eager_replayed_captured": [sum(r[7] == k for r in rows) for k in (0, 1, 2)]}
     rec.update(extra)
     print(json.dumps(rec), flush=True)
     return rec
@@ -116,11 +124,26 @@ def ragged_config(forms, gens=50, side=64, a0=512, a_final=4739, step=512, perio
         try:
             rt.upload_catalog(catalog)
             rt.upload_secrets_csr(offsets, ids)
-            rt.set_eval_precision(rtm.EVAL_FP32 if form == "fp32" else rtm.EVAL_BF16_RESCORED)
+            rt.set_eval_precision(rtm.EVAL_FP32 if form.startswith("fp32") else rtm.EVAL_BF16_RESCORED)
             rt.bootstrap(n, seed=7, variance_scale=1.0, tail_std=0.1)
+            # the launch shape for this workload (waves, rounds per item, workgroups per CU), chosen on
+            # generation 0 by the autotuner
+            _, wave_ms, shape = rt.autotune_evaluation(rtm.word_counts(a0, a0))
+            rt.set_step_graph(form.endswith("-graph"))
+            repacks = []
             rows = _run_generations(rt, gens, lambda g: rtm.word_counts(schedule[g], schedule[g]), injection_for,
-                                    cfg)
+                                    cfg, repacks=repacks)
+            # repack throughput: algorithmic bytes 2 x survivors x old stride (each surviving parent read and
+            # written once) over the gather + scatter kernels
+            gbs = [2 * sv * stv / (ms * 1e-3) / 1e9 for sv, stv, ms, _ in repacks if ms > 0]
             recs.append(_record("ragged", form, rows, {
+                "autotuned_shape": {"waves": shape.waves, "rounds_per_item": shape.rounds_per_item,
+                                    "workgroups_per_cu": shape.workgroups_per_cu,
+                                    "waves_sweep_ms": [round(x, 3) for x in wave_ms]},
+                "repacks": [{"survivors": sv, "old_stride": stv, "copy_ms": round(ms, 3), "repack_ms": round(rm, 3),
+                             "algorithmic_GBps": round(2 * sv * stv / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+                            for sv, stv, ms, rm in repacks],
+                "repack_algorithmic_GBps_mean": float(np.mean(gbs)) if gbs else None,
                 "genomes": n, "actions": [schedule[0], schedule[gens]], "widenings": sum(
                     schedule[g + 1] != schedule[g] for g in range(gens)),
                 "secrets_per_cell": [int(np.min(np.diff(offsets))), int(np.max(np.diff(offsets))),
@@ -129,9 +152,9 @@ def ragged_config(forms, gens=50, side=64, a0=512, a_final=4739, step=512, perio
                 "final_population": int(rt.info().current_generation_size)}))
         finally:
             rt.close()
-    if len(recs) == 2:
+    if len(recs) >= 2:
         print(json.dumps({"config": "ragged", "same_best_fitness_trajectory":
-                          recs[0]["best_fitness"] == recs[1]["best_fitness"]}), flush=True)
+                          all(r["best_fitness"] == recs[0]["best_fitness"] for r in recs)}), flush=True)
 
 
 def published_config(forms, reps=5):

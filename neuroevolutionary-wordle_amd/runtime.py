@@ -91,7 +91,13 @@ class StageTimes(C.Structure):
                 ("bookkeeping_ms", C.c_float), ("prepare_ms", C.c_float), ("decisions", C.c_uint64),
                 ("encoded_turns", C.c_uint64), ("games", C.c_uint64), ("children", C.c_uint64),
                 ("rescored", C.c_uint64), ("fallbacks", C.c_uint64), ("spilled", C.c_uint64),
-                ("step_graph_ms", C.c_float), ("step_graph", C.c_uint32)]
+                ("step_graph_ms", C.c_float), ("step_graph", C.c_uint32), ("repack_survivors", C.c_uint64),
+                ("repack_old_stride", C.c_uint64), ("repack_copy_ms", C.c_float), ("reserved0", C.c_uint32)]
+
+
+class EvalShape(C.Structure):
+    _fields_ = [("waves", C.c_uint32), ("rounds_per_item", C.c_uint32), ("workgroups_per_cu", C.c_uint32),
+                ("reserved0", C.c_uint32)]
 
 
 GAME_TRACE_DTYPE = np.dtype([("guesses", "<u4", 6), ("best", "<f4", 6), ("second", "<f4", 6),
@@ -213,7 +219,7 @@ EXPORTED_SYMBOLS = [
     "ne_build_shard_csr", "ne_set_eval_precision", "ne_enable_screening", "ne_advance_generation_screened",
     "ne_fingerprint", "ne_get_probe_states", "ne_surrogate_predict", "ne_surrogate_train_step",
     "ne_surrogate_get_params", "ne_surrogate_set_params", "ne_rank_correlation", "ne_autotune_evaluation",
-    "ne_set_eval_shape", "ne_set_step_graph", "ne_create_prebreeding_checkpoint", "ne_restore_prebreeding_checkpoint",
+    "ne_set_eval_shape", "ne_set_eval_shape_ex", "ne_get_eval_shape", "ne_set_step_graph", "ne_create_prebreeding_checkpoint", "ne_restore_prebreeding_checkpoint",
     "ne_resume_generation_from_checkpoint", "ne_write_checkpoint", "ne_read_checkpoint", "ne_clone_checkpoint",
     "ne_get_checkpoint_info", "ne_get_checkpoint_generation", "ne_get_checkpoint_plan", "ne_get_checkpoint_genome",
     "ne_free_checkpoint", "ne_training_data_identity_hash", "ne_collect_fitness_telemetry",
@@ -276,6 +282,8 @@ def load_library(path=LIB_PATH):
         "ne_autotune_evaluation": (i32, [p, C.POINTER(WordCounts), C.POINTER(ShardReleaseHistory), C.POINTER(u32),
                                          p]),
         "ne_set_eval_shape": (i32, [p, u32]),
+        "ne_set_eval_shape_ex": (i32, [p, C.POINTER(EvalShape)]),
+        "ne_get_eval_shape": (i32, [p, C.POINTER(EvalShape)]),
         "ne_set_step_graph": (i32, [p, u32]),
         "ne_create_prebreeding_checkpoint": (i32, [p, u32, C.POINTER(WordCounts), C.POINTER(AssemblyConfig),
                                                    C.POINTER(PendingInjection), p, u64, u32, C.POINTER(Callbacks),
@@ -587,13 +595,22 @@ class SlabRuntime:
 
     # --- evaluation shape
     def autotune_evaluation(self, counts):
-        """Times the current generation's evaluation at 1..4 waves per workgroup and keeps the fastest;
-        returns (waves, [ms for 1..4 waves])."""
+        """Coordinate search over the evaluation's launch shape (waves 1..4, rounds per item, workgroups per
+        CU); keeps the fastest; returns (waves, [ms for 1..4 waves], the chosen EvalShape)."""
         w = C.c_uint32()
         t = np.zeros(4, np.float32)
         self._check("ne_autotune_evaluation",
                     self.lib.ne_autotune_evaluation(self.h, C.byref(counts), None, C.byref(w), _ptr(t)))
-        return w.value, t.tolist()
+        return w.value, t.tolist(), self.eval_shape()
+
+    def eval_shape(self):
+        s = EvalShape()
+        self._check("ne_get_eval_shape", self.lib.ne_get_eval_shape(self.h, C.byref(s)))
+        return s
+
+    def set_eval_shape_ex(self, waves=0, rounds_per_item=1, workgroups_per_cu=0):
+        s = EvalShape(waves=waves, rounds_per_item=rounds_per_item, workgroups_per_cu=workgroups_per_cu)
+        self._check("ne_set_eval_shape_ex", self.lib.ne_set_eval_shape_ex(self.h, C.byref(s)))
 
     def set_eval_shape(self, waves):
         self._check("ne_set_eval_shape", self.lib.ne_set_eval_shape(self.h, waves))

@@ -413,23 +413,29 @@ def test_injection_of_a_word_without_hint_grids_fails(pkg, orc):
 
 
 def test_autotuned_shapes_give_identical_results(pkg, grid16):
-    """Every workgroup shape of the persistent evaluation kernel yields the same fitness and counters; the
-    autotuner keeps the fastest and later evaluations use it."""
+    """Every launch shape of the persistent evaluation kernel -- waves per workgroup, rounds of tiles per work
+    item, workgroups per CU -- yields the same fitness and counters; the autotuner keeps the fastest and later
+    evaluations use it."""
     wl, rt = grid16
     counts = pkg.runtime.word_counts(512, 512)
     ref = None
     for w in (1, 2, 3, 4):
-        rt.set_eval_shape(w)
-        rt.evaluate(counts)
-        _, fit, evals, _ = rt.generation(0)
-        st = rt.stage_times()
-        got = (fit.view(np.uint32).copy(), evals.copy(), st.decisions, st.encoded_turns)
-        if ref is None:
-            ref = got
-        assert all(np.array_equal(a, b) for a, b in zip(got, ref)), w
-    waves, times = rt.autotune_evaluation(counts)
+        for r in (1, 2, 4):
+            for g in (0, 1, 2):
+                rt.set_eval_shape_ex(w, r, g)
+                rt.evaluate(counts)
+                _, fit, evals, _ = rt.generation(0)
+                st = rt.stage_times()
+                got = (fit.view(np.uint32).copy(), evals.copy(), st.decisions, st.encoded_turns)
+                if ref is None:
+                    ref = got
+                assert all(np.array_equal(a, b) for a, b in zip(got, ref)), (w, r, g)
+    rt.set_eval_shape_ex(0, 1, 0)
+    waves, times, shape = rt.autotune_evaluation(counts)
     assert waves in (1, 2, 3, 4) and all(t > 0 for t in times)
-    assert times[waves - 1] == min(times)
+    assert shape.waves == waves and shape.rounds_per_item in (1, 2, 4) and shape.workgroups_per_cu in range(0, 64)
+    if shape.rounds_per_item == 1 and shape.workgroups_per_cu == 0:
+        assert times[waves - 1] == min(times)
     _, fit, _, _ = rt.generation(0)
     assert np.array_equal(fit.view(np.uint32), ref[0])
-    rt.set_eval_shape(0)
+    rt.set_eval_shape_ex(0, 1, 0)
