@@ -576,6 +576,27 @@ typedef struct ne_selected_action {
  * stream: hipStream_t or NULL (own stream). */
 int ne_model_create(const void *genome_bytes, uint64_t genome_byte_count, const uint8_t *action_letters,
                     uint64_t action_count, void *stream, ne_model **out_model);
+/* Genome layouts a model may serve: the encoder's hidden width (0: one 145 -> 64 layer, this build's
+ * evaluation layout; 128: the reference's 145 -> 128 -> 64, encoder_spec.hpp:19-20) and the storage of every
+ * parameter (NE_STORAGE_FP32, or NE_STORAGE_FP16: IEEE binary16 as the reference stores its genomes,
+ * common/float16.hpp:9-13; computed in fp32 either way). */
+enum { NE_STORAGE_FP32 = 0, NE_STORAGE_FP16 = 1 };
+typedef struct ne_model_layout {
+    uint32_t encoder_hidden;             /* 0 or 128 */
+    uint32_t storage;                    /* NE_STORAGE_* */
+    uint32_t reserved[2];
+} ne_model_layout;
+/* The genome bytes of a layout at action_count actions (0 for an invalid layout): FP16, the reference's
+ * ComputeDynamicGenomeStrideBytes (dynamic_layout.hpp:15-35): 2 x (parameters + 38 x action_count); FP32,
+ * this build's slot stride: 4 x (parameters + 38 x action_count) rounded up to 256 bytes.  Parameters:
+ * 132,928 (145 -> 64 encoder) or 150,528 (145 -> 128 -> 64). */
+uint64_t ne_model_genome_bytes(const ne_model_layout *layout, uint64_t action_count);
+/* TryCreateSingleModelDeviceRuntime (single_model_device_runtime.hpp:38-40) for a genome of any layout
+ * above -- with {128, NE_STORAGE_FP16} the reference's own genome bytes; genome_byte_count must be
+ * ne_model_genome_bytes(layout, action_count). */
+int ne_model_create_with_layout(const void *genome_bytes, uint64_t genome_byte_count, const ne_model_layout *layout,
+                                const uint8_t *action_letters, uint64_t action_count, void *stream,
+                                ne_model **out_model);
 void ne_model_destroy(ne_model *model);
 /* TrySelectNextGuessWithSingleModelDeviceRuntime (single_model_device_runtime.hpp:44-47): the policy
  * forward on the board, the masked argmax over the actions (a guessed word, any copy of it, is never

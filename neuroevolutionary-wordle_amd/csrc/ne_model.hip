@@ -5,6 +5,7 @@
 // separate multiply and add), and each action's score is summed fixed features first, then tail values,
 // so the guesses and scores are those of the reference's fp32 policy bit for bit
 // (src/inference/dynamic_policy.hpp:62-150).
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -24,12 +25,42 @@ __device__ inline bool Better(float s, uint32_t i, float best, uint32_t bi) {
     return s > best || (s == best && i < bi);
 }
 
+// Parameter offsets (elements) of a genome layout in the reference's order (policy_model.hpp:9-12,
+// shared_encoder.hpp:16-30, dense_trunk.hpp:25-40): the encoder (145 -> 64, or 145 -> kHidden -> 64), the
+// trunk 321 -> 256 -> 128 -> 64, each layer's weights [out][in] then its biases, then the tail rows.
+template <int kHidden>
+struct ModelLayout {
+    static constexpr uint64_t kEncOut1 = kHidden == 0 ? uint64_t(kEncOut) : uint64_t(kHidden);
+    static constexpr uint64_t kEncW1 = 0, kEncB1 = kEncOut1 * kTurnFeatures;
+    static constexpr uint64_t kEncW2 = kEncB1 + kEncOut1;  // hidden layouts only
+    static constexpr uint64_t kEncB2 = kEncW2 + (kHidden == 0 ? 0 : uint64_t(kEncOut) * kHidden);
+    static constexpr uint64_t kW0 = kHidden == 0 ? kEncW2 : kEncB2 + kEncOut;
+    static constexpr uint64_t kB0 = kW0 + uint64_t(kH0) * kModelIn, kW1 = kB0 + kH0;
+    static constexpr uint64_t kB1 = kW1 + uint64_t(kH1) * kH0, kW2 = kB1 + kH1;
+    static constexpr uint64_t kB2 = kW2 + uint64_t(kPol) * kH1, kP = kB2 + kPol;
+};
+static_assert(ModelLayout<0>::kP == kPolicyParams && ModelLayout<0>::kW0 == kOffW0, "145->64 layout");
+static_assert(ModelLayout<128>::kP == 150528, "the reference's 145->128->64 layout");
+
+// A parameter as fp32: stored fp32, or stored IEEE binary16 (common/float16.hpp:9-13), widened exactly.
+template <bool kFp16>
+__device__ __forceinline__ float Param(const void* g, uint64_t i) {
+    if constexpr (kFp16)
+        return __half2float(static_cast<const __half*>(g)[i]);
+    else
+        return static_cast<const float*>(g)[i];
+}
+
 // One board per workgroup: validity, encoder (per occupied turn), trunk 321 -> 256 -> 128 -> 64, then the
-// masked argmax with the runner-up score over the first A actions.
-__global__ __launch_bounds__(kInferThreads) void SelectGuessKernel(const float* genome, const uint8_t* words,
+// masked argmax with the runner-up score over the first A actions.  kHidden: the encoder's hidden width (0:
+// one 145 -> 64 layer); kFp16: parameters stored as binary16.
+template <int kHidden, bool kFp16>
+__global__ __launch_bounds__(kInferThreads) void SelectGuessKernel(const void* genome, const uint8_t* words,
                                                                    uint32_t A, const ne_board* boards,
                                                                    ne_selected_action* out) {
+    using L = ModelLayout<kHidden>;
     __shared__ float s_x[kModelIn];
+    __shared__ float s_hid[kHidden > 0 ? kMaxTurns * kHidden : 1];
     __shared__ float s_h0[kH0];
     __shared__ float s_h1[kH1];
     __shared__ float s_pol[kPol];
@@ -71,36 +102,51 @@ __global__ __launch_bounds__(kInferThreads) void SelectGuessKernel(const float* 
         }
         return;
     }
-    // the shared encoder on each occupied turn (one-hot of 5 letters and 5 tiles, no activation)
-    for (uint32_t idx = tid; idx < tc * uint32_t(kEncOut); idx += kInferThreads) {
-        const uint32_t t = idx / kEncOut, u = idx % kEncOut;
-        const float* w = genome + kOffEncW + uint64_t(u) * kTurnFeatures;
-        float sum = genome[kOffEncB + u];
+    // the shared encoder on each occupied turn (shared_encoder.hpp:94-101): a one-hot of 5 letters and 5
+    // tiles through one linear layer (145 -> 64), or through kHidden ReLU units and a linear layer to 64
+    constexpr uint32_t kFirst = uint32_t(L::kEncOut1);
+    for (uint32_t idx = tid; idx < tc * kFirst; idx += kInferThreads) {
+        const uint32_t t = idx / kFirst, u = idx % kFirst;
+        const uint64_t w = L::kEncW1 + uint64_t(u) * kTurnFeatures;
+        float sum = Param<kFp16>(genome, L::kEncB1 + u);
         for (int k = 0; k < kTurnFeatures; ++k) {
             const bool on = k < 130 ? s_letters[t][k / kAlpha] == k % kAlpha : s_tiles[t][(k - 130) / 3] == (k - 130) % 3;
-            sum += w[k] * (on ? 1.0f : 0.0f);
+            sum += Param<kFp16>(genome, w + k) * (on ? 1.0f : 0.0f);
         }
-        s_x[1 + t * kEncOut + u] = sum;
+        if constexpr (kHidden == 0)
+            s_x[1 + t * kEncOut + u] = sum;
+        else
+            s_hid[t * kHidden + u] = sum < 0.0f ? 0.0f : sum;
     }
     __syncthreads();
+    if constexpr (kHidden > 0) {
+        for (uint32_t idx = tid; idx < tc * uint32_t(kEncOut); idx += kInferThreads) {
+            const uint32_t t = idx / kEncOut, u = idx % kEncOut;
+            const uint64_t w = L::kEncW2 + uint64_t(u) * kHidden;
+            float sum = Param<kFp16>(genome, L::kEncB2 + u);
+            for (int h = 0; h < kHidden; ++h) sum += Param<kFp16>(genome, w + h) * s_hid[t * kHidden + h];
+            s_x[1 + t * kEncOut + u] = sum;
+        }
+        __syncthreads();
+    }
     {  // 321 -> 256, ReLU
-        const float* w = genome + kOffW0 + uint64_t(tid) * kModelIn;
-        float sum = genome[kOffB0 + tid];
-        for (int k = 0; k < kModelIn; ++k) sum += w[k] * s_x[k];
+        const uint64_t w = L::kW0 + uint64_t(tid) * kModelIn;
+        float sum = Param<kFp16>(genome, L::kB0 + tid);
+        for (int k = 0; k < kModelIn; ++k) sum += Param<kFp16>(genome, w + k) * s_x[k];
         s_h0[tid] = sum < 0.0f ? 0.0f : sum;
     }
     __syncthreads();
     if (tid < uint32_t(kH1)) {  // 256 -> 128, ReLU
-        const float* w = genome + kOffW1 + uint64_t(tid) * kH0;
-        float sum = genome[kOffB1 + tid];
-        for (int k = 0; k < kH0; ++k) sum += w[k] * s_h0[k];
+        const uint64_t w = L::kW1 + uint64_t(tid) * kH0;
+        float sum = Param<kFp16>(genome, L::kB1 + tid);
+        for (int k = 0; k < kH0; ++k) sum += Param<kFp16>(genome, w + k) * s_h0[k];
         s_h1[tid] = sum < 0.0f ? 0.0f : sum;
     }
     __syncthreads();
     if (tid < uint32_t(kPol)) {  // 128 -> 64, the policy vector
-        const float* w = genome + kOffW2 + uint64_t(tid) * kH1;
-        float sum = genome[kOffB2 + tid];
-        for (int k = 0; k < kH1; ++k) sum += w[k] * s_h1[k];
+        const uint64_t w = L::kW2 + uint64_t(tid) * kH1;
+        float sum = Param<kFp16>(genome, L::kB2 + tid);
+        for (int k = 0; k < kH1; ++k) sum += Param<kFp16>(genome, w + k) * s_h1[k];
         s_pol[tid] = sum;
     }
     __syncthreads();
@@ -108,7 +154,6 @@ __global__ __launch_bounds__(kInferThreads) void SelectGuessKernel(const float* 
     bool have = false, have2 = false;
     float best = 0.0f, second = 0.0f;
     uint32_t bi = 0;
-    const float* tails = genome + kPolicyParams;
     for (uint32_t a = tid; a < A; a += kInferThreads) {
         const uint8_t* w = words + 5 * uint64_t(a);
         const uint8_t w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4];
@@ -122,8 +167,8 @@ __global__ __launch_bounds__(kInferThreads) void SelectGuessKernel(const float* 
             const int c = (w0 == l) + (w1 == l) + (w2 == l) + (w3 == l) + (w4 == l);
             s += s_pol[l] * (c == 0 ? -1.0f : float(c));
         }
-        const float* tail = tails + uint64_t(a) * kTail;
-        for (int j = 0; j < kTail; ++j) s += s_pol[kFixed + j] * tail[j];
+        const uint64_t tail = L::kP + uint64_t(a) * kTail;
+        for (int j = 0; j < kTail; ++j) s += s_pol[kFixed + j] * Param<kFp16>(genome, tail + j);
         if (!have || Better(s, a, best, bi)) {
             if (have && (!have2 || best > second)) second = best;
             have2 = have2 || have;
@@ -216,12 +261,39 @@ int ScratchAddress(int device, void** out) {
     return NE_OK;
 }
 
+// The kernel of a layout.
+void LaunchSelectGuess(uint32_t enc_hidden, bool fp16, uint32_t count, hipStream_t stream, const void* genome,
+                       const uint8_t* words, uint32_t A, const ne_board* boards, ne_selected_action* out) {
+    const dim3 grid(count), block(kInferThreads);
+    if (enc_hidden == 0 && !fp16)
+        hipLaunchKernelGGL((SelectGuessKernel<0, false>), grid, block, 0, stream, genome, words, A, boards, out);
+    else if (enc_hidden == 0)
+        hipLaunchKernelGGL((SelectGuessKernel<0, true>), grid, block, 0, stream, genome, words, A, boards, out);
+    else if (!fp16)
+        hipLaunchKernelGGL((SelectGuessKernel<128, false>), grid, block, 0, stream, genome, words, A, boards, out);
+    else
+        hipLaunchKernelGGL((SelectGuessKernel<128, true>), grid, block, 0, stream, genome, words, A, boards, out);
+}
+
+uint64_t LayoutParams(uint32_t enc_hidden) { return enc_hidden == 0 ? ModelLayout<0>::kP : ModelLayout<128>::kP; }
+
 }  // namespace
+
+extern "C" uint64_t ne_model_genome_bytes(const ne_model_layout* layout, uint64_t action_count) {
+    if (!layout || (layout->encoder_hidden != 0 && layout->encoder_hidden != 128) ||
+        (layout->storage != NE_STORAGE_FP32 && layout->storage != NE_STORAGE_FP16) || action_count == 0)
+        return 0;
+    const uint64_t values = LayoutParams(layout->encoder_hidden) + uint64_t(kTail) * action_count;
+    if (layout->storage == NE_STORAGE_FP16) return 2 * values;  // ComputeDynamicGenomeStrideBytes (2-byte alignment)
+    return (4 * values + 255) / 256 * 256;                       // this build's fp32 slot stride
+}
 
 struct ne_model {
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    float* genome = nullptr;
+    uint32_t enc_hidden = 0;
+    bool fp16 = false;
+    void* genome = nullptr;
     uint8_t* words = nullptr;
     uint64_t action_count = 0;
     ne_board* boards = nullptr;
@@ -232,17 +304,22 @@ struct ne_model {
 
 extern "C" {
 
-int ne_model_create(const void* genome_bytes, uint64_t genome_byte_count, const uint8_t* action_letters,
-                    uint64_t action_count, void* stream, ne_model** out_model) {
+int ne_model_create_with_layout(const void* genome_bytes, uint64_t genome_byte_count, const ne_model_layout* layout,
+                                const uint8_t* action_letters, uint64_t action_count, void* stream,
+                                ne_model** out_model) {
     if (!genome_bytes || !action_letters || !out_model || action_count == 0 || action_count > 0xFFFFFFFFull)
         return NE_INVALID_ARGUMENT;
     *out_model = nullptr;
-    // IsValidCreateInputs (single_model_device_runtime.cu:24-29) on this build's slot stride
-    if (genome_byte_count != SlotStrideBytes(action_count)) return NE_INVALID_ARGUMENT;
+    // IsValidCreateInputs (single_model_device_runtime.cu:24-29): the genome is exactly one genome of the
+    // layout for action_count actions
+    const uint64_t expect = ne_model_genome_bytes(layout, action_count);
+    if (expect == 0 || genome_byte_count != expect) return NE_INVALID_ARGUMENT;
     for (uint64_t i = 0; i < 5 * action_count; ++i)
         if (action_letters[i] >= kAlpha) return NE_INVALID_ARGUMENT;
     ne_model* m = new (std::nothrow) ne_model();
     if (!m) return NE_DEVICE_FAILURE;
+    m->enc_hidden = layout->encoder_hidden;
+    m->fp16 = layout->storage == NE_STORAGE_FP16;
     bool ok = true;
     if (stream) {
         m->stream = reinterpret_cast<hipStream_t>(stream);
@@ -266,6 +343,13 @@ int ne_model_create(const void* genome_bytes, uint64_t genome_byte_count, const 
     }
     *out_model = m;
     return NE_OK;
+}
+
+int ne_model_create(const void* genome_bytes, uint64_t genome_byte_count, const uint8_t* action_letters,
+                    uint64_t action_count, void* stream, ne_model** out_model) {
+    const ne_model_layout fp32_layout{0, NE_STORAGE_FP32, {0, 0}};  // this build's slot: 145 -> 64, fp32
+    return ne_model_create_with_layout(genome_bytes, genome_byte_count, &fp32_layout, action_letters, action_count,
+                                       stream, out_model);
 }
 
 void ne_model_destroy(ne_model* m) {
@@ -295,8 +379,8 @@ int ne_model_select_guesses(ne_model* m, const ne_board* boards, uint64_t count,
     bool ok = hipMemcpyAsync(m->boards, boards, count * sizeof(ne_board), hipMemcpyHostToDevice, m->stream) ==
               hipSuccess;
     if (ok) {
-        hipLaunchKernelGGL(SelectGuessKernel, dim3(uint32_t(count)), dim3(kInferThreads), 0, m->stream, m->genome,
-                           m->words, uint32_t(m->action_count), m->boards, m->results);
+        LaunchSelectGuess(m->enc_hidden, m->fp16, uint32_t(count), m->stream, m->genome, m->words,
+                          uint32_t(m->action_count), m->boards, m->results);
         ok = hipGetLastError() == hipSuccess;
     }
     ok = ok && hipMemcpyAsync(out, m->results, count * sizeof(ne_selected_action), hipMemcpyDeviceToHost,
@@ -318,8 +402,8 @@ int ne_model_select_guesses_noalloc(ne_model* m, const ne_board* boards, uint64_
     bool ok = hipMemcpyAsync(d_boards, boards, count * sizeof(ne_board), hipMemcpyHostToDevice, m->stream) ==
               hipSuccess;
     if (ok) {
-        hipLaunchKernelGGL(SelectGuessKernel, dim3(uint32_t(count)), dim3(kInferThreads), 0, m->stream, m->genome,
-                           m->words, uint32_t(m->action_count), d_boards, d_results);
+        LaunchSelectGuess(m->enc_hidden, m->fp16, uint32_t(count), m->stream, m->genome, m->words,
+                          uint32_t(m->action_count), d_boards, d_results);
         ok = hipGetLastError() == hipSuccess;
     }
     ok = ok && hipMemcpyAsync(out, d_results, count * sizeof(ne_selected_action), hipMemcpyDeviceToHost,

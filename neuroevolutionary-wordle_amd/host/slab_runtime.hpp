@@ -646,6 +646,21 @@ inline bool TryCreateSingleModelDeviceRuntime(const std::uint8_t* host_genome_by
     return true;
 }
 
+// The same for a genome of another layout, e.g. the reference's own bytes ({128, NE_STORAGE_FP16}: the
+// 145 -> 128 -> 64 encoder stored as binary16, ComputeDynamicGenomeStrideBytes(action_count) bytes).
+inline bool TryCreateSingleModelDeviceRuntime(const std::uint8_t* host_genome_bytes, std::size_t genome_byte_count,
+                                              const ne_model_layout& layout,
+                                              const std::vector<std::uint8_t>& action_letters,
+                                              SingleModelDeviceRuntime& runtime_out) {
+    runtime_out = {};
+    ne_model* m = nullptr;
+    if (ne_model_create_with_layout(host_genome_bytes, genome_byte_count, &layout, action_letters.data(),
+                                    action_letters.size() / 5, nullptr, &m) != NE_OK)
+        return false;
+    runtime_out = {m, action_letters.size() / 5};
+    return true;
+}
+
 inline void DestroySingleModelDeviceRuntime(SingleModelDeviceRuntime& runtime) noexcept {
     ne_model_destroy(runtime.handle);
     runtime = {};

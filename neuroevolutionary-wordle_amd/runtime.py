@@ -224,7 +224,7 @@ EXPORTED_SYMBOLS = [
     "ne_get_checkpoint_info", "ne_get_checkpoint_generation", "ne_get_checkpoint_plan", "ne_get_checkpoint_genome",
     "ne_free_checkpoint", "ne_training_data_identity_hash", "ne_collect_fitness_telemetry",
     "ne_collect_convergence_telemetry", "ne_convergence_sample_plan", "ne_maybe_release_training_shard",
-    "ne_model_create", "ne_model_destroy", "ne_model_select_guess", "ne_model_select_guesses",
+    "ne_model_create", "ne_model_create_with_layout", "ne_model_genome_bytes", "ne_model_destroy", "ne_model_select_guess", "ne_model_select_guesses",
     "ne_model_select_guesses_noalloc",
     "ne_write_winner_artifact", "ne_write_winner_artifact_in_directory", "ne_read_winner_artifact",
 ]
@@ -308,6 +308,8 @@ def load_library(path=LIB_PATH):
         "ne_model_create": (i32, [p, u64, p, u64, p, C.POINTER(p)]),
         "ne_model_destroy": (None, [p]),
         "ne_model_select_guess": (i32, [p, C.POINTER(Board), C.POINTER(SelectedAction)]),
+        "ne_model_create_with_layout": (i32, [p, u64, C.POINTER(ModelLayout), p, u64, p, C.POINTER(p)]),
+        "ne_model_genome_bytes": (u64, [C.POINTER(ModelLayout), u64]),
         "ne_model_select_guesses": (i32, [p, p, u64, p]),
         "ne_model_select_guesses_noalloc": (i32, [p, p, u64, p]),
         "ne_write_winner_artifact": (i32, [C.c_char_p, C.c_char_p, p, p, u64, C.POINTER(WinnerMetadata)]),
@@ -793,18 +795,35 @@ class ReleaseHistory:
 
 
 NOALLOC_MAX_BOARDS = 8192  # NE_NOALLOC_MAX_BOARDS
+STORAGE_FP32, STORAGE_FP16 = 0, 1
+
+
+class ModelLayout(C.Structure):
+    _fields_ = [("encoder_hidden", C.c_uint32), ("storage", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+
+
+def model_genome_bytes(encoder_hidden, storage, action_count):
+    lay = ModelLayout(encoder_hidden=encoder_hidden, storage=storage)
+    return int(load_library().ne_model_genome_bytes(C.byref(lay), action_count))
 
 
 class Model:
     """TryCreateSingleModelDeviceRuntime ... DestroySingleModelDeviceRuntime: one genome serving guesses."""
 
-    def __init__(self, genome, action_letters):
+    def __init__(self, genome, action_letters, encoder_hidden=None, storage=None):
+        """genome: this build's fp32 slot bytes (145 -> 64 encoder), or -- with encoder_hidden and storage
+        given -- the bytes of that layout (e.g. 128, STORAGE_FP16: the reference's own genome)."""
         self.lib = load_library()
         self.h = None
         g = np.ascontiguousarray(genome).view(np.uint8)
         w = np.ascontiguousarray(action_letters, dtype=np.uint8).reshape(-1, 5)
         h = C.c_void_p()
-        st = self.lib.ne_model_create(_ptr(g), g.size, _ptr(w), w.shape[0], None, C.byref(h))
+        if encoder_hidden is None and storage is None:
+            st = self.lib.ne_model_create(_ptr(g), g.size, _ptr(w), w.shape[0], None, C.byref(h))
+        else:
+            lay = ModelLayout(encoder_hidden=encoder_hidden or 0, storage=storage or STORAGE_FP32)
+            st = self.lib.ne_model_create_with_layout(_ptr(g), g.size, C.byref(lay), _ptr(w), w.shape[0], None,
+                                                      C.byref(h))
         if st != 0:
             raise RuntimeError_("ne_model_create", st)
         self.h = h

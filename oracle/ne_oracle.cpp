@@ -710,13 +710,13 @@ void oro_fixed_features(const uint8_t* word, float* out) { FixedFeatures(word, o
 // single_model_device_runtime.cu:122-181; dynamic_policy.hpp:62-150): the policy forward and the masked
 // argmax over the first A catalog words, in fp32 and in fp64.  Returns kPolicyForwardFailed for an
 // invalid or finished board, kActionSelectionFailed when every action was guessed.
-int oro_select_guess(const float* genome, uint64_t A, const uint8_t* catalog, const uint8_t* letters,
-                     const uint8_t* tiles, int tc, uint64_t* index32, float* best32, float* second32,
-                     uint64_t* index64, double* best64, double* second64) {
+int oro_select_guess_layout(const float* genome, int enc_hidden, uint64_t A, const uint8_t* catalog,
+                            const uint8_t* letters, const uint8_t* tiles, int tc, uint64_t* index32, float* best32,
+                            float* second32, uint64_t* index64, double* best64, double* second64) {
     if (tc < 0 || tc > kMaxTurns) return kPolicyForwardFailed;
     for (int i = 0; i < 5 * tc; ++i)
         if (letters[i] >= kAlpha || tiles[i] > 2) return kPolicyForwardFailed;
-    const Layout L = MakeLayout(0);
+    const Layout L = MakeLayout(enc_hidden);
     const Grid g = BoardGrid(letters, tiles, tc);
     float pol32[kPol];
     double pol64[kPol];
@@ -729,6 +729,13 @@ int oro_select_guess(const float* genome, uint64_t A, const uint8_t* catalog, co
         !SelectAction<double>(pol64, tails, catalog, fixed.data(), A, g, index64, best64, second64))
         return kActionSelectionFailed;
     return kOk;
+}
+
+int oro_select_guess(const float* genome, uint64_t A, const uint8_t* catalog, const uint8_t* letters,
+                     const uint8_t* tiles, int tc, uint64_t* index32, float* best32, float* second32,
+                     uint64_t* index64, double* best64, double* second64) {
+    return oro_select_guess_layout(genome, 0, A, catalog, letters, tiles, tc, index32, best32, second32, index64,
+                                   best64, second64);
 }
 
 // One turn through the shared encoder (shared_encoder.hpp:186-207).

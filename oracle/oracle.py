@@ -29,6 +29,7 @@ _SIGS = {
     "oro_policy_forward": (I32, [P, I32, P, P, I32, P, P]),
     "oro_fixed_features": (None, [P, P]),
     "oro_select_guess": (I32, [P, U64, P, P, P, I32, P, P, P, P, P, P]),
+    "oro_select_guess_layout": (I32, [P, I32, U64, P, P, P, I32, P, P, P, P, P, P]),
     "oro_encode_turn": (None, [P, I32, P, P, P, P]),
     "oro_score_action": (F32, [P, P, P]),
     "oro_evaluate": (I32, [P, U64, I32, U64, U64, P, U64, P, P, I32, I32, P, P, P, P, P, P]),
@@ -147,9 +148,9 @@ def policy_forward(genome, turns, enc_hidden=0):
     return o32, o64
 
 
-def select_guess(genome, A, catalog, turns):
+def select_guess(genome, A, catalog, turns, enc_hidden=0):
     """The next guess for a board of (word letters[5], tiles[5]) turns: dict with status, the fp32 choice
-    (index, best, second) and the fp64 one."""
+    (index, best, second) and the fp64 one.  genome: fp32 values in the layout of `enc_hidden`."""
     g = np.ascontiguousarray(genome, np.float32)
     cat = np.ascontiguousarray(catalog, np.uint8)
     tc = len(turns)
@@ -161,8 +162,8 @@ def select_guess(genome, A, catalog, turns):
     i32, i64 = C.c_uint64(), C.c_uint64()
     b32, s32 = C.c_float(), C.c_float()
     b64, s64 = C.c_double(), C.c_double()
-    st = lib().oro_select_guess(ptr(g), A, ptr(cat), ptr(letters), ptr(tiles), tc, C.byref(i32), C.byref(b32),
-                                C.byref(s32), C.byref(i64), C.byref(b64), C.byref(s64))
+    st = lib().oro_select_guess_layout(ptr(g), enc_hidden, A, ptr(cat), ptr(letters), ptr(tiles), tc, C.byref(i32),
+                                       C.byref(b32), C.byref(s32), C.byref(i64), C.byref(b64), C.byref(s64))
     return dict(status=st, index=i32.value, best=np.float32(b32.value), second=np.float32(s32.value),
                 index64=i64.value, best64=b64.value, second64=s64.value)
 
