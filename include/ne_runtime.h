@@ -97,7 +97,7 @@ typedef struct ne_assembly_config {
     float reserved0;
 } ne_assembly_config;
 
-/* PendingOutputEmbeddingInjection (genotype_slab/device_runtime.hpp:352-356). */
+/* PendingOutputEmbeddingInjection (genotype_slab/device_runtime.hpp:26-32). */
 typedef struct ne_pending_injection {
     uint32_t enabled;
     uint32_t reserved0;

@@ -10,7 +10,7 @@ namespace ne {
 
 // ---- model sizes (reference src/model/input_encoder/encoder_spec.hpp:10-20,
 // model_input/model_input_spec.hpp:11-15, dense_trunk/dense_trunk.hpp:15-18,
-// output_embedding/output_embedding.hpp:151-153)
+// output_embedding/output_embedding.hpp:19-21)
 constexpr int kAlpha = 26;
 constexpr int kTurnFeatures = 145;
 constexpr int kEncOut = 64;
@@ -26,7 +26,7 @@ constexpr float kFitnessFloor = 1.40129846e-45f;  // reference src/spatial/grid.
 
 // Genome = the reference PolicyModelParameters order (policy_model.hpp:9-12) with each dense
 // layer row-major [out][in] followed by its biases, then A tail rows of 38 values
-// (genome/dynamic_layout.hpp:179-224).  Values are fp32.  Offsets in floats, 145->64 encoder.
+// (genome/dynamic_layout.hpp:15-71).  Values are fp32.  Offsets in floats, 145->64 encoder.
 constexpr uint64_t kOffEncW = 0;                                   // [64][145]
 constexpr uint64_t kOffEncB = kOffEncW + uint64_t(kEncOut) * kTurnFeatures;  // [64]
 constexpr uint64_t kOffW0 = kOffEncB + kEncOut;                    // [256][321]
@@ -51,7 +51,7 @@ __host__ __device__ constexpr uint64_t SlotStrideBytes(uint64_t action_count) {
 
 // ---- scoring layout ----------------------------------------------------------------------------
 // An action's score is the 64-term dot product of its embedding row [26 fixed | 38 tail] with the
-// policy vector (output_embedding.hpp:207-225).  On v_mfma_f32_16x16x4_f32 it takes 17 k-steps per
+// policy vector (output_embedding.hpp:97-119).  On v_mfma_f32_16x16x4_f32 it takes 17 k-steps per
 // tile of 16 actions: 7 over the fixed features and 10 over the tail, so that every k-step reads a
 // single source.  Lane group g supplies, at step s, policy dim ScoreDim(s, g) (-1: a zero pad).
 constexpr int kFixedSteps = 7;

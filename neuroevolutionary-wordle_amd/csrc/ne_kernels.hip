@@ -45,7 +45,7 @@ __device__ __forceinline__ void WaitChunk() { __builtin_amdgcn_s_waitcnt(0x0F70)
 // Register-only Wordle feedback (no per-letter count array): a non-green position p of letter L is
 // yellow iff the number of earlier non-green positions of letter L is below the number of
 // non-green solution positions holding L -- the same result as the reference's two passes
-// (src/wordle/feedback.hpp:170-203).
+// (src/wordle/feedback.hpp:103-136).
 __device__ inline uint32_t FeedbackRegs(uint32_t guess, uint32_t sol) {
     uint32_t g[5], s[5];
     bool green[5];
@@ -109,7 +109,7 @@ __device__ __forceinline__ void AddVirgin(v4f (&acc)[16], const TrunkLane& tl, b
 }
 
 // Encoder 145->64 on one turn from the LDS copy (transposed [feature][out]): dims 16gq..16gq+15.
-// Features in increasing index order, 5 letters then 5 tiles (turn_features.hpp:60-81), added to
+// Features in increasing index order, 5 letters then 5 tiles (turn_features.hpp:28-49), added to
 // the bias one by one -- the reference's sequential sum over a one-hot input.
 __device__ __forceinline__ void EncodeTurnLds(const float* s_enc, const float* s_encb, uint32_t code, uint32_t fb,
                                               uint32_t gq, float (&e)[16]) {
@@ -1198,7 +1198,7 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
                         if (d >= 0) dst[d] = pol[s];
                     }
                 }
-                // ---- apply the guess (wordle_grid.hpp:308-316)
+                // ---- apply the guess (wordle_grid.hpp:69-77)
                 if (active) {
                     if (besta == kInvalidSlot) {
                         if (gq == 0) SetStatus(p.status, 8);  // kActionSelectionFailed

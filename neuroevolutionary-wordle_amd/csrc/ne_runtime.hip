@@ -282,7 +282,7 @@ uint32_t HostFreeCount(ne_runtime* rt) {
 }
 
 uint64_t SlotCountForBudget(uint64_t budget, uint64_t action_count, uint64_t ceiling) {
-    // PopulationSizeForGenotypeBudgetBytes (genome/dynamic_layout.hpp:191-205) on this build's stride.
+    // PopulationSizeForGenotypeBudgetBytes (genome/dynamic_layout.hpp:37-52) on this build's stride.
     const uint64_t stride = SlotStrideBytes(action_count);
     if (stride == 0 || budget < stride) return 0;
     const uint64_t n = budget / stride;
@@ -1578,7 +1578,7 @@ int ne_upload_catalog(ne_runtime* rt, const uint8_t* letters, uint64_t word_coun
         }
         codes[a] = code;
         canon[a] = first_index.emplace(code, uint32_t(a)).first->second;
-        // MaterializeFixedWordFeatures (output_embedding.hpp:179-205): count, or -1 when absent
+        // MaterializeFixedWordFeatures (output_embedding.hpp:59-73): count, or -1 when absent
         for (int l = 0; l < kAlpha; ++l) fixed[a * kFixed + l] = counts[l] == 0 ? -1.0f : float(counts[l]);
     }
     float absmax[kFixed] = {0.0f};

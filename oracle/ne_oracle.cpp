@@ -29,7 +29,7 @@ namespace {
 // ------------------------------------------------------------------------------------------------
 // Sizes (reference: src/model/input_encoder/encoder_spec.hpp:10-20,
 // src/model/model_input/model_input_spec.hpp:11-15, src/model/dense_trunk/dense_trunk.hpp:15-18,
-// src/model/output_embedding/output_embedding.hpp:151-153)
+// src/model/output_embedding/output_embedding.hpp:19-21)
 constexpr int kWordLen = 5;
 constexpr int kAlpha = 26;
 constexpr int kTurnFeatures = 145;  // 5*26 letters + 5*3 tiles
@@ -50,7 +50,7 @@ enum Status : int {
 // Genome layout in float elements: the reference order of PolicyModelParameters
 // (src/model/policy_model/policy_model.hpp:9-12, shared_encoder.hpp:108-122, dense_trunk.hpp:25-40),
 // each layer row-major [out][in] then biases, followed by A rows of 38 tail values
-// (src/genetic_algorithm/genome/dynamic_layout.hpp:179-224).  Stored as fp32 here.
+// (src/genetic_algorithm/genome/dynamic_layout.hpp:15-71).  Stored as fp32 here.
 struct Layout {
     int enc_hidden = 0;                      // 0: 145->64 encoder ; 128: 145->128->64 encoder
     size_t enc_w1 = 0, enc_b1 = 0, enc_w2 = 0, enc_b2 = 0;
@@ -217,7 +217,7 @@ float NormalFromWords(uint32_t a, uint32_t b) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// Wordle rules (reference src/wordle/feedback.hpp:170-203, wordle_grid.hpp:252-316).
+// Wordle rules (reference src/wordle/feedback.hpp:103-136, wordle_grid.hpp:13-77).
 struct Grid {
     uint8_t sol[5];
     uint8_t guess[kMaxTurns][5];
@@ -261,8 +261,8 @@ bool Append(Grid& g, const uint8_t* w) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// Policy forward, sequential order of the reference (shared_encoder.hpp:129-193,
-// wordle_grid_state.hpp:307-328, dense_trunk.hpp:44-114).  T = float or double.
+// Policy forward, sequential order of the reference (shared_encoder.hpp:38-58,94-115,
+// wordle_grid_state.hpp:26-64, dense_trunk.hpp:44-114).  T = float or double.
 template <typename T>
 void DenseApply(const float* W, const float* b, int in, int out, const T* x, T* y, bool relu) {
     for (int o = 0; o < out; ++o) {
@@ -304,7 +304,7 @@ bool PolicyForward(const float* g, const Layout& L, const Grid& grid, T* policy)
     return true;
 }
 
-// Fixed word features: letter count, or -1 when absent (output_embedding.hpp:179-205).
+// Fixed word features: letter count, or -1 when absent (output_embedding.hpp:59-73).
 void FixedFeatures(const uint8_t* w, float* f) {
     int counts[kAlpha] = {0};
     for (int p = 0; p < 5; ++p) ++counts[w[p]];
@@ -738,7 +738,7 @@ int oro_select_guess(const float* genome, uint64_t A, const uint8_t* catalog, co
                                    best64, second64);
 }
 
-// One turn through the shared encoder (shared_encoder.hpp:186-207).
+// One turn through the shared encoder (shared_encoder.hpp:94-115).
 void oro_encode_turn(const float* genome, int enc_hidden, const uint8_t* letters, const uint8_t* tiles, float* out32,
                      double* out64) {
     const Layout L = MakeLayout(enc_hidden);
