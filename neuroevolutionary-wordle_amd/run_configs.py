@@ -97,7 +97,7 @@ def plain_config(name, gens, forms):
                           all(r["best_fitness"] == recs[0]["best_fitness"] for r in recs)}), flush=True)
 
 
-def ragged_config(forms, gens=50, side=64, a0=512, a_final=4739, step=512, period=5):
+def ragged_config(forms, gens=50, side=64, a0=512, a_final=4739, step=512, period=5, autotune=True):
     n = side * side
     catalog = synth.make_catalog(a_final, seed=synth.CATALOG_SEED, distinct_letters=True)
     pool = synth.sample_secrets(a_final, 1030)
@@ -127,8 +127,11 @@ def ragged_config(forms, gens=50, side=64, a0=512, a_final=4739, step=512, perio
             rt.set_eval_precision(rtm.EVAL_FP32 if form.startswith("fp32") else rtm.EVAL_BF16_RESCORED)
             rt.bootstrap(n, seed=7, variance_scale=1.0, tail_std=0.1)
             # the launch shape for this workload (waves, rounds per item, workgroups per CU), chosen on
-            # generation 0 by the autotuner
-            _, wave_ms, shape = rt.autotune_evaluation(rtm.word_counts(a0, a0))
+            # generation 0 by the autotuner, or the host's default shape
+            if autotune:
+                _, wave_ms, shape = rt.autotune_evaluation(rtm.word_counts(a0, a0))
+            else:
+                wave_ms, shape = [], rt.eval_shape()
             rt.set_step_graph(form.endswith("-graph"))
             repacks = []
             rows = _run_generations(rt, gens, lambda g: rtm.word_counts(schedule[g], schedule[g]), injection_for,
@@ -137,6 +140,7 @@ def ragged_config(forms, gens=50, side=64, a0=512, a_final=4739, step=512, perio
             # written once) over the gather + scatter kernels
             gbs = [2 * sv * stv / (ms * 1e-3) / 1e9 for sv, stv, ms, _ in repacks if ms > 0]
             recs.append(_record("ragged", form, rows, {
+                "autotuned": autotune,
                 "autotuned_shape": {"waves": shape.waves, "rounds_per_item": shape.rounds_per_item,
                                     "workgroups_per_cu": shape.workgroups_per_cu,
                                     "waves_sweep_ms": [round(x, 3) for x in wave_ms]},
@@ -201,15 +205,15 @@ def published_config(forms, reps=5):
 TARGET_FORMS = ("bf16", "bf16-graph", "surrogate", "surrogate-rho0")
 
 
-def wallclock_to_target(seeds, gens, cap_factor=2, forms=TARGET_FORMS, log=sys.stderr):
-    """Wall-clock to target at 64x64 / 4,739 actions: per seed, the plain FP32 run's best fitness over `gens`
-    generations is the target; every other form runs until its best-so-far reaches it (at most
-    cap_factor * gens generations).  Forms: "bf16" (BF16-screened evaluation, the FP32 form's decisions),
-    "bf16-graph" (the same, every step one captured hipGraph), "surrogate" (surrogate-screened evolution,
-    brood 4x, best-predicted 25% + 5% audit, BF16-screened evaluation, plain steps while the audit's rank
-    correlation is below 0.5) and "surrogate-rho0" (the same gated at 0).  Host wall-clock per step, steps
-    only.  Returns (per-seed records, aggregate)."""
-    wl = synth.workload("grid64_a4739")
+def wallclock_to_target(seeds, gens, cap_factor=2, forms=TARGET_FORMS, log=sys.stderr, workload="grid64_a4739"):
+    """Wall-clock to target (64x64 / 4,739 actions unless `workload` names another): per seed, the plain FP32
+    run's best fitness over `gens` generations is the target; every other form runs until its best-so-far
+    reaches it (at most cap_factor * gens generations).  Forms: "bf16" (BF16-screened evaluation, the FP32
+    form's decisions), "bf16-graph" (the same, every step one captured hipGraph), "surrogate"
+    (surrogate-screened evolution, brood 4x, best-predicted 25% + 5% audit, BF16-screened evaluation in the
+    persistent kernel, plain steps while the audit's rank correlation is below 0.5) and "surrogate-rho0" (the
+    same gated at 0).  Host wall-clock per step, steps only.  Returns (per-seed records, aggregate)."""
+    wl = synth.workload(workload)
     A, side = wl["actions"], wl["side"]
     n = side * side
     stride = rtm.slot_stride_bytes(A)
@@ -256,7 +260,8 @@ def wallclock_to_target(seeds, gens, cap_factor=2, forms=TARGET_FORMS, log=sys.s
         target = trace_p[-1][1]
         t_plain = next(tt for tt, b in trace_p if b >= target)
         g_plain = next(i + 1 for i, (tt, b) in enumerate(trace_p) if b >= target)
-        rec = {"config": "target", "seed": seed, "genomes": n, "actions": A, "plain_generations": gens,
+        rec = {"config": "target", "workload": workload, "seed": seed, "genomes": n, "actions": A,
+               "plain_generations": gens,
                "target_fitness": target, "forms": {}}
         rec["forms"]["fp32"] = {"seconds_to_target": t_plain, "generations_to_target": g_plain, "reached": True,
                                 "seconds_per_generation": trace_p[-1][0] / len(trace_p)}
@@ -292,32 +297,42 @@ def wallclock_to_target(seeds, gens, cap_factor=2, forms=TARGET_FORMS, log=sys.s
     return out, agg
 
 
-def target_config(seeds, gens, cap_factor=2):
-    recs, agg = wallclock_to_target(seeds, gens, cap_factor)
+def target_config(seeds, gens, cap_factor=2, forms=TARGET_FORMS, workload="grid64_a4739"):
+    recs, agg = wallclock_to_target(seeds, gens, cap_factor, forms=forms, workload=workload)
     for r in recs:
         print(json.dumps(r), flush=True)
-    print(json.dumps(dict(agg, config="target")), flush=True)
+    print(json.dumps(dict(agg, config="target", workload=workload)), flush=True)
     return recs
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("config", choices=["grid16", "grid128", "ragged", "target", "published"])
-    ap.add_argument("--forms", default="fp32,bf16")
+    ap.add_argument("config", choices=["grid16", "grid128", "ragged", "target", "target128", "published"])
+    ap.add_argument("--forms", default="")
     ap.add_argument("--generations", type=int, default=0)
     ap.add_argument("--seeds", type=int, default=10)
+    ap.add_argument("--first-seed", type=int, default=7)
+    ap.add_argument("--no-autotune", action="store_true", help="ragged: the default launch shape")
     args = ap.parse_args()
     forms = [f for f in args.forms.split(",") if f]
+    seeds = [args.first_seed + 10 * i for i in range(args.seeds)]
+    if args.config in ("target", "target128"):
+        target_forms = tuple(forms) if forms else TARGET_FORMS
+        if args.config == "target":
+            target_config(seeds, args.generations or 200, forms=target_forms)
+        else:  # the combined form at 128x128: BF16 evaluation + persistent kernel + surrogate screening
+            target_config(seeds, args.generations or 100, forms=tuple(forms) if forms else ("bf16", "surrogate"),
+                          workload="grid128_a4739")
+        return
+    forms = forms or ["fp32", "bf16"]
     if args.config == "grid16":
         plain_config("grid16_a512", args.generations or 20, forms)
     elif args.config == "grid128":
         plain_config("grid128_a4739", args.generations or 3, forms)
     elif args.config == "published":
         published_config(forms)
-    elif args.config == "target":
-        target_config([7 + 10 * i for i in range(args.seeds)], args.generations or 200)
     else:
-        ragged_config(forms, gens=args.generations or 50)
+        ragged_config(forms, gens=args.generations or 50, autotune=not args.no_autotune)
 
 
 if __name__ == "__main__":
