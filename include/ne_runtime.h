@@ -291,12 +291,15 @@ typedef struct ne_eval_shape {
                                     over ceil(tiles / waves) items; r: over ceil(tiles / (r x waves))) */
     uint32_t workgroups_per_cu;  /* persistent workgroups per CU; 0: as many as the kernel's registers and
                                     LDS allow */
-    uint32_t reserved0;
+    uint32_t tiles_per_wave;     /* game tiles of 16 per wave: 0 or 1, or 2 -- every wave plays two tiles, so
+                                    a workgroup streams one genome's operands once per decision for 2 x waves
+                                    tiles (untraced FP32 form; other forms play one tile per wave) */
 } ne_eval_shape;
 /* Autotuner of the evaluation's launch shape (no reference counterpart): a coordinate search over the
- * three knobs of ne_eval_shape, each candidate timed with HIP events on one evaluation of the current
- * generation -- waves 1..4 (rounds 1, all workgroups), then rounds 2 and 4, then one or two workgroups per
- * CU fewer than fit -- keeping the fastest for later evaluations of this handle.  Results do not depend on
+ * four knobs of ne_eval_shape, each candidate timed with HIP events on one evaluation of the current
+ * generation -- waves 1..4 (one tile per wave, rounds 1, all workgroups), then two tiles per wave at waves
+ * 1..4 (untraced FP32 form), then rounds 2 and 4, then one or two workgroups per CU fewer than fit --
+ * keeping the fastest for later evaluations of this handle.  Results do not depend on
  * the shape; the generation is left evaluated (with the chosen shape) as by ne_evaluate.  chosen_waves
  * (optional) receives the chosen waves, timings_ms (optional) the 4 times of the waves sweep;
  * ne_get_eval_shape reports the whole choice. */

@@ -584,6 +584,24 @@ __device__ __forceinline__ void StreamChunks(uint8_t* b0, uint8_t* b1, uint32_t 
     }
 }
 
+// StreamChunks with one callback per chunk: per_chunk(buf, c) for the buffer holding chunk c.
+template <typename FI, typename FC>
+__device__ __forceinline__ void StreamChunksWhole(uint8_t* b0, uint8_t* b1, uint32_t chunks, bool compute, FI&& issue,
+                                                  FC&& per_chunk) {
+    WaitChunk();
+    __syncthreads();
+    auto one = [&](const uint8_t* buf, uint8_t* next, uint32_t c) __device__ {
+        if (c + 1 < chunks) issue(next, c + 1);
+        if (compute) per_chunk(buf, c);
+        WaitChunk();
+        __syncthreads();
+    };
+    for (uint32_t c = 0; c < chunks; c += 2) {
+        one(b0, b1, c);
+        if (c + 1 < chunks) one(b1, b0, c + 1);
+    }
+}
+
 // Masked argmax state of one lane: candidates in increasing action order, the first unmasked one
 // taken whatever its score, later ones only when strictly greater (evaluation_ops.cuh:429-445).
 struct ArgmaxLane {
@@ -775,6 +793,19 @@ __device__ __forceinline__ void ActionOperands(Rsrc img_r, Rsrc tail_r, uint32_t
     ev[kScoreSteps - 1] = gq == 3 ? 0.0f : ev[kScoreSteps - 1];
 }
 
+// Layer-0 accumulators between decisions.  They are needed only by the fold and the trunk, so they wait in
+// a global scratch area of the wave while the scan runs, leaving their 64 VGPRs to the scan's operand
+// reads.  A wave's area is [game tile q][neuron tile t][lane] v4f (16 KB per game tile), addressed through
+// a buffer resource (the wave's base) with the lane's 16-byte slot as the only per-lane offset and the
+// (q, t) part as a scalar offset, so no per-(q, t) lane addresses are kept in registers.
+constexpr uint32_t kAccTileFloats = 16 * 64 * 4;  // one tile's layer-0 accumulators: [neuron tile][lane] v4f
+__device__ __forceinline__ v4f AccLoad(Rsrc r, uint32_t lane, uint32_t q, uint32_t t) {
+    return Buf4(r, lane * 16, (q * 16 + t) * 1024);
+}
+__device__ __forceinline__ void AccStore(Rsrc r, uint32_t lane, uint32_t q, uint32_t t, const v4f& v) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, v), r, lane * 16, (q * 16 + t) * 1024, 0);
+}
+
 // BF16 screening bound.  With round-to-nearest-even conversions |bf16(x) - x| <= 2^-9 |x|, so each
 // product is off by at most 2^-8 (1 + 2^-10) |e p|; the FP32 accumulation errors of both the BF16
 // and the FP32 contraction (<= 96 and 68 roundings of 2^-24) fit in the 3% margin.  Hence
@@ -835,6 +866,8 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
         const uint32_t A = p.action_count;
         const uint32_t a_tiles = (A + 15) / 16;
         const uint32_t chunks = (a_tiles + kCT - 1) / kCT;
+        const Rsrc acc_r = MakeRsrc(p.acc_scratch + (uint64_t(blockIdx.x) * kEvalMaxWaves + wave) * 2 * kAccTileFloats,
+                                    kAccTileFloats * sizeof(float));
         const Rsrc img_r = MakeRsrc(p.img, p.img_bytes);
         const Rsrc tail_r = MakeRsrc(genome + kPolicyParams, uint32_t(p.stride - kPolicyParams * 4));
 
@@ -929,7 +962,9 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
             }
 
             const int base_tc = fresh ? 0 : 2;
-            v4f acc[16];  // layer-0 pre-activations, neuron tile t rows 4*gq+r, game column j
+            // layer-0 pre-activations, neuron tile t rows 4*gq+r, game column j; between decisions the FP32
+            // form parks them in scratch (its scan then has their registers), the BF16 form keeps them
+            v4f acc[16];
             int folded = 0;
             for (int step = 0; step < kMaxTurns; ++step) {
                 if (!__syncthreads_or(active ? 1 : 0)) break;  // the whole workgroup leaves together
@@ -938,7 +973,12 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
                 float pol[kScoreSteps];
                 if (wave_active) {
                     const int ttc = base_tc + step;  // turn count of every active game in this tile
-                    if (step == 0 || (fresh && step == 1)) {
+                    if (!(step == 0 || (fresh && step == 1))) {
+                        if constexpr (!kBf16) {
+    #pragma unroll
+                            for (int t = 0; t < 16; ++t) acc[t] = AccLoad(acc_r, lane, 0, uint32_t(t));
+                        }
+                    } else {
                         InitLayer0(acc, tl);
                         if (fresh && step == 0) {  // acc += W0[:, 0] (virgin flag), from the contiguous copy
     #pragma unroll
@@ -964,6 +1004,10 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
                     else
                         issue(b, 0);
                 });
+                if (!kBf16 && wave_active) {  // parked until the next decision's fold
+    #pragma unroll
+                    for (int t = 0; t < 16; ++t) AccStore(acc_r, lane, 0, uint32_t(t), acc[t]);
+                }
                 if (!wave_active) {
     #pragma unroll
                     for (int s = 0; s < kScoreSteps; ++s) pol[s] = 0.0f;
@@ -974,12 +1018,25 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
                 uint32_t besta = kInvalidSlot;
                 if constexpr (!kBf16) {
                     ArgmaxLane am = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot, -__builtin_inff()};
-                    StreamChunks<kCT>(s_buf0, s_buf1, chunks, wave_active, issue,
-                                      [&](const uint8_t* buf, int k, uint32_t at) __device__ {
-                                          if (!kTrace && k == 0) ShareArgmaxGate(am);
-                                          ScoreTile<kTrace>(Fp32Img<kCT>(buf, k), Fp32Tails<kCT>(buf, k), lane, j, gq, pol,
-                                                            at * 16 + 4 * gq, A, hist, am);
-                                      });
+                    // per chunk: the operands of tile k + 1 are read from LDS before tile k's contraction, so
+                    // their latency hides behind its 17 MFMAs
+                    StreamChunksWhole(s_buf0, s_buf1, chunks, wave_active, issue, [&](const uint8_t* buf, uint32_t c) __device__ {
+                        if (!kTrace) ShareArgmaxGate(am);
+                        float ev[kScoreSteps];
+                        TileOperands(Fp32Img<kCT>(buf, 0), Fp32Tails<kCT>(buf, 0), lane, j, gq, ev);
+#pragma unroll
+                        for (int k = 0; k < kCT; ++k) {
+                            float nx[kScoreSteps];
+                            if (k + 1 < kCT) TileOperands(Fp32Img<kCT>(buf, k + 1), Fp32Tails<kCT>(buf, k + 1), lane, j, gq, nx);
+                            const uint32_t at = c * uint32_t(kCT) + uint32_t(k);
+                            ArgmaxUpdate<kTrace>(Fp32Scores(ev, pol), Fp32Img<kCT>(buf, k) + 512 + 4 * gq, at * 16 + 4 * gq, A,
+                                                 hist, am);
+                            if (k + 1 < kCT) {
+#pragma unroll
+                                for (int s2 = 0; s2 < kScoreSteps; ++s2) ev[s2] = nx[s2];
+                            }
+                        }
+                    });
                     best = am.best;
                     second = am.second;
                     besta = am.besta;
@@ -1256,6 +1313,457 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
                     atomicAdd(p.counters + 3, (unsigned long long)fallbacks);
                 }
             }
+        }
+    }
+}
+
+// ================================================================================================
+// Evaluation with two game tiles per wave (the untraced FP32 form).
+//
+// The same games, decisions and arithmetic as EvaluateKernel<false, false>, with every wave playing two
+// tiles of 16 games: a 3-wave workgroup then holds all 96 games of a 32-secret cell, so a genome's trunk
+// weights and tail rows stream through LDS once per decision instead of once per split, and every operand
+// read from LDS feeds two independent MFMA chains.  The layer-0 accumulators of two tiles (2 x 64 VGPRs)
+// do not fit beside the rest, so they live in a global scratch area of the workgroup (16 KB per tile,
+// lane-major, so each access of a wave is one contiguous KB): the fold loads, updates and stores them, and
+// the trunk reads them back one neuron tile per layer-1 chunk.  Per tile, every MFMA runs in the same
+// order as in EvaluateKernel, so both kernels give the same guesses, scores and fitness.
+
+// One lane's game: lane (j, gq) of a tile plays game column j (all four lane groups hold the same state).
+struct GameLane {
+    uint32_t hist[5];  // canonical indices of the guessed words, most recent first
+    uint32_t last_code, last_fb, prev_code, prev_fb;  // the two most recent turns and their feedback
+    uint32_t sol;
+    int tc;
+    uint32_t decisions;
+    bool won, active, valid;
+};
+
+// Episode initialisation (evaluation_ops.cuh:171-237), as in EvaluateKernel: fresh games first, then the
+// two prefilled variants of every local secret.
+__device__ __forceinline__ void InitGameLane(const EvalArgs& p, const uint32_t* local, uint32_t W, uint32_t tiles_fresh,
+                                             uint32_t tiles, uint32_t tile, uint32_t j, uint32_t gq, GameLane& g) {
+    const bool fresh = tile < tiles_fresh;
+    const uint32_t game = fresh ? tile * 16 + j : (tile - tiles_fresh) * 16 + j;
+    g.valid = tile < tiles && (fresh ? game < W : game < 2 * W);
+#pragma unroll
+    for (int h = 0; h < 5; ++h) g.hist[h] = kInvalidSlot;
+    g.last_code = g.last_fb = g.prev_code = g.prev_fb = 0;
+    g.sol = 0;
+    g.tc = 0;
+    g.decisions = 0;
+    g.won = false;
+    g.active = false;
+    if (!g.valid) return;
+    const uint32_t entry = fresh ? game : (game >> 1);
+    const uint32_t variant = fresh ? 0u : 1u + (game & 1u);
+    g.sol = p.codes[local[entry]];
+    bool ok = true;
+    if (variant != 0) {
+        const uint32_t first = variant == 1 ? entry + 1 : entry + 3;
+        const uint32_t w1 = local[first % W], w2 = local[(first + 1) % W];
+        const uint32_t c1 = p.codes[w1];
+        const uint32_t c2 = p.codes[w2];
+        g.tc = 1;
+        g.won = c1 == g.sol;
+        if (g.won) {
+            ok = false;  // the second prefill would be appended to a finished grid
+        } else {
+            g.prev_code = c1;
+            g.prev_fb = FeedbackRegs(c1, g.sol);
+            g.last_code = c2;
+            g.last_fb = FeedbackRegs(c2, g.sol);
+            g.hist[0] = p.canon[w2];
+            g.hist[1] = p.canon[w1];
+            g.tc = 2;
+            g.won = c2 == g.sol;
+        }
+    }
+    if (!ok) {
+        if (gq == 0) SetStatus(p.status, 6);  // kGuessAppendFailed
+    } else {
+        g.active = !g.won && g.tc < kMaxTurns;
+    }
+}
+
+// One pass of the layer-0 fold over the 16 neuron tiles of a wave's two game tiles, one neuron tile at a
+// time, so only 4 accumulators per game tile are live: tile q (when on_q) starts from the bias (+ W0[:, 0]
+// where virgin_q) when restart_q, else from scratch; where add_q it gains W0[:, 1+64 slot ...] . e_q (the
+// 16 MFMAs of FoldSlot, same order) -- one slot for both tiles, whose W0 operands are loaded once; the
+// result goes back to scratch.  Everything neuron tile t + 1 starts from (stored accumulators or bias and
+// virgin column, and its W0 operands) loads while tile t folds.
+__device__ __forceinline__ void FoldPairPass(Rsrc acc_r, const TrunkLane& tl, Rsrc enc_r, int slot,
+                                             bool on0, bool restart0, bool virgin0, bool add0, const float (&e0)[16],
+                                             bool on1, bool restart1, bool virgin1, bool add1, const float (&e1)[16],
+                                             uint32_t lane) {
+    const uint32_t so = uint32_t(64 * slot) * 4;
+    const bool any_add = add0 || add1;
+    const v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    // the loads neuron tile t starts from: W0 operands, and per game tile its base (stored accumulators or
+    // the bias) and the virgin column
+    auto issue = [&](int t, v4f (&w)[4], v4f& b0, v4f& v0, v4f& b1, v4f& v1) __device__ {
+        const uint32_t row = uint32_t(16 * t * kModelIn) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[q] = any_add ? Buf4(tl.gr, tl.off_fold, so + row + 16 * q) : zero;
+        const uint32_t voff = uint32_t(kTurnFeatures * kEncOut + 16 * t + 4 * tl.gq) * 4;
+        b0 = !on0 ? zero : (restart0 ? Buf4(tl.gr, tl.off_b0, 64 * t) : AccLoad(acc_r, lane, 0, t));
+        v0 = (on0 && virgin0) ? Buf4(enc_r, voff, 0) : zero;
+        b1 = !on1 ? zero : (restart1 ? Buf4(tl.gr, tl.off_b0, 64 * t) : AccLoad(acc_r, lane, 1, t));
+        v1 = (on1 && virgin1) ? Buf4(enc_r, voff, 0) : zero;
+    };
+    v4f w[4], b0, v0, b1, v1;
+    issue(0, w, b0, v0, b1, v1);
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        v4f nw[4], nb0, nv0, nb1, nv1;
+        if (t + 1 < 16) issue(t + 1, nw, nb0, nv0, nb1, nv1);
+        v4f a0 = virgin0 ? b0 + v0 : b0;
+        v4f a1 = virgin1 ? b1 + v1 : b1;
+        if (add0 && add1) {
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                a0 = Mfma(w[s / 4][s % 4], e0[s], a0);
+                a1 = Mfma(w[s / 4][s % 4], e1[s], a1);
+            }
+        } else if (add0) {
+#pragma unroll
+            for (int s = 0; s < 16; ++s) a0 = Mfma(w[s / 4][s % 4], e0[s], a0);
+        } else if (add1) {
+#pragma unroll
+            for (int s = 0; s < 16; ++s) a1 = Mfma(w[s / 4][s % 4], e1[s], a1);
+        }
+        if (on0) AccStore(acc_r, lane, 0, t, a0);
+        if (on1) AccStore(acc_r, lane, 1, t, a1);
+        __builtin_amdgcn_sched_barrier(0);
+        if (t + 1 < 16) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) w[q] = nw[q];
+            b0 = nb0;
+            v0 = nv0;
+            b1 = nb1;
+            v1 = nv1;
+        }
+    }
+}
+
+// Layer-0 fold of a wave's two game tiles for decision `step` (EvaluateKernel's fold, same order per
+// accumulator).  The older prefilled turn (slot 0) of a prefilled tile went in before the first decision
+// (FoldPrefill), so every decision adds at most one slot per tile, that of its latest turn: a fresh tile
+// restarts at its first decision (bias + virgin column, no slot) and at its second (bias + slot 0, the grid
+// no longer empty); otherwise a tile adds slot ttc - 1 to its stored accumulators.  Tiles of the same kind
+// add the same slot in one pass.
+__device__ __forceinline__ void FoldPair(Rsrc acc_r, const TrunkLane& tl, Rsrc enc_r,
+                                         const GameLane& g0, const GameLane& g1, bool act0, bool act1, bool fresh0,
+                                         bool fresh1, int step, uint32_t lane) {
+    const int ttc0 = (fresh0 ? 0 : 2) + step, ttc1 = (fresh1 ? 0 : 2) + step;
+    const bool rs0 = fresh0 && step <= 1, rs1 = fresh1 && step <= 1;
+    const bool add0 = act0 && ttc0 > 0, add1 = act1 && ttc1 > 0;
+    float e0[16], e1[16];
+    if (add0) EncodeTurnGlobal(enc_r, tl, g0.last_code, g0.last_fb, e0);
+    if (add1) EncodeTurnGlobal(enc_r, tl, g1.last_code, g1.last_fb, e1);
+    const int slot0 = ttc0 - 1, slot1 = ttc1 - 1;
+    if (!(add0 && add1) || slot0 == slot1) {
+        FoldPairPass(acc_r, tl, enc_r, add0 ? slot0 : slot1, act0, rs0, rs0 && step == 0, add0, e0, act1,
+                     rs1, rs1 && step == 0, add1, e1, lane);
+    } else {  // a fresh and a prefilled tile: one pass each
+        FoldPairPass(acc_r, tl, enc_r, slot0, act0, rs0, rs0 && step == 0, add0, e0, false, false, false,
+                     false, e1, lane);
+        FoldPairPass(acc_r, tl, enc_r, slot1, false, false, false, false, e0, act1, rs1, rs1 && step == 0,
+                     add1, e1, lane);
+    }
+}
+
+// Before the first decision: the accumulators of every active prefilled tile become bias + W0[:, slot 0]
+// . enc(older prefilled turn).
+__device__ __forceinline__ void FoldPrefill(Rsrc acc_r, const TrunkLane& tl, Rsrc enc_r,
+                                            const GameLane& g0, const GameLane& g1, bool pre0, bool pre1,
+                                            uint32_t lane) {
+    float e0[16], e1[16];
+    if (pre0) EncodeTurnGlobal(enc_r, tl, g0.prev_code, g0.prev_fb, e0);
+    if (pre1) EncodeTurnGlobal(enc_r, tl, g1.prev_code, g1.prev_fb, e1);
+    FoldPairPass(acc_r, tl, enc_r, 0, pre0, true, false, pre0, e0, pre1, true, false, pre1, e1, lane);
+}
+
+// Layers 1 and 2 of two tiles for the whole workgroup, TrunkStream's chunks and MFMA order per tile;
+// the layer-0 accumulators come from scratch (acc_r), one neuron tile per layer-1 chunk, loaded
+// one chunk ahead.
+// Layers 1 and 2 of two tiles for the whole workgroup, TrunkStream's chunks and MFMA order per tile;
+// the layer-0 accumulators come from scratch (acc_r), one neuron tile per layer-1 chunk, loaded
+// one chunk ahead.  The layer-2 biases travel with the first layer-2 chunk (one 256-byte transfer behind
+// it), so no register holds them during layer 1.
+constexpr uint32_t kTrunkBiasOff = 9216;  // byte offset of the layer-2 biases in that chunk's buffer
+static_assert(kTrunkBiasOff >= kTrunkReadBytes && kTrunkBiasOff % 16 == 0, "behind every layer-2 read");
+template <typename FN>
+__device__ __forceinline__ void TrunkStreamPair(Rsrc acc_r, const TrunkLane& tl,
+                                                float (&pol0)[kScoreSteps], float (&pol1)[kScoreSteps], bool compute,
+                                                uint8_t* b0, uint8_t* b1, uint32_t wave, uint32_t waves, uint32_t lane,
+                                                FN&& next_stage) {
+    static_assert(kTrunkW1Chunks % 2 == 0 && kTrunkW2Chunks % 2 == 0, "the last chunk sits in b1");
+    const uint32_t lw1 = TrunkLaneW1(lane), lw2 = TrunkLaneW2(lane);
+    v4f h1a[8], h1b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        h1a[u] = Buf4(tl.gr, tl.off_b1, 64 * u);
+        h1b[u] = h1a[u];
+    }
+    v4f xa = AccLoad(acc_r, lane, 0, 0);
+    v4f xb = AccLoad(acc_r, lane, 1, 0);
+    WaitChunk();
+    __syncthreads();
+    const uint32_t rd1 = 64 * tl.j + 16 * tl.gq;
+#pragma unroll
+    for (uint32_t t = 0; t < kTrunkW1Chunks; ++t) {
+        uint8_t* cur = (t & 1u) ? b1 : b0;
+        uint8_t* nxt = (t & 1u) ? b0 : b1;
+        IssueTrunkChunk(nxt, tl.gr, t + 1, wave, waves, lw1, lw2);
+        v4f na = xa, nb = xb;
+        if (t + 1 < kTrunkW1Chunks) {
+            na = AccLoad(acc_r, lane, 0, t + 1);
+            nb = AccLoad(acc_r, lane, 1, t + 1);
+        } else if (wave == 0) {  // the layer-2 biases, B2[0..64), behind the first layer-2 chunk
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(tl.gr, (LdsPtr)(nxt + kTrunkBiasOff), 4, 4 * lane,
+                                                     uint32_t(kOffB2) * 4, 0, 0);
+        }
+        if (compute) {
+            float ha[4], hb[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                ha[r] = fmaxf(xa[r], 0.0f);
+                hb[r] = fmaxf(xb[r], 0.0f);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) {
+                const v4f w = *reinterpret_cast<const v4f*>(cur + u * 1024 + rd1);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    h1a[u] = Mfma(w[r], ha[r], h1a[u]);
+                    h1b[u] = Mfma(w[r], hb[r], h1b[u]);
+                }
+            }
+        }
+        WaitChunk();
+        __syncthreads();
+        xa = na;
+        xb = nb;
+    }
+    // layer 2 starts from the biases: pv[u][r] = B2[ScoreDim(4u + r, gq)] (pads read neighbouring LDS
+    // words and are zeroed below), as TrunkStream loads them
+    constexpr int kRowTiles = (kScoreSteps + 3) / 4;
+    v4f pva[kRowTiles], pvb[kRowTiles];
+    {
+        const float* sb = reinterpret_cast<const float*>(b0 + kTrunkBiasOff);
+#pragma unroll
+        for (int u = 0; u < kRowTiles; ++u)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int st = 4 * u + r < kScoreSteps ? 4 * u + r : 0;
+                pva[u][r] = sb[st < kFixedSteps ? 7 * tl.gq + st : 19 + 10 * tl.gq + st];
+                pvb[u][r] = pva[u][r];
+            }
+    }
+    const uint32_t jr = tl.j & 3u, jg = tl.j >> 2;
+    const uint32_t rd_f = (7 * jg + jr) * (32 * 4) + 16 * tl.gq;
+    const uint32_t rd_t = (19 + 10 * jg + jr) * (32 * 4) + 16 * tl.gq;
+    const uint32_t rd_1 = jr + 4 < uint32_t(kFixedSteps) ? rd_f + 512 : rd_t + 512;
+#pragma unroll
+    for (uint32_t c = 0; c < kTrunkW2Chunks; ++c) {
+        uint8_t* cur = (c & 1u) ? b1 : b0;
+        uint8_t* nxt = (c & 1u) ? b0 : b1;
+        if (c + 1 < kTrunkW2Chunks)
+            IssueTrunkChunk(nxt, tl.gr, kTrunkW1Chunks + c + 1, wave, waves, lw1, lw2);
+        else
+            next_stage(nxt);
+        if (compute) {
+#pragma unroll
+            for (uint32_t h = 0; h < 2; ++h) {
+                const uint32_t t = 2 * c + h;
+                float ha[4], hb[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    ha[r] = fmaxf(h1a[t][r], 0.0f);
+                    hb[r] = fmaxf(h1b[t][r], 0.0f);
+                }
+#pragma unroll
+                for (int u = 0; u < kRowTiles; ++u) {
+                    const uint32_t rd = u == 0 ? rd_f : (u == 1 ? rd_1 : rd_t + 512 * uint32_t(u));
+                    const v4f w = *reinterpret_cast<const v4f*>(cur + rd + 64 * h);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        pva[u] = Mfma(w[r], ha[r], pva[u]);
+                        pvb[u] = Mfma(w[r], hb[r], pvb[u]);
+                    }
+                }
+            }
+        }
+        WaitChunk();
+        __syncthreads();
+    }
+#pragma unroll
+    for (int s = 0; s < kScoreSteps; ++s) {
+        const bool pad = ScoreDim(s, int(tl.gq)) < 0;
+        pol0[s] = pad ? 0.0f : pva[s / 4][s % 4];
+        pol1[s] = pad ? 0.0f : pvb[s / 4][s % 4];
+    }
+}
+
+// Applies a game's decision (wordle_grid.hpp:69-77), as EvaluateKernel does.
+__device__ __forceinline__ void ApplyGuess(const EvalArgs& p, uint32_t besta, uint32_t gq, GameLane& g) {
+    if (!g.active) return;
+    if (besta == kInvalidSlot) {
+        if (gq == 0) SetStatus(p.status, 8);  // kActionSelectionFailed
+        g.active = false;
+        return;
+    }
+    const uint32_t code = p.codes[besta];
+#pragma unroll
+    for (int t = 4; t > 0; --t) g.hist[t] = g.hist[t - 1];
+    g.hist[0] = p.canon[besta];
+    g.last_code = code;  // (the older turn is no longer needed: every later fold adds the latest turn only)
+    g.last_fb = FeedbackRegs(code, g.sol);
+    ++g.tc;
+    ++g.decisions;
+    g.won = code == g.sol;
+    g.active = !g.won && g.tc < kMaxTurns;
+}
+
+// Scores the finished episodes of a tile (evaluation_ops.cuh:239-242) into the genome's raw fitness and
+// the decision / encoded-turn counters.
+__device__ __forceinline__ void FinishTile(const EvalArgs& p, uint32_t n, bool fresh, bool tile_exists, uint32_t lane,
+                                           uint32_t gq, const GameLane& g) {
+    float score = (g.valid && g.won) ? 10.0f + float(kMaxTurns - g.tc) : 0.0f;
+    uint32_t dec = (g.valid && gq == 0) ? g.decisions : 0u;
+    const uint32_t base_tc = fresh ? 0u : 2u;
+    uint32_t enc = (g.valid && gq == 0 && g.decisions > 0) ? base_tc + g.decisions - 1u : 0u;
+    if (gq != 0) score = 0.0f;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        score += __shfl_xor(score, off);
+        dec += __shfl_xor(dec, off);
+        enc += __shfl_xor(enc, off);
+    }
+    if (lane == 0 && tile_exists) {
+        atomicAdd(p.raw_fitness + n, score);
+        atomicAdd(p.counters + 0, (unsigned long long)dec);
+        atomicAdd(p.counters + 1, (unsigned long long)enc);
+    }
+}
+
+__global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void EvaluatePairKernel(EvalArgs p) {
+    constexpr int kCT = kChunkTiles;
+    using G = ChunkGeom<kCT>;
+    static_assert(kTrunkChunkBytes <= G::kBufBytes && kTrunkBiasOff + 4 * 68 <= G::kBufBytes, "trunk chunk");
+    __shared__ __align__(16) uint8_t s_buf0[G::kBufBytes];
+    __shared__ __align__(16) uint8_t s_buf1[G::kBufBytes];
+    __shared__ uint32_t s_item;
+    const uint32_t tid = threadIdx.x, bd = blockDim.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t j = lane & 15u;
+    const uint32_t gq = lane >> 4;
+    const uint32_t waves = bd >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const Rsrc acc_r = MakeRsrc(p.acc_scratch + (uint64_t(blockIdx.x) * kEvalMaxWaves + wave) * 2 * kAccTileFloats,
+                                2 * kAccTileFloats * sizeof(float));
+    // Persistent: (genome, split) items from the work list; split s of genome n plays tiles
+    // 2 (s waves + r splits waves + wave) + {0, 1} (r = 0, 1, ...).
+    for (;;) {
+        __syncthreads();  // the previous item is done with the LDS buffers
+        if (tid == 0) s_item = atomicAdd(p.work_counter, 1u);
+        __syncthreads();
+        const uint32_t item = s_item;
+        if (item >= p.work_count) break;
+        const uint32_t n = p.work[2 * item], split = p.work[2 * item + 1];
+        const uint32_t slot = p.slot_indices[n];
+        if (slot >= p.slot_count) {
+            if (tid == 0) SetStatus(p.status, 4);  // kInvalidGeneration
+            continue;
+        }
+        const float* genome = reinterpret_cast<const float*>(p.slab + uint64_t(slot) * p.stride);
+        const Rsrc enc_r = MakeRsrc(p.enc_t + uint64_t(n) * kEncTStride, uint32_t(kEncTStride * 4));
+        const uint32_t W = p.cell_offsets[n + 1] - p.cell_offsets[n];
+        if (W == 0) {
+            if (tid == 0) SetStatus(p.status, 5);  // kInvalidTrainingShard
+            continue;
+        }
+        if (tid == 0 && split == 0) p.local_w[n] = W;
+        const uint32_t* local = p.secret_ids + p.cell_offsets[n];
+        const uint32_t tiles_fresh = (W + 15) / 16;
+        const uint32_t tiles = tiles_fresh + (2 * W + 15) / 16;
+        const TrunkLane tl = MakeTrunkLane(genome, j, gq);
+        const uint32_t A = p.action_count;
+        const uint32_t a_tiles = (A + 15) / 16;
+        const uint32_t chunks = (a_tiles + kCT - 1) / kCT;
+        const Rsrc img_r = MakeRsrc(p.img, p.img_bytes);
+        const Rsrc tail_r = MakeRsrc(genome + kPolicyParams, uint32_t(p.stride - kPolicyParams * 4));
+        auto issue = [&](uint8_t* buf, uint32_t c) __device__ {
+            IssueChunk(buf, img_r, tail_r, c, wave, waves, lane, G::kImgBytes, G::kTailBytes, G::kImgDma, G::kTailDma);
+        };
+        const uint32_t per_round = p.splits * waves * 2;
+        const uint32_t first_tile = split * waves * 2;
+        const uint32_t rounds = tiles > first_tile ? (tiles - first_tile + per_round - 1) / per_round : 0u;
+        for (uint32_t round = 0; round < rounds; ++round) {
+            const uint32_t tile0 = first_tile + round * per_round + 2 * wave, tile1 = tile0 + 1;
+            const bool fresh0 = tile0 < tiles_fresh, fresh1 = tile1 < tiles_fresh;
+            GameLane g0, g1;
+            InitGameLane(p, local, W, tiles_fresh, tiles, tile0, j, gq, g0);
+            InitGameLane(p, local, W, tiles_fresh, tiles, tile1, j, gq, g1);
+            {
+                const bool pre0 = !fresh0 && __any(g0.active ? 1 : 0) != 0;
+                const bool pre1 = !fresh1 && __any(g1.active ? 1 : 0) != 0;
+                if (pre0 || pre1) FoldPrefill(acc_r, tl, enc_r, g0, g1, pre0, pre1, lane);
+            }
+            for (int step = 0; step < kMaxTurns; ++step) {
+                if (!__syncthreads_or((g0.active || g1.active) ? 1 : 0)) break;  // the workgroup leaves together
+                const bool act0 = __any(g0.active ? 1 : 0) != 0;
+                const bool act1 = __any(g1.active ? 1 : 0) != 0;
+                IssueTrunkChunk(s_buf0, tl.gr, 0, wave, waves, TrunkLaneW1(lane), TrunkLaneW2(lane));
+                FoldPair(acc_r, tl, enc_r, g0, g1, act0, act1, fresh0, fresh1, step, lane);
+                const bool compute = act0 || act1;
+                float pol0[kScoreSteps], pol1[kScoreSteps];
+                TrunkStreamPair(acc_r, tl, pol0, pol1, compute, s_buf0, s_buf1, wave, waves, lane,
+                                [&](uint8_t* b) __device__ { issue(b, 0); });
+                if (!act0) {
+#pragma unroll
+                    for (int s = 0; s < kScoreSteps; ++s) pol0[s] = 0.0f;
+                }
+                if (!act1) {
+#pragma unroll
+                    for (int s = 0; s < kScoreSteps; ++s) pol1[s] = 0.0f;
+                }
+                // ---- scoring: S = E . P over all actions for both tiles, masked argmax (lowest index on ties)
+                ArgmaxLane am0 = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot, -__builtin_inff()};
+                ArgmaxLane am1 = am0;
+                StreamChunksWhole(s_buf0, s_buf1, chunks, compute, issue, [&](const uint8_t* buf, uint32_t c) __device__ {
+                    ShareArgmaxGate(am0);
+                    ShareArgmaxGate(am1);
+                    float ev[kScoreSteps];
+                    TileOperands(Fp32Img<kCT>(buf, 0), Fp32Tails<kCT>(buf, 0), lane, j, gq, ev);
+#pragma unroll
+                    for (int k = 0; k < kCT; ++k) {
+                        float nx[kScoreSteps];  // the next tile's operands, read during this tile's MFMAs
+                        if (k + 1 < kCT) TileOperands(Fp32Img<kCT>(buf, k + 1), Fp32Tails<kCT>(buf, k + 1), lane, j, gq, nx);
+                        v4f s0 = {0.0f, 0.0f, 0.0f, 0.0f}, s1 = s0;
+#pragma unroll
+                        for (int s = 0; s < kScoreSteps; ++s) {
+                            s0 = Mfma(ev[s], pol0[s], s0);
+                            s1 = Mfma(ev[s], pol1[s], s1);
+                        }
+                        const uint32_t at = c * uint32_t(kCT) + uint32_t(k);
+                        const uint32_t* cn = Fp32Img<kCT>(buf, k) + 512 + 4 * gq;
+                        ArgmaxUpdate<false>(s0, cn, at * 16 + 4 * gq, A, g0.hist, am0);
+                        ArgmaxUpdate<false>(s1, cn, at * 16 + 4 * gq, A, g1.hist, am1);
+                        if (k + 1 < kCT) {
+#pragma unroll
+                            for (int s = 0; s < kScoreSteps; ++s) ev[s] = nx[s];
+                        }
+                    }
+                });
+                ReduceArgmax(am0.best, am0.second, am0.besta);
+                ReduceArgmax(am1.best, am1.second, am1.besta);
+                ApplyGuess(p, am0.besta, gq, g0);
+                ApplyGuess(p, am1.besta, gq, g1);
+            }
+            FinishTile(p, n, fresh0, tile0 < tiles, lane, gq, g0);
+            FinishTile(p, n, fresh1, tile1 < tiles, lane, gq, g1);
         }
     }
 }
@@ -2296,9 +2804,14 @@ void LaunchTransposeEncoder(const uint8_t* slab, uint64_t stride, const uint32_t
     hipLaunchKernelGGL(TransposeEncoderKernel, dim3(n), dim3(256), 0, s, slab, stride, slots, slot_count, enc_t);
 }
 
-void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, bool trace, bool bf16, hipStream_t s) {
+void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, uint32_t tiles_per_wave, bool trace, bool bf16,
+                    hipStream_t s) {
     const dim3 grid(blocks);
     const dim3 block(64 * waves);
+    if (tiles_per_wave == 2 && !trace && !bf16) {
+        hipLaunchKernelGGL(EvaluatePairKernel, grid, block, 0, s, a);
+        return;
+    }
     if (bf16) {
         if (trace)
             hipLaunchKernelGGL((EvaluateKernel<true, true>), grid, block, 0, s, a);
@@ -2314,9 +2827,12 @@ void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, bool tra
 
 uint32_t EvaluateMaxWaves() { return kEvalMaxWaves; }
 
-uint32_t EvaluateResidentPerCu(uint32_t waves, bool trace, bool bf16) {
+uint64_t EvaluateAccScratchBytes() { return uint64_t(kEvalMaxWaves) * 2 * kAccTileFloats * sizeof(float); }
+
+uint32_t EvaluateResidentPerCu(uint32_t waves, uint32_t tiles_per_wave, bool trace, bool bf16) {
     // what the registers and LDS of the kernel built here allow, as the runtime reports it
-    const void* k = bf16 ? (trace ? reinterpret_cast<const void*>(&EvaluateKernel<true, true>)
+    const void* k = (tiles_per_wave == 2 && !trace && !bf16) ? reinterpret_cast<const void*>(&EvaluatePairKernel)
+                  : bf16 ? (trace ? reinterpret_cast<const void*>(&EvaluateKernel<true, true>)
                                   : reinterpret_cast<const void*>(&EvaluateKernel<false, true>))
                          : (trace ? reinterpret_cast<const void*>(&EvaluateKernel<true, false>)
                                   : reinterpret_cast<const void*>(&EvaluateKernel<false, false>));

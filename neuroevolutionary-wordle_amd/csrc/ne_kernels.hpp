@@ -48,6 +48,7 @@ struct EvalArgs {
     uint32_t work_count;
     uint32_t* work_counter;        // next item (0 at launch)
     uint32_t splits;               // the largest split count of a genome
+    float* acc_scratch;            // [workgroup][kEvalMaxWaves][2][16 KB] layer-0 accumulators between decisions
 };
 
 struct PlanArgs {
@@ -238,9 +239,13 @@ void LaunchPackBf16(const uint8_t* slab, uint64_t stride, const uint32_t* slots,
 void LaunchTransposeEncoder(const uint8_t* slab, uint64_t stride, const uint32_t* slots, uint32_t n,
                             uint32_t slot_count, float* enc_t, hipStream_t s);
 // Persistent evaluation: `blocks` workgroups of `waves` waves drain the work list.
-void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, bool trace, bool bf16, hipStream_t s);
+// tiles_per_wave 2 selects the two-tiles-per-wave kernel (untraced FP32 form only)
+void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, uint32_t tiles_per_wave, bool trace, bool bf16,
+                    hipStream_t s);
 // Workgroups of `waves` waves of the evaluation kernel that fit on one CU at once (0 if unknown).
-uint32_t EvaluateResidentPerCu(uint32_t waves, bool trace, bool bf16);
+uint32_t EvaluateResidentPerCu(uint32_t waves, uint32_t tiles_per_wave, bool trace, bool bf16);
+// bytes of layer-0 accumulator scratch one evaluation workgroup uses (either kernel)
+uint64_t EvaluateAccScratchBytes();
 // The most waves per evaluation workgroup this build supports (its LDS tables are sized for them).
 uint32_t EvaluateMaxWaves();
 void LaunchFinalize(const float* raw, const uint32_t* local_w, uint32_t n, float* fitness, uint32_t* evals,

@@ -37,7 +37,7 @@ struct StageEvents {
 // One evaluation's launch description: the kernel arguments, the workgroup shape and the games played.
 struct EvalSetup {
     EvalArgs a;
-    uint32_t blocks, waves;
+    uint32_t blocks, waves, tiles_per_wave;
     bool bf16, trace;
     uint64_t total_games;
 };
@@ -167,6 +167,9 @@ struct ne_runtime {
     uint32_t tuned_waves = 0;           // 0: the idle-slot heuristic picks the workgroup size
     uint32_t tuned_rounds = 1;          // rounds of `waves` tiles one work item plays
     uint32_t tuned_wg_per_cu = 0;       // persistent workgroups per CU (0: as many as fit)
+    uint32_t tuned_tpw = 1;             // game tiles per wave (2: the two-tiles-per-wave FP32 kernel)
+    float* eval_acc = nullptr;          // the evaluation's layer-0 scratch, EvaluateAccScratchBytes() per workgroup
+    uint64_t eval_acc_bytes = 0;
 
     // captured generation steps (ne_set_step_graph): one graph per buffer parity, the evaluation's host
     // preparation cached while its inputs stay the same
@@ -536,9 +539,13 @@ static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32
     a.game_trace = trace ? rt->d_trace : nullptr;
     a.policy_trace = (rt->trace_flags & NE_TRACE_POLICY) ? rt->d_policy_trace : nullptr;
     a.game_offsets = trace ? rt->d_game_offsets : nullptr;
+    // Game tiles per wave: 2 runs the two-tiles-per-wave kernel (untraced FP32 form only), which plays a
+    // workgroup's 2 x waves tiles with one operand stream per decision.
+    const uint32_t tpw = (rt->tuned_tpw == 2 && !trace && !bf16) ? 2u : 1u;
     // Waves per workgroup: 6-wave workgroups of this kernel's register footprint fit once per CU, 2-4
-    // wave ones fill a CU (12 waves); pick the size that leaves the fewest idle wave slots over all
-    // genomes (ties: the larger).  A genome's tiles then spread over ceil(max_tiles / waves) workgroups.
+    // wave ones fill a CU (12 waves); pick the size that leaves the fewest idle tile slots over all
+    // genomes (ties: the larger).  A genome's tiles then spread over ceil(max_tiles / (tpw x waves))
+    // workgroups.
     std::vector<uint32_t> tiles(n);
     for (uint64_t c = 0; c < n; ++c) {
         const uint64_t W = offs[c + 1] - offs[c];
@@ -547,28 +554,29 @@ static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32
     uint32_t waves = 1;
     uint64_t best_cost = ~0ull;
     for (uint32_t w = 4; w >= 1; --w) {
+        const uint64_t per = uint64_t(w) * tpw;
         uint64_t cost = 0;
-        for (uint64_t c = 0; c < n; ++c) cost += (tiles[c] + w - 1) / w * w;
+        for (uint64_t c = 0; c < n; ++c) cost += (tiles[c] + per - 1) / per * per;
         if (cost < best_cost) {
             best_cost = cost;
             waves = w;
         }
     }
     if (rt->tuned_waves != 0) waves = rt->tuned_waves;  // the autotuner's choice (ne_autotune_evaluation)
-    // Work list for the persistent kernel: split-major (every genome's first `waves` tiles, then the next
-    // ...), genomes by descending tile count within a split, so no workgroup slot is spent on an empty
-    // split and the longest genomes start first.  An item plays `rounds` rounds of `waves` tiles (split s of
-    // genome n: tiles s*waves + r*splits*waves + wave), so a genome's tiles spread over
-    // ceil(tiles / (rounds x waves)) items.
+    // Work list for the persistent kernel: split-major (every genome's first tpw x waves tiles, then the
+    // next ...), genomes by descending tile count within a split, so no workgroup slot is spent on an
+    // empty split and the longest genomes start first.  An item plays `rounds` rounds of tpw x waves
+    // tiles, so a genome's tiles spread over ceil(tiles / (rounds x tpw x waves)) items.
     const uint32_t rounds = std::max<uint32_t>(1, rt->tuned_rounds);
-    const uint32_t splits = uint32_t((max_tiles + uint64_t(waves) * rounds - 1) / (uint64_t(waves) * rounds));
+    const uint64_t per_item = uint64_t(waves) * tpw;
+    const uint32_t splits = uint32_t((max_tiles + per_item * rounds - 1) / (per_item * rounds));
     std::vector<uint32_t> order(n);
     for (uint64_t c = 0; c < n; ++c) order[c] = uint32_t(c);
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return tiles[x] > tiles[y]; });
     std::vector<uint32_t> work;
     for (uint32_t sp = 0; sp < splits; ++sp)
         for (const uint32_t g : order) {
-            if (uint64_t(sp) * waves >= tiles[g]) break;
+            if (uint64_t(sp) * per_item >= tiles[g]) break;
             work.push_back(g);
             work.push_back(sp);
         }
@@ -587,10 +595,22 @@ static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32
     a.splits = splits;
     // resident workgroups per CU, as the kernel's registers and LDS allow (3 waves per SIMD at 168 VGPRs
     // and 4 workgroups per CU by LDS in the FP32 form)
-    const uint32_t resident = EvaluateResidentPerCu(waves, trace, bf16);
+    const uint32_t resident = EvaluateResidentPerCu(waves, tpw, trace, bf16);
     uint64_t per_cu = resident != 0 ? resident : std::min<uint64_t>(12 / waves, 4);
     if (rt->tuned_wg_per_cu != 0) per_cu = std::min<uint64_t>(per_cu, rt->tuned_wg_per_cu);
     const uint32_t blocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(items, rt->cu_count * per_cu)));
+    {
+        // where the layer-0 accumulators of every resident workgroup's tiles wait between decisions (kept:
+        // the captured step bakes the pointer in)
+        const uint64_t need = uint64_t(blocks) * EvaluateAccScratchBytes();
+        if (need > rt->eval_acc_bytes) {
+            Free(rt->eval_acc);
+            rt->eval_acc_bytes = 0;
+            if (!Alloc(&rt->eval_acc, need / sizeof(float))) return Fail(rt, NE_DEVICE_FAILURE);
+            rt->eval_acc_bytes = need;
+        }
+        a.acc_scratch = rt->eval_acc;
+    }
     if (rt->enc_t == nullptr && !Alloc(&rt->enc_t, rt->max_gen * kEncTStride))
         return Fail(rt, NE_DEVICE_FAILURE);
     a.enc_t = rt->enc_t;
@@ -610,6 +630,7 @@ static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32
     out->a = a;
     out->blocks = blocks;
     out->waves = waves;
+    out->tiles_per_wave = tpw;
     out->bf16 = bf16;
     out->trace = trace;
     out->total_games = total_games;
@@ -637,7 +658,7 @@ static void LaunchEvaluation(ne_runtime* rt, const EvalSetup& s, uint64_t n, flo
                        s.a.action_count, rt->bf16_ops, s.a.bf16_stride, rt->tail_absmax, rt->stream);
     if (timed) EventEnd(rt, 8);
     if (timed) EventBegin(rt, 0);
-    LaunchEvaluate(s.a, s.blocks, s.waves, s.trace, s.bf16, rt->stream);
+    LaunchEvaluate(s.a, s.blocks, s.waves, s.tiles_per_wave, s.trace, s.bf16, rt->stream);
     if (timed) EventEnd(rt, 0);
 }
 
@@ -1497,6 +1518,7 @@ void ne_destroy(ne_runtime* rt) {
     Free(rt->breaks);
     Free(rt->rank_weights);
     Free(rt->enc_t);
+    Free(rt->eval_acc);
     Free(rt->bf16_ops);
     Free(rt->tail_absmax);
     Free(rt->raw);
@@ -1871,7 +1893,8 @@ static int TryGraphStep(ne_runtime* rt, uint32_t seed, const ne_word_counts* cou
     auto eval_key = [&](uint64_t (&k)[10]) {
         const uint64_t v[10] = {rt->input_epoch, n, std::min(rt->action_count, counts->action_space_word_count),
                                 rt->eval_mode,
-                                rt->tuned_waves | (uint64_t(rt->tuned_rounds) << 16) | (uint64_t(rt->tuned_wg_per_cu) << 32),
+                                rt->tuned_waves | (uint64_t(rt->tuned_rounds) << 16) | (uint64_t(rt->tuned_wg_per_cu) << 32) |
+                                    (uint64_t(rt->tuned_tpw) << 48),
                                 uint64_t(uintptr_t(rt->slab)), rt->stride,
                                 rt->slot_count, rt->action_count, rt->max_gen};
         std::memcpy(k, v, sizeof(v));
@@ -2242,11 +2265,12 @@ int ne_autotune_evaluation(ne_runtime* rt, const ne_word_counts* counts, const n
                            uint32_t* chosen_waves, float* timings_ms) {
     if (!rt || !counts) return Fail(rt, NE_INVALID_ARGUMENT);
     // coordinate search, one knob at a time, each candidate timed on one evaluation of the current generation
-    // (results do not depend on the shape): waves per workgroup, then rounds per work item, then workgroups
-    // per CU
-    const uint32_t saved[3] = {rt->tuned_waves, rt->tuned_rounds, rt->tuned_wg_per_cu};
-    auto time_shape = [&](uint32_t w, uint32_t r, uint32_t g, float* ms) -> int {
+    // (results do not depend on the shape): waves per workgroup, then (untraced FP32 form) two game tiles
+    // per wave at every workgroup size, then rounds per work item, then workgroups per CU
+    const uint32_t saved[4] = {rt->tuned_waves, rt->tuned_rounds, rt->tuned_wg_per_cu, rt->tuned_tpw};
+    auto time_shape = [&](uint32_t w, uint32_t tpw, uint32_t r, uint32_t g, float* ms) -> int {
         rt->tuned_waves = w;
+        rt->tuned_tpw = tpw;
         rt->tuned_rounds = r;
         rt->tuned_wg_per_cu = g;
         rt->times = ne_stage_times{};
@@ -2255,16 +2279,17 @@ int ne_autotune_evaluation(ne_runtime* rt, const ne_word_counts* counts, const n
             rt->tuned_waves = saved[0];
             rt->tuned_rounds = saved[1];
             rt->tuned_wg_per_cu = saved[2];
+            rt->tuned_tpw = saved[3];
             return st;
         }
         *ms = rt->times.evaluate_ms;
         return NE_OK;
     };
-    uint32_t bw = 0, br = 1, bg = 0;
+    uint32_t bw = 0, bt = 1, br = 1, bg = 0;
     float best_ms = 0.0f;
     for (uint32_t w = 1; w <= 4; ++w) {
         float ms = 0.0f;
-        const int st = time_shape(w, 1, 0, &ms);
+        const int st = time_shape(w, 1, 1, 0, &ms);
         if (st != NE_OK) return st;
         if (timings_ms) timings_ms[w - 1] = ms;
         if (bw == 0 || ms < best_ms) {
@@ -2272,20 +2297,32 @@ int ne_autotune_evaluation(ne_runtime* rt, const ne_word_counts* counts, const n
             best_ms = ms;
         }
     }
+    const bool trace = rt->trace_flags != 0, bf16 = (rt->eval_mode & 0xFFu) == NE_EVAL_BF16_RESCORED;
+    if (!trace && !bf16) {
+        for (uint32_t w = 1; w <= 4; ++w) {
+            float ms = 0.0f;
+            const int st = time_shape(w, 2, 1, 0, &ms);
+            if (st != NE_OK) return st;
+            if (ms < best_ms) {
+                bw = w;
+                bt = 2;
+                best_ms = ms;
+            }
+        }
+    }
     for (uint32_t r = 2; r <= 4; r *= 2) {
         float ms = 0.0f;
-        const int st = time_shape(bw, r, 0, &ms);
+        const int st = time_shape(bw, bt, r, 0, &ms);
         if (st != NE_OK) return st;
         if (ms < best_ms) {
             br = r;
             best_ms = ms;
         }
     }
-    const uint32_t resident = std::max<uint32_t>(1, EvaluateResidentPerCu(bw, rt->trace_flags != 0,
-                                                                         (rt->eval_mode & 0xFFu) == NE_EVAL_BF16_RESCORED));
+    const uint32_t resident = std::max<uint32_t>(1, EvaluateResidentPerCu(bw, bt, trace, bf16));
     for (uint32_t g = resident - 1; g >= 1 && g + 2 >= resident; --g) {
         float ms = 0.0f;
-        const int st = time_shape(bw, br, g, &ms);
+        const int st = time_shape(bw, bt, br, g, &ms);
         if (st != NE_OK) return st;
         if (ms < best_ms) {
             bg = g;
@@ -2293,6 +2330,7 @@ int ne_autotune_evaluation(ne_runtime* rt, const ne_word_counts* counts, const n
         }
     }
     rt->tuned_waves = bw;
+    rt->tuned_tpw = bt;
     rt->tuned_rounds = br;
     rt->tuned_wg_per_cu = bg;
     if (chosen_waves) *chosen_waves = bw;
@@ -2304,14 +2342,15 @@ int ne_get_eval_shape(ne_runtime* rt, ne_eval_shape* out) {
     out->waves = rt->tuned_waves;
     out->rounds_per_item = rt->tuned_rounds;
     out->workgroups_per_cu = rt->tuned_wg_per_cu;
-    out->reserved0 = 0;
+    out->tiles_per_wave = rt->tuned_tpw;
     return NE_OK;
 }
 
 int ne_set_eval_shape_ex(ne_runtime* rt, const ne_eval_shape* shape) {
     if (!rt || !shape || shape->waves > EvaluateMaxWaves() || shape->rounds_per_item > 64 ||
-        shape->workgroups_per_cu > 64)
+        shape->workgroups_per_cu > 64 || shape->tiles_per_wave > 2)
         return Fail(rt, NE_INVALID_ARGUMENT);
+    rt->tuned_tpw = std::max<uint32_t>(1, shape->tiles_per_wave);
     rt->tuned_waves = shape->waves;
     rt->tuned_rounds = std::max<uint32_t>(1, shape->rounds_per_item);
     rt->tuned_wg_per_cu = shape->workgroups_per_cu;

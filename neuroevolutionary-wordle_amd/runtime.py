@@ -97,7 +97,7 @@ class StageTimes(C.Structure):
 
 class EvalShape(C.Structure):
     _fields_ = [("waves", C.c_uint32), ("rounds_per_item", C.c_uint32), ("workgroups_per_cu", C.c_uint32),
-                ("reserved0", C.c_uint32)]
+                ("tiles_per_wave", C.c_uint32)]
 
 
 GAME_TRACE_DTYPE = np.dtype([("guesses", "<u4", 6), ("best", "<f4", 6), ("second", "<f4", 6),
@@ -610,8 +610,9 @@ class SlabRuntime:
         self._check("ne_get_eval_shape", self.lib.ne_get_eval_shape(self.h, C.byref(s)))
         return s
 
-    def set_eval_shape_ex(self, waves=0, rounds_per_item=1, workgroups_per_cu=0):
-        s = EvalShape(waves=waves, rounds_per_item=rounds_per_item, workgroups_per_cu=workgroups_per_cu)
+    def set_eval_shape_ex(self, waves=0, rounds_per_item=1, workgroups_per_cu=0, tiles_per_wave=1):
+        s = EvalShape(waves=waves, rounds_per_item=rounds_per_item, workgroups_per_cu=workgroups_per_cu,
+                      tiles_per_wave=tiles_per_wave)
         self._check("ne_set_eval_shape_ex", self.lib.ne_set_eval_shape_ex(self.h, C.byref(s)))
 
     def set_eval_shape(self, waves):
