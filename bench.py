@@ -125,6 +125,9 @@ def main():
     ap.add_argument("--no-wallclock", action="store_true", help="skip the wall-clock-to-target leg")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="time every form at the built-in launch shape instead of the autotuner's")
+    ap.add_argument("--eval-shape", default="", metavar="W,R,G,T",
+                    help="time every form at this launch shape (waves, rounds per item, workgroups per CU, game "
+                         "tiles per wave) instead of the autotuner's")
     ap.add_argument("--wallclock-seeds", default="207,217,227",
                     help="seeds of the wall-clock-to-target leg (none of them was used to choose a setting)")
     ap.add_argument("--wallclock-generations", type=int, default=200,
@@ -180,11 +183,17 @@ def main():
         rt.set_eval_precision(mode)
         rt.set_step_graph(False)
         rt.bootstrap(n, seed=7, variance_scale=1.0, tail_std=0.1)
-        if args.autotune:
+        if args.eval_shape:
+            w, r, g, t = (int(x) for x in args.eval_shape.split(","))
+            rt.set_eval_shape_ex(w, r, g, t)
+            shapes[(mode, graph)] = {"waves": w, "rounds_per_item": r, "workgroups_per_cu": g, "tiles_per_wave": t,
+                                     "fixed": True}
+        elif args.autotune:
             # the evaluation's launch shape for this form, chosen on generation 0 (results do not depend on it)
             _, wave_ms, shape = rt.autotune_evaluation(counts)
             shapes[(mode, graph)] = {"waves": shape.waves, "rounds_per_item": shape.rounds_per_item,
                                      "workgroups_per_cu": shape.workgroups_per_cu,
+                                     "tiles_per_wave": shape.tiles_per_wave,
                                      "waves_sweep_ms": [round(x, 3) for x in wave_ms]}
         rt.set_step_graph(graph)
         gen = 0
