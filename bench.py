@@ -45,16 +45,22 @@ def kernel_source_hash():
     return h.hexdigest()
 
 
-def measured_traffic():
-    """HBM bytes per evaluation launch from the committed FETCH_SIZE / WRITE_SIZE measurement of this
-    workload (scaled by the plain-copy calibration recorded with it) and whether it was measured on the
-    kernel sources of this tree: (bytes or None, stale)."""
+KERNEL_NAMES = ("EvaluateKernel<false,false>", "EvaluateStreamedKernel", "EvaluatePairKernel")
+
+
+def measured_traffic(kernel=0):
+    """HBM bytes per evaluation launch of the FP32 form's `kernel` (0 registers, 1 streamed, 2 paired) from the
+    committed FETCH_SIZE / WRITE_SIZE measurement of this workload (scaled by the plain-copy calibration
+    recorded with it) and whether it was measured on the kernel sources of this tree: (bytes or None, stale)."""
     if not os.path.exists(TRAFFIC_FILE):
         return None, True
     with open(TRAFFIC_FILE) as f:
         t = json.load(f)
     stale = t.get("kernel_source_sha256") != kernel_source_hash()
-    return t["fetch_bytes"] * t["fetch_scale"] + t["write_bytes"] * t["write_scale"], stale
+    k = t.get("kernels", {}).get(KERNEL_NAMES[kernel])
+    if k is None:
+        return None, True
+    return k["fetch_bytes"] * t["fetch_scale"] + k["write_bytes"] * t["write_scale"], stale
 
 
 def cpu_model():
@@ -125,9 +131,9 @@ def main():
     ap.add_argument("--no-wallclock", action="store_true", help="skip the wall-clock-to-target leg")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="time every form at the built-in launch shape instead of the autotuner's")
-    ap.add_argument("--eval-shape", default="", metavar="W,R,G,T",
-                    help="time every form at this launch shape (waves, rounds per item, workgroups per CU, game "
-                         "tiles per wave) instead of the autotuner's")
+    ap.add_argument("--eval-shape", default="", metavar="W,R,G,K",
+                    help="time every form at this launch shape (waves, rounds per item, workgroups per CU, "
+                         "evaluation kernel: 0 registers, 1 streamed, 2 paired) instead of the autotuner's")
     ap.add_argument("--wallclock-seeds", default="207,217,227",
                     help="seeds of the wall-clock-to-target leg (none of them was used to choose a setting)")
     ap.add_argument("--wallclock-generations", type=int, default=200,
@@ -184,16 +190,17 @@ def main():
         rt.set_step_graph(False)
         rt.bootstrap(n, seed=7, variance_scale=1.0, tail_std=0.1)
         if args.eval_shape:
-            w, r, g, t = (int(x) for x in args.eval_shape.split(","))
-            rt.set_eval_shape_ex(w, r, g, t)
-            shapes[(mode, graph)] = {"waves": w, "rounds_per_item": r, "workgroups_per_cu": g, "tiles_per_wave": t,
-                                     "fixed": True}
+            w, r, g, k = (int(x) for x in args.eval_shape.split(","))
+            rt.set_eval_shape_ex(w, r, g, k)
+            shapes[(mode, graph)] = {"waves": w, "rounds_per_item": r, "workgroups_per_cu": g, "kernel_id": k,
+                                     "kernel": ["registers", "streamed", "paired"][k], "fixed": True}
         elif args.autotune:
             # the evaluation's launch shape for this form, chosen on generation 0 (results do not depend on it)
             _, wave_ms, shape = rt.autotune_evaluation(counts)
             shapes[(mode, graph)] = {"waves": shape.waves, "rounds_per_item": shape.rounds_per_item,
+                                     "kernel_id": shape.kernel,
                                      "workgroups_per_cu": shape.workgroups_per_cu,
-                                     "tiles_per_wave": shape.tiles_per_wave,
+                                     "kernel": ["registers", "streamed", "paired"][shape.kernel],
                                      "waves_sweep_ms": [round(x, 3) for x in wave_ms]}
         rt.set_step_graph(graph)
         gen = 0
@@ -228,7 +235,9 @@ def main():
     eval_avg_ms = fp32["eval_ms"]
     achieved = fp32["flops"] / (eval_avg_ms * 1e-3) / 1e12
     summary = fp32["summary"]
-    traffic, traffic_stale = measured_traffic()
+    # the evaluation kernel the FP32 form was timed with (the built-in one unless a shape was chosen)
+    fp32_kernel = shapes.get((rtm.EVAL_FP32, False), {}).get("kernel_id", 0)
+    traffic, traffic_stale = measured_traffic(fp32_kernel)
     bf16_out = None
     if bf16 is not None:
         bf16_out = {
@@ -268,10 +277,11 @@ def main():
                    "mutation_probability": 0.01, "mutation_sigma": 0.02},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_MFMA_TFLOPS, "traffic": None if traffic_stale else traffic,
-                     "kernel": "EvaluateKernel", "kernel_ms": eval_avg_ms},
-        "traffic_measurement": {"bytes_per_launch": traffic, "stale": traffic_stale, "file": os.path.relpath(
-            TRAFFIC_FILE, ROOT), "note": "FETCH_SIZE + WRITE_SIZE passes; stale when the kernel sources changed "
-                                         "since (then roofline.traffic is null)"},
+                     "kernel": KERNEL_NAMES[fp32_kernel], "kernel_ms": eval_avg_ms},
+        "traffic_measurement": {"bytes_per_launch": traffic, "stale": traffic_stale, "kernel": KERNEL_NAMES[fp32_kernel],
+                                "file": os.path.relpath(TRAFFIC_FILE, ROOT),
+                                "note": "FETCH_SIZE + WRITE_SIZE passes of this kernel; stale when the kernel sources "
+                                        "changed since (then roofline.traffic is null)"},
         "stages_ms": {"evaluate": st.evaluate_ms, "finalize": st.finalize_ms, "plan": st.plan_ms,
                       "assemble": st.assemble_ms, "bookkeeping": st.bookkeeping_ms},
         "generations_per_s": args.steps / fp32["elapsed"],
@@ -279,8 +289,8 @@ def main():
         "bf16_screened_fp32_rescored": bf16_out,
         "captured_step_graph": graph_out,
         "best_fitness": summary.best_fitness,
-        "launch_shapes": {("fp32" if m == rtm.EVAL_FP32 else "bf16") + ("_graph" if g else ""): v
-                          for (m, g), v in shapes.items()},
+        "launch_shapes": {("fp32" if m == rtm.EVAL_FP32 else "bf16") + ("_graph" if g else ""):
+                          {k: x for k, x in v.items() if k != "kernel_id"} for (m, g), v in shapes.items()},
         "cpu_baseline": cpu,
     }
     if not args.no_wallclock:

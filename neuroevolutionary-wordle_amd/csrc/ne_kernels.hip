@@ -1493,7 +1493,7 @@ __device__ __forceinline__ void FoldPrefill(Rsrc acc_r, const TrunkLane& tl, Rsr
 // it), so no register holds them during layer 1.
 constexpr uint32_t kTrunkBiasOff = 9216;  // byte offset of the layer-2 biases in that chunk's buffer
 static_assert(kTrunkBiasOff >= kTrunkReadBytes && kTrunkBiasOff % 16 == 0, "behind every layer-2 read");
-template <typename FN>
+template <bool kTwo, typename FN>
 __device__ __forceinline__ void TrunkStreamPair(Rsrc acc_r, const TrunkLane& tl,
                                                 float (&pol0)[kScoreSteps], float (&pol1)[kScoreSteps], bool compute,
                                                 uint8_t* b0, uint8_t* b1, uint32_t wave, uint32_t waves, uint32_t lane,
@@ -1507,7 +1507,7 @@ __device__ __forceinline__ void TrunkStreamPair(Rsrc acc_r, const TrunkLane& tl,
         h1b[u] = h1a[u];
     }
     v4f xa = AccLoad(acc_r, lane, 0, 0);
-    v4f xb = AccLoad(acc_r, lane, 1, 0);
+    v4f xb = kTwo ? AccLoad(acc_r, lane, 1, 0) : xa;
     WaitChunk();
     __syncthreads();
     const uint32_t rd1 = 64 * tl.j + 16 * tl.gq;
@@ -1519,7 +1519,7 @@ __device__ __forceinline__ void TrunkStreamPair(Rsrc acc_r, const TrunkLane& tl,
         v4f na = xa, nb = xb;
         if (t + 1 < kTrunkW1Chunks) {
             na = AccLoad(acc_r, lane, 0, t + 1);
-            nb = AccLoad(acc_r, lane, 1, t + 1);
+            if (kTwo) nb = AccLoad(acc_r, lane, 1, t + 1);
         } else if (wave == 0) {  // the layer-2 biases, B2[0..64), behind the first layer-2 chunk
             __builtin_amdgcn_raw_ptr_buffer_load_lds(tl.gr, (LdsPtr)(nxt + kTrunkBiasOff), 4, 4 * lane,
                                                      uint32_t(kOffB2) * 4, 0, 0);
@@ -1537,7 +1537,7 @@ __device__ __forceinline__ void TrunkStreamPair(Rsrc acc_r, const TrunkLane& tl,
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     h1a[u] = Mfma(w[r], ha[r], h1a[u]);
-                    h1b[u] = Mfma(w[r], hb[r], h1b[u]);
+                    if (kTwo) h1b[u] = Mfma(w[r], hb[r], h1b[u]);
                 }
             }
         }
@@ -1590,7 +1590,7 @@ __device__ __forceinline__ void TrunkStreamPair(Rsrc acc_r, const TrunkLane& tl,
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         pva[u] = Mfma(w[r], ha[r], pva[u]);
-                        pvb[u] = Mfma(w[r], hb[r], pvb[u]);
+                        if (kTwo) pvb[u] = Mfma(w[r], hb[r], pvb[u]);
                     }
                 }
             }
@@ -1648,8 +1648,9 @@ __device__ __forceinline__ void FinishTile(const EvalArgs& p, uint32_t n, bool f
     }
 }
 
-__global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void EvaluatePairKernel(EvalArgs p) {
-    constexpr int kCT = kChunkTiles;
+template <int kTiles, int kCT>
+__device__ __forceinline__ void EvaluateStreamed(const EvalArgs& p) {
+    constexpr bool kTwo = kTiles == 2;
     using G = ChunkGeom<kCT>;
     static_assert(kTrunkChunkBytes <= G::kBufBytes && kTrunkBiasOff + 4 * 68 <= G::kBufBytes, "trunk chunk");
     __shared__ __align__(16) uint8_t s_buf0[G::kBufBytes];
@@ -1664,7 +1665,7 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
     const Rsrc acc_r = MakeRsrc(p.acc_scratch + (uint64_t(blockIdx.x) * kEvalMaxWaves + wave) * 2 * kAccTileFloats,
                                 2 * kAccTileFloats * sizeof(float));
     // Persistent: (genome, split) items from the work list; split s of genome n plays tiles
-    // 2 (s waves + r splits waves + wave) + {0, 1} (r = 0, 1, ...).
+    // kTiles (s waves + r splits waves + wave) + q, q < kTiles (r = 0, 1, ...).
     for (;;) {
         __syncthreads();  // the previous item is done with the LDS buffers
         if (tid == 0) s_item = atomicAdd(p.work_counter, 1u);
@@ -1697,30 +1698,30 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
         auto issue = [&](uint8_t* buf, uint32_t c) __device__ {
             IssueChunk(buf, img_r, tail_r, c, wave, waves, lane, G::kImgBytes, G::kTailBytes, G::kImgDma, G::kTailDma);
         };
-        const uint32_t per_round = p.splits * waves * 2;
-        const uint32_t first_tile = split * waves * 2;
+        const uint32_t per_round = p.splits * waves * kTiles;
+        const uint32_t first_tile = split * waves * kTiles;
         const uint32_t rounds = tiles > first_tile ? (tiles - first_tile + per_round - 1) / per_round : 0u;
         for (uint32_t round = 0; round < rounds; ++round) {
-            const uint32_t tile0 = first_tile + round * per_round + 2 * wave, tile1 = tile0 + 1;
-            const bool fresh0 = tile0 < tiles_fresh, fresh1 = tile1 < tiles_fresh;
+            const uint32_t tile0 = first_tile + round * per_round + kTiles * wave, tile1 = kTwo ? tile0 + 1 : tiles;
+            const bool fresh0 = tile0 < tiles_fresh, fresh1 = kTwo && tile1 < tiles_fresh;
             GameLane g0, g1;
             InitGameLane(p, local, W, tiles_fresh, tiles, tile0, j, gq, g0);
             InitGameLane(p, local, W, tiles_fresh, tiles, tile1, j, gq, g1);
             {
                 const bool pre0 = !fresh0 && __any(g0.active ? 1 : 0) != 0;
-                const bool pre1 = !fresh1 && __any(g1.active ? 1 : 0) != 0;
+                const bool pre1 = kTwo && !fresh1 && __any(g1.active ? 1 : 0) != 0;
                 if (pre0 || pre1) FoldPrefill(acc_r, tl, enc_r, g0, g1, pre0, pre1, lane);
             }
             for (int step = 0; step < kMaxTurns; ++step) {
-                if (!__syncthreads_or((g0.active || g1.active) ? 1 : 0)) break;  // the workgroup leaves together
+                if (!__syncthreads_or((g0.active || (kTwo && g1.active)) ? 1 : 0)) break;  // the workgroup leaves together
                 const bool act0 = __any(g0.active ? 1 : 0) != 0;
-                const bool act1 = __any(g1.active ? 1 : 0) != 0;
+                const bool act1 = kTwo && __any(g1.active ? 1 : 0) != 0;
                 IssueTrunkChunk(s_buf0, tl.gr, 0, wave, waves, TrunkLaneW1(lane), TrunkLaneW2(lane));
                 FoldPair(acc_r, tl, enc_r, g0, g1, act0, act1, fresh0, fresh1, step, lane);
                 const bool compute = act0 || act1;
                 float pol0[kScoreSteps], pol1[kScoreSteps];
-                TrunkStreamPair(acc_r, tl, pol0, pol1, compute, s_buf0, s_buf1, wave, waves, lane,
-                                [&](uint8_t* b) __device__ { issue(b, 0); });
+                TrunkStreamPair<kTwo>(acc_r, tl, pol0, pol1, compute, s_buf0, s_buf1, wave, waves, lane,
+                                      [&](uint8_t* b) __device__ { issue(b, 0); });
                 if (!act0) {
 #pragma unroll
                     for (int s = 0; s < kScoreSteps; ++s) pol0[s] = 0.0f;
@@ -1734,7 +1735,7 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
                 ArgmaxLane am1 = am0;
                 StreamChunksWhole(s_buf0, s_buf1, chunks, compute, issue, [&](const uint8_t* buf, uint32_t c) __device__ {
                     ShareArgmaxGate(am0);
-                    ShareArgmaxGate(am1);
+                    if (kTwo) ShareArgmaxGate(am1);
                     float ev[kScoreSteps];
                     TileOperands(Fp32Img<kCT>(buf, 0), Fp32Tails<kCT>(buf, 0), lane, j, gq, ev);
 #pragma unroll
@@ -1745,12 +1746,12 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
 #pragma unroll
                         for (int s = 0; s < kScoreSteps; ++s) {
                             s0 = Mfma(ev[s], pol0[s], s0);
-                            s1 = Mfma(ev[s], pol1[s], s1);
+                            if (kTwo) s1 = Mfma(ev[s], pol1[s], s1);
                         }
                         const uint32_t at = c * uint32_t(kCT) + uint32_t(k);
                         const uint32_t* cn = Fp32Img<kCT>(buf, k) + 512 + 4 * gq;
                         ArgmaxUpdate<false>(s0, cn, at * 16 + 4 * gq, A, g0.hist, am0);
-                        ArgmaxUpdate<false>(s1, cn, at * 16 + 4 * gq, A, g1.hist, am1);
+                        if (kTwo) ArgmaxUpdate<false>(s1, cn, at * 16 + 4 * gq, A, g1.hist, am1);
                         if (k + 1 < kCT) {
 #pragma unroll
                             for (int s = 0; s < kScoreSteps; ++s) ev[s] = nx[s];
@@ -1758,14 +1759,34 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
                     }
                 });
                 ReduceArgmax(am0.best, am0.second, am0.besta);
-                ReduceArgmax(am1.best, am1.second, am1.besta);
                 ApplyGuess(p, am0.besta, gq, g0);
-                ApplyGuess(p, am1.besta, gq, g1);
+                if (kTwo) {
+                    ReduceArgmax(am1.best, am1.second, am1.besta);
+                    ApplyGuess(p, am1.besta, gq, g1);
+                }
             }
             FinishTile(p, n, fresh0, tile0 < tiles, lane, gq, g0);
-            FinishTile(p, n, fresh1, tile1 < tiles, lane, gq, g1);
+            if (kTwo) FinishTile(p, n, fresh1, tile1 < tiles, lane, gq, g1);
         }
     }
+}
+
+__global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void EvaluatePairKernel(EvalArgs p) {
+    EvaluateStreamed<2, kChunkTiles>(p);
+}
+
+// One game tile per wave with the streamed accumulators: 128 VGPRs (4 waves per SIMD) and chunks of 3 action
+// tiles (31 KB of LDS), so five 3-wave workgroups share a CU instead of four.  Measured at 64x64 / 4,739
+// actions against EvaluateKernel: 14.66 vs 14.87 ms; ragged 64x64: 201 vs 213.5 ms.  Tried and slower: 2 or
+// 4 tiles per chunk, 5 waves per SIMD.
+#ifndef NE_STREAMED_WAVES_PER_EU
+#define NE_STREAMED_WAVES_PER_EU 4
+#endif
+#ifndef NE_STREAMED_CHUNK_TILES
+#define NE_STREAMED_CHUNK_TILES 3
+#endif
+__global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves_per_eu(NE_STREAMED_WAVES_PER_EU, NE_STREAMED_WAVES_PER_EU))) void EvaluateStreamedKernel(EvalArgs p) {
+    EvaluateStreamed<1, NE_STREAMED_CHUNK_TILES>(p);
 }
 
 // ================================================================================================
@@ -2804,12 +2825,16 @@ void LaunchTransposeEncoder(const uint8_t* slab, uint64_t stride, const uint32_t
     hipLaunchKernelGGL(TransposeEncoderKernel, dim3(n), dim3(256), 0, s, slab, stride, slots, slot_count, enc_t);
 }
 
-void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, uint32_t tiles_per_wave, bool trace, bool bf16,
+void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, uint32_t kernel, bool trace, bool bf16,
                     hipStream_t s) {
     const dim3 grid(blocks);
     const dim3 block(64 * waves);
-    if (tiles_per_wave == 2 && !trace && !bf16) {
+    if (!trace && !bf16 && kernel == kEvalKernelPaired) {
         hipLaunchKernelGGL(EvaluatePairKernel, grid, block, 0, s, a);
+        return;
+    }
+    if (!trace && !bf16 && kernel == kEvalKernelStreamed) {
+        hipLaunchKernelGGL(EvaluateStreamedKernel, grid, block, 0, s, a);
         return;
     }
     if (bf16) {
@@ -2829,9 +2854,11 @@ uint32_t EvaluateMaxWaves() { return kEvalMaxWaves; }
 
 uint64_t EvaluateAccScratchBytes() { return uint64_t(kEvalMaxWaves) * 2 * kAccTileFloats * sizeof(float); }
 
-uint32_t EvaluateResidentPerCu(uint32_t waves, uint32_t tiles_per_wave, bool trace, bool bf16) {
+uint32_t EvaluateResidentPerCu(uint32_t waves, uint32_t kernel, bool trace, bool bf16) {
     // what the registers and LDS of the kernel built here allow, as the runtime reports it
-    const void* k = (tiles_per_wave == 2 && !trace && !bf16) ? reinterpret_cast<const void*>(&EvaluatePairKernel)
+    const bool plain = !trace && !bf16;
+    const void* k = (plain && kernel == kEvalKernelPaired) ? reinterpret_cast<const void*>(&EvaluatePairKernel)
+                  : (plain && kernel == kEvalKernelStreamed) ? reinterpret_cast<const void*>(&EvaluateStreamedKernel)
                   : bf16 ? (trace ? reinterpret_cast<const void*>(&EvaluateKernel<true, true>)
                                   : reinterpret_cast<const void*>(&EvaluateKernel<false, true>))
                          : (trace ? reinterpret_cast<const void*>(&EvaluateKernel<true, false>)

@@ -239,11 +239,14 @@ void LaunchPackBf16(const uint8_t* slab, uint64_t stride, const uint32_t* slots,
 void LaunchTransposeEncoder(const uint8_t* slab, uint64_t stride, const uint32_t* slots, uint32_t n,
                             uint32_t slot_count, float* enc_t, hipStream_t s);
 // Persistent evaluation: `blocks` workgroups of `waves` waves drain the work list.
-// tiles_per_wave 2 selects the two-tiles-per-wave kernel (untraced FP32 form only)
-void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, uint32_t tiles_per_wave, bool trace, bool bf16,
+// The evaluation kernels of the untraced FP32 form (identical results); the other forms always use the first.
+constexpr uint32_t kEvalKernelRegisters = 0;  // one tile per wave, layer-0 accumulators in registers
+constexpr uint32_t kEvalKernelStreamed = 1;   // one tile per wave, accumulators in scratch, 4 waves per SIMD
+constexpr uint32_t kEvalKernelPaired = 2;     // two tiles per wave, accumulators in scratch
+void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, uint32_t kernel, bool trace, bool bf16,
                     hipStream_t s);
 // Workgroups of `waves` waves of the evaluation kernel that fit on one CU at once (0 if unknown).
-uint32_t EvaluateResidentPerCu(uint32_t waves, uint32_t tiles_per_wave, bool trace, bool bf16);
+uint32_t EvaluateResidentPerCu(uint32_t waves, uint32_t kernel, bool trace, bool bf16);
 // bytes of layer-0 accumulator scratch one evaluation workgroup uses (either kernel)
 uint64_t EvaluateAccScratchBytes();
 // The most waves per evaluation workgroup this build supports (its LDS tables are sized for them).

@@ -37,7 +37,7 @@ struct StageEvents {
 // One evaluation's launch description: the kernel arguments, the workgroup shape and the games played.
 struct EvalSetup {
     EvalArgs a;
-    uint32_t blocks, waves, tiles_per_wave;
+    uint32_t blocks, waves, kernel;
     bool bf16, trace;
     uint64_t total_games;
 };
@@ -167,7 +167,7 @@ struct ne_runtime {
     uint32_t tuned_waves = 0;           // 0: the idle-slot heuristic picks the workgroup size
     uint32_t tuned_rounds = 1;          // rounds of `waves` tiles one work item plays
     uint32_t tuned_wg_per_cu = 0;       // persistent workgroups per CU (0: as many as fit)
-    uint32_t tuned_tpw = 1;             // game tiles per wave (2: the two-tiles-per-wave FP32 kernel)
+    uint32_t tuned_kernel = 0;          // evaluation kernel of the untraced FP32 form (kEvalKernel*)
     float* eval_acc = nullptr;          // the evaluation's layer-0 scratch, EvaluateAccScratchBytes() per workgroup
     uint64_t eval_acc_bytes = 0;
 
@@ -539,9 +539,10 @@ static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32
     a.game_trace = trace ? rt->d_trace : nullptr;
     a.policy_trace = (rt->trace_flags & NE_TRACE_POLICY) ? rt->d_policy_trace : nullptr;
     a.game_offsets = trace ? rt->d_game_offsets : nullptr;
-    // Game tiles per wave: 2 runs the two-tiles-per-wave kernel (untraced FP32 form only), which plays a
-    // workgroup's 2 x waves tiles with one operand stream per decision.
-    const uint32_t tpw = (rt->tuned_tpw == 2 && !trace && !bf16) ? 2u : 1u;
+    // Kernel (untraced FP32 form only): the paired kernel plays two game tiles per wave, a workgroup's
+    // 2 x waves tiles with one operand stream per decision; the others one tile per wave.
+    const uint32_t kernel = (!trace && !bf16) ? rt->tuned_kernel : kEvalKernelRegisters;
+    const uint32_t tpw = kernel == kEvalKernelPaired ? 2u : 1u;
     // Waves per workgroup: 6-wave workgroups of this kernel's register footprint fit once per CU, 2-4
     // wave ones fill a CU (12 waves); pick the size that leaves the fewest idle tile slots over all
     // genomes (ties: the larger).  A genome's tiles then spread over ceil(max_tiles / (tpw x waves))
@@ -595,7 +596,7 @@ static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32
     a.splits = splits;
     // resident workgroups per CU, as the kernel's registers and LDS allow (3 waves per SIMD at 168 VGPRs
     // and 4 workgroups per CU by LDS in the FP32 form)
-    const uint32_t resident = EvaluateResidentPerCu(waves, tpw, trace, bf16);
+    const uint32_t resident = EvaluateResidentPerCu(waves, kernel, trace, bf16);
     uint64_t per_cu = resident != 0 ? resident : std::min<uint64_t>(12 / waves, 4);
     if (rt->tuned_wg_per_cu != 0) per_cu = std::min<uint64_t>(per_cu, rt->tuned_wg_per_cu);
     const uint32_t blocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(items, rt->cu_count * per_cu)));
@@ -630,7 +631,7 @@ static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32
     out->a = a;
     out->blocks = blocks;
     out->waves = waves;
-    out->tiles_per_wave = tpw;
+    out->kernel = kernel;
     out->bf16 = bf16;
     out->trace = trace;
     out->total_games = total_games;
@@ -658,7 +659,7 @@ static void LaunchEvaluation(ne_runtime* rt, const EvalSetup& s, uint64_t n, flo
                        s.a.action_count, rt->bf16_ops, s.a.bf16_stride, rt->tail_absmax, rt->stream);
     if (timed) EventEnd(rt, 8);
     if (timed) EventBegin(rt, 0);
-    LaunchEvaluate(s.a, s.blocks, s.waves, s.tiles_per_wave, s.trace, s.bf16, rt->stream);
+    LaunchEvaluate(s.a, s.blocks, s.waves, s.kernel, s.trace, s.bf16, rt->stream);
     if (timed) EventEnd(rt, 0);
 }
 
@@ -1894,7 +1895,7 @@ static int TryGraphStep(ne_runtime* rt, uint32_t seed, const ne_word_counts* cou
         const uint64_t v[10] = {rt->input_epoch, n, std::min(rt->action_count, counts->action_space_word_count),
                                 rt->eval_mode,
                                 rt->tuned_waves | (uint64_t(rt->tuned_rounds) << 16) | (uint64_t(rt->tuned_wg_per_cu) << 32) |
-                                    (uint64_t(rt->tuned_tpw) << 48),
+                                    (uint64_t(rt->tuned_kernel) << 48),
                                 uint64_t(uintptr_t(rt->slab)), rt->stride,
                                 rt->slot_count, rt->action_count, rt->max_gen};
         std::memcpy(k, v, sizeof(v));
@@ -2267,10 +2268,10 @@ int ne_autotune_evaluation(ne_runtime* rt, const ne_word_counts* counts, const n
     // coordinate search, one knob at a time, each candidate timed on one evaluation of the current generation
     // (results do not depend on the shape): waves per workgroup, then (untraced FP32 form) two game tiles
     // per wave at every workgroup size, then rounds per work item, then workgroups per CU
-    const uint32_t saved[4] = {rt->tuned_waves, rt->tuned_rounds, rt->tuned_wg_per_cu, rt->tuned_tpw};
-    auto time_shape = [&](uint32_t w, uint32_t tpw, uint32_t r, uint32_t g, float* ms) -> int {
+    const uint32_t saved[4] = {rt->tuned_waves, rt->tuned_rounds, rt->tuned_wg_per_cu, rt->tuned_kernel};
+    auto time_shape = [&](uint32_t w, uint32_t kern, uint32_t r, uint32_t g, float* ms) -> int {
         rt->tuned_waves = w;
-        rt->tuned_tpw = tpw;
+        rt->tuned_kernel = kern;
         rt->tuned_rounds = r;
         rt->tuned_wg_per_cu = g;
         rt->times = ne_stage_times{};
@@ -2279,17 +2280,17 @@ int ne_autotune_evaluation(ne_runtime* rt, const ne_word_counts* counts, const n
             rt->tuned_waves = saved[0];
             rt->tuned_rounds = saved[1];
             rt->tuned_wg_per_cu = saved[2];
-            rt->tuned_tpw = saved[3];
+            rt->tuned_kernel = saved[3];
             return st;
         }
         *ms = rt->times.evaluate_ms;
         return NE_OK;
     };
-    uint32_t bw = 0, bt = 1, br = 1, bg = 0;
+    uint32_t bw = 0, bt = kEvalKernelRegisters, br = 1, bg = 0;
     float best_ms = 0.0f;
     for (uint32_t w = 1; w <= 4; ++w) {
         float ms = 0.0f;
-        const int st = time_shape(w, 1, 1, 0, &ms);
+        const int st = time_shape(w, kEvalKernelRegisters, 1, 0, &ms);
         if (st != NE_OK) return st;
         if (timings_ms) timings_ms[w - 1] = ms;
         if (bw == 0 || ms < best_ms) {
@@ -2299,16 +2300,17 @@ int ne_autotune_evaluation(ne_runtime* rt, const ne_word_counts* counts, const n
     }
     const bool trace = rt->trace_flags != 0, bf16 = (rt->eval_mode & 0xFFu) == NE_EVAL_BF16_RESCORED;
     if (!trace && !bf16) {
-        for (uint32_t w = 1; w <= 4; ++w) {
-            float ms = 0.0f;
-            const int st = time_shape(w, 2, 1, 0, &ms);
-            if (st != NE_OK) return st;
-            if (ms < best_ms) {
-                bw = w;
-                bt = 2;
-                best_ms = ms;
+        for (const uint32_t kern : {kEvalKernelStreamed, kEvalKernelPaired})
+            for (uint32_t w = 1; w <= 4; ++w) {
+                float ms = 0.0f;
+                const int st = time_shape(w, kern, 1, 0, &ms);
+                if (st != NE_OK) return st;
+                if (ms < best_ms) {
+                    bw = w;
+                    bt = kern;
+                    best_ms = ms;
+                }
             }
-        }
     }
     for (uint32_t r = 2; r <= 4; r *= 2) {
         float ms = 0.0f;
@@ -2330,7 +2332,7 @@ int ne_autotune_evaluation(ne_runtime* rt, const ne_word_counts* counts, const n
         }
     }
     rt->tuned_waves = bw;
-    rt->tuned_tpw = bt;
+    rt->tuned_kernel = bt;
     rt->tuned_rounds = br;
     rt->tuned_wg_per_cu = bg;
     if (chosen_waves) *chosen_waves = bw;
@@ -2342,15 +2344,15 @@ int ne_get_eval_shape(ne_runtime* rt, ne_eval_shape* out) {
     out->waves = rt->tuned_waves;
     out->rounds_per_item = rt->tuned_rounds;
     out->workgroups_per_cu = rt->tuned_wg_per_cu;
-    out->tiles_per_wave = rt->tuned_tpw;
+    out->kernel = rt->tuned_kernel;
     return NE_OK;
 }
 
 int ne_set_eval_shape_ex(ne_runtime* rt, const ne_eval_shape* shape) {
     if (!rt || !shape || shape->waves > EvaluateMaxWaves() || shape->rounds_per_item > 64 ||
-        shape->workgroups_per_cu > 64 || shape->tiles_per_wave > 2)
+        shape->workgroups_per_cu > 64 || shape->kernel > kEvalKernelPaired)
         return Fail(rt, NE_INVALID_ARGUMENT);
-    rt->tuned_tpw = std::max<uint32_t>(1, shape->tiles_per_wave);
+    rt->tuned_kernel = shape->kernel;
     rt->tuned_waves = shape->waves;
     rt->tuned_rounds = std::max<uint32_t>(1, shape->rounds_per_item);
     rt->tuned_wg_per_cu = shape->workgroups_per_cu;

@@ -23,6 +23,8 @@ STATUS_NAMES = {
 }
 
 EVAL_FP32, EVAL_BF16_RESCORED, EVAL_FORCE_FALLBACK = 0, 1, 0x100
+# evaluation kernels of the untraced FP32 form (identical results; ne_eval_shape.kernel)
+EVAL_KERNEL_REGISTERS, EVAL_KERNEL_STREAMED, EVAL_KERNEL_PAIRED = 0, 1, 2
 TRACE_GAMES = 1
 TRACE_POLICY = 2
 TRACE_XOVER = 4
@@ -97,7 +99,7 @@ class StageTimes(C.Structure):
 
 class EvalShape(C.Structure):
     _fields_ = [("waves", C.c_uint32), ("rounds_per_item", C.c_uint32), ("workgroups_per_cu", C.c_uint32),
-                ("tiles_per_wave", C.c_uint32)]
+                ("kernel", C.c_uint32)]
 
 
 GAME_TRACE_DTYPE = np.dtype([("guesses", "<u4", 6), ("best", "<f4", 6), ("second", "<f4", 6),
@@ -610,9 +612,9 @@ class SlabRuntime:
         self._check("ne_get_eval_shape", self.lib.ne_get_eval_shape(self.h, C.byref(s)))
         return s
 
-    def set_eval_shape_ex(self, waves=0, rounds_per_item=1, workgroups_per_cu=0, tiles_per_wave=1):
+    def set_eval_shape_ex(self, waves=0, rounds_per_item=1, workgroups_per_cu=0, kernel=0):
         s = EvalShape(waves=waves, rounds_per_item=rounds_per_item, workgroups_per_cu=workgroups_per_cu,
-                      tiles_per_wave=tiles_per_wave)
+                      kernel=kernel)
         self._check("ne_set_eval_shape_ex", self.lib.ne_set_eval_shape_ex(self.h, C.byref(s)))
 
     def set_eval_shape(self, waves):

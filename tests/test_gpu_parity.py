@@ -414,40 +414,42 @@ def test_injection_of_a_word_without_hint_grids_fails(pkg, orc):
 
 def test_autotuned_shapes_give_identical_results(pkg, grid16):
     """Every launch shape of the persistent evaluation kernel -- waves per workgroup, rounds of tiles per work
-    item, workgroups per CU, one or two game tiles per wave (the second kernel) -- yields the same fitness and
-    counters; the autotuner keeps the fastest and later evaluations use it."""
+    item, workgroups per CU, and the evaluation kernel (accumulators in registers, streamed through scratch,
+    two tiles per wave) -- yields the same fitness and counters; the autotuner keeps the fastest and later
+    evaluations use it."""
     wl, rt = grid16
     counts = pkg.runtime.word_counts(512, 512)
     ref = None
-    for tpw in (1, 2):
+    for kern in (0, 1, 2):
         for w in (1, 2, 3, 4):
             for r in (1, 2, 4):
                 for g in (0, 1, 2):
-                    rt.set_eval_shape_ex(w, r, g, tpw)
+                    rt.set_eval_shape_ex(w, r, g, kern)
                     rt.evaluate(counts)
                     _, fit, evals, _ = rt.generation(0)
                     st = rt.stage_times()
                     got = (fit.view(np.uint32).copy(), evals.copy(), st.decisions, st.encoded_turns)
                     if ref is None:
                         ref = got
-                    assert all(np.array_equal(a, b) for a, b in zip(got, ref)), (tpw, w, r, g)
+                    assert all(np.array_equal(a, b) for a, b in zip(got, ref)), (kern, w, r, g)
     rt.set_eval_shape_ex(0, 1, 0)
     waves, times, shape = rt.autotune_evaluation(counts)
     assert waves in (1, 2, 3, 4) and all(t > 0 for t in times)
     assert shape.waves == waves and shape.rounds_per_item in (1, 2, 4) and shape.workgroups_per_cu in range(0, 64)
-    assert shape.tiles_per_wave in (1, 2)
-    if shape.tiles_per_wave == 1 and shape.rounds_per_item == 1 and shape.workgroups_per_cu == 0:
+    assert shape.kernel in (0, 1, 2)
+    if shape.kernel == 0 and shape.rounds_per_item == 1 and shape.workgroups_per_cu == 0:
         assert times[waves - 1] == min(times)
     _, fit, _, _ = rt.generation(0)
     assert np.array_equal(fit.view(np.uint32), ref[0])
     rt.set_eval_shape_ex(0, 1, 0)
 
 
-def test_two_tiles_per_wave_on_ragged_cells(pkg):
-    """The two-tiles-per-wave kernel against the one-tile kernel on cells of 4..70 secrets (partial tiles, odd
-    tile counts, fresh and prefilled tiles in one wave, waves whose second tile does not exist), over several
-    generations so the slab holds bred genomes: identical fitness, evaluation counts, decisions and encoded
-    turns for every shape.  The BF16-screened form ignores the knob and gives the same fitness as well."""
+def test_evaluation_kernels_agree_on_ragged_cells(pkg):
+    """The streamed and paired evaluation kernels against the register kernel on cells of 4..70 secrets
+    (partial tiles, odd tile counts, fresh and prefilled tiles in one wave, waves whose second tile does not
+    exist), over several generations so the slab holds bred genomes: identical fitness, evaluation counts,
+    decisions and encoded turns for every shape.  The BF16-screened form ignores the knob and gives the same
+    fitness as well."""
     rtm, synth = pkg.runtime, pkg.synth
     A = 700
     catalog = synth.make_catalog(A, seed=synth.CATALOG_SEED, distinct_letters=True)
@@ -462,16 +464,17 @@ def test_two_tiles_per_wave_on_ragged_cells(pkg):
     cfg = rtm.default_assembly_config(mutation_probability=0.01, mutation_sigma=0.02)
     for gen in range(3):
         ref = None
-        for tpw, w, r in ((1, 0, 1), (2, 0, 1), (2, 1, 1), (2, 2, 2), (2, 3, 1), (2, 4, 4)):
+        for kern, w, r in ((0, 0, 1), (2, 0, 1), (2, 1, 1), (2, 2, 2), (2, 3, 1), (2, 4, 4), (1, 0, 1), (1, 1, 1),
+                           (1, 3, 2), (1, 4, 4)):
             rt.set_eval_precision(rtm.EVAL_FP32)
-            rt.set_eval_shape_ex(w, r, 0, tpw)
+            rt.set_eval_shape_ex(w, r, 0, kern)
             rt.evaluate(counts)
             _, fit, evals, _ = rt.generation(0)
             st = rt.stage_times()
             got = (fit.view(np.uint32).copy(), evals.copy(), st.decisions, st.encoded_turns)
             if ref is None:
                 ref = got
-            assert all(np.array_equal(a, b) for a, b in zip(got, ref)), (gen, tpw, w, r)
+            assert all(np.array_equal(a, b) for a, b in zip(got, ref)), (gen, kern, w, r)
         rt.set_eval_precision(rtm.EVAL_BF16_RESCORED)
         rt.evaluate(counts)
         assert np.array_equal(rt.generation(0)[1].view(np.uint32), ref[0])
