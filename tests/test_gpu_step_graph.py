@@ -50,8 +50,11 @@ def _compare(a, b, where):
         assert a[k] == b[k], (where, k)
 
 
-@pytest.mark.parametrize("mode_name", ["EVAL_FP32", "EVAL_BF16_RESCORED"])
-def test_graph_steps_equal_eager_steps(pkg, mode_name):
+@pytest.mark.parametrize("mode_name,kernel", [("EVAL_FP32", 0), ("EVAL_BF16_RESCORED", 0), ("EVAL_FP32", 1),
+                                              ("EVAL_FP32", 2)])
+def test_graph_steps_equal_eager_steps(pkg, mode_name, kernel):
+    """The captured runtime uses the given evaluation kernel (registers, streamed, paired), the eager one the
+    register kernel."""
     rtm = pkg.runtime
     mode = getattr(rtm, mode_name)
     wl = pkg.synth.workload("grid16_a512")
@@ -60,6 +63,7 @@ def test_graph_steps_equal_eager_steps(pkg, mode_name):
     cfg2 = rtm.default_assembly_config(mutation_probability=0.05, mutation_sigma=0.01)
     eager = _make(pkg, wl, mode)
     graph = _make(pkg, wl, mode, graph=True)
+    graph.set_eval_shape_ex(0, 1, 0, kernel)
     try:
         kinds = []
         for g in range(6):
@@ -107,13 +111,15 @@ def test_step_without_room_runs_eagerly(pkg):
 
 
 def test_graph_step_at_full_size(pkg):
-    """64x64 genomes, 4,739 actions: graph and eager steps agree, and the replay is timed."""
+    """64x64 genomes, 4,739 actions: graph steps with the streamed kernel (the autotuner's choice there) and
+    eager steps with the register kernel agree, and the replay is timed."""
     rtm = pkg.runtime
     wl = pkg.synth.workload("grid64_a4739")
     counts = rtm.word_counts(wl["actions"], wl["actions"])
     cfg = rtm.default_assembly_config(mutation_probability=0.01, mutation_sigma=0.02)
     eager = _make(pkg, wl, rtm.EVAL_FP32)
     graph = _make(pkg, wl, rtm.EVAL_FP32, graph=True)
+    graph.set_eval_shape_ex(3, 1, 0, rtm.EVAL_KERNEL_STREAMED)
     try:
         for g in range(3):
             eager.advance(9 + g, counts, cfg)
