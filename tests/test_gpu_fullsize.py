@@ -3,8 +3,8 @@ shards of 32 -> 393,216 games, one evaluation of generation 0 in the FP32 form a
 form.
 
 Bar: over every game of the grid, the BF16 form takes the FP32 form's guesses with chosen scores equal
-bit for bit and identical fitness, and the untraced kernels (the ones timed) give the traced kernels'
-fitness bit for bit; on the first 1,024 genomes (98,304 games) the FP32 form agrees with the oracle's
+bit for bit and identical fitness, and the untraced kernels (the ones timed: three FP32 kernels and the BF16
+form) give the traced kernels' fitness bit for bit; on the first 1,024 genomes (98,304 games) the FP32 form agrees with the oracle's
 FP32 evaluation on >= 99.9% of the decisions taken on a common trajectory, and every genome without a
 near-tie decision has the oracle's fitness bit for bit; on 128 genomes spread over the grid (12,288
 games), every FP32 policy vector lies within 1e-5 (normwise) of the host fp64 forward on the state the GPU
@@ -39,11 +39,18 @@ def fullsize(pkg):
     games = 3 * int(wl["offsets"][-1])
     counts = rtm.word_counts(A, A)
     out = {}
-    for form, mode in (("fp32_untraced", rtm.EVAL_FP32), ("bf16_untraced", rtm.EVAL_BF16_RESCORED)):
+    # untraced: the BF16 form and the three FP32 kernels (the streamed one is what the autotuner picks and
+    # the bench times here)
+    for form, mode, kernel in (("fp32_untraced", rtm.EVAL_FP32, rtm.EVAL_KERNEL_REGISTERS),
+                               ("fp32_streamed_untraced", rtm.EVAL_FP32, rtm.EVAL_KERNEL_STREAMED),
+                               ("fp32_paired_untraced", rtm.EVAL_FP32, rtm.EVAL_KERNEL_PAIRED),
+                               ("bf16_untraced", rtm.EVAL_BF16_RESCORED, rtm.EVAL_KERNEL_REGISTERS)):
         rt.set_eval_precision(mode)
         rt.set_trace(0)
+        rt.set_eval_shape_ex(3, 1, 0, kernel)
         rt.evaluate(counts)
         out[form] = dict(fitness=rt.generation(0)[1].copy(), decisions=int(rt.stage_times().decisions))
+    rt.set_eval_shape_ex(0, 1, 0, rtm.EVAL_KERNEL_REGISTERS)
     for form, mode, trace in (("fp32", rtm.EVAL_FP32, rtm.TRACE_GAMES | rtm.TRACE_POLICY),
                               ("bf16", rtm.EVAL_BF16_RESCORED, rtm.TRACE_GAMES)):
         rt.set_eval_precision(mode)
@@ -80,10 +87,13 @@ def test_bf16_form_takes_every_fp32_decision_at_full_size(fullsize):
 
 
 def test_untraced_kernels_equal_traced_ones_at_full_size(fullsize):
+    """Every untraced kernel (the timed ones: the register, streamed and paired FP32 kernels and the BF16
+    form) gives the traced kernel's fitness bit for bit and its decision count."""
     _, _, out = fullsize
-    for form in ("fp32", "bf16"):
-        assert np.array_equal(out[form + "_untraced"]["fitness"].view(np.uint32), out[form]["fitness"].view(np.uint32))
-        assert out[form + "_untraced"]["decisions"] == out[form]["decisions"]
+    for form, traced in (("fp32", "fp32"), ("fp32_streamed", "fp32"), ("fp32_paired", "fp32"), ("bf16", "bf16")):
+        assert np.array_equal(out[form + "_untraced"]["fitness"].view(np.uint32),
+                              out[traced]["fitness"].view(np.uint32)), form
+        assert out[form + "_untraced"]["decisions"] == out[traced]["decisions"], form
 
 
 def test_fp32_form_agrees_with_the_oracle_on_1024_genomes(fullsize, orc):
