@@ -817,6 +817,21 @@ constexpr uint32_t kMaxCandidates = 32;  // per game; more within the window -> 
 // 68 FP32 fallbacks per evaluation at 64x64 / 4,739 actions, 11.8 instead of 9.5 ms.
 constexpr uint32_t kLaneCandidates = 8;
 
+// The next (genome, split) item of a persistent evaluation workgroup (called by one thread): from the
+// queue of its own XCD while that lasts, then from the other queues in turn; `q` and `tried` carry the
+// queue being drained between calls.  A genome's items sit together in one queue, so the workgroups
+// playing its splits run side by side on one XCD and share its L2 copy of the genome's weights and
+// embedding rows.  Returns work_count once every queue is empty.
+__device__ __forceinline__ uint32_t TakeItem(const EvalArgs& p, uint32_t& q, uint32_t& tried) {
+    while (tried < kWorkQueues) {
+        const uint32_t item = p.queue_begin[q] + atomicAdd(p.work_counter + q * kWorkCounterStride, 1u);
+        if (item < p.queue_begin[q + 1]) return item;
+        q = q + 1 == kWorkQueues ? 0u : q + 1;
+        ++tried;
+    }
+    return p.work_count;
+}
+
 template <bool kTrace, bool kBf16>
 __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void EvaluateKernel(EvalArgs p) {
     constexpr int kCT = kBf16 ? kChunkTilesBf16 : kChunkTiles;  // FP32 stream (the BF16 form's fallback)
@@ -826,12 +841,13 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
     __shared__ __align__(16) uint8_t s_buf0[kBufBytes];
     __shared__ __align__(16) uint8_t s_buf1[kBufBytes];
 
-    // Persistent: the workgroup takes (genome, split) items from the work list until it is empty; split s
-    // of genome n plays tiles s*waves + r*splits*waves + wave (r = 0, 1, ...) of n's game tiles.
+    // Persistent: the workgroup takes (genome, split) items from the work queues until they are empty;
+    // split s of genome n plays tiles s*waves + r*splits*waves + wave (r = 0, 1, ...) of n's game tiles.
     __shared__ uint32_t s_item;
+    uint32_t queue = blockIdx.x % kWorkQueues, tried = 0;
     for (;;) {
         __syncthreads();  // the previous item is done with the LDS buffers
-        if (threadIdx.x == 0) s_item = atomicAdd(p.work_counter, 1u);
+        if (threadIdx.x == 0) s_item = TakeItem(p, queue, tried);
         __syncthreads();
         const uint32_t item = s_item;
         if (item >= p.work_count) break;
@@ -1664,11 +1680,12 @@ __device__ __forceinline__ void EvaluateStreamed(const EvalArgs& p) {
     const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const Rsrc acc_r = MakeRsrc(p.acc_scratch + (uint64_t(blockIdx.x) * kEvalMaxWaves + wave) * 2 * kAccTileFloats,
                                 2 * kAccTileFloats * sizeof(float));
-    // Persistent: (genome, split) items from the work list; split s of genome n plays tiles
+    // Persistent: (genome, split) items from the work queues; split s of genome n plays tiles
     // kTiles (s waves + r splits waves + wave) + q, q < kTiles (r = 0, 1, ...).
+    uint32_t queue = blockIdx.x % kWorkQueues, tried = 0;
     for (;;) {
         __syncthreads();  // the previous item is done with the LDS buffers
-        if (tid == 0) s_item = atomicAdd(p.work_counter, 1u);
+        if (tid == 0) s_item = TakeItem(p, queue, tried);
         __syncthreads();
         const uint32_t item = s_item;
         if (item >= p.work_count) break;

@@ -16,6 +16,11 @@ struct SummaryRecord {
     uint32_t best_slot;
 };
 
+// The persistent evaluation's work queues: one per XCD of the MI355X (workgroups are dispatched to the 8
+// XCDs round-robin, so workgroup b runs on XCD b mod 8), each queue's counter on its own 128-byte line.
+constexpr uint32_t kWorkQueues = 8;
+constexpr uint32_t kWorkCounterStride = 32;
+
 struct EvalArgs {
     const uint8_t* slab;
     uint64_t stride;
@@ -44,9 +49,10 @@ struct EvalArgs {
     GameTrace* game_trace;         // optional (trace mode)
     float* policy_trace;           // optional: [games][6][64]
     const uint64_t* game_offsets;  // [N] first game index of each individual (trace mode)
-    const uint32_t* work;          // [work_count][2] (genome, split) items, taken in order
+    const uint32_t* work;          // [work_count][2] (genome, split) items in kWorkQueues queues
+    const uint32_t* queue_begin;   // [kWorkQueues + 1]: queue q holds items queue_begin[q] .. queue_begin[q + 1] - 1
     uint32_t work_count;
-    uint32_t* work_counter;        // next item (0 at launch)
+    uint32_t* work_counter;        // [kWorkQueues][kWorkCounterStride]: next item of each queue (0 at launch)
     uint32_t splits;               // the largest split count of a genome
     float* acc_scratch;            // [workgroup][kEvalMaxWaves][2][16 KB] layer-0 accumulators between decisions
 };

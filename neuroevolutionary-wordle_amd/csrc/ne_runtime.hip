@@ -564,33 +564,52 @@ static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32
         }
     }
     if (rt->tuned_waves != 0) waves = rt->tuned_waves;  // the autotuner's choice (ne_autotune_evaluation)
-    // Work list for the persistent kernel: split-major (every genome's first tpw x waves tiles, then the
-    // next ...), genomes by descending tile count within a split, so no workgroup slot is spent on an
-    // empty split and the longest genomes start first.  An item plays `rounds` rounds of tpw x waves
-    // tiles, so a genome's tiles spread over ceil(tiles / (rounds x tpw x waves)) items.
+    // Work list for the persistent kernel, in one queue per XCD (a workgroup drains its own XCD's queue,
+    // then helps with the others): genomes by descending tile count, dealt round-robin over the queues, so
+    // the longest genomes start first and the queues carry equal work.  Within a queue the FP32 form goes
+    // split-major (every genome's first split, then the second ...: the longest items first), the BF16
+    // form genome-major (a genome's splits one after another, so they run side by side on one XCD and
+    // read its screening operands through one L2: 6% less HBM traffic, 64x64 / 4,739 evaluation 9.4 ->
+    // 8.9-9.1 ms; the same order makes the FP32 streamed kernel ~2% slower).  Neither skips an empty
+    // split.  An item plays `rounds` rounds of tpw x waves tiles, so a genome's tiles spread over
+    // ceil(tiles / (rounds x tpw x waves)) items.  Layout: the kWorkQueues + 1 queue bounds, then the
+    // items from word kWorkHeader on.
     const uint32_t rounds = std::max<uint32_t>(1, rt->tuned_rounds);
     const uint64_t per_item = uint64_t(waves) * tpw;
     const uint32_t splits = uint32_t((max_tiles + per_item * rounds - 1) / (per_item * rounds));
     std::vector<uint32_t> order(n);
     for (uint64_t c = 0; c < n; ++c) order[c] = uint32_t(c);
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return tiles[x] > tiles[y]; });
-    std::vector<uint32_t> work;
-    for (uint32_t sp = 0; sp < splits; ++sp)
-        for (const uint32_t g : order) {
-            if (uint64_t(sp) * per_item >= tiles[g]) break;
-            work.push_back(g);
-            work.push_back(sp);
+    constexpr uint64_t kWorkHeader = 16;
+    static_assert(kWorkQueues + 1 <= kWorkHeader, "queue bounds");
+    std::vector<uint32_t> work(kWorkHeader, 0u);
+    auto push = [&](uint32_t g, uint32_t sp) {
+        if (uint64_t(sp) * per_item >= tiles[g]) return;
+        work.push_back(g);
+        work.push_back(sp);
+    };
+    for (uint32_t q = 0; q < kWorkQueues; ++q) {
+        work[q] = uint32_t((work.size() - kWorkHeader) / 2);
+        if (bf16) {
+            for (uint64_t r = q; r < n; r += kWorkQueues)
+                for (uint32_t sp = 0; sp < splits; ++sp) push(order[r], sp);
+        } else {
+            for (uint32_t sp = 0; sp < splits; ++sp)
+                for (uint64_t r = q; r < n; r += kWorkQueues) push(order[r], sp);
         }
-    const uint64_t items = work.size() / 2;
-    if (items > rt->work_cap) {
+    }
+    const uint64_t items = (work.size() - kWorkHeader) / 2;
+    work[kWorkQueues] = uint32_t(items);
+    if (items > rt->work_cap || rt->d_work == nullptr) {
         Free(rt->d_work);
         rt->work_cap = 0;
-        if (!Alloc(&rt->d_work, 2 * items)) return Fail(rt, NE_DEVICE_FAILURE);
+        if (!Alloc(&rt->d_work, kWorkHeader + 2 * items)) return Fail(rt, NE_DEVICE_FAILURE);
         rt->work_cap = items;
     }
     if (!Ok(hipMemcpyAsync(rt->d_work, work.data(), work.size() * 4, hipMemcpyHostToDevice, rt->stream)))
         return Fail(rt, NE_DEVICE_FAILURE);
-    a.work = rt->d_work;
+    a.queue_begin = rt->d_work;
+    a.work = rt->d_work + kWorkHeader;
     a.work_count = uint32_t(items);
     a.work_counter = rt->d_work_counter;
     a.splits = splits;
@@ -649,7 +668,7 @@ static void LaunchEvaluation(ne_runtime* rt, const EvalSetup& s, uint64_t n, flo
     LaunchFillU32(rt->local_w, 0u, rt->max_gen, rt->stream);
     LaunchFillU32(reinterpret_cast<uint32_t*>(rt->counters), 0u, 8, rt->stream);
     LaunchFillU32(reinterpret_cast<uint32_t*>(rt->status), 0u, 1, rt->stream);
-    LaunchFillU32(rt->d_work_counter, 0u, 1, rt->stream);
+    LaunchFillU32(rt->d_work_counter, 0u, kWorkQueues * kWorkCounterStride, rt->stream);
     if (s.bf16) LaunchFillU32(rt->tail_absmax, 0u, n * 40, rt->stream);
     if (timed) EventBegin(rt, 8);
     LaunchTransposeEncoder(rt->slab, rt->stride, s.a.slot_indices, uint32_t(n), uint32_t(rt->slot_count), rt->enc_t,
@@ -1454,7 +1473,7 @@ static int CreateCore(const ne_runtime_config& c, uint64_t slots, uint64_t max_g
           Alloc(&rt->breaks, max_gen * kBreakStride);
     ok &= Alloc(&rt->raw, max_gen) && Alloc(&rt->local_w, max_gen) && Alloc(&rt->counters, 4) &&
           Alloc(&rt->status, 1) && Alloc(&rt->summary, 1) && Alloc(&rt->d_offsets, max_gen + 1) &&
-          Alloc(&rt->d_work_counter, 1);
+          Alloc(&rt->d_work_counter, kWorkQueues * kWorkCounterStride);
     for (auto& e : rt->ev) ok &= Ok(hipEventCreate(&e.begin)) && Ok(hipEventCreate(&e.end));
     if (ok) {
         ok &= Ok(hipMemsetAsync(rt->slab, 0, slots * rt->stride, rt->stream));
