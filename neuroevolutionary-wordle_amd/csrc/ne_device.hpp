@@ -51,24 +51,30 @@ __host__ __device__ constexpr uint64_t SlotStrideBytes(uint64_t action_count) {
 
 // ---- scoring layout ----------------------------------------------------------------------------
 // An action's score is the 64-term dot product of its embedding row [26 fixed | 38 tail] with the
-// policy vector (output_embedding.hpp:97-119).  On v_mfma_f32_16x16x4_f32 it takes 17 k-steps per
-// tile of 16 actions: 7 over the fixed features and 10 over the tail, so that every k-step reads a
-// single source.  Lane group g supplies, at step s, policy dim ScoreDim(s, g) (-1: a zero pad).
-constexpr int kFixedSteps = 7;
-constexpr int kTailSteps = 10;
-constexpr int kScoreSteps = kFixedSteps + kTailSteps;
+// policy vector (output_embedding.hpp:97-119).  On v_mfma_f32_16x16x4_f32 it takes 16 k-steps per
+// tile of 16 actions, with no pad: lane group g supplies, at step s, policy dim ScoreDim(s, g) --
+// first its ScoreFixedCount(g) fixed dims (groups 0, 1: 6; groups 2, 3: 7), then its tail dims
+// (10; 9).  Steps 0..5 read fixed features only and steps 7..15 tail values only; step 6 reads the
+// tail in groups 0, 1 and the fixed features in groups 2, 3.
+constexpr int kScoreSteps = 16;
+__host__ __device__ constexpr int ScoreFixedCount(int g) { return g < 2 ? 6 : 7; }
+__host__ __device__ constexpr int ScoreFixedBase(int g) { return 6 * g + (g == 3 ? 1 : 0); }  // 0, 6, 12, 19
+__host__ __device__ constexpr int ScoreTailBase(int g) { return 10 * g - (g == 3 ? 1 : 0); }  // 0, 10, 20, 29
+// Policy dim of step s for lane group g in the tail part (s >= ScoreFixedCount(g)) is ScoreTailRow(g) + s.
+__host__ __device__ constexpr int ScoreTailRow(int g) { return kFixed + ScoreTailBase(g) - ScoreFixedCount(g); }
+// First of the 10 tail values a lane group reads for steps 6..15 (groups 2, 3 read one value before
+// their first tail dim, in place of step 6, which they take from the fixed features).
+__host__ __device__ constexpr int ScoreTailRead(int g) { return ScoreTailRow(g) - kFixed + 6; }  // 0, 10, 19, 28
 __host__ __device__ inline int ScoreDim(int s, int g) {
-    if (s < kFixedSteps) {
-        const int d = kFixedSteps * g + s;
-        return d < kFixed ? d : -1;
-    }
-    const int t = kTailSteps * g + (s - kFixedSteps);
-    return t < kTail ? kFixed + t : -1;
+    return s < ScoreFixedCount(g) ? ScoreFixedBase(g) + s : ScoreTailRow(g) + s;
 }
+// The lane group holding fixed dim d (at step d - ScoreFixedBase(group)).
+__host__ __device__ inline int ScoreFixedGroup(int d) { return d < 12 ? d / 6 : (d < 19 ? 2 : 3); }
 // Per tile of 16 actions, the catalog-wide image read by the scoring loop (identical in HBM and
 // LDS): two planes of [64 lanes][4] fixed-feature values (lane i + 16g holds steps 4h..4h+3 of
-// action row i for lane group g; step 7 and pads are 0), then the 16 actions' canonical indices
-// (the lowest catalog index holding the same word; a guessed word masks every copy of itself).
+// action row i for lane group g; steps past the group's fixed dims are 0), then the 16 actions'
+// canonical indices (the lowest catalog index holding the same word; a guessed word masks every copy
+// of itself).
 constexpr int kImgTileWords = 2 * 64 * 4 + 16;  // 528
 constexpr int kImgTileBytes = 4 * kImgTileWords;
 

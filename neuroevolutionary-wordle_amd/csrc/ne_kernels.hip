@@ -189,8 +189,8 @@ __device__ __forceinline__ void FoldSlot(v4f (&acc)[16], const TrunkLane& tl, in
     }
 }
 
-// Layers 1 and 2: pol[s] = policy dim ScoreDim(s, gq) of game column j (0 where that is a pad).
-// Layer 2 runs 5 row tiles; row i of tile u computes policy dim ScoreDim(4u + i%4, i/4), so the
+// Layers 1 and 2: pol[s] = policy dim ScoreDim(s, gq) of game column j.
+// Layer 2 runs 4 row tiles; row i of tile u computes policy dim ScoreDim(4u + i%4, i/4), so the
 // accumulator register r of tile u in lane group gq is exactly scoring step 4u + r.
 __device__ __forceinline__ void TrunkToPolicy(const v4f (&acc)[16], const TrunkLane& tl, float (&pol)[kScoreSteps]) {
     v4f h1[8];
@@ -214,25 +214,28 @@ __device__ __forceinline__ void TrunkToPolicy(const v4f (&acc)[16], const TrunkL
             __builtin_amdgcn_sched_barrier(0);
         }
     }
-    // Policy dim of step s in lane group g: 7g + s for s < 7, else 19 + 10g + s (pads read any
-    // in-bounds value and are zeroed below).
-    constexpr int kRowTiles = (kScoreSteps + 3) / 4;
+    // Policy dim of step s in lane group g: ScoreFixedBase(g) + s for its fixed steps, else
+    // ScoreTailRow(g) + s.
+    constexpr int kRowTiles = kScoreSteps / 4;
     const uint32_t jr = tl.j & 3u, jg = tl.j >> 2;
-    const uint32_t row_f = 7u * jg + jr, row_t = 19u + 10u * jg + jr;
+    const uint32_t nf_j = uint32_t(ScoreFixedCount(int(jg)));
+    const uint32_t row_f = uint32_t(ScoreFixedBase(int(jg))) + jr, row_t = uint32_t(ScoreTailRow(int(jg))) + jr;
     uint32_t roff[kRowTiles];
 #pragma unroll
     for (int u = 0; u < kRowTiles; ++u) {
-        const uint32_t d = min(4u * u + jr < uint32_t(kFixedSteps) ? row_f + 4u * u : row_t + 4u * u, 63u);
+        const uint32_t d = 4u * u + jr < nf_j ? row_f + 4u * u : row_t + 4u * u;
         roff[u] = uint32_t(kOffW2 + d * kH1 + 4 * tl.gq) * 4;
     }
-    const uint32_t bias_f = uint32_t(kOffB2 + 7 * tl.gq) * 4, bias_t = uint32_t(kOffB2 + 19 + 10 * tl.gq) * 4;
+    const int nf = ScoreFixedCount(int(tl.gq));
+    const uint32_t bias_f = uint32_t(kOffB2 + ScoreFixedBase(int(tl.gq))) * 4;
+    const uint32_t bias_t = uint32_t(kOffB2 + ScoreTailRow(int(tl.gq))) * 4;
     v4f pv[kRowTiles];
 #pragma unroll
     for (int u = 0; u < kRowTiles; ++u)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int st = 4 * u + r < kScoreSteps ? 4 * u + r : 0;
-            pv[u][r] = BufF(tl.gr, st < kFixedSteps ? bias_f : bias_t, uint32_t(4 * st));
+            const int st = 4 * u + r;
+            pv[u][r] = BufF(tl.gr, st < nf ? bias_f : bias_t, uint32_t(4 * st));
         }
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
@@ -248,7 +251,7 @@ __device__ __forceinline__ void TrunkToPolicy(const v4f (&acc)[16], const TrunkL
         __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
-    for (int s = 0; s < kScoreSteps; ++s) pol[s] = ScoreDim(s, int(tl.gq)) < 0 ? 0.0f : pv[s / 4][s % 4];
+    for (int s = 0; s < kScoreSteps; ++s) pol[s] = pv[s / 4][s % 4];
 }
 
 // ---- layers 1 and 2 with their weights streamed through LDS ----------------------------------
@@ -282,8 +285,9 @@ __device__ __forceinline__ void IssueTrunkChunk(uint8_t* buf, Rsrc gr, uint32_t 
         __builtin_amdgcn_raw_ptr_buffer_load_lds(gr, (LdsPtr)(buf + i * 1024), 16, voff, base + i * row_bytes, 0, 0);
 }
 
-// Buffers handed to TrunkStream hold its largest layer-2 read: a pad row (19 + 30 + 3 + 16), last 16 bytes.
-constexpr uint32_t kTrunkReadBytes = (19 + 30 + 3 + 16) * 32 * 4 + 64 + 48 + 16;
+// Buffers handed to TrunkStream hold its largest layer-2 read: row 63's last 16 bytes, the chunk's end.
+constexpr uint32_t kTrunkReadBytes = 63 * 32 * 4 + 64 + 48 + 16;
+static_assert(kTrunkReadBytes == kTrunkChunkBytes, "layer-2 reads stay inside the chunk");
 
 // Layers 1 and 2 for the whole workgroup (every wave calls it; `compute` = this wave has games).
 // Chunk 0 must be in flight into b0; chunk i uses b0 when i is even, b1 when odd.  During the last
@@ -297,7 +301,7 @@ __device__ __forceinline__ void TrunkStream(const v4f (&acc)[16], const TrunkLan
     v4f h1[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) h1[u] = Buf4(tl.gr, tl.off_b1, 64 * u);
-    constexpr int kRowTiles = (kScoreSteps + 3) / 4;
+    constexpr int kRowTiles = kScoreSteps / 4;
     v4f pv[kRowTiles];
     WaitChunk();
     __syncthreads();
@@ -309,13 +313,15 @@ __device__ __forceinline__ void TrunkStream(const v4f (&acc)[16], const TrunkLan
         uint8_t* nxt = (t & 1u) ? b0 : b1;
         IssueTrunkChunk(nxt, tl.gr, t + 1, wave, waves, lw1, lw2);
         if (t == kTrunkW1Chunks - 1) {  // layer-2 biases, landing during the last layer-1 chunk
-            const uint32_t bias_f = uint32_t(kOffB2 + 7 * tl.gq) * 4, bias_t = uint32_t(kOffB2 + 19 + 10 * tl.gq) * 4;
+            const int nf = ScoreFixedCount(int(tl.gq));
+            const uint32_t bias_f = uint32_t(kOffB2 + ScoreFixedBase(int(tl.gq))) * 4;
+            const uint32_t bias_t = uint32_t(kOffB2 + ScoreTailRow(int(tl.gq))) * 4;
 #pragma unroll
             for (int u = 0; u < kRowTiles; ++u)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int st = 4 * u + r < kScoreSteps ? 4 * u + r : 0;
-                    pv[u][r] = BufF(tl.gr, st < kFixedSteps ? bias_f : bias_t, uint32_t(4 * st));
+                    const int st = 4 * u + r;
+                    pv[u][r] = BufF(tl.gr, st < nf ? bias_f : bias_t, uint32_t(4 * st));
                 }
         }
         if (compute) {
@@ -333,13 +339,13 @@ __device__ __forceinline__ void TrunkStream(const v4f (&acc)[16], const TrunkLan
         __syncthreads();
     }
     // layer 2: chunk c holds inputs 32c..32c+31, i.e. input tiles t = 2c, 2c + 1.  Lane (j, gq) of row
-    // tile u reads row ScoreDim(4u + j%4, j/4): 7jg + jr + 4u while that is a fixed dim (u = 0, and
-    // u = 1 for jr < 3), else 19 + 10jg + jr + 4u -- three base addresses plus immediates.  Rows past
-    // 63 are pads (their results are zeroed below); they read bytes of the buffer past the chunk.
+    // tile u reads row ScoreDim(4u + j%4, j/4): ScoreFixedBase(jg) + jr + 4u while that is a fixed dim
+    // (u = 0; u = 1 for jr < 2, and jr = 2 in groups 2, 3), else ScoreTailRow(jg) + jr + 4u -- three base
+    // addresses plus immediates.
     const uint32_t jr = tl.j & 3u, jg = tl.j >> 2;
-    const uint32_t rd_f = (7 * jg + jr) * (32 * 4) + 16 * tl.gq;
-    const uint32_t rd_t = (19 + 10 * jg + jr) * (32 * 4) + 16 * tl.gq;
-    const uint32_t rd_1 = jr + 4 < uint32_t(kFixedSteps) ? rd_f + 512 : rd_t + 512;
+    const uint32_t rd_f = (uint32_t(ScoreFixedBase(int(jg))) + jr) * (32 * 4) + 16 * tl.gq;
+    const uint32_t rd_t = (uint32_t(ScoreTailRow(int(jg))) + jr) * (32 * 4) + 16 * tl.gq;
+    const uint32_t rd_1 = jr + 4 < uint32_t(ScoreFixedCount(int(jg))) ? rd_f + 512 : rd_t + 512;
 #pragma unroll
     for (uint32_t c = 0; c < kTrunkW2Chunks; ++c) {
         uint8_t* cur = (c & 1u) ? b1 : b0;
@@ -368,7 +374,7 @@ __device__ __forceinline__ void TrunkStream(const v4f (&acc)[16], const TrunkLan
         __syncthreads();
     }
 #pragma unroll
-    for (int s = 0; s < kScoreSteps; ++s) pol[s] = ScoreDim(s, int(tl.gq)) < 0 ? 0.0f : pv[s / 4][s % 4];
+    for (int s = 0; s < kScoreSteps; ++s) pol[s] = pv[s / 4][s % 4];
 }
 
 // Stages a genome's 145->64 encoder into LDS, transposed to [feature][out].
@@ -473,8 +479,8 @@ __global__ __launch_bounds__(320) void PackBf16Kernel(const uint8_t* slab, uint6
             const int d = Bf16SlotDim(m, 8 * g + e);
             float x = 0.0f;
             if (a < A) {
-                if (d < kFixed) {  // fixed dim d sits at step d % 7 of lane group d / 7 in the FP32 image
-                    const int gg = d / kFixedSteps, st = d % kFixedSteps;
+                if (d < kFixed) {  // fixed dim d: step d - ScoreFixedBase(gg) of its lane group in the FP32 image
+                    const int gg = ScoreFixedGroup(d), st = d - ScoreFixedBase(gg);
                     x = __builtin_bit_cast(float, img[uint64_t(at) * kImgTileWords + (st / 4) * 256 +
                                                       4 * (i + 16 * uint32_t(gg)) + (st % 4)]);
                 } else {
@@ -619,33 +625,31 @@ __device__ __forceinline__ void ShareArgmaxGate(ArgmaxLane& am) {
     am.gate = fmaxf(g, __shfl_xor(g, 32));
 }
 
-// The 17 embedding values lane (j, gq) feeds into the scoring contraction for action row j of a tile:
-// fixed steps from the two image planes, tail steps from the row (lane group 3's last two steps are
-// pads; what lies there is the next row, so they are zeroed).
+// The 16 embedding values lane (j, gq) feeds into the scoring contraction for action row j of a tile:
+// fixed steps from the two image planes, then 10 tail values of the row from ScoreTailRead(gq) on
+// (steps 6..15; lane groups 2, 3 take step 6 from the image instead).  The tail values are read one
+// dword at a time: a row is 38 floats, so a lane group's first tail value is only dword aligned.
 __device__ __forceinline__ void TileOperands(const uint32_t* img_tile, const float* tail_tile, uint32_t lane,
                                              uint32_t j, uint32_t gq, float (&ev)[kScoreSteps]) {
     const float* fx = reinterpret_cast<const float*>(img_tile);
     const v4f f0 = *reinterpret_cast<const v4f*>(fx + 4 * lane);
     const v4f f1 = *reinterpret_cast<const v4f*>(fx + 256 + 4 * lane);
+    const float* tr = tail_tile + j * kTail + uint32_t(ScoreTailRead(int(gq)));
+    float tv[10];
+#pragma unroll
+    for (int q = 0; q < 10; ++q) tv[q] = tr[q];
     ev[0] = f0[0];
     ev[1] = f0[1];
     ev[2] = f0[2];
     ev[3] = f0[3];
     ev[4] = f1[0];
     ev[5] = f1[1];
-    ev[6] = f1[2];
-    const float* tr = tail_tile + j * kTail + kTailSteps * gq;
+    ev[6] = gq < 2 ? tv[0] : f1[2];
 #pragma unroll
-    for (int q = 0; q < kTailSteps / 2; ++q) {
-        const float2 v = *reinterpret_cast<const float2*>(tr + 2 * q);
-        ev[kFixedSteps + 2 * q] = v.x;
-        ev[kFixedSteps + 2 * q + 1] = v.y;
-    }
-    ev[kScoreSteps - 2] = gq == 3 ? 0.0f : ev[kScoreSteps - 2];
-    ev[kScoreSteps - 1] = gq == 3 ? 0.0f : ev[kScoreSteps - 1];
+    for (int q = 1; q < 10; ++q) ev[6 + q] = tv[q];
 }
 
-// The FP32 scoring contraction of one tile: 17 dependent v_mfma_f32_16x16x4_f32 steps in a fixed
+// The FP32 scoring contraction of one tile: 16 dependent v_mfma_f32_16x16x4_f32 steps in a fixed
 // order.  Every FP32 score of this build, streamed or rescored, comes from exactly this sequence.
 __device__ __forceinline__ v4f Fp32Scores(const float (&ev)[kScoreSteps], const float (&pol)[kScoreSteps]) {
     v4f sc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -768,29 +772,30 @@ __device__ inline float2 BufF2(Rsrc r, uint32_t voff) {
     return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
 }
 
-// The 17 FP32 operands of action a for lane group gq, read from HBM (rescoring one candidate): the
+// The 16 FP32 operands of action a for lane group gq, read from HBM (rescoring one candidate): the
 // same values TileOperands reads from LDS for that action's row.
 __device__ __forceinline__ void ActionOperands(Rsrc img_r, Rsrc tail_r, uint32_t a, uint32_t gq,
                                                float (&ev)[kScoreSteps]) {
     const uint32_t base = ((a >> 4) * uint32_t(kImgTileWords) + 4 * ((a & 15u) + 16 * gq)) * 4;
     const v4f f0 = Buf4(img_r, base, 0);
     const v4f f1 = Buf4(img_r, base + 1024, 0);
+    const uint32_t toff = (a * uint32_t(kTail) + uint32_t(ScoreTailRead(int(gq)))) * 4;
+    float tv[10];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {  // buffer loads need only dword alignment
+        const float2 v = BufF2(tail_r, toff + 8 * q);
+        tv[2 * q] = v.x;
+        tv[2 * q + 1] = v.y;
+    }
     ev[0] = f0[0];
     ev[1] = f0[1];
     ev[2] = f0[2];
     ev[3] = f0[3];
     ev[4] = f1[0];
     ev[5] = f1[1];
-    ev[6] = f1[2];
-    const uint32_t toff = (a * uint32_t(kTail) + kTailSteps * gq) * 4;
+    ev[6] = gq < 2 ? tv[0] : f1[2];
 #pragma unroll
-    for (int q = 0; q < kTailSteps / 2; ++q) {
-        const float2 v = BufF2(tail_r, toff + 8 * q);
-        ev[kFixedSteps + 2 * q] = v.x;
-        ev[kFixedSteps + 2 * q + 1] = v.y;
-    }
-    ev[kScoreSteps - 2] = gq == 3 ? 0.0f : ev[kScoreSteps - 2];
-    ev[kScoreSteps - 1] = gq == 3 ? 0.0f : ev[kScoreSteps - 1];
+    for (int q = 1; q < 10; ++q) ev[6 + q] = tv[q];
 }
 
 // Layer-0 accumulators between decisions.  They are needed only by the fold and the trunk, so they wait in
@@ -1035,7 +1040,7 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
                 if constexpr (!kBf16) {
                     ArgmaxLane am = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot, -__builtin_inff()};
                     // per chunk: the operands of tile k + 1 are read from LDS before tile k's contraction, so
-                    // their latency hides behind its 17 MFMAs
+                    // their latency hides behind its 16 MFMAs
                     StreamChunksWhole(s_buf0, s_buf1, chunks, wave_active, issue, [&](const uint8_t* buf, uint32_t c) __device__ {
                         if (!kTrace) ShareArgmaxGate(am);
                         float ev[kScoreSteps];
@@ -1562,25 +1567,26 @@ __device__ __forceinline__ void TrunkStreamPair(Rsrc acc_r, const TrunkLane& tl,
         xa = na;
         xb = nb;
     }
-    // layer 2 starts from the biases: pv[u][r] = B2[ScoreDim(4u + r, gq)] (pads read neighbouring LDS
-    // words and are zeroed below), as TrunkStream loads them
-    constexpr int kRowTiles = (kScoreSteps + 3) / 4;
+    // layer 2 starts from the biases: pv[u][r] = B2[ScoreDim(4u + r, gq)], as TrunkStream loads them
+    constexpr int kRowTiles = kScoreSteps / 4;
     v4f pva[kRowTiles], pvb[kRowTiles];
     {
         const float* sb = reinterpret_cast<const float*>(b0 + kTrunkBiasOff);
+        const int nf = ScoreFixedCount(int(tl.gq));
+        const int bf = ScoreFixedBase(int(tl.gq)), bt = ScoreTailRow(int(tl.gq));
 #pragma unroll
         for (int u = 0; u < kRowTiles; ++u)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int st = 4 * u + r < kScoreSteps ? 4 * u + r : 0;
-                pva[u][r] = sb[st < kFixedSteps ? 7 * tl.gq + st : 19 + 10 * tl.gq + st];
+                const int st = 4 * u + r;
+                pva[u][r] = sb[st < nf ? bf + st : bt + st];
                 pvb[u][r] = pva[u][r];
             }
     }
     const uint32_t jr = tl.j & 3u, jg = tl.j >> 2;
-    const uint32_t rd_f = (7 * jg + jr) * (32 * 4) + 16 * tl.gq;
-    const uint32_t rd_t = (19 + 10 * jg + jr) * (32 * 4) + 16 * tl.gq;
-    const uint32_t rd_1 = jr + 4 < uint32_t(kFixedSteps) ? rd_f + 512 : rd_t + 512;
+    const uint32_t rd_f = (uint32_t(ScoreFixedBase(int(jg))) + jr) * (32 * 4) + 16 * tl.gq;
+    const uint32_t rd_t = (uint32_t(ScoreTailRow(int(jg))) + jr) * (32 * 4) + 16 * tl.gq;
+    const uint32_t rd_1 = jr + 4 < uint32_t(ScoreFixedCount(int(jg))) ? rd_f + 512 : rd_t + 512;
 #pragma unroll
     for (uint32_t c = 0; c < kTrunkW2Chunks; ++c) {
         uint8_t* cur = (c & 1u) ? b1 : b0;
@@ -1616,9 +1622,8 @@ __device__ __forceinline__ void TrunkStreamPair(Rsrc acc_r, const TrunkLane& tl,
     }
 #pragma unroll
     for (int s = 0; s < kScoreSteps; ++s) {
-        const bool pad = ScoreDim(s, int(tl.gq)) < 0;
-        pol0[s] = pad ? 0.0f : pva[s / 4][s % 4];
-        pol1[s] = pad ? 0.0f : pvb[s / 4][s % 4];
+        pol0[s] = pva[s / 4][s % 4];
+        pol1[s] = pvb[s / 4][s % 4];
     }
 }
 
@@ -2796,7 +2801,7 @@ __global__ __launch_bounds__(512) void InjectKernel(InjectArgs p) {
         // mean over the word's 3 grids in grid order: ((0 + p0) + p1) + p2, times 1/3
         float* row = genome + kPolicyParams + uint64_t(p.parent_action_count + word) * kTail;
 #pragma unroll
-        for (int q = kFixedSteps; q < kScoreSteps; ++q) {
+        for (int q = 6; q < kScoreSteps; ++q) {  // steps 0..5 hold fixed dims in every lane group
             const float p1 = __shfl_down(pol[q], 1);
             const float p2 = __shfl_down(pol[q], 2);
             const float mean = (((0.0f + pol[q]) + p1) + p2) * (1.0f / 3.0f);

@@ -1626,7 +1626,7 @@ int ne_upload_catalog(ne_runtime* rt, const uint8_t* letters, uint64_t word_coun
     float absmax[kFixed] = {0.0f};
     for (uint64_t a = 0; a < word_count; ++a)
         for (int d = 0; d < kFixed; ++d) absmax[d] = std::max(absmax[d], std::fabs(fixed[a * kFixed + d]));
-    // scoring image: lane i + 16g of a tile holds steps 0..6 of action row i for lane group g
+    // scoring image: lane i + 16g of a tile holds the fixed steps of action row i for lane group g
     const uint64_t tiles = (word_count + 15) / 16;
     std::vector<uint32_t> img(tiles * kImgTileWords, 0u);
     for (uint64_t t = 0; t < tiles; ++t) {
@@ -1634,9 +1634,8 @@ int ne_upload_catalog(ne_runtime* rt, const uint8_t* letters, uint64_t word_coun
         for (int lane = 0; lane < 64; ++lane) {
             const uint64_t a = t * 16 + (lane & 15);
             if (a >= word_count) continue;
-            for (int s = 0; s < kFixedSteps; ++s) {
-                const int d = ScoreDim(s, lane >> 4);
-                const float v = d < 0 ? 0.0f : fixed[a * kFixed + d];
+            for (int s = 0; s < ScoreFixedCount(lane >> 4); ++s) {
+                const float v = fixed[a * kFixed + ScoreDim(s, lane >> 4)];
                 std::memcpy(&tile[(s / 4) * 256 + 4 * lane + (s % 4)], &v, 4);
             }
         }
