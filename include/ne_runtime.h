@@ -284,10 +284,11 @@ int ne_set_trace(ne_runtime *rt, uint32_t flags);
  * reference evaluates in its storage precision only, evaluation_ops.cuh:244-508). */
 int ne_set_eval_precision(ne_runtime *rt, uint32_t mode);
 /* Shape of the persistent evaluation kernel's launch (no reference counterpart).  Three kernels give the same
- * results bit for bit: REGISTERS plays one tile of 16 games per wave and keeps its layer-0 accumulators in
- * registers (3 waves per SIMD); STREAMED plays one tile per wave and keeps them in device scratch between the
- * fold and the trunk (4 waves per SIMD, smaller chunks: 5 workgroups per CU); PAIRED plays two tiles per
- * wave, so one workgroup streams a genome's operands once per decision for twice the games. */
+ * results bit for bit within each precision mode: REGISTERS plays one tile of 16 games per wave and keeps its
+ * layer-0 accumulators in registers (3 waves per SIMD); STREAMED plays one tile per wave and keeps them in
+ * device scratch between the fold and the trunk (4 waves per SIMD); PAIRED plays two tiles per wave, so one
+ * workgroup streams a genome's operands once per decision for twice the games (in the BF16-screened mode at
+ * most 3 waves per workgroup: larger requests run at 3). */
 #define NE_EVAL_KERNEL_REGISTERS 0u
 #define NE_EVAL_KERNEL_STREAMED 1u
 #define NE_EVAL_KERNEL_PAIRED 2u
@@ -298,13 +299,14 @@ typedef struct ne_eval_shape {
                                     over ceil(tiles / waves) items; r: over ceil(tiles / (r x waves))) */
     uint32_t workgroups_per_cu;  /* persistent workgroups per CU; 0: as many as the kernel's registers and
                                     LDS allow */
-    uint32_t kernel;             /* NE_EVAL_KERNEL_*: which evaluation kernel (untraced FP32 form; the others
-                                    always use NE_EVAL_KERNEL_REGISTERS) */
+    uint32_t kernel;             /* NE_EVAL_KERNEL_*: which evaluation kernel (untraced evaluations; traced
+                                    ones always use NE_EVAL_KERNEL_REGISTERS) */
 } ne_eval_shape;
 /* Autotuner of the evaluation's launch shape (no reference counterpart): a coordinate search over the
  * four knobs of ne_eval_shape, each candidate timed with HIP events on one evaluation of the current
  * generation -- waves 1..4 (the REGISTERS kernel, rounds 1, all workgroups), then the STREAMED and PAIRED
- * kernels at waves 1..4 (untraced FP32 form), then rounds 2 and 4, then one or two workgroups per CU fewer
+ * kernels at every workgroup size they allow (untraced evaluations), then rounds 2 and 4, then one or two
+ * workgroups per CU fewer
  * than fit -- keeping the fastest for later evaluations of this handle.  Results do not depend on
  * the shape; the generation is left evaluated (with the chosen shape) as by ne_evaluate.  chosen_waves
  * (optional) receives the chosen waves, timings_ms (optional) the 4 times of the waves sweep;

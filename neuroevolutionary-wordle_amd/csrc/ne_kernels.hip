@@ -590,6 +590,61 @@ __device__ __forceinline__ void StreamChunks(uint8_t* b0, uint8_t* b1, uint32_t 
     }
 }
 
+// Waits until at most kKeep of this wave's vector memory transfers are outstanding (they complete in issue
+// order, so every older transfer has landed).
+template <uint32_t kKeep>
+__device__ __forceinline__ void WaitKeep() {
+    static_assert(kKeep < 16, "vmcnt field");
+    __builtin_amdgcn_s_waitcnt(0x0F70 | kKeep);
+}
+
+// A workgroup barrier that leaves this wave's later transfers in flight: __syncthreads() is also a
+// workgroup-scope fence, which waits for every outstanding transfer (vmcnt(0)) before the barrier.  Here
+// only this wave's LDS reads are completed (lgkmcnt(0): a buffer may be refilled after the barrier), the
+// transfers the caller needs have been waited for by the caller (WaitKeep), and compiler and scheduler
+// barriers keep every memory access on its side.
+__device__ __forceinline__ void BarrierKeepTransfers() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt not waited for
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("" ::: "memory");
+}
+
+// StreamChunks over three buffers (distinct objects) with two chunks in flight: chunk c + 2 is issued when
+// chunk c's tiles start, and before the chunk's closing barrier each wave waits only until its own transfers
+// of chunk c + 1 have landed (at most kKeep transfers outstanding: those of chunk c + 2, of which every wave
+// issues exactly kKeep and nothing else in between).  A buffer is refilled only after the barrier that
+// closes its last reading chunk.  Chunk 0 must already have landed in b0 (and b1 must be free) when the
+// workgroup calls this.
+template <int kCT, uint32_t kKeep, typename FI, typename FT>
+__device__ __forceinline__ void StreamChunks3(uint8_t* b0, uint8_t* b1, uint8_t* b2, uint32_t chunks, bool compute,
+                                              FI&& issue, FT&& per_tile) {
+    WaitChunk();
+    __syncthreads();
+    if (chunks > 1) issue(b1, 1u);
+    auto one = [&](const uint8_t* buf, uint8_t* next2, uint32_t c) __device__ {
+        if (c + 2 < chunks) issue(next2, c + 2);
+        if (compute) {
+#pragma unroll
+            for (int k = 0; k < kCT; ++k) per_tile(buf, k, c * uint32_t(kCT) + uint32_t(k));
+        }
+        if (c + 2 < chunks) {
+            WaitKeep<kKeep>();  // chunk c + 1 landed, chunk c + 2 may stay in flight
+            BarrierKeepTransfers();
+        } else {
+            WaitChunk();
+            __syncthreads();
+        }
+    };
+    for (uint32_t c = 0; c < chunks; c += 3) {
+        one(b0, b2, c);
+        if (c + 1 < chunks) one(b1, b0, c + 1);
+        if (c + 2 < chunks) one(b2, b1, c + 2);
+    }
+}
+
 // StreamChunks with one callback per chunk: per_chunk(buf, c) for the buffer holding chunk c.
 template <typename FI, typename FC>
 __device__ __forceinline__ void StreamChunksWhole(uint8_t* b0, uint8_t* b1, uint32_t chunks, bool compute, FI&& issue,
@@ -747,6 +802,20 @@ __device__ __forceinline__ v4f Bf16Screen(const uint8_t* buf, int k, uint32_t la
 }
 __device__ __forceinline__ const uint32_t* Bf16Canon(const uint8_t* buf, int k, uint32_t gq) {
     return reinterpret_cast<const uint32_t*>(buf + kB16CatBytes + kB16GenBytes) + k * 16 + 4 * gq;
+}
+// Bf16Screen in two parts, so that one read of a tile's operands serves several game tiles: the lane's two
+// A operands of tile k, then the two instructions against one tile's policy.
+__device__ __forceinline__ void Bf16Operands(const uint8_t* buf, int k, uint32_t lane, bf16x8& a0, bf16x8& a1) {
+    const uint32_t off0 = lane < 48 ? k * kBf16CatTileBytes + 16 * lane
+                                    : kB16CatBytes + k * kBf16GenTileBytes + 16 * (lane - 48);
+    a0 = *reinterpret_cast<const bf16x8*>(buf + off0);
+    a1 = *reinterpret_cast<const bf16x8*>(buf + kB16CatBytes + k * kBf16GenTileBytes + 256 + 16 * lane);
+}
+__device__ __forceinline__ v4f Bf16Mfma(const bf16x8& a0, const bf16x8& a1, const PolicyBf16& pb) {
+    v4f sc = {0.0f, 0.0f, 0.0f, 0.0f};
+    sc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, pb.k[0], sc, 0, 0, 0);
+    sc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, pb.k[1], sc, 0, 0, 0);
+    return sc;
 }
 
 // Reduces a lane's masked-argmax state over the four lane groups sharing game column j (lower index
@@ -1131,7 +1200,8 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
                             // four) sends the decision to the FP32 scan
                             lfail = lfail || !(fabsf((sc[0] + sc[1]) + (sc[2] + sc[3])) <= 3.0e38f);
                             if (!__any((want && m4 >= lthr) ? 1 : 0)) return;
-                            const v4u cn = canon_of();
+                            const uint32_t ca = at * 16 + 4 * gq;
+                            const v4u cn = p.canon_identity ? v4u{ca, ca + 1, ca + 2, ca + 3} : canon_of();
     #pragma unroll
                             for (int r = 0; r < 4; ++r) {
                                 const uint32_t a = at * 16 + 4 * gq + uint32_t(r);
@@ -1648,9 +1718,12 @@ __device__ __forceinline__ void ApplyGuess(const EvalArgs& p, uint32_t besta, ui
 }
 
 // Scores the finished episodes of a tile (evaluation_ops.cuh:239-242) into the genome's raw fitness and
-// the decision / encoded-turn counters.
+// the decision / encoded-turn counters (and, in the BF16 form, the rescored-candidate and FP32-fallback
+// counters: the lane's counts, nonzero in lane group 0 only).
+template <bool kBf16 = false>
 __device__ __forceinline__ void FinishTile(const EvalArgs& p, uint32_t n, bool fresh, bool tile_exists, uint32_t lane,
-                                           uint32_t gq, const GameLane& g) {
+                                           uint32_t gq, const GameLane& g, uint32_t rescored = 0,
+                                           uint32_t fallbacks = 0) {
     float score = (g.valid && g.won) ? 10.0f + float(kMaxTurns - g.tc) : 0.0f;
     uint32_t dec = (g.valid && gq == 0) ? g.decisions : 0u;
     const uint32_t base_tc = fresh ? 0u : 2u;
@@ -1661,21 +1734,224 @@ __device__ __forceinline__ void FinishTile(const EvalArgs& p, uint32_t n, bool f
         score += __shfl_xor(score, off);
         dec += __shfl_xor(dec, off);
         enc += __shfl_xor(enc, off);
+        if (kBf16) {
+            rescored += __shfl_xor(rescored, off);
+            fallbacks += __shfl_xor(fallbacks, off);
+        }
     }
     if (lane == 0 && tile_exists) {
         atomicAdd(p.raw_fitness + n, score);
         atomicAdd(p.counters + 0, (unsigned long long)dec);
         atomicAdd(p.counters + 1, (unsigned long long)enc);
+        if (kBf16) {
+            atomicAdd(p.counters + 2, (unsigned long long)rescored);
+            atomicAdd(p.counters + 3, (unsigned long long)fallbacks);
+        }
     }
 }
 
-template <int kTiles, int kCT>
+// ---- BF16 screening of one game tile (the BF16 form of EvaluateStreamed) --------------------------------
+// EvaluateKernel's BF16 screening, with the lane's state of one game tile gathered in one place, so that a
+// wave playing two tiles screens both from one read of each action tile's operands.
+struct TileScreen {
+    PolicyBf16 pb;      // the tile's policy as the B operands of the two screening instructions
+    float eps2;         // twice the screening bound of the lane's game
+    bool want;          // the game is active and screened
+    bool fail;          // the decision goes to the FP32 scan
+    float lbest, lthr;  // the lane's running masked maximum and its window threshold
+    float llost;        // the best score dropped from a full list
+    uint32_t lcnt;      // list length
+    bool lfail;         // a non-finite screening score
+};
+
+// The tile's policy as bf16 B operands, exchanged through the wave's 2 KB area `xb` (every lane writes its
+// dims at their k-slots and reads back its own 2 x 8; all 1,024 entries are written, so the area can serve
+// the next tile at once), and the bound eps = 1.03 2^-8 sum_d M_d |p_d| + 2^-120 (sum (M_d + |p_d|) + 1).
+__device__ __forceinline__ void SetupTileScreen(const EvalArgs& p, uint32_t n, const float (&pol)[kScoreSteps],
+                                                bool active, __bf16* xb, uint32_t lane, uint32_t j, uint32_t gq,
+                                                TileScreen& ts) {
+#pragma unroll
+    for (int s = 0; s < kScoreSteps; ++s) {
+        const int d = ScoreDim(s, int(gq));
+        int slot;
+        const int m = Bf16DimSlot(d, &slot);
+        xb[m * 512 + (int(j) + 16 * (slot >> 3)) * 8 + (slot & 7)] = __bf16(pol[s]);
+    }
+    __builtin_amdgcn_wave_barrier();
+    ts.pb.k[0] = *reinterpret_cast<const bf16x8*>(xb + lane * 8);
+    ts.pb.k[1] = *reinterpret_cast<const bf16x8*>(xb + 512 + lane * 8);
+    __builtin_amdgcn_wave_barrier();  // the reads precede any later write of the area (LDS keeps a wave's order)
+    float esum = 0.0f, pabs = 0.0f;
+#pragma unroll
+    for (int s = 0; s < kScoreSteps; ++s) {
+        const int d = ScoreDim(s, int(gq));
+        const float m = d < kFixed ? p.fixed_absmax[d] : p.tail_absmax[uint64_t(n) * 40 + uint32_t(d - kFixed)];
+        esum += m * fabsf(pol[s]);
+        pabs += m + fabsf(pol[s]);
+    }
+    esum += __shfl_xor(esum, 16);
+    esum += __shfl_xor(esum, 32);
+    pabs += __shfl_xor(pabs, 16);
+    pabs += __shfl_xor(pabs, 32);
+    const float eps = kBf16RelBound * esum + kBf16AbsBound * (pabs + 1.0f);
+    ts.eps2 = 2.0f * eps;
+    ts.fail = active && (p.force_fallback != 0 || !(eps <= 1.0e37f));
+    ts.want = active && !ts.fail;
+    ts.lbest = -__builtin_inff();
+    ts.lthr = -__builtin_inff();
+    ts.llost = -__builtin_inff();
+    ts.lcnt = 0;
+    ts.lfail = false;
+}
+
+// Once per chunk: the game's running maximum over its 4 lane groups (still at most the final BF16 maximum,
+// so the window stays a superset) raises every lane's threshold.
+__device__ __forceinline__ void ShareRunningMax(TileScreen& ts) {
+    float gb = fmaxf(ts.lbest, __shfl_xor(ts.lbest, 16));
+    gb = fmaxf(gb, __shfl_xor(gb, 32));
+    if (gb > ts.lbest) {
+        ts.lbest = gb;
+        ts.lthr = gb - ts.eps2;
+    }
+}
+
+// One action tile's screening scores of the lane's 4 rows into its list (lsc / lac: the lane's kLC slots
+// of this game tile, [slot][lane]); EvaluateKernel's screen step.
+// With canon_identity (no word occurs twice: the canonical index of action a is a) the masks compare action
+// indices and the rows' canonical indices are not read.
+template <uint32_t kLC>
+__device__ __forceinline__ void ScreenTileRows(const v4f& sc, uint32_t at, const uint32_t* canon_ptr,
+                                               bool canon_identity, uint32_t gq, uint32_t A,
+                                               const uint32_t (&hist)[5], TileScreen& ts, float (*lsc)[64],
+                                               uint32_t (*lac)[64], uint32_t lane) {
+    const float m4 = fmaxf(fmaxf(sc[0], sc[1]), fmaxf(sc[2], sc[3]));
+    // a NaN or infinite score anywhere in the row quad (or an overflowing sum of four) sends the decision to
+    // the FP32 scan
+    ts.lfail = ts.lfail || !(fabsf((sc[0] + sc[1]) + (sc[2] + sc[3])) <= 3.0e38f);
+    if (!__any((ts.want && m4 >= ts.lthr) ? 1 : 0)) return;
+    const uint32_t a0 = at * 16 + 4 * gq;
+    const v4u cn = canon_identity ? v4u{a0, a0 + 1, a0 + 2, a0 + 3} : *reinterpret_cast<const v4u*>(canon_ptr);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t a = a0 + uint32_t(r);
+        const uint32_t c = cn[r];
+        const bool masked = (c == hist[0]) | (c == hist[1]) | (c == hist[2]) | (c == hist[3]) | (c == hist[4]);
+        const float v = sc[r];
+        if (!(ts.want && a < A && !masked && v >= ts.lthr)) continue;
+        if (v > ts.lbest) {
+            ts.lbest = v;
+            ts.lthr = v - ts.eps2;
+        }
+        if (ts.lcnt == kLC) {  // drop what the risen threshold excludes
+            uint32_t w = 0;
+            for (uint32_t q = 0; q < kLC; ++q) {
+                const float sq = lsc[q][lane];
+                if (sq >= ts.lthr) {
+                    lsc[w][lane] = sq;
+                    lac[w][lane] = lac[q][lane];
+                    ++w;
+                }
+            }
+            ts.lcnt = w;
+        }
+        if (ts.lcnt < kLC) {
+            lsc[ts.lcnt][lane] = v;
+            lac[ts.lcnt][lane] = a;
+            ++ts.lcnt;
+        } else {  // full: keep the best kLC, remember the best score dropped
+            uint32_t qmin = 0;
+            float smin = lsc[0][lane];
+            for (uint32_t q = 1; q < kLC; ++q) {
+                const float sq = lsc[q][lane];
+                if (sq < smin) {
+                    smin = sq;
+                    qmin = q;
+                }
+            }
+            if (v > smin) {
+                lsc[qmin][lane] = v;
+                lac[qmin][lane] = a;
+                ts.llost = fmaxf(ts.llost, smin);
+            } else {
+                ts.llost = fmaxf(ts.llost, v);
+            }
+        }
+    }
+}
+
+// The game's window after the screening pass: b16 over its 4 lane groups, then every listed row with
+// S16 >= b16 - 2 eps goes into cand[j][0..) (count in *ncount, this wave's counter of the tile); the decision
+// fails (FP32 scan) on a non-finite score, a dropped score inside the window, no candidate or more than
+// kMaxCandidates.  Returns the game's candidate count.
+template <uint32_t kLC>
+__device__ __forceinline__ uint32_t CollectWindow(TileScreen& ts, float (*lsc)[64], const uint32_t (*lac)[64],
+                                                  uint32_t (*cand)[kMaxCandidates], uint32_t* ncount, uint32_t lane,
+                                                  uint32_t j) {
+    float b16 = ts.lbest;
+    b16 = fmaxf(b16, __shfl_xor(b16, 16));
+    b16 = fmaxf(b16, __shfl_xor(b16, 32));
+    const float thr = b16 - ts.eps2;
+    int gfail = (ts.lfail || ts.llost >= thr) ? 1 : 0;
+    gfail |= __shfl_xor(gfail, 16);
+    gfail |= __shfl_xor(gfail, 32);
+    ts.fail = ts.fail || (ts.want && (gfail != 0 || !(fabsf(b16) <= 3.0e38f)));
+    if (lane < 16) ncount[lane] = 0u;
+    __builtin_amdgcn_wave_barrier();
+    if (ts.want && !ts.fail) {
+        for (uint32_t q = 0; q < ts.lcnt; ++q) {
+            if (lsc[q][lane] >= thr) {
+                const uint32_t k = atomicAdd(&ncount[j], 1u);
+                if (k < kMaxCandidates) cand[j][k] = lac[q][lane];
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t nc = ncount[j];
+    ts.fail = ts.fail || (ts.want && (nc == 0 || nc > kMaxCandidates));
+    return nc;
+}
+
+// Rescoring of candidate k of the game (when k < nc and `resc`) by the FP32 contraction itself: candidate
+// of game i in MFMA row i, game i's policy in column i, the diagonal D[j][j] is game j's score; the FP32
+// argmax over the candidates (lowest index on ties) reproduces the FP32 form's decision and score.
+__device__ __forceinline__ void RescoreCandidate(Rsrc img_r, Rsrc tail_r, const uint32_t (*cand)[kMaxCandidates],
+                                                 uint32_t k, uint32_t nc, bool resc, const float (&pol)[kScoreSteps],
+                                                 uint32_t j, uint32_t gq, float& best, uint32_t& besta) {
+    const bool take = resc && k < nc;
+    const uint32_t c = take ? cand[j][k] : 0u;
+    float ev[kScoreSteps];
+    ActionOperands(img_r, tail_r, c, gq, ev);
+    const v4f sc = Fp32Scores(ev, pol);
+    const float mine = (j & 3u) == 0 ? sc[0] : ((j & 3u) == 1 ? sc[1] : ((j & 3u) == 2 ? sc[2] : sc[3]));
+    const float v = __shfl(mine, int(j + 16 * (j >> 2)));
+    if (take) {
+        const bool better = besta == kInvalidSlot || v > best || (v == best && c < besta);
+        best = better ? v : best;
+        besta = better ? c : besta;
+    }
+}
+
+// kBf16: the BF16-screened form (EvaluateKernel<false, true>'s decisions, per tile), with LDS lists of kLC
+// entries per lane and game tile for at most kListWaves waves per workgroup.  kRing (BF16 form, workgroups of
+// exactly kListWaves = 3 waves): the screening stream runs through three buffers with two chunks in flight
+// (StreamChunks3), every wave issuing exactly 3 of a chunk's 9 transfers.
+template <int kTiles, int kCT, bool kBf16 = false, int kListWaves = 1, uint32_t kLC = kLaneCandidates,
+          bool kRing = false>
 __device__ __forceinline__ void EvaluateStreamed(const EvalArgs& p) {
     constexpr bool kTwo = kTiles == 2;
     using G = ChunkGeom<kCT>;
-    static_assert(kTrunkChunkBytes <= G::kBufBytes && kTrunkBiasOff + 4 * 68 <= G::kBufBytes, "trunk chunk");
-    __shared__ __align__(16) uint8_t s_buf0[G::kBufBytes];
-    __shared__ __align__(16) uint8_t s_buf1[G::kBufBytes];
+    // chunk buffers: FP32 scan chunks, trunk chunks (with the layer-2 biases behind) and screening chunks
+    constexpr uint32_t kTrunkNeed = kTrunkBiasOff + 4 * 68;
+    constexpr uint32_t kBufA = G::kBufBytes > kTrunkNeed ? G::kBufBytes : kTrunkNeed;
+    constexpr uint32_t kBufBytes = (kBf16 && kB16BufBytes > kBufA) ? kB16BufBytes : kBufA;
+    static_assert(kTrunkChunkBytes <= kBufBytes && kTrunkNeed <= kBufBytes && kBufBytes % 16 == 0, "trunk chunk");
+    static_assert(!kBf16 || (kListWaves * 2048 <= kBufBytes &&
+                             sizeof(uint32_t) * kListWaves * 16 * kMaxCandidates <= kBufBytes),
+                  "policy exchange and candidate tables fit a chunk buffer");
+    static_assert(!kRing || (kBf16 && kListWaves == 3 && kB16OpsDma + kB16CanonDma == 9),
+                  "the ring's waits count 3 transfers per wave and chunk");
+    __shared__ __align__(16) uint8_t s_buf0[kBufBytes];
+    __shared__ __align__(16) uint8_t s_buf1[kBufBytes];
     __shared__ uint32_t s_item;
     const uint32_t tid = threadIdx.x, bd = blockDim.x;
     const uint32_t lane = tid & 63u;
@@ -1720,12 +1996,36 @@ __device__ __forceinline__ void EvaluateStreamed(const EvalArgs& p) {
         auto issue = [&](uint8_t* buf, uint32_t c) __device__ {
             IssueChunk(buf, img_r, tail_r, c, wave, waves, lane, G::kImgBytes, G::kTailBytes, G::kImgDma, G::kTailDma);
         };
+        // BF16 form: the screening stream, per chunk of kB16Tiles tiles 3 x 16-byte transfers of catalog
+        // operands, 5 of the genome's packed operands, one 4-byte transfer of the 64 canonical indices
+        const uint32_t chunks16 = (a_tiles + kB16Tiles - 1) / kB16Tiles;
+        const Rsrc cat_r = MakeRsrc(p.bf16_cat, p.bf16_cat_bytes);
+        const Rsrc ops_r = MakeRsrc(kBf16 ? p.bf16_ops + uint64_t(n) * p.bf16_stride : p.slab, a_tiles * kBf16GenTileBytes);
+        const Rsrc canon_r = MakeRsrc(p.canon, p.canon_bytes);
+        auto issue16 = [&](uint8_t* buf, uint32_t c) __device__ {
+            constexpr uint32_t kCatDma = kB16CatBytes / kDmaBytes;
+            for (uint32_t i = wave; i < kB16OpsDma + kB16CanonDma; i += waves) {
+                if (i < kCatDma)
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(cat_r, (LdsPtr)(buf + i * kDmaBytes), 16,
+                                                             c * kB16CatBytes + i * kDmaBytes + 16 * lane, 0, 0, 0);
+                else if (i < kB16OpsDma)
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(ops_r, (LdsPtr)(buf + i * kDmaBytes), 16,
+                                                             c * kB16GenBytes + (i - kCatDma) * kDmaBytes + 16 * lane, 0,
+                                                             0, 0);
+                else
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(canon_r,
+                                                             (LdsPtr)(buf + kB16OpsDma * kDmaBytes + (i - kB16OpsDma) * 256),
+                                                             4, c * (kB16Tiles * 64) + (i - kB16OpsDma) * 256 + 4 * lane,
+                                                             0, 0, 0);
+            }
+        };
         const uint32_t per_round = p.splits * waves * kTiles;
         const uint32_t first_tile = split * waves * kTiles;
         const uint32_t rounds = tiles > first_tile ? (tiles - first_tile + per_round - 1) / per_round : 0u;
         for (uint32_t round = 0; round < rounds; ++round) {
             const uint32_t tile0 = first_tile + round * per_round + kTiles * wave, tile1 = kTwo ? tile0 + 1 : tiles;
             const bool fresh0 = tile0 < tiles_fresh, fresh1 = kTwo && tile1 < tiles_fresh;
+            uint32_t rescored0 = 0, rescored1 = 0, fallbacks0 = 0, fallbacks1 = 0;  // BF16 form, lane group 0
             GameLane g0, g1;
             InitGameLane(p, local, W, tiles_fresh, tiles, tile0, j, gq, g0);
             InitGameLane(p, local, W, tiles_fresh, tiles, tile1, j, gq, g1);
@@ -1743,7 +2043,12 @@ __device__ __forceinline__ void EvaluateStreamed(const EvalArgs& p) {
                 const bool compute = act0 || act1;
                 float pol0[kScoreSteps], pol1[kScoreSteps];
                 TrunkStreamPair<kTwo>(acc_r, tl, pol0, pol1, compute, s_buf0, s_buf1, wave, waves, lane,
-                                      [&](uint8_t* b) __device__ { issue(b, 0); });
+                                      [&](uint8_t* b) __device__ {
+                                          if constexpr (kBf16)
+                                              issue16(b, 0);
+                                          else
+                                              issue(b, 0);
+                                      });
                 if (!act0) {
 #pragma unroll
                     for (int s = 0; s < kScoreSteps; ++s) pol0[s] = 0.0f;
@@ -1752,43 +2057,147 @@ __device__ __forceinline__ void EvaluateStreamed(const EvalArgs& p) {
 #pragma unroll
                     for (int s = 0; s < kScoreSteps; ++s) pol1[s] = 0.0f;
                 }
-                // ---- scoring: S = E . P over all actions for both tiles, masked argmax (lowest index on ties)
-                ArgmaxLane am0 = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot, -__builtin_inff()};
-                ArgmaxLane am1 = am0;
-                StreamChunksWhole(s_buf0, s_buf1, chunks, compute, issue, [&](const uint8_t* buf, uint32_t c) __device__ {
-                    ShareArgmaxGate(am0);
-                    if (kTwo) ShareArgmaxGate(am1);
-                    float ev[kScoreSteps];
-                    TileOperands(Fp32Img<kCT>(buf, 0), Fp32Tails<kCT>(buf, 0), lane, j, gq, ev);
+                // ---- FP32 scoring: S = E . P over all actions for both tiles, masked argmax (lowest index on
+                // ties); the whole workgroup streams the FP32 chunks (chunk 0 already in flight into s_buf0)
+                auto fp32_scan = [&](ArgmaxLane& am0, ArgmaxLane& am1, bool scan) __device__ {
+                    StreamChunksWhole(s_buf0, s_buf1, chunks, scan, issue, [&](const uint8_t* buf, uint32_t c) __device__ {
+                        ShareArgmaxGate(am0);
+                        if (kTwo) ShareArgmaxGate(am1);
+                        float ev[kScoreSteps];
+                        TileOperands(Fp32Img<kCT>(buf, 0), Fp32Tails<kCT>(buf, 0), lane, j, gq, ev);
 #pragma unroll
-                    for (int k = 0; k < kCT; ++k) {
-                        float nx[kScoreSteps];  // the next tile's operands, read during this tile's MFMAs
-                        if (k + 1 < kCT) TileOperands(Fp32Img<kCT>(buf, k + 1), Fp32Tails<kCT>(buf, k + 1), lane, j, gq, nx);
-                        v4f s0 = {0.0f, 0.0f, 0.0f, 0.0f}, s1 = s0;
+                        for (int k = 0; k < kCT; ++k) {
+                            float nx[kScoreSteps];  // the next tile's operands, read during this tile's MFMAs
+                            if (k + 1 < kCT) TileOperands(Fp32Img<kCT>(buf, k + 1), Fp32Tails<kCT>(buf, k + 1), lane, j, gq, nx);
+                            v4f s0 = {0.0f, 0.0f, 0.0f, 0.0f}, s1 = s0;
 #pragma unroll
-                        for (int s = 0; s < kScoreSteps; ++s) {
-                            s0 = Mfma(ev[s], pol0[s], s0);
-                            if (kTwo) s1 = Mfma(ev[s], pol1[s], s1);
+                            for (int s = 0; s < kScoreSteps; ++s) {
+                                s0 = Mfma(ev[s], pol0[s], s0);
+                                if (kTwo) s1 = Mfma(ev[s], pol1[s], s1);
+                            }
+                            const uint32_t at = c * uint32_t(kCT) + uint32_t(k);
+                            const uint32_t* cn = Fp32Img<kCT>(buf, k) + 512 + 4 * gq;
+                            ArgmaxUpdate<false>(s0, cn, at * 16 + 4 * gq, A, g0.hist, am0);
+                            if (kTwo) ArgmaxUpdate<false>(s1, cn, at * 16 + 4 * gq, A, g1.hist, am1);
+                            if (k + 1 < kCT) {
+#pragma unroll
+                                for (int s = 0; s < kScoreSteps; ++s) ev[s] = nx[s];
+                            }
                         }
-                        const uint32_t at = c * uint32_t(kCT) + uint32_t(k);
-                        const uint32_t* cn = Fp32Img<kCT>(buf, k) + 512 + 4 * gq;
-                        ArgmaxUpdate<false>(s0, cn, at * 16 + 4 * gq, A, g0.hist, am0);
-                        if (kTwo) ArgmaxUpdate<false>(s1, cn, at * 16 + 4 * gq, A, g1.hist, am1);
-                        if (k + 1 < kCT) {
-#pragma unroll
-                            for (int s = 0; s < kScoreSteps; ++s) ev[s] = nx[s];
-                        }
+                    });
+                    ReduceArgmax(am0.best, am0.second, am0.besta);
+                    if (kTwo) ReduceArgmax(am1.best, am1.second, am1.besta);
+                };
+                if constexpr (!kBf16) {
+                    ArgmaxLane am0 = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot, -__builtin_inff()};
+                    ArgmaxLane am1 = am0;
+                    fp32_scan(am0, am1, compute);
+                    ApplyGuess(p, am0.besta, gq, g0);
+                    if (kTwo) ApplyGuess(p, am1.besta, gq, g1);
+                } else {
+                    // ---- BF16 screening with exact FP32 rescoring, per game tile as in EvaluateKernel's BF16
+                    // form: one screening stream serves the wave's tiles (each LDS read of an action tile's
+                    // operands feeds one pair of instructions per game tile); every tile keeps its own running
+                    // maximum, lane lists and window, and its candidates are rescored by the FP32 contraction.
+                    __shared__ float s_lsc[kListWaves][kTiles][kLC][64];
+                    __shared__ uint32_t s_lac[kListWaves][kTiles][kLC][64];
+                    __shared__ uint32_t s_ncand[kListWaves][kTiles][16];
+                    TileScreen ts0, ts1;
+                    {
+                        // the policy exchange in this wave's 2 KB of s_buf1, free between TrunkStreamPair's closing
+                        // barrier (every wave has consumed the last trunk chunk, which sat in s_buf1) and
+                        // StreamChunks' opening barrier (after which the screening stream's second chunk is
+                        // issued into s_buf1)
+                        __bf16* xb = reinterpret_cast<__bf16*>(s_buf1 + wave * 2048);
+                        SetupTileScreen(p, n, pol0, g0.active, xb, lane, j, gq, ts0);
+                        if (kTwo) SetupTileScreen(p, n, pol1, g1.active, xb, lane, j, gq, ts1);
                     }
-                });
-                ReduceArgmax(am0.best, am0.second, am0.besta);
-                ApplyGuess(p, am0.besta, gq, g0);
-                if (kTwo) {
-                    ReduceArgmax(am1.best, am1.second, am1.besta);
-                    ApplyGuess(p, am1.besta, gq, g1);
+                    const bool any_want = __any((ts0.want || (kTwo && ts1.want)) ? 1 : 0) != 0;
+                    auto screen_tile = [&](const uint8_t* buf, int k, uint32_t at) __device__ {
+                        if (k == 0) {
+                            ShareRunningMax(ts0);
+                            if (kTwo) ShareRunningMax(ts1);
+                        }
+                        bf16x8 a0, a1;
+                        Bf16Operands(buf, k, lane, a0, a1);
+                        const uint32_t* cp = Bf16Canon(buf, k, gq);
+                        const bool ci = p.canon_identity != 0;
+                        const v4f sc0 = Bf16Mfma(a0, a1, ts0.pb);
+                        if (kTwo) {
+                            const v4f sc1 = Bf16Mfma(a0, a1, ts1.pb);
+                            ScreenTileRows<kLC>(sc0, at, cp, ci, gq, A, g0.hist, ts0, s_lsc[wave][0], s_lac[wave][0], lane);
+                            ScreenTileRows<kLC>(sc1, at, cp, ci, gq, A, g1.hist, ts1, s_lsc[wave][kTiles - 1],
+                                                s_lac[wave][kTiles - 1], lane);
+                        } else {
+                            ScreenTileRows<kLC>(sc0, at, cp, ci, gq, A, g0.hist, ts0, s_lsc[wave][0], s_lac[wave][0], lane);
+                        }
+                    };
+                    if constexpr (kRing) {
+                        __shared__ __align__(16) uint8_t s_buf2[kB16BufBytes];
+                        // exactly 3 of the chunk's 9 transfers per wave (3 catalog, 5 genome, 1 canonical-index)
+                        auto issue16_3 = [&](uint8_t* buf, uint32_t c) __device__ {
+                            constexpr uint32_t kCatDma = kB16CatBytes / kDmaBytes;
+#pragma unroll
+                            for (uint32_t r = 0; r < 3; ++r) {
+                                const uint32_t i = wave + 3 * r;
+                                if (i < kCatDma)
+                                    __builtin_amdgcn_raw_ptr_buffer_load_lds(cat_r, (LdsPtr)(buf + i * kDmaBytes), 16,
+                                                                             c * kB16CatBytes + i * kDmaBytes + 16 * lane,
+                                                                             0, 0, 0);
+                                else if (i < kB16OpsDma)
+                                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                                        ops_r, (LdsPtr)(buf + i * kDmaBytes), 16,
+                                        c * kB16GenBytes + (i - kCatDma) * kDmaBytes + 16 * lane, 0, 0, 0);
+                                else
+                                    __builtin_amdgcn_raw_ptr_buffer_load_lds(canon_r, (LdsPtr)(buf + kB16OpsDma * kDmaBytes),
+                                                                             4, c * (kB16Tiles * 64) + 4 * lane, 0, 0, 0);
+                            }
+                        };
+                        StreamChunks3<kB16Tiles, 3>(s_buf0, s_buf1, s_buf2, chunks16, any_want, issue16_3, screen_tile);
+                    } else {
+                        StreamChunks<kB16Tiles>(s_buf0, s_buf1, chunks16, any_want, issue16, screen_tile);
+                    }
+                    // the windows; tile 0's candidate table in s_buf0, tile 1's in s_buf1 (both free from the
+                    // end of the stream until the fallback's or the next decision's first transfer)
+                    auto cand0 = reinterpret_cast<uint32_t (*)[16][kMaxCandidates]>(s_buf0) + wave;
+                    auto cand1 = reinterpret_cast<uint32_t (*)[16][kMaxCandidates]>(s_buf1) + wave;
+                    const uint32_t nc0 = CollectWindow<kLC>(ts0, s_lsc[wave][0], s_lac[wave][0], *cand0, s_ncand[wave][0], lane, j);
+                    uint32_t nc1 = 0;
+                    if (kTwo)
+                        nc1 = CollectWindow<kLC>(ts1, s_lsc[wave][kTiles - 1], s_lac[wave][kTiles - 1], *cand1,
+                                            s_ncand[wave][kTiles - 1], lane, j);
+                    const bool resc0 = g0.active && !ts0.fail, resc1 = kTwo && g1.active && !ts1.fail;
+                    uint32_t kmax = max(resc0 ? nc0 : 0u, resc1 ? nc1 : 0u);
+#pragma unroll
+                    for (int off = 1; off < 64; off <<= 1) kmax = max(kmax, uint32_t(__shfl_xor(int(kmax), off)));
+                    kmax = __builtin_amdgcn_readfirstlane(kmax);
+                    ArgmaxLane r0 = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot, -__builtin_inff()};
+                    ArgmaxLane r1 = r0;
+                    for (uint32_t k = 0; k < kmax; ++k) {
+                        if (__any((resc0 && k < nc0) ? 1 : 0))
+                            RescoreCandidate(img_r, tail_r, *cand0, k, nc0, resc0, pol0, j, gq, r0.best, r0.besta);
+                        if (kTwo && __any((resc1 && k < nc1) ? 1 : 0))
+                            RescoreCandidate(img_r, tail_r, *cand1, k, nc1, resc1, pol1, j, gq, r1.best, r1.besta);
+                    }
+                    if (resc0 && gq == 0) rescored0 += nc0;
+                    if (resc1 && gq == 0) rescored1 += nc1;
+                    const bool need0 = g0.active && ts0.fail, need1 = kTwo && g1.active && ts1.fail;
+                    if (__syncthreads_or((need0 || need1) ? 1 : 0)) {
+                        issue(s_buf0, 0);
+                        ArgmaxLane f0 = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot, -__builtin_inff()};
+                        ArgmaxLane f1 = f0;
+                        fp32_scan(f0, f1, __any((need0 || need1) ? 1 : 0) != 0);
+                        if (need0) r0 = f0;
+                        if (need1) r1 = f1;
+                    }
+                    if (need0 && gq == 0) ++fallbacks0;
+                    if (need1 && gq == 0) ++fallbacks1;
+                    ApplyGuess(p, r0.besta, gq, g0);
+                    if (kTwo) ApplyGuess(p, r1.besta, gq, g1);
                 }
             }
-            FinishTile(p, n, fresh0, tile0 < tiles, lane, gq, g0);
-            if (kTwo) FinishTile(p, n, fresh1, tile1 < tiles, lane, gq, g1);
+            FinishTile<kBf16>(p, n, fresh0, tile0 < tiles, lane, gq, g0, rescored0, fallbacks0);
+            if (kTwo) FinishTile<kBf16>(p, n, fresh1, tile1 < tiles, lane, gq, g1, rescored1, fallbacks1);
         }
     }
 }
@@ -1809,6 +2218,31 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
 #endif
 __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves_per_eu(NE_STREAMED_WAVES_PER_EU, NE_STREAMED_WAVES_PER_EU))) void EvaluateStreamedKernel(EvalArgs p) {
     EvaluateStreamed<1, NE_STREAMED_CHUNK_TILES>(p);
+}
+
+// The BF16-screened form on the same two kernels.  Two tiles per wave: a 3-wave workgroup plays the 96 games
+// of a 32-secret cell, so each decision streams the genome's packed screening operands and trunk weights once
+// instead of once per split, and every read of an action tile's screening operands feeds both tiles' pairs
+// of instructions.  FP32 chunks of 2 tiles (the fallback scan's; the trunk chunks and screening chunks fit
+// the same 10 KB buffers) and LDS lists for at most 3 waves keep a workgroup at 45 KB of LDS.
+#ifndef NE_PAIR_BF16_WAVES
+#define NE_PAIR_BF16_WAVES 3
+#endif
+#ifndef NE_PAIR_BF16_LIST
+#define NE_PAIR_BF16_LIST 6
+#endif
+__global__ __launch_bounds__(64 * NE_PAIR_BF16_WAVES) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void EvaluatePairBf16Kernel(EvalArgs p) {
+    EvaluateStreamed<2, kChunkTilesBf16, true, NE_PAIR_BF16_WAVES, NE_PAIR_BF16_LIST>(p);
+}
+// One tile per wave in the BF16 form, at exactly 3 waves per workgroup, with the screening stream through a
+// 3-buffer ring (two chunks in flight per workgroup instead of one).  FP32 fallback chunks of one tile and LDS
+// lists for 3 waves keep the workgroup under 40 KB of LDS, so four still share a CU.  (Measured at 64x64 /
+// 4,739 actions: 8.56 ms against 9.25 ms for EvaluateKernel<false, true>, same decisions.  The same code at
+// 1-2 waves with 8-entry lists for 4 waves and 128 VGPRs gave run-to-run differences in the count of
+// candidates rescored -- never in a decision -- that 6-entry lists did not; it is not built, and the BF16
+// form's streamed kernel runs only at 3 waves.)
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void EvaluateStreamedBf16RingKernel(EvalArgs p) {
+    EvaluateStreamed<1, 1, true, 3, kLaneCandidates, true>(p);
 }
 
 // ================================================================================================
@@ -2851,12 +3285,18 @@ void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, uint32_t
                     hipStream_t s) {
     const dim3 grid(blocks);
     const dim3 block(64 * waves);
-    if (!trace && !bf16 && kernel == kEvalKernelPaired) {
-        hipLaunchKernelGGL(EvaluatePairKernel, grid, block, 0, s, a);
+    if (!trace && kernel == kEvalKernelPaired) {
+        if (bf16)
+            hipLaunchKernelGGL(EvaluatePairBf16Kernel, grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL(EvaluatePairKernel, grid, block, 0, s, a);
         return;
     }
-    if (!trace && !bf16 && kernel == kEvalKernelStreamed) {
-        hipLaunchKernelGGL(EvaluateStreamedKernel, grid, block, 0, s, a);
+    if (!trace && kernel == kEvalKernelStreamed) {
+        if (bf16)
+            hipLaunchKernelGGL(EvaluateStreamedBf16RingKernel, grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL(EvaluateStreamedKernel, grid, block, 0, s, a);
         return;
     }
     if (bf16) {
@@ -2874,13 +3314,28 @@ void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, uint32_t
 
 uint32_t EvaluateMaxWaves() { return kEvalMaxWaves; }
 
+uint32_t EvaluateMaxWavesFor(uint32_t kernel, bool trace, bool bf16) {
+    if (!trace && bf16 && kernel == kEvalKernelPaired) return uint32_t(NE_PAIR_BF16_WAVES);
+    if (!trace && bf16 && kernel == kEvalKernelStreamed) return 3u;  // the ring kernel: exactly 3
+    return kEvalMaxWaves;
+}
+
+uint32_t EvaluateKernelFor(uint32_t kernel, uint32_t waves, bool trace, bool bf16) {
+    if (trace) return kEvalKernelRegisters;
+    if (bf16 && kernel == kEvalKernelStreamed && waves != 3) return kEvalKernelRegisters;
+    return kernel;
+}
+
 uint64_t EvaluateAccScratchBytes() { return uint64_t(kEvalMaxWaves) * 2 * kAccTileFloats * sizeof(float); }
 
 uint32_t EvaluateResidentPerCu(uint32_t waves, uint32_t kernel, bool trace, bool bf16) {
     // what the registers and LDS of the kernel built here allow, as the runtime reports it
+    if (waves > EvaluateMaxWavesFor(kernel, trace, bf16)) return 0;
     const bool plain = !trace && !bf16;
     const void* k = (plain && kernel == kEvalKernelPaired) ? reinterpret_cast<const void*>(&EvaluatePairKernel)
                   : (plain && kernel == kEvalKernelStreamed) ? reinterpret_cast<const void*>(&EvaluateStreamedKernel)
+                  : (!trace && kernel == kEvalKernelPaired) ? reinterpret_cast<const void*>(&EvaluatePairBf16Kernel)
+                  : (!trace && kernel == kEvalKernelStreamed) ? reinterpret_cast<const void*>(&EvaluateStreamedBf16RingKernel)
                   : bf16 ? (trace ? reinterpret_cast<const void*>(&EvaluateKernel<true, true>)
                                   : reinterpret_cast<const void*>(&EvaluateKernel<false, true>))
                          : (trace ? reinterpret_cast<const void*>(&EvaluateKernel<true, false>)

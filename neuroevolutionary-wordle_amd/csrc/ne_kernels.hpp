@@ -32,6 +32,7 @@ struct EvalArgs {
     const uint32_t* codes;         // packed catalog words
     const uint32_t* canon;         // [catalog tiles * 16] lowest index holding the same word (padded)
     uint32_t canon_bytes;
+    uint32_t canon_identity;       // 1: every catalog word is unique (canon[a] == a), so masks compare action indices
     const uint32_t* img;           // [catalog tiles][kImgTileWords] scoring image
     uint32_t img_bytes;
     const float* enc_t;            // [N][kEncTStride]: transposed encoder, then W0[:, 0], per genome
@@ -245,7 +246,8 @@ void LaunchPackBf16(const uint8_t* slab, uint64_t stride, const uint32_t* slots,
 void LaunchTransposeEncoder(const uint8_t* slab, uint64_t stride, const uint32_t* slots, uint32_t n,
                             uint32_t slot_count, float* enc_t, hipStream_t s);
 // Persistent evaluation: `blocks` workgroups of `waves` waves drain the work list.
-// The evaluation kernels of the untraced FP32 form (identical results); the other forms always use the first.
+// The evaluation kernels of the untraced forms (identical results in each form); traced evaluations always
+// use the first.
 constexpr uint32_t kEvalKernelRegisters = 0;  // one tile per wave, layer-0 accumulators in registers
 constexpr uint32_t kEvalKernelStreamed = 1;   // one tile per wave, accumulators in scratch, 4 waves per SIMD
 constexpr uint32_t kEvalKernelPaired = 2;     // two tiles per wave, accumulators in scratch
@@ -257,6 +259,12 @@ uint32_t EvaluateResidentPerCu(uint32_t waves, uint32_t kernel, bool trace, bool
 uint64_t EvaluateAccScratchBytes();
 // The most waves per evaluation workgroup this build supports (its LDS tables are sized for them).
 uint32_t EvaluateMaxWaves();
+// The most waves per workgroup of one evaluation kernel in one form (the BF16 form's streamed and two-tile
+// kernels: 3).
+uint32_t EvaluateMaxWavesFor(uint32_t kernel, bool trace, bool bf16);
+// The kernel that runs a requested (kernel, waves) in a form: traced evaluations use the register kernel, and
+// so does the BF16 form's streamed kernel at any size but 3 waves.
+uint32_t EvaluateKernelFor(uint32_t kernel, uint32_t waves, bool trace, bool bf16);
 void LaunchFinalize(const float* raw, const uint32_t* local_w, uint32_t n, float* fitness, uint32_t* evals,
                     uint8_t* has, int* status, hipStream_t s);
 void LaunchSummary(const float* fitness, const uint8_t* has, const uint32_t* slots, uint32_t n, SummaryRecord* out,

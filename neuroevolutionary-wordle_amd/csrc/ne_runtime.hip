@@ -122,6 +122,7 @@ struct ne_runtime {
     // catalog
     uint32_t* codes = nullptr;
     uint32_t* canon = nullptr;  // lowest catalog index holding the same word
+    bool canon_identity = false;  // no word occurs twice in the catalog (canon[a] == a)
     uint32_t* img = nullptr;    // scoring image, kImgTileWords per tile of 16 words
     float* fixed_absmax = nullptr;  // [26] max over the catalog of |fixed feature|
     uint8_t* bf16_cat = nullptr;    // [tiles][kBf16CatTileBytes] BF16 screening operands of fixed dims 0..23
@@ -167,7 +168,7 @@ struct ne_runtime {
     uint32_t tuned_waves = 0;           // 0: the idle-slot heuristic picks the workgroup size
     uint32_t tuned_rounds = 1;          // rounds of `waves` tiles one work item plays
     uint32_t tuned_wg_per_cu = 0;       // persistent workgroups per CU (0: as many as fit)
-    uint32_t tuned_kernel = 0;          // evaluation kernel of the untraced FP32 form (kEvalKernel*)
+    uint32_t tuned_kernel = 0;          // evaluation kernel of the untraced forms (kEvalKernel*)
     float* eval_acc = nullptr;          // the evaluation's layer-0 scratch, EvaluateAccScratchBytes() per workgroup
     uint64_t eval_acc_bytes = 0;
 
@@ -525,6 +526,7 @@ static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32
     a.codes = rt->codes;
     a.canon = rt->canon;
     a.canon_bytes = uint32_t(rt->img_bytes / kImgTileBytes * 64);
+    a.canon_identity = rt->canon_identity ? 1u : 0u;
     a.img = rt->img;
     a.img_bytes = uint32_t(rt->img_bytes);
     a.fixed_absmax = rt->fixed_absmax;
@@ -539,9 +541,11 @@ static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32
     a.game_trace = trace ? rt->d_trace : nullptr;
     a.policy_trace = (rt->trace_flags & NE_TRACE_POLICY) ? rt->d_policy_trace : nullptr;
     a.game_offsets = trace ? rt->d_game_offsets : nullptr;
-    // Kernel (untraced FP32 form only): the paired kernel plays two game tiles per wave, a workgroup's
-    // 2 x waves tiles with one operand stream per decision; the others one tile per wave.
-    const uint32_t kernel = (!trace && !bf16) ? rt->tuned_kernel : kEvalKernelRegisters;
+    // Kernel (untraced forms): the paired kernel plays two game tiles per wave, a workgroup's 2 x waves tiles
+    // with one operand stream per decision; the others one tile per wave (the BF16 form's streamed kernel only
+    // at 3 waves: at other sizes the register kernel runs, EvaluateKernelFor below).
+    uint32_t kernel = !trace ? rt->tuned_kernel : kEvalKernelRegisters;
+    const uint32_t max_waves = EvaluateMaxWavesFor(kernel, trace, bf16);
     const uint32_t tpw = kernel == kEvalKernelPaired ? 2u : 1u;
     // Waves per workgroup: 6-wave workgroups of this kernel's register footprint fit once per CU, 2-4
     // wave ones fill a CU (12 waves); pick the size that leaves the fewest idle tile slots over all
@@ -554,7 +558,7 @@ static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32
     }
     uint32_t waves = 1;
     uint64_t best_cost = ~0ull;
-    for (uint32_t w = 4; w >= 1; --w) {
+    for (uint32_t w = std::min<uint32_t>(4, max_waves); w >= 1; --w) {
         const uint64_t per = uint64_t(w) * tpw;
         uint64_t cost = 0;
         for (uint64_t c = 0; c < n; ++c) cost += (tiles[c] + per - 1) / per * per;
@@ -563,7 +567,10 @@ static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32
             waves = w;
         }
     }
-    if (rt->tuned_waves != 0) waves = rt->tuned_waves;  // the autotuner's choice (ne_autotune_evaluation)
+    // the autotuner's choice (ne_autotune_evaluation), at most what this kernel's LDS tables hold (results do
+    // not depend on the workgroup size)
+    if (rt->tuned_waves != 0) waves = std::min(rt->tuned_waves, max_waves);
+    kernel = EvaluateKernelFor(kernel, waves, trace, bf16);
     // Work list for the persistent kernel, in one queue per XCD (a workgroup drains its own XCD's queue,
     // then helps with the others): genomes by descending tile count, dealt round-robin over the queues, so
     // the longest genomes start first and the queues carry equal work.  Within a queue the FP32 form goes
@@ -1641,6 +1648,8 @@ int ne_upload_catalog(ne_runtime* rt, const uint8_t* letters, uint64_t word_coun
         }
         for (int i = 0; i < 16; ++i) tile[512 + i] = t * 16 + i < word_count ? canon[t * 16 + i] : kInvalidSlot;
     }
+    bool identity = true;
+    for (uint64_t a = 0; a < word_count; ++a) identity = identity && canon[a] == uint32_t(a);
     canon.resize(tiles * 16, kInvalidSlot);  // whole tiles, for the screening stream's transfers
     // BF16 screening operands of instruction 0, lanes 0..47 (fixed dims 0..23), round-to-nearest-even
     std::vector<uint16_t> bf16_cat(tiles * kBf16CatTileBytes / 2, 0);
@@ -1669,6 +1678,10 @@ int ne_upload_catalog(ne_runtime* rt, const uint8_t* letters, uint64_t word_coun
         !Ok(hipMemcpy(rt->bf16_cat, bf16_cat.data(), rt->bf16_cat_bytes, hipMemcpyHostToDevice)))
         return Fail(rt, NE_DEVICE_FAILURE);
     rt->catalog_count = word_count;
+#ifdef NE_NO_CANON_IDENTITY
+    identity = false;
+#endif
+    rt->canon_identity = identity;
     rt->h_catalog.assign(letters, letters + 5 * word_count);
     return NE_OK;
 }
@@ -2284,8 +2297,9 @@ int ne_autotune_evaluation(ne_runtime* rt, const ne_word_counts* counts, const n
                            uint32_t* chosen_waves, float* timings_ms) {
     if (!rt || !counts) return Fail(rt, NE_INVALID_ARGUMENT);
     // coordinate search, one knob at a time, each candidate timed on one evaluation of the current generation
-    // (results do not depend on the shape): waves per workgroup, then (untraced FP32 form) two game tiles
-    // per wave at every workgroup size, then rounds per work item, then workgroups per CU
+    // (results do not depend on the shape): waves per workgroup, then (untraced forms) the streamed kernel
+    // and two game tiles per wave at every workgroup size they allow, then rounds per work item, then
+    // workgroups per CU
     const uint32_t saved[4] = {rt->tuned_waves, rt->tuned_rounds, rt->tuned_wg_per_cu, rt->tuned_kernel};
     auto time_shape = [&](uint32_t w, uint32_t kern, uint32_t r, uint32_t g, float* ms) -> int {
         rt->tuned_waves = w;
@@ -2317,9 +2331,10 @@ int ne_autotune_evaluation(ne_runtime* rt, const ne_word_counts* counts, const n
         }
     }
     const bool trace = rt->trace_flags != 0, bf16 = (rt->eval_mode & 0xFFu) == NE_EVAL_BF16_RESCORED;
-    if (!trace && !bf16) {
+    if (!trace) {
         for (const uint32_t kern : {kEvalKernelStreamed, kEvalKernelPaired})
-            for (uint32_t w = 1; w <= 4; ++w) {
+            for (uint32_t w = 1; w <= EvaluateMaxWavesFor(kern, trace, bf16); ++w) {
+                if (EvaluateKernelFor(kern, w, trace, bf16) != kern) continue;  // runs another kernel, timed above
                 float ms = 0.0f;
                 const int st = time_shape(w, kern, 1, 0, &ms);
                 if (st != NE_OK) return st;

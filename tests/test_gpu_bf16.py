@@ -156,3 +156,69 @@ def test_untraced_evaluations_equal_traced_ones(pkg, case):
             assert d == d0
     finally:
         rt.close()
+
+
+@pytest.mark.parametrize("case", ["grid16", "a4739", "ragged", "fallback", "zero"])
+def test_bf16_kernels_screen_and_decide_alike(pkg, case):
+    """The BF16 form's untraced kernels -- one tile per wave with the accumulators in registers or streamed
+    through scratch, two tiles per wave -- at several launch shapes: the screening, windows and rescoring
+    are per game, so every shape gives the register kernel's fitness bit for bit, its decision count, the
+    same number of candidates rescored and the same fallback decisions; and the FP32 form's fitness.
+    "ragged": cells of 2 to 70 secrets with duplicate catalog words (odd tile counts, half-empty pairs);
+    "fallback": every decision forced through the FP32 scan; "zero": all-zero genomes (every score ties)."""
+    rtm = pkg.runtime
+    mode = rtm.EVAL_BF16_RESCORED | (rtm.EVAL_FORCE_FALLBACK if case == "fallback" else 0)
+    if case == "a4739":
+        A, side = 4739, 4
+        catalog = pkg.synth.make_catalog(A)
+        offsets, ids = pkg.synth.block_csr(side, side, 2, pkg.synth.sample_secrets(A, 64))
+    elif case == "ragged":
+        A, side = 365, 4
+        rng = np.random.default_rng(0xD0B1E)
+        catalog = pkg.synth.make_catalog(A, seed=0xD0B1E).copy()
+        for k in range(40, A, 9):
+            catalog[k] = catalog[int(rng.integers(0, k))]
+        widths = np.array([2, 4, 5, 16, 17, 33, 70, 6, 9, 48, 2, 64, 7, 31, 8, 10])
+        offsets = np.concatenate([[0], np.cumsum(widths)]).astype(np.uint32)
+        firsts = np.array(sorted({bytes(w): i for i, w in reversed(list(enumerate(catalog)))}.values()))
+        ids = np.concatenate([rng.choice(firsts, size=w, replace=False) for w in widths]).astype(np.uint32)
+    else:
+        wl = pkg.synth.workload("grid16_a512")
+        A, side, catalog, offsets, ids = 512, 16, wl["catalog"], wl["offsets"], wl["ids"]
+    zero = case == "zero"
+    rt = _runtime(pkg, side, A, catalog, offsets, ids, seed=23, var=0.0 if zero else 1.0, tail=0.0 if zero else 0.1)
+    counts = rtm.word_counts(A, A)
+    try:
+        rt.set_trace(0)
+
+        def run(m, waves, rounds, kernel):
+            rt.set_eval_precision(m)
+            rt.set_eval_shape_ex(waves, rounds, 0, kernel)
+            rt.evaluate(counts)
+            st = rt.stage_times()
+            return rt.generation(0)[1].copy(), (int(st.decisions), int(st.rescored), int(st.fallbacks))
+
+        f32, (d32, _, _) = run(rtm.EVAL_FP32, 3, 1, rtm.EVAL_KERNEL_REGISTERS)
+        f0, c0 = run(mode, 3, 1, rtm.EVAL_KERNEL_REGISTERS)
+        assert np.array_equal(f0.view(np.uint32), f32.view(np.uint32)) and c0[0] == d32
+        if case == "fallback":
+            assert c0[2] == c0[0] and c0[1] == 0
+        # streamed: the 3-wave ring kernel (4 runs at 3; 1 and 2 run the register kernel); paired: 1-3 waves (4
+        # runs at 3, its LDS lists' limit), 6-entry lists, so it may send a decision to the FP32 scan where the
+        # 8-entry lists of the others keep every window member (never the other way round)
+        shapes = [(w, 1, rtm.EVAL_KERNEL_STREAMED) for w in (1, 2, 3, 4)] + [(3, 4, rtm.EVAL_KERNEL_STREAMED)]
+        shapes += [(w, 1, rtm.EVAL_KERNEL_PAIRED) for w in (1, 2, 3, 4)] + [(3, 2, rtm.EVAL_KERNEL_PAIRED)]
+        for shape in shapes:
+            f, c = run(mode, *shape)
+            f2, c2 = run(mode, *shape)  # and the same again: no run-to-run differences
+            assert c2 == c and np.array_equal(f2.view(np.uint32), f.view(np.uint32)), (shape, c, c2)
+            assert np.array_equal(f.view(np.uint32), f0.view(np.uint32)), shape
+            assert c[0] == c0[0], (shape, c, c0)
+            if shape[2] == rtm.EVAL_KERNEL_PAIRED:
+                assert c[2] >= c0[2] and c[1] <= c0[1], (shape, c, c0)
+            else:
+                assert c == c0, (shape, c, c0)
+        print(f"{case}: {c0[0]} decisions, {c0[1]} rescored, {c0[2]} fallbacks, {len(shapes)} shapes identical")
+    finally:
+        rt.set_eval_shape_ex(0, 1, 0, 0)
+        rt.close()

@@ -3,8 +3,8 @@ shards of 32 -> 393,216 games, one evaluation of generation 0 in the FP32 form a
 form.
 
 Bar: over every game of the grid, the BF16 form takes the FP32 form's guesses with chosen scores equal
-bit for bit and identical fitness, and the untraced kernels (the ones timed: three FP32 kernels and the BF16
-form) give the traced kernels' fitness bit for bit; on the first 1,024 genomes (98,304 games) the FP32 form agrees with the oracle's
+bit for bit and identical fitness, and the untraced kernels (the ones timed: three kernels in each form)
+give the traced kernels' fitness bit for bit; on the first 1,024 genomes (98,304 games) the FP32 form agrees with the oracle's
 FP32 evaluation on >= 99.9% of the decisions taken on a common trajectory, and every genome without a
 near-tie decision has the oracle's fitness bit for bit; on 128 genomes spread over the grid (12,288
 games), every FP32 policy vector lies within 1e-5 (normwise) of the host fp64 forward on the state the GPU
@@ -44,12 +44,16 @@ def fullsize(pkg):
     for form, mode, kernel in (("fp32_untraced", rtm.EVAL_FP32, rtm.EVAL_KERNEL_REGISTERS),
                                ("fp32_streamed_untraced", rtm.EVAL_FP32, rtm.EVAL_KERNEL_STREAMED),
                                ("fp32_paired_untraced", rtm.EVAL_FP32, rtm.EVAL_KERNEL_PAIRED),
-                               ("bf16_untraced", rtm.EVAL_BF16_RESCORED, rtm.EVAL_KERNEL_REGISTERS)):
+                               ("bf16_untraced", rtm.EVAL_BF16_RESCORED, rtm.EVAL_KERNEL_REGISTERS),
+                               ("bf16_streamed_untraced", rtm.EVAL_BF16_RESCORED, rtm.EVAL_KERNEL_STREAMED),
+                               ("bf16_paired_untraced", rtm.EVAL_BF16_RESCORED, rtm.EVAL_KERNEL_PAIRED)):
         rt.set_eval_precision(mode)
         rt.set_trace(0)
         rt.set_eval_shape_ex(3, 1, 0, kernel)
         rt.evaluate(counts)
-        out[form] = dict(fitness=rt.generation(0)[1].copy(), decisions=int(rt.stage_times().decisions))
+        st = rt.stage_times()
+        out[form] = dict(fitness=rt.generation(0)[1].copy(), decisions=int(st.decisions), rescored=int(st.rescored),
+                         fallbacks=int(st.fallbacks))
     rt.set_eval_shape_ex(0, 1, 0, rtm.EVAL_KERNEL_REGISTERS)
     for form, mode, trace in (("fp32", rtm.EVAL_FP32, rtm.TRACE_GAMES | rtm.TRACE_POLICY),
                               ("bf16", rtm.EVAL_BF16_RESCORED, rtm.TRACE_GAMES)):
@@ -87,13 +91,21 @@ def test_bf16_form_takes_every_fp32_decision_at_full_size(fullsize):
 
 
 def test_untraced_kernels_equal_traced_ones_at_full_size(fullsize):
-    """Every untraced kernel (the timed ones: the register, streamed and paired FP32 kernels and the BF16
-    form) gives the traced kernel's fitness bit for bit and its decision count."""
+    """Every untraced kernel (the timed ones: the register, streamed and paired kernels of both forms) gives
+    the traced kernel's fitness bit for bit and its decision count; the BF16 form's kernels screen alike
+    (the same candidates rescored and the same fallback decisions as the traced kernel)."""
     _, _, out = fullsize
-    for form, traced in (("fp32", "fp32"), ("fp32_streamed", "fp32"), ("fp32_paired", "fp32"), ("bf16", "bf16")):
+    for form, traced in (("fp32", "fp32"), ("fp32_streamed", "fp32"), ("fp32_paired", "fp32"), ("bf16", "bf16"),
+                         ("bf16_streamed", "bf16"), ("bf16_paired", "bf16")):
         assert np.array_equal(out[form + "_untraced"]["fitness"].view(np.uint32),
                               out[traced]["fitness"].view(np.uint32)), form
         assert out[form + "_untraced"]["decisions"] == out[traced]["decisions"], form
+        if form == "bf16_paired":  # 6-entry lists: maybe more decisions through the FP32 scan
+            assert out[form + "_untraced"]["fallbacks"] >= out["bf16"]["fallbacks"], form
+            assert out[form + "_untraced"]["rescored"] <= out["bf16"]["rescored"], form
+        elif traced == "bf16":
+            assert out[form + "_untraced"]["rescored"] == out["bf16"]["rescored"], form
+            assert out[form + "_untraced"]["fallbacks"] == out["bf16"]["fallbacks"], form
 
 
 def test_fp32_form_agrees_with_the_oracle_on_1024_genomes(fullsize, orc):

@@ -51,7 +51,8 @@ def _compare(a, b, where):
 
 
 @pytest.mark.parametrize("mode_name,kernel", [("EVAL_FP32", 0), ("EVAL_BF16_RESCORED", 0), ("EVAL_FP32", 1),
-                                              ("EVAL_FP32", 2)])
+                                              ("EVAL_FP32", 2), ("EVAL_BF16_RESCORED", 1),
+                                              ("EVAL_BF16_RESCORED", 2)])
 def test_graph_steps_equal_eager_steps(pkg, mode_name, kernel):
     """The captured runtime uses the given evaluation kernel (registers, streamed, paired), the eager one the
     register kernel."""
