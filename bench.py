@@ -46,10 +46,13 @@ def kernel_source_hash():
 
 
 KERNEL_NAMES = ("EvaluateKernel<false,false>", "EvaluateStreamedKernel", "EvaluatePairKernel")
+# the BF16-screened form's kernels (its streamed kernel is the 3-wave ring; at other sizes the register kernel runs)
+KERNEL_NAMES_BF16 = ("EvaluateKernel<false,true>", "EvaluateStreamedBf16RingKernel", "EvaluatePairBf16Kernel")
+PEAK_BF16_MFMA_TFLOPS = 2500.0
 
 
-def measured_traffic(kernel=0):
-    """HBM bytes per evaluation launch of the FP32 form's `kernel` (0 registers, 1 streamed, 2 paired) from the
+def measured_traffic(kernel=0, bf16=False):
+    """HBM bytes per evaluation launch of one form's `kernel` (0 registers, 1 streamed, 2 paired) from the
     committed FETCH_SIZE / WRITE_SIZE measurement of this workload (scaled by the plain-copy calibration
     recorded with it) and whether it was measured on the kernel sources of this tree: (bytes or None, stale)."""
     if not os.path.exists(TRAFFIC_FILE):
@@ -57,7 +60,7 @@ def measured_traffic(kernel=0):
     with open(TRAFFIC_FILE) as f:
         t = json.load(f)
     stale = t.get("kernel_source_sha256") != kernel_source_hash()
-    k = t.get("kernels", {}).get(KERNEL_NAMES[kernel])
+    k = t.get("kernels", {}).get((KERNEL_NAMES_BF16 if bf16 else KERNEL_NAMES)[kernel])
     if k is None:
         return None, True
     return k["fetch_bytes"] * t["fetch_scale"] + k["write_bytes"] * t["write_scale"], stale
@@ -240,10 +243,25 @@ def main():
     traffic, traffic_stale = measured_traffic(fp32_kernel)
     bf16_out = None
     if bf16 is not None:
+        sh = shapes.get((rtm.EVAL_BF16_RESCORED, False), {})
+        k16 = sh.get("kernel_id", 0)
+        if k16 == 1 and sh.get("waves") != 3:
+            k16 = 0  # the BF16 form's streamed kernel exists at 3 waves only
+        t16, t16_stale = measured_traffic(k16, bf16=True)
+        f16 = bf16["flops"] / (bf16["eval_ms"] * 1e-3) / 1e12
         bf16_out = {
             "value": games_per_gen * args.steps / bf16["elapsed"], "unit": "game-evals/s",
             "ms_per_step": bf16["elapsed"] / args.steps * 1e3, "generations_per_s": args.steps / bf16["elapsed"],
             "evaluate_ms": bf16["eval_ms"],
+            "kernel": KERNEL_NAMES_BF16[k16],
+            # the same algorithmic FLOP as the FP32 form's roofline, over the BF16 dense MFMA peak (most of the
+            # work -- the scoring -- runs on BF16 MFMA; the trunk and the rescoring stay FP32), and the measured
+            # HBM bytes of the kernel per launch over its time against the 8 TB/s peak
+            "roofline": {"achieved_tflops": f16, "frac_of_bf16_peak": f16 / PEAK_BF16_MFMA_TFLOPS,
+                         "traffic": None if t16_stale else t16,
+                         "traffic_GBps": None if (t16_stale or t16 is None) else t16 / (bf16["eval_ms"] * 1e-3) / 1e9,
+                         "traffic_frac_of_hbm_peak": None if (t16_stale or t16 is None)
+                         else t16 / (bf16["eval_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS},
             "candidates_rescored_per_decision": bf16["rescored"] / max(1, bf16["decisions"]),
             "fp32_fallback_decisions": bf16["fallbacks"],
             "same_fitness_as_fp32": bool(np.array_equal(bf16["fitness"].view(np.uint32),

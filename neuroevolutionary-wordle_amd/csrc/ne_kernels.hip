@@ -618,18 +618,15 @@ __device__ __forceinline__ void BarrierKeepTransfers() {
 // issues exactly kKeep and nothing else in between).  A buffer is refilled only after the barrier that
 // closes its last reading chunk.  Chunk 0 must already have landed in b0 (and b1 must be free) when the
 // workgroup calls this.
-template <int kCT, uint32_t kKeep, typename FI, typename FT>
-__device__ __forceinline__ void StreamChunks3(uint8_t* b0, uint8_t* b1, uint8_t* b2, uint32_t chunks, bool compute,
-                                              FI&& issue, FT&& per_tile) {
+template <uint32_t kKeep, typename FI, typename FC>
+__device__ __forceinline__ void StreamChunks3Whole(uint8_t* b0, uint8_t* b1, uint8_t* b2, uint32_t chunks, bool compute,
+                                                   FI&& issue, FC&& per_chunk) {
     WaitChunk();
     __syncthreads();
     if (chunks > 1) issue(b1, 1u);
     auto one = [&](const uint8_t* buf, uint8_t* next2, uint32_t c) __device__ {
         if (c + 2 < chunks) issue(next2, c + 2);
-        if (compute) {
-#pragma unroll
-            for (int k = 0; k < kCT; ++k) per_tile(buf, k, c * uint32_t(kCT) + uint32_t(k));
-        }
+        if (compute) per_chunk(buf, c);
         if (c + 2 < chunks) {
             WaitKeep<kKeep>();  // chunk c + 1 landed, chunk c + 2 may stay in flight
             BarrierKeepTransfers();
@@ -643,6 +640,15 @@ __device__ __forceinline__ void StreamChunks3(uint8_t* b0, uint8_t* b1, uint8_t*
         if (c + 1 < chunks) one(b1, b0, c + 1);
         if (c + 2 < chunks) one(b2, b1, c + 2);
     }
+}
+// The same with per_tile(buf, k, at) for the kCT tiles of each chunk.
+template <int kCT, uint32_t kKeep, typename FI, typename FT>
+__device__ __forceinline__ void StreamChunks3(uint8_t* b0, uint8_t* b1, uint8_t* b2, uint32_t chunks, bool compute,
+                                              FI&& issue, FT&& per_tile) {
+    StreamChunks3Whole<kKeep>(b0, b1, b2, chunks, compute, issue, [&](const uint8_t* buf, uint32_t c) __device__ {
+#pragma unroll
+        for (int k = 0; k < kCT; ++k) per_tile(buf, k, c * uint32_t(kCT) + uint32_t(k));
+    });
 }
 
 // StreamChunks with one callback per chunk: per_chunk(buf, c) for the buffer holding chunk c.
@@ -1948,8 +1954,8 @@ __device__ __forceinline__ void EvaluateStreamed(const EvalArgs& p) {
     static_assert(!kBf16 || (kListWaves * 2048 <= kBufBytes &&
                              sizeof(uint32_t) * kListWaves * 16 * kMaxCandidates <= kBufBytes),
                   "policy exchange and candidate tables fit a chunk buffer");
-    static_assert(!kRing || (kBf16 && kListWaves == 3 && kB16OpsDma + kB16CanonDma == 9),
-                  "the ring's waits count 3 transfers per wave and chunk");
+    static_assert(!kRing || (kBf16 && kListWaves == 3 && kB16OpsDma + kB16CanonDma == 9 && kTiles == 1),
+                  "the ring's waits count 3 transfers per wave and chunk (3-wave workgroups, one tile per wave)");
     __shared__ __align__(16) uint8_t s_buf0[kBufBytes];
     __shared__ __align__(16) uint8_t s_buf1[kBufBytes];
     __shared__ uint32_t s_item;

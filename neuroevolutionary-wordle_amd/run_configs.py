@@ -212,7 +212,8 @@ def wallclock_to_target(seeds, gens, cap_factor=2, forms=TARGET_FORMS, log=sys.s
     form's decisions), "bf16-graph" (the same, every step one captured hipGraph), "surrogate"
     (surrogate-screened evolution, brood 4x, best-predicted 25% + 5% audit, BF16-screened evaluation in the
     persistent kernel, plain steps while the audit's rank correlation is below 0.5) and "surrogate-rho0" (the
-    same gated at 0).  Host wall-clock per step, steps only.  Returns (per-seed records, aggregate)."""
+    same gated at 0).  Every form runs at the launch shape the autotuner picks for it on generation 0.  Host
+    wall-clock per step, steps only.  Returns (per-seed records, aggregate)."""
     wl = synth.workload(workload)
     A, side = wl["actions"], wl["side"]
     n = side * side
@@ -229,6 +230,9 @@ def wallclock_to_target(seeds, gens, cap_factor=2, forms=TARGET_FORMS, log=sys.s
             rt.upload_secrets_csr(wl["offsets"], wl["ids"])
             rt.set_eval_precision(mode)
             rt.bootstrap(n, seed=seed, variance_scale=1.0, tail_std=0.1)
+            # each form at its own autotuned launch shape (chosen on generation 0, before the timed steps;
+            # results do not depend on it)
+            rt.autotune_evaluation(counts)
             return rt
 
         def run(rt, step, limit, label, target=None):
