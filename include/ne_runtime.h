@@ -370,9 +370,12 @@ typedef struct ne_screen_report {
     uint32_t audited;            /* brood members in the audit slice */
     float audit_spearman;        /* rank correlation of prediction and fitness over the audit slice
                                     (predictions made before training on them; NaN below 3 members) */
-    float train_loss;            /* 1/2 mean squared error of the last minibatch (standardised fitness) */
+    float train_loss;            /* 1/2 mean squared error of the last minibatch (standardised fitness) of the
+                                    most recently completed training: the previous step's (a step's training
+                                    runs on its own stream, overlapping the next step's planning, brood
+                                    assembly and fingerprints; that step's predictions wait for it) */
     float fingerprint_ms;        /* encoder transpose + fingerprint kernel */
-    float surrogate_ms;          /* prediction + training */
+    float surrogate_ms;          /* prediction + the most recently completed training */
     float best_fitness;          /* of the new generation */
     float average_fitness;
     uint32_t reserved0;

@@ -214,6 +214,18 @@ struct ne_runtime {
     float* sg_coef1 = nullptr;  // {c1, c2} of a direct ne_surrogate_train_step
     hipGraph_t train_graph = nullptr;       // one generation's training steps, captured once
     hipGraphExec_t train_exec = nullptr;
+    // The training runs on its own stream (default priority), overlapping the next step's planning, brood
+    // assembly and fingerprints; the next prediction (and every surrogate call) waits for it on the device.
+    hipStream_t train_stream = nullptr;
+    hipEvent_t train_ready = nullptr;       // on the main stream: the replay rows of this step are in
+    hipEvent_t train_t0 = nullptr, train_done = nullptr;  // on the training stream
+    bool train_pending = false;             // a training not yet ordered before the main stream's later work
+    bool train_unread = false;              // its loss and time not yet read on the host
+    uint32_t* h_bidx = nullptr;             // pinned: minibatch indices, standardisation and bias corrections,
+    float* h_coef = nullptr;                //   and the last minibatch's outputs and targets (the loss)
+    float* h_loss = nullptr;
+    float last_loss = std::nanf("");        // of the most recently completed training
+    float last_train_ms = 0.0f;
 
     // telemetry
     FitnessStatsRecord* d_fstats = nullptr;
@@ -1245,22 +1257,49 @@ static void AdamCoef(uint64_t t, float* c) {
 
 // One Adam step of the surrogate on a device minibatch (x [n][192], t [n]); coef (device) holds its bias
 // corrections.  Launches only, so it can be captured.
-static void SurrogateStep(ne_runtime* rt, const float* x, const float* t, uint32_t n, const float* coef) {
-    LaunchSurrogateForward(rt->sg_params, x, n, rt->by, rt->bh1, rt->bh2, rt->stream);
-    LaunchSurrogateBackprop(rt->sg_params, rt->bh1, rt->bh2, rt->by, t, n, rt->bdy, rt->bdh1, rt->bdh2, rt->stream);
+static void SurrogateStep(ne_runtime* rt, const float* x, const float* t, uint32_t n, const float* coef,
+                          hipStream_t stream) {
+    LaunchSurrogateForward(rt->sg_params, x, n, rt->by, rt->bh1, rt->bh2, stream);
+    LaunchSurrogateBackprop(rt->sg_params, rt->bh1, rt->bh2, rt->by, t, n, rt->bdy, rt->bdh1, rt->bdh2, stream);
     LaunchSurrogateAdam(rt->sg_params, rt->sg_m, rt->sg_v, x, rt->bh1, rt->bh2, rt->bdy, rt->bdh1, rt->bdh2, n,
-                        rt->screen.learning_rate, coef, rt->stream);
+                        rt->screen.learning_rate, coef, stream);
 }
 
 // The launches of one generation's training (steps x {gather, forward, backprop, Adam}), reading their
 // indices, standardisation and bias corrections from device buffers filled before each run.
-static void TrainingLaunches(ne_runtime* rt) {
+static void TrainingLaunches(ne_runtime* rt, hipStream_t stream) {
     const uint32_t bs = rt->screen.batch_size;
     for (uint32_t st = 0; st < rt->screen.train_steps; ++st) {
         LaunchReplayGather(rt->replay_x, rt->replay_f, rt->bidx + uint64_t(st) * bs, bs, rt->sg_coef, rt->bx, rt->bt,
-                           rt->stream);
-        SurrogateStep(rt, rt->bx, rt->bt, bs, rt->sg_coef + 2 + 2 * uint64_t(st));
+                           stream);
+        SurrogateStep(rt, rt->bx, rt->bt, bs, rt->sg_coef + 2 + 2 * uint64_t(st), stream);
     }
+}
+
+// Orders the main stream's later work after the training in flight (device side).
+static int JoinTraining(ne_runtime* rt) {
+    if (!rt->train_pending) return NE_OK;
+    if (!Ok(hipStreamWaitEvent(rt->stream, rt->train_done, 0))) return Fail(rt, NE_DEVICE_FAILURE);
+    rt->train_pending = false;
+    return NE_OK;
+}
+
+// Waits on the host for the training in flight and reads its loss (1/2 MSE of its last minibatch) and time.
+static int ReadTraining(ne_runtime* rt) {
+    if (JoinTraining(rt) != NE_OK) return NE_DEVICE_FAILURE;
+    if (!rt->train_unread) return NE_OK;
+    if (!Ok(hipEventSynchronize(rt->train_done))) return Fail(rt, NE_DEVICE_FAILURE);
+    const uint32_t bs = rt->screen.batch_size;
+    double l = 0.0;
+    for (uint32_t b = 0; b < bs; ++b) {
+        const double d = double(rt->h_loss[b]) - double(rt->h_loss[bs + b]);
+        l += 0.5 * d * d;
+    }
+    rt->last_loss = float(l / double(bs));
+    float ms = 0.0f;
+    rt->last_train_ms = Ok(hipEventElapsedTime(&ms, rt->train_t0, rt->train_done)) ? ms : 0.0f;
+    rt->train_unread = false;
+    return NE_OK;
 }
 
 // Mean and standard deviation of the replay buffer's fitness (the training targets' standardisation).
@@ -1275,52 +1314,52 @@ static void ReplayStats(const ne_runtime* rt, float* mu, float* sigma) {
     *sigma = sd > 1e-12 ? float(sd) : 1.0f;
 }
 
-// train_steps minibatches drawn uniformly from the replay buffer; returns 1/2 MSE of the last one.
-static int TrainFromReplay(ne_runtime* rt, float* loss) {
-    *loss = std::nanf("");
+// train_steps minibatches drawn uniformly from the replay buffer, on the training stream after the main
+// stream's work so far (the replay rows appended): started here, not waited for (JoinTraining, ReadTraining).
+static int StartTraining(ne_runtime* rt) {
     const uint32_t bs = rt->screen.batch_size;
     if (rt->replay_count < bs) return NE_OK;
+    if (ReadTraining(rt) != NE_OK) return NE_DEVICE_FAILURE;  // the pinned buffers are free again
     float mu = 0.0f, sigma = 1.0f;
     ReplayStats(rt, &mu, &sigma);
     // every step's minibatch indices in one upload (draw of row b of step s: Philox(s, b; seed, tag)),
-    // then the steps back to back on the stream
+    // then the steps back to back on the training stream
     const uint32_t steps = rt->screen.train_steps;
-    std::vector<uint32_t> idx(uint64_t(steps) * bs);
     for (uint32_t st = 0; st < steps; ++st) {
         const uint64_t s_abs = rt->sg_step + 1 + st;
         for (uint32_t b = 0; b < bs; ++b)
-            idx[uint64_t(st) * bs + b] = Philox4x32(uint32_t(s_abs), b, uint32_t(s_abs >> 32), 0, rt->screen.seed,
-                                                    kTagBatch).x % uint32_t(rt->replay_count);
+            rt->h_bidx[uint64_t(st) * bs + b] = Philox4x32(uint32_t(s_abs), b, uint32_t(s_abs >> 32), 0,
+                                                           rt->screen.seed, kTagBatch).x % uint32_t(rt->replay_count);
     }
-    std::vector<float> coef(2 + 2 * uint64_t(steps));
-    coef[0] = mu;
-    coef[1] = 1.0f / sigma;
-    for (uint32_t st = 0; st < steps; ++st) AdamCoef(rt->sg_step + 1 + st, &coef[2 + 2 * st]);
-    if (!Ok(hipMemcpyAsync(rt->bidx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, rt->stream)) ||
-        !Ok(hipMemcpyAsync(rt->sg_coef, coef.data(), coef.size() * 4, hipMemcpyHostToDevice, rt->stream)))
+    rt->h_coef[0] = mu;
+    rt->h_coef[1] = 1.0f / sigma;
+    for (uint32_t st = 0; st < steps; ++st) AdamCoef(rt->sg_step + 1 + st, &rt->h_coef[2 + 2 * st]);
+    hipStream_t ts = rt->train_stream;
+    if (!Ok(hipEventRecord(rt->train_ready, rt->stream)) || !Ok(hipStreamWaitEvent(ts, rt->train_ready, 0)) ||
+        !Ok(hipEventRecord(rt->train_t0, ts)) ||
+        !Ok(hipMemcpyAsync(rt->bidx, rt->h_bidx, uint64_t(steps) * bs * 4, hipMemcpyHostToDevice, ts)) ||
+        !Ok(hipMemcpyAsync(rt->sg_coef, rt->h_coef, (2 + 2 * uint64_t(steps)) * 4, hipMemcpyHostToDevice, ts)))
         return Fail(rt, NE_DEVICE_FAILURE);
     if (rt->screen.flags & NE_SCREEN_EAGER_TRAINING) {
-        TrainingLaunches(rt);
+        TrainingLaunches(rt, ts);
     } else {
         // the steps as one hipGraph: captured on first use, replayed every generation (kernels only)
         if (!rt->train_exec) {
-            if (!Ok(hipStreamBeginCapture(rt->stream, hipStreamCaptureModeThreadLocal))) return Fail(rt, NE_DEVICE_FAILURE);
-            TrainingLaunches(rt);
-            if (!Ok(hipStreamEndCapture(rt->stream, &rt->train_graph)) ||
+            if (!Ok(hipStreamBeginCapture(ts, hipStreamCaptureModeThreadLocal))) return Fail(rt, NE_DEVICE_FAILURE);
+            TrainingLaunches(rt, ts);
+            if (!Ok(hipStreamEndCapture(ts, &rt->train_graph)) ||
                 !Ok(hipGraphInstantiate(&rt->train_exec, rt->train_graph, nullptr, nullptr, 0)))
                 return Fail(rt, NE_DEVICE_FAILURE);
         }
-        if (!Ok(hipGraphLaunch(rt->train_exec, rt->stream))) return Fail(rt, NE_DEVICE_FAILURE);
+        if (!Ok(hipGraphLaunch(rt->train_exec, ts))) return Fail(rt, NE_DEVICE_FAILURE);
     }
     rt->sg_step += steps;
-    std::vector<float> y(bs), t(bs);
-    if (!Ok(hipMemcpyAsync(y.data(), rt->by, bs * 4, hipMemcpyDeviceToHost, rt->stream)) ||
-        !Ok(hipMemcpyAsync(t.data(), rt->bt, bs * 4, hipMemcpyDeviceToHost, rt->stream)) ||
-        !Ok(hipStreamSynchronize(rt->stream)))
+    if (!Ok(hipMemcpyAsync(rt->h_loss, rt->by, bs * 4, hipMemcpyDeviceToHost, ts)) ||
+        !Ok(hipMemcpyAsync(rt->h_loss + bs, rt->bt, bs * 4, hipMemcpyDeviceToHost, ts)) ||
+        !Ok(hipEventRecord(rt->train_done, ts)))
         return Fail(rt, NE_DEVICE_FAILURE);
-    double l = 0.0;
-    for (uint32_t b = 0; b < bs; ++b) l += 0.5 * double(y[b] - t[b]) * double(y[b] - t[b]);
-    *loss = float(l / double(bs));
+    rt->train_pending = true;
+    rt->train_unread = true;
     return NE_OK;
 }
 
@@ -1522,6 +1561,7 @@ int ne_create(const ne_runtime_config* config, ne_runtime** out_runtime) {
 void ne_destroy(ne_runtime* rt) {
     if (!rt) return;
     if (rt->stream) (void)hipStreamSynchronize(rt->stream);
+    if (rt->train_stream) (void)hipStreamSynchronize(rt->train_stream);
     for (StepGraph& g : rt->step_graphs) DestroyStepGraph(g);
     Free(rt->d_step);
     Free(rt->d_readback);
@@ -1605,6 +1645,11 @@ void ne_destroy(ne_runtime* rt) {
         if (e.begin) (void)hipEventDestroy(e.begin);
         if (e.end) (void)hipEventDestroy(e.end);
     }
+    if (rt->train_stream) (void)hipStreamDestroy(rt->train_stream);
+    for (hipEvent_t e : {rt->train_ready, rt->train_t0, rt->train_done})
+        if (e) (void)hipEventDestroy(e);
+    for (void* h : {(void*)rt->h_bidx, (void*)rt->h_coef, (void*)rt->h_loss})
+        if (h) (void)hipHostFree(h);
     if (rt->own_stream && rt->stream) (void)hipStreamDestroy(rt->stream);
     delete rt;
 }
@@ -2403,6 +2448,25 @@ int ne_enable_screening(ne_runtime* rt, const ne_screen_config* config) {
     if (rt->catalog_count == 0) return Fail(rt, NE_INVALID_TRAINING_SHARD);
     if (!DrawProbes(rt, std::min<uint64_t>({rt->action_count, rt->catalog_count, kProbeVocabulary}), c.seed))
         return Fail(rt, NE_INVALID_TRAINING_SHARD);
+    if (rt->train_stream && !Ok(hipStreamSynchronize(rt->train_stream))) return Fail(rt, NE_DEVICE_FAILURE);
+    rt->train_pending = rt->train_unread = false;
+    if (!rt->train_stream &&
+        (!Ok(hipStreamCreateWithFlags(&rt->train_stream, hipStreamNonBlocking)) ||
+         !Ok(hipEventCreateWithFlags(&rt->train_ready, hipEventDisableTiming)) || !Ok(hipEventCreate(&rt->train_t0)) ||
+         !Ok(hipEventCreate(&rt->train_done))))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    for (void* h : {(void*)rt->h_bidx, (void*)rt->h_coef, (void*)rt->h_loss})
+        if (h) (void)hipHostFree(h);
+    rt->h_bidx = nullptr;
+    rt->h_coef = nullptr;
+    rt->h_loss = nullptr;
+    if (!Ok(hipHostMalloc(reinterpret_cast<void**>(&rt->h_bidx),
+                          std::max<uint64_t>(uint64_t(c.train_steps) * c.batch_size, 1) * 4)) ||
+        !Ok(hipHostMalloc(reinterpret_cast<void**>(&rt->h_coef), (2 + 2 * uint64_t(c.train_steps)) * 4)) ||
+        !Ok(hipHostMalloc(reinterpret_cast<void**>(&rt->h_loss), 2 * uint64_t(c.batch_size) * 4)))
+        return Fail(rt, NE_DEVICE_FAILURE);
+    rt->last_loss = std::nanf("");
+    rt->last_train_ms = 0.0f;
     const bool realloc = !rt->screen_on || rt->screen.replay_capacity != c.replay_capacity;
     const uint64_t bmax = std::max<uint64_t>(c.batch_size, 1024);
     if (realloc) {
@@ -2508,13 +2572,17 @@ int ne_advance_generation_screened(ne_runtime* rt, uint32_t generation_seed, con
     EventBegin(rt, 9);
     LaunchFingerprints(rt, rt->nxt_slots, uint32_t(BN), rt->pairs, rt->cur_fit, rt->fp);
     EventEnd(rt, 9);
+    // the previous step's training (on the training stream, overlapping this step so far) before the
+    // predictions
+    if (JoinTraining(rt) != NE_OK) return NE_DEVICE_FAILURE;
     EventBegin(rt, 10);
     LaunchSurrogateForward(rt->sg_params, rt->fp, uint32_t(BN), rt->pred, nullptr, nullptr, rt->stream);
     EventEnd(rt, 10);
     st = Finish(rt);
     if (st != NE_OK) return st;
+    if (ReadTraining(rt) != NE_OK) return NE_DEVICE_FAILURE;  // done by now: its loss and time
     rep.fingerprint_ms = EventMs(rt, 9);
-    float sg_ms = EventMs(rt, 10);
+    float sg_ms = EventMs(rt, 10) + rt->last_train_ms;
     std::vector<float> pred(BN);
     std::vector<uint32_t> brood_slots(BN);
     if (!Ok(hipMemcpy(pred.data(), rt->pred, BN * 4, hipMemcpyDeviceToHost)) ||
@@ -2588,11 +2656,11 @@ int ne_advance_generation_screened(ne_runtime* rt, uint32_t generation_seed, con
     for (uint64_t e = first; e < E; ++e) rt->h_replay_f[(rt->replay_head + e - first) % cap] = e_fit[e];
     rt->replay_head = (rt->replay_head + (E - first)) % cap;
     rt->replay_count = std::min<uint64_t>(rt->replay_count + (E - first), cap);
-    EventBegin(rt, 10);
-    float loss = std::nanf("");
-    st = TrainFromReplay(rt, &loss);
+    // training starts on its own stream and overlaps the rest of this step and the next step's planning,
+    // brood assembly and fingerprints; the next step's predictions wait for it
+    const float loss = rt->last_loss;  // of the most recently completed training (the previous step's)
+    st = StartTraining(rt);
     if (st != NE_OK) return st;
-    EventEnd(rt, 10);
 
     // the new generation: the selected children; every other brood member goes back to the free stack
     std::vector<uint32_t> keep(BN, 0), sel_slots(N);
@@ -2611,7 +2679,6 @@ int ne_advance_generation_screened(ne_runtime* rt, uint32_t generation_seed, con
     if (!ok) return Fail(rt, NE_DEVICE_FAILURE);
     st = Finish(rt);
     if (st != NE_OK) return st;
-    sg_ms += EventMs(rt, 10);
     rt->next_size = N;
     const ne_stage_times eval_times = rt->times;
     SwapGenerations(rt);
@@ -2698,8 +2765,10 @@ int ne_surrogate_predict(ne_runtime* rt, const float* x, uint64_t count, float* 
         Free(dx);
         return Fail(rt, NE_DEVICE_FAILURE);
     }
-    int st = Ok(hipMemcpyAsync(dx, x, count * kSgIn * 4, hipMemcpyHostToDevice, rt->stream)) ? NE_OK
-                                                                                             : Fail(rt, NE_DEVICE_FAILURE);
+    int st = (JoinTraining(rt) == NE_OK && Ok(hipMemcpyAsync(dx, x, count * kSgIn * 4, hipMemcpyHostToDevice,
+                                                                rt->stream)))
+                 ? NE_OK
+                 : Fail(rt, NE_DEVICE_FAILURE);
     if (st == NE_OK) {
         LaunchSurrogateForward(rt->sg_params, dx, uint32_t(count), dy, nullptr, nullptr, rt->stream);
         st = Finish(rt);
@@ -2714,13 +2783,14 @@ int ne_surrogate_train_step(ne_runtime* rt, const float* x, const float* t, uint
     if (!rt || !x || !t || count == 0) return Fail(rt, NE_INVALID_ARGUMENT);
     if (!rt->screen_on) return Fail(rt, NE_UNSUPPORTED);
     if (count > std::max<uint64_t>(rt->screen.batch_size, 1024)) return Fail(rt, NE_INVALID_ARGUMENT);
+    if (ReadTraining(rt) != NE_OK) return NE_DEVICE_FAILURE;  // its batch buffers are reused here
     float coef[2];
     AdamCoef(++rt->sg_step, coef);
     if (!Ok(hipMemcpyAsync(rt->bx, x, count * kSgIn * 4, hipMemcpyHostToDevice, rt->stream)) ||
         !Ok(hipMemcpyAsync(rt->bt, t, count * 4, hipMemcpyHostToDevice, rt->stream)) ||
         !Ok(hipMemcpyAsync(rt->sg_coef1, coef, sizeof(coef), hipMemcpyHostToDevice, rt->stream)))
         return Fail(rt, NE_DEVICE_FAILURE);
-    SurrogateStep(rt, rt->bx, rt->bt, uint32_t(count), rt->sg_coef1);
+    SurrogateStep(rt, rt->bx, rt->bt, uint32_t(count), rt->sg_coef1, rt->stream);
     return Finish(rt);
 }
 
@@ -2729,7 +2799,8 @@ int ne_surrogate_get_params(ne_runtime* rt, float* dst, uint64_t capacity, uint6
     if (!rt->screen_on) return Fail(rt, NE_UNSUPPORTED);
     if (count) *count = kSgParams;
     if (capacity < kSgParams) return Fail(rt, NE_INVALID_ARGUMENT);
-    if (Sync(rt) != NE_OK || !Ok(hipMemcpy(dst, rt->sg_params, kSgParams * 4, hipMemcpyDeviceToHost)))
+    if (JoinTraining(rt) != NE_OK || Sync(rt) != NE_OK ||
+        !Ok(hipMemcpy(dst, rt->sg_params, kSgParams * 4, hipMemcpyDeviceToHost)))
         return Fail(rt, NE_DEVICE_FAILURE);
     return NE_OK;
 }
@@ -2737,7 +2808,8 @@ int ne_surrogate_get_params(ne_runtime* rt, float* dst, uint64_t capacity, uint6
 int ne_surrogate_set_params(ne_runtime* rt, const float* src, uint64_t count) {
     if (!rt || !src || count != kSgParams) return Fail(rt, NE_INVALID_ARGUMENT);
     if (!rt->screen_on) return Fail(rt, NE_UNSUPPORTED);
-    bool ok = Ok(hipMemcpyAsync(rt->sg_params, src, kSgParams * 4, hipMemcpyHostToDevice, rt->stream));
+    bool ok = JoinTraining(rt) == NE_OK;
+    ok &= Ok(hipMemcpyAsync(rt->sg_params, src, kSgParams * 4, hipMemcpyHostToDevice, rt->stream));
     ok &= Ok(hipMemsetAsync(rt->sg_m, 0, kSgParams * 4, rt->stream));
     ok &= Ok(hipMemsetAsync(rt->sg_v, 0, kSgParams * 4, rt->stream));
     if (!ok) return Fail(rt, NE_DEVICE_FAILURE);
