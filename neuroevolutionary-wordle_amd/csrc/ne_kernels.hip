@@ -3291,6 +3291,8 @@ void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, uint32_t
                     hipStream_t s) {
     const dim3 grid(blocks);
     const dim3 block(64 * waves);
+    // the BF16 form's streamed kernel is the ring, which counts exactly 3 transfers per wave and chunk
+    kernel = EvaluateKernelFor(kernel, waves, trace, bf16);
     if (!trace && kernel == kEvalKernelPaired) {
         if (bf16)
             hipLaunchKernelGGL(EvaluatePairBf16Kernel, grid, block, 0, s, a);
