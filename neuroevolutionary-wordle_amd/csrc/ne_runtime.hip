@@ -149,7 +149,7 @@ struct ne_runtime {
     uint64_t trace_capacity = 0;
 
     // timing
-    StageEvents ev[13];  // 12: the repack's gather and scatter copies
+    StageEvents ev[14];  // 12: the repack's gather and scatter copies; 13: the brood's second phase
     ne_stage_times times{};
     ne_fitness_summary last_summary{};
     bool have_summary = false;
@@ -729,18 +729,14 @@ static void RecordEvaluation(ne_runtime* rt, const SummaryRecord& sr, const unsi
     rt->have_summary = true;
 }
 
-static int EvaluateCore(ne_runtime* rt, uint64_t action_space, const uint32_t* slots, uint64_t n,
-                        const std::vector<uint32_t>& offs, const std::vector<uint32_t>& secs, float* out_fit,
-                        uint32_t* out_evals, uint8_t* out_has, uint64_t generation_index) {
-    EvalSetup s;
-    int st = PrepareEvaluation(rt, action_space, slots, n, offs, secs, &s);
-    if (st != NE_OK) return st;
+// The device half of a prepared evaluation and its host record: every launch, then one wait (a failing
+// kernel's status is the first one set, so finalisation behind it cannot hide it).
+static int RunEvaluation(ne_runtime* rt, const EvalSetup& s, const uint32_t* slots, uint64_t n, float* out_fit,
+                         uint32_t* out_evals, uint8_t* out_has, uint64_t generation_index) {
     LaunchEvaluation(rt, s, n, out_fit, out_evals, out_has, true);
-    if (!Ok(hipGetLastError())) return Fail(rt, NE_DEVICE_FAILURE);
-    st = Finish(rt);
-    if (st != NE_OK) return st;
     LaunchFinalizeSummary(rt, slots, n, out_fit, out_evals, out_has, true);
-    st = Finish(rt);
+    if (!Ok(hipGetLastError())) return Fail(rt, NE_DEVICE_FAILURE);
+    int st = Finish(rt);
     if (st != NE_OK) return st;
     unsigned long long counters[4] = {0, 0, 0, 0};
     SummaryRecord sr{};
@@ -753,6 +749,15 @@ static int EvaluateCore(ne_runtime* rt, uint64_t action_space, const uint32_t* s
     RecordEvaluation(rt, sr, counters, s.total_games, n, generation_index);
     rt->last_status = NE_OK;
     return NE_OK;
+}
+
+static int EvaluateCore(ne_runtime* rt, uint64_t action_space, const uint32_t* slots, uint64_t n,
+                        const std::vector<uint32_t>& offs, const std::vector<uint32_t>& secs, float* out_fit,
+                        uint32_t* out_evals, uint8_t* out_has, uint64_t generation_index) {
+    EvalSetup s;
+    const int st = PrepareEvaluation(rt, action_space, slots, n, offs, secs, &s);
+    if (st != NE_OK) return st;
+    return RunEvaluation(rt, s, slots, n, out_fit, out_evals, out_has, generation_index);
 }
 
 
@@ -1424,30 +1429,27 @@ static int AssembleBrood(ne_runtime* rt, uint32_t seed, const ne_assembly_config
     LaunchBreed(b, uint32_t(BN), rt->stream);
     EventEnd(rt, 3);
     LaunchPopSlots(rt->free_count, uint32_t(BN), rt->stream);
-    st = Finish(rt);
-    if (st != NE_OK) return st;
-    rt->times.assemble_ms = EventMs(rt, 3);
+    // not waited for: the caller's next wait checks the status and reads event 3
     rt->times.children = BN;
-    return NE_OK;
+    return Ok(hipGetLastError()) ? NE_OK : Fail(rt, NE_DEVICE_FAILURE);
 }
 
-// Phase 2: the rest of each listed child (device list of `count` brood rows), in its own slot.
+// Phase 2: the rest of each listed child (device list of `count` brood rows), in its own slot.  Launched
+// inside event 13, not waited for.
 static int BreedRest(ne_runtime* rt, uint32_t seed, const ne_assembly_config& cfg, const uint32_t* rows,
                      uint64_t count) {
     const uint32_t begin = BroodPhase1End(rt), end = uint32_t(rt->stride / 4);
-    if (begin >= end || count == 0) return NE_OK;
-    BreedArgs b = BroodBreedArgs(rt, seed, cfg);
-    b.elem_begin = begin;
-    b.elem_end = end;
-    b.child_list = rows;
-    b.given_slots = rt->nxt_slots;
-    EventBegin(rt, 3);
-    LaunchBreed(b, uint32_t(count), rt->stream);
-    EventEnd(rt, 3);
-    const int st = Finish(rt);
-    if (st != NE_OK) return st;
-    rt->times.assemble_ms += EventMs(rt, 3);
-    return NE_OK;
+    EventBegin(rt, 13);
+    if (begin < end && count != 0) {
+        BreedArgs b = BroodBreedArgs(rt, seed, cfg);
+        b.elem_begin = begin;
+        b.elem_end = end;
+        b.child_list = rows;
+        b.given_slots = rt->nxt_slots;
+        LaunchBreed(b, uint32_t(count), rt->stream);
+    }
+    EventEnd(rt, 13);
+    return Ok(hipGetLastError()) ? NE_OK : Fail(rt, NE_DEVICE_FAILURE);
 }
 
 // Fingerprints of n genomes (device slot list) into out (device [n][192]).
@@ -2578,8 +2580,17 @@ int ne_advance_generation_screened(ne_runtime* rt, uint32_t generation_seed, con
     EventBegin(rt, 10);
     LaunchSurrogateForward(rt->sg_params, rt->fp, uint32_t(BN), rt->pred, nullptr, nullptr, rt->stream);
     EventEnd(rt, 10);
+    // the audit: a Philox draw per brood member (keyed by member and parent generation), drawn while the
+    // device assembles and fingerprints the brood
+    std::vector<uint8_t> audited(BN, 0);
+    for (uint64_t i = 0; i < BN; ++i) {
+        const U4 w = Philox4x32(uint32_t(i), uint32_t(rt->cur_index), uint32_t(rt->cur_index >> 32), 0,
+                                rt->screen.seed, kTagAudit);
+        audited[i] = UniformOpenClosed(w.x) <= rt->screen.audit_fraction ? 1 : 0;
+    }
     st = Finish(rt);
     if (st != NE_OK) return st;
+    if (screened) rt->times.assemble_ms = EventMs(rt, 3);  // the brood's first phase
     if (ReadTraining(rt) != NE_OK) return NE_DEVICE_FAILURE;  // done by now: its loss and time
     rep.fingerprint_ms = EventMs(rt, 9);
     float sg_ms = EventMs(rt, 10) + rt->last_train_ms;
@@ -2589,8 +2600,7 @@ int ne_advance_generation_screened(ne_runtime* rt, uint32_t generation_seed, con
         !Ok(hipMemcpy(brood_slots.data(), rt->nxt_slots, BN * 4, hipMemcpyDeviceToHost)))
         return Fail(rt, NE_DEVICE_FAILURE);
 
-    // selection: each cell's best-predicted child (lowest brood index on ties); audit: a Philox draw
-    // per brood member (keyed by member and parent generation)
+    // selection: each cell's best-predicted child (lowest brood index on ties), then the audited members
     std::vector<uint32_t> rows_e;
     std::vector<int64_t> entry_of(BN, -1);
     rows_e.reserve(BN);
@@ -2603,9 +2613,7 @@ int ne_advance_generation_screened(ne_runtime* rt, uint32_t generation_seed, con
     }
     std::vector<uint32_t> audit;
     for (uint64_t i = 0; i < BN; ++i) {
-        const U4 w = Philox4x32(uint32_t(i), uint32_t(rt->cur_index), uint32_t(rt->cur_index >> 32), 0,
-                                rt->screen.seed, kTagAudit);
-        if (!(UniformOpenClosed(w.x) <= rt->screen.audit_fraction)) continue;
+        if (!audited[i]) continue;
         audit.push_back(uint32_t(i));
         if (entry_of[i] < 0) {
             entry_of[i] = int64_t(rows_e.size());
@@ -2630,12 +2638,17 @@ int ne_advance_generation_screened(ne_runtime* rt, uint32_t generation_seed, con
         if (!Ok(hipMemsetAsync(rt->refcounts, 0, N * sizeof(uint32_t), rt->stream))) return Fail(rt, NE_DEVICE_FAILURE);
         LaunchCollectReleased(rt->cur_slots, uint32_t(N), rt->refcounts, rt->slot_state, rt->free_stack,
                               rt->free_count, uint32_t(rt->slot_count), rt->status, rt->stream);
-        st = Finish(rt);
-        if (st != NE_OK) return st;
     }
+    // the evaluation's host preparation while the device completes the brood, then one wait (the
+    // evaluation clears the status word, so the brood's status is read first)
+    EvalSetup eval_setup;
+    st = PrepareEvaluation(rt, counts->action_space_word_count, rt->e_slots, E, eo, es, &eval_setup);
+    if (st != NE_OK) return st;
+    st = Finish(rt);
+    if (st != NE_OK) return st;
+    if (screened) rt->times.assemble_ms += EventMs(rt, 13);
     const float assemble_ms = rt->times.assemble_ms, book_ms = rt->times.bookkeeping_ms;
-    st = EvaluateCore(rt, counts->action_space_word_count, rt->e_slots, E, eo, es, rt->e_fit, rt->e_evals, rt->e_has,
-                      rt->cur_index + 1);
+    st = RunEvaluation(rt, eval_setup, rt->e_slots, E, rt->e_fit, rt->e_evals, rt->e_has, rt->cur_index + 1);
     if (st != NE_OK) return st;
     std::vector<float> e_fit(E);
     if (!Ok(hipMemcpy(e_fit.data(), rt->e_fit, E * 4, hipMemcpyDeviceToHost))) return Fail(rt, NE_DEVICE_FAILURE);
@@ -2676,6 +2689,8 @@ int ne_advance_generation_screened(ne_runtime* rt, uint32_t generation_seed, con
     ok &= Ok(hipMemcpyAsync(rt->nxt_fit, rt->e_fit, N * sizeof(float), hipMemcpyDeviceToDevice, rt->stream));
     ok &= Ok(hipMemcpyAsync(rt->nxt_evals, rt->e_evals, N * sizeof(uint32_t), hipMemcpyDeviceToDevice, rt->stream));
     ok &= Ok(hipMemcpyAsync(rt->nxt_has, rt->e_has, N, hipMemcpyDeviceToDevice, rt->stream));
+    // the new generation's summary behind the copies, one wait for both
+    LaunchSummary(rt->nxt_fit, rt->nxt_has, rt->nxt_slots, uint32_t(N), rt->summary, rt->status, rt->stream);
     if (!ok) return Fail(rt, NE_DEVICE_FAILURE);
     st = Finish(rt);
     if (st != NE_OK) return st;
@@ -2684,9 +2699,6 @@ int ne_advance_generation_screened(ne_runtime* rt, uint32_t generation_seed, con
     SwapGenerations(rt);
     rt->cur_evaluated = true;
     RecordFitnessInputs(rt, counts->action_space_word_count, *offs, *secs, N);
-    LaunchSummary(rt->cur_fit, rt->cur_has, rt->cur_slots, uint32_t(N), rt->summary, rt->status, rt->stream);
-    st = Finish(rt);
-    if (st != NE_OK) return st;
     SummaryRecord sr{};
     if (!Ok(hipMemcpy(&sr, rt->summary, sizeof(sr), hipMemcpyDeviceToHost))) return Fail(rt, NE_DEVICE_FAILURE);
     rt->last_summary.best_fitness = sr.best_fitness;
