@@ -2159,7 +2159,29 @@ __device__ __forceinline__ void EvaluateStreamed(const EvalArgs& p) {
                                                                              4, c * (kB16Tiles * 64) + 4 * lane, 0, 0, 0);
                             }
                         };
-                        StreamChunks3<kB16Tiles, 3>(s_buf0, s_buf1, s_buf2, chunks16, any_want, issue16_3, screen_tile);
+                        // software-pipelined: tile k + 1's operand reads and instructions are issued before tile k's
+                        // rows are screened, so their latency overlaps that screening (the same scores, screened
+                        // in the same order)
+                        static_assert(!kRing || !kTwo, "one game tile per wave");
+                        auto screen_chunk = [&](const uint8_t* buf, uint32_t c) __device__ {
+                            ShareRunningMax(ts0);
+                            const bool ci = p.canon_identity != 0;
+                            bf16x8 a0, a1;
+                            Bf16Operands(buf, 0, lane, a0, a1);
+                            v4f sc = Bf16Mfma(a0, a1, ts0.pb);
+#pragma unroll
+                            for (int k = 0; k < int(kB16Tiles); ++k) {
+                                v4f scn = sc;
+                                if (k + 1 < int(kB16Tiles)) {
+                                    Bf16Operands(buf, k + 1, lane, a0, a1);
+                                    scn = Bf16Mfma(a0, a1, ts0.pb);
+                                }
+                                ScreenTileRows<kLC>(sc, c * kB16Tiles + uint32_t(k), Bf16Canon(buf, k, gq), ci, gq, A,
+                                                    g0.hist, ts0, s_lsc[wave][0], s_lac[wave][0], lane);
+                                sc = scn;
+                            }
+                        };
+                        StreamChunks3Whole<3>(s_buf0, s_buf1, s_buf2, chunks16, any_want, issue16_3, screen_chunk);
                     } else {
                         StreamChunks<kB16Tiles>(s_buf0, s_buf1, chunks16, any_want, issue16, screen_tile);
                     }
