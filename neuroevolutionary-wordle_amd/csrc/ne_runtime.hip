@@ -184,6 +184,10 @@ struct ne_runtime {
     uint64_t input_epoch = 1;               // bumped by every catalog or secrets upload
     EvalSetup cached_eval{};
     uint64_t cached_eval_key[10] = {};
+    // the same for eager evaluations (EvaluateCore): the key, and the per-cell lists the buffers hold
+    EvalSetup eager_eval{};
+    uint64_t eager_eval_key[10] = {};
+    std::vector<uint32_t> eager_offsets, eager_secrets;
 
     // surrogate screening
     bool screen_on = false;
@@ -729,20 +733,29 @@ static void RecordEvaluation(ne_runtime* rt, const SummaryRecord& sr, const unsi
     rt->have_summary = true;
 }
 
-// The device half of a prepared evaluation and its host record: every launch, then one wait (a failing
-// kernel's status is the first one set, so finalisation behind it cannot hide it).
+// The device half of a prepared evaluation and its host record: every launch, the status, summary and counters
+// gathered into one block, then one wait and one read (a failing kernel's status is the first one set, so
+// finalisation behind it cannot hide it).
 static int RunEvaluation(ne_runtime* rt, const EvalSetup& s, const uint32_t* slots, uint64_t n, float* out_fit,
                          uint32_t* out_evals, uint8_t* out_has, uint64_t generation_index) {
+    if (!rt->d_readback && (!Alloc(&rt->d_readback, 1) ||
+                            !Ok(hipHostMalloc((void**)&rt->h_readback, sizeof(StepReadback)))))
+        return Fail(rt, NE_DEVICE_FAILURE);
     LaunchEvaluation(rt, s, n, out_fit, out_evals, out_has, true);
     LaunchFinalizeSummary(rt, slots, n, out_fit, out_evals, out_has, true);
-    if (!Ok(hipGetLastError())) return Fail(rt, NE_DEVICE_FAILURE);
-    int st = Finish(rt);
-    if (st != NE_OK) return st;
-    unsigned long long counters[4] = {0, 0, 0, 0};
-    SummaryRecord sr{};
-    if (!Ok(hipMemcpy(counters, rt->counters, sizeof(counters), hipMemcpyDeviceToHost)) ||
-        !Ok(hipMemcpy(&sr, rt->summary, sizeof(sr), hipMemcpyDeviceToHost)))
+    LaunchStepReadback(rt->status, rt->free_count, rt->summary, rt->counters, rt->d_readback, rt->stream);
+    if (!Ok(hipGetLastError()) ||
+        !Ok(hipMemcpyAsync(rt->h_readback, rt->d_readback, sizeof(StepReadback), hipMemcpyDeviceToHost, rt->stream)))
         return Fail(rt, NE_DEVICE_FAILURE);
+    if (Sync(rt) != NE_OK) return NE_DEVICE_FAILURE;
+    const StepReadback rb = *rt->h_readback;
+    if (rb.status != 0) {
+        (void)hipMemset(rt->status, 0, sizeof(int));
+        return Fail(rt, rb.status);
+    }
+    unsigned long long counters[4];
+    std::memcpy(counters, rb.counters, sizeof(counters));
+    const SummaryRecord sr = rb.summary;
     rt->times.evaluate_ms = EventMs(rt, 0);
     rt->times.prepare_ms = EventMs(rt, 8);
     rt->times.finalize_ms = EventMs(rt, 1);
@@ -751,12 +764,45 @@ static int RunEvaluation(ne_runtime* rt, const EvalSetup& s, const uint32_t* slo
     return NE_OK;
 }
 
+// What an evaluation's host preparation depends on besides the per-cell lists: the inputs' epoch (any other
+// preparation or upload since changes it), the size, action space, form and launch shape, and the slab.
+static void EagerEvalKey(const ne_runtime* rt, uint64_t action_space, uint64_t n, uint64_t (&k)[10]) {
+    const uint64_t v[10] = {rt->input_epoch, n, std::min(rt->action_count, action_space), rt->eval_mode,
+                            rt->tuned_waves | (uint64_t(rt->tuned_rounds) << 16) |
+                                (uint64_t(rt->tuned_wg_per_cu) << 32) | (uint64_t(rt->tuned_kernel) << 48),
+                            uint64_t(uintptr_t(rt->slab)), rt->stride, rt->slot_count, rt->action_count, rt->max_gen};
+    std::memcpy(k, v, sizeof(v));
+}
+
+// An evaluation of n genomes: the host preparation (validation, the CSR and work-list uploads) is kept while
+// nothing else has prepared an evaluation or uploaded inputs since, the key is the same and the per-cell
+// lists are equal (a generation step evaluates the same cells' secrets generation after generation); traced
+// evaluations always prepare (their trace buffers are cleared there).
 static int EvaluateCore(ne_runtime* rt, uint64_t action_space, const uint32_t* slots, uint64_t n,
                         const std::vector<uint32_t>& offs, const std::vector<uint32_t>& secs, float* out_fit,
                         uint32_t* out_evals, uint8_t* out_has, uint64_t generation_index) {
+    const bool trace = (rt->trace_flags & (NE_TRACE_GAMES | NE_TRACE_POLICY)) != 0;
+    uint64_t key[10];
+    EagerEvalKey(rt, action_space, n, key);
+    const bool same_lists = offs.size() >= n + 1 && rt->eager_offsets.size() == n + 1 &&
+                            std::equal(offs.begin(), offs.begin() + n + 1, rt->eager_offsets.begin()) &&
+                            secs.size() >= offs[n] && rt->eager_secrets.size() == offs[n] &&
+                            std::equal(secs.begin(), secs.begin() + offs[n], rt->eager_secrets.begin());
     EvalSetup s;
-    const int st = PrepareEvaluation(rt, action_space, slots, n, offs, secs, &s);
-    if (st != NE_OK) return st;
+    if (!trace && same_lists && std::memcmp(key, rt->eager_eval_key, sizeof(key)) == 0) {
+        s = rt->eager_eval;
+        s.a.slot_indices = slots;
+    } else {
+        std::memset(rt->eager_eval_key, 0, sizeof(rt->eager_eval_key));
+        const int st = PrepareEvaluation(rt, action_space, slots, n, offs, secs, &s);
+        if (st != NE_OK) return st;
+        if (!trace) {
+            rt->eager_eval = s;
+            EagerEvalKey(rt, action_space, n, rt->eager_eval_key);  // after the preparation's own epoch bump
+            rt->eager_offsets.assign(offs.begin(), offs.begin() + n + 1);
+            rt->eager_secrets.assign(secs.begin(), secs.begin() + offs[n]);
+        }
+    }
     return RunEvaluation(rt, s, slots, n, out_fit, out_evals, out_has, generation_index);
 }
 
