@@ -393,17 +393,24 @@ __global__ void TransposeEncoderKernel(const uint8_t* slab, uint64_t stride, con
                                        uint32_t slot_count, float* enc_t) {
     const uint32_t slot = slot_indices[blockIdx.x];
     if (slot >= slot_count) return;  // reported by the evaluation kernel
-    __shared__ float s_w[kEncOut * kTurnFeatures];  // [out][feature], as stored in the genome
+    __shared__ __align__(16) float s_w[kEncOut * kTurnFeatures];  // [out][feature], as stored in the genome
     const float* genome = reinterpret_cast<const float*>(slab + uint64_t(slot) * stride);
     float* dst = enc_t + uint64_t(blockIdx.x) * kEncTStride;
+    static_assert(kOffEncW % 4 == 0 && (kEncOut * kTurnFeatures) % 4 == 0 && kEncTStride % 4 == 0 && kEncOut % 4 == 0,
+                  "16-byte moves");
     for (uint32_t o = threadIdx.x; o < uint32_t(kH0); o += blockDim.x)
         dst[kTurnFeatures * kEncOut + o] = genome[kOffW0 + o * kModelIn];
-    for (uint32_t idx = threadIdx.x; idx < uint32_t(kEncOut * kTurnFeatures); idx += blockDim.x)
-        s_w[idx] = genome[kOffEncW + idx];
+    // 16-byte loads into LDS, then 16-byte stores of four consecutive outputs of one feature
+    constexpr uint32_t kVec = uint32_t(kEncOut * kTurnFeatures) / 4;
+    for (uint32_t v = threadIdx.x; v < kVec; v += blockDim.x)
+        *reinterpret_cast<v4f*>(s_w + 4 * v) = Load4(genome + kOffEncW + 4 * v);
     __syncthreads();
-    for (uint32_t idx = threadIdx.x; idx < uint32_t(kEncOut * kTurnFeatures); idx += blockDim.x) {
-        const uint32_t k = idx / kEncOut, o = idx - k * kEncOut;
-        dst[idx] = s_w[o * kTurnFeatures + k];  // 145 = 17 (mod 64): consecutive o hit distinct banks
+    for (uint32_t v = threadIdx.x; v < kVec; v += blockDim.x) {
+        const uint32_t k = (4 * v) / kEncOut, o = 4 * v - k * kEncOut;
+        v4f t;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) t[i] = s_w[(o + uint32_t(i)) * kTurnFeatures + k];
+        *reinterpret_cast<v4f*>(dst + 4 * v) = t;
     }
 }
 
