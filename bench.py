@@ -230,6 +230,7 @@ def main():
 
     fp32 = run_form(rtm.EVAL_FP32)
     bf16 = run_form(rtm.EVAL_BF16_RESCORED) if not args.fp32_only else None
+    trunk16 = run_form(rtm.EVAL_BF16_TRUNK) if not args.fp32_only else None
     fp32_graph = run_form(rtm.EVAL_FP32, graph=True)
     bf16_graph = run_form(rtm.EVAL_BF16_RESCORED, graph=True) if not args.fp32_only else None
     rt.close()
@@ -267,6 +268,27 @@ def main():
             "same_fitness_as_fp32": bool(np.array_equal(bf16["fitness"].view(np.uint32),
                                                         fp32["fitness"].view(np.uint32))),
         }
+    t16_out = None
+    if trunk16 is not None:
+        sh = shapes.get((rtm.EVAL_BF16_TRUNK, False), {})
+        kt = sh.get("kernel_id", 0)
+        if kt == 1 and sh.get("waves") != 3:
+            kt = 0
+        ft = trunk16["flops"] / (trunk16["eval_ms"] * 1e-3) / 1e12
+        # not bit-identical to the FP32 form: the trunk's split-BF16 products differ by ~2^-16 relative, so the
+        # steps' trajectories part after the first differing decision (decision agreement at full size is a GPU
+        # test); fitness agreement is reported on the last generation of the same steps
+        t16_out = {
+            "value": games_per_gen * args.steps / trunk16["elapsed"], "unit": "game-evals/s",
+            "ms_per_step": trunk16["elapsed"] / args.steps * 1e3, "generations_per_s": args.steps / trunk16["elapsed"],
+            "evaluate_ms": trunk16["eval_ms"], "kernel": KERNEL_NAMES_BF16[kt],
+            "roofline": {"achieved_tflops": ft, "frac_of_bf16_peak": ft / PEAK_BF16_MFMA_TFLOPS},
+            "candidates_rescored_per_decision": trunk16["rescored"] / max(1, trunk16["decisions"]),
+            "fp32_fallback_decisions": trunk16["fallbacks"],
+            "fitness_equal_to_fp32_fraction": float(np.mean(trunk16["fitness"] == fp32["fitness"])),
+            "best_fitness": float(trunk16["summary"].best_fitness),
+        }
+
     def graph_record(rec, eager):
         return {"value": games_per_gen * args.steps / rec["elapsed"], "unit": "game-evals/s",
                 "ms_per_step": rec["elapsed"] / args.steps * 1e3, "generations_per_s": args.steps / rec["elapsed"],
@@ -305,9 +327,11 @@ def main():
         "generations_per_s": args.steps / fp32["elapsed"],
         "decisions_per_generation": fp32["decisions"] / args.steps,
         "bf16_screened_fp32_rescored": bf16_out,
+        "bf16_trunk_bf16_screened": t16_out,
         "captured_step_graph": graph_out,
         "best_fitness": summary.best_fitness,
-        "launch_shapes": {("fp32" if m == rtm.EVAL_FP32 else "bf16") + ("_graph" if g else ""):
+        "launch_shapes": {{rtm.EVAL_FP32: "fp32", rtm.EVAL_BF16_RESCORED: "bf16", rtm.EVAL_BF16_TRUNK: "bf16_trunk"}[m]
+                          + ("_graph" if g else ""):
                           {k: x for k, x in v.items() if k != "kernel_id"} for (m, g), v in shapes.items()},
         "cpu_baseline": cpu,
     }

@@ -174,6 +174,11 @@ typedef struct ne_stage_times {
 enum {
     NE_EVAL_FP32 = 0,
     NE_EVAL_BF16_RESCORED = 1,
+    NE_EVAL_BF16_TRUNK = 2,          /* BF16_RESCORED with the trunk's layers 1 and 2 (256->128->64) on BF16
+                                        MFMA, FP32 accumulate: the candidates are rescored in FP32 against
+                                        the policy this trunk produces, so guesses equal the FP32 mode's
+                                        wherever the trunk's rounding does not reorder the top candidates
+                                        (agreement is measured, not exact) */
     NE_EVAL_FORCE_FALLBACK = 0x100   /* with BF16: settle every decision by the FP32 scan (diagnostic) */
 };
 

@@ -20,6 +20,31 @@ __device__ inline void SetStatus(int* status, int code) { atomicCAS(status, 0, c
 
 __device__ inline v4f Mfma(float a, float b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
+// BF16 trunk (layers 1 and 2 of the NE_EVAL_BF16_TRUNK form) on split operands: every weight w and every
+// ReLU'd input x is split into hi = bf16(v) and lo = bf16(v - hi) (round to nearest even; v - hi is exact in
+// fp32), and w.x is taken as three products lo_w.hi_x + hi_w.lo_x + hi_w.hi_x on v_mfma_f32_16x16x16_bf16
+// (FP32 accumulate; the dropped lo.lo term is ~2^-16 of the product).  The instruction takes, per lane, the
+// 4 k-values that the four v_mfma_f32_16x16x4_f32 instructions r = 0..3 of the FP32 trunk take one by one
+// (lane group g holds k-slots 4g..4g+3 of both operands), so the FP32 trunk's operand reads and accumulator
+// layout serve it unchanged.
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+struct Split16 {
+    v4s hi, lo;
+};
+__device__ __forceinline__ Split16 SplitBf16(float x0, float x1, float x2, float x3) {
+    const bf16x4 h = {__bf16(x0), __bf16(x1), __bf16(x2), __bf16(x3)};
+    const bf16x4 l = {__bf16(x0 - float(h[0])), __bf16(x1 - float(h[1])), __bf16(x2 - float(h[2])),
+                      __bf16(x3 - float(h[3]))};
+    return {__builtin_bit_cast(v4s, h), __builtin_bit_cast(v4s, l)};
+}
+__device__ __forceinline__ v4f MfmaT16(const v4f& w, const Split16& x, v4f c) {
+    const Split16 ws = SplitBf16(w[0], w[1], w[2], w[3]);
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ws.lo, x.hi, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ws.hi, x.lo, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ws.hi, x.hi, c, 0, 0, 0);
+}
+
 __device__ inline v4f Load4(const float* p) { return *reinterpret_cast<const v4f*>(p); }
 
 // Raw buffer loads: one 128-bit descriptor (SGPRs) per genome, a 32-bit per-lane byte offset and a
@@ -292,7 +317,7 @@ static_assert(kTrunkReadBytes == kTrunkChunkBytes, "layer-2 reads stay inside th
 // Layers 1 and 2 for the whole workgroup (every wave calls it; `compute` = this wave has games).
 // Chunk 0 must be in flight into b0; chunk i uses b0 when i is even, b1 when odd.  During the last
 // chunk (in b1) next_stage(b0) starts whatever follows (the scoring stream's first chunk).
-template <typename FN>
+template <bool kT16 = false, typename FN>
 __device__ __forceinline__ void TrunkStream(const v4f (&acc)[16], const TrunkLane& tl, float (&pol)[kScoreSteps],
                                             bool compute, uint8_t* b0, uint8_t* b1, uint32_t wave, uint32_t waves,
                                             uint32_t lane, FN&& next_stage) {
@@ -328,11 +353,16 @@ __device__ __forceinline__ void TrunkStream(const v4f (&acc)[16], const TrunkLan
             float hv[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) hv[r] = fmaxf(acc[t][r], 0.0f);
+            const Split16 hb = SplitBf16(hv[0], hv[1], hv[2], hv[3]);
 #pragma unroll
             for (uint32_t u = 0; u < 8; ++u) {
                 const v4f w = *reinterpret_cast<const v4f*>(cur + u * 1024 + rd1);
+                if constexpr (kT16) {
+                    h1[u] = MfmaT16(w, hb, h1[u]);
+                } else {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) h1[u] = Mfma(w[r], hv[r], h1[u]);
+                    for (int r = 0; r < 4; ++r) h1[u] = Mfma(w[r], hv[r], h1[u]);
+                }
             }
         }
         WaitChunk();
@@ -361,12 +391,17 @@ __device__ __forceinline__ void TrunkStream(const v4f (&acc)[16], const TrunkLan
                 float hv[4];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) hv[r] = fmaxf(h1[t][r], 0.0f);
+                const Split16 hb = SplitBf16(hv[0], hv[1], hv[2], hv[3]);
 #pragma unroll
                 for (int u = 0; u < kRowTiles; ++u) {
                     const uint32_t rd = u == 0 ? rd_f : (u == 1 ? rd_1 : rd_t + 512 * uint32_t(u));
                     const v4f w = *reinterpret_cast<const v4f*>(cur + rd + 64 * h);
+                    if constexpr (kT16) {
+                        pv[u] = MfmaT16(w, hb, pv[u]);
+                    } else {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) pv[u] = Mfma(w[r], hv[r], pv[u]);
+                        for (int r = 0; r < 4; ++r) pv[u] = Mfma(w[r], hv[r], pv[u]);
+                    }
                 }
             }
         }
@@ -919,7 +954,7 @@ __device__ __forceinline__ uint32_t TakeItem(const EvalArgs& p, uint32_t& q, uin
     return p.work_count;
 }
 
-template <bool kTrace, bool kBf16>
+template <bool kTrace, bool kBf16, bool kT16 = false>
 __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void EvaluateKernel(EvalArgs p) {
     constexpr int kCT = kBf16 ? kChunkTilesBf16 : kChunkTiles;  // FP32 stream (the BF16 form's fallback)
     using G = ChunkGeom<kCT>;
@@ -1101,7 +1136,7 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
                 }
                 // layers 1 and 2; the scoring stream's first chunk goes into s_buf0 during the last chunk
                 static_assert(kTrunkChunkBytes <= kBufBytes && kTrunkReadBytes <= kBufBytes, "trunk chunk");
-                TrunkStream(acc, tl, pol, wave_active, s_buf0, s_buf1, wave, waves, lane, [&](uint8_t* b) __device__ {
+                TrunkStream<kT16>(acc, tl, pol, wave_active, s_buf0, s_buf1, wave, waves, lane, [&](uint8_t* b) __device__ {
                     if constexpr (kBf16)
                         issue16(b, 0);
                     else
@@ -1597,7 +1632,7 @@ __device__ __forceinline__ void FoldPrefill(Rsrc acc_r, const TrunkLane& tl, Rsr
 // it), so no register holds them during layer 1.
 constexpr uint32_t kTrunkBiasOff = 9216;  // byte offset of the layer-2 biases in that chunk's buffer
 static_assert(kTrunkBiasOff >= kTrunkReadBytes && kTrunkBiasOff % 16 == 0, "behind every layer-2 read");
-template <bool kTwo, typename FN>
+template <bool kTwo, bool kT16 = false, typename FN>
 __device__ __forceinline__ void TrunkStreamPair(Rsrc acc_r, const TrunkLane& tl,
                                                 float (&pol0)[kScoreSteps], float (&pol1)[kScoreSteps], bool compute,
                                                 uint8_t* b0, uint8_t* b1, uint32_t wave, uint32_t waves, uint32_t lane,
@@ -1635,13 +1670,19 @@ __device__ __forceinline__ void TrunkStreamPair(Rsrc acc_r, const TrunkLane& tl,
                 ha[r] = fmaxf(xa[r], 0.0f);
                 hb[r] = fmaxf(xb[r], 0.0f);
             }
+            const Split16 ta = SplitBf16(ha[0], ha[1], ha[2], ha[3]), tb = SplitBf16(hb[0], hb[1], hb[2], hb[3]);
 #pragma unroll
             for (uint32_t u = 0; u < 8; ++u) {
                 const v4f w = *reinterpret_cast<const v4f*>(cur + u * 1024 + rd1);
+                if constexpr (kT16) {
+                    h1a[u] = MfmaT16(w, ta, h1a[u]);
+                    if (kTwo) h1b[u] = MfmaT16(w, tb, h1b[u]);
+                } else {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    h1a[u] = Mfma(w[r], ha[r], h1a[u]);
-                    if (kTwo) h1b[u] = Mfma(w[r], hb[r], h1b[u]);
+                    for (int r = 0; r < 4; ++r) {
+                        h1a[u] = Mfma(w[r], ha[r], h1a[u]);
+                        if (kTwo) h1b[u] = Mfma(w[r], hb[r], h1b[u]);
+                    }
                 }
             }
         }
@@ -1688,14 +1729,20 @@ __device__ __forceinline__ void TrunkStreamPair(Rsrc acc_r, const TrunkLane& tl,
                     ha[r] = fmaxf(h1a[t][r], 0.0f);
                     hb[r] = fmaxf(h1b[t][r], 0.0f);
                 }
+                const Split16 ta = SplitBf16(ha[0], ha[1], ha[2], ha[3]), tb = SplitBf16(hb[0], hb[1], hb[2], hb[3]);
 #pragma unroll
                 for (int u = 0; u < kRowTiles; ++u) {
                     const uint32_t rd = u == 0 ? rd_f : (u == 1 ? rd_1 : rd_t + 512 * uint32_t(u));
                     const v4f w = *reinterpret_cast<const v4f*>(cur + rd + 64 * h);
+                    if constexpr (kT16) {
+                        pva[u] = MfmaT16(w, ta, pva[u]);
+                        if (kTwo) pvb[u] = MfmaT16(w, tb, pvb[u]);
+                    } else {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        pva[u] = Mfma(w[r], ha[r], pva[u]);
-                        if (kTwo) pvb[u] = Mfma(w[r], hb[r], pvb[u]);
+                        for (int r = 0; r < 4; ++r) {
+                            pva[u] = Mfma(w[r], ha[r], pva[u]);
+                            if (kTwo) pvb[u] = Mfma(w[r], hb[r], pvb[u]);
+                        }
                     }
                 }
             }
@@ -1949,7 +1996,7 @@ __device__ __forceinline__ void RescoreCandidate(Rsrc img_r, Rsrc tail_r, const 
 // exactly kListWaves = 3 waves): the screening stream runs through three buffers with two chunks in flight
 // (StreamChunks3), every wave issuing exactly 3 of a chunk's 9 transfers.
 template <int kTiles, int kCT, bool kBf16 = false, int kListWaves = 1, uint32_t kLC = kLaneCandidates,
-          bool kRing = false>
+          bool kRing = false, bool kT16 = false>
 __device__ __forceinline__ void EvaluateStreamed(const EvalArgs& p) {
     constexpr bool kTwo = kTiles == 2;
     using G = ChunkGeom<kCT>;
@@ -2055,7 +2102,7 @@ __device__ __forceinline__ void EvaluateStreamed(const EvalArgs& p) {
                 FoldPair(acc_r, tl, enc_r, g0, g1, act0, act1, fresh0, fresh1, step, lane);
                 const bool compute = act0 || act1;
                 float pol0[kScoreSteps], pol1[kScoreSteps];
-                TrunkStreamPair<kTwo>(acc_r, tl, pol0, pol1, compute, s_buf0, s_buf1, wave, waves, lane,
+                TrunkStreamPair<kTwo, kT16>(acc_r, tl, pol0, pol1, compute, s_buf0, s_buf1, wave, waves, lane,
                                       [&](uint8_t* b) __device__ {
                                           if constexpr (kBf16)
                                               issue16(b, 0);
@@ -2266,8 +2313,9 @@ __global__ __launch_bounds__(64 * NE_EVAL_MAX_WAVES) __attribute__((amdgpu_waves
 #ifndef NE_PAIR_BF16_LIST
 #define NE_PAIR_BF16_LIST 6
 #endif
+template <bool kT16>
 __global__ __launch_bounds__(64 * NE_PAIR_BF16_WAVES) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void EvaluatePairBf16Kernel(EvalArgs p) {
-    EvaluateStreamed<2, kChunkTilesBf16, true, NE_PAIR_BF16_WAVES, NE_PAIR_BF16_LIST>(p);
+    EvaluateStreamed<2, kChunkTilesBf16, true, NE_PAIR_BF16_WAVES, NE_PAIR_BF16_LIST, false, kT16>(p);
 }
 // One tile per wave in the BF16 form, at exactly 3 waves per workgroup, with the screening stream through a
 // 3-buffer ring (two chunks in flight per workgroup instead of one).  FP32 fallback chunks of one tile and LDS
@@ -2276,8 +2324,9 @@ __global__ __launch_bounds__(64 * NE_PAIR_BF16_WAVES) __attribute__((amdgpu_wave
 // 1-2 waves with 8-entry lists for 4 waves and 128 VGPRs gave run-to-run differences in the count of
 // candidates rescored -- never in a decision -- that 6-entry lists did not; it is not built, and the BF16
 // form's streamed kernel runs only at 3 waves.)
+template <bool kT16>
 __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void EvaluateStreamedBf16RingKernel(EvalArgs p) {
-    EvaluateStreamed<1, 1, true, 3, kLaneCandidates, true>(p);
+    EvaluateStreamed<1, 1, true, 3, kLaneCandidates, true, kT16>(p);
 }
 
 // ================================================================================================
@@ -3322,21 +3371,31 @@ void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, uint32_t
     const dim3 block(64 * waves);
     // the BF16 form's streamed kernel is the ring, which counts exactly 3 transfers per wave and chunk
     kernel = EvaluateKernelFor(kernel, waves, trace, bf16);
+    const bool t16 = bf16 && a.trunk_bf16 != 0;  // the BF16-trunk form: the BF16 kernels with kT16
     if (!trace && kernel == kEvalKernelPaired) {
-        if (bf16)
-            hipLaunchKernelGGL(EvaluatePairBf16Kernel, grid, block, 0, s, a);
+        if (t16)
+            hipLaunchKernelGGL(EvaluatePairBf16Kernel<true>, grid, block, 0, s, a);
+        else if (bf16)
+            hipLaunchKernelGGL(EvaluatePairBf16Kernel<false>, grid, block, 0, s, a);
         else
             hipLaunchKernelGGL(EvaluatePairKernel, grid, block, 0, s, a);
         return;
     }
     if (!trace && kernel == kEvalKernelStreamed) {
-        if (bf16)
-            hipLaunchKernelGGL(EvaluateStreamedBf16RingKernel, grid, block, 0, s, a);
+        if (t16)
+            hipLaunchKernelGGL(EvaluateStreamedBf16RingKernel<true>, grid, block, 0, s, a);
+        else if (bf16)
+            hipLaunchKernelGGL(EvaluateStreamedBf16RingKernel<false>, grid, block, 0, s, a);
         else
             hipLaunchKernelGGL(EvaluateStreamedKernel, grid, block, 0, s, a);
         return;
     }
-    if (bf16) {
+    if (t16) {
+        if (trace)
+            hipLaunchKernelGGL((EvaluateKernel<true, true, true>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((EvaluateKernel<false, true, true>), grid, block, 0, s, a);
+    } else if (bf16) {
         if (trace)
             hipLaunchKernelGGL((EvaluateKernel<true, true>), grid, block, 0, s, a);
         else
@@ -3365,14 +3424,21 @@ uint32_t EvaluateKernelFor(uint32_t kernel, uint32_t waves, bool trace, bool bf1
 
 uint64_t EvaluateAccScratchBytes() { return uint64_t(kEvalMaxWaves) * 2 * kAccTileFloats * sizeof(float); }
 
-uint32_t EvaluateResidentPerCu(uint32_t waves, uint32_t kernel, bool trace, bool bf16) {
+uint32_t EvaluateResidentPerCu(uint32_t waves, uint32_t kernel, bool trace, bool bf16, bool t16) {
     // what the registers and LDS of the kernel built here allow, as the runtime reports it
     if (waves > EvaluateMaxWavesFor(kernel, trace, bf16)) return 0;
     const bool plain = !trace && !bf16;
+    t16 = t16 && bf16;
     const void* k = (plain && kernel == kEvalKernelPaired) ? reinterpret_cast<const void*>(&EvaluatePairKernel)
                   : (plain && kernel == kEvalKernelStreamed) ? reinterpret_cast<const void*>(&EvaluateStreamedKernel)
-                  : (!trace && kernel == kEvalKernelPaired) ? reinterpret_cast<const void*>(&EvaluatePairBf16Kernel)
-                  : (!trace && kernel == kEvalKernelStreamed) ? reinterpret_cast<const void*>(&EvaluateStreamedBf16RingKernel)
+                  : (!trace && kernel == kEvalKernelPaired)
+                      ? (t16 ? reinterpret_cast<const void*>(&EvaluatePairBf16Kernel<true>)
+                             : reinterpret_cast<const void*>(&EvaluatePairBf16Kernel<false>))
+                  : (!trace && kernel == kEvalKernelStreamed)
+                      ? (t16 ? reinterpret_cast<const void*>(&EvaluateStreamedBf16RingKernel<true>)
+                             : reinterpret_cast<const void*>(&EvaluateStreamedBf16RingKernel<false>))
+                  : t16 ? (trace ? reinterpret_cast<const void*>(&EvaluateKernel<true, true, true>)
+                                 : reinterpret_cast<const void*>(&EvaluateKernel<false, true, true>))
                   : bf16 ? (trace ? reinterpret_cast<const void*>(&EvaluateKernel<true, true>)
                                   : reinterpret_cast<const void*>(&EvaluateKernel<false, true>))
                          : (trace ? reinterpret_cast<const void*>(&EvaluateKernel<true, false>)

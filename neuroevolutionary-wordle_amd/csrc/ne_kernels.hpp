@@ -38,6 +38,7 @@ struct EvalArgs {
     const float* enc_t;            // [N][kEncTStride]: transposed encoder, then W0[:, 0], per genome
     const float* fixed_absmax;     // [26] max over the catalog of |fixed feature d| (BF16 bound)
     uint32_t force_fallback;       // BF16 mode: settle every decision with the FP32 scan (diagnostic)
+    uint32_t trunk_bf16;           // BF16 mode: trunk layers 1 and 2 on BF16 MFMA (FP32 accumulate)
     const uint8_t* bf16_cat;       // BF16 mode: [catalog tiles][kBf16CatTileBytes] catalog screening operands
     uint32_t bf16_cat_bytes;
     const uint8_t* bf16_ops;       // BF16 mode: [N][a_tiles][kBf16GenTileBytes] per-genome screening operands
@@ -254,7 +255,7 @@ constexpr uint32_t kEvalKernelPaired = 2;     // two tiles per wave, accumulator
 void LaunchEvaluate(const EvalArgs& a, uint32_t blocks, uint32_t waves, uint32_t kernel, bool trace, bool bf16,
                     hipStream_t s);
 // Workgroups of `waves` waves of the evaluation kernel that fit on one CU at once (0 if unknown).
-uint32_t EvaluateResidentPerCu(uint32_t waves, uint32_t kernel, bool trace, bool bf16);
+uint32_t EvaluateResidentPerCu(uint32_t waves, uint32_t kernel, bool trace, bool bf16, bool t16 = false);
 // bytes of layer-0 accumulator scratch one evaluation workgroup uses (either kernel)
 uint64_t EvaluateAccScratchBytes();
 // The most waves per evaluation workgroup this build supports (its LDS tables are sized for them).

@@ -549,7 +549,8 @@ static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32
     a.bf16_cat = rt->bf16_cat;
     a.bf16_cat_bytes = uint32_t(rt->bf16_cat_bytes);
     a.force_fallback = (rt->eval_mode & NE_EVAL_FORCE_FALLBACK) ? 1u : 0u;
-    const bool bf16 = (rt->eval_mode & 0xFFu) == NE_EVAL_BF16_RESCORED;
+    const bool bf16 = (rt->eval_mode & 0xFFu) == NE_EVAL_BF16_RESCORED || (rt->eval_mode & 0xFFu) == NE_EVAL_BF16_TRUNK;
+    a.trunk_bf16 = (rt->eval_mode & 0xFFu) == NE_EVAL_BF16_TRUNK ? 1u : 0u;
     a.raw_fitness = rt->raw;
     a.local_w = rt->local_w;
     a.counters = rt->counters;
@@ -638,7 +639,7 @@ static int PrepareEvaluation(ne_runtime* rt, uint64_t action_space, const uint32
     a.splits = splits;
     // resident workgroups per CU, as the kernel's registers and LDS allow (3 waves per SIMD at 168 VGPRs
     // and 4 workgroups per CU by LDS in the FP32 form)
-    const uint32_t resident = EvaluateResidentPerCu(waves, kernel, trace, bf16);
+    const uint32_t resident = EvaluateResidentPerCu(waves, kernel, trace, bf16, a.trunk_bf16 != 0);
     uint64_t per_cu = resident != 0 ? resident : std::min<uint64_t>(12 / waves, 4);
     if (rt->tuned_wg_per_cu != 0) per_cu = std::min<uint64_t>(per_cu, rt->tuned_wg_per_cu);
     const uint32_t blocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(items, rt->cu_count * per_cu)));
@@ -2330,7 +2331,7 @@ int ne_get_stage_times(ne_runtime* rt, ne_stage_times* out) {
 int ne_set_eval_precision(ne_runtime* rt, uint32_t mode) {
     if (!rt) return NE_INVALID_ARGUMENT;
     const uint32_t base = mode & 0xFFu;
-    if ((base != NE_EVAL_FP32 && base != NE_EVAL_BF16_RESCORED) || (mode & ~0x1FFu) != 0)
+    if ((base != NE_EVAL_FP32 && base != NE_EVAL_BF16_RESCORED && base != NE_EVAL_BF16_TRUNK) || (mode & ~0x1FFu) != 0)
         return Fail(rt, NE_INVALID_ARGUMENT);
     rt->eval_mode = mode;
     return NE_OK;
@@ -2423,7 +2424,9 @@ int ne_autotune_evaluation(ne_runtime* rt, const ne_word_counts* counts, const n
             best_ms = ms;
         }
     }
-    const bool trace = rt->trace_flags != 0, bf16 = (rt->eval_mode & 0xFFu) == NE_EVAL_BF16_RESCORED;
+    const bool trace = rt->trace_flags != 0,
+               bf16 = (rt->eval_mode & 0xFFu) == NE_EVAL_BF16_RESCORED || (rt->eval_mode & 0xFFu) == NE_EVAL_BF16_TRUNK;
+    const bool t16 = (rt->eval_mode & 0xFFu) == NE_EVAL_BF16_TRUNK;
     if (!trace) {
         for (const uint32_t kern : {kEvalKernelStreamed, kEvalKernelPaired})
             for (uint32_t w = 1; w <= EvaluateMaxWavesFor(kern, trace, bf16); ++w) {
@@ -2447,7 +2450,7 @@ int ne_autotune_evaluation(ne_runtime* rt, const ne_word_counts* counts, const n
             best_ms = ms;
         }
     }
-    const uint32_t resident = std::max<uint32_t>(1, EvaluateResidentPerCu(bw, bt, trace, bf16));
+    const uint32_t resident = std::max<uint32_t>(1, EvaluateResidentPerCu(bw, bt, trace, bf16, t16));
     for (uint32_t g = resident - 1; g >= 1 && g + 2 >= resident; --g) {
         float ms = 0.0f;
         const int st = time_shape(bw, bt, br, g, &ms);

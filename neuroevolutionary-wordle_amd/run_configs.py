@@ -202,7 +202,7 @@ def published_config(forms, reps=5):
                 rt.close()
 
 
-TARGET_FORMS = ("bf16", "bf16-graph", "surrogate", "surrogate-rho0")
+TARGET_FORMS = ("bf16", "bf16-graph", "t16", "surrogate", "surrogate-rho0", "surrogate-t16")
 
 
 def wallclock_to_target(seeds, gens, cap_factor=2, forms=TARGET_FORMS, log=sys.stderr, workload="grid64_a4739"):
@@ -211,8 +211,10 @@ def wallclock_to_target(seeds, gens, cap_factor=2, forms=TARGET_FORMS, log=sys.s
     reaches it (at most cap_factor * gens generations).  Forms: "bf16" (BF16-screened evaluation, the FP32
     form's decisions), "bf16-graph" (the same, every step one captured hipGraph), "surrogate"
     (surrogate-screened evolution, brood 4x, best-predicted 25% + 5% audit, BF16-screened evaluation in the
-    persistent kernel, plain steps while the audit's rank correlation is below 0.5) and "surrogate-rho0" (the
-    same gated at 0).  Every form runs at the launch shape the autotuner picks for it on generation 0.  Host
+    persistent kernel, plain steps while the audit's rank correlation is below 0.5), "surrogate-rho0" (the
+    same gated at 0), "t16" (the BF16-trunk form: BF16 screening with the trunk's layers 1 and 2 on split-BF16
+    MFMA; its decisions are the FP32 form's on >= 99.9% of the decisions, not all, so its trajectory may
+    differ) and "surrogate-t16" (surrogate screening, gate 0.5, on the BF16-trunk form).  Every form runs at the launch shape the autotuner picks for it on generation 0.  Host
     wall-clock per step, steps only.  Returns (per-seed records, aggregate)."""
     wl = synth.workload(workload)
     A, side = wl["actions"], wl["side"]
@@ -220,7 +222,7 @@ def wallclock_to_target(seeds, gens, cap_factor=2, forms=TARGET_FORMS, log=sys.s
     stride = rtm.slot_stride_bytes(A)
     counts = rtm.word_counts(A, A)
     cfg = rtm.default_assembly_config(mutation_probability=0.01, mutation_sigma=0.02)
-    gates = {"surrogate": 0.5, "surrogate-rho0": 0.0}
+    gates = {"surrogate": 0.5, "surrogate-rho0": 0.0, "surrogate-t16": 0.5}
     out = []
     for seed in seeds:
         def make(mode, brood):
@@ -271,7 +273,7 @@ def wallclock_to_target(seeds, gens, cap_factor=2, forms=TARGET_FORMS, log=sys.s
                                 "seconds_per_generation": trace_p[-1][0] / len(trace_p)}
         for form in forms:
             surrogate = form in gates
-            rt = make(rtm.EVAL_BF16_RESCORED, 4 if surrogate else 1)
+            rt = make(rtm.EVAL_BF16_TRUNK if form.endswith("t16") else rtm.EVAL_BF16_RESCORED, 4 if surrogate else 1)
             try:
                 if surrogate:
                     rt.enable_screening(rtm.default_screen_config(spearman_threshold=gates[form]))

@@ -22,7 +22,7 @@ STATUS_NAMES = {
     18: "unsupported configuration", 19: "callback failed", 20: "checkpoint invalid", 21: "checkpoint I/O failure",
 }
 
-EVAL_FP32, EVAL_BF16_RESCORED, EVAL_FORCE_FALLBACK = 0, 1, 0x100
+EVAL_FP32, EVAL_BF16_RESCORED, EVAL_BF16_TRUNK, EVAL_FORCE_FALLBACK = 0, 1, 2, 0x100
 # evaluation kernels of the untraced FP32 form (identical results; ne_eval_shape.kernel)
 EVAL_KERNEL_REGISTERS, EVAL_KERNEL_STREAMED, EVAL_KERNEL_PAIRED = 0, 1, 2
 TRACE_GAMES = 1
@@ -579,7 +579,7 @@ class SlabRuntime:
         self._check("ne_set_trace", self.lib.ne_set_trace(self.h, flags))
 
     def set_eval_precision(self, mode):
-        """EVAL_FP32, or EVAL_BF16_RESCORED (optionally | EVAL_FORCE_FALLBACK)."""
+        """EVAL_FP32, EVAL_BF16_RESCORED or EVAL_BF16_TRUNK (the BF16 ones optionally | EVAL_FORCE_FALLBACK)."""
         self._check("ne_set_eval_precision", self.lib.ne_set_eval_precision(self.h, mode))
 
     def download_trace(self, kind, count):

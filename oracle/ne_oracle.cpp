@@ -304,6 +304,51 @@ bool PolicyForward(const float* g, const Layout& L, const Grid& grid, T* policy)
     return true;
 }
 
+// BF16 trunk (the NE_EVAL_BF16_TRUNK form; no reference counterpart -- the reference's trunk is fp32,
+// dense_trunk.hpp:44-114): layer 0 exactly as the fp32 forward; layers 1 and 2 on split bfloat16 operands
+// (three-term "bf16x3" products): each weight w and each ReLU'd input x is split into hi = bf16(v) and
+// lo = bf16(v - hi), both round to nearest even, and w.x is taken as hi_w.hi_x + hi_w.lo_x + lo_w.hi_x
+// (the lo.lo term, ~2^-16 of the product, is dropped).  The exact products are summed in fp64 from the fp32
+// bias and each output is rounded to fp32 -- an fp32-accumulating MFMA differs from this by its own
+// accumulation rounding only.
+float Bf16Round(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u) return x;  // inf / NaN pass through
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+
+void DenseBf16(const float* W, const float* b, int in, int out, const float* x, float* y, bool relu) {
+    for (int o = 0; o < out; ++o) {
+        double sum = double(b[o]);
+        const float* row = W + size_t(o) * in;
+        for (int k = 0; k < in; ++k) {
+            const float wh = Bf16Round(row[k]), xh = Bf16Round(x[k]);
+            const float wl = Bf16Round(row[k] - wh), xl = Bf16Round(x[k] - xh);
+            sum += double(wh) * double(xh) + double(wh) * double(xl) + double(wl) * double(xh);
+        }
+        const float v = float(sum);
+        y[o] = (relu && v < 0.0f) ? 0.0f : v;
+    }
+}
+
+bool PolicyForwardT16(const float* g, const Layout& L, const Grid& grid, float* policy) {
+    if (grid.tc > 5 || IsFinished(grid)) return false;
+    float x[kModelIn];
+    for (int k = 0; k < kModelIn; ++k) x[k] = 0.0f;
+    x[0] = grid.tc == 0 ? 1.0f : 0.0f;
+    for (int t = 0; t < grid.tc; ++t) EncodeTurn<float>(g, L, grid.guess[t], grid.tile[t], x + 1 + 64 * t);
+    float h0[kH0], h1[kH1];
+    DenseApply<float>(g + L.w0, g + L.b0, kModelIn, kH0, x, h0, true);
+    DenseBf16(g + L.w1, g + L.b1, kH0, kH1, h0, h1, true);
+    DenseBf16(g + L.w2, g + L.b2, kH1, kPol, h1, policy, false);
+    return true;
+}
+
 // Fixed word features: letter count, or -1 when absent (output_embedding.hpp:59-73).
 void FixedFeatures(const uint8_t* w, float* f) {
     int counts[kAlpha] = {0};
@@ -706,6 +751,17 @@ int oro_policy_forward(const float* genome, int enc_hidden, const uint8_t* lette
 
 void oro_fixed_features(const uint8_t* word, float* out) { FixedFeatures(word, out); }
 
+// The BF16-trunk forward of one board (PolicyForwardT16) and the rounding it uses.
+int oro_policy_forward_t16(const float* genome, int enc_hidden, const uint8_t* letters, const uint8_t* tiles, int tc,
+                           float* out32) {
+    const Layout L = MakeLayout(enc_hidden);
+    const Grid g = BoardGrid(letters, tiles, tc);
+    return PolicyForwardT16(genome, L, g, out32) ? kOk : kPolicyForwardFailed;
+}
+void oro_bf16_round(const float* in, float* out, uint64_t n) {
+    for (uint64_t i = 0; i < n; ++i) out[i] = Bf16Round(in[i]);
+}
+
 // The next guess for a played board (TrySelectNextGuessWithSingleModelDeviceRuntime,
 // single_model_device_runtime.cu:122-181; dynamic_policy.hpp:62-150): the policy forward and the masked
 // argmax over the first A catalog words, in fp32 and in fp64.  Returns kPolicyForwardFailed for an
@@ -765,7 +821,7 @@ struct OroGameTrace {
 // Evaluate a population (TryEvaluateCurrentGenerationFitnessOnDevice semantics,
 // slab_runtime.cu:1086-1175 with the per-cell CSR of secrets instead of the shard walk).
 // genomes: pointers to each individual's fp32 genome (stride-agnostic).  mode 0 = fp32 sequential,
-// 1 = fp64.  traces (optional): per game, indexed by game_offsets[n] + slot.
+// 1 = fp64, 2 = BF16 trunk (PolicyForwardT16) with fp32 sequential scoring.  traces (optional): per game, indexed by game_offsets[n] + slot.
 int oro_evaluate(const float* const* genomes, uint64_t N, int enc_hidden, uint64_t A_genome, uint64_t A_runtime,
                  const uint8_t* catalog, uint64_t n_catalog, const uint32_t* cell_offsets, const uint32_t* secret_ids,
                  int mode, int threads, float* fitness, float* raw, uint32_t* local_w, uint64_t* decisions,
@@ -794,9 +850,11 @@ int oro_evaluate(const float* const* genomes, uint64_t N, int enc_hidden, uint64
             for (int i = 0; i < 6; ++i) { tr.guesses[i] = 0xFFFFFFFFu; tr.best[i] = tr.second[i] = 0.0f; }
             while (st == kOk && !IsFinished(grid)) {
                 uint64_t bi = 0;
-                if (mode == 0) {
+                if (mode == 0 || mode == 2) {
                     float pol[kPol], b, s2;
-                    if (!PolicyForward<float>(g, L, grid, pol)) { st = kPolicyForwardFailed; break; }
+                    if (!(mode == 0 ? PolicyForward<float>(g, L, grid, pol) : PolicyForwardT16(g, L, grid, pol))) {
+                        st = kPolicyForwardFailed; break;
+                    }
                     if (!SelectAction<float>(pol, tails, catalog, fixed.data(), A, grid, &bi, &b, &s2)) {
                         st = kActionSelectionFailed; break;
                     }

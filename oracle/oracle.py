@@ -28,6 +28,8 @@ _SIGS = {
     "oro_feedback": (None, [P, P, P]),
     "oro_policy_forward": (I32, [P, I32, P, P, I32, P, P]),
     "oro_fixed_features": (None, [P, P]),
+    "oro_policy_forward_t16": (I32, [P, I32, P, P, I32, P]),
+    "oro_bf16_round": (None, [P, P, U64]),
     "oro_select_guess": (I32, [P, U64, P, P, P, I32, P, P, P, P, P, P]),
     "oro_select_guess_layout": (I32, [P, I32, U64, P, P, P, I32, P, P, P, P, P, P]),
     "oro_encode_turn": (None, [P, I32, P, P, P, P]),
@@ -146,6 +148,30 @@ def policy_forward(genome, turns, enc_hidden=0):
     if st != 0:
         raise RuntimeError(f"oro_policy_forward status {st}")
     return o32, o64
+
+
+def policy_forward_t16(genome, turns, enc_hidden=0):
+    """The BF16-trunk policy (evaluation mode 2) of a board of (word letters[5], tiles[5]) turns: fp32."""
+    g = np.ascontiguousarray(genome, np.float32)
+    tc = len(turns)
+    letters = np.zeros((max(tc, 1), 5), np.uint8)
+    tiles = np.zeros((max(tc, 1), 5), np.uint8)
+    for i, (w, t) in enumerate(turns):
+        letters[i] = w
+        tiles[i] = t
+    out = np.zeros(64, np.float32)
+    st = lib().oro_policy_forward_t16(ptr(g), enc_hidden, ptr(letters), ptr(tiles), tc, ptr(out))
+    if st != 0:
+        raise RuntimeError(f"oro_policy_forward_t16 status {st}")
+    return out
+
+
+def bf16_round(x):
+    """Round-to-nearest-even bfloat16 rounding of fp32 values, returned as fp32."""
+    a = np.ascontiguousarray(x, np.float32).ravel()
+    out = np.empty_like(a)
+    lib().oro_bf16_round(ptr(a), ptr(out), a.size)
+    return out.reshape(np.shape(x))
 
 
 def select_guess(genome, A, catalog, turns, enc_hidden=0):

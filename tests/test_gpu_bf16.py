@@ -111,7 +111,7 @@ def test_eval_precision_rejects_unknown_modes(pkg):
     wl = pkg.synth.workload("grid16_a512")
     rt = _runtime(pkg, 16, 512, wl["catalog"], wl["offsets"], wl["ids"])
     try:
-        for bad in (2, 0x200, 0x101 | 0x1000):
+        for bad in (3, 0x200, 0x101 | 0x1000):
             with pytest.raises(pkg.runtime.RuntimeError_):
                 rt.set_eval_precision(bad)
     finally:

@@ -46,7 +46,8 @@ def fullsize(pkg):
                                ("fp32_paired_untraced", rtm.EVAL_FP32, rtm.EVAL_KERNEL_PAIRED),
                                ("bf16_untraced", rtm.EVAL_BF16_RESCORED, rtm.EVAL_KERNEL_REGISTERS),
                                ("bf16_streamed_untraced", rtm.EVAL_BF16_RESCORED, rtm.EVAL_KERNEL_STREAMED),
-                               ("bf16_paired_untraced", rtm.EVAL_BF16_RESCORED, rtm.EVAL_KERNEL_PAIRED)):
+                               ("bf16_paired_untraced", rtm.EVAL_BF16_RESCORED, rtm.EVAL_KERNEL_PAIRED),
+                               ("t16_streamed_untraced", rtm.EVAL_BF16_TRUNK, rtm.EVAL_KERNEL_STREAMED)):
         rt.set_eval_precision(mode)
         rt.set_trace(0)
         rt.set_eval_shape_ex(3, 1, 0, kernel)
@@ -56,7 +57,8 @@ def fullsize(pkg):
                          fallbacks=int(st.fallbacks))
     rt.set_eval_shape_ex(0, 1, 0, rtm.EVAL_KERNEL_REGISTERS)
     for form, mode, trace in (("fp32", rtm.EVAL_FP32, rtm.TRACE_GAMES | rtm.TRACE_POLICY),
-                              ("bf16", rtm.EVAL_BF16_RESCORED, rtm.TRACE_GAMES)):
+                              ("bf16", rtm.EVAL_BF16_RESCORED, rtm.TRACE_GAMES),
+                              ("t16", rtm.EVAL_BF16_TRUNK, rtm.TRACE_GAMES)):
         rt.set_eval_precision(mode)
         rt.set_trace(trace)
         rt.evaluate(counts)
@@ -106,6 +108,32 @@ def test_untraced_kernels_equal_traced_ones_at_full_size(fullsize):
         elif traced == "bf16":
             assert out[form + "_untraced"]["rescored"] == out["bf16"]["rescored"], form
             assert out[form + "_untraced"]["fallbacks"] == out["bf16"]["fallbacks"], form
+
+
+def test_bf16_trunk_form_agrees_with_the_fp32_form_at_full_size(fullsize):
+    """The BF16-trunk form (trunk layers 1 and 2 on split-BF16 MFMA) against the FP32 form over all 393,216
+    games: >= 99.9% of the decisions on common trajectories are the FP32 form's (the bar for the BF16 trunk);
+    its timed kernel (the streamed ring) has the traced register kernel's fitness and decision count."""
+    wl, games, out = fullsize
+    a, b = out["fp32"]["games"], out["t16"]["games"]
+    compared = agreed = 0
+    on_path = None
+    for k in range(6):
+        live = (a["decisions"] > k) & (b["decisions"] > k)
+        if k > 0:
+            live &= on_path
+        eq = a["guesses"][:, k] == b["guesses"][:, k]
+        compared += int(live.sum())
+        agreed += int((live & eq).sum())
+        on_path = live & eq if k == 0 else on_path & eq
+    same_games = int(np.all(a["guesses"] == b["guesses"], axis=1).sum())
+    same_fit = float(np.mean(out["t16"]["fitness"] == out["fp32"]["fitness"]))
+    assert compared > 1_500_000 and agreed / compared >= 0.999, (agreed, compared)
+    tu, tt = out["t16_streamed_untraced"], out["t16"]
+    assert np.array_equal(tu["fitness"].view(np.uint32), tt["fitness"].view(np.uint32))
+    assert tu["decisions"] == tt["decisions"]
+    print(f"[full size] BF16 trunk vs FP32: {agreed}/{compared} = {agreed / compared:.6f} decisions agree on common "
+          f"trajectories, {same_games}/{len(a)} games identical, fitness equal on {same_fit:.4f} of the genomes")
 
 
 def test_fp32_form_agrees_with_the_oracle_on_1024_genomes(fullsize, orc):
