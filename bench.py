@@ -45,13 +45,17 @@ def kernel_source_hash():
     return h.hexdigest()
 
 
-KERNEL_NAMES = ("EvaluateKernel<false,false>", "EvaluateStreamedKernel", "EvaluatePairKernel")
+KERNEL_NAMES = ("EvaluateKernel<false,false,false>", "EvaluateStreamedKernel", "EvaluatePairKernel")
 # the BF16-screened form's kernels (its streamed kernel is the 3-wave ring; at other sizes the register kernel runs)
-KERNEL_NAMES_BF16 = ("EvaluateKernel<false,true>", "EvaluateStreamedBf16RingKernel", "EvaluatePairBf16Kernel")
+KERNEL_NAMES_BF16 = ("EvaluateKernel<false,true,false>", "EvaluateStreamedBf16RingKernel<false>",
+                     "EvaluatePairBf16Kernel<false>")
+# the BF16-trunk form's kernels (the BF16 kernels with the split-BF16 trunk)
+KERNEL_NAMES_T16 = ("EvaluateKernel<false,true,true>", "EvaluateStreamedBf16RingKernel<true>",
+                    "EvaluatePairBf16Kernel<true>")
 PEAK_BF16_MFMA_TFLOPS = 2500.0
 
 
-def measured_traffic(kernel=0, bf16=False):
+def measured_traffic(kernel=0, bf16=False, t16=False):
     """HBM bytes per evaluation launch of one form's `kernel` (0 registers, 1 streamed, 2 paired) from the
     committed FETCH_SIZE / WRITE_SIZE measurement of this workload (scaled by the plain-copy calibration
     recorded with it) and whether it was measured on the kernel sources of this tree: (bytes or None, stale)."""
@@ -60,7 +64,7 @@ def measured_traffic(kernel=0, bf16=False):
     with open(TRAFFIC_FILE) as f:
         t = json.load(f)
     stale = t.get("kernel_source_sha256") != kernel_source_hash()
-    k = t.get("kernels", {}).get((KERNEL_NAMES_BF16 if bf16 else KERNEL_NAMES)[kernel])
+    k = t.get("kernels", {}).get((KERNEL_NAMES_T16 if t16 else KERNEL_NAMES_BF16 if bf16 else KERNEL_NAMES)[kernel])
     if k is None:
         return None, True
     return k["fetch_bytes"] * t["fetch_scale"] + k["write_bytes"] * t["write_scale"], stale
@@ -275,14 +279,17 @@ def main():
         if kt == 1 and sh.get("waves") != 3:
             kt = 0
         ft = trunk16["flops"] / (trunk16["eval_ms"] * 1e-3) / 1e12
+        tt, tt_stale = measured_traffic(kt, t16=True)
         # not bit-identical to the FP32 form: the trunk's split-BF16 products differ by ~2^-16 relative, so the
         # steps' trajectories part after the first differing decision (decision agreement at full size is a GPU
         # test); fitness agreement is reported on the last generation of the same steps
         t16_out = {
             "value": games_per_gen * args.steps / trunk16["elapsed"], "unit": "game-evals/s",
             "ms_per_step": trunk16["elapsed"] / args.steps * 1e3, "generations_per_s": args.steps / trunk16["elapsed"],
-            "evaluate_ms": trunk16["eval_ms"], "kernel": KERNEL_NAMES_BF16[kt],
-            "roofline": {"achieved_tflops": ft, "frac_of_bf16_peak": ft / PEAK_BF16_MFMA_TFLOPS},
+            "evaluate_ms": trunk16["eval_ms"], "kernel": KERNEL_NAMES_T16[kt],
+            "roofline": {"achieved_tflops": ft, "frac_of_bf16_peak": ft / PEAK_BF16_MFMA_TFLOPS,
+                         "traffic": None if tt_stale else tt,
+                         "traffic_GBps": None if (tt_stale or tt is None) else tt / (trunk16["eval_ms"] * 1e-3) / 1e9},
             "candidates_rescored_per_decision": trunk16["rescored"] / max(1, trunk16["decisions"]),
             "fp32_fallback_decisions": trunk16["fallbacks"],
             "fitness_equal_to_fp32_fraction": float(np.mean(trunk16["fitness"] == fp32["fitness"])),
