@@ -2330,7 +2330,8 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
 }
 
 // ================================================================================================
-// Child fingerprint for surrogate screening.  One workgroup of 4 waves per child; wave d plays the
+// Child fingerprint for surrogate screening (kT16: the trunk's layers 1 and 2 on split-BF16 MFMA, as the
+// BF16-trunk evaluation form runs them).  One workgroup of 4 waves per child; wave d plays the
 // 16 probe boards of depth d (lane column k = probe secret k): the turns are folded into layer 0
 // exactly as in a game, layers 1 and 2 stream through LDS, all actions are scored in FP32 with the
 // masked argmax of a decision.  Per probe p = 16 d + k the child's guess g yields 3 inputs:
@@ -2339,6 +2340,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
 //            same feedback as the secret), else 0; depth 0 (no hints): the fitness of parent k % 2,
 //            as (f - shift) * scale;
 //   [3p + 2] 1 if g is the secret.
+template <bool kT16>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAVES_PER_EU, NE_EVAL_WAVES_PER_EU))) void FingerprintKernel(FingerprintArgs p) {
     constexpr int kCT = kChunkTiles;
     using G = ChunkGeom<kCT>;
@@ -2383,7 +2385,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NE_EVAL_WAV
         FoldSlot(acc, tl, int(ts), e);
     }
     float pol[kScoreSteps];
-    TrunkStream(acc, tl, pol, true, s_buf0, s_buf1, wave, waves, lane, [&](uint8_t* b) __device__ { issue(b, 0); });
+    TrunkStream<kT16>(acc, tl, pol, true, s_buf0, s_buf1, wave, waves, lane, [&](uint8_t* b) __device__ { issue(b, 0); });
     ArgmaxLane am = {-__builtin_inff(), -__builtin_inff(), kInvalidSlot, -__builtin_inff()};
     StreamChunks<kCT>(s_buf0, s_buf1, chunks, true, issue, [&](const uint8_t* buf, int k, uint32_t at) __device__ {
         if (k == 0) ShareArgmaxGate(am);
@@ -3357,7 +3359,10 @@ void LaunchPackBf16(const uint8_t* slab, uint64_t stride, const uint32_t* slots,
 }
 
 void LaunchFingerprint(const FingerprintArgs& a, uint32_t n, hipStream_t s) {
-    hipLaunchKernelGGL(FingerprintKernel, dim3(n), dim3(64 * kProbeDepths), 0, s, a);
+    if (a.trunk_bf16)
+        hipLaunchKernelGGL(FingerprintKernel<true>, dim3(n), dim3(64 * kProbeDepths), 0, s, a);
+    else
+        hipLaunchKernelGGL(FingerprintKernel<false>, dim3(n), dim3(64 * kProbeDepths), 0, s, a);
 }
 
 void LaunchTransposeEncoder(const uint8_t* slab, uint64_t stride, const uint32_t* slots, uint32_t n,

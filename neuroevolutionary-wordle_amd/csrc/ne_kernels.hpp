@@ -207,6 +207,7 @@ struct FingerprintArgs {
     float parent_shift, parent_scale;  // parent fitness input = (f - shift) * scale
     float* out;                    // [n][kFingerprintDims]
     int* status;
+    uint32_t trunk_bf16;           // 1: the trunk's layers 1 and 2 on split-BF16 MFMA (the BF16-trunk form)
 };
 
 void LaunchFingerprint(const FingerprintArgs& a, uint32_t n, hipStream_t s);

@@ -1520,6 +1520,7 @@ static void LaunchFingerprints(ne_runtime* rt, const uint32_t* slots, uint32_t n
     f.parent_scale = kParentFitnessScale;
     f.out = out;
     f.status = rt->status;
+    f.trunk_bf16 = (rt->eval_mode & 0xFFu) == NE_EVAL_BF16_TRUNK ? 1u : 0u;
     LaunchFingerprint(f, n, rt->stream);
 }
 

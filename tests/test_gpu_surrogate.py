@@ -80,8 +80,14 @@ def test_surrogate_adam_steps_match_oracle(pkg, small):
         assert np.max(np.abs(got - params)) > 0.5 * lr
 
 
-def test_fingerprints_match_oracle(pkg, orc, small):
+@pytest.mark.parametrize("precision", ["fp32", "bf16_trunk"])
+def test_fingerprints_match_oracle(pkg, orc, small, precision):
+    """Probe play of three children against the oracle's fp64 policy on every probe whose top-2 margin is
+    certain; "bf16_trunk": the fingerprint kernel of the BF16-trunk form (split-BF16 trunk layers 1 and 2,
+    ~1e-5 relative from the FP32 trunk, far inside the margin)."""
     wl, rt = small
+    rtm = pkg.runtime
+    rt.set_eval_precision(rtm.EVAL_BF16_TRUNK if precision == "bf16_trunk" else rtm.EVAL_FP32)
     slots, _, _, _ = rt.generation(0)
     probes = rt.probe_states()
     assert probes.shape == (16, 4)
@@ -111,6 +117,7 @@ def test_fingerprints_match_oracle(pkg, orc, small):
                 np.testing.assert_allclose(g, ref[3 * p:3 * p + 3], rtol=0, atol=1e-6, err_msg=f"genome {i} probe {p}")
                 checked += 1
     assert checked >= 150
+    rt.set_eval_precision(rtm.EVAL_FP32)
 
 
 def _screened_run(pkg, wl, gens, screen_kw, seed=7):
