@@ -2854,7 +2854,9 @@ static_assert(kBreedBatch >= 1 && kBreedBatch <= 8, "breed batch");
 // (BreedAndMutateGenome, genome_ops.cuh:386-407, on counter-based draws).  Writes the full slot.
 __global__ __launch_bounds__(256) void BreedKernel(BreedArgs p) {
     const uint32_t bc = blockIdx.y;  // entry of the batch
-    const uint32_t child = p.child_list != nullptr ? p.child_list[bc] : p.child_offset + bc;
+    const uint32_t brood = p.interleave_cells != 0 ? gridDim.y / p.interleave_cells : 1u;
+    const uint32_t entry = p.interleave_cells != 0 ? (bc % brood) * p.interleave_cells + bc / brood : bc;
+    const uint32_t child = p.child_list != nullptr ? p.child_list[bc] : p.child_offset + entry;
     __shared__ XoverRecord s_rec;
     __shared__ const float* s_p1;
     __shared__ const float* s_p2;
@@ -2873,7 +2875,7 @@ __global__ __launch_bounds__(256) void BreedKernel(BreedArgs p) {
         const uint32_t fcb = p.free_count_dev != nullptr ? *p.free_count_dev : p.free_count_before;
         const bool room = p.given_slots != nullptr || fcb >= gridDim.y;
         const uint32_t cs = p.given_slots != nullptr ? p.given_slots[child]
-                                                     : (room ? p.free_stack[fcb - 1 - bc] : kInvalidSlot);
+                                                     : (room ? p.free_stack[fcb - 1 - entry] : kInvalidSlot);
         s_bad = (!room || sa >= p.slot_count || sb >= p.slot_count || cs >= p.slot_count) ? 1 : 0;
         if (!room)
             SetStatus(p.status, 14);  // kSlabFull

@@ -117,6 +117,10 @@ struct BreedArgs {
     const uint32_t* step;          // optional: device {gen_index, seed} in place of the fields
     const uint32_t* free_count_dev;  // optional: the free count read on the device in place of
                                      // free_count_before; fewer free slots than the batch -> slab full
+    uint32_t interleave_cells;     // optional (brood of B x cells children, no child list): batch entry e
+                                   // breeds child (e % B) x cells + e / B, so a cell's B brood members run
+                                   // side by side and share their neighbourhood's parents in the caches;
+                                   // each child keeps the slot and draws of its own index
 };
 
 // What a captured generation step leaves for one host read after its replay.

@@ -1472,6 +1472,10 @@ static int AssembleBrood(ne_runtime* rt, uint32_t seed, const ne_assembly_config
     b.child_offset = 0;
     b.elem_begin = 0;
     b.elem_end = BroodPhase1End(rt);
+    // a cell's brood members side by side (child c's cell is c mod n): the parents of the children in flight
+    // then fit the caches better
+    const uint64_t n_cells = rt->cur_size;
+    if (n_cells != 0 && BN % n_cells == 0 && BN / n_cells > 1) b.interleave_cells = uint32_t(n_cells);
     EventBegin(rt, 3);
     LaunchBreed(b, uint32_t(BN), rt->stream);
     EventEnd(rt, 3);
