@@ -141,7 +141,7 @@ def main():
     ap.add_argument("--eval-shape", default="", metavar="W,R,G,K",
                     help="time every form at this launch shape (waves, rounds per item, workgroups per CU, "
                          "evaluation kernel: 0 registers, 1 streamed, 2 paired) instead of the autotuner's")
-    ap.add_argument("--wallclock-seeds", default="207,217,227",
+    ap.add_argument("--wallclock-seeds", default="207,217,227,237,247",
                     help="seeds of the wall-clock-to-target leg (none of them was used to choose a setting)")
     ap.add_argument("--wallclock-generations", type=int, default=200,
                     help="plain FP32 generations that define the target (other forms stop at twice as many)")
