@@ -1,9 +1,11 @@
 // ne_kernels.hip -- HIP kernels of the GA generation step for gfx950 (CDNA4, wave64).
 //
-// Evaluation: one workgroup per genome (blockIdx.y splits large game lists), one wave per tile of
-// 16 games.  All dense contractions run on v_mfma_f32_16x16x4_f32 in the transposed form
-// (weights = A operand, games = B operand), so each layer's accumulator layout is directly the
-// next layer's B operand and activations never leave registers.
+// Evaluation: persistent workgroups take (genome, split) items from per-XCD work queues; one wave plays
+// one tile of 16 games (two in the paired kernels).  The dense contractions run in the transposed form
+// (weights = A operand, games = B operand), so each layer's accumulator layout is directly the next
+// layer's B operand: on v_mfma_f32_16x16x4_f32 in the FP32 form; the BF16 forms screen the actions on
+// v_mfma_f32_16x16x32_bf16 and rescore candidates on the FP32 contraction, and the BF16-trunk form runs
+// the trunk's layers 1 and 2 on split operands with v_mfma_f32_16x16x16_bf16.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -286,7 +288,8 @@ __device__ __forceinline__ void TrunkToPolicy(const v4f (&acc)[16], const TrunkL
 // chunk 16 + c is W2[0..64)[32c..32c+32) as [64][32] (8 KB).  A transfer (one instruction, 64 lanes
 // x 16 bytes) covers 16 rows of a W1 chunk or 8 rows of a W2 chunk; its per-lane byte offset is
 // fixed and the chunk / transfer part is uniform (an SGPR), so issuing needs no per-lane arithmetic.
-// The MFMA order is exactly that of TrunkToPolicy, so the results are identical.
+// The MFMA order is exactly that of TrunkToPolicy, so the results are identical (kT16 = false; with kT16
+// layers 1 and 2 take the split-BF16 products of MfmaT16, in the same order in every kernel).
 constexpr uint32_t kTrunkDma = 8;                       // transfers per chunk
 constexpr uint32_t kTrunkW1Chunks = kH0 / 16;           // 16
 constexpr uint32_t kTrunkW2Chunks = kH1 / 32;           // 4
