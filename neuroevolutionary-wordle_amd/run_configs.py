@@ -303,6 +303,57 @@ def wallclock_to_target(seeds, gens, cap_factor=2, forms=TARGET_FORMS, log=sys.s
     return out, agg
 
 
+def agreement_config(seeds, checkpoints=(0, 25, 100), workload="grid64_a4739"):
+    """Decision agreement of the BF16-trunk form with the FP32 form on evolving populations: per seed, plain FP32
+    generations run, and at each checkpoint generation the current population is evaluated traced in both forms;
+    decisions are compared while a game's two trajectories coincide."""
+    wl = synth.workload(workload)
+    A, side = wl["actions"], wl["side"]
+    n = side * side
+    stride = rtm.slot_stride_bytes(A)
+    counts = rtm.word_counts(A, A)
+    cfg = rtm.default_assembly_config(mutation_probability=0.01, mutation_sigma=0.02)
+    games = 3 * int(wl["offsets"][-1])
+    for seed in seeds:
+        rt = pkg.SlabRuntime(slab_bytes=2 * n * stride + stride, generation_bytes=n * stride, action_count=A,
+                             grid_columns=side)
+        rt.upload_catalog(wl["catalog"])
+        rt.upload_secrets_csr(wl["offsets"], wl["ids"])
+        rt.bootstrap(n, seed=seed, variance_scale=1.0, tail_std=0.1)
+        try:
+            for g in range(max(checkpoints) + 1):
+                if g in checkpoints:
+                    out = {}
+                    for name, mode in (("fp32", rtm.EVAL_FP32), ("t16", rtm.EVAL_BF16_TRUNK)):
+                        rt.set_eval_precision(mode)
+                        rt.set_trace(rtm.TRACE_GAMES)
+                        rt.evaluate(counts)
+                        out[name] = (rt.download_trace(rtm.TRACE_GAMES, games), rt.generation(0)[1].copy())
+                    rt.set_trace(0)
+                    rt.set_eval_precision(rtm.EVAL_FP32)
+                    a, b = out["fp32"][0], out["t16"][0]
+                    compared = agreed = 0
+                    on_path = None
+                    for k in range(6):
+                        live = (a["decisions"] > k) & (b["decisions"] > k)
+                        if k > 0:
+                            live &= on_path
+                        eq = a["guesses"][:, k] == b["guesses"][:, k]
+                        compared += int(live.sum())
+                        agreed += int((live & eq).sum())
+                        on_path = live & eq if k == 0 else on_path & eq
+                    print(json.dumps({"config": "agreement", "workload": workload, "seed": seed, "generation": g,
+                                      "decisions_compared": compared, "decisions_agreeing": agreed,
+                                      "agreement": agreed / compared,
+                                      "games_identical": int(np.all(a["guesses"] == b["guesses"], axis=1).sum()),
+                                      "games": games,
+                                      "fitness_equal_fraction": float(np.mean(out["fp32"][1] == out["t16"][1])),
+                                      "best_fitness_fp32": float(out["fp32"][1].max())}), flush=True)
+                rt.advance(seed + 2 + g, counts, cfg)
+        finally:
+            rt.close()
+
+
 def target_config(seeds, gens, cap_factor=2, forms=TARGET_FORMS, workload="grid64_a4739"):
     recs, agg = wallclock_to_target(seeds, gens, cap_factor, forms=forms, workload=workload)
     for r in recs:
@@ -313,7 +364,7 @@ def target_config(seeds, gens, cap_factor=2, forms=TARGET_FORMS, workload="grid6
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("config", choices=["grid16", "grid128", "ragged", "target", "target128", "published"])
+    ap.add_argument("config", choices=["grid16", "grid128", "ragged", "target", "target128", "published", "agreement"])
     ap.add_argument("--forms", default="")
     ap.add_argument("--generations", type=int, default=0)
     ap.add_argument("--seeds", type=int, default=10)
@@ -322,6 +373,9 @@ def main():
     args = ap.parse_args()
     forms = [f for f in args.forms.split(",") if f]
     seeds = [args.first_seed + 10 * i for i in range(args.seeds)]
+    if args.config == "agreement":  # BF16-trunk vs FP32 decisions on evolving populations (64x64 / 4,739)
+        agreement_config(seeds)
+        return
     if args.config in ("target", "target128"):
         target_forms = tuple(forms) if forms else TARGET_FORMS
         if args.config == "target":
